@@ -1,0 +1,212 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+The oracle is a C++ restatement of RELAX's per-time-step solve
+(oracle/relax_oracle.cpp).  It is the checker the HIP library is compared
+against in tests/, in __graft_entry__.smoke() and in the cpu_baseline leg of
+bench.py; the product never imports it.
+
+Array conventions (numpy, C order, so the flat memory equals the reference's
+Fortran column-major arrays):
+  vector field  (sx3, sx2, sx1+2) float32/float64   -- REAL*4 (sx1+2,sx2,sx3)
+  plane         (sx2, sx1+2)
+  tensor field  (sx3, sx2, sx1, 6)                  -- TYPE(TENSOR) (sx1,sx2,sx3)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+LAYER = np.dtype([("z", "f8"), ("gammadot0", "f8"), ("stressexponent", "f8"),
+                  ("cohesion", "f8"), ("friction", "f8"), ("Gk", "f8")])
+TENSOR_LAYER = np.dtype([("z", "f4"), ("dum", "f4"), ("t", "f4", (6,))])
+PATCH = np.dtype([("x1", "f8"), ("x2", "f8"), ("x3", "f8"), ("lx", "f8"), ("lz", "f8"),
+                  ("slip", "f8"), ("ss", "f8"), ("ds", "f8"), ("v", "f8"), ("vss", "f8"),
+                  ("vds", "f8"), ("taus", "f8"), ("sig", "f4", (6,))])
+PLANE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("width", "f8"), ("length", "f8"),
+                  ("strike", "f8"), ("dip", "f8"), ("rake", "f8"), ("px2", "i4"), ("px3", "i4")])
+
+
+class OrcModel(ctypes.Structure):
+    _fields_ = [("sx1", ctypes.c_int), ("sx2", ctypes.c_int), ("sx3", ctypes.c_int),
+                ("dx1", ctypes.c_double), ("dx2", ctypes.c_double), ("dx3", ctypes.c_double),
+                ("lam", ctypes.c_double), ("mu", ctypes.c_double), ("gam", ctypes.c_double),
+                ("beta", ctypes.c_double),
+                ("linear", ctypes.c_void_p), ("nonlinear", ctypes.c_void_p),
+                ("faultcreep", ctypes.c_void_p), ("prestress", ctypes.c_void_p),
+                ("np", ctypes.c_int), ("planes", ctypes.c_void_p), ("patches", ctypes.c_void_p),
+                ("interval", ctypes.c_double), ("odt", ctypes.c_double), ("tscale", ctypes.c_double),
+                ("skip", ctypes.c_int), ("ne", ctypes.c_int), ("event_times", ctypes.c_void_p)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        _LIB = ctypes.CDLL(path)
+        _LIB.orc_tensoramplitude_f.restype = ctypes.c_double
+        _LIB.orc_tensoramplitude_d.restype = ctypes.c_double
+        assert _LIB.orc_sizeof_patch() == PATCH.itemsize
+        assert _LIB.orc_sizeof_model() == ctypes.sizeof(OrcModel)
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _sfx(a):
+    return "f" if a.dtype == np.float32 else "d"
+
+
+def _d(x):
+    return ctypes.c_double(float(x))
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise RuntimeError("%s failed with status %d" % (what, rc))
+
+
+# --- fourier.f90 -----------------------------------------------------------
+def fft3(data, sx1, sx2, sx3, dx1, dx2, dx3, direction):
+    _chk(getattr(lib(), "orc_fft3_" + _sfx(data))(_p(data), sx1, sx2, sx3, _d(dx1), _d(dx2), _d(dx3), direction), "fft3")
+
+
+def fft2(data, sx1, sx2, dx1, dx2, direction):
+    _chk(getattr(lib(), "orc_fft2_" + _sfx(data))(_p(data), sx1, sx2, _d(dx1), _d(dx2), direction), "fft2")
+
+
+# --- green.f90 -------------------------------------------------------------
+def elasticresponse(lam, mu, f1, f2, f3, dx1, dx2, dx3):
+    sx3, sx2, p = f1.shape
+    getattr(lib(), "orc_elasticresponse_" + _sfx(f1))(_d(lam), _d(mu), _p(f1), _p(f2), _p(f3), p - 2, sx2, sx3,
+                                                     _d(dx1), _d(dx2), _d(dx3))
+
+
+def surfacetractioncowling(lam, mu, gamma, u1, u2, u3, dx1, dx2, dx3):
+    sx3, sx2, p = u1.shape
+    out = [np.zeros((sx2, p), u1.dtype) for _ in range(3)]
+    getattr(lib(), "orc_surfacetractioncowling_" + _sfx(u1))(
+        _d(lam), _d(mu), _d(gamma), _p(u1), _p(u2), _p(u3), p - 2, sx2, sx3, _d(dx1), _d(dx2), _d(dx3),
+        _p(out[0]), _p(out[1]), _p(out[2]))
+    return out
+
+
+def cerrutimodified(p1, p2, p3, lam, mu, gamma, u1, u2, u3, dx1, dx2, dx3):
+    sx3, sx2, p = u1.shape
+    getattr(lib(), "orc_cerrutimodified_" + _sfx(u1))(
+        _p(p1), _p(p2), _p(p3), _d(lam), _d(mu), _d(gamma), _p(u1), _p(u2), _p(u3), p - 2, sx2, sx3,
+        _d(dx1), _d(dx2), _d(dx3))
+
+
+def greenfunctioncowling(c1, c2, c3, t1, t2, t3, dx1, dx2, dx3, lam, mu, gamma):
+    sx3, sx2, p = c1.shape
+    _chk(getattr(lib(), "orc_greenfunctioncowling_" + _sfx(c1))(
+        _p(c1), _p(c2), _p(c3), _p(t1), _p(t2), _p(t3), p - 2, sx2, sx3, _d(dx1), _d(dx2), _d(dx3),
+        _d(lam), _d(mu), _d(gamma)), "greenfunctioncowling")
+
+
+# --- elastic3d.f90 ---------------------------------------------------------
+def stressupdate(v1, v2, v3, lam, mu, dx1, dx2, dx3, sig):
+    sx3, sx2, sx1, _ = sig.shape
+    _chk(getattr(lib(), "orc_stressupdate_" + _sfx(sig))(
+        _p(v1), _p(v2), _p(v3), _d(lam), _d(mu), _d(dx1), _d(dx2), _d(dx3), sx1, sx2, sx3, _p(sig)), "stressupdate")
+
+
+def equivalentbodyforce(sig, dx1, dx2, dx3, c1, c2, c3, t1, t2, t3, mask=None):
+    sx3, sx2, sx1, _ = sig.shape
+    _chk(getattr(lib(), "orc_equivalentbodyforce_" + _sfx(sig))(
+        _p(sig), _d(dx1), _d(dx2), _d(dx3), sx1, sx2, sx3, _p(c1), _p(c2), _p(c3), _p(t1), _p(t2), _p(t3),
+        _p(mask)), "equivalentbodyforce")
+
+
+def tensoramplitude(t, dx1, dx2, dx3):
+    n = t.size // 6
+    return getattr(lib(), "orc_tensoramplitude_" + _sfx(t))(_p(t), ctypes.c_long(n), _d(dx1), _d(dx2), _d(dx3))
+
+
+def momentdensityshear(mu, slip, x, y, z, L, W, strike, dip, rake, beta, dx1, dx2, dx3, sig):
+    sx3, sx2, sx1, _ = sig.shape
+    getattr(lib(), "orc_momentdensityshear_" + _sfx(sig))(
+        _d(mu), _d(slip), _d(x), _d(y), _d(z), _d(L), _d(W), _d(strike), _d(dip), _d(rake), _d(beta),
+        sx1, sx2, sx3, _d(dx1), _d(dx2), _d(dx3), _p(sig))
+
+
+# --- viscoelastic3d.f90 / friction3d.f90 -----------------------------------
+def viscouseigenstress(mu, structure, prestress, sig, moment, maxwelltime=None, dgammadot0=None, gamma=None):
+    sx3, sx2, sx1, _ = sig.shape
+    mw = None if maxwelltime is None else np.array([maxwelltime], np.float64)
+    _chk(getattr(lib(), "orc_viscouseigenstress_" + _sfx(sig))(
+        _d(mu), _p(structure), _p(prestress), _p(sig), sx1, sx2, sx3, _p(moment), _p(mw), _p(dgammadot0),
+        _p(gamma)), "viscouseigenstress")
+    return None if mw is None else float(mw[0])
+
+
+def frictioneigenstress(x, y, z, L, W, strike, dip, rake, beta, sig, mu, structure, dx1, dx2, dx3, moment,
+                        maxwelltime=None, vel=None):
+    sx3, sx2, sx1, _ = sig.shape
+    mw = None if maxwelltime is None else np.array([maxwelltime], np.float64)
+    _chk(getattr(lib(), "orc_frictioneigenstress_" + _sfx(sig))(
+        _d(x), _d(y), _d(z), _d(L), _d(W), _d(strike), _d(dip), _d(rake), _d(beta), _p(sig), _d(mu),
+        _p(structure), sx1, sx2, sx3, _d(dx1), _d(dx2), _d(dx3), _p(moment), _p(mw), _p(vel)),
+        "frictioneigenstress")
+    return None if mw is None else float(mw[0])
+
+
+def monitorfriction(x, y, z, L, W, strike, dip, rake, beta, sx3_full, dx1, dx2, dx3, sig, structure, patch):
+    sx3, sx2, sx1, _ = sig.shape
+    px3, px2 = patch.shape
+    _chk(getattr(lib(), "orc_monitorfriction_" + _sfx(sig))(
+        _d(x), _d(y), _d(z), _d(L), _d(W), _d(strike), _d(dip), _d(rake), _d(beta), sx1, sx2, sx3_full, sx3,
+        _d(dx1), _d(dx2), _d(dx3), _p(sig), _p(structure), _p(patch), px2, px3), "monitorfriction")
+
+
+def frictionadd(patch, dt):
+    lib().orc_frictionadd(_p(patch), ctypes.c_long(patch.size), _d(dt))
+
+
+# --- relax.f90 RK2 loop ----------------------------------------------------
+def rk2_run(model, state, nsteps, dt_replay=None):
+    """Advance `state` (dict of u1,u2,u3,tau,sig,gamma arrays, t, oi, i) in place.
+
+    `model` is a dict: sx1,sx2,sx3,dx1,dx2,dx3,lam,mu,gam,beta, linear/nonlinear/
+    faultcreep (LAYER arrays of sx3/2 or None), prestress (TENSOR_LAYER sx3/2),
+    planes (PLANE array or None), patches (PATCH array or None), interval, odt,
+    tscale, skip, event_times.  Returns an (nsteps, 8) array of
+    Dt, tm, maxwell(1:3), power, C:E^i, t.
+    """
+    keep = []
+
+    def ptr(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a)
+        keep.append(a)
+        return a.ctypes.data
+
+    ev = np.asarray(model.get("event_times", [0.0]), np.float64)
+    m = OrcModel(model["sx1"], model["sx2"], model["sx3"], model["dx1"], model["dx2"], model["dx3"],
+                 model["lam"], model["mu"], model["gam"], model.get("beta", 0.2),
+                 ptr(model.get("linear")), ptr(model.get("nonlinear")), ptr(model.get("faultcreep")),
+                 ptr(model["prestress"]), 0 if model.get("planes") is None else len(model["planes"]),
+                 ptr(model.get("planes")), None if model.get("patches") is None else model["patches"].ctypes.data,
+                 model["interval"], model.get("odt", 1.0), model.get("tscale", 1.0), model.get("skip", 1),
+                 len(ev), ptr(ev))
+    sfx = _sfx(state["u1"])
+    t = ctypes.c_double(state.get("t", 0.0))
+    oi = ctypes.c_int(state.get("oi", 1))
+    it = ctypes.c_int(state.get("i", 0))
+    info = np.zeros((nsteps, 8), np.float64)
+    dtr = None if dt_replay is None else np.ascontiguousarray(dt_replay, np.float64)
+    rc = getattr(lib(), "orc_rk2_run_" + sfx)(
+        ctypes.byref(m), _p(state["u1"]), _p(state["u2"]), _p(state["u3"]), _p(state["tau"]), _p(state["sig"]),
+        _p(state["gamma"]), ctypes.byref(t), ctypes.byref(oi), ctypes.byref(it), nsteps, _p(dtr), _p(info))
+    _chk(rc, "rk2_run")
+    done = it.value - state.get("i", 0)
+    state["t"], state["oi"], state["i"] = t.value, oi.value, it.value
+    return info[:done]

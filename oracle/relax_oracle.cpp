@@ -1,0 +1,1053 @@
+// relax_oracle.cpp -- CPU restatement of RELAX's per-time-step elastic +
+// relaxation solve.  TEST INFRASTRUCTURE ONLY: it is the checker that the HIP
+// library is compared against (tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py).  Nothing in the product links or calls it.
+//
+// Every routine follows one reference routine; the file:line of the routine it
+// restates is given above each function (paths relative to the reference's
+// src/ directory).  The layout is the reference's ALIGN_DATA layout (FFTW/MKL
+// builds, src/include.f90:45-56):
+//   vector fields   REAL*4 (sx1+2, sx2, sx3)    i1 fastest, row pitch sx1+2
+//   tensor fields   TYPE(TENSOR) (sx1, sx2, sx3) 6 floats per point, AoS
+//   surface planes  REAL*4 (sx1+2, sx2)
+// The template parameter R is the storage type: R=float reproduces the
+// reference's FP32 storage with its FP64 intermediates at the same places;
+// R=double is the all-FP64 build of the same pipeline.
+//
+// Parity status of each function (how it is pinned) is listed in the header
+// of tests/test_oracle.py.  Functions whose only pin is internal consistency
+// say so there.
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+typedef std::complex<double> cd;
+
+// Planted-mistake hook for the tests that prove each pin can fail: 0 = the
+// faithful restatement; any other value switches on one deliberate error.
+static int g_mut = 0;
+#define MUT(k) (g_mut == (k))
+
+static const double PI = 3.141592653589793115997963468544185161;
+static const double PI2 = 6.28318530717958623199592693708837032318;
+static const double DEG2RAD = 0.01745329251994329547437168059786927;
+
+// ---------------------------------------------------------------------------
+// ABI structures (types.f90:26-97, SEQUENCE layouts)
+// ---------------------------------------------------------------------------
+extern "C" {
+struct orc_layer { double z, gammadot0, stressexponent, cohesion, friction, Gk; };
+struct orc_tensor_layer { float z, dum; float t[6]; };
+struct orc_patch {
+  double x1, x2, x3, lx, lz, slip, ss, ds, v, vss, vds, taus;
+  float sig[6];
+};
+struct orc_plane { double x, y, z, width, length, strike, dip, rake; int px2, px3; };
+}
+
+template <class R> struct T6 { R s11, s12, s13, s22, s23, s33; };
+
+// ---------------------------------------------------------------------------
+// FFT (power-of-two sizes).  Numerically an exact DFT in FP64; stores R.
+// fourier.f90:177-201 (FFTW: R2C forward, in place, padded), :330-355 (ctfft),
+// sign -1 forward (fourier.f90:36), forward scaled by dx1*dx2*dx3, inverse by
+// 1/(sx1*dx1*sx2*dx2*sx3*dx3); fft2 :372-396 / :510-535 likewise in 2-D.
+// ---------------------------------------------------------------------------
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+static void fft1d(cd* a, int n, long stride, int sign) {
+  // iterative radix-2 decimation in time, bit-reversed input reorder
+  for (int i = 1, j = 0; i < n; ++i) {
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i * stride], a[j * stride]);
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    for (int k = 0; k < len / 2; ++k) {
+      double ang = sign * PI2 * k / len;
+      cd w(std::cos(ang), std::sin(ang));
+      for (int i = 0; i < n; i += len) {
+        cd u = a[(i + k) * stride];
+        cd v = a[(i + k + len / 2) * stride] * w;
+        a[(i + k) * stride] = u + v;
+        a[(i + k + len / 2) * stride] = u - v;
+      }
+    }
+  }
+}
+
+// forward R2C along x1 then C2C along x2 (and x3): data is R[(sx1+2)*sx2*sx3]
+template <class R>
+static int fftn_forward(R* data, int sx1, int sx2, int sx3, double scale) {
+  if (!is_pow2(sx1) || !is_pow2(sx2) || !is_pow2(sx3)) return 11;
+  const int h1 = sx1 / 2 + 1;
+  const long pitch = sx1 + 2;
+  std::vector<cd> spec((size_t)h1 * sx2 * sx3);
+#pragma omp parallel
+  {
+    std::vector<cd> row(sx1);
+#pragma omp for
+    for (long r = 0; r < (long)sx2 * sx3; ++r) {
+      for (int i = 0; i < sx1; ++i) row[i] = cd((double)data[r * pitch + i], 0.0);
+      fft1d(row.data(), sx1, 1, -1);
+      for (int k = 0; k < h1; ++k) spec[r * h1 + k] = row[k];
+    }
+  }
+  if (sx2 > 1) {
+#pragma omp parallel for
+    for (long c = 0; c < (long)h1 * sx3; ++c) {
+      long i3 = c / h1, k1 = c % h1;
+      fft1d(&spec[i3 * h1 * sx2 + k1], sx2, h1, -1);
+    }
+  }
+  if (sx3 > 1) {
+#pragma omp parallel for
+    for (long c = 0; c < (long)h1 * sx2; ++c)
+      fft1d(&spec[c], sx3, (long)h1 * sx2, -1);
+  }
+#pragma omp parallel for
+  for (long r = 0; r < (long)sx2 * sx3; ++r)
+    for (int k = 0; k < h1; ++k) {
+      cd z = spec[r * h1 + k] * scale;
+      data[r * pitch + 2 * k] = (R)z.real();
+      data[r * pitch + 2 * k + 1] = (R)z.imag();
+    }
+  return 0;
+}
+
+template <class R>
+static int fftn_inverse(R* data, int sx1, int sx2, int sx3, double scale) {
+  if (!is_pow2(sx1) || !is_pow2(sx2) || !is_pow2(sx3)) return 11;
+  const int h1 = sx1 / 2 + 1;
+  const long pitch = sx1 + 2;
+  std::vector<cd> spec((size_t)h1 * sx2 * sx3);
+#pragma omp parallel for
+  for (long r = 0; r < (long)sx2 * sx3; ++r)
+    for (int k = 0; k < h1; ++k)
+      spec[r * h1 + k] = cd((double)data[r * pitch + 2 * k], (double)data[r * pitch + 2 * k + 1]);
+  if (sx3 > 1) {
+#pragma omp parallel for
+    for (long c = 0; c < (long)h1 * sx2; ++c)
+      fft1d(&spec[c], sx3, (long)h1 * sx2, +1);
+  }
+  if (sx2 > 1) {
+#pragma omp parallel for
+    for (long c = 0; c < (long)h1 * sx3; ++c) {
+      long i3 = c / h1, k1 = c % h1;
+      fft1d(&spec[i3 * h1 * sx2 + k1], sx2, h1, +1);
+    }
+  }
+#pragma omp parallel
+  {
+    std::vector<cd> row(sx1);
+#pragma omp for
+    for (long r = 0; r < (long)sx2 * sx3; ++r) {
+      // Hermitian extension; the imaginary parts of the k1=0 and Nyquist
+      // bins do not contribute to a real output
+      for (int k = 0; k < h1; ++k) row[k] = spec[r * h1 + k];
+      row[0] = cd(row[0].real(), 0.0);
+      row[sx1 / 2] = cd(row[sx1 / 2].real(), 0.0);
+      for (int k = 1; k < sx1 / 2; ++k) row[sx1 - k] = std::conj(row[k]);
+      fft1d(row.data(), sx1, 1, +1);
+      for (int i = 0; i < sx1; ++i) data[r * pitch + i] = (R)(row[i].real() * scale);
+      data[r * pitch + sx1] = 0;
+      data[r * pitch + sx1 + 1] = 0;
+    }
+  }
+  return 0;
+}
+
+// fourier.f90:177-201 / :330-355
+template <class R>
+static int fft3(R* data, int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, int direction) {
+  if (direction == -1) return fftn_forward(data, sx1, sx2, sx3, MUT(12) ? dx1 * dx2 : dx1 * dx2 * dx3);
+  return fftn_inverse(data, sx1, sx2, sx3, 1.0 / (sx1 * dx1 * sx2 * dx2 * sx3 * dx3));
+}
+// fourier.f90:372-396 / :510-535
+template <class R>
+static int fft2(R* data, int sx1, int sx2, double dx1, double dx2, int direction) {
+  if (direction == -1) return fftn_forward(data, sx1, sx2, 1, dx1 * dx2);
+  return fftn_inverse(data, sx1, sx2, 1, 1.0 / (sx1 * dx1 * sx2 * dx2));
+}
+
+// ---------------------------------------------------------------------------
+// wavenumbers, fourier.f90:64-81 (cycles per unit length, negative above
+// Nyquist in x2 and x3, k1 >= 0).  Indices here are 0-based.
+// ---------------------------------------------------------------------------
+static inline double wavenumber1(int i1, int sx1, double dx1) { return (double)i1 / (sx1 * dx1); }
+static inline double wavenumber23(int i, int sx, double dx) {
+  // 1-based: i < sx/2+1 -> (i-1)/(sx dx), else -(sx-i+1)/(sx dx)
+  int i1b = i + 1;
+  if (i1b < sx / 2 + 1) return ((double)i1b - 1.0) / (sx * dx);
+  return -((double)(sx - i1b) + 1.0) / (sx * dx);
+}
+
+// ---------------------------------------------------------------------------
+// elasticresponse, green.f90:51-101
+// ---------------------------------------------------------------------------
+template <class R>
+static void elasticresponse(double lambda, double mu, R* f1, R* f2, R* f3, int sx1, int sx2, int sx3,
+                            double dx1, double dx2, double dx3) {
+  const double ratio1 = (lambda + mu) / (lambda + 2.0 * mu) / mu / (PI2 * PI2);
+  const double ratio2 = mu / (lambda + mu);
+  const long pitch = sx1 + 2;
+#pragma omp parallel for
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    double k3 = wavenumber23(i3, sx3, dx3);
+    for (int i2 = 0; i2 < sx2; ++i2) {
+      double k2 = wavenumber23(i2, sx2, dx2);
+      for (int i1 = 0; i1 < sx1 / 2 + 1; ++i1) {
+        double k1 = wavenumber1(i1, sx1, dx1);
+        double r2 = k1 * k1 + k2 * k2 + k3 * k3;
+        double denom = ratio1 / (r2 * r2);
+        long o = (i2 + (long)sx2 * i3) * pitch + 2 * i1;
+        cd c1(f1[o], f1[o + 1]), c2(f2[o], f2[o + 1]), c3(f3[o], f3[o + 1]);
+        cd b1 = ((k2 * k2 + k3 * k3 + (MUT(2) ? 0.0 : ratio2) * r2) * c1 - (MUT(1) ? k2 : k1) * (k2 * c2 + k3 * c3)) * denom;
+        cd b2 = ((k1 * k1 + k3 * k3 + ratio2 * r2) * c2 - k2 * (k1 * c1 + k3 * c3)) * denom;
+        cd b3 = ((k1 * k1 + k2 * k2 + ratio2 * r2) * c3 - k3 * (k1 * c1 + k2 * c2)) * denom;
+        f1[o] = (R)b1.real(); f1[o + 1] = (R)b1.imag();
+        f2[o] = (R)b2.real(); f2[o + 1] = (R)b2.imag();
+        f3[o] = (R)b3.real(); f3[o + 1] = (R)b3.imag();
+      }
+    }
+  }
+  // zero wavenumber, no net body force (green.f90:96-99)
+  f1[0] = f1[1] = 0; f2[0] = f2[1] = 0; f3[0] = f3[1] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// surfacetractioncowling, green.f90:327-377.  p accumulates in R in i3 order.
+// ---------------------------------------------------------------------------
+template <class R>
+static void surfacetractioncowling(double lambda, double mu, double gamma, const R* u1, const R* u2, const R* u3,
+                                   int sx1, int sx2, int sx3, double dx1, double dx2, double dx3,
+                                   R* p1, R* p2, R* p3) {
+  const double modulus = lambda + 2.0 * mu;
+  const double alpha = (lambda + mu) / (lambda + 2.0 * mu);
+  const double grav = 2.0 * mu * alpha * gamma;
+  const long pitch = sx1 + 2;
+  const cd I(0.0, MUT(3) ? -PI2 : PI2);
+  std::fill(p1, p1 + pitch * sx2, (R)0);
+  std::fill(p2, p2 + pitch * sx2, (R)0);
+  std::fill(p3, p3 + pitch * sx2, (R)0);
+#pragma omp parallel for
+  for (int i2 = 0; i2 < sx2; ++i2) {
+    double k2 = wavenumber23(i2, sx2, dx2);
+    for (int i3 = 0; i3 < sx3; ++i3) {
+      double k3 = wavenumber23(i3, sx3, dx3);
+      for (int i1 = 0; i1 < sx1 / 2 + 1; ++i1) {
+        double k1 = wavenumber1(i1, sx1, dx1);
+        long o = (i2 + (long)sx2 * i3) * pitch + 2 * i1;
+        cd c1(u1[o], u1[o + 1]), c2(u2[o], u2[o + 1]), c3(u3[o], u3[o + 1]);
+        cd s1 = I * mu * (k3 * c1 + k1 * c3);
+        cd s2 = I * mu * (k3 * c2 + k2 * c3);
+        cd s3 = I * (modulus * k3 * c3 + lambda * (k1 * c1 + k2 * c2)) - (MUT(4) ? 0.0 : grav) * c3;
+        long q = i2 * pitch + 2 * i1;
+        p1[q] = p1[q] + (R)s1.real(); p1[q + 1] = p1[q + 1] + (R)s1.imag();
+        p2[q] = p2[q] + (R)s2.real(); p2[q + 1] = p2[q + 1] + (R)s2.imag();
+        p3[q] = p3[q] + (R)s3.real(); p3[q + 1] = p3[q + 1] + (R)s3.imag();
+      }
+    }
+  }
+  const R d = (R)(sx3 * dx3);
+  for (long q = 0; q < pitch * sx2; ++q) { p1[q] /= d; p2[q] /= d; p3[q] /= d; }
+}
+
+// ---------------------------------------------------------------------------
+// cerrutimodified, green.f90:520-612 (with cerrutisolmodified :571-610)
+// ---------------------------------------------------------------------------
+template <class R> struct CX { typedef std::complex<R> type; };
+
+template <class R>
+static void cerrutisolmodified(double mu, std::complex<R> p1, std::complex<R> p2, std::complex<R> p3,
+                               double alpha, double gamma, std::complex<R>& u1, std::complex<R>& u2,
+                               std::complex<R>& u3, double k1, double k2, double k3) {
+  const cd i(0.0, 1.0);
+  double beta = PI2 * std::sqrt(k1 * k1 + k2 * k2);
+  double fi = (2 * beta) / ((PI2 * PI2) * (k3 * k3) + beta * beta);
+  double h = gamma / beta;
+  if (0 == k1 && 0 == k2) {
+    u1 = u2 = u3 = std::complex<R>(0, 0);
+    return;
+  }
+  cd P1((double)p1.real(), (double)p1.imag()), P2((double)p2.real(), (double)p2.imag()),
+      P3((double)p3.real(), (double)p3.imag());
+  double temp = (1.0 / (2.0 * mu * beta * beta * beta)) * fi;
+  cd b1 = temp * P1, b2 = temp * P2;
+  cd b3 = (beta * P3 - (MUT(6) ? -i : i) * (1.0 - alpha) * (PI2 * k1 * P1 + PI2 * k2 * P2)) /
+          (2.0 * alpha * mu * beta * beta * beta * beta * (MUT(5) ? 1.0 : 1 + h));
+  cd tmp = alpha * i * beta * PI2 * b3 * (1.0 - 1.0 / alpha - i * PI2 * k3 * fi);
+  cd v1 = MUT(7) ? cd(0.0) : tmp * k1 * fi, v2 = tmp * k2 * fi;
+  cd v3 = -alpha * beta * beta * b3 * (1.0 / alpha - i * fi * PI2 * k3);
+  cd w1 = v1 + (-(2.0 * beta * beta * b1) + (alpha * (PI2 * PI2) * k1 * (b1 * k1 + b2 * k2) * (1.0 - (i * PI2 * k3 * fi))));
+  cd w2 = v2 + (-(2.0 * beta * beta * b2) + (alpha * (PI2 * PI2) * k2 * (b1 * k1 + b2 * k2) * (1.0 - (i * PI2 * k3 * fi))));
+  cd w3 = v3 * fi + alpha * beta * PI2 * (k1 * b1 + k2 * b2) * PI2 * k3 * fi;
+  u1 = std::complex<R>((R)w1.real(), (R)w1.imag());
+  u2 = std::complex<R>((R)w2.real(), (R)w2.imag());
+  u3 = std::complex<R>((R)w3.real(), (R)w3.imag());
+}
+
+template <class R>
+static void cerrutimodified(const R* p1, const R* p2, const R* p3, double lambda, double mu, double gamma,
+                            R* u1, R* u2, R* u3, int sx1, int sx2, int sx3, double dx1, double dx2, double dx3) {
+  const double alpha = (lambda + mu) / (lambda + 2 * mu);
+  const long pitch = sx1 + 2;
+#pragma omp parallel for
+  for (int i2 = 0; i2 < sx2; ++i2) {
+    double k2 = wavenumber23(i2, sx2, dx2);
+    for (int i1 = 0; i1 < sx1 / 2 + 1; ++i1) {
+      double k1 = wavenumber1(i1, sx1, dx1);
+      long q = i2 * pitch + 2 * i1;
+      std::complex<R> t1(p1[q], p1[q + 1]), t2(p2[q], p2[q + 1]), t3(p3[q], p3[q + 1]);
+      for (int i3 = 0; i3 < sx3; ++i3) {
+        double k3 = wavenumber23(i3, sx3, dx3);
+        std::complex<R> b1, b2, b3;
+        cerrutisolmodified<R>(mu, t1, t2, t3, alpha, gamma, b1, b2, b3, k1, k2, k3);
+        long o = (i2 + (long)sx2 * i3) * pitch + 2 * i1;
+        u1[o] += b1.real(); u1[o + 1] += b1.imag();
+        u2[o] += b2.real(); u2[o + 1] += b2.imag();
+        u3[o] += b3.real(); u3[o + 1] += b3.imag();
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// greenfunctioncowling, green.f90:1008-1129 (non-CUDA, cerrutimodified path)
+// ---------------------------------------------------------------------------
+template <class R>
+static int greenfunctioncowling(R* c1, R* c2, R* c3, R* t1, R* t2, R* t3, int sx1, int sx2, int sx3,
+                                double dx1, double dx2, double dx3, double lambda, double mu, double gamma) {
+  const long pitch = sx1 + 2;
+  std::vector<R> p1(pitch * sx2, 0), p2(pitch * sx2, 0), p3(pitch * sx2, 0);
+  int rc = 0;
+  rc |= fft3(c1, sx1, sx2, sx3, dx1, dx2, dx3, -1);
+  rc |= fft3(c2, sx1, sx2, sx3, dx1, dx2, dx3, -1);
+  rc |= fft3(c3, sx1, sx2, sx3, dx1, dx2, dx3, -1);
+  rc |= fft2(t1, sx1, sx2, dx1, dx2, -1);
+  rc |= fft2(t2, sx1, sx2, dx1, dx2, -1);
+  rc |= fft2(t3, sx1, sx2, dx1, dx2, -1);
+  if (rc) return rc;
+  elasticresponse(lambda, mu, c1, c2, c3, sx1, sx2, sx3, dx1, dx2, dx3);
+  surfacetractioncowling(lambda, mu, gamma, c1, c2, c3, sx1, sx2, sx3, dx1, dx2, dx3, p1.data(), p2.data(), p3.data());
+  for (long q = 0; q < pitch * sx2; ++q) {
+    p1[q] = t1[q] - p1[q]; p2[q] = t2[q] - p2[q]; p3[q] = t3[q] - p3[q];
+  }
+  cerrutimodified(p1.data(), p2.data(), p3.data(), lambda, mu, gamma, c1, c2, c3, sx1, sx2, sx3, dx1, dx2, dx3);
+  fft3(c1, sx1, sx2, sx3, dx1, dx2, dx3, 1);
+  fft3(c2, sx1, sx2, sx3, dx1, dx2, dx3, 1);
+  fft3(c3, sx1, sx2, sx3, dx1, dx2, dx3, 1);
+  fft2(t1, sx1, sx2, dx1, dx2, 1);
+  fft2(t2, sx1, sx2, dx1, dx2, 1);
+  fft2(t3, sx1, sx2, dx1, dx2, 1);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// FIR differentiators: optimalfilter, elastic3d.f90:846-907; taps kernel1.inc
+// (fir1 = 0.5) and kernel7.inc (single-precision literals stored in REAL*8).
+// ---------------------------------------------------------------------------
+static void optimalfilter1(int sx, bool vertical, double* ker, int* len, int* err) {
+  static const float fir7f[7] = {8.77856e-01f, -2.81913e-01f, +6.22696e-02f, +2.82441e-02f,
+                                 -5.09029e-02f, +4.20471e-02f, -1.59409e-02f};
+  if (!vertical && sx >= 2 && sx <= 4) { *len = 1; ker[0] = 0.5; return; }
+  if (sx >= 5 && sx <= 14) { *len = 7; for (int l = 0; l < 7; ++l) ker[l] = (double)fir7f[l]; return; }
+  if (sx >= 15) { *len = 1; ker[0] = 0.5; return; }
+  *err = 2;
+}
+
+template <class R> static inline R rnd(double x) { return (R)x; }
+
+// isotropicstressstrain, elastic3d.f90:129-142; tensortrace :287 sums in the
+// storage precision before widening.
+template <class R>
+static inline void isotropicstressstrain(T6<R>& t, double lambda, double mu) {
+  double epskk = (double)(t.s11 + t.s22 + t.s33);
+  R m2 = (R)(2.0 * mu);
+  t.s11 *= m2; t.s12 *= m2; t.s13 *= m2; t.s22 *= m2; t.s23 *= m2; t.s33 *= m2;
+  t.s11 = (R)(t.s11 + lambda * epskk);
+  t.s22 = (R)(t.s22 + lambda * epskk);
+  t.s33 = (R)(t.s33 + lambda * epskk);
+}
+
+// ---------------------------------------------------------------------------
+// stressupdate, elastic3d.f90:956-1180 (localstrain_fir2 :1032,
+// localstrain_ani :1140).  v is padded (sx1+2, sx2, sx3) with sx3 the depth
+// extent of the tensor field (half the FFT grid in the RK2 loop).
+// ---------------------------------------------------------------------------
+template <class R>
+static int stressupdate(const R* v1, const R* v2, const R* v3, double lambda, double mu, double dx1, double dx2,
+                        double dx3, int sx1, int sx2, int sx3, R* sig, long vpitch) {
+  double ker1[16], ker2[16], ker3[16];
+  int len1 = 0, len2 = 0, len3 = 0, err = 0;
+  optimalfilter1(sx1, false, ker1, &len1, &err);
+  optimalfilter1(sx2, false, ker2, &len2, &err);
+  optimalfilter1(sx3, true, ker3, &len3, &err);
+  if (err) return err;
+  for (int l = 0; l < 16; ++l) { ker1[l] /= dx1; ker2[l] /= dx2; ker3[l] /= dx3; }
+  auto V = [&](const R* v, int i1, int i2, int i3) { return v[i1 + vpitch * (i2 + (long)sx2 * i3)]; };
+  T6<R>* S = reinterpret_cast<T6<R>*>(sig);
+#pragma omp parallel for
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    bool boundary = !((i3 + 1 > len3) && (i3 + 1 < sx3 - len3 + 1));
+    int i3p = 0, i3m = 0;
+    double px3 = 0;
+    if (boundary) {
+      if (i3 == 0) { px3 = MUT(8) ? 2.0 * dx3 : dx3; i3p = 1; i3m = 0; }
+      else if (i3 == sx3 - 1) { px3 = dx3; i3p = sx3 - 1; i3m = sx3 - 2; }
+      else { px3 = dx3 * 2.0; i3m = i3 - 1; i3p = i3 + 1; }
+    }
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        T6<R> e = {0, 0, 0, 0, 0, 0};
+        for (int l = 1; l <= len1; ++l) {
+          int m = (sx1 + i1 - l) % sx1, p = (i1 + l) % sx1;
+          e.s11 = rnd<R>(e.s11 + (double)(V(v1, p, i2, i3) - V(v1, m, i2, i3)) * ker1[l - 1]);
+          e.s12 = rnd<R>(e.s12 + (double)(V(v2, p, i2, i3) - V(v2, m, i2, i3)) * ker1[l - 1]);
+          e.s13 = rnd<R>(e.s13 + (double)(V(v3, p, i2, i3) - V(v3, m, i2, i3)) * ker1[l - 1]);
+        }
+        for (int l = 1; l <= len2; ++l) {
+          int m = (sx2 + i2 - l) % sx2, p = (i2 + l) % sx2;
+          e.s12 = rnd<R>(e.s12 + (double)(V(v1, i1, p, i3) - V(v1, i1, m, i3)) * ker2[l - 1]);
+          e.s22 = rnd<R>(e.s22 + (double)(V(v2, i1, p, i3) - V(v2, i1, m, i3)) * ker2[l - 1]);
+          e.s23 = rnd<R>(e.s23 + (double)(V(v3, i1, p, i3) - V(v3, i1, m, i3)) * ker2[l - 1]);
+        }
+        if (boundary) {
+          e.s13 = rnd<R>(e.s13 + (double)(V(v1, i1, i2, i3p) - V(v1, i1, i2, i3m)) / px3);
+          e.s23 = rnd<R>(e.s23 + (double)(V(v2, i1, i2, i3p) - V(v2, i1, i2, i3m)) / px3);
+          e.s33 = rnd<R>((double)(V(v3, i1, i2, i3p) - V(v3, i1, i2, i3m)) / px3);
+        } else {
+          for (int l = 1; l <= len3; ++l) {
+            int m = i3 - l, p = i3 + l;
+            e.s13 = rnd<R>(e.s13 + (double)(V(v1, i1, i2, p) - V(v1, i1, i2, m)) * ker3[l - 1]);
+            e.s23 = rnd<R>(e.s23 + (double)(V(v2, i1, i2, p) - V(v2, i1, i2, m)) * ker3[l - 1]);
+            e.s33 = rnd<R>(e.s33 + (double)(V(v3, i1, i2, p) - V(v3, i1, i2, m)) * ker3[l - 1]);
+          }
+        }
+        e.s12 = rnd<R>(e.s12 / 2.0);
+        e.s13 = rnd<R>(e.s13 / 2.0);
+        e.s23 = rnd<R>(e.s23 / 2.0);
+        isotropicstressstrain(e, lambda, mu);
+        T6<R>& s = S[i1 + (long)sx1 * (i2 + (long)sx2 * i3)];
+        s.s11 += e.s11; s.s12 += e.s12; s.s13 += e.s13;
+        s.s22 += e.s22; s.s23 += e.s23; s.s33 += e.s33;
+      }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// equivalentbodyforce, elastic3d.f90:1214-1506 (localdivergence_fir2 :1324,
+// localdivergence_ani :1431); FP64 accumulators, c -= REAL(f).
+// ---------------------------------------------------------------------------
+template <class R>
+static int equivalentbodyforce(const R* sig, double dx1, double dx2, double dx3, int sx1, int sx2, int sx3,
+                               R* c1, R* c2, R* c3, R* t1, R* t2, R* t3, long cpitch, const R* mask) {
+  double ker1[16], ker2[16], ker3[16];
+  int len1 = 0, len2 = 0, len3 = 0, err = 0;
+  optimalfilter1(sx1, false, ker1, &len1, &err);
+  optimalfilter1(sx2, false, ker2, &len2, &err);
+  optimalfilter1(sx3, true, ker3, &len3, &err);
+  if (err) return err;
+  for (int l = 0; l < 16; ++l) { ker1[l] /= dx1; ker2[l] /= dx2; ker3[l] /= dx3; }
+  const T6<R>* S = reinterpret_cast<const T6<R>*>(sig);
+  auto at = [&](int i1, int i2, int i3) -> const T6<R>& { return S[i1 + (long)sx1 * (i2 + (long)sx2 * i3)]; };
+  for (int i2 = 0; i2 < sx2; ++i2)
+    for (int i1 = 0; i1 < sx1; ++i1) {
+      t1[i1 + cpitch * i2] += at(i1, i2, 0).s13;
+      t2[i1 + cpitch * i2] += at(i1, i2, 0).s23;
+      t3[i1 + cpitch * i2] += at(i1, i2, 0).s33;
+    }
+#pragma omp parallel for
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    if (mask && mask[i3] == 0) continue;
+    bool boundary = !((i3 + 1 > len3) && (i3 + 1 < sx3 - len3 + 1));
+    int i3p = 0, i3m = 0;
+    double px3 = 0;
+    if (boundary) {
+      if (i3 == 0) { px3 = dx3; i3p = 1; i3m = 0; }
+      else if (i3 == sx3 - 1) { px3 = dx3; i3p = sx3 - 1; i3m = sx3 - 2; }
+      else { px3 = dx3 * 2.0; i3m = i3 - 1; i3p = i3 + 1; }
+    }
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        double f1 = 0, f2 = 0, f3 = 0;
+        for (int l = 1; l <= len1; ++l) {
+          int m = (sx1 + i1 - l) % sx1, p = (i1 + l) % sx1;
+          f1 += (double)(at(p, i2, i3).s11 - at(m, i2, i3).s11) * ker1[l - 1];
+          f2 += (double)(at(p, i2, i3).s12 - at(m, i2, i3).s12) * ker1[l - 1];
+          f3 += (double)(at(p, i2, i3).s13 - at(m, i2, i3).s13) * ker1[l - 1];
+        }
+        for (int l = 1; l <= len2; ++l) {
+          int m = (sx2 + i2 - l) % sx2, p = (i2 + l) % sx2;
+          f1 += (double)(at(i1, p, i3).s12 - at(i1, m, i3).s12) * ker2[l - 1];
+          f2 += (double)(at(i1, p, i3).s22 - at(i1, m, i3).s22) * (MUT(9) ? -ker2[l - 1] : ker2[l - 1]);
+          f3 += (double)(at(i1, p, i3).s23 - at(i1, m, i3).s23) * ker2[l - 1];
+        }
+        if (boundary) {
+          f1 += (double)(at(i1, i2, i3p).s13 - at(i1, i2, i3m).s13) / px3;
+          f2 += (double)(at(i1, i2, i3p).s23 - at(i1, i2, i3m).s23) / px3;
+          f3 += (double)(at(i1, i2, i3p).s33 - at(i1, i2, i3m).s33) / px3;
+        } else {
+          for (int l = 1; l <= len3; ++l) {
+            int m = i3 - l, p = i3 + l;
+            f1 += (double)(at(i1, i2, p).s13 - at(i1, i2, m).s13) * ker3[l - 1];
+            f2 += (double)(at(i1, i2, p).s23 - at(i1, i2, m).s23) * ker3[l - 1];
+            f3 += (double)(at(i1, i2, p).s33 - at(i1, i2, m).s33) * ker3[l - 1];
+          }
+        }
+        long o = i1 + cpitch * (i2 + (long)sx2 * i3);
+        c1[o] = c1[o] - (R)f1;
+        c2[o] = c2[o] - (R)f2;
+        c3[o] = c3[o] - (R)f3;
+      }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// tensor helpers, elastic3d.f90:150-330
+// ---------------------------------------------------------------------------
+template <class R> static inline T6<R> tensordeviatoric(const T6<R>& t) {
+  R diag = (R)((double)(t.s11 + t.s22 + t.s33) / 3.0);
+  return T6<R>{t.s11 - diag, t.s12, t.s13, t.s22 - diag, t.s23, t.s33 - diag};
+}
+template <class R> static inline double tensornorm(const T6<R>& t) {
+  return std::sqrt(((double)(t.s11 * t.s11) + 2.0 * (double)(t.s12 * t.s12) + 2.0 * (double)(t.s13 * t.s13) +
+                    (double)(t.s22 * t.s22) + 2.0 * (double)(t.s23 * t.s23) + (double)(t.s33 * t.s33)) / 2.0);
+}
+
+// ---------------------------------------------------------------------------
+// viscouseigenstress, viscoelastic3d.f90:217-314
+// ---------------------------------------------------------------------------
+template <class R>
+static int viscouseigenstress(double mu, const orc_layer* structure, const orc_tensor_layer* prestress,
+                              const R* sig, int sx1, int sx2, int sx3, R* moment, double* maxwelltime,
+                              const R* dgammadot0, R* gamma, long gpitch) {
+  const T6<R>* S = reinterpret_cast<const T6<R>*>(sig);
+  T6<R>* M = reinterpret_cast<T6<R>*>(moment);
+  for (int i3 = 0; i3 < sx3; ++i3)
+    if (structure[i3].stressexponent < 0.999999) return 1;
+  float tm = maxwelltime ? (float)*maxwelltime : 1e30f;
+#pragma omp parallel for reduction(min : tm)
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    double power = structure[i3].stressexponent;
+    double cohesion = structure[i3].cohesion;
+    T6<R> pt = {(R)prestress[i3].t[0], (R)prestress[i3].t[1], (R)prestress[i3].t[2],
+                (R)prestress[i3].t[3], (R)prestress[i3].t[4], (R)prestress[i3].t[5]};
+    T6<R> sp = tensordeviatoric(pt);
+    double taup = tensornorm(sp);
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        long o = i1 + (long)sx1 * (i2 + (long)sx2 * i3);
+        double gammadot0 = structure[i3].gammadot0;
+        if (dgammadot0) gammadot0 = gammadot0 + dgammadot0[o];
+        if (1.0e-20 > gammadot0) continue;
+        T6<R> s = tensordeviatoric(S[o]);
+        T6<R> ss = {s.s11 + sp.s11, s.s12 + sp.s12, s.s13 + sp.s13, s.s22 + sp.s22, s.s23 + sp.s23, s.s33 + sp.s33};
+        double tau = tensornorm(ss);
+        double tauc = std::max(0.0, tau - cohesion);
+        if (tauc <= 1.0e-20) continue;
+        double gammadot = gammadot0 * std::pow(tauc / mu, MUT(10) ? power : power - 1);
+        double gammadotp = gammadot0 * std::pow(taup / mu, power - 1);
+        R a = (R)(2.0 * gammadot), b = (R)(2.0 * gammadotp);
+        T6<R>& m = M[o];
+        m.s11 = m.s11 + (a * s.s11 - b * sp.s11);
+        m.s12 = m.s12 + (a * s.s12 - b * sp.s12);
+        m.s13 = m.s13 + (a * s.s13 - b * sp.s13);
+        m.s22 = m.s22 + (a * s.s22 - b * sp.s22);
+        m.s23 = m.s23 + (a * s.s23 - b * sp.s23);
+        m.s33 = m.s33 + (a * s.s33 - b * sp.s33);
+        tm = std::min(tm, (float)(1.0 / gammadot));
+        if (gamma) gamma[i1 + gpitch * (i2 + (long)sx2 * i3)] = (R)(gammadot * tauc / mu);
+      }
+  }
+  if (maxwelltime) *maxwelltime = std::min((double)tm, *maxwelltime);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// gauss, omega, my_sinh, shiftedcoordinates, fix
+// (elastic3d.f90:525-531, :549-562, :89-97, :4027-4048, :70-83)
+// ---------------------------------------------------------------------------
+static inline double gauss(double x, double sigma) {
+  return std::exp(-0.5 * (x / sigma) * (x / sigma)) / std::sqrt(PI2) / sigma;
+}
+static inline double omega(double x, double beta) {
+  double ax = std::fabs(x);
+  if (ax <= (1.0 - 2.0 * beta) / (1.0 - beta) / 2.0) return 1.0;
+  if (ax < 1.0 / (1 - beta) / 2.0) {
+    double c = std::cos(PI * ((1.0 - beta) * ax - 0.5 + beta) / 2.0 / beta);
+    return c * c;
+  }
+  return 0.0;
+}
+static inline double my_sinh(double x) {
+  if (std::fabs(x) > 11.0) return std::copysign(1.0, x) * std::sinh(11.0);
+  return std::sinh(x);
+}
+static inline double shifted1(int i /*0-based*/, int sx, double dx) {
+  int ib = i + 1;
+  if (ib <= sx / 2) return (double)(ib - 1) * dx;
+  return (double)(ib - sx - 1) * dx;
+}
+static inline int fix(double number) {
+  double f = std::floor(number), c = std::ceil(number);
+  return (number - f > 0.5) ? (int)c : (int)f;
+}
+
+// ---------------------------------------------------------------------------
+// frictioneigenstress, friction3d.f90:82-241 (CPU path)
+// ---------------------------------------------------------------------------
+template <class R>
+static int frictioneigenstress(double x, double y, double z, double L, double W, double strike, double dip,
+                               double rake, double beta, const R* sig, double mu, const orc_layer* structure,
+                               int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, R* moment,
+                               double* maxwelltime, R* vel, long vpitch) {
+  const T6<R>* S = reinterpret_cast<const T6<R>*>(sig);
+  T6<R>* M = reinterpret_cast<T6<R>*>(moment);
+  float tm = maxwelltime ? (float)*maxwelltime : 1e30f;
+  const double scaling = std::sqrt(PI2) * dx1;
+  const double cstrike = std::cos(strike), sstrike = std::sin(strike);
+  const double cdip = std::cos(dip), sdip = std::sin(dip);
+  const double cr = std::cos(rake), sr = std::sin(rake);
+  const double Wp = W * (1.0 + 2.0 * beta) / 2.0, Lp = L * (1.0 + 2.0 * beta) / 2.0;
+  double x2r = cstrike * x - sstrike * y;
+  const double xr = cdip * x2r - sdip * z;
+  const double yr = sstrike * x + cstrike * y;
+  const double zr = sdip * x2r + cdip * z;
+  const double n[3] = {+cdip * cstrike, -cdip * sstrike, -sdip};
+  const double r[3] = {sstrike * cr + cstrike * sdip * sr, cstrike * cr - sstrike * sdip * sr, cdip * sr};
+  int status = 0;
+#pragma omp parallel for reduction(min : tm) reduction(max : status)
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    double x3 = (double)i3 * dx3;
+    if ((std::fabs(x3 - z) > Lp) && (std::fabs(x3 + z) > Lp)) continue;
+    double vo = structure[i3].gammadot0, tauc = structure[i3].stressexponent;
+    double friction = structure[i3].friction, cohesion = structure[i3].cohesion;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        double x1 = shifted1(i1, sx1, dx1), x2 = shifted1(i2, sx2, dx2);
+        if ((std::fabs(x1 - x) > std::max(Wp, Lp)) || (std::fabs(x2 - y) > std::max(Wp, Lp))) continue;
+        double x2rr = cstrike * x1 - sstrike * x2;
+        double x1s = cdip * x2rr - sdip * x3;
+        double x1i = cdip * x2rr + sdip * x3;
+        if ((std::fabs(x1s - xr) > 7.01 * dx1) && (std::fabs(x1i - xr) > 7.01 * dx1)) continue;
+        double x2s = sstrike * x1 + cstrike * x2;
+        double x3s = sdip * x2rr + cdip * x3;
+        double x3i = -sdip * x2rr + cdip * x3;
+        double temp1 = gauss(x1s - xr, dx1);
+        double temp2 = omega((x2s - yr) / W, beta);
+        double temp3 = omega((x3s - zr) / L, beta);
+        double sourc = temp1 * temp2 * temp3;
+        temp1 = gauss(x1i - xr, dx1);
+        temp3 = omega((x3i + zr) / L, beta);
+        double image = temp1 * temp2 * temp3;
+        double impulse = MUT(11) ? sourc : sourc + image;
+        long o = i1 + (long)sx1 * (i2 + (long)sx2 * i3);
+        const T6<R>& s = S[o];
+        double t[3] = {s.s11 * n[0] + s.s12 * n[1] + s.s13 * n[2], s.s12 * n[0] + s.s22 * n[1] + s.s23 * n[2],
+                       s.s13 * n[0] + s.s23 * n[1] + s.s33 * n[2]};
+        double taun = t[0] * n[0] + t[1] * n[1] + t[2] * n[2];
+        double ts[3] = {t[0] - taun * n[0], t[1] - taun * n[1], t[2] - taun * n[2]};
+        double taus = std::sqrt(ts[0] * ts[0] + ts[1] * ts[1] + ts[2] * ts[2]);
+        double tau = std::max(0.0, taus + friction * taun - cohesion);
+        if ((ts[0] * r[0] + ts[1] * r[1] + ts[2] * r[2]) < 0.0 && std::fabs(rake) < PI2 * 1.5) continue;
+        if ((tau / tauc) > 20) { status = 5; continue; }
+        ts[0] /= taus; ts[1] /= taus; ts[2] /= taus;
+        double gammadot = vo * 2.0 * my_sinh(tau / tauc);
+        float cand = (float)(tau / mu / gammadot * (std::min(L, W) / std::sqrt(dx1 * dx3)));
+        if (cand < tm) tm = cand;  // MIN ignores a NaN candidate
+        if (vel) {
+          long vo2 = i1 + vpitch * (i2 + (long)sx2 * i3);
+          vel[vo2] = (R)(vel[vo2] + gammadot * impulse * scaling);
+        }
+        double a = 2.0 * mu * impulse * gammadot;
+        double b[3] = {a * n[0], a * n[1], a * n[2]};
+        T6<R>& m = M[o];
+        m.s11 += (R)(ts[0] * b[0]);
+        m.s12 += (R)((ts[0] * b[1] + ts[1] * b[0]) / 2.0);
+        m.s13 += (R)((ts[0] * b[2] + ts[2] * b[0]) / 2.0);
+        m.s22 += (R)(ts[1] * b[1]);
+        m.s23 += (R)((ts[1] * b[2] + ts[2] * b[1]) / 2.0);
+        m.s33 += (R)(ts[2] * b[2]);
+      }
+  }
+  if (status) return status;
+  if (maxwelltime) *maxwelltime = std::min((double)tm, *maxwelltime);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// monitorfriction, friction3d.f90:263-416; shiftedindex elastic3d.f90:4063-4101
+// ---------------------------------------------------------------------------
+template <class R>
+static int monitorfriction(double x, double y, double z, double L, double W, double strike, double dip,
+                           double rake, double beta, int sx1, int sx2, int sx3, int sx3_alloc, double dx1,
+                           double dx2, double dx3, const R* sig, const orc_layer* structure, orc_patch* patch,
+                           int px2, int px3) {
+  (void)L; (void)W; (void)beta;
+  const T6<R>* S = reinterpret_cast<const T6<R>*>(sig);
+  const double cstrike = std::cos(strike), sstrike = std::sin(strike);
+  const double cdip = std::cos(dip), sdip = std::sin(dip);
+  const double cr = std::cos(rake), sr = std::sin(rake);
+  const double sv[3] = {sstrike, cstrike, 0.0};
+  const double dv[3] = {-cstrike * sdip, +sstrike * sdip, -cdip};
+  const double n[3] = {+cdip * cstrike, -cdip * sstrike, -sdip};
+  const double r[3] = {sstrike * cr + cstrike * sdip * sr, cstrike * cr - sstrike * sdip * sr, cdip * sr};
+  for (int j3 = 1; j3 <= px3; ++j3)
+    for (int j2 = 1; j2 <= px2; ++j2) {
+      orc_patch& P = patch[(j2 - 1) + (long)px2 * (j3 - 1)];
+      double x2 = cstrike * x - sstrike * y;
+      double xr = cdip * x2 - sdip * z;
+      double yr = sstrike * x + cstrike * y;
+      double zr = sdip * x2 + cdip * z;
+      yr = (double)(float)yr + (double)(float)(((double)j2 - (double)px2 / 2.0 - 1.0) * dx2);
+      zr = (double)(float)zr + (double)(float)(((double)j3 - (double)px3 / 2.0 - 1.0) * dx3);
+      double x2p = cdip * xr + sdip * zr;
+      double X1 = cstrike * x2p + sstrike * yr;
+      double X2 = -sstrike * x2p + cstrike * yr;
+      double X3 = -sdip * xr + cdip * zr;
+      P.x1 = X1; P.x2 = X2; P.x3 = X3; P.lx = yr; P.lz = zr;
+      if ((X1 > (double)(sx1 / 2 - 1) * dx1) || (X1 < -(double)(sx1 / 2) * dx1) ||
+          (X2 > (double)(sx2 / 2 - 1) * dx2) || (X2 < -(double)(sx2 / 2) * dx2) ||
+          (X3 > (double)(sx3 - 1) * dx3) || (X3 < 0.0))
+        continue;
+      int i1 = (sx1 + fix(X1 / dx1)) % sx1;
+      int i2 = (sx2 + fix(X2 / dx2)) % sx2;
+      int i3 = fix(X3 / dx3);
+      if (i3 >= sx3_alloc) return 15;  // beyond the allocated stress field
+      double vo = structure[i3].gammadot0, tauc = structure[i3].stressexponent;
+      double friction = structure[i3].friction, cohesion = structure[i3].cohesion;
+      const T6<R>& s = S[i1 + (long)sx1 * (i2 + (long)sx2 * i3)];
+      P.sig[0] = (float)s.s11; P.sig[1] = (float)s.s12; P.sig[2] = (float)s.s13;
+      P.sig[3] = (float)s.s22; P.sig[4] = (float)s.s23; P.sig[5] = (float)s.s33;
+      double t[3] = {s.s11 * n[0] + s.s12 * n[1] + s.s13 * n[2], s.s12 * n[0] + s.s22 * n[1] + s.s23 * n[2],
+                     s.s13 * n[0] + s.s23 * n[1] + s.s33 * n[2]};
+      double taun = t[0] * n[0] + t[1] * n[1] + t[2] * n[2];
+      double ts[3] = {t[0] - taun * n[0], t[1] - taun * n[1], t[2] - taun * n[2]};
+      double taus = std::sqrt(ts[0] * ts[0] + ts[1] * ts[1] + ts[2] * ts[2]);
+      double tau = std::max(0.0, taus + friction * taun - cohesion);
+      if ((ts[0] * r[0] + ts[1] * r[1] + ts[2] * r[2]) < 0.0 && std::fabs(rake) < PI2 * 1.5) continue;
+      P.taus = taus;
+      P.v = vo * 2.0 * my_sinh(tau / tauc);
+      ts[0] /= taus; ts[1] /= taus; ts[2] /= taus;
+      P.vss = P.v * (ts[0] * sv[0] + ts[1] * sv[1] + ts[2] * sv[2]);
+      P.vds = P.v * (ts[0] * dv[0] + ts[1] * dv[1] + ts[2] * dv[2]);
+    }
+  return 0;
+}
+
+// frictionadd, friction3d.f90:424-453
+static void frictionadd(orc_patch* patch, long npatch, double dt) {
+  for (long k = 0; k < npatch; ++k) {
+    orc_patch& P = patch[k];
+    P.ss = P.ss + dt * P.vss;
+    P.ds = P.ds + dt * P.vds;
+    P.slip = std::pow(P.ds * P.ds + P.ss * P.ss, 0.5);
+  }
+}
+
+// tensoramplitude, elastic3d.f90:446-466
+template <class R>
+static double tensoramplitude(const R* t, long npts, double dx1, double dx2, double dx3) {
+  const T6<R>* T = reinterpret_cast<const T6<R>*>(t);
+  double a = 0;
+  for (long o = 0; o < npts; ++o) a += tensornorm(T[o]);
+  return a * (dx1 * dx2 * dx3);
+}
+
+// momentdensityshear, elastic3d.f90:2661-2750 (coseismic set-up of tau)
+template <class R>
+static void momentdensityshear(double mu, double slip, double x, double y, double z, double L, double W,
+                               double strike, double dip, double rake, double beta, int sx1, int sx2, int sx3,
+                               double dx1, double dx2, double dx3, R* sig) {
+  T6<R>* S = reinterpret_cast<T6<R>*>(sig);
+  const float rmu = 2.0f * (float)mu;
+  const double cstrike = std::cos(strike), sstrike = std::sin(strike);
+  const double cdip = std::cos(dip), sdip = std::sin(dip);
+  const double cr = std::cos(rake), sr = std::sin(rake);
+  const double Wp = W * (1.0 + 2.0 * beta) / 2.0, Lp = L * (1.0 + 2.0 * beta) / 2.0;
+  double x2r = cstrike * x - sstrike * y;
+  const double xr = cdip * x2r - sdip * z, yr = sstrike * x + cstrike * y, zr = sdip * x2r + cdip * z;
+#pragma omp parallel for
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    double x3 = (double)i3 * dx3;
+    if (std::fabs(x3 - z) > Lp) continue;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        double x1 = shifted1(i1, sx1, dx1), x2 = shifted1(i2, sx2, dx2);
+        if ((std::fabs(x1 - x) > Wp) || (std::fabs(x2 - y) > Wp)) continue;
+        double x2rr = cstrike * x1 - sstrike * x2;
+        double x1s = cdip * x2rr - sdip * x3, x1i = cdip * x2rr + sdip * x3;
+        if ((std::fabs(x1s - xr) > 7.01 * dx1) && (std::fabs(x1i - xr) > 7.01 * dx1)) continue;
+        double x2s = sstrike * x1 + cstrike * x2;
+        double x3s = sdip * x2rr + cdip * x3, x3i = -sdip * x2rr + cdip * x3;
+        double temp1 = gauss(x1s - xr, dx1), temp2 = omega((x2s - yr) / W, beta), temp3 = omega((x3s - zr) / L, beta);
+        double aperture = temp1 * temp2 * temp3;
+        temp1 = gauss(x1i - xr, dx1);
+        temp3 = omega((x3i + zr) / L, beta);
+        aperture = aperture + temp1 * temp2 * temp3;
+        double n[3] = {+cdip * cstrike * aperture, -cdip * sstrike * aperture, -sdip * aperture};
+        double s[3] = {sstrike * cr + cstrike * sdip * sr, cstrike * cr - sstrike * sdip * sr, cdip * sr};
+        for (int k = 0; k < 3; ++k) s[k] *= slip;
+        R e[6] = {(R)(n[0] * s[0]), (R)((n[0] * s[1] + n[1] * s[0]) / 2.0), (R)((n[0] * s[2] + n[2] * s[0]) / 2.0),
+                  (R)(n[1] * s[1]), (R)((n[1] * s[2] + n[2] * s[1]) / 2.0), (R)(n[2] * s[2])};
+        T6<R>& o = S[i1 + (long)sx1 * (i2 + (long)sx2 * i3)];
+        o.s11 += (R)rmu * e[0]; o.s12 += (R)rmu * e[1]; o.s13 += (R)rmu * e[2];
+        o.s22 += (R)rmu * e[3]; o.s23 += (R)rmu * e[4]; o.s33 += (R)rmu * e[5];
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// RK2 loop: relax.f90:555-890 with integrationstep :1420-1452 and the
+// isoutput counter :1364-1376 (oi advances whenever isoutput holds).
+// Mechanisms: linear and power-law viscosity and fault creep; no transient
+// creep, no interseismic loading, no time-dependent surface loads and no
+// coseismic events after t=0 (event times only truncate Dt).
+// ---------------------------------------------------------------------------
+extern "C" {
+struct orc_model {
+  int sx1, sx2, sx3;
+  double dx1, dx2, dx3, lambda, mu, gam, beta;
+  const orc_layer* linear;     // sx3/2 entries or NULL
+  const orc_layer* nonlinear;  // sx3/2 entries or NULL
+  const orc_layer* faultcreep; // sx3/2 entries or NULL
+  const orc_tensor_layer* prestress;  // sx3/2 entries
+  int np;
+  const orc_plane* planes;
+  orc_patch* patches;  // concatenated px2*px3 per plane
+  double interval, odt, tscale;
+  int skip;
+  int ne;
+  const double* event_times;
+};
+}
+
+static const double STEP_MAX = 1e7;
+
+template <class R>
+static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* gamma, double* t_io, int* oi_io,
+                   int* i_io, int nsteps, const double* dt_replay, double* info) {
+  const int sx1 = m->sx1, sx2 = m->sx2, sx3 = m->sx3, h3 = sx3 / 2;
+  const long pitch = sx1 + 2;
+  const long nv = pitch * sx2 * sx3, nu = pitch * sx2 * h3, nt = pitch * sx2, npts = (long)sx1 * sx2 * h3;
+  std::vector<R> v1(nv), v2(nv), v3(nv), t1(nt), t2(nt), t3(nt), moment(6 * npts, 0);
+  double t = *t_io;
+  int oi = *oi_io, it = *i_io;
+  const int e = 1;  // single event at t=0
+  for (int step = 0; step < nsteps; ++step) {
+    if (t >= m->interval) break;
+    ++it;
+    double maxwell[5] = {STEP_MAX, STEP_MAX, STEP_MAX, STEP_MAX, STEP_MAX};
+    double mech[5] = {0, 0, 0, 0, 0};
+    std::fill(moment.begin(), moment.end(), (R)0);
+    int rc;
+    if (m->linear) {
+      rc = viscouseigenstress(m->mu, m->linear, m->prestress, sig, sx1, sx2, h3, moment.data(), &maxwell[0], (const R*)0, (R*)0, pitch);
+      if (rc) return rc;
+      mech[0] = 1;
+    }
+    if (m->nonlinear) {
+      rc = viscouseigenstress(m->mu, m->nonlinear, m->prestress, sig, sx1, sx2, h3, moment.data(), &maxwell[1], (const R*)0, (R*)0, pitch);
+      if (rc) return rc;
+      mech[1] = 1;
+    }
+    if (m->faultcreep) {
+      for (int k = 0; k < m->np; ++k) {
+        const orc_plane& P = m->planes[k];
+        rc = frictioneigenstress(P.x, P.y, P.z, P.width, P.length, P.strike, P.dip, P.rake, m->beta, sig, m->mu,
+                                 m->faultcreep, sx1, sx2, h3, m->dx1, m->dx2, m->dx3, moment.data(), &maxwell[2],
+                                 (R*)0, pitch);
+        if (rc) return rc;
+      }
+      mech[2] = 1;
+    }
+    double tm = 1.0 / (mech[0] / maxwell[0] + mech[1] / maxwell[1] + mech[2] / maxwell[2] + mech[3] / maxwell[3] +
+                       mech[4] / maxwell[4]);
+    tm = std::min(tm, STEP_MAX);
+    // integrationstep, relax.f90:1420-1452
+    double Dt = tm / 10.0;
+    if (0 == m->skip) {
+      if ((t + Dt) >= ((double)oi * m->odt) - Dt * 0.04) Dt = (double)oi * m->odt - t;
+    } else {
+      Dt = Dt * m->tscale;
+      if (e < m->ne) {
+        if ((t + Dt - m->event_times[e]) >= 0.0) Dt = m->event_times[e] - t;
+      }
+    }
+    if (dt_replay) Dt = dt_replay[step];
+    // sig <- moment; v = 0; t = 0; equivalentbodyforce; green
+    for (long o = 0; o < 6 * npts; ++o) sig[o] = moment[o];
+    std::fill(v1.begin(), v1.end(), (R)0); std::fill(v2.begin(), v2.end(), (R)0); std::fill(v3.begin(), v3.end(), (R)0);
+    std::fill(t1.begin(), t1.end(), (R)0); std::fill(t2.begin(), t2.end(), (R)0); std::fill(t3.begin(), t3.end(), (R)0);
+    rc = equivalentbodyforce(sig, m->dx1, m->dx2, m->dx3, sx1, sx2, h3, v1.data(), v2.data(), v3.data(), t1.data(),
+                             t2.data(), t3.data(), pitch, (const R*)0);
+    if (rc) return rc;
+    rc = greenfunctioncowling(v1.data(), v2.data(), v3.data(), t1.data(), t2.data(), t3.data(), sx1, sx2, sx3, m->dx1,
+                              m->dx2, m->dx3, m->lambda, m->mu, m->gam);
+    if (rc) return rc;
+    const R hdt = (R)(MUT(13) ? Dt : Dt / 2);
+    for (long o = 0; o < nu; ++o) {
+      v1[o] = hdt * v1[o] + u1[o];
+      v2[o] = hdt * v2[o] + u2[o];
+      v3[o] = hdt * v3[o] + u3[o];
+    }
+    for (long o = 0; o < 6 * npts; ++o) sig[o] = (-hdt) * sig[o] + (R)(-1) * tau[o];
+    // corrector
+    stressupdate(v1.data(), v2.data(), v3.data(), m->lambda, m->mu, m->dx1, m->dx2, m->dx3, sx1, sx2, h3, sig, pitch);
+    std::fill(moment.begin(), moment.end(), (R)0);
+    const orc_layer* visc[2] = {m->linear, m->nonlinear};
+    for (int k = 0; k < 2; ++k) {
+      if (!visc[k]) continue;
+      std::fill(v1.begin(), v1.end(), (R)0);
+      rc = viscouseigenstress(m->mu, visc[k], m->prestress, sig, sx1, sx2, h3, moment.data(), (double*)0, (const R*)0, v1.data(), pitch);
+      if (rc) return rc;
+      const R cdt = (R)Dt;
+      for (long o = 0; o < nu; ++o) gamma[o] = gamma[o] + cdt * v1[o];
+    }
+    if (m->faultcreep) {
+      long off = 0;
+      for (int k = 0; k < m->np; ++k) {
+        const orc_plane& P = m->planes[k];
+        rc = frictioneigenstress(P.x, P.y, P.z, P.width, P.length, P.strike, P.dip, P.rake, m->beta, sig, m->mu,
+                                 m->faultcreep, sx1, sx2, h3, m->dx1, m->dx2, m->dx3, moment.data(), (double*)0,
+                                 (R*)0, pitch);
+        if (rc) return rc;
+        if (m->patches) {
+          rc = monitorfriction(P.x, P.y, P.z, P.width, P.length, P.strike, P.dip, P.rake, m->beta, sx1, sx2, sx3, h3,
+                               m->dx1, m->dx2, m->dx3, sig, m->faultcreep, m->patches + off, P.px2, P.px3);
+          if (rc) return rc;
+          off += (long)P.px2 * P.px3;
+        }
+      }
+    }
+    std::fill(v1.begin(), v1.end(), (R)0); std::fill(v2.begin(), v2.end(), (R)0); std::fill(v3.begin(), v3.end(), (R)0);
+    std::fill(t1.begin(), t1.end(), (R)0); std::fill(t2.begin(), t2.end(), (R)0); std::fill(t3.begin(), t3.end(), (R)0);
+    rc = equivalentbodyforce(moment.data(), m->dx1, m->dx2, m->dx3, sx1, sx2, h3, v1.data(), v2.data(), v3.data(),
+                             t1.data(), t2.data(), t3.data(), pitch, (const R*)0);
+    if (rc) return rc;
+    rc = greenfunctioncowling(v1.data(), v2.data(), v3.data(), t1.data(), t2.data(), t3.data(), sx1, sx2, sx3, m->dx1,
+                              m->dx2, m->dx3, m->lambda, m->mu, m->gam);
+    if (rc) return rc;
+    const R cdt = (R)Dt;
+    for (long o = 0; o < nu; ++o) {
+      u1[o] = u1[o] + cdt * v1[o];
+      u2[o] = u2[o] + cdt * v2[o];
+      u3[o] = u3[o] + cdt * v3[o];
+    }
+    for (long o = 0; o < 6 * npts; ++o) tau[o] = tau[o] + cdt * moment[o];
+    if (m->patches) {
+      long off = 0;
+      for (int k = 0; k < m->np; ++k) {
+        frictionadd(m->patches + off, (long)m->planes[k].px2 * m->planes[k].px3, Dt);
+        off += (long)m->planes[k].px2 * m->planes[k].px3;
+      }
+    }
+    t = t + Dt;
+    for (long o = 0; o < 6 * npts; ++o) sig[o] = (R)(-1) * tau[o];
+    stressupdate(u1, u2, u3, m->lambda, m->mu, m->dx1, m->dx2, m->dx3, sx1, sx2, h3, sig, pitch);
+    // isoutput, relax.f90:1364-1376, advances the output counter
+    double etime = m->ne > 0 ? m->event_times[e - 1] : 0.0;
+    bool out = ((0 == m->skip) && (std::fabs(t - oi * m->odt) < 1e-6 * m->odt)) ||
+               ((0 < m->skip) && ((it - 1) % m->skip == 0)) || (std::fabs(t - etime) < 1e-6);
+    if (info) {
+      double* I = info + 8 * step;
+      I[0] = Dt; I[1] = tm; I[2] = maxwell[0]; I[3] = maxwell[1]; I[4] = maxwell[2];
+      I[5] = tensoramplitude(moment.data(), npts, m->dx1, m->dx2, m->dx3);
+      I[6] = tensoramplitude(tau, npts, m->dx1, m->dx2, m->dx3);
+      I[7] = t;
+    }
+    if (out) ++oi;
+  }
+  *t_io = t; *oi_io = oi; *i_io = it;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI of the oracle (suffix _f: FP32 storage, _d: FP64 storage)
+// ---------------------------------------------------------------------------
+#define ORC_EXPORT extern "C" __attribute__((visibility("default")))
+#define ORC_BOTH(SUFFIX, R)                                                                                          \
+  ORC_EXPORT int orc_fft3_##SUFFIX(R* d, int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, int dir) { \
+    return fft3(d, sx1, sx2, sx3, dx1, dx2, dx3, dir);                                                               \
+  }                                                                                                                  \
+  ORC_EXPORT int orc_fft2_##SUFFIX(R* d, int sx1, int sx2, double dx1, double dx2, int dir) {                        \
+    return fft2(d, sx1, sx2, dx1, dx2, dir);                                                                         \
+  }                                                                                                                  \
+  ORC_EXPORT void orc_elasticresponse_##SUFFIX(double lambda, double mu, R* f1, R* f2, R* f3, int sx1, int sx2,    \
+                                               int sx3, double dx1, double dx2, double dx3) {                        \
+    elasticresponse(lambda, mu, f1, f2, f3, sx1, sx2, sx3, dx1, dx2, dx3);                                           \
+  }                                                                                                                  \
+  ORC_EXPORT void orc_surfacetractioncowling_##SUFFIX(double lambda, double mu, double gamma, const R* u1,         \
+                                                      const R* u2, const R* u3, int sx1, int sx2, int sx3,          \
+                                                      double dx1, double dx2, double dx3, R* p1, R* p2, R* p3) {    \
+    surfacetractioncowling(lambda, mu, gamma, u1, u2, u3, sx1, sx2, sx3, dx1, dx2, dx3, p1, p2, p3);                 \
+  }                                                                                                                  \
+  ORC_EXPORT void orc_cerrutimodified_##SUFFIX(const R* p1, const R* p2, const R* p3, double lambda, double mu,    \
+                                               double gamma, R* u1, R* u2, R* u3, int sx1, int sx2, int sx3,        \
+                                               double dx1, double dx2, double dx3) {                                 \
+    cerrutimodified(p1, p2, p3, lambda, mu, gamma, u1, u2, u3, sx1, sx2, sx3, dx1, dx2, dx3);                        \
+  }                                                                                                                  \
+  ORC_EXPORT int orc_greenfunctioncowling_##SUFFIX(R* c1, R* c2, R* c3, R* t1, R* t2, R* t3, int sx1, int sx2,     \
+                                                   int sx3, double dx1, double dx2, double dx3, double lambda,      \
+                                                   double mu, double gamma) {                                        \
+    return greenfunctioncowling(c1, c2, c3, t1, t2, t3, sx1, sx2, sx3, dx1, dx2, dx3, lambda, mu, gamma);            \
+  }                                                                                                                  \
+  ORC_EXPORT int orc_stressupdate_##SUFFIX(const R* v1, const R* v2, const R* v3, double lambda, double mu,        \
+                                           double dx1, double dx2, double dx3, int sx1, int sx2, int sx3, R* sig) {  \
+    return stressupdate(v1, v2, v3, lambda, mu, dx1, dx2, dx3, sx1, sx2, sx3, sig, (long)sx1 + 2);                   \
+  }                                                                                                                  \
+  ORC_EXPORT int orc_equivalentbodyforce_##SUFFIX(const R* sig, double dx1, double dx2, double dx3, int sx1,       \
+                                                  int sx2, int sx3, R* c1, R* c2, R* c3, R* t1, R* t2, R* t3,       \
+                                                  const R* mask) {                                                   \
+    return equivalentbodyforce(sig, dx1, dx2, dx3, sx1, sx2, sx3, c1, c2, c3, t1, t2, t3, (long)sx1 + 2, mask);      \
+  }                                                                                                                  \
+  ORC_EXPORT int orc_viscouseigenstress_##SUFFIX(double mu, const orc_layer* st, const orc_tensor_layer* pre,      \
+                                                 const R* sig, int sx1, int sx2, int sx3, R* moment,                \
+                                                 double* maxwelltime, const R* dg0, R* gamma) {                      \
+    return viscouseigenstress(mu, st, pre, sig, sx1, sx2, sx3, moment, maxwelltime, dg0, gamma, (long)sx1 + 2);      \
+  }                                                                                                                  \
+  ORC_EXPORT int orc_frictioneigenstress_##SUFFIX(double x, double y, double z, double L, double W, double strike, \
+                                                  double dip, double rake, double beta, const R* sig, double mu,    \
+                                                  const orc_layer* st, int sx1, int sx2, int sx3, double dx1,       \
+                                                  double dx2, double dx3, R* moment, double* maxwelltime, R* vel) { \
+    return frictioneigenstress(x, y, z, L, W, strike, dip, rake, beta, sig, mu, st, sx1, sx2, sx3, dx1, dx2, dx3,    \
+                               moment, maxwelltime, vel, (long)sx1 + 2);                                             \
+  }                                                                                                                  \
+  ORC_EXPORT int orc_monitorfriction_##SUFFIX(double x, double y, double z, double L, double W, double strike,     \
+                                              double dip, double rake, double beta, int sx1, int sx2, int sx3,      \
+                                              int sx3_alloc, double dx1, double dx2, double dx3, const R* sig,      \
+                                              const orc_layer* st, orc_patch* patch, int px2, int px3) {            \
+    return monitorfriction(x, y, z, L, W, strike, dip, rake, beta, sx1, sx2, sx3, sx3_alloc, dx1, dx2, dx3, sig,     \
+                           st, patch, px2, px3);                                                                     \
+  }                                                                                                                  \
+  ORC_EXPORT double orc_tensoramplitude_##SUFFIX(const R* t, long npts, double dx1, double dx2, double dx3) {      \
+    return tensoramplitude(t, npts, dx1, dx2, dx3);                                                                  \
+  }                                                                                                                  \
+  ORC_EXPORT void orc_momentdensityshear_##SUFFIX(double mu, double slip, double x, double y, double z, double L,  \
+                                                  double W, double strike, double dip, double rake, double beta,    \
+                                                  int sx1, int sx2, int sx3, double dx1, double dx2, double dx3,    \
+                                                  R* sig) {                                                          \
+    momentdensityshear(mu, slip, x, y, z, L, W, strike, dip, rake, beta, sx1, sx2, sx3, dx1, dx2, dx3, sig);         \
+  }                                                                                                                  \
+  ORC_EXPORT int orc_rk2_run_##SUFFIX(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* gamma,           \
+                                      double* t, int* oi, int* it, int nsteps, const double* dt_replay,             \
+                                      double* info) {                                                                \
+    return rk2_run(m, u1, u2, u3, tau, sig, gamma, t, oi, it, nsteps, dt_replay, info);                              \
+  }
+
+ORC_BOTH(f, float)
+ORC_BOTH(d, double)
+
+ORC_EXPORT void orc_frictionadd(orc_patch* patch, long npatch, double dt) { frictionadd(patch, npatch, dt); }
+ORC_EXPORT void orc_set_mutation(int k) { g_mut = k; }
+ORC_EXPORT int orc_sizeof_patch(void) { return (int)sizeof(orc_patch); }
+ORC_EXPORT int orc_sizeof_model(void) { return (int)sizeof(orc_model); }
