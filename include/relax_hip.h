@@ -1,0 +1,222 @@
+/*
+ * relax_hip.h -- C ABI of the MI355X (gfx950) implementation of RELAX's
+ * per-time-step elastic + relaxation solve.
+ *
+ * Two layers:
+ *  1. Per-routine entry points with host pointers, one per reference
+ *     subroutine of the hot path.  Arguments keep the reference's order and
+ *     meaning; arrays are plain pointers plus explicit sizes.  Each call copies
+ *     its arrays to the GPU, runs the HIP kernels and copies the results back.
+ *     `align_data` selects the reference's space-domain layout: 1 = padded rows
+ *     of sx1+2 floats (ALIGN_DATA builds: FFTW, MKL, SGI; src/include.f90:45-56),
+ *     0 = packed rows of sx1 floats (the built-in ctfft build).
+ *  2. A device-resident model handle for the RK2 loop (relax.f90:555-890):
+ *     the whole state stays in HBM between steps and one call advances it by
+ *     one predictor/corrector step.
+ *
+ * Every function returns a status: RELAX_OK or one of the RELAX_ERR_* codes.
+ * Nothing in this library exits the process; the reference's STOP
+ * conditions map to the codes below.
+ *
+ * Paths in citations are relative to the reference's src/ directory.
+ */
+#ifndef RELAX_HIP_H
+#define RELAX_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define RELAX_OK 0
+#define RELAX_ERR_POWER 1      /* stress exponent < 1 (viscoelastic3d.f90:262-267, STOP 1) */
+#define RELAX_ERR_FILTER 2     /* dimension too small for a FIR filter (elastic3d.f90:886, STOP 2) */
+#define RELAX_ERR_FRICTION 5   /* tau/tauc > 20 in fault creep (friction3d.f90:204-215, STOP 5) */
+#define RELAX_ERR_ARG 10       /* invalid argument (null pointer, bad flag) */
+#define RELAX_ERR_SIZE 11      /* grid size not supported (powers of two, 16..1024) */
+#define RELAX_ERR_NOMEM 12     /* device allocation failed */
+#define RELAX_ERR_HIP 13       /* HIP runtime error */
+#define RELAX_ERR_ADMISSION 14 /* model does not fit the HBM budget (cuelastic.cu:2096-2137 analogue) */
+#define RELAX_ERR_RANGE 15     /* patch centre outside the stress field (elastic3d.f90:4068-4095) */
+#define RELAX_ERR_NODEVICE 16  /* no HIP device */
+
+/* ---- ABI structures: the reference's SEQUENCE types (types.f90) ---------- */
+typedef struct { float s11, s12, s13, s22, s23, s33; } relax_tensor_t;          /* types.f90:41-44 */
+typedef struct { double z, gammadot0, stressexponent, cohesion, friction, Gk; } relax_layer_t; /* types.f90:31-34 */
+typedef struct { float z, dum; relax_tensor_t t; } relax_tensor_layer_t;        /* types.f90:52-56 */
+typedef struct {                                                                 /* types.f90:76-90 */
+  double x1, x2, x3, lx, lz, slip, ss, ds, v, vss, vds, taus;
+  relax_tensor_t sig;
+} relax_slippatch_t;
+typedef struct {                                                                 /* types.f90:92-97 */
+  double x, y, z, width, length, strike, dip, rake;
+  int px2, px3; /* the patch array is passed separately */
+} relax_plane_t;
+
+/* ---- library ------------------------------------------------------------ */
+const char* relax_hip_version(void);
+const char* relax_hip_status_string(int status);
+int relax_hip_init(int device); /* selects the device; optional */
+
+/* ---- fourier.f90 --------------------------------------------------------- */
+/* fft3, fourier.f90:177-201 (FFTW) / :330-355 (ctfft): in-place R2C
+ * (direction -1, scaled by dx1*dx2*dx3) or C2R (direction +1, scaled by
+ * 1/(sx1*dx1*sx2*dx2*sx3*dx3)) of a (sx1+2)*sx2*sx3 float buffer. */
+int relax_fft3(float* data, int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, int direction,
+               int align_data);
+/* fft2, fourier.f90:372-396 / :510-535, the same in two dimensions. */
+int relax_fft2(float* data, int sx1, int sx2, double dx1, double dx2, int direction, int align_data);
+
+/* ---- green.f90 (Fourier-domain arrays, complex interleaved, sx1/2+1 per row) */
+/* elasticresponse, green.f90:51-101 */
+int relax_elasticresponse(double lambda, double mu, float* f1, float* f2, float* f3, int sx1, int sx2, int sx3,
+                          double dx1, double dx2, double dx3);
+/* surfacetractioncowling, green.f90:327-377; p1..p3 are (sx1+2)*sx2 outputs */
+int relax_surfacetractioncowling(double lambda, double mu, double gamma, const float* u1, const float* u2,
+                                 const float* u3, int sx1, int sx2, int sx3, double dx1, double dx2, double dx3,
+                                 float* p1, float* p2, float* p3);
+/* cerrutimodified, green.f90:520-612 */
+int relax_cerrutimodified(const float* p1, const float* p2, const float* p3, double lambda, double mu,
+                          double gamma, float* u1, float* u2, float* u3, int sx1, int sx2, int sx3, double dx1,
+                          double dx2, double dx3);
+/* greenfunctioncowling, green.f90:1008-1129: body forces c1..c3 (full grid,
+ * (sx1+2)*sx2*sx3) and surface tractions t1..t3 ((sx1+2)*sx2) in, the
+ * displacement field out in c1..c3 (in place). */
+int relax_greenfunctioncowling(float* c1, float* c2, float* c3, float* t1, float* t2, float* t3, int sx1,
+                               int sx2, int sx3, double dx1, double dx2, double dx3, double lambda, double mu,
+                               double gamma, int align_data);
+
+/* ---- elastic3d.f90 ------------------------------------------------------- */
+/* stressupdate, elastic3d.f90:956-1180: sig += C:eps(v); sx3 is the depth
+ * extent of the tensor field (half the FFT grid in the RK2 loop). */
+int relax_stressupdate(const float* v1, const float* v2, const float* v3, double lambda, double mu, double dx1,
+                       double dx2, double dx3, int sx1, int sx2, int sx3, relax_tensor_t* sig, int align_data);
+/* equivalentbodyforce, elastic3d.f90:1214-1506: c -= div(sig),
+ * t += sig(:,:,1).e3; mask (sx3 floats) may be NULL. */
+int relax_equivalentbodyforce(const relax_tensor_t* sig, double dx1, double dx2, double dx3, int sx1, int sx2,
+                              int sx3, float* c1, float* c2, float* c3, float* t1, float* t2, float* t3,
+                              const float* mask, int align_data);
+/* fieldadd, elastic3d.f90:3774-3812: u = c1*u + c2*v over n floats
+ * (c1, c2 may be NULL = absent optional argument). */
+int relax_fieldadd(float* u, const float* v, long n, const float* c1, const float* c2);
+/* tensorfieldadd, elastic3d.f90:3880-3959: t1 = c1*t1 + c2*t2 over n tensors */
+int relax_tensorfieldadd(relax_tensor_t* t1, const relax_tensor_t* t2, long n, const float* c1, const float* c2);
+/* tensoramplitude, elastic3d.f90:446-466 */
+int relax_tensoramplitude(const relax_tensor_t* t, long n, double dx1, double dx2, double dx3, double* out);
+/* momentdensityshear, elastic3d.f90:2661-2750 (coseismic set-up of tau) */
+int relax_momentdensityshear(double mu, double slip, double x, double y, double z, double L, double W,
+                             double strike, double dip, double rake, double beta, int sx1, int sx2, int sx3,
+                             double dx1, double dx2, double dx3, relax_tensor_t* sig);
+
+/* ---- viscoelastic3d.f90 / friction3d.f90 --------------------------------- */
+/* viscouseigenstress, viscoelastic3d.f90:217-314.  maxwelltime, dgammadot0
+ * and gamma are the optional arguments (NULL = absent). */
+int relax_viscouseigenstress(double mu, const relax_layer_t* structure, int nstructure,
+                             const relax_tensor_layer_t* prestress, int nprestress, const relax_tensor_t* sig,
+                             int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, relax_tensor_t* moment,
+                             double* maxwelltime, const float* dgammadot0, float* gamma, int align_data);
+/* frictioneigenstress, friction3d.f90:82-241 (CPU path semantics) */
+int relax_frictioneigenstress(double x, double y, double z, double L, double W, double strike, double dip,
+                              double rake, double beta, const relax_tensor_t* sig,
+                              const relax_tensor_layer_t* prestress, double mu, const relax_layer_t* structure,
+                              int sx1, int sx2, int sx3, double dx1, double dx2, double dx3,
+                              relax_tensor_t* moment, double* maxwelltime, float* vel, int align_data);
+/* monitorfriction, friction3d.f90:263-416; sx3 is the depth bound used for
+ * the range test (the driver passes the full grid), sx3_alloc the depth of
+ * the stress array. patch is px2*px3 (j2 fastest). */
+int relax_monitorfriction(double x, double y, double z, double L, double W, double strike, double dip,
+                          double rake, double beta, int sx1, int sx2, int sx3, int sx3_alloc, double dx1,
+                          double dx2, double dx3, const relax_tensor_t* sig, const relax_tensor_layer_t* prestress,
+                          const relax_layer_t* structure, relax_slippatch_t* patch, int px2, int px3);
+/* frictionadd, friction3d.f90:424-453, for the patches of one plane */
+int relax_frictionadd(relax_slippatch_t* patch, long npatch, double dt);
+
+/* ---- device-resident model: the RK2 loop --------------------------------- */
+typedef struct relax_model relax_model_t;
+
+typedef struct {
+  int sx1, sx2, sx3;                      /* full FFT grid (powers of two) */
+  double dx1, dx2, dx3;
+  double lambda, mu, gamma;               /* gamma: Cowling coefficient (in%gam) */
+  double beta;                            /* taper width of the friction planes */
+  double interval, odt, tscale;           /* time line (relax.f90:1420-1452) */
+  int skip;
+  int ne;                                 /* number of event times */
+  const double* event_times;              /* ne entries; event 1 is at t=0 */
+} relax_model_desc_t;
+
+typedef struct {
+  int i;              /* iteration index after the step (relax.f90:555) */
+  double Dt, tm;      /* time step and relaxation time of the step */
+  double maxwell[3];  /* linear, power-law, fault creep (STEP_MAX when inactive) */
+  double power;       /* tensoramplitude(moment), relax.f90:1027-1046 */
+  double cei;         /* tensoramplitude(tau) */
+  double t;           /* time after the step */
+  int oi;             /* output counter after the step */
+  int status;         /* RELAX_OK or the error of the step */
+} relax_step_info_t;
+
+/* Bytes of HBM one model of this size keeps resident (for admission). */
+size_t relax_model_footprint(const relax_model_desc_t* desc);
+int relax_model_create(const relax_model_desc_t* desc, relax_model_t** out);
+int relax_model_destroy(relax_model_t* m);
+/* which: 0 = linear viscosity (in%linearstruc), 1 = power law
+ * (in%nonlinearstruc), 2 = fault creep (in%faultcreepstruc).  layers are
+ * the per-depth samples of viscoelasticstructure (sx3/2 entries), or NULL
+ * to remove the mechanism. */
+int relax_model_set_structure(relax_model_t* m, int which, const relax_layer_t* layers, int n);
+int relax_model_set_prestress(relax_model_t* m, const relax_tensor_layer_t* prestress, int n);
+/* friction planes (in%n), each with its px2*px3 patches (may be NULL) */
+int relax_model_set_planes(relax_model_t* m, const relax_plane_t* planes, int np,
+                           const relax_slippatch_t* patches);
+int relax_model_get_patches(relax_model_t* m, relax_slippatch_t* patches);
+/* State in the reference's layout: u1..u3 and gamma are sx3/2 planes of
+ * rows of sx1+2 (align_data=1) or sx1 (align_data=0) floats; tau and sig are
+ * TENSOR fields (sx1, sx2, sx3/2).  sig may be NULL on upload: it is then
+ * computed as -tau + C:eps(u) (relax.f90:369-371). */
+int relax_model_upload(relax_model_t* m, const float* u1, const float* u2, const float* u3,
+                       const relax_tensor_t* tau, const relax_tensor_t* sig, const float* gamma, double t,
+                       int oi, int i, int align_data);
+int relax_model_download(relax_model_t* m, float* u1, float* u2, float* u3, relax_tensor_t* tau,
+                         relax_tensor_t* sig, float* gamma, double* t, int align_data);
+/* One RK2 predictor/corrector step (relax.f90:555-890).  dt_override
+ * replaces the adaptive step when not NULL (replay of a Dt sequence).
+ * info may be NULL.  The step is asynchronous on the model's stream unless
+ * info is requested. */
+int relax_model_step(relax_model_t* m, const double* dt_override, relax_step_info_t* info);
+/* Run up to nsteps steps or until t >= interval; info holds nsteps entries
+ * or is NULL.  *done receives the number of steps taken. */
+int relax_model_run(relax_model_t* m, int nsteps, const double* dt_override, relax_step_info_t* info, int* done);
+int relax_model_synchronize(relax_model_t* m);
+/* Coseismic set-up on the device.  add_shear adds momentdensityshear
+ * (elastic3d.f90:2661-2750) of n rectangular patches (centre x,y,z, width,
+ * length, strike, dip, rake in the internal convention, input.f90:1780-1800)
+ * with slip[k] to tau, as dislocations does (relax.f90:1199-1206).
+ * coseismic then sets u = greenfunctioncowling(equivalentbodyforce(tau)) and
+ * sig = -tau + C:eps(u) (relax.f90:318-371 with the FIR divergence of tau in
+ * place of the analytic source() body force). */
+int relax_model_add_shear(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n);
+int relax_model_coseismic(relax_model_t* m);
+/* Phase timers (the HIP-event counterpart of the reference's PAPI phase
+ * timers, papi_prof.c): runs nsteps steps with an event after every launch
+ * and returns, per kernel kind, the summed milliseconds and the number of
+ * launches (arrays of RELAX_PROF_KINDS entries). */
+#define RELAX_PROF_BODYFORCE 0    /* rheology + divergence stencil */
+#define RELAX_PROF_FFT_X_FWD 1    /* x pass, R2C */
+#define RELAX_PROF_FFT_Y 2        /* x2 pass, both directions */
+#define RELAX_PROF_GREEN_Z 3      /* x3 pass + Green + traction + Cerruti */
+#define RELAX_PROF_FFT_X_INV 4    /* x pass, C2R + stage update */
+#define RELAX_PROF_STRESS_PRED 5  /* strain stencil + predictor update */
+#define RELAX_PROF_STRESS_CORR 6  /* strain stencil + corrector update */
+#define RELAX_PROF_SMALL 7        /* traction planes, Nyquist planes, scalars */
+#define RELAX_PROF_KINDS 8
+int relax_model_profile(relax_model_t* m, int nsteps, double* ms, int* launches);
+/* The model's HIP stream (hipStream_t) for callers that order their own work. */
+void* relax_model_stream(relax_model_t* m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,50 @@
+"""Build the gfx950 HIP library relax_amd/librelax_hip.so with hipcc.
+
+Usage: python -m relax_amd.build   (or relax_amd.build.build() from Python)
+"""
+import concurrent.futures
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "librelax_hip.so")
+OBJ = os.path.join(HERE, "_build")
+SOURCES = ["fft.hip", "green.hip", "stencil.hip", "model.hip", "compat.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-I" + os.path.join(HERE, "..", "include")]
+
+
+def _compile(src):
+    obj = os.path.join(OBJ, src.replace(".hip", ".o"))
+    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    return src, obj, r.returncode, r.stdout + r.stderr
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        results = list(ex.map(_compile, SOURCES))
+    errs = [(s, log) for s, _, rc, log in results if rc != 0]
+    for s, _, rc, log in results:
+        if verbose or rc != 0:
+            filtered = "\n".join(l for l in log.splitlines() if "packed-fp32-ops" not in l)
+            if filtered.strip():
+                sys.stderr.write("== %s ==\n%s\n" % (s, filtered))
+    if errs:
+        raise RuntimeError("hipcc failed for: " + ", ".join(s for s, _ in errs))
+    tmp = OUT + ".tmp"
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for _, o, _, _ in results]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,100 @@
+// common.h -- shared definitions of the gfx950 implementation.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/relax_hip.h"
+
+#define RELAX_PI 3.141592653589793115997963468544185161
+#define RELAX_PI2 6.28318530717958623199592693708837032318
+
+// Return a status from a HIP call.
+#define HIP_TRY(expr)                                     \
+  do {                                                    \
+    hipError_t e_ = (expr);                               \
+    if (e_ != hipSuccess) return relax_hip_error(e_, #expr); \
+  } while (0)
+
+int relax_hip_error(hipError_t e, const char* what);
+
+// ----------------------------------------------------------------- complex --
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// multiply by +i or -i
+__device__ __forceinline__ float2 mul_i(float2 a) { return make_float2(-a.y, a.x); }
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+// --------------------------------------------------------------- grid info --
+// Wavenumber in cycles per unit length (fourier.f90:64-81), 0-based index.
+__host__ __device__ __forceinline__ float wavenumber23f(int i, int n, float inv_ndx) {
+  return (float)(i < n / 2 ? i : i - n) * inv_ndx;
+}
+
+// Per-model constant block passed by value to the kernels.
+struct Grid {
+  int n1, n2, n3, h3;  // full FFT grid and the depth of the half grid
+  int m1;              // n1/2: complex samples of an x row without the Nyquist bin
+  long H;              // n1*n2*h3 points of the half grid
+  float dx1, dx2, dx3;
+};
+
+// Tensor fields are stored as six planes (SoA) of H floats:
+// s11, s12, s13, s22, s23, s33.
+struct T6 {
+  float s11, s12, s13, s22, s23, s33;
+};
+__device__ __forceinline__ T6 load_t6(const float* __restrict__ t, long H, long o) {
+  return T6{t[o], t[H + o], t[2 * H + o], t[3 * H + o], t[4 * H + o], t[5 * H + o]};
+}
+__device__ __forceinline__ void store_t6(float* __restrict__ t, long H, long o, const T6& v) {
+  t[o] = v.s11; t[H + o] = v.s12; t[2 * H + o] = v.s13;
+  t[3 * H + o] = v.s22; t[4 * H + o] = v.s23; t[5 * H + o] = v.s33;
+}
+
+// Device tables of the rheology, one entry per depth of the half grid.
+struct Mechanism {
+  const float* gammadot0;  // h3
+  const float* power;      // h3 (stress exponent n)
+  const float* cohesion;   // h3
+  int active;
+};
+
+struct Prestress {
+  const float* sp;    // 6*h3 deviatoric prestress per depth (AoS per depth)
+  const float* taup;  // h3 norm of the deviatoric prestress
+};
+
+// Friction planes in device form (friction3d.f90:82-241); angles in radians.
+struct PlaneDev {
+  double x, y, z, L, W, beta;
+  double cstrike, sstrike, cdip, sdip, rake;
+  double Wp, Lp, xr, yr, zr;
+  double n[3], r[3];
+  double minlw;  // min(L,W)/sqrt(dx1*dx3) of the relaxation-time estimate
+};
+struct FrictionTables {
+  const float* v0;        // h3 (gammadot0 of the friction structure)
+  const float* tauc;      // h3 ((a-b) sigma, stored in stressexponent)
+  const float* friction;  // h3
+  const float* cohesion;  // h3
+  const PlaneDev* planes;
+  int np;
+  int active;
+};
+
+// Device-side scalars of one model's time line.
+struct StepScalars {
+  double t, Dt, tm;
+  double maxwell[3];
+  double power, cei;
+  int oi, it, status, pad;
+  unsigned int tmin_bits[3];  // running minima (positive floats as ordered ints)
+  unsigned int pad2;
+};

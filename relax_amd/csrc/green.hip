@@ -1,0 +1,360 @@
+// green.hip -- Fourier-domain elastic Green's function with the Cowling
+// surface correction (green.f90:51-101 elasticresponse, :327-377
+// surfacetractioncowling, :520-612 cerrutimodified, :1008-1129
+// greenfunctioncowling; the CUDA kernels cuElasticResKernel, cuSurfaceKernel,
+// scaleAndSub and cuCerrutiModKernel of cugreen.cu are replaced).
+//
+// The hot-path kernel is k_green_columns: one workgroup owns TX (k1,k2)
+// columns of all three displacement components.  Because the body force lives
+// only in the upper half of the FFT grid and only the upper half of the
+// displacement is kept (relax.f90:682, :833-835), the x3 transforms are
+// pruned: the even and odd k3 halves are two half-length FFTs of the input
+// and of the input times w^i3, and the inverse keeps only the upper half.
+// Between them the workgroup applies the Green's function, reduces the
+// surface traction over k3 in a fixed order (no atomics: the result is
+// bitwise reproducible), forms p = t - p and adds the Cerruti correction.
+#include <cmath>
+
+#include "fft_lds.h"
+#include "kernels.h"
+
+namespace {
+
+struct ColumnConsts {
+  float ratio1, ratio2;       // elasticresponse: (l+m)/(l+2m)/m/(2pi)^2, m/(l+m)
+  float mu2pi, mod2pi, lam2pi, grav;  // traction: 2 pi mu, 2 pi (l+2m), 2 pi l, 2 mu alpha gamma
+  float alpha, inv_alpha;
+  float inv_n3dx3;
+  float ik1, ik2, ik3;        // 1/(n dx) per axis
+};
+
+__host__ ColumnConsts make_consts(const GreenConst& gc) {
+  ColumnConsts c;
+  const double l = gc.lambda, m = gc.mu;
+  const double alpha = (l + m) / (l + 2.0 * m);
+  c.ratio1 = (float)((l + m) / (l + 2.0 * m) / m / (RELAX_PI2 * RELAX_PI2));
+  c.ratio2 = (float)(m / (l + m));
+  c.mu2pi = (float)(RELAX_PI2 * m);
+  c.mod2pi = (float)(RELAX_PI2 * (l + 2.0 * m));
+  c.lam2pi = (float)(RELAX_PI2 * l);
+  c.grav = (float)(2.0 * m * alpha * gc.gamma);
+  c.alpha = (float)alpha;
+  c.inv_alpha = (float)(1.0 / alpha);
+  c.inv_n3dx3 = (float)(gc.n3 * gc.dx3);  // the reference divides by REAL(sx3*dx3,4)
+  c.ik1 = (float)(1.0 / (gc.n1 * gc.dx1));
+  c.ik2 = (float)(1.0 / (gc.n2 * gc.dx2));
+  c.ik3 = (float)(1.0 / (gc.n3 * gc.dx3));
+  return c;
+}
+
+// elasticresponse at one wavenumber (green.f90:63-86)
+__device__ __forceinline__ void elastic_point(const ColumnConsts& C, float k1, float k2, float k3, float2& c1,
+                                              float2& c2, float2& c3) {
+  const float r2 = k1 * k1 + k2 * k2 + k3 * k3;
+  const float denom = C.ratio1 / (r2 * r2);
+  const float a1 = k2 * k2 + k3 * k3 + C.ratio2 * r2;
+  const float a2 = k1 * k1 + k3 * k3 + C.ratio2 * r2;
+  const float a3 = k1 * k1 + k2 * k2 + C.ratio2 * r2;
+  const float2 d1 = make_float2(k2 * c2.x + k3 * c3.x, k2 * c2.y + k3 * c3.y);
+  const float2 d2 = make_float2(k1 * c1.x + k3 * c3.x, k1 * c1.y + k3 * c3.y);
+  const float2 d3 = make_float2(k1 * c1.x + k2 * c2.x, k1 * c1.y + k2 * c2.y);
+  const float2 b1 = make_float2((a1 * c1.x - k1 * d1.x) * denom, (a1 * c1.y - k1 * d1.y) * denom);
+  const float2 b2 = make_float2((a2 * c2.x - k2 * d2.x) * denom, (a2 * c2.y - k2 * d2.y) * denom);
+  const float2 b3 = make_float2((a3 * c3.x - k3 * d3.x) * denom, (a3 * c3.y - k3 * d3.y) * denom);
+  c1 = b1; c2 = b2; c3 = b3;
+}
+
+// summand of surfacetractioncowling (green.f90:357-359)
+__device__ __forceinline__ void traction_point(const ColumnConsts& C, float k1, float k2, float k3, float2 c1,
+                                               float2 c2, float2 c3, float2& s1, float2& s2, float2& s3) {
+  // i 2 pi mu (k3 c1 + k1 c3)
+  const float2 a = make_float2(k3 * c1.x + k1 * c3.x, k3 * c1.y + k1 * c3.y);
+  s1 = make_float2(-C.mu2pi * a.y, C.mu2pi * a.x);
+  const float2 b = make_float2(k3 * c2.x + k2 * c3.x, k3 * c2.y + k2 * c3.y);
+  s2 = make_float2(-C.mu2pi * b.y, C.mu2pi * b.x);
+  // i 2 pi ((l+2m) k3 c3 + l (k1 c1 + k2 c2)) - grav c3
+  const float2 d = make_float2(C.mod2pi * k3 * c3.x + C.lam2pi * (k1 * c1.x + k2 * c2.x),
+                               C.mod2pi * k3 * c3.y + C.lam2pi * (k1 * c1.y + k2 * c2.y));
+  s3 = make_float2(-d.y - C.grav * c3.x, d.x - C.grav * c3.y);
+}
+
+// Per-column part of cerrutisolmodified (green.f90:571-598): everything that
+// does not depend on k3, in FP64 then rounded.
+struct CerrutiCol {
+  float beta, beta2, cb1;  // cb1 = 1/(2 mu beta^3)
+  float2 b3;               // k3-independent b3
+  float k1, k2;
+  int zero;                // k1 == k2 == 0: no correction
+};
+
+__device__ __forceinline__ CerrutiCol cerruti_column(const ColumnConsts& C, double mu, double alpha, double gamma,
+                                                     float k1, float k2, float2 p1, float2 p2, float2 p3) {
+  CerrutiCol cc;
+  cc.k1 = k1; cc.k2 = k2;
+  cc.zero = (k1 == 0.f && k2 == 0.f);
+  if (cc.zero) { cc.beta = cc.beta2 = cc.cb1 = 0.f; cc.b3 = make_float2(0.f, 0.f); return cc; }
+  const double dk1 = k1, dk2 = k2;
+  const double beta = RELAX_PI2 * sqrt(dk1 * dk1 + dk2 * dk2);
+  const double h = gamma / beta;
+  const double den = 1.0 / (2.0 * alpha * mu * beta * beta * beta * beta * (1.0 + h));
+  // b3 = (beta p3 - i (1-alpha) 2 pi (k1 p1 + k2 p2)) / (2 alpha mu beta^4 (1+h))
+  const double qx = RELAX_PI2 * (dk1 * p1.x + dk2 * p2.x), qy = RELAX_PI2 * (dk1 * p1.y + dk2 * p2.y);
+  const double b3x = (beta * p3.x + (1.0 - alpha) * qy) * den;
+  const double b3y = (beta * p3.y - (1.0 - alpha) * qx) * den;
+  cc.beta = (float)beta;
+  cc.beta2 = (float)(beta * beta);
+  cc.cb1 = (float)(1.0 / (2.0 * mu * beta * beta * beta));
+  cc.b3 = make_float2((float)b3x, (float)b3y);
+  return cc;
+}
+
+// The k3-dependent part of cerrutisolmodified (green.f90:596-606).
+__device__ __forceinline__ void cerruti_point(const ColumnConsts& C, const CerrutiCol& cc, float2 p1, float2 p2,
+                                              float k3, float2& u1, float2& u2, float2& u3) {
+  if (cc.zero) return;
+  const float tk3 = (float)RELAX_PI2 * k3;
+  const float fi = 2.f * cc.beta / (tk3 * tk3 + cc.beta2);
+  const float t = cc.cb1 * fi;
+  const float2 b1 = cscale(p1, t), b2 = cscale(p2, t);
+  const float2 b3 = cc.b3;
+  const float kf = tk3 * fi;  // 2 pi k3 fi
+  // tmp = alpha i beta 2pi b3 (1 - 1/alpha - i kf)
+  const float2 g = make_float2(1.f - C.inv_alpha, -kf);
+  const float2 tmp = cscale(mul_i(cmul(b3, g)), C.alpha * cc.beta * (float)RELAX_PI2);
+  const float2 v1 = cscale(tmp, cc.k1 * fi), v2 = cscale(tmp, cc.k2 * fi);
+  // v3 = -alpha beta^2 b3 (1/alpha - i kf)
+  const float2 v3 = cscale(cmul(b3, make_float2(C.inv_alpha, -kf)), -C.alpha * cc.beta2);
+  // common: alpha (2pi)^2 (b1 k1 + b2 k2) (1 - i kf)
+  const float2 bk = make_float2(b1.x * cc.k1 + b2.x * cc.k2, b1.y * cc.k1 + b2.y * cc.k2);
+  const float2 com = cscale(cmul(bk, make_float2(1.f, -kf)), C.alpha * (float)(RELAX_PI2 * RELAX_PI2));
+  const float tb2 = 2.f * cc.beta2;
+  u1 = cadd(u1, cadd(v1, make_float2(-tb2 * b1.x + cc.k1 * com.x, -tb2 * b1.y + cc.k1 * com.y)));
+  u2 = cadd(u2, cadd(v2, make_float2(-tb2 * b2.x + cc.k2 * com.x, -tb2 * b2.y + cc.k2 * com.y)));
+  // u3 = v3 fi + alpha beta 2pi (k1 b1 + k2 b2) 2pi k3 fi
+  const float s = C.alpha * cc.beta * (float)RELAX_PI2 * kf;
+  u3 = cadd(u3, make_float2(v3.x * fi + s * bk.x, v3.y * fi + s * bk.y));
+}
+
+}  // namespace
+
+// ---------------------------------------------------- fused column pass ----
+template <int H3, int TX, int NT>
+__global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu, double alpha, double gamma,
+                                                      int n1, int n2, int n3, float2* W, float2* NQ,
+                                                      const float2* __restrict__ TM,
+                                                      const float2* __restrict__ TN,
+                                                      const float2* __restrict__ tw3) {
+  static_assert(NT % TX == 0, "a thread keeps one column");
+  constexpr int NE = 3 * H3 * TX;  // input elements
+  __shared__ float2 s[2 * NE];     // [c][half][m][tx]
+  __shared__ float2 red[3][NT];
+  __shared__ float2 pv[3][TX];
+  const int tid = threadIdx.x;
+  const int tx = tid % TX;
+  const int m1 = n1 / 2;
+  const int nmain = (m1 / TX) * n2;
+  const int bid = blockIdx.x;
+  float2* base;
+  const float2* tbase;
+  long cs, zs, tcs;
+  int k1i, k2i;
+  if (bid < nmain) {
+    const int k2 = bid / (m1 / TX), k10 = (bid % (m1 / TX)) * TX;
+    base = W + (long)k2 * m1 + k10;
+    cs = (long)H3 * n2 * m1; zs = (long)n2 * m1;
+    tbase = TM + (long)k2 * m1 + k10; tcs = (long)n2 * m1;
+    k1i = k10 + tx; k2i = k2;
+  } else {
+    const int k20 = (bid - nmain) * TX;
+    base = NQ + k20;
+    cs = (long)H3 * n2; zs = n2;
+    tbase = TN + k20; tcs = n2;
+    k1i = m1; k2i = k20 + tx;
+  }
+  const float k1 = (float)k1i * C.ik1;
+  const float k2 = wavenumber23f(k2i, n2, C.ik2);
+
+  // load the upper half and its w^i3-shifted copy
+  for (int e = tid; e < NE; e += NT) {
+    const int t = e % TX, z = (e / TX) % H3, c = e / (TX * H3);
+    const float2 x = base[c * cs + z * zs + t];
+    s[((2 * c) * H3 + z) * TX + t] = x;
+    s[((2 * c + 1) * H3 + z) * TX + t] = cmul(x, tw3[z]);
+  }
+  __syncthreads();
+  lds_fft<H3, 6 * TX, NT, TX, 1, H3 * TX, TX, -1, 2, true>(s, tw3, tid);
+
+  // Green's function and the traction partial sums of this thread's column
+  float2 ps1 = make_float2(0.f, 0.f), ps2 = ps1, ps3 = ps1;
+  for (int e = tid; e < 2 * H3 * TX; e += NT) {
+    const int slot = e / TX, half = slot / H3, m = slot % H3;
+    const int k3i = 2 * m + half;
+    const float k3 = wavenumber23f(k3i, n3, C.ik3);
+    float2& c1 = s[((0 + half) * H3 + m) * TX + tx];
+    float2& c2 = s[((2 + half) * H3 + m) * TX + tx];
+    float2& c3 = s[((4 + half) * H3 + m) * TX + tx];
+    float2 a1 = c1, a2 = c2, a3 = c3;
+    if (k1i == 0 && k2i == 0 && k3i == 0) {
+      a1 = a2 = a3 = make_float2(0.f, 0.f);
+    } else {
+      elastic_point(C, k1, k2, k3, a1, a2, a3);
+    }
+    c1 = a1; c2 = a2; c3 = a3;
+    float2 s1, s2, s3;
+    traction_point(C, k1, k2, k3, a1, a2, a3, s1, s2, s3);
+    ps1 = cadd(ps1, s1); ps2 = cadd(ps2, s2); ps3 = cadd(ps3, s3);
+  }
+  red[0][tid] = ps1; red[1][tid] = ps2; red[2][tid] = ps3;
+  __syncthreads();
+  if (tid < 3 * TX) {
+    const int c = tid / TX, t = tid % TX;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int j = 0; j < NT / TX; ++j) acc = cadd(acc, red[c][t + j * TX]);
+    const float2 tt = tbase[c * tcs + t];
+    // p = t - p / (sx3 dx3)   (green.f90:373-375, :1082-1084)
+    pv[c][t] = make_float2(tt.x - acc.x / C.inv_n3dx3, tt.y - acc.y / C.inv_n3dx3);
+  }
+  __syncthreads();
+  {
+    const float2 p1 = pv[0][tx], p2 = pv[1][tx], p3 = pv[2][tx];
+    const CerrutiCol cc = cerruti_column(C, mu, alpha, gamma, k1, k2, p1, p2, p3);
+    for (int e = tid; e < 2 * H3 * TX; e += NT) {
+      const int slot = e / TX, half = slot / H3, m = slot % H3;
+      const float k3 = wavenumber23f(2 * m + half, n3, C.ik3);
+      float2& c1 = s[((0 + half) * H3 + m) * TX + tx];
+      float2& c2 = s[((2 + half) * H3 + m) * TX + tx];
+      float2& c3 = s[((4 + half) * H3 + m) * TX + tx];
+      float2 a1 = c1, a2 = c2, a3 = c3;
+      cerruti_point(C, cc, p1, p2, k3, a1, a2, a3);
+      c1 = a1; c2 = a2; c3 = a3;
+    }
+  }
+  __syncthreads();
+  lds_fft<H3, 6 * TX, NT, TX, 1, H3 * TX, TX, +1, 2, true>(s, tw3, tid);
+  // upper half of the inverse: y[n] = E'[n] + w^-n O'[n]
+  for (int e = tid; e < NE; e += NT) {
+    const int t = e % TX, z = (e / TX) % H3, c = e / (TX * H3);
+    const float2 ev = s[((2 * c) * H3 + z) * TX + t];
+    const float2 od = s[((2 * c + 1) * H3 + z) * TX + t];
+    base[c * cs + z * zs + t] = cadd(ev, cmul(od, cconj(tw3[z])));
+  }
+}
+
+template <int H3>
+struct GreenTile {
+  static constexpr int TX = H3 >= 256 ? 4 : (H3 >= 128 ? 8 : 16);
+};
+
+int launch_green_columns(const GreenConst& gc, int h3, float2* W, float2* NQ, const float2* TM, const float2* TN,
+                         hipStream_t st) {
+  const float2* tw3 = twiddle_table(gc.n3);
+  if (!tw3) return RELAX_ERR_NOMEM;
+  const ColumnConsts C = make_consts(gc);
+  const double alpha = (gc.lambda + gc.mu) / (gc.lambda + 2.0 * gc.mu);
+  const int m1 = gc.n1 / 2;
+  constexpr int NT = 256;
+#define X(HH)                                                                                          \
+  case HH: {                                                                                           \
+    constexpr int TX = GreenTile<HH>::TX;                                                              \
+    if (m1 % TX || gc.n2 % TX) return RELAX_ERR_SIZE;                                                  \
+    const int nb = (m1 / TX) * gc.n2 + gc.n2 / TX;                                                     \
+    k_green_columns<HH, TX, NT><<<nb, NT, 0, st>>>(C, gc.mu, alpha, gc.gamma, gc.n1, gc.n2, gc.n3, W, NQ, \
+                                                   TM, TN, tw3);                                       \
+    break;                                                                                             \
+  }
+  switch (h3) {
+    X(8) X(16) X(32) X(64) X(128) X(256)
+    default: return RELAX_ERR_SIZE;
+  }
+#undef X
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
+// --------------------------------------- per-routine padded-spectrum kernels --
+// Spectra are (sx1/2+1) complex per row, rows (k2) then planes (k3).
+__global__ void k_elastic_padded(ColumnConsts C, int n1, int n2, int n3, float2* f1, float2* f2, float2* f3) {
+  const int h1 = n1 / 2 + 1;
+  const long n = (long)h1 * n2 * n3;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < n; o += (long)gridDim.x * blockDim.x) {
+    const int i1 = o % h1, i2 = (o / h1) % n2, i3 = o / ((long)h1 * n2);
+    float2 a1 = f1[o], a2 = f2[o], a3 = f3[o];
+    if (i1 == 0 && i2 == 0 && i3 == 0) {
+      a1 = a2 = a3 = make_float2(0.f, 0.f);
+    } else {
+      elastic_point(C, (float)i1 * C.ik1, wavenumber23f(i2, n2, C.ik2), wavenumber23f(i3, n3, C.ik3), a1, a2, a3);
+    }
+    f1[o] = a1; f2[o] = a2; f3[o] = a3;
+  }
+}
+
+// one thread per (k1,k2) column, k3 summed in the reference's order
+__global__ void k_surface_padded(ColumnConsts C, int n1, int n2, int n3, const float2* u1, const float2* u2,
+                                 const float2* u3, const float2* t1, const float2* t2, const float2* t3,
+                                 float2* p1, float2* p2, float2* p3) {
+  const int h1 = n1 / 2 + 1;
+  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (o >= (long)h1 * n2) return;
+  const int i1 = o % h1, i2 = o / h1;
+  const float k1 = (float)i1 * C.ik1, k2 = wavenumber23f(i2, n2, C.ik2);
+  float2 a1 = make_float2(0.f, 0.f), a2 = a1, a3 = a1;
+  for (int i3 = 0; i3 < n3; ++i3) {
+    const long q = o + (long)i3 * h1 * n2;
+    float2 s1, s2, s3;
+    traction_point(C, k1, k2, wavenumber23f(i3, n3, C.ik3), u1[q], u2[q], u3[q], s1, s2, s3);
+    a1 = cadd(a1, s1); a2 = cadd(a2, s2); a3 = cadd(a3, s3);
+  }
+  a1 = make_float2(a1.x / C.inv_n3dx3, a1.y / C.inv_n3dx3);
+  a2 = make_float2(a2.x / C.inv_n3dx3, a2.y / C.inv_n3dx3);
+  a3 = make_float2(a3.x / C.inv_n3dx3, a3.y / C.inv_n3dx3);
+  if (t1) {  // greenfunctioncowling: p = t - p
+    a1 = csub(t1[o], a1); a2 = csub(t2[o], a2); a3 = csub(t3[o], a3);
+  }
+  p1[o] = a1; p2[o] = a2; p3[o] = a3;
+}
+
+__global__ void k_cerruti_padded(ColumnConsts C, double mu, double alpha, double gamma, int n1, int n2, int n3,
+                                 const float2* p1, const float2* p2, const float2* p3, float2* u1, float2* u2,
+                                 float2* u3) {
+  const int h1 = n1 / 2 + 1;
+  const long n = (long)h1 * n2 * n3;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < n; o += (long)gridDim.x * blockDim.x) {
+    const int i1 = o % h1, i2 = (o / h1) % n2, i3 = o / ((long)h1 * n2);
+    const long q = o % ((long)h1 * n2);
+    const float k1 = (float)i1 * C.ik1, k2 = wavenumber23f(i2, n2, C.ik2);
+    const CerrutiCol cc = cerruti_column(C, mu, alpha, gamma, k1, k2, p1[q], p2[q], p3[q]);
+    float2 a1 = u1[o], a2 = u2[o], a3 = u3[o];
+    cerruti_point(C, cc, p1[q], p2[q], wavenumber23f(i3, n3, C.ik3), a1, a2, a3);
+    u1[o] = a1; u2[o] = a2; u3[o] = a3;
+  }
+}
+
+static dim3 grid_for(long n) {
+  long b = (n + 255) / 256;
+  return dim3((unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b)));
+}
+
+int launch_elastic_padded(const GreenConst& gc, float2* f1, float2* f2, float2* f3, hipStream_t st) {
+  const ColumnConsts C = make_consts(gc);
+  k_elastic_padded<<<grid_for((long)(gc.n1 / 2 + 1) * gc.n2 * gc.n3), 256, 0, st>>>(C, gc.n1, gc.n2, gc.n3, f1, f2,
+                                                                                      f3);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
+int launch_surface_padded(const GreenConst& gc, const float2* u1, const float2* u2, const float2* u3,
+                          const float2* t1, const float2* t2, const float2* t3, float2* p1, float2* p2,
+                          float2* p3, hipStream_t st) {
+  const ColumnConsts C = make_consts(gc);
+  const long n = (long)(gc.n1 / 2 + 1) * gc.n2;
+  k_surface_padded<<<(unsigned)((n + 127) / 128), 128, 0, st>>>(C, gc.n1, gc.n2, gc.n3, u1, u2, u3, t1, t2, t3, p1,
+                                                                 p2, p3);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
+int launch_cerruti_padded(const GreenConst& gc, const float2* p1, const float2* p2, const float2* p3,
+                          float2* u1, float2* u2, float2* u3, hipStream_t st) {
+  const ColumnConsts C = make_consts(gc);
+  const double alpha = (gc.lambda + gc.mu) / (gc.lambda + 2.0 * gc.mu);
+  k_cerruti_padded<<<grid_for((long)(gc.n1 / 2 + 1) * gc.n2 * gc.n3), 256, 0, st>>>(
+      C, gc.mu, alpha, gc.gamma, gc.n1, gc.n2, gc.n3, p1, p2, p3, u1, u2, u3);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}

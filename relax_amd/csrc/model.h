@@ -1,0 +1,52 @@
+// model.h -- the device-resident model handle (internal).
+#pragma once
+#include <vector>
+
+#include "kernels.h"
+#include "rheology.h"
+#include "stencil.h"
+
+struct relax_model {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  relax_model_desc_t desc{};
+  std::vector<double> events;
+  Grid g{};
+  // state and work buffers (device)
+  float* u = nullptr;      // 3 planes of H: u1, u2, u3 (packed rows of sx1)
+  float* sig = nullptr;    // 6 planes of H (SoA tensor)
+  float* tau = nullptr;    // 6 planes of H
+  float* gam = nullptr;    // H
+  float* W = nullptr;      // 3H: body force -> spectrum -> predictor displacement
+  float2* NQ = nullptr;    // 3 * sx2 * sx3/2: Nyquist bins of the work spectrum
+  float* T = nullptr;      // 3 * sx1 * sx2: surface traction, then its spectrum
+  float2* TN = nullptr;    // 3 * sx2: Nyquist bins of the traction spectrum
+  StepScalars* sc = nullptr;
+  double2* partials = nullptr;
+  long nblk = 0;
+  float* tables = nullptr;  // per-depth rheology tables (17 * sx3/2 floats)
+  PlaneDev* planes_dev = nullptr;
+  relax_slippatch_t* patches_dev = nullptr;
+  std::vector<relax_plane_t> host_planes;
+  std::vector<PlaneDev> plane_dev_host;
+  std::vector<long> patch_off;
+  StepScalars* host_sc = nullptr;  // pinned
+  int host_cap = 0;
+  int mech[3] = {0, 0, 0};
+  RheoParams R{};
+  StencilConst S{};
+  GreenConst gc{};
+  // phase timers (relax_model_profile)
+  bool prof_on = false;
+  int prof_n = 0;
+  std::vector<hipEvent_t> prof_ev;
+  std::vector<int> prof_kind;
+};
+
+PlaneDev make_plane(const relax_plane_t& p, double beta, double dx1, double dx3);
+int launch_frictionadd(relax_slippatch_t* patch, long n, double dt, hipStream_t st);
+int launch_monitorfriction(const PlaneDev& P, double rake, int px2, int px3, const Grid& g, double dx1, double dx2,
+                           double dx3, int sx3_range, const float* sig, const float* v0, const float* tauc,
+                           const float* fric, const float* coh, relax_slippatch_t* patch, StepScalars* sc,
+                           hipStream_t st);
+int launch_aos_soa(const float* aos, float* soa, long H, bool to_soa, hipStream_t st);

@@ -1,0 +1,156 @@
+// rheology.h -- pointwise eigenstress rates, evaluated inside the stencil
+// kernels so the moment-density field never round-trips through HBM.
+//   viscous power law: viscouseigenstress, viscoelastic3d.f90:217-314
+//   fault creep:       frictioneigenstress, friction3d.f90:82-241 (CPU path)
+#pragma once
+#include "common.h"
+
+struct RheoParams {
+  float mu;
+  Mechanism mech[2];  // 0: linear (in%linearstruc), 1: power law (in%nonlinearstruc)
+  Prestress pre;
+  FrictionTables fr;
+  int n1, n2;
+  double dx1, dx2, dx3;
+};
+
+// tensordeviatoric, elastic3d.f90:275-285
+__device__ __forceinline__ T6 deviatoric(const T6& t) {
+  const float diag = (t.s11 + t.s22 + t.s33) / 3.0f;
+  return T6{t.s11 - diag, t.s12, t.s13, t.s22 - diag, t.s23, t.s33 - diag};
+}
+// tensornorm, elastic3d.f90:311-320: sqrt((sum of squares, off-diagonals twice)/2)
+__device__ __forceinline__ float tnorm(const T6& t) {
+  return sqrtf((t.s11 * t.s11 + 2.f * t.s12 * t.s12 + 2.f * t.s13 * t.s13 + t.s22 * t.s22 +
+                2.f * t.s23 * t.s23 + t.s33 * t.s33) * 0.5f);
+}
+__device__ __forceinline__ double tnorm_d(const T6& t) {
+  return sqrt(((double)t.s11 * t.s11 + 2.0 * t.s12 * t.s12 + 2.0 * t.s13 * t.s13 + (double)t.s22 * t.s22 +
+               2.0 * t.s23 * t.s23 + (double)t.s33 * t.s33) * 0.5);
+}
+
+// gauss, omega, my_sinh: elastic3d.f90:525-531, :549-562, :89-97
+__device__ __forceinline__ double d_gauss(double x, double sigma) {
+  const double r = x / sigma;
+  return exp(-0.5 * r * r) / sqrt(RELAX_PI2) / sigma;
+}
+__device__ __forceinline__ double d_omega(double x, double beta) {
+  const double ax = fabs(x);
+  if (ax <= (1.0 - 2.0 * beta) / (1.0 - beta) / 2.0) return 1.0;
+  if (ax < 1.0 / (1 - beta) / 2.0) {
+    const double c = cos(RELAX_PI * ((1.0 - beta) * ax - 0.5 + beta) / 2.0 / beta);
+    return c * c;
+  }
+  return 0.0;
+}
+__device__ __forceinline__ double d_my_sinh(double x) {
+  if (fabs(x) > 11.0) return copysign(sinh(11.0), x);
+  return sinh(x);
+}
+// shiftedcoordinates, elastic3d.f90:4027-4048 (0-based index)
+__device__ __forceinline__ double d_shifted(int i, int n, double dx) {
+  return (double)(i < n / 2 ? i : i - n) * dx;
+}
+
+// Viscous contribution of one mechanism; returns 1/gammadot (or +inf when
+// the point is skipped) and adds gammadot*tauc/mu to *grate.
+__device__ __forceinline__ float viscous_point(const Mechanism& M, const Prestress& P, float mu, int i3,
+                                               const T6& sd, T6& m, float* grate) {
+  const float g0 = M.gammadot0[i3];
+  if (1.0e-20 > (double)g0) return __builtin_inff();
+  const float* sp = P.sp + 6 * i3;
+  const T6 ss{sd.s11 + sp[0], sd.s12 + sp[1], sd.s13 + sp[2], sd.s22 + sp[3], sd.s23 + sp[4], sd.s33 + sp[5]};
+  const float tau = tnorm(ss);
+  const float tauc = fmaxf(0.f, tau - M.cohesion[i3]);
+  if ((double)tauc <= 1.0e-20) return __builtin_inff();
+  const float ex = M.power[i3] - 1.f;
+  const float gd = g0 * powf(tauc / mu, ex);
+  const float gdp = g0 * powf(P.taup[i3] / mu, ex);
+  const float a = 2.f * gd, b = 2.f * gdp;
+  m.s11 += a * sd.s11 - b * sp[0];
+  m.s12 += a * sd.s12 - b * sp[1];
+  m.s13 += a * sd.s13 - b * sp[2];
+  m.s22 += a * sd.s22 - b * sp[3];
+  m.s23 += a * sd.s23 - b * sp[4];
+  m.s33 += a * sd.s33 - b * sp[5];
+  if (grate) *grate = gd * tauc / mu;
+  return 1.f / gd;
+}
+
+// Fault-creep contribution of one plane at grid point (i1,i2,i3).
+// Returns the candidate relaxation time (NaN-free: a NaN candidate becomes
+// +inf, as MIN keeps the running value) and sets *status on overflow.
+__device__ __forceinline__ float friction_point(const RheoParams& R, const PlaneDev& P, int i1, int i2, int i3,
+                                                const T6& s, T6& m, float* vel, int* status) {
+  const double dx1 = R.dx1;
+  const double x3 = (double)i3 * R.dx3;
+  if ((fabs(x3 - P.z) > P.Lp) && (fabs(x3 + P.z) > P.Lp)) return __builtin_inff();
+  const double x1 = d_shifted(i1, R.n1, R.dx1), x2 = d_shifted(i2, R.n2, R.dx2);
+  const double wl = fmax(P.Wp, P.Lp);
+  if ((fabs(x1 - P.x) > wl) || (fabs(x2 - P.y) > wl)) return __builtin_inff();
+  const double x2r = P.cstrike * x1 - P.sstrike * x2;
+  const double x1s = P.cdip * x2r - P.sdip * x3;
+  const double x1i = P.cdip * x2r + P.sdip * x3;
+  if ((fabs(x1s - P.xr) > 7.01 * dx1) && (fabs(x1i - P.xr) > 7.01 * dx1)) return __builtin_inff();
+  const double x2s = P.sstrike * x1 + P.cstrike * x2;
+  const double x3s = P.sdip * x2r + P.cdip * x3;
+  const double x3i = -P.sdip * x2r + P.cdip * x3;
+  const double temp2 = d_omega((x2s - P.yr) / P.W, P.beta);
+  const double sourc = d_gauss(x1s - P.xr, dx1) * temp2 * d_omega((x3s - P.zr) / P.L, P.beta);
+  const double image = d_gauss(x1i - P.xr, dx1) * temp2 * d_omega((x3i + P.zr) / P.L, P.beta);
+  const double impulse = sourc + image;
+  const double* n = P.n;
+  const double t0 = s.s11 * n[0] + s.s12 * n[1] + s.s13 * n[2];
+  const double t1 = s.s12 * n[0] + s.s22 * n[1] + s.s23 * n[2];
+  const double t2 = s.s13 * n[0] + s.s23 * n[1] + s.s33 * n[2];
+  const double taun = t0 * n[0] + t1 * n[1] + t2 * n[2];
+  double ts0 = t0 - taun * n[0], ts1 = t1 - taun * n[1], ts2 = t2 - taun * n[2];
+  const double taus = sqrt(ts0 * ts0 + ts1 * ts1 + ts2 * ts2);
+  const double fric = R.fr.friction[i3], coh = R.fr.cohesion[i3];
+  const double tau = fmax(0.0, taus + fric * taun - coh);
+  if ((ts0 * P.r[0] + ts1 * P.r[1] + ts2 * P.r[2]) < 0.0 && fabs(P.rake) < RELAX_PI2 * 1.5)
+    return __builtin_inff();
+  const double tauc = R.fr.tauc[i3];
+  if ((tau / tauc) > 20) { *status = RELAX_ERR_FRICTION; return __builtin_inff(); }
+  ts0 /= taus; ts1 /= taus; ts2 /= taus;
+  const double gd = (double)R.fr.v0[i3] * 2.0 * d_my_sinh(tau / tauc);
+  float cand = (float)(tau / (double)R.mu / gd * P.minlw);
+  if (cand != cand) cand = __builtin_inff();
+  if (vel) *vel = (float)((double)*vel + gd * impulse * sqrt(RELAX_PI2) * dx1);
+  const double a = 2.0 * (double)R.mu * impulse * gd;
+  const double b0 = a * n[0], b1 = a * n[1], b2 = a * n[2];
+  m.s11 += (float)(ts0 * b0);
+  m.s12 += (float)((ts0 * b1 + ts1 * b0) / 2.0);
+  m.s13 += (float)((ts0 * b2 + ts2 * b0) / 2.0);
+  m.s22 += (float)(ts1 * b1);
+  m.s23 += (float)((ts1 * b2 + ts2 * b1) / 2.0);
+  m.s33 += (float)(ts2 * b2);
+  return cand;
+}
+
+// Moment-density rate of all mechanisms at one point (relax.f90:568-612).
+// tmin[3] (optional) receives the per-mechanism minima of the relaxation
+// time; grate (optional) the viscous strain rate summed over mechanisms.
+template <bool TM>
+__device__ __forceinline__ T6 rheology(const RheoParams& R, const T6& s, int i1, int i2, int i3, float* tmin,
+                                       float* grate_lin, float* grate_nl, int* status) {
+  T6 m{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (R.mech[0].active || R.mech[1].active) {
+    const T6 sd = deviatoric(s);
+    if (R.mech[0].active) {
+      const float t = viscous_point(R.mech[0], R.pre, R.mu, i3, sd, m, grate_lin);
+      if (TM) tmin[0] = fminf(tmin[0], t);
+    }
+    if (R.mech[1].active) {
+      const float t = viscous_point(R.mech[1], R.pre, R.mu, i3, sd, m, grate_nl);
+      if (TM) tmin[1] = fminf(tmin[1], t);
+    }
+  }
+  if (R.fr.active) {
+    for (int k = 0; k < R.fr.np; ++k) {
+      const float t = friction_point(R, R.fr.planes[k], i1, i2, i3, s, m, nullptr, status);
+      if (TM) tmin[2] = fminf(tmin[2], t);
+    }
+  }
+  return m;
+}

@@ -1,0 +1,223 @@
+// stencil.hip -- FIR strain and divergence stencils of the RK2 step, with the
+// rheology and the stage updates fused in.
+//
+//   k_bodyforce: moment = rheology(sig) evaluated on an LDS tile with a
+//     one-point halo, marched down x3 through a three-plane ring, then
+//     f = -div(moment) (equivalentbodyforce, elastic3d.f90:1214-1506, with
+//     the 1-tap centred differentiator fir1, kernel1.inc) and the surface
+//     traction t = moment(:,:,1).e3.  The predictor pass also reduces the
+//     relaxation time of every mechanism (viscoelastic3d.f90:303,
+//     friction3d.f90:223).
+//   k_stress: strain from the centred differences of u or v with one-sided
+//     differences at the top and bottom planes (stressupdate,
+//     elastic3d.f90:956-1180), Hooke's law, and the stage update of the RK2
+//     loop (relax.f90:679-701, :833-890) in the same pass.
+#include "kernels.h"
+#include "rheology.h"
+#include "stencil.h"
+
+template <int TXB, int TYB, bool TM, bool RAW>
+__global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, RheoParams R, StencilConst S,
+                                                        const float* __restrict__ sig, float* __restrict__ W,
+                                                        float* __restrict__ T, StepScalars* sc, int zc) {
+  constexpr int NT = TXB * TYB;
+  constexpr int HX = TXB + 2, HY = TYB + 2, HP = HX * HY;
+  __shared__ float ring[3][6][HP];
+  __shared__ float rmin[3][NT / 64];
+  const int tid = threadIdx.x;
+  const int tx = tid % TXB, ty = tid / TXB;
+  const int n1 = g.n1, n2 = g.n2, h3 = g.h3;
+  const long H = g.H;
+  const int x0 = blockIdx.x * TXB, y0 = blockIdx.y * TYB;
+  const int z0 = blockIdx.z * zc, z1 = min(z0 + zc, h3);
+  float tmin[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+  int status = 0;
+  const int e0 = (ty + 1) * HX + (tx + 1);
+  const int i1 = x0 + tx, i2 = y0 + ty;
+
+  auto output = [&](int zo) {
+    const float* cur = &ring[zo % 3][0][0];
+    const float* up = &ring[(zo + 2) % 3][0][0];
+    const float* dn = &ring[(zo + 1) % 3][0][0];
+    float v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int ci = (c == 0) ? 2 : (c == 1 ? 4 : 5);  // s13, s23, s33
+      const float* a = cur + ci * HP;
+      if (zo == 0) v[c] = (dn[ci * HP + e0] - a[e0]) * S.idx3;
+      else if (zo == h3 - 1) v[c] = (a[e0] - up[ci * HP + e0]) * S.idx3;
+      else v[c] = (dn[ci * HP + e0] - up[ci * HP + e0]) * S.k3h;
+    }
+    const float* s11 = cur, * s12 = cur + HP, * s13 = cur + 2 * HP, * s22 = cur + 3 * HP, * s23 = cur + 4 * HP;
+    const float f1 = (s11[e0 + 1] - s11[e0 - 1]) * S.k1h + (s12[e0 + HX] - s12[e0 - HX]) * S.k2h + v[0];
+    const float f2 = (s12[e0 + 1] - s12[e0 - 1]) * S.k1h + (s22[e0 + HX] - s22[e0 - HX]) * S.k2h + v[1];
+    const float f3 = (s13[e0 + 1] - s13[e0 - 1]) * S.k1h + (s23[e0 + HX] - s23[e0 - HX]) * S.k2h + v[2];
+    const long o = i1 + (long)n1 * (i2 + (long)n2 * zo);
+    W[o] = -f1;
+    W[H + o] = -f2;
+    W[2 * H + o] = -f3;
+    if (zo == 0) {
+      const long q = i1 + (long)n1 * i2, P = (long)n1 * n2;
+      T[q] = s13[e0];
+      T[P + q] = s23[e0];
+      T[2 * P + q] = cur[5 * HP + e0];
+    }
+  };
+
+  const int zs = max(z0 - 1, 0), ze = min(z1, h3 - 1);
+  for (int z = zs; z <= ze; ++z) {
+    float* slot = &ring[z % 3][0][0];
+    for (int e = tid; e < HP; e += NT) {
+      const int hx = e % HX, hy = e / HX;
+      const int j1 = (x0 + hx - 1 + n1) % n1, j2 = (y0 + hy - 1 + n2) % n2;
+      const long o = j1 + (long)n1 * (j2 + (long)n2 * z);
+      const T6 s = load_t6(sig, H, o);
+      const T6 m = RAW ? s : rheology<TM>(R, s, j1, j2, z, tmin, nullptr, nullptr, &status);
+      slot[e] = m.s11; slot[HP + e] = m.s12; slot[2 * HP + e] = m.s13;
+      slot[3 * HP + e] = m.s22; slot[4 * HP + e] = m.s23; slot[5 * HP + e] = m.s33;
+    }
+    __syncthreads();
+    const int zo = z - 1;
+    if (zo >= z0 && zo < z1) output(zo);
+    __syncthreads();
+  }
+  if (z1 == h3 && h3 - 1 >= z0) output(h3 - 1);
+
+  if (status) atomicMax(&sc->status, status);
+  if (TM) {
+    // wavefront minima, then one atomic per block (positive floats order as
+    // unsigned integers, so the result does not depend on the order)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float v = tmin[k];
+      for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+      if ((tid & 63) == 0) rmin[k][tid >> 6] = v;
+    }
+    __syncthreads();
+    if (tid < 3) {
+      float v = rmin[tid][0];
+      for (int w = 1; w < NT / 64; ++w) v = fminf(v, rmin[tid][w]);
+      if (v < __builtin_inff()) atomicMin(&sc->tmin_bits[tid], __float_as_uint(v));
+    }
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_stress(Grid g, RheoParams R, StencilConst S, const float* __restrict__ X,
+                                               float* sig, float* tau, float* gam, StepScalars* sc,
+                                               double2* partials) {
+  __shared__ double2 red[4];
+  const int tid = threadIdx.x;
+  const long o = (long)blockIdx.x * 256 + tid;
+  const int n1 = g.n1, n2 = g.n2, h3 = g.h3;
+  const long H = g.H;
+  double pw = 0.0, ce = 0.0;
+  int status = 0;
+  if (o < H) {
+    const int i1 = (int)(o % n1), i2 = (int)((o / n1) % n2), i3 = (int)(o / ((long)n1 * n2));
+    const long row = o - i1;
+    const long xp = row + (i1 + 1 == n1 ? 0 : i1 + 1), xm = row + (i1 == 0 ? n1 - 1 : i1 - 1);
+    const long plane = (long)n1 * n2;
+    const long pbase = o - (long)i2 * n1;
+    const long yp = pbase + (long)(i2 + 1 == n2 ? 0 : i2 + 1) * n1, ym = pbase + (long)(i2 == 0 ? n2 - 1 : i2 - 1) * n1;
+    long zp, zm;
+    float wz;
+    if (i3 == 0) { zp = o + plane; zm = o; wz = S.idx3; }
+    else if (i3 == h3 - 1) { zp = o; zm = o - plane; wz = S.idx3; }
+    else { zp = o + plane; zm = o - plane; wz = S.k3h; }
+    const float* X0 = X;
+    const float* X1 = X + H;
+    const float* X2 = X + 2 * H;
+    T6 e;
+    e.s11 = (X0[xp] - X0[xm]) * S.k1h;
+    e.s12 = (X1[xp] - X1[xm]) * S.k1h;
+    e.s13 = (X2[xp] - X2[xm]) * S.k1h;
+    e.s12 = e.s12 + (X0[yp] - X0[ym]) * S.k2h;
+    e.s22 = (X1[yp] - X1[ym]) * S.k2h;
+    e.s23 = (X2[yp] - X2[ym]) * S.k2h;
+    e.s13 = e.s13 + (X0[zp] - X0[zm]) * wz;
+    e.s23 = e.s23 + (X1[zp] - X1[zm]) * wz;
+    e.s33 = (X2[zp] - X2[zm]) * wz;
+    e.s12 *= 0.5f; e.s13 *= 0.5f; e.s23 *= 0.5f;
+    // isotropicstressstrain, elastic3d.f90:129-142
+    const float tr = e.s11 + e.s22 + e.s33;
+    e.s11 = e.s11 * S.mu2 + S.lam * tr;
+    e.s12 *= S.mu2; e.s13 *= S.mu2;
+    e.s22 = e.s22 * S.mu2 + S.lam * tr;
+    e.s23 *= S.mu2;
+    e.s33 = e.s33 * S.mu2 + S.lam * tr;
+    const T6 t = load_t6(tau, H, o);
+    T6 out;
+    if (MODE == STRESS_SET) {
+      out = T6{-t.s11 + e.s11, -t.s12 + e.s12, -t.s13 + e.s13, -t.s22 + e.s22, -t.s23 + e.s23, -t.s33 + e.s33};
+    } else if (MODE == STRESS_PRED) {
+      const T6 s = load_t6(sig, H, o);
+      const T6 m = rheology<false>(R, s, i1, i2, i3, nullptr, nullptr, nullptr, &status);
+      const float c = -(float)(sc->Dt / 2);
+      out = T6{(c * m.s11 - t.s11) + e.s11, (c * m.s12 - t.s12) + e.s12, (c * m.s13 - t.s13) + e.s13,
+               (c * m.s22 - t.s22) + e.s22, (c * m.s23 - t.s23) + e.s23, (c * m.s33 - t.s33) + e.s33};
+    } else {
+      const T6 s = load_t6(sig, H, o);
+      float gl = 0.f, gn = 0.f;
+      const T6 m = rheology<false>(R, s, i1, i2, i3, nullptr, &gl, &gn, &status);
+      const float dt = (float)sc->Dt;
+      const T6 tn{t.s11 + dt * m.s11, t.s12 + dt * m.s12, t.s13 + dt * m.s13,
+                  t.s22 + dt * m.s22, t.s23 + dt * m.s23, t.s33 + dt * m.s33};
+      store_t6(tau, H, o, tn);
+      if (R.mech[0].active || R.mech[1].active) {
+        float gv = gam[o];
+        if (R.mech[0].active) gv = gv + dt * gl;
+        if (R.mech[1].active) gv = gv + dt * gn;
+        gam[o] = gv;
+      }
+      out = T6{-tn.s11 + e.s11, -tn.s12 + e.s12, -tn.s13 + e.s13, -tn.s22 + e.s22, -tn.s23 + e.s23,
+               -tn.s33 + e.s33};
+      pw = tnorm_d(m);
+      ce = tnorm_d(tn);
+    }
+    store_t6(sig, H, o, out);
+  }
+  if (MODE == STRESS_CORR) {
+    for (int off = 32; off > 0; off >>= 1) {
+      pw += __shfl_xor(pw, off);
+      ce += __shfl_xor(ce, off);
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = make_double2(pw, ce);
+    __syncthreads();
+    if (tid == 0) {
+      double2 a = red[0];
+      for (int w = 1; w < 4; ++w) { a.x += red[w].x; a.y += red[w].y; }
+      partials[blockIdx.x] = a;
+    }
+  }
+  if (status) atomicMax(&sc->status, status);
+}
+
+int launch_bodyforce(const Grid& g, const RheoParams& R, const StencilConst& S, const float* sig, float* W,
+                     float* T, StepScalars* sc, int mode, hipStream_t st) {
+  const int zc = 8;
+  const unsigned gz = (unsigned)((g.h3 + zc - 1) / zc);
+#define L(TX, TY)                                                                                        \
+  {                                                                                                      \
+    const dim3 grid(g.n1 / TX, g.n2 / TY, gz);                                                           \
+    if (mode == BODY_PRED) k_bodyforce<TX, TY, true, false><<<grid, 256, 0, st>>>(g, R, S, sig, W, T, sc, zc);  \
+    else if (mode == BODY_CORR) k_bodyforce<TX, TY, false, false><<<grid, 256, 0, st>>>(g, R, S, sig, W, T, sc, zc); \
+    else k_bodyforce<TX, TY, false, true><<<grid, 256, 0, st>>>(g, R, S, sig, W, T, sc, zc);            \
+  }
+  if (g.n1 % 64 == 0 && g.n2 % 4 == 0) L(64, 4)
+  else if (g.n1 % 32 == 0 && g.n2 % 8 == 0) L(32, 8)
+  else return RELAX_ERR_SIZE;
+#undef L
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
+long stress_blocks(const Grid& g) { return (g.H + 255) / 256; }
+
+int launch_stress(int mode, const Grid& g, const RheoParams& R, const StencilConst& S, const float* X, float* sig,
+                  float* tau, float* gam, StepScalars* sc, double2* partials, hipStream_t st) {
+  const unsigned nb = (unsigned)stress_blocks(g);
+  if (mode == STRESS_SET) k_stress<STRESS_SET><<<nb, 256, 0, st>>>(g, R, S, X, sig, tau, gam, sc, partials);
+  else if (mode == STRESS_PRED) k_stress<STRESS_PRED><<<nb, 256, 0, st>>>(g, R, S, X, sig, tau, gam, sc, partials);
+  else k_stress<STRESS_CORR><<<nb, 256, 0, st>>>(g, R, S, X, sig, tau, gam, sc, partials);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}

@@ -1,0 +1,141 @@
+"""Device-resident RK2 loop of RELAX (relax.f90:555-890) through the C ABI."""
+import ctypes
+
+import numpy as np
+
+from ._lib import LAYER, PATCH, PLANE, TENSOR_LAYER, ModelDesc, StepInfo, check, load, ptr
+
+LINEAR, POWERLAW, FAULTCREEP = 0, 1, 2
+
+
+def footprint(sx1, sx2, sx3):
+    d = ModelDesc(sx1, sx2, sx3, 1.0, 1.0, 1.0, 1.0, 1.0, 0.0, 0.2, 1.0, 1.0, 1.0, 1, 0, None)
+    return int(load().relax_model_footprint(ctypes.byref(d)))
+
+
+class Model:
+    """One RELAX model held in HBM.
+
+    State is exchanged in the reference's layout: u1..u3 and gamma as
+    (sx3/2, sx2, sx1+2) float32 (align_data=1) or (sx3/2, sx2, sx1), tau and
+    sig as (sx3/2, sx2, sx1, 6) float32.
+    """
+
+    def __init__(self, sx1, sx2, sx3, dx1, dx2, dx3, lam, mu, gamma=0.0, beta=0.2, interval=1e30, odt=1.0,
+                 tscale=1.0, skip=1, event_times=(0.0,)):
+        self._lib = load()
+        self.shape = (sx1, sx2, sx3)
+        self.dx = (dx1, dx2, dx3)
+        self._events = np.ascontiguousarray(event_times, np.float64)
+        self._desc = ModelDesc(sx1, sx2, sx3, dx1, dx2, dx3, lam, mu, gamma, beta, interval, odt, tscale, skip,
+                               len(self._events), self._events.ctypes.data)
+        h = ctypes.c_void_p()
+        check(self._lib.relax_model_create(ctypes.byref(self._desc), ctypes.byref(h)), "relax_model_create")
+        self._h = h
+        self.planes = None
+
+    def close(self):
+        if self._h:
+            self._lib.relax_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_structure(self, which, layers):
+        if layers is None:
+            check(self._lib.relax_model_set_structure(self._h, which, None, 0), "set_structure")
+            return
+        layers = np.ascontiguousarray(layers, LAYER)
+        check(self._lib.relax_model_set_structure(self._h, which, ptr(layers), len(layers)), "set_structure")
+
+    def set_prestress(self, prestress):
+        prestress = np.ascontiguousarray(prestress, TENSOR_LAYER)
+        check(self._lib.relax_model_set_prestress(self._h, ptr(prestress), len(prestress)), "set_prestress")
+
+    def set_planes(self, planes, patches=None):
+        planes = np.ascontiguousarray(planes, PLANE)
+        self.planes = planes
+        pa = None if patches is None else np.ascontiguousarray(patches, PATCH)
+        check(self._lib.relax_model_set_planes(self._h, ptr(planes), len(planes), ptr(pa)), "set_planes")
+
+    def get_patches(self):
+        n = int(sum(int(p["px2"]) * int(p["px3"]) for p in self.planes)) if self.planes is not None else 0
+        out = np.zeros(n, PATCH)
+        if n:
+            check(self._lib.relax_model_get_patches(self._h, ptr(out)), "get_patches")
+        return out
+
+    def upload(self, u1, u2, u3, tau, sig=None, gamma=None, t=0.0, oi=1, i=0, align_data=1):
+        arrs = [np.ascontiguousarray(a, np.float32) if a is not None else None for a in (u1, u2, u3, tau, sig, gamma)]
+        check(self._lib.relax_model_upload(self._h, *[ptr(a) for a in arrs], t, oi, i, align_data), "upload")
+
+    def download(self, align_data=1):
+        sx1, sx2, sx3 = self.shape
+        h3 = sx3 // 2
+        p = sx1 + 2 if align_data else sx1
+        u = [np.zeros((h3, sx2, p), np.float32) for _ in range(3)]
+        tau = np.zeros((h3, sx2, sx1, 6), np.float32)
+        sig = np.zeros_like(tau)
+        gam = np.zeros((h3, sx2, p), np.float32)
+        t = ctypes.c_double()
+        check(self._lib.relax_model_download(self._h, ptr(u[0]), ptr(u[1]), ptr(u[2]), ptr(tau), ptr(sig), ptr(gam),
+                                             ctypes.byref(t), align_data), "download")
+        return dict(u1=u[0], u2=u[1], u3=u[2], tau=tau, sig=sig, gamma=gam, t=t.value)
+
+    def step(self, dt=None, info=True):
+        """One predictor/corrector step; returns the step record as a dict."""
+        rec = StepInfo() if info else None
+        d = None if dt is None else ctypes.byref(ctypes.c_double(dt))
+        check(self._lib.relax_model_step(self._h, d, None if rec is None else ctypes.byref(rec)), "step")
+        return None if rec is None else _info(rec)
+
+    def run(self, nsteps, dt_replay=None):
+        recs = (StepInfo * max(nsteps, 1))()
+        done = ctypes.c_int()
+        dts = None if dt_replay is None else np.ascontiguousarray(dt_replay, np.float64)
+        check(self._lib.relax_model_run(self._h, nsteps, ptr(dts), recs, ctypes.byref(done)), "run")
+        return [_info(recs[k]) for k in range(done.value)]
+
+    def add_shear(self, patches, slip):
+        """momentdensityshear of each patch (PLANE array, internal convention) into tau."""
+        patches = np.ascontiguousarray(patches, PLANE)
+        slip = np.ascontiguousarray(slip, np.float64)
+        check(self._lib.relax_model_add_shear(self._h, ptr(patches), ptr(slip), len(patches)), "add_shear")
+
+    def coseismic(self):
+        """u = Green(body force of tau), sig = -tau + C:eps(u)."""
+        check(self._lib.relax_model_coseismic(self._h), "coseismic")
+
+    def profile(self, nsteps):
+        """Run nsteps steps with HIP-event phase timers; returns
+        {kind: (total_ms, launches)} for the kernel kinds of PROFILE_KINDS."""
+        ms = np.zeros(len(PROFILE_KINDS), np.float64)
+        n = np.zeros(len(PROFILE_KINDS), np.int32)
+        check(self._lib.relax_model_profile(self._h, nsteps, ptr(ms), ptr(n)), "profile")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(PROFILE_KINDS)}
+
+    def synchronize(self):
+        check(self._lib.relax_model_synchronize(self._h), "synchronize")
+
+
+# kernel kinds of relax_model_profile, in the order of RELAX_PROF_* (relax_hip.h)
+PROFILE_KINDS = ("bodyforce", "fft_x_fwd", "fft_y", "green_z", "fft_x_inv", "stress_pred", "stress_corr", "small")
+
+
+def _info(r):
+    return dict(i=r.i, Dt=r.Dt, tm=r.tm, maxwell=list(r.maxwell), power=r.power, cei=r.cei, t=r.t, oi=r.oi,
+                status=r.status)

@@ -1,0 +1,100 @@
+"""Input-side helpers that turn a RELAX description into the per-depth tables
+and internal-convention fault geometry the hot path consumes."""
+import math
+
+import numpy as np
+
+from ._lib import LAYER, PLANE
+
+DEG2RAD = 0.01745329251994329547437168059786927
+
+
+def fix(number):
+    """fix, elastic3d.f90:70-83: rounds half down."""
+    f, c = math.floor(number), math.ceil(number)
+    return c if (number - f) > 0.5 else f
+
+
+def viscoelasticstructure(layers, sx3, dx3):
+    """viscoelasticstructure, elastic3d.f90:732-829.
+
+    layers: sequence of (z, gammadot0, stressexponent, cohesion, friction, Gk)
+    interfaces; returns a LAYER array of sx3 entries sampled every dx3 with
+    linear interpolation between interfaces projected onto multiples of dx3
+    and the last layer continued downward (in single precision, as the
+    reference's REAL() there)."""
+    lay = np.asarray([tuple(l) for l in layers], LAYER)
+    nv = len(lay)
+    if nv <= 0:
+        raise ValueError("invalid elastic structure")
+    vs = np.zeros(sx3, LAYER)
+    vs["friction"] = 0.6
+    vs["stressexponent"] = lay[0]["stressexponent"]
+    z0 = fix(lay[0]["z"] / dx3) * dx3
+    i3e = 1
+    keys = ("gammadot0", "stressexponent", "friction", "cohesion", "Gk")
+    v0 = dict.fromkeys(keys, 0.0)
+    v1 = dict.fromkeys(keys, 0.0)
+    z1 = z0
+    for k in range(nv):
+        z1 = fix(lay[k]["z"] / dx3) * dx3
+        if z1 < z0:
+            raise ValueError("invalid mechanical structure")
+        for key in keys:
+            v1[key] = float(lay[k][key])
+        if z1 == z0:
+            i3e = fix(z1 / dx3 + 1)
+        else:
+            i3s = fix(z0 / dx3) + 1
+            i3e = min(fix(z1 / dx3 + 1), sx3)
+            for i3 in range(i3s, i3e + 1):
+                z = (i3 - 1.0) * dx3
+                for key in keys:
+                    vs[i3 - 1][key] = ((z - z0) * v1[key] + (z1 - z) * v0[key]) / (z1 - z0)
+        z0 = z1
+        v0 = dict(v1)
+    if fix(z1 / dx3) < sx3 - 1:
+        for key in keys:
+            vs[key][i3e - 1:] = float(np.float32(v1[key]))
+    return vs
+
+
+def wangconvention(slip, x, y, z, length, width, strike, dip, rake, x0=0.0, y0=0.0, rot=0.0):
+    """wangconvention, input.f90:1780-1800: deck geometry (upper corner,
+    angles in degrees from north / horizontal) -> internal convention (patch
+    centre, radians, dip measured from the vertical)."""
+    slip = -slip
+    strike = -90.0 - strike
+    dip = 90.0 - dip
+    strike, dip, rake = strike * DEG2RAD, dip * DEG2RAD, rake * DEG2RAD
+    x = x - x0 - length / 2.0 * math.sin(strike) + width / 2.0 * math.sin(dip) * math.cos(strike)
+    y = y - y0 - length / 2.0 * math.cos(strike) - width / 2.0 * math.sin(dip) * math.sin(strike)
+    z = z + width / 2.0 * math.cos(dip)
+    if rot != 0.0:  # rotation, input.f90:1809-1822
+        a = rot * DEG2RAD
+        x, y = x * math.cos(a) + y * math.sin(a), -x * math.sin(a) + y * math.cos(a)
+    strike = strike + rot * DEG2RAD
+    return slip, x, y, z, strike, dip, rake
+
+
+def patches(deck):
+    """Convert deck fault lines (slip, x, y, z, length, width, strike, dip,
+    rake) to a PLANE array in the internal convention and the slips."""
+    out = np.zeros(len(deck), PLANE)
+    slips = np.zeros(len(deck))
+    for k, (s, x, y, z, L, W, st, dp, rk) in enumerate(deck):
+        s2, x2, y2, z2, st2, dp2, rk2 = wangconvention(s, x, y, z, L, W, st, dp, rk)
+        out[k] = (x2, y2, z2, W, L, st2, dp2, rk2, 0, 0)
+        slips[k] = s2
+    return out, slips
+
+
+def friction_plane(x, y, z, length, width, strike, dip, rake, dx2, dx3):
+    """A fault-creep plane as the parser builds it (input.f90:1079-1093): the
+    rake is turned by 180 degrees, the geometry follows Wang's convention and
+    the patch grid is px2 = fix(length/dx2) by px3 = fix(width/dx3)."""
+    rake = rake - 180.0 if rake >= 0.0 else rake + 180.0
+    _, x2, y2, z2, st2, dp2, rk2 = wangconvention(0.0, x, y, z, length, width, strike, dip, rake)
+    p = np.zeros(1, PLANE)
+    p[0] = (x2, y2, z2, width, length, st2, dp2, rk2, fix(length / dx2), fix(width / dx3))
+    return p

@@ -1,0 +1,133 @@
+"""Seeded synthetic workloads of the RK2 step (units: km, MPa, yr; slip in km).
+
+  c0   128x128x64, dx 0.5: one vertical strike-slip fault (20 x 10 km, 1 m),
+       linear Maxwell below 20 km (eta 1e19 Pa s); 8 steps to 10 yr
+  c0b  the same with sx3 = 128 so that the Maxwell layer is sampled
+  c1   512x512x256, dx 0.5: 40 patches (5 x 5 km, slip 0-3 m, random
+       orientation), power law n = 3 below a 25 km lid with the reference
+       strain rate growing exponentially with depth (e-fold 10 km); 20 steps
+  c2   256x256x128, dx 1: rate-strengthening afterslip plane (64 x 32 km,
+       64 x 32 patches, dip 20, rake 90) loaded by a 32 x 16 km coseismic
+       thrust up dip of it, power law n = 3.5 below 30 km (eta0 1e18); 30 steps
+  c3   one of 64 batched models on 128^3, dx 1: 1-10 patches, log-uniform
+       eta in [1e17, 1e20], n in {1, 2, 3, 3.5}, 15-40 km lid; 10 steps
+
+Viscosity maps to the reference strain rate gammadot0 = mu/eta (per year);
+for n > 1 eta is the effective viscosity at tau_ref = 1 MPa, so
+gammadot0 = (mu/eta) (mu/tau_ref)^(n-1).
+"""
+import math
+
+import numpy as np
+
+from . import structure
+from ._lib import LAYER, TENSOR_LAYER
+from .model import FAULTCREEP, LINEAR, POWERLAW, Model
+
+MU = LAM = 3.0e4          # MPa
+GAM = 8.33e-4             # Cowling coefficient, 1/km
+BETA = 0.2                # taper width
+SEC_PER_YR = 3.15576e7
+SEED0 = 20261202
+
+
+def gdot0(eta, n=1.0, mu_pa=3.0e10, tau_ref=1.0):
+    """Reference strain rate (1/yr) of viscosity eta (Pa s)."""
+    return mu_pa / eta * SEC_PER_YR * (MU / tau_ref) ** (n - 1.0)
+
+
+def _deck_patch(rng, lo=-40.0, hi=40.0, zmax=15.0, size=5.0, smax=3e-3):
+    strike = rng.uniform(0, 360)
+    dip = rng.uniform(30, 90)
+    rake = (0.0 if rng.uniform() < 0.5 else 90.0) + rng.uniform(-30, 30)
+    return (rng.uniform(0, smax), rng.uniform(lo, hi), rng.uniform(lo, hi), rng.uniform(0, zmax), size, size,
+            strike, dip, rake)
+
+
+def spec(name):
+    """Description of one workload (a dict the builders below consume)."""
+    if name in ("c0", "c0b"):
+        sx3 = 64 if name == "c0" else 128
+        return dict(name=name, sx1=128, sx2=128, sx3=sx3, dx=0.5, linear=[(20.0, gdot0(1e19), 1.0, 0, 0.6, 0)],
+                    nonlinear=None, friction=None, planes=None,
+                    deck=[(1e-3, -10.0, 0.0, 0.0, 20.0, 10.0, 0.0, 90.0, 0.0)],
+                    interval=10.0, odt=1.25, skip=0, tscale=1.0, nsteps=8)
+    if name == "c1":
+        rng = np.random.default_rng(SEED0 + 1)
+        deck = [_deck_patch(rng) for _ in range(40)]
+        g25 = gdot0(1e19, 3.0)
+        nonlin = [(25.0 + 0.5 * k, g25 * math.exp((0.5 * k) / 10.0), 3.0, 0.0, 0.6, 0.0) for k in range(78)]
+        return dict(name=name, sx1=512, sx2=512, sx3=256, dx=0.5, linear=None, nonlinear=nonlin, friction=None,
+                    planes=None, deck=deck, interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=20)
+    if name == "c2":
+        rng = np.random.default_rng(SEED0 + 2)
+        dip = 20.0
+        # coseismic thrust: 32 x 16 patches of 1 km from the surface down dip
+        deck = []
+        for j3 in range(16):
+            for j2 in range(32):
+                z = j3 * math.sin(math.radians(dip))
+                y = -(j3 + 0.0) * math.cos(math.radians(dip))
+                deck.append((rng.uniform(0, 3e-3), -16.0 + j2, y, z, 1.0, 1.0, 0.0, dip, 90.0))
+        ztop = 16 * math.sin(math.radians(dip))
+        ytop = -16 * math.cos(math.radians(dip))
+        plane = dict(x=-32.0, y=ytop, z=ztop, length=64.0, width=32.0, strike=0.0, dip=dip, rake=90.0)
+        nonlin = [(30.0, gdot0(1e18, 3.5), 3.5, 0.0, 0.6, 0.0)]
+        return dict(name=name, sx1=256, sx2=256, sx3=128, dx=1.0, linear=None, nonlinear=nonlin,
+                    friction=[(0.0, 1e-3, 1.0, 0.0, 0.6, 0.0)], planes=[plane], deck=deck,
+                    interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=30)
+    if name.startswith("c3"):
+        idx = int(name[3:]) if len(name) > 3 else 0
+        rng = np.random.default_rng(SEED0 + 3 + 1000 * idx)
+        n = [1.0, 2.0, 3.0, 3.5][rng.integers(0, 4)]
+        eta = 10 ** rng.uniform(17, 20)
+        lid = rng.uniform(15, 40)
+        npatch = int(rng.integers(1, 11))
+        deck = []
+        for _ in range(npatch):
+            p = list(_deck_patch(rng, -30, 30, 10.0, 5.0, 5e-3))
+            p[0] = rng.uniform(0.5e-3, 5e-3)
+            deck.append(tuple(p))
+        lay = [(lid, gdot0(eta, n), n, 0.0, 0.6, 0.0)]
+        return dict(name=name, sx1=128, sx2=128, sx3=128, dx=1.0, linear=lay if n == 1.0 else None,
+                    nonlinear=None if n == 1.0 else lay, friction=None, planes=None, deck=deck,
+                    interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=10)
+    raise KeyError(name)
+
+
+def tables(s):
+    """Per-depth tables (sx3/2 entries) of the mechanisms of a spec."""
+    h3 = s["sx3"] // 2
+    out = {}
+    for key in ("linear", "nonlinear", "friction"):
+        out[key] = None if s[key] is None else structure.viscoelasticstructure(s[key], h3, s["dx"])
+    out["prestress"] = np.zeros(h3, TENSOR_LAYER)
+    return out
+
+
+def planes(s):
+    """Fault-creep planes of a spec (PLANE array) or None."""
+    if not s["planes"]:
+        return None
+    ps = [structure.friction_plane(p["x"], p["y"], p["z"], p["length"], p["width"], p["strike"], p["dip"],
+                                   p["rake"], s["dx"], s["dx"]) for p in s["planes"]]
+    return np.concatenate(ps)
+
+
+def build(s):
+    """Create the model of a spec on the GPU with its coseismic state."""
+    dx = s["dx"]
+    m = Model(s["sx1"], s["sx2"], s["sx3"], dx, dx, dx, LAM, MU, GAM, BETA, interval=s["interval"], odt=s["odt"],
+              tscale=s["tscale"], skip=s["skip"])
+    t = tables(s)
+    m.set_prestress(t["prestress"])
+    m.set_structure(LINEAR, t["linear"])
+    m.set_structure(POWERLAW, t["nonlinear"])
+    pl = planes(s)
+    if pl is not None:
+        m.set_planes(pl)
+    m.set_structure(FAULTCREEP, t["friction"])
+    patches, slips = structure.patches(s["deck"])
+    m.add_shear(patches, slips)
+    m.coseismic()
+    return m

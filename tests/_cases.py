@@ -1,0 +1,73 @@
+"""Shared helpers of the GPU parity tests (not collected by pytest)."""
+import numpy as np
+
+import oracle as O
+from relax_amd import structure, workloads
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def small_spec(kind, sx=64, sx3=64, dx=1.0, seed=0):
+    """Oracle-sized variants of the workloads: same physics, smaller grid."""
+    rng = np.random.default_rng(1000 + seed)
+    if kind == "maxwell":
+        s = workloads.spec("c0b")
+        s.update(sx1=sx, sx2=sx, sx3=sx3, dx=dx, linear=[(6.0, 0.2, 1.0, 0.0, 0.6, 0.0)], interval=1e9, skip=1,
+                 deck=[(1e-3, -5.0, 0.0, 0.0, 10.0, 6.0, 0.0, 90.0, 0.0)])
+        return s
+    if kind == "powerlaw":
+        deck = [workloads._deck_patch(rng, -10, 10, 5.0, 4.0, 3e-3) for _ in range(4)]
+        lay = [(8.0 + k, 5e6 * np.exp(k / 10.0), 3.0, 0.0, 0.6, 0.0) for k in range(20)]
+        return dict(name="powerlaw", sx1=sx, sx2=sx, sx3=sx3, dx=dx, linear=None, nonlinear=lay, friction=None,
+                    planes=None, deck=deck, interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=4)
+    if kind == "afterslip":
+        dip = 20.0
+        deck = [(rng.uniform(0, 3e-3), -6.0 + j2, -j3 * np.cos(np.radians(dip)), j3 * np.sin(np.radians(dip)),
+                 1.0, 1.0, 0.0, dip, 90.0) for j3 in range(6) for j2 in range(12)]
+        plane = dict(x=-12.0, y=-6 * np.cos(np.radians(dip)), z=6 * np.sin(np.radians(dip)), length=24.0,
+                     width=12.0, strike=0.0, dip=dip, rake=90.0)
+        return dict(name="afterslip", sx1=sx, sx2=sx, sx3=sx3, dx=dx, linear=None,
+                    nonlinear=[(12.0, 4e7, 3.5, 0.0, 0.6, 0.0)], friction=[(0.0, 1e-3, 1.0, 0.0, 0.6, 0.0)],
+                    planes=[plane], deck=deck, interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=4)
+    raise KeyError(kind)
+
+
+def oracle_model(s):
+    t = workloads.tables(s)
+    pl = workloads.planes(s)
+    m = dict(sx1=s["sx1"], sx2=s["sx2"], sx3=s["sx3"], dx1=s["dx"], dx2=s["dx"], dx3=s["dx"], lam=workloads.LAM,
+             mu=workloads.MU, gam=workloads.GAM, beta=workloads.BETA, linear=t["linear"], nonlinear=t["nonlinear"],
+             faultcreep=t["friction"], prestress=t["prestress"], interval=s["interval"], odt=s["odt"],
+             tscale=s["tscale"], skip=s["skip"], event_times=[0.0])
+    if pl is not None:
+        m["planes"] = np.ascontiguousarray(pl.astype(O.PLANE))
+        m["patches"] = np.zeros(int(sum(int(p["px2"]) * int(p["px3"]) for p in pl)), O.PATCH)
+    return m
+
+
+def oracle_initial_state(s, dtype=np.float32):
+    """tau from momentdensityshear of the deck patches, u from the Green's
+    function of its body force, sig = -tau + C:eps(u) -- the same set-up the
+    model's add_shear/coseismic perform on the device."""
+    sx1, sx2, sx3, dx = s["sx1"], s["sx2"], s["sx3"], s["dx"]
+    h3 = sx3 // 2
+    tau = np.zeros((h3, sx2, sx1, 6), dtype)
+    pa, slips = structure.patches(s["deck"])
+    for p, sl in zip(pa, slips):
+        O.momentdensityshear(workloads.MU, sl, p["x"], p["y"], p["z"], p["width"], p["length"], p["strike"],
+                             p["dip"], p["rake"], workloads.BETA, dx, dx, dx, tau)
+    c = [np.zeros((sx3, sx2, sx1 + 2), dtype) for _ in range(3)]
+    t = [np.zeros((sx2, sx1 + 2), dtype) for _ in range(3)]
+    O.equivalentbodyforce(tau, dx, dx, dx, c[0][:h3], c[1][:h3], c[2][:h3], t[0], t[1], t[2])
+    O.greenfunctioncowling(c[0], c[1], c[2], t[0], t[1], t[2], dx, dx, dx, workloads.LAM, workloads.MU,
+                           workloads.GAM)
+    u = [np.ascontiguousarray(a[:h3]) for a in c]
+    sig = -tau
+    sig = np.ascontiguousarray(sig)
+    O.stressupdate(u[0], u[1], u[2], workloads.LAM, workloads.MU, dx, dx, dx, sig)
+    return dict(u1=u[0], u2=u[1], u3=u[2], tau=tau, sig=sig, gamma=np.zeros_like(u[0]), t=0.0, oi=1, i=0)

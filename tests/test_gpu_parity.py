@@ -1,0 +1,382 @@
+"""Parity of the HIP library with the oracle, through the C ABI (GPU only).
+
+Tolerances (relative L2, FP32): transforms and pointwise spectral routines
+1e-5; stencils and rheology 1e-5; composed RK2 trajectories 1e-4 (the
+tolerance stated for the fields after N steps); integer and index work
+(patch sampling indices, counters) exact.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from _cases import oracle_initial_state, oracle_model, rel_l2, small_spec
+from relax_amd import (LAYER, PATCH, TENSOR_LAYER, RelaxError, elastic3d, fourier, friction3d, green,
+                       viscoelastic3d, workloads)
+from relax_amd.model import Model
+
+pytestmark = pytest.mark.gpu
+LAM = MU = 3.0e4
+
+
+def rnd(rng, *shape):
+    return rng.standard_normal(shape).astype(np.float32)
+
+
+# ------------------------------------------------------------- fourier ----
+@pytest.mark.parametrize("shape", [(16, 32, 64), (64, 64, 32), (32, 128, 16), (128, 16, 256)])
+@pytest.mark.parametrize("align", [1, 0])
+def test_fft3_roundtrip_and_oracle(shape, align):
+    rng = np.random.default_rng(11)
+    sx3, sx2, sx1 = shape
+    dx = (0.5, 0.75, 1.25)
+    x = rnd(rng, sx3, sx2, sx1)
+    ref = np.zeros((sx3, sx2, sx1 + 2), np.float32)
+    ref[..., :sx1] = x
+    buf = ref.copy() if align else np.concatenate([x.ravel(), np.zeros(2 * sx2 * sx3, np.float32)]).reshape(ref.shape)
+    fourier.fft3(buf, sx1, sx2, sx3, *dx, -1, align_data=align)
+    O.fft3(ref, sx1, sx2, sx3, *dx, -1)
+    assert rel_l2(buf, ref) < 2e-6
+    fourier.fft3(buf, sx1, sx2, sx3, *dx, 1, align_data=align)
+    back = buf[..., :sx1] if align else buf.ravel()[:x.size].reshape(x.shape)
+    assert rel_l2(back, x) < 2e-6
+
+
+@pytest.mark.parametrize("shape", [(16, 32), (256, 64), (512, 512)])
+def test_fft2_matches_oracle(shape):
+    rng = np.random.default_rng(12)
+    sx2, sx1 = shape
+    a = np.zeros((sx2, sx1 + 2), np.float32)
+    a[:, :sx1] = rnd(rng, sx2, sx1)
+    b = a.copy()
+    fourier.fft2(a, sx1, sx2, 0.5, 0.5, -1)
+    O.fft2(b, sx1, sx2, 0.5, 0.5, -1)
+    assert rel_l2(a, b) < 2e-6
+
+
+# --------------------------------------------------------------- green ----
+def spectra(rng, sx1, sx2, sx3):
+    return [rnd(rng, sx3, sx2, sx1 + 2) for _ in range(3)]
+
+
+@pytest.mark.parametrize("shape", [(32, 16, 64), (64, 64, 64)])
+def test_elasticresponse(shape):
+    rng = np.random.default_rng(13)
+    sx1, sx2, sx3 = shape
+    a = spectra(rng, sx1, sx2, sx3)
+    b = [x.copy() for x in a]
+    green.elasticresponse(LAM, MU, *a, 0.5, 0.5, 0.5)
+    O.elasticresponse(LAM, MU, *b, 0.5, 0.5, 0.5)
+    for x, y in zip(a, b):
+        assert rel_l2(x, y) < 1e-5
+
+
+@pytest.mark.parametrize("gam", [0.0, 8.33e-4, 0.05])
+def test_surfacetraction_and_cerruti(gam):
+    rng = np.random.default_rng(14)
+    sx1, sx2, sx3 = 32, 64, 64
+    u = spectra(rng, sx1, sx2, sx3)
+    p = green.surfacetractioncowling(LAM, MU, gam, *u, 0.5, 0.5, 0.5)
+    q = O.surfacetractioncowling(LAM, MU, gam, *u, 0.5, 0.5, 0.5)
+    for x, y in zip(p, q):
+        assert rel_l2(x, y) < 1e-5
+    v = [x.copy() for x in u]
+    green.cerrutimodified(*q, LAM, MU, gam, *u, 0.5, 0.5, 0.5)
+    O.cerrutimodified(*q, LAM, MU, gam, *v, 0.5, 0.5, 0.5)
+    for x, y in zip(u, v):
+        assert rel_l2(x, y) < 1e-5
+
+
+def body_force_case(sx1, sx2, sx3, dx, seed=15):
+    s = small_spec("maxwell", sx1, sx3, dx, seed)
+    s["sx2"] = sx2
+    st = oracle_initial_state(dict(s, sx1=sx1, sx2=sx2, sx3=sx3))
+    h3 = sx3 // 2
+    c = [np.zeros((sx3, sx2, sx1 + 2), np.float32) for _ in range(3)]
+    t = [np.zeros((sx2, sx1 + 2), np.float32) for _ in range(3)]
+    O.equivalentbodyforce(st["tau"], dx, dx, dx, c[0][:h3], c[1][:h3], c[2][:h3], t[0], t[1], t[2])
+    return c, t
+
+
+@pytest.mark.parametrize("shape,align", [((64, 64, 64), 1), ((64, 32, 128), 1), ((32, 64, 64), 0)])
+def test_greenfunctioncowling(shape, align):
+    sx1, sx2, sx3 = shape
+    dx = 0.5
+    c, t = body_force_case(sx1, sx2, sx3, dx)
+    rc = [x.copy() for x in c]
+    rt = [x.copy() for x in t]
+    O.greenfunctioncowling(*rc, *rt, dx, dx, dx, LAM, MU, 8.33e-4)
+    if align:
+        gc, gt = [x.copy() for x in c], [x.copy() for x in t]
+    else:  # packed rows inside the same buffers (ctfft build)
+        gc = [np.concatenate([x[..., :sx1].ravel(), np.zeros(2 * sx2 * sx3, np.float32)]).reshape(x.shape) for x in c]
+        gt = [np.concatenate([x[:, :sx1].ravel(), np.zeros(2 * sx2, np.float32)]).reshape(x.shape) for x in t]
+    green.greenfunctioncowling(*gc, *gt, dx, dx, dx, LAM, MU, 8.33e-4, align_data=align)
+    for k in range(3):
+        out = gc[k][..., :sx1] if align else gc[k].ravel()[:sx1 * sx2 * sx3].reshape(sx3, sx2, sx1)
+        assert rel_l2(out, rc[k][..., :sx1]) < 1e-5
+        tt = gt[k][:, :sx1] if align else gt[k].ravel()[:sx1 * sx2].reshape(sx2, sx1)
+        assert rel_l2(tt, rt[k][:, :sx1]) < 1e-5
+
+
+# ------------------------------------------------------------ stencils ----
+@pytest.mark.parametrize("shape", [(64, 32, 24), (8, 16, 12), (128, 64, 32)])
+@pytest.mark.parametrize("align", [1, 0])
+def test_stressupdate(shape, align):
+    rng = np.random.default_rng(16)
+    sx1, sx2, h3 = shape
+    p = sx1 + 2 if align else sx1
+    v = [rnd(rng, h3, sx2, p) for _ in range(3)]
+    sig = rnd(rng, h3, sx2, sx1, 6)
+    ref = sig.copy()
+    vp = [np.ascontiguousarray(np.pad(a[..., :sx1], ((0, 0), (0, 0), (0, 2)))) for a in v]
+    elastic3d.stressupdate(*v, LAM, MU, 0.5, 0.7, 0.9, sig, align_data=align)
+    O.stressupdate(*vp, LAM, MU, 0.5, 0.7, 0.9, ref)
+    assert rel_l2(sig, ref) < 1e-6
+
+
+@pytest.mark.parametrize("shape", [(64, 32, 24), (16, 8, 10)])
+def test_equivalentbodyforce(shape):
+    rng = np.random.default_rng(17)
+    sx1, sx2, h3 = shape
+    sig = rnd(rng, h3, sx2, sx1, 6)
+    c = [rnd(rng, h3, sx2, sx1 + 2) for _ in range(3)]
+    t = [rnd(rng, sx2, sx1 + 2) for _ in range(3)]
+    mask = (rng.uniform(size=h3) > 0.3).astype(np.float32)
+    rc, rt = [x.copy() for x in c], [x.copy() for x in t]
+    elastic3d.equivalentbodyforce(sig, 0.5, 0.6, 0.7, *c, *t, mask=mask)
+    O.equivalentbodyforce(sig, 0.5, 0.6, 0.7, *rc, *rt, mask=mask)
+    for a, b in zip(c + t, rc + rt):
+        assert rel_l2(a, b) < 1e-6
+
+
+def test_field_algebra_and_amplitude():
+    rng = np.random.default_rng(18)
+    u, v = rnd(rng, 1000), rnd(rng, 1000)
+    for c1, c2 in [(None, None), (0.5, None), (None, -2.0), (0.25, 3.0)]:
+        a = u.copy()
+        elastic3d.fieldadd(a, v, c1, c2)
+        ref = (u if c1 is None else np.float32(c1) * u) + (v if c2 is None else np.float32(c2) * v)
+        np.testing.assert_array_equal(a, ref)
+    t1, t2 = rnd(rng, 50, 6), rnd(rng, 50, 6)
+    for c1, c2, ref in [(0.0, 0.0, np.zeros_like(t1)), (0.0, 2.0, np.float32(2) * t2),
+                        (1.5, -1.0, np.float32(1.5) * t1 + np.float32(-1) * t2), (None, None, t2 + t2)]:
+        a = t1.copy()
+        elastic3d.tensorfieldadd(a, t2, c1, c2)
+        np.testing.assert_array_equal(a, ref)
+    t = rnd(rng, 4, 5, 6, 6)
+    assert abs(elastic3d.tensoramplitude(t, 0.5, 0.5, 2.0) / O.tensoramplitude(t.astype(np.float64), 0.5, 0.5,
+                                                                                  2.0) - 1) < 1e-6
+
+
+def test_momentdensityshear():
+    sig = np.zeros((24, 64, 64, 6), np.float32)
+    ref = sig.copy()
+    args = (MU, 2e-3, 1.0, -2.0, 4.0, 6.0, 8.0, 0.4, 0.3, 1.1, 0.2, 0.5, 0.5, 0.5)
+    elastic3d.momentdensityshear(*args, sig)
+    O.momentdensityshear(*args, ref)
+    assert np.abs(ref).max() > 0 and rel_l2(sig, ref) < 1e-6
+
+
+# ------------------------------------------------------------ rheology ----
+def layers(h3, g0=0.2, n=3.0, coh=0.1):
+    st = np.zeros(h3, LAYER)
+    st["gammadot0"] = np.linspace(0, 1, h3) * g0
+    st["stressexponent"] = n
+    st["cohesion"] = coh
+    st["friction"] = 0.6
+    return st
+
+
+@pytest.mark.parametrize("n", [1.0, 3.0, 3.5])
+def test_viscouseigenstress(n):
+    rng = np.random.default_rng(19)
+    h3, sx2, sx1 = 12, 16, 32
+    sig = rnd(rng, h3, sx2, sx1, 6) * 50
+    st = layers(h3, 0.2 if n == 1 else 2e6, n)
+    pre = np.zeros(h3, TENSOR_LAYER)
+    pre["t"][:, 1] = 3.0
+    pre["t"][:, 0] = -1.0
+    dg = (rng.uniform(size=(h3, sx2, sx1)) * 0.01).astype(np.float32)
+    m, g = np.zeros_like(sig), np.zeros((h3, sx2, sx1 + 2), np.float32)
+    rm, rg = m.copy(), g.copy()
+    tm = viscoelastic3d.viscouseigenstress(MU, st, pre, sig, m, 1e7, dg, g)
+    rtm = O.viscouseigenstress(MU, st, pre, sig, rm, 1e7, dg, rg)
+    assert rel_l2(m, rm) < 1e-5 and rel_l2(g, rg) < 1e-5
+    assert abs(tm / rtm - 1) < 1e-5
+
+
+def test_viscous_errors_and_ignored_structure():
+    sig = np.ones((4, 16, 16, 6), np.float32)
+    with pytest.raises(RelaxError) as e:
+        viscoelastic3d.viscouseigenstress(MU, layers(4, n=0.5), np.zeros(4, TENSOR_LAYER), sig, np.zeros_like(sig))
+    assert e.value.status == 1
+    m = np.zeros_like(sig)  # a structure of the wrong size is ignored (viscoelastic3d.f90:240-241)
+    viscoelastic3d.viscouseigenstress(MU, layers(3), np.zeros(3, TENSOR_LAYER), sig, m)
+    assert np.abs(m).max() == 0
+
+
+def friction_layers(h3):
+    st = np.zeros(h3, LAYER)
+    st["gammadot0"] = 1e-3
+    st["stressexponent"] = 1.0
+    st["friction"] = 0.6
+    return st
+
+
+def test_frictioneigenstress():
+    rng = np.random.default_rng(20)
+    h3, sx = 32, 64
+    sig = rnd(rng, h3, sx, sx, 6) * 2.0
+    st = friction_layers(h3)
+    args = (1.0, -2.0, 6.0, 10.0, 16.0, 0.3, 0.8, 1.2, 0.2)
+    m, v = np.zeros_like(sig), np.zeros((h3, sx, sx + 2), np.float32)
+    rm, rv = m.copy(), v.copy()
+    tm = friction3d.frictioneigenstress(*args, sig, MU, st, 0.5, 0.5, 0.5, m, 1e7, v)
+    rtm = O.frictioneigenstress(*args, sig, MU, st, 0.5, 0.5, 0.5, rm, 1e7, rv)
+    assert np.abs(rm).max() > 0
+    assert rel_l2(m, rm) < 1e-5 and rel_l2(v, rv) < 1e-5 and abs(tm / rtm - 1) < 1e-5
+    sig[...] = 0
+    sig[..., 1] = 50.0  # tau / tauc > 20
+    with pytest.raises(RelaxError) as e:
+        friction3d.frictioneigenstress(*args, sig, MU, st, 0.5, 0.5, 0.5, np.zeros_like(sig))
+    assert e.value.status == 5
+
+
+def test_monitorfriction_frictionadd():
+    rng = np.random.default_rng(21)
+    h3, sx = 32, 64
+    sig = rnd(rng, h3, sx, sx, 6)
+    st = friction_layers(h3)
+    patch = np.zeros((6, 8), PATCH)
+    ref = np.zeros((6, 8), O.PATCH)
+    args = (1.0, -2.0, 5.0, 6.0, 8.0, 0.3, 0.8, 1.2, 0.2, 2 * h3, 0.5, 0.5, 0.5)
+    friction3d.monitorfriction(*args, sig, st, patch)
+    O.monitorfriction(*args, sig.astype(np.float64), st, ref)
+    for f in ("x1", "x2", "x3", "lx", "lz"):
+        np.testing.assert_allclose(patch[f], ref[f], rtol=1e-12, atol=1e-12)
+    np.testing.assert_array_equal(patch["sig"], ref["sig"])  # the same grid samples
+    for f in ("v", "vss", "vds", "taus"):
+        np.testing.assert_allclose(patch[f], ref[f], rtol=1e-9, atol=1e-15)
+    friction3d.frictionadd(patch, 0.3)
+    O.frictionadd(ref, 0.3)
+    for f in ("ss", "ds", "slip"):
+        np.testing.assert_allclose(patch[f], ref[f], rtol=1e-12, atol=1e-18)
+
+
+# --------------------------------------------------------- RK2 (model) ----
+def run_pair(s, nsteps, replay=True):
+    st = oracle_initial_state(s)
+    om = oracle_model(s)
+    ost = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in st.items()}
+    info = O.rk2_run(om, ost, nsteps)
+    g = workloads.build(s)
+    recs = g.run(nsteps, dt_replay=info[:, 0] if replay else None)
+    out = g.download()
+    patches = g.get_patches() if om.get("patches") is not None else None
+    g.close()
+    return st, ost, info, out, recs, om, patches
+
+
+@pytest.mark.parametrize("kind,nsteps", [("maxwell", 6), ("powerlaw", 4), ("afterslip", 4)])
+def test_rk2_trajectory_matches_oracle(kind, nsteps):
+    s = small_spec(kind)
+    st, ost, info, out, recs, om, patches = run_pair(s, nsteps)
+    assert len(recs) == nsteps
+    for k in ("u1", "u2", "u3"):
+        assert rel_l2(out[k][..., :s["sx1"]], ost[k][..., :s["sx1"]]) < 1e-4, k
+    for k in ("sig", "tau"):
+        assert rel_l2(out[k], ost[k]) < 1e-4, k
+    if np.abs(ost["gamma"]).max() > 0:
+        assert rel_l2(out["gamma"][..., :s["sx1"]], ost["gamma"][..., :s["sx1"]]) < 1e-4
+    for k, r in enumerate(recs):
+        assert abs(r["power"] / info[k, 5] - 1) < 1e-4
+        assert abs(r["cei"] / info[k, 6] - 1) < 1e-4
+        assert abs(r["t"] - info[k, 7]) <= 1e-9 * abs(info[k, 7])
+    if patches is not None:
+        ref = om["patches"]
+        assert np.abs(ref["v"]).max() > 0
+        assert rel_l2(patches["slip"], ref["slip"]) < 1e-4
+        assert rel_l2(patches["v"], ref["v"]) < 1e-4
+
+
+@pytest.mark.parametrize("kind", ["maxwell", "powerlaw", "afterslip"])
+def test_rk2_adaptive_step_matches_oracle(kind):
+    """Without replay the device-side Dt (min 1/gammadot, integrationstep)
+    follows the oracle's to FP32 accuracy."""
+    s = small_spec(kind)
+    st, ost, info, out, recs, om, patches = run_pair(s, 3, replay=False)
+    for k, r in enumerate(recs):
+        assert abs(r["Dt"] / info[k, 0] - 1) < 1e-5
+        for j in range(3):
+            assert abs(r["maxwell"][j] / info[k, 2 + j] - 1) < 1e-5
+    assert rel_l2(out["sig"], ost["sig"]) < 1e-4
+
+
+def test_coseismic_setup_matches_oracle():
+    s = small_spec("powerlaw")
+    st = oracle_initial_state(s)
+    g = workloads.build(s)
+    out = g.download()
+    g.close()
+    assert rel_l2(out["tau"], st["tau"]) < 1e-6
+    for k in ("u1", "u2", "u3"):
+        assert rel_l2(out[k][..., :s["sx1"]], st[k][..., :s["sx1"]]) < 1e-5
+    assert rel_l2(out["sig"], st["sig"]) < 1e-5
+
+
+def test_output_time_truncation_c0():
+    """c0 literally: the Maxwell layer lies below the grid, so tm = STEP_MAX
+    and Dt lands on the output times: 8 steps of 1.25 yr to 10 yr."""
+    s = workloads.spec("c0")
+    g = workloads.build(s)
+    recs = g.run(20)
+    g.close()
+    assert len(recs) == 8
+    np.testing.assert_allclose([r["t"] for r in recs], 1.25 * np.arange(1, 9), rtol=1e-12)
+    np.testing.assert_allclose([r["maxwell"][0] for r in recs], 1e7)
+
+
+def test_rk2_is_bitwise_reproducible():
+    s = small_spec("afterslip")
+    outs = []
+    for _ in range(2):
+        g = workloads.build(s)
+        g.run(3)
+        outs.append(g.download())
+        g.close()
+    for k in ("u1", "sig", "tau", "gamma"):
+        np.testing.assert_array_equal(outs[0][k], outs[1][k])
+
+
+def test_uniform_decay_at_full_size():
+    """512x512x256: a uniform deviatoric stress has no divergence, so u stays
+    zero and sigma decays by exactly (1 - h l + (h l)^2/2) per step, h l = 0.2."""
+    sx1 = sx2 = 512
+    sx3 = 256
+    h3 = sx3 // 2
+    with Model(sx1, sx2, sx3, 0.5, 0.5, 0.5, LAM, MU, 8.33e-4) as g:
+        st = np.zeros(h3, LAYER)
+        st["gammadot0"] = 0.5
+        st["stressexponent"] = 1.0
+        g.set_prestress(np.zeros(h3, TENSOR_LAYER))
+        g.set_structure(0, st)
+        z = np.zeros((h3, sx2, sx1 + 2), np.float32)
+        tau = np.zeros((h3, sx2, sx1, 6), np.float32)
+        tau[..., 1] = -2.0
+        g.upload(z, z, z, tau)
+        recs = g.run(3)
+        out = g.download()
+    f = (1 - 0.2 + 0.02) ** 3
+    np.testing.assert_allclose(out["sig"][..., 1], 2.0 * f, rtol=2e-6)
+    assert np.abs(out["u1"]).max() == 0 and np.abs(out["u3"]).max() == 0
+    np.testing.assert_allclose([r["Dt"] for r in recs], 0.2, rtol=1e-6)
+
+
+def test_model_rejects_bad_input():
+    with pytest.raises(RelaxError) as e:
+        Model(48, 64, 64, 1.0, 1.0, 1.0, LAM, MU)
+    assert e.value.status == 11
+    with Model(64, 64, 64, 1.0, 1.0, 1.0, LAM, MU) as g:
+        with pytest.raises(RelaxError) as e:
+            g.set_structure(1, layers(32, n=0.5))
+        assert e.value.status == 1
