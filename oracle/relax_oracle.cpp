@@ -708,8 +708,9 @@ static int monitorfriction(double x, double y, double z, double L, double W, dou
       double xr = cdip * x2 - sdip * z;
       double yr = sstrike * x + cstrike * y;
       double zr = sdip * x2 + cdip * z;
-      yr = (double)(float)yr + (double)(float)(((double)j2 - (double)px2 / 2.0 - 1.0) * dx2);
-      zr = (double)(float)zr + (double)(float)(((double)j3 - (double)px3 / 2.0 - 1.0) * dx3);
+      // yr=REAL(yr)+REAL(...): a single-precision sum stored in REAL*8 (friction3d.f90:335-336)
+      yr = (double)((float)yr + (float)(((double)j2 - (double)px2 / 2.0 - 1.0) * dx2));
+      zr = (double)((float)zr + (float)(((double)j3 - (double)px3 / 2.0 - 1.0) * dx3));
       double x2p = cdip * xr + sdip * zr;
       double X1 = cstrike * x2p + sstrike * yr;
       double X2 = -sstrike * x2p + cstrike * yr;
