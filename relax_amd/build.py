@@ -46,5 +46,25 @@ def build(verbose=False):
     return OUT
 
 
+def build_fortran():
+    """Compile MODULE relax_hip and the Fortran drop-in test driver with
+    flang (skipped when the image has no flang)."""
+    import shutil
+    flang = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+    if not os.path.exists(flang):
+        return None
+    root = os.path.dirname(HERE)
+    bdir = os.path.join(root, "tests", "fortran", "_bin")
+    os.makedirs(bdir, exist_ok=True)
+    exe = os.path.join(bdir, "drop_in")
+    cmd = [flang, "-cpp", "-O2", "-J", bdir, os.path.join(HERE, "fortran", "relax_hip.f90"),
+           os.path.join(root, "tests", "fortran", "drop_in.f90"), "-L" + HERE, "-lrelax_hip",
+           "-Wl,-rpath,$ORIGIN/../../../relax_amd", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

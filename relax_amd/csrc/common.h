@@ -80,10 +80,11 @@ struct PlaneDev {
   double minlw;  // min(L,W)/sqrt(dx1*dx3) of the relaxation-time estimate
 };
 struct FrictionTables {
-  const float* v0;        // h3 (gammadot0 of the friction structure)
-  const float* tauc;      // h3 ((a-b) sigma, stored in stressexponent)
-  const float* friction;  // h3
-  const float* cohesion;  // h3
+  // double precision, as LAYER_STRUCT (types.f90:31-34)
+  const double* v0;        // h3 (gammadot0 of the friction structure)
+  const double* tauc;      // h3 ((a-b) sigma, stored in stressexponent)
+  const double* friction;  // h3
+  const double* cohesion;  // h3
   const PlaneDev* planes;
   int np;
   int active;

@@ -261,12 +261,13 @@ __global__ void k_friction_generic(RheoParams R, int plane, int n3, const float*
 
 // ------------------------------------------------------- field algebra ----
 __global__ void k_fieldadd(float* u, const float* v, long n, int has1, float c1, int has2, float c2) {
+#pragma clang fp contract(off)
   for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < n; o += (long)gridDim.x * blockDim.x) {
     // each product is rounded before the sum, as in the reference
     const float a = u[o], b = v[o];
-    if (has1 && has2) u[o] = __fadd_rn(__fmul_rn(c1, a), __fmul_rn(c2, b));
-    else if (has1) u[o] = __fadd_rn(__fmul_rn(c1, a), b);
-    else if (has2) u[o] = __fadd_rn(a, __fmul_rn(c2, b));
+    if (has1 && has2) u[o] = c1 * a + c2 * b;
+    else if (has1) u[o] = c1 * a + b;
+    else if (has2) u[o] = a + c2 * b;
     else u[o] = a + b;
   }
 }
@@ -274,16 +275,17 @@ __global__ void k_fieldadd(float* u, const float* v, long n, int has1, float c1,
 // tensorfieldadd, elastic3d.f90:3880-3959 (the last branch, with neither
 // coefficient present, returns t2 + t2 as the reference does)
 __global__ void k_tensorfieldadd(float* t1, const float* t2, long n, int has1, float c1, int has2, float c2) {
+#pragma clang fp contract(off)
   for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < 6 * n; o += (long)gridDim.x * blockDim.x) {
     const float a = t1[o], b = t2[o];
     float r;
     if (has1 && has2) {
       if (0.f == c1) r = (0.f == c2) ? 0.f : c2 * b;
-      else r = __fadd_rn(__fmul_rn(c1, a), __fmul_rn(c2, b));
+      else r = c1 * a + c2 * b;
     } else if (has1) {
-      r = __fadd_rn(__fmul_rn(c1, a), b);
+      r = c1 * a + b;
     } else if (has2) {
-      r = __fadd_rn(a, __fmul_rn(c2, b));
+      r = a + c2 * b;
     } else {
       r = b + b;
     }
@@ -748,20 +750,20 @@ int relax_frictioneigenstress(double x, double y, double z, double L, double W, 
   const long N = (long)sx1 * sx2 * sx3;
   const long vp = align_data ? sx1 + 2 : sx1;
   const size_t vb = (size_t)vp * sx2 * sx3 * 4;
-  std::vector<float> h(4 * sx3);
+  std::vector<double> h(4 * sx3);
   for (int i = 0; i < sx3; ++i) {
-    h[i] = (float)structure[i].gammadot0;
-    h[sx3 + i] = (float)structure[i].stressexponent;
-    h[2 * sx3 + i] = (float)structure[i].friction;
-    h[3 * sx3 + i] = (float)structure[i].cohesion;
+    h[i] = structure[i].gammadot0;
+    h[sx3 + i] = structure[i].stressexponent;
+    h[2 * sx3 + i] = structure[i].friction;
+    h[3 * sx3 + i] = structure[i].cohesion;
   }
   relax_plane_t pl{x, y, z, /*width*/ L, /*length*/ W, strike, dip, rake, 0, 0};
   const PlaneDev P = make_plane(pl, beta, dx1, dx3);
   Buf tab, pd, s, m, v, tb, stt;
-  if ((rc = tab.alloc(h.size() * 4)) || (rc = pd.alloc(sizeof(PlaneDev))) || (rc = s.alloc(N * 24)) ||
+  if ((rc = tab.alloc(h.size() * 8)) || (rc = pd.alloc(sizeof(PlaneDev))) || (rc = s.alloc(N * 24)) ||
       (rc = m.alloc(N * 24)) || (rc = tb.alloc(4)) || (rc = stt.alloc(4)))
     return rc;
-  HIP_TRY(hipMemcpy(tab.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(tab.p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(pd.p, &P, sizeof(P), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s.p, sig, N * 24, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(m.p, moment, N * 24, hipMemcpyHostToDevice));
@@ -776,7 +778,8 @@ int relax_frictioneigenstress(double x, double y, double z, double L, double W, 
   RheoParams R;
   std::memset(&R, 0, sizeof(R));
   R.mu = (float)mu;
-  R.fr.v0 = tab.f(); R.fr.tauc = tab.f() + sx3; R.fr.friction = tab.f() + 2 * sx3; R.fr.cohesion = tab.f() + 3 * sx3;
+  const double* td = (const double*)tab.p;
+  R.fr.v0 = td; R.fr.tauc = td + sx3; R.fr.friction = td + 2 * sx3; R.fr.cohesion = td + 3 * sx3;
   R.fr.planes = (const PlaneDev*)pd.p; R.fr.np = 1; R.fr.active = 1;
   R.n1 = sx1; R.n2 = sx2; R.dx1 = dx1; R.dx2 = dx2; R.dx3 = dx3;
   k_friction_generic<<<g1d(N), 256>>>(R, 0, sx3, s.f(), m.f(), vel ? v.f() : nullptr, vp, (unsigned int*)tb.p,
@@ -811,26 +814,27 @@ int relax_monitorfriction(double x, double y, double z, double L, double W, doub
   g.H = (long)sx1 * sx2 * sx3_alloc;
   g.dx1 = (float)dx1; g.dx2 = (float)dx2; g.dx3 = (float)dx3;
   const long H = g.H, np = (long)px2 * px3;
-  std::vector<float> h(4 * sx3_alloc);
+  std::vector<double> h(4 * sx3_alloc);
   for (int i = 0; i < sx3_alloc; ++i) {
-    h[i] = (float)structure[i].gammadot0;
-    h[sx3_alloc + i] = (float)structure[i].stressexponent;
-    h[2 * sx3_alloc + i] = (float)structure[i].friction;
-    h[3 * sx3_alloc + i] = (float)structure[i].cohesion;
+    h[i] = structure[i].gammadot0;
+    h[sx3_alloc + i] = structure[i].stressexponent;
+    h[2 * sx3_alloc + i] = structure[i].friction;
+    h[3 * sx3_alloc + i] = structure[i].cohesion;
   }
   relax_plane_t pl{x, y, z, /*width*/ L, /*length*/ W, strike, dip, rake, px2, px3};
   const PlaneDev P = make_plane(pl, beta, dx1, dx3);
   Buf tab, aos, soa, pa, sc;
-  if ((rc = tab.alloc(h.size() * 4)) || (rc = aos.alloc(H * 24)) || (rc = soa.alloc(H * 24)) ||
+  if ((rc = tab.alloc(h.size() * 8)) || (rc = aos.alloc(H * 24)) || (rc = soa.alloc(H * 24)) ||
       (rc = pa.alloc(np * sizeof(relax_slippatch_t))) || (rc = sc.alloc(sizeof(StepScalars))))
     return rc;
-  HIP_TRY(hipMemcpy(tab.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(tab.p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(aos.p, sig, H * 24, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(pa.p, patch, np * sizeof(relax_slippatch_t), hipMemcpyHostToDevice));
   HIP_TRY(hipMemset(sc.p, 0, sizeof(StepScalars)));
   if ((rc = launch_aos_soa(aos.f(), soa.f(), H, true, 0))) return rc;
-  if ((rc = launch_monitorfriction(P, rake, px2, px3, g, dx1, dx2, dx3, sx3, soa.f(), tab.f(), tab.f() + sx3_alloc,
-                                   tab.f() + 2 * sx3_alloc, tab.f() + 3 * sx3_alloc, (relax_slippatch_t*)pa.p,
+  const double* td = (const double*)tab.p;
+  if ((rc = launch_monitorfriction(P, rake, px2, px3, g, dx1, dx2, dx3, sx3, soa.f(), td, td + sx3_alloc,
+                                   td + 2 * sx3_alloc, td + 3 * sx3_alloc, (relax_slippatch_t*)pa.p,
                                    (StepScalars*)sc.p, 0)))
     return rc;
   if ((rc = sync_status(RELAX_OK))) return rc;

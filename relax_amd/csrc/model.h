@@ -25,6 +25,8 @@ struct relax_model {
   double2* partials = nullptr;
   long nblk = 0;
   float* tables = nullptr;  // per-depth rheology tables (17 * sx3/2 floats)
+  double* ftab = nullptr;   // fault-creep tables: v0, (a-b)sigma, friction, cohesion
+
   PlaneDev* planes_dev = nullptr;
   relax_slippatch_t* patches_dev = nullptr;
   std::vector<relax_plane_t> host_planes;
@@ -46,7 +48,7 @@ struct relax_model {
 PlaneDev make_plane(const relax_plane_t& p, double beta, double dx1, double dx3);
 int launch_frictionadd(relax_slippatch_t* patch, long n, double dt, hipStream_t st);
 int launch_monitorfriction(const PlaneDev& P, double rake, int px2, int px3, const Grid& g, double dx1, double dx2,
-                           double dx3, int sx3_range, const float* sig, const float* v0, const float* tauc,
-                           const float* fric, const float* coh, relax_slippatch_t* patch, StepScalars* sc,
+                           double dx3, int sx3_range, const float* sig, const double* v0, const double* tauc,
+                           const double* fric, const double* coh, relax_slippatch_t* patch, StepScalars* sc,
                            hipStream_t st);
 int launch_aos_soa(const float* aos, float* soa, long H, bool to_soa, hipStream_t st);

@@ -99,8 +99,8 @@ __device__ __forceinline__ int d_fix(double number) {
 // monitorfriction, friction3d.f90:263-416: one thread per patch
 __global__ void k_monitorfriction(PlaneDev P, double rake, int px2, int px3, Grid g, double dx1, double dx2,
                                   double dx3, int sx3_range, const float* __restrict__ sig,
-                                  const float* __restrict__ v0, const float* __restrict__ tauc_t,
-                                  const float* __restrict__ fric, const float* __restrict__ coh,
+                                  const double* __restrict__ v0, const double* __restrict__ tauc_t,
+                                  const double* __restrict__ fric, const double* __restrict__ coh,
                                   relax_slippatch_t* patch, StepScalars* sc) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= px2 * px3) return;
@@ -136,10 +136,10 @@ __global__ void k_monitorfriction(PlaneDev P, double rake, int px2, int px3, Gri
   const double taun = t0 * n[0] + t1 * n[1] + t2 * n[2];
   double ts0 = t0 - taun * n[0], ts1 = t1 - taun * n[1], ts2 = t2 - taun * n[2];
   const double taus = sqrt(ts0 * ts0 + ts1 * ts1 + ts2 * ts2);
-  const double tau = fmax(0.0, taus + (double)fric[i3] * taun - (double)coh[i3]);
+  const double tau = fmax(0.0, taus + fric[i3] * taun - coh[i3]);
   if ((ts0 * P.r[0] + ts1 * P.r[1] + ts2 * P.r[2]) < 0.0 && fabs(rake) < RELAX_PI2 * 1.5) return;
   Q.taus = taus;
-  const double v = (double)v0[i3] * 2.0 * d_my_sinh(tau / (double)tauc_t[i3]);
+  const double v = v0[i3] * 2.0 * d_my_sinh(tau / tauc_t[i3]);
   Q.v = v;
   ts0 /= taus; ts1 /= taus; ts2 /= taus;
   Q.vss = v * (ts0 * sv[0] + ts1 * sv[1] + ts2 * sv[2]);
@@ -162,8 +162,8 @@ int launch_frictionadd(relax_slippatch_t* patch, long n, double dt, hipStream_t 
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }
 int launch_monitorfriction(const PlaneDev& P, double rake, int px2, int px3, const Grid& g, double dx1, double dx2,
-                           double dx3, int sx3_range, const float* sig, const float* v0, const float* tauc,
-                           const float* fric, const float* coh, relax_slippatch_t* patch, StepScalars* sc,
+                           double dx3, int sx3_range, const float* sig, const double* v0, const double* tauc,
+                           const double* fric, const double* coh, relax_slippatch_t* patch, StepScalars* sc,
                            hipStream_t st) {
   const int n = px2 * px3;
   if (n <= 0) return RELAX_OK;
@@ -208,7 +208,7 @@ size_t relax_model_footprint(const relax_model_desc_t* d) {
 
 static void free_model(relax_model_t* m) {
   if (!m) return;
-  void* ptrs[] = {m->u, m->sig, m->tau, m->gam, m->W, m->NQ, m->T, m->TN, m->sc, m->partials, m->tables,
+  void* ptrs[] = {m->u, m->sig, m->tau, m->gam, m->W, m->NQ, m->T, m->TN, m->sc, m->partials, m->tables, m->ftab,
                   m->planes_dev, m->patches_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -254,6 +254,7 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
   m->nblk = stress_blocks(g);
   alloc((void**)&m->partials, m->nblk * sizeof(double2));
   alloc((void**)&m->tables, 17L * g.h3 * 4);
+  alloc((void**)&m->ftab, 4L * g.h3 * 8);
   if (ok && hipHostMalloc((void**)&m->host_sc, sizeof(StepScalars) * 64) != hipSuccess) ok = false;
   m->host_cap = 64;
   if (!ok) { free_model(m); return RELAX_ERR_NOMEM; }
@@ -262,7 +263,8 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
       hipMemsetAsync(m->sig, 0, 6 * H * 4, m->stream) != hipSuccess ||
       hipMemsetAsync(m->tau, 0, 6 * H * 4, m->stream) != hipSuccess ||
       hipMemsetAsync(m->gam, 0, H * 4, m->stream) != hipSuccess ||
-      hipMemsetAsync(m->tables, 0, 17L * g.h3 * 4, m->stream) != hipSuccess) {
+      hipMemsetAsync(m->tables, 0, 17L * g.h3 * 4, m->stream) != hipSuccess ||
+      hipMemsetAsync(m->ftab, 0, 4L * g.h3 * 8, m->stream) != hipSuccess) {
     free_model(m);
     return RELAX_ERR_HIP;
   }
@@ -278,10 +280,10 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
     R.mech[k].cohesion = m->tables + (3 * k + 2) * h3;
     R.mech[k].active = 0;
   }
-  R.fr.v0 = m->tables + 6 * h3;
-  R.fr.tauc = m->tables + 7 * h3;
-  R.fr.friction = m->tables + 8 * h3;
-  R.fr.cohesion = m->tables + 9 * h3;
+  R.fr.v0 = m->ftab;
+  R.fr.tauc = m->ftab + h3;
+  R.fr.friction = m->ftab + 2 * h3;
+  R.fr.cohesion = m->ftab + 3 * h3;
   R.pre.sp = m->tables + 10 * h3;  // 6*h3
   R.pre.taup = m->tables + 16 * h3;
   R.n1 = g.n1; R.n2 = g.n2;
@@ -329,23 +331,26 @@ int relax_model_set_structure(relax_model_t* m, int which, const relax_layer_t* 
   // viscouseigenstress returns without effect when the structure does not
   // match the grid (viscoelastic3d.f90:240-241); here that is an error
   if (n != h3) return RELAX_ERR_ARG;
-  std::vector<float> a(h3), b(h3), c(h3), e(h3);
+  std::vector<float> a(h3), b(h3), c(h3);
+  std::vector<double> fd(4 * h3);
   for (int i = 0; i < h3; ++i) {
     if (which < 2 && layers[i].stressexponent < 0.999999) return RELAX_ERR_POWER;
     a[i] = (float)layers[i].gammadot0;
     b[i] = (float)layers[i].stressexponent;
     c[i] = (float)layers[i].cohesion;
-    e[i] = (float)layers[i].friction;
+    fd[i] = layers[i].gammadot0;
+    fd[h3 + i] = layers[i].stressexponent;
+    fd[2 * h3 + i] = layers[i].friction;
+    fd[3 * h3 + i] = layers[i].cohesion;
   }
-  float* base = m->tables + (which < 2 ? 3 * which : 6) * (long)h3;
-  HIP_TRY(hipMemcpyAsync(base, a.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
-  HIP_TRY(hipMemcpyAsync(base + h3, b.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
   if (which < 2) {
+    float* base = m->tables + 3 * which * (long)h3;
+    HIP_TRY(hipMemcpyAsync(base, a.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
+    HIP_TRY(hipMemcpyAsync(base + h3, b.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
     HIP_TRY(hipMemcpyAsync(base + 2 * h3, c.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
     m->R.mech[which].active = 1;
   } else {
-    HIP_TRY(hipMemcpyAsync(base + 2 * h3, e.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
-    HIP_TRY(hipMemcpyAsync(base + 3 * h3, c.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
+    HIP_TRY(hipMemcpyAsync(m->ftab, fd.data(), 4L * h3 * 8, hipMemcpyHostToDevice, m->stream));
     m->R.fr.active = m->R.fr.np > 0;
   }
   m->mech[which] = 1;
@@ -381,8 +386,8 @@ PlaneDev make_plane(const relax_plane_t& p, double beta, double dx1, double dx3)
   P.cdip = std::cos(p.dip); P.sdip = std::sin(p.dip);
   P.rake = p.rake;
   const double cr = std::cos(p.rake), sr = std::sin(p.rake);
-  P.Wp = p.width * (1.0 + 2.0 * beta) / 2.0;
-  P.Lp = p.length * (1.0 + 2.0 * beta) / 2.0;
+  P.Wp = P.W * (1.0 + 2.0 * beta) / 2.0;
+  P.Lp = P.L * (1.0 + 2.0 * beta) / 2.0;
   const double x2r = P.cstrike * p.x - P.sstrike * p.y;
   P.xr = P.cdip * x2r - P.sdip * p.z;
   P.yr = P.sstrike * p.x + P.cstrike * p.y;

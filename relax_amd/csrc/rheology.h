@@ -113,7 +113,7 @@ __device__ __forceinline__ float friction_point(const RheoParams& R, const Plane
   const double tauc = R.fr.tauc[i3];
   if ((tau / tauc) > 20) { *status = RELAX_ERR_FRICTION; return __builtin_inff(); }
   ts0 /= taus; ts1 /= taus; ts2 /= taus;
-  const double gd = (double)R.fr.v0[i3] * 2.0 * d_my_sinh(tau / tauc);
+  const double gd = R.fr.v0[i3] * 2.0 * d_my_sinh(tau / tauc);
   float cand = (float)(tau / (double)R.mu / gd * P.minlw);
   if (cand != cand) cand = __builtin_inff();
   if (vel) *vel = (float)((double)*vel + gd * impulse * sqrt(RELAX_PI2) * dx1);

@@ -27,7 +27,7 @@ def small_spec(kind, sx=64, sx3=64, dx=1.0, seed=0):
                     planes=None, deck=deck, interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=4)
     if kind == "afterslip":
         dip = 20.0
-        deck = [(rng.uniform(0, 3e-3), -6.0 + j2, -j3 * np.cos(np.radians(dip)), j3 * np.sin(np.radians(dip)),
+        deck = [(rng.uniform(0, 1e-4), -6.0 + j2, -j3 * np.cos(np.radians(dip)), j3 * np.sin(np.radians(dip)),
                  1.0, 1.0, 0.0, dip, 90.0) for j3 in range(6) for j2 in range(12)]
         plane = dict(x=-12.0, y=-6 * np.cos(np.radians(dip)), z=6 * np.sin(np.radians(dip)), length=24.0,
                      width=12.0, strike=0.0, dip=dip, rake=90.0)

@@ -235,8 +235,13 @@ def test_frictioneigenstress():
     rtm = O.frictioneigenstress(*args, sig, MU, st, 0.5, 0.5, 0.5, rm, 1e7, rv)
     assert np.abs(rm).max() > 0
     assert rel_l2(m, rm) < 1e-5 and rel_l2(v, rv) < 1e-5 and abs(tm / rtm - 1) < 1e-5
-    sig[...] = 0
-    sig[..., 1] = 50.0  # tau / tauc > 20
+    # shear traction of 50 MPa along the rake: tau / tauc > 20 (STOP 5)
+    strike, dip, rake = args[5], args[6], args[7]
+    n = np.array([np.cos(dip) * np.cos(strike), -np.cos(dip) * np.sin(strike), -np.sin(dip)])
+    r = np.array([np.sin(strike) * np.cos(rake) + np.cos(strike) * np.sin(dip) * np.sin(rake),
+                  np.cos(strike) * np.cos(rake) - np.sin(strike) * np.sin(dip) * np.sin(rake), np.cos(dip) * np.sin(rake)])
+    S = 25.0 * (np.outer(r, n) + np.outer(n, r))
+    sig[...] = np.float32([S[0, 0], S[0, 1], S[0, 2], S[1, 1], S[1, 2], S[2, 2]])
     with pytest.raises(RelaxError) as e:
         friction3d.frictioneigenstress(*args, sig, MU, st, 0.5, 0.5, 0.5, np.zeros_like(sig))
     assert e.value.status == 5
