@@ -63,6 +63,7 @@ struct Mechanism {
   const float* gammadot0;  // h3
   const float* power;      // h3 (stress exponent n)
   const float* cohesion;   // h3
+  const float* gdp;        // h3: gammadot0 (taup/mu)^(n-1), the prestress rate
   int active;
 };
 

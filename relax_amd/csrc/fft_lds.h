@@ -1,9 +1,10 @@
 // fft_lds.h -- batched power-of-two complex FFTs held in LDS.
 //
-// Stockham autosort (decimation in frequency), radix 4 with one trailing
-// radix-2 stage when log2(L) is odd.  The transform runs in place: every
-// thread reads all inputs of its butterflies into registers, the workgroup
-// synchronises, then every thread writes its outputs.
+// Stockham autosort (decimation in frequency) with radix-8 stages and one
+// trailing radix-4 or radix-2 stage (512 = 8*8*8, 256 = 8*8*4, 128 = 8*8*2).
+// The transform runs in place: every thread reads all inputs of its
+// butterflies into registers, the workgroup synchronises, then every thread
+// writes its outputs.
 //
 // Element j of batch b lives at  buf[(b % B0) * S0 + (b / B0) * S1 + j * JS].
 // Twiddles come from a table tw[x] = exp(-2 pi i x / (L * TWS)) in global
@@ -19,79 +20,107 @@ __device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int idx
   return SIGN < 0 ? w : make_float2(w.x, -w.y);
 }
 
+// multiply by sigma*i (sigma = SIGN)
+template <int SIGN>
+__device__ __forceinline__ float2 mul_si(float2 a) { return SIGN < 0 ? mul_mi(a) : mul_i(a); }
+
+// small DFTs in natural order, y_t = sum_r a_r exp(SIGN 2 pi i r t / R)
+template <int SIGN>
+__device__ __forceinline__ void dft2(float2* a) {
+  const float2 x = a[0], y = a[1];
+  a[0] = cadd(x, y);
+  a[1] = csub(x, y);
+}
+template <int SIGN>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  const float2 s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = mul_si<SIGN>(csub(a1, a3));
+  a0 = cadd(s02, s13);
+  a2 = csub(s02, s13);
+  a1 = cadd(d02, d13);
+  a3 = csub(d02, d13);
+}
+template <int SIGN>
+__device__ __forceinline__ void dft4(float2* a) { dft4<SIGN>(a[0], a[1], a[2], a[3]); }
+template <int SIGN>
+__device__ __forceinline__ void dft8(float2* a) {
+  float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4<SIGN>(e0, e1, e2, e3);
+  dft4<SIGN>(o0, o1, o2, o3);
+  const float r = 0.70710678118654752440f;
+  // w^t o_t with w = exp(SIGN 2 pi i / 8)
+  const float2 w1 = make_float2(r * (o1.x - SIGN * o1.y), r * (o1.y + SIGN * o1.x));
+  const float2 w2 = mul_si<SIGN>(o2);
+  const float2 w3 = make_float2(r * (-o3.x - SIGN * o3.y), r * (-o3.y + SIGN * o3.x));
+  a[0] = cadd(e0, o0); a[4] = csub(e0, o0);
+  a[1] = cadd(e1, w1); a[5] = csub(e1, w1);
+  a[2] = cadd(e2, w2); a[6] = csub(e2, w2);
+  a[3] = cadd(e3, w3); a[7] = csub(e3, w3);
+}
+template <int R, int SIGN>
+__device__ __forceinline__ void dft(float2* a) {
+  if (R == 8) dft8<SIGN>(a);
+  else if (R == 4) dft4<SIGN>(a);
+  else dft2<SIGN>(a);
+}
+
+// One Stockham stage of radix R on sub-transforms of length N (stride L/N).
+template <int L, int N, int R, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB>
+__device__ __forceinline__ void lds_stage(float2* buf, const float2* __restrict__ tw, int tid) {
+  constexpr int S = L / N;   // interleaved sub-transforms
+  constexpr int M = N / R;   // length of the outputs' sub-transforms
+  constexpr int Q = L / R;   // butterflies per batch
+  constexpr int NBF = NB * Q;
+  constexpr int K = (NBF + NT - 1) / NT;
+  float2 a[K][R];
+  int ob[K], pp[K], qq[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int u = tid + k * NT;
+    ob[k] = -1;
+    if ((NBF % NT) == 0 || u < NBF) {
+      int b, r;
+      if (MAPB) { b = u % NB; r = u / NB; } else { r = u % Q; b = u / Q; }
+      const int q = r % S, p = r / S;
+      const int o = (b % B0) * S0 + (b / B0) * S1;
+      ob[k] = o; pp[k] = p; qq[k] = q;
+#pragma unroll
+      for (int t = 0; t < R; ++t) a[k][t] = buf[o + (q + S * (p + t * M)) * JS];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (ob[k] < 0) continue;
+    const int p = pp[k], q = qq[k], o = ob[k];
+    dft<R, SIGN>(a[k]);
+    if (M > 1) {
+      const int step = (L / N) * TWS;
+#pragma unroll
+      for (int t = 1; t < R; ++t) a[k][t] = cmul(a[k][t], twiddle<SIGN>(tw, p * t * step));
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) buf[o + (q + S * (R * p + t)) * JS] = a[k][t];
+  }
+  __syncthreads();
+}
+
+template <int L, int N, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB>
+struct LdsStages {
+  static __device__ __forceinline__ void run(float2* buf, const float2* __restrict__ tw, int tid) {
+    constexpr int R = N >= 8 ? 8 : N;
+    lds_stage<L, N, R, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB>(buf, tw, tid);
+    LdsStages<L, N / R, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB>::run(buf, tw, tid);
+  }
+};
+template <int L, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB>
+struct LdsStages<L, 1, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB> {
+  static __device__ __forceinline__ void run(float2*, const float2* __restrict__, int) {}
+};
+
 // MAPB: consecutive threads take consecutive batches (use when S0 == 1);
 // otherwise consecutive threads take consecutive butterflies of one batch.
 template <int L, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB>
 __device__ __forceinline__ void lds_fft(float2* buf, const float2* __restrict__ tw, int tid) {
-  constexpr int LOG2L = ilog2c(L);
-  constexpr int N4 = LOG2L / 2;
-  constexpr bool HAS_R2 = (LOG2L & 1) != 0;
-  constexpr int Q4 = L / 4;  // butterflies per batch in a radix-4 stage
-  constexpr int NBF4 = NB * Q4;
-  constexpr int K4 = (NBF4 + NT - 1) / NT;
-
-#pragma unroll
-  for (int st = 0; st < N4; ++st) {
-    const int n = L >> (2 * st);  // current sub-transform length
-    const int s = 1 << (2 * st);  // number of interleaved sub-transforms
-    const int m = n >> 2;
-    float2 a[K4][4];
-    int ob[K4], pp[K4], qq[K4];
-#pragma unroll
-    for (int k = 0; k < K4; ++k) {
-      const int u = tid + k * NT;
-      ob[k] = -1;
-      if ((NBF4 % NT) == 0 || u < NBF4) {
-        int b, r;
-        if (MAPB) { b = u % NB; r = u / NB; } else { r = u % Q4; b = u / Q4; }
-        const int q = r & (s - 1), p = r >> (2 * st);
-        const int o = (b % B0) * S0 + (b / B0) * S1;
-        ob[k] = o; pp[k] = p; qq[k] = q;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) a[k][t] = buf[o + (q + s * (p + t * m)) * JS];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K4; ++k) {
-      if (ob[k] < 0) continue;
-      const int p = pp[k], q = qq[k], o = ob[k];
-      const float2 apc = cadd(a[k][0], a[k][2]), amc = csub(a[k][0], a[k][2]);
-      const float2 bpd = cadd(a[k][1], a[k][3]), bmd = csub(a[k][1], a[k][3]);
-      const float2 jb = SIGN < 0 ? mul_mi(bmd) : mul_i(bmd);
-      float2 y0 = cadd(apc, bpd);
-      float2 y1 = cadd(amc, jb);
-      float2 y2 = csub(apc, bpd);
-      float2 y3 = csub(amc, jb);
-      if (st + 1 < N4 || HAS_R2) {  // the last radix-4 stage of a pure radix-4 transform has p == 0
-        const int step = (L / n) * TWS;
-        y1 = cmul(y1, twiddle<SIGN>(tw, p * step));
-        y2 = cmul(y2, twiddle<SIGN>(tw, 2 * p * step));
-        y3 = cmul(y3, twiddle<SIGN>(tw, 3 * p * step));
-      }
-      buf[o + (q + s * (4 * p + 0)) * JS] = y0;
-      buf[o + (q + s * (4 * p + 1)) * JS] = y1;
-      buf[o + (q + s * (4 * p + 2)) * JS] = y2;
-      buf[o + (q + s * (4 * p + 3)) * JS] = y3;
-    }
-    __syncthreads();
-  }
-  if (HAS_R2) {
-    constexpr int S = L / 2;  // radix-2 stage: n == 2, p == 0
-    constexpr int NBF2 = NB * S;
-    constexpr int K2 = (NBF2 + NT - 1) / NT;
-#pragma unroll
-    for (int k = 0; k < K2; ++k) {
-      const int u = tid + k * NT;
-      if ((NBF2 % NT) == 0 || u < NBF2) {
-        int b, q;
-        if (MAPB) { b = u % NB; q = u / NB; } else { q = u % S; b = u / S; }
-        const int o = (b % B0) * S0 + (b / B0) * S1;
-        const float2 x0 = buf[o + q * JS], x1 = buf[o + (q + S) * JS];
-        buf[o + q * JS] = cadd(x0, x1);
-        buf[o + (q + S) * JS] = csub(x0, x1);
-      }
-    }
-    __syncthreads();
-  }
+  LdsStages<L, L, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB>::run(buf, tw, tid);
 }

@@ -35,6 +35,7 @@ struct relax_model {
   StepScalars* host_sc = nullptr;  // pinned
   int host_cap = 0;
   int mech[3] = {0, 0, 0};
+  std::vector<double> hg0[2], hn[2], htaup;  // host copies for the prestress-rate table
   RheoParams R{};
   StencilConst S{};
   GreenConst gc{};

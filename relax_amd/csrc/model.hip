@@ -253,7 +253,7 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
   alloc((void**)&m->sc, sizeof(StepScalars));
   m->nblk = stress_blocks(g);
   alloc((void**)&m->partials, m->nblk * sizeof(double2));
-  alloc((void**)&m->tables, 17L * g.h3 * 4);
+  alloc((void**)&m->tables, 19L * g.h3 * 4);
   alloc((void**)&m->ftab, 4L * g.h3 * 8);
   if (ok && hipHostMalloc((void**)&m->host_sc, sizeof(StepScalars) * 64) != hipSuccess) ok = false;
   m->host_cap = 64;
@@ -263,7 +263,7 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
       hipMemsetAsync(m->sig, 0, 6 * H * 4, m->stream) != hipSuccess ||
       hipMemsetAsync(m->tau, 0, 6 * H * 4, m->stream) != hipSuccess ||
       hipMemsetAsync(m->gam, 0, H * 4, m->stream) != hipSuccess ||
-      hipMemsetAsync(m->tables, 0, 17L * g.h3 * 4, m->stream) != hipSuccess ||
+      hipMemsetAsync(m->tables, 0, 19L * g.h3 * 4, m->stream) != hipSuccess ||
       hipMemsetAsync(m->ftab, 0, 4L * g.h3 * 8, m->stream) != hipSuccess) {
     free_model(m);
     return RELAX_ERR_HIP;
@@ -278,6 +278,7 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
     R.mech[k].gammadot0 = m->tables + (3 * k + 0) * h3;
     R.mech[k].power = m->tables + (3 * k + 1) * h3;
     R.mech[k].cohesion = m->tables + (3 * k + 2) * h3;
+    R.mech[k].gdp = m->tables + (17 + k) * h3;
     R.mech[k].active = 0;
   }
   R.fr.v0 = m->ftab;
@@ -319,6 +320,21 @@ int relax_model_synchronize(relax_model_t* m) {
   return RELAX_OK;
 }
 
+// prestress rate of each depth, gammadot0 (taup/mu)^(n-1)
+// (viscoelastic3d.f90:290), computed once in double
+static int update_gdp(relax_model_t* m, int k) {
+  const int h3 = m->g.h3;
+  if (m->hg0[k].empty()) return RELAX_OK;
+  std::vector<float> g(h3);
+  for (int i = 0; i < h3; ++i) {
+    const double taup = m->htaup.empty() ? 0.0 : m->htaup[i];
+    g[i] = (float)(m->hg0[k][i] * std::pow(taup / m->desc.mu, m->hn[k][i] - 1.0));
+  }
+  HIP_TRY(hipMemcpyAsync(m->tables + (17L + k) * h3, g.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  return RELAX_OK;
+}
+
 int relax_model_set_structure(relax_model_t* m, int which, const relax_layer_t* layers, int n) {
   if (!m || which < 0 || which > 2) return RELAX_ERR_ARG;
   const int h3 = m->g.h3;
@@ -348,7 +364,12 @@ int relax_model_set_structure(relax_model_t* m, int which, const relax_layer_t* 
     HIP_TRY(hipMemcpyAsync(base, a.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
     HIP_TRY(hipMemcpyAsync(base + h3, b.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
     HIP_TRY(hipMemcpyAsync(base + 2 * h3, c.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
+    m->hg0[which].assign(h3, 0.0);
+    m->hn[which].assign(h3, 1.0);
+    for (int i = 0; i < h3; ++i) { m->hg0[which][i] = layers[i].gammadot0; m->hn[which][i] = layers[i].stressexponent; }
     m->R.mech[which].active = 1;
+    int rc = update_gdp(m, which);
+    if (rc) return rc;
   } else {
     HIP_TRY(hipMemcpyAsync(m->ftab, fd.data(), 4L * h3 * 8, hipMemcpyHostToDevice, m->stream));
     m->R.fr.active = m->R.fr.np > 0;
@@ -373,6 +394,11 @@ int relax_model_set_prestress(relax_model_t* m, const relax_tensor_layer_t* p, i
   float* spd = m->tables + 10L * h3;
   HIP_TRY(hipMemcpyAsync(spd, sp.data(), 6L * h3 * 4, hipMemcpyHostToDevice, m->stream));
   HIP_TRY(hipMemcpyAsync(m->tables + 16L * h3, taup.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
+  m->htaup.assign(taup.begin(), taup.end());
+  for (int k = 0; k < 2; ++k) {
+    int rc = update_gdp(m, k);
+    if (rc) return rc;
+  }
   HIP_TRY(hipStreamSynchronize(m->stream));
   return RELAX_OK;
 }

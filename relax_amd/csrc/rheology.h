@@ -52,8 +52,22 @@ __device__ __forceinline__ double d_shifted(int i, int n, double dx) {
   return (double)(i < n / 2 ? i : i - n) * dx;
 }
 
+// x^e for the stress exponents of the power law: exact forms for e = n-1 in
+// {0, 1, 2, 2.5} (n = 1, 2, 3, 3.5), otherwise exp2(e log2 x) on the
+// hardware transcendentals (relative error ~ |e log2 x| 2^-23).
+__device__ __forceinline__ float pow_exponent(float x, float e) {
+  if (e == 0.f) return 1.f;
+  if (e == 1.f) return x;
+  if (e == 2.f) return x * x;
+  if (e == 2.5f) return x * x * sqrtf(x);
+  if (e == 3.f) return x * x * x;
+  return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x));
+}
+
 // Viscous contribution of one mechanism; returns 1/gammadot (or +inf when
-// the point is skipped) and adds gammadot*tauc/mu to *grate.
+// the point is skipped) and adds gammadot*tauc/mu to *grate.  The prestress
+// rate gammadot0 (taup/mu)^(n-1) depends on depth only and comes from a
+// table (M.gdp).
 __device__ __forceinline__ float viscous_point(const Mechanism& M, const Prestress& P, float mu, int i3,
                                                const T6& sd, T6& m, float* grate) {
   const float g0 = M.gammadot0[i3];
@@ -64,8 +78,8 @@ __device__ __forceinline__ float viscous_point(const Mechanism& M, const Prestre
   const float tauc = fmaxf(0.f, tau - M.cohesion[i3]);
   if ((double)tauc <= 1.0e-20) return __builtin_inff();
   const float ex = M.power[i3] - 1.f;
-  const float gd = g0 * powf(tauc / mu, ex);
-  const float gdp = g0 * powf(P.taup[i3] / mu, ex);
+  const float gd = g0 * pow_exponent(tauc / mu, ex);
+  const float gdp = M.gdp[i3];
   const float a = 2.f * gd, b = 2.f * gdp;
   m.s11 += a * sd.s11 - b * sp[0];
   m.s12 += a * sd.s12 - b * sp[1];
@@ -131,7 +145,7 @@ __device__ __forceinline__ float friction_point(const RheoParams& R, const Plane
 // Moment-density rate of all mechanisms at one point (relax.f90:568-612).
 // tmin[3] (optional) receives the per-mechanism minima of the relaxation
 // time; grate (optional) the viscous strain rate summed over mechanisms.
-template <bool TM>
+template <bool TM, bool FRIC = true>
 __device__ __forceinline__ T6 rheology(const RheoParams& R, const T6& s, int i1, int i2, int i3, float* tmin,
                                        float* grate_lin, float* grate_nl, int* status) {
   T6 m{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -146,7 +160,7 @@ __device__ __forceinline__ T6 rheology(const RheoParams& R, const T6& s, int i1,
       if (TM) tmin[1] = fminf(tmin[1], t);
     }
   }
-  if (R.fr.active) {
+  if (FRIC && R.fr.active) {
     for (int k = 0; k < R.fr.np; ++k) {
       const float t = friction_point(R, R.fr.planes[k], i1, i2, i3, s, m, nullptr, status);
       if (TM) tmin[2] = fminf(tmin[2], t);

@@ -16,7 +16,7 @@
 #include "rheology.h"
 #include "stencil.h"
 
-template <int TXB, int TYB, bool TM, bool RAW>
+template <int TXB, int TYB, bool TM, bool RAW, bool FRIC>
 __global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, RheoParams R, StencilConst S,
                                                         const float* __restrict__ sig, float* __restrict__ W,
                                                         float* __restrict__ T, StepScalars* sc, int zc) {
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, RheoParams R, St
       const int j1 = (x0 + hx - 1 + n1) % n1, j2 = (y0 + hy - 1 + n2) % n2;
       const long o = j1 + (long)n1 * (j2 + (long)n2 * z);
       const T6 s = load_t6(sig, H, o);
-      const T6 m = RAW ? s : rheology<TM>(R, s, j1, j2, z, tmin, nullptr, nullptr, &status);
+      const T6 m = RAW ? s : rheology<TM, FRIC>(R, s, j1, j2, z, tmin, nullptr, nullptr, &status);
       slot[e] = m.s11; slot[HP + e] = m.s12; slot[2 * HP + e] = m.s13;
       slot[3 * HP + e] = m.s22; slot[4 * HP + e] = m.s23; slot[5 * HP + e] = m.s33;
     }
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, RheoParams R, St
   }
 }
 
-template <int MODE>
+template <int MODE, bool FRIC>
 __global__ __launch_bounds__(256) void k_stress(Grid g, RheoParams R, StencilConst S, const float* __restrict__ X,
                                                float* sig, float* tau, float* gam, StepScalars* sc,
                                                double2* partials) {
@@ -152,14 +152,14 @@ __global__ __launch_bounds__(256) void k_stress(Grid g, RheoParams R, StencilCon
       out = T6{-t.s11 + e.s11, -t.s12 + e.s12, -t.s13 + e.s13, -t.s22 + e.s22, -t.s23 + e.s23, -t.s33 + e.s33};
     } else if (MODE == STRESS_PRED) {
       const T6 s = load_t6(sig, H, o);
-      const T6 m = rheology<false>(R, s, i1, i2, i3, nullptr, nullptr, nullptr, &status);
+      const T6 m = rheology<false, FRIC>(R, s, i1, i2, i3, nullptr, nullptr, nullptr, &status);
       const float c = -(float)(sc->Dt / 2);
       out = T6{(c * m.s11 - t.s11) + e.s11, (c * m.s12 - t.s12) + e.s12, (c * m.s13 - t.s13) + e.s13,
                (c * m.s22 - t.s22) + e.s22, (c * m.s23 - t.s23) + e.s23, (c * m.s33 - t.s33) + e.s33};
     } else {
       const T6 s = load_t6(sig, H, o);
       float gl = 0.f, gn = 0.f;
-      const T6 m = rheology<false>(R, s, i1, i2, i3, nullptr, &gl, &gn, &status);
+      const T6 m = rheology<false, FRIC>(R, s, i1, i2, i3, nullptr, &gl, &gn, &status);
       const float dt = (float)sc->Dt;
       const T6 tn{t.s11 + dt * m.s11, t.s12 + dt * m.s12, t.s13 + dt * m.s13,
                   t.s22 + dt * m.s22, t.s23 + dt * m.s23, t.s33 + dt * m.s33};
@@ -195,18 +195,21 @@ __global__ __launch_bounds__(256) void k_stress(Grid g, RheoParams R, StencilCon
 
 int launch_bodyforce(const Grid& g, const RheoParams& R, const StencilConst& S, const float* sig, float* W,
                      float* T, StepScalars* sc, int mode, hipStream_t st) {
-  const int zc = 8;
+  const int zc = 16;
   const unsigned gz = (unsigned)((g.h3 + zc - 1) / zc);
-#define L(TX, TY)                                                                                        \
+#define LF(TX, TY, F)                                                                                    \
   {                                                                                                      \
     const dim3 grid(g.n1 / TX, g.n2 / TY, gz);                                                           \
-    if (mode == BODY_PRED) k_bodyforce<TX, TY, true, false><<<grid, 256, 0, st>>>(g, R, S, sig, W, T, sc, zc);  \
-    else if (mode == BODY_CORR) k_bodyforce<TX, TY, false, false><<<grid, 256, 0, st>>>(g, R, S, sig, W, T, sc, zc); \
-    else k_bodyforce<TX, TY, false, true><<<grid, 256, 0, st>>>(g, R, S, sig, W, T, sc, zc);            \
+    if (mode == BODY_PRED) k_bodyforce<TX, TY, true, false, F><<<grid, TX * TY, 0, st>>>(g, R, S, sig, W, T, sc, zc); \
+    else if (mode == BODY_CORR) k_bodyforce<TX, TY, false, false, F><<<grid, TX * TY, 0, st>>>(g, R, S, sig, W, T, sc, zc); \
+    else k_bodyforce<TX, TY, false, true, false><<<grid, TX * TY, 0, st>>>(g, R, S, sig, W, T, sc, zc);   \
   }
-  if (g.n1 % 64 == 0 && g.n2 % 4 == 0) L(64, 4)
+#define L(TX, TY) { if (fric) LF(TX, TY, true) else LF(TX, TY, false) }
+  const bool fric = R.fr.active && R.fr.np > 0;
+  if (g.n1 % 32 == 0 && g.n2 % 16 == 0) L(32, 16)
   else if (g.n1 % 32 == 0 && g.n2 % 8 == 0) L(32, 8)
   else return RELAX_ERR_SIZE;
+#undef LF
 #undef L
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }
@@ -216,8 +219,11 @@ long stress_blocks(const Grid& g) { return (g.H + 255) / 256; }
 int launch_stress(int mode, const Grid& g, const RheoParams& R, const StencilConst& S, const float* X, float* sig,
                   float* tau, float* gam, StepScalars* sc, double2* partials, hipStream_t st) {
   const unsigned nb = (unsigned)stress_blocks(g);
-  if (mode == STRESS_SET) k_stress<STRESS_SET><<<nb, 256, 0, st>>>(g, R, S, X, sig, tau, gam, sc, partials);
-  else if (mode == STRESS_PRED) k_stress<STRESS_PRED><<<nb, 256, 0, st>>>(g, R, S, X, sig, tau, gam, sc, partials);
-  else k_stress<STRESS_CORR><<<nb, 256, 0, st>>>(g, R, S, X, sig, tau, gam, sc, partials);
+  const bool fric = R.fr.active && R.fr.np > 0;
+#define K(MODE, F) k_stress<MODE, F><<<nb, 256, 0, st>>>(g, R, S, X, sig, tau, gam, sc, partials)
+  if (mode == STRESS_SET) K(STRESS_SET, false);
+  else if (mode == STRESS_PRED) { if (fric) K(STRESS_PRED, true); else K(STRESS_PRED, false); }
+  else { if (fric) K(STRESS_CORR, true); else K(STRESS_CORR, false); }
+#undef K
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }

@@ -12,9 +12,12 @@ Parity status per function:
   fft3 / fft2 ............... pinned (numpy.fft, delta closed form)
   elasticresponse ........... pinned (Navier operator inverts it exactly)
   surfacetractioncowling .... pinned (space-domain Hooke's law route)
-  cerrutimodified ........... PARTIALLY pinned (k1=k2=0 case, mirror
-                              symmetry, linearity; the closed form itself has
-                              no exact independent identity -- see DESIGN.md)
+  cerrutimodified ........... pinned (its spectrum is the continuous Fourier
+                              transform of a depth profile (A + B z) e^(-beta z)
+                              that solves the homogeneous Navier equation and
+                              carries the applied traction, Cowling buoyancy
+                              included, at z = 0+; plus k1=k2=0, mirror
+                              symmetry, linearity)
   greenfunctioncowling ...... pinned through its parts + interior Navier
                               equation + free-surface traction reduction
   stressupdate / equivalentbodyforce ... pinned (FIR closed forms)
@@ -216,6 +219,62 @@ def test_cerruti_mirror_symmetry_x2():
     Ur = _cerruti(Pr, 16, 1.0, 0.02)
     for c, s in ((0, 1), (1, -1), (2, 1)):
         np.testing.assert_allclose(Ur[c], s * U[c][:, idx, :], rtol=1e-12, atol=1e-20)
+
+
+def _cerruti_surface_solution(P, gam, i1=1, i2=2, sx1=8, sx2=8, sx3=4096, dx=1.0, dx3=0.01):
+    """Depth profile of the correction for one horizontal wavenumber.
+
+    cerrutimodified returns samples of the continuous Fourier transform over
+    x3 (the depth z) of a profile that, for z > 0, must be (A + B z) e^(-beta z)
+    with beta = 2 pi |k_h|: the decaying solutions of the homogeneous Navier
+    equation.  The profile is recovered by an inverse DFT on a fine, long
+    depth grid and fitted by that family; returns (A, B, fit residual, beta,
+    a1, a2) with a_j = 2 pi k_j."""
+    p = [np.zeros((sx2, sx1 + 2)) for _ in range(3)]
+    for c in range(3):
+        p[c][i2, 2 * i1], p[c][i2, 2 * i1 + 1] = P[c].real, P[c].imag
+    u = [np.zeros((sx3, sx2, sx1 + 2)) for _ in range(3)]
+    O.cerrutimodified(p[0], p[1], p[2], LAM, MU, gam, u[0], u[1], u[2], dx, dx, dx3)
+    prof = [np.fft.ifft(a[:, i2, 2 * i1] + 1j * a[:, i2, 2 * i1 + 1]) / dx3 for a in u]
+    a1, a2 = 2 * np.pi * i1 / (sx1 * dx), 2 * np.pi * i2 / (sx2 * dx)
+    beta = np.hypot(a1, a2)
+    z = np.arange(sx3) * dx3
+    sel = (z > 0.5) & (z < 5.0)
+    M = np.stack([np.exp(-beta * z[sel]), z[sel] * np.exp(-beta * z[sel])], 1)
+    AB = [np.linalg.lstsq(M, c[sel], rcond=None)[0] for c in prof]
+    res = max(np.abs(M @ AB[c] - prof[c][sel]).max() / np.abs(prof[c][sel]).max() for c in range(3))
+    A = np.array([x[0] for x in AB])
+    B = np.array([x[1] for x in AB])
+    return A, B, res, beta, a1, a2
+
+
+@pytest.mark.parametrize("mut,gam", [(0, 0.0), (0, 0.05), (5, 0.05), (6, 0.0), (7, 0.0)])
+def test_cerruti_solves_the_surface_traction_problem(mut, gam, mutation):
+    """Boussinesq-Cerruti check: the corrected displacement obeys
+    mu (u'' - beta^2 u) + (lam+mu) grad div u = 0 for z > 0, and its traction
+    at z = 0+, with the buoyancy term -2 mu alpha gamma u3 of the surface
+    traction (green.f90:327-377), equals the applied p."""
+    P = np.array([0.3 + 0.2j, -0.7 + 0.1j, 0.5 - 0.4j])
+    mutation(mut)
+    A, B, res, beta, a1, a2 = _cerruti_surface_solution(P, gam)
+    mutation(0)
+    assert res < 1e-5  # the profile is of the form (A + B z) e^(-beta z)
+    alpha = (LAM + MU) / (LAM + 2 * MU)
+    d = B - beta * A  # u'(0+)
+    t = np.array([MU * (d[0] + 1j * a1 * A[2]), MU * (d[1] + 1j * a2 * A[2]),
+                  (LAM + 2 * MU) * d[2] + LAM * 1j * (a1 * A[0] + a2 * A[1]) - 2 * MU * alpha * gam * A[2]])
+    # Navier operator on (A + B z) e^(-beta z): e^(-beta z) and z e^(-beta z) parts
+    ia = np.array([1j * a1, 1j * a2])
+    divA, divB = ia @ A[:2] + d[2], ia @ B[:2] - beta * B[2]
+    nav0 = -2 * beta * MU * B + (LAM + MU) * np.array([1j * a1 * divA, 1j * a2 * divA, divB - beta * divA])
+    nav1 = (LAM + MU) * np.array([1j * a1 * divB, 1j * a2 * divB, -beta * divB])
+    nav = max(np.abs(nav0).max(), np.abs(nav1).max()) / (MU * beta * np.abs(A).max())
+    err = np.abs(t - P).max() / np.abs(P).max()
+    if mut == 0:
+        assert nav < 1e-5
+        assert err < 1e-4
+    else:
+        assert err > 1e-2
 
 
 def test_cerruti_linear_in_traction():
