@@ -31,6 +31,18 @@ __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y
 __device__ __forceinline__ float2 mul_i(float2 a) { return make_float2(-a.y, a.x); }
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 
+// ------------------------------------------------------ workgroup order ----
+// The dispatcher hands workgroup b to XCD b % 8.  Tiles that share cache
+// lines or halo rows should meet in one XCD's L2, so workgroup b takes tile
+// (b % 8) * q + b / 8: each XCD walks a contiguous range of q tiles (the
+// last nb % 8 tiles keep their own index).
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  constexpr int NX = 8;
+  const int q = nb / NX;
+  if (b >= q * NX) return b;
+  return (b % NX) * q + b / NX;
+}
+
 // --------------------------------------------------------------- grid info --
 // Wavenumber in cycles per unit length (fourier.f90:64-81), 0-based index.
 __host__ __device__ __forceinline__ float wavenumber23f(int i, int n, float inv_ndx) {

@@ -51,8 +51,10 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(const float* __restrict__ in, 
   constexpr int RPB = (2048 / M) > 0 ? 2048 / M : 1;
   constexpr int PM = M + 1;  // row pitch in LDS (breaks bank aliasing between rows)
   __shared__ float2 s[RPB * PM];
+  __shared__ float2 stw[N1];  // twiddles in LDS: off the global-latency path of every stage
   const int tid = threadIdx.x;
   const long row0 = (long)blockIdx.x * RPB;
+  for (int e = tid; e < N1; e += NT) stw[e] = tw[e];
   for (int e = tid; e < RPB * M; e += NT) {
     const int b = e / M, j = e % M;
     const long row = row0 + b;
@@ -61,14 +63,14 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(const float* __restrict__ in, 
     s[b * PM + j] = v;
   }
   __syncthreads();
-  lds_fft<M, RPB, NT, RPB, PM, 0, 1, -1, 2, false>(s, tw, tid);
+  lds_fft<M, RPB, NT, RPB, PM, 0, 1, -1, 2, false>(s, stw, tid);
   for (int e = tid; e < RPB * M; e += NT) {
     const int b = e / M, k = e % M;
     const long row = row0 + b;
     if (row >= nrows) continue;
     const float2 zk = s[b * PM + k];
     const float2 zr = cconj(s[b * PM + ((M - k) & (M - 1))]);
-    const float2 w = tw[k];
+    const float2 w = stw[k];
     const float2 sum = cadd(zk, zr), dif = csub(zk, zr);
     // X[k] = (Z[k] + conj Z[M-k])/2 - i w^k (Z[k] - conj Z[M-k])/2
     const float2 wd = mul_mi(cmul(w, dif));
@@ -93,8 +95,10 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(const float2* in, long in_pitc
   constexpr int EPT = (RPB * M + NT - 1) / NT;
   __shared__ float2 s[RPB * PM];
   __shared__ float2 xm[RPB];
+  __shared__ float2 stw[N1];
   const int tid = threadIdx.x;
   const long row0 = (long)blockIdx.x * RPB;
+  for (int e = tid; e < N1; e += NT) stw[e] = tw[e];
   for (int e = tid; e < RPB * M; e += NT) {
     const int b = e / M, j = e % M;
     const long row = row0 + b;
@@ -116,7 +120,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(const float2* in, long in_pitc
     const int b = e / M, k = e % M;
     const float2 xk = s[b * PM + k];
     const float2 xr = cconj(k == 0 ? xm[b] : s[b * PM + (M - k)]);
-    const float2 w = cconj(tw[k]);  // exp(+2 pi i k / N1)
+    const float2 w = cconj(stw[k]);  // exp(+2 pi i k / N1)
     // Z'[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k])
     z[i] = cadd(cadd(xk, xr), mul_i(cmul(w, csub(xk, xr))));
   }
@@ -128,7 +132,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(const float2* in, long in_pitc
     s[(e / M) * PM + (e % M)] = z[i];
   }
   __syncthreads();
-  lds_fft<M, RPB, NT, RPB, PM, 0, 1, +1, 2, false>(s, tw, tid);
+  lds_fft<M, RPB, NT, RPB, PM, 0, 1, +1, 2, false>(s, stw, tid);
   float coef = 1.f;
   if (EPI == EPI_PRED) coef = (float)(sc->Dt / 2);
   if (EPI == EPI_CORR) coef = (float)sc->Dt;
@@ -148,9 +152,14 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(const float2* in, long in_pitc
 }
 
 // --------------------------------------------------------- strided axis ----
+// Columns per workgroup of the strided transforms (compile-time tile choice;
+// RELAX_COLS_TX512 overrides the 512-point tile).
+#ifndef RELAX_COLS_TX512
+#define RELAX_COLS_TX512 8
+#endif
 template <int L>
 struct ColTile {
-  static constexpr int TX = L >= 1024 ? 8 : (L >= 128 ? 16 : (2048 / L > 64 ? 64 : 2048 / L));
+  static constexpr int TX = L >= 1024 ? 8 : (L == 512 ? RELAX_COLS_TX512 : (L >= 128 ? 16 : (2048 / L > 64 ? 64 : 2048 / L)));
 };
 
 template <int L, int NT, int SIGN>
@@ -158,16 +167,20 @@ __global__ __launch_bounds__(NT) void k_fft_cols(float2* data, long jstride, lon
                                                  const float2* __restrict__ tw) {
   constexpr int TX = ColTile<L>::TX;
   __shared__ float2 s[L * TX];
+  __shared__ float2 stw[L];
   const int tid = threadIdx.x;
-  const int col0 = blockIdx.x * TX;
-  float2* base = data + (long)blockIdx.y * pstride;
+  for (int e = tid; e < L; e += NT) stw[e] = tw[e];
+  const int nchunk = (ncols + TX - 1) / TX;
+  const int bid = xcd_tile(blockIdx.x, gridDim.x);
+  const int col0 = (bid % nchunk) * TX;
+  float2* base = data + (long)(bid / nchunk) * pstride;
   for (int e = tid; e < L * TX; e += NT) {
     const int b = e % TX, j = e / TX;
     const int col = col0 + b;
     s[e] = col < ncols ? base[j * jstride + col] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  lds_fft<L, TX, NT, TX, 1, 0, TX, SIGN, 1, true>(s, tw, tid);
+  lds_fft<L, TX, NT, TX, 1, 0, TX, SIGN, 1, true>(s, stw, tid);
   for (int e = tid; e < L * TX; e += NT) {
     const int b = e % TX, j = e / TX;
     const int col = col0 + b;
@@ -182,7 +195,9 @@ __global__ __launch_bounds__(NT) void k_fft_rows(float2* data, long pitch, long 
   constexpr int RPB = (2048 / L) > 0 ? 2048 / L : 1;
   constexpr int PL = L + 1;
   __shared__ float2 s[RPB * PL];
+  __shared__ float2 stw[L];
   const int tid = threadIdx.x;
+  for (int e = tid; e < L; e += NT) stw[e] = tw[e];
   const long row0 = (long)blockIdx.x * RPB;
   for (int e = tid; e < RPB * L; e += NT) {
     const int b = e / L, j = e % L;
@@ -190,7 +205,7 @@ __global__ __launch_bounds__(NT) void k_fft_rows(float2* data, long pitch, long 
     s[b * PL + j] = row < nrows ? data[row * pitch + j] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  lds_fft<L, RPB, NT, RPB, PL, 0, 1, SIGN, 1, false>(s, tw, tid);
+  lds_fft<L, RPB, NT, RPB, PL, 0, 1, SIGN, 1, false>(s, stw, tid);
   for (int e = tid; e < RPB * L; e += NT) {
     const int b = e / L, j = e % L;
     const long row = row0 + b;
@@ -246,7 +261,7 @@ int launch_fft_cols(int L, int sign, float2* data, long jstride, long pstride, i
 #undef T
     default: return RELAX_ERR_SIZE;
   }
-  const dim3 grid((unsigned)((ncols + tx - 1) / tx), (unsigned)nplanes);
+  const dim3 grid((unsigned)(((ncols + tx - 1) / tx) * (long)nplanes));
 #define X(N, S) case N: k_fft_cols<N, NT_FFT, S><<<grid, NT_FFT, 0, st>>>(data, jstride, pstride, ncols, tw); break;
   if (sign < 0) {
     switch (L) { X(16, -1) X(32, -1) X(64, -1) X(128, -1) X(256, -1) X(512, -1) X(1024, -1) default: return RELAX_ERR_SIZE; }

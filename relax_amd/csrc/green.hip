@@ -144,16 +144,23 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
                                                       const float2* __restrict__ TM,
                                                       const float2* __restrict__ TN,
                                                       const float2* __restrict__ tw3) {
-  static_assert(NT % TX == 0, "a thread keeps one column");
-  constexpr int NE = 3 * H3 * TX;  // input elements
-  __shared__ float2 s[2 * NE];     // [c][half][m][tx]
-  __shared__ float2 red[3][NT];
+  static_assert(NT % 64 == 0 && 64 % TX == 0, "a thread keeps one column; whole waves");
+  constexpr int NW = NT / 64;
+  // LDS: six sequences (component c, half) of H3 x TX complex.  The stride
+  // between sequences is padded by 12 float2 so that the lanes of a wave,
+  // which walk batches across sequences in the FFT stages, spread over the
+  // LDS banks instead of landing on the same 16.
+  constexpr int SB = H3 * TX + 12;
+  __shared__ float2 s[6 * SB];
+  __shared__ float2 red[3][NW][TX];
   __shared__ float2 pv[3][TX];
+  __shared__ float2 stw[2 * H3];  // the n3-point twiddles, staged in LDS
   const int tid = threadIdx.x;
   const int tx = tid % TX;
+  for (int e = tid; e < 2 * H3; e += NT) stw[e] = tw3[e];
   const int m1 = n1 / 2;
   const int nmain = (m1 / TX) * n2;
-  const int bid = blockIdx.x;
+  const int bid = xcd_tile(blockIdx.x, gridDim.x);
   float2* base;
   const float2* tbase;
   long cs, zs, tcs;
@@ -175,14 +182,14 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
   const float k2 = wavenumber23f(k2i, n2, C.ik2);
 
   // load the upper half and its w^i3-shifted copy
-  for (int e = tid; e < NE; e += NT) {
+  for (int e = tid; e < 3 * H3 * TX; e += NT) {
     const int t = e % TX, z = (e / TX) % H3, c = e / (TX * H3);
     const float2 x = base[c * cs + z * zs + t];
-    s[((2 * c) * H3 + z) * TX + t] = x;
-    s[((2 * c + 1) * H3 + z) * TX + t] = cmul(x, tw3[z]);
+    s[(2 * c) * SB + z * TX + t] = x;
+    s[(2 * c + 1) * SB + z * TX + t] = cmul(x, tw3[z]);
   }
   __syncthreads();
-  lds_fft<H3, 6 * TX, NT, TX, 1, H3 * TX, TX, -1, 2, true>(s, tw3, tid);
+  lds_fft<H3, 6 * TX, NT, TX, 1, SB, TX, -1, 2, true>(s, stw, tid);
 
   // Green's function and the traction partial sums of this thread's column
   float2 ps1 = make_float2(0.f, 0.f), ps2 = ps1, ps3 = ps1;
@@ -190,9 +197,9 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
     const int slot = e / TX, half = slot / H3, m = slot % H3;
     const int k3i = 2 * m + half;
     const float k3 = wavenumber23f(k3i, n3, C.ik3);
-    float2& c1 = s[((0 + half) * H3 + m) * TX + tx];
-    float2& c2 = s[((2 + half) * H3 + m) * TX + tx];
-    float2& c3 = s[((4 + half) * H3 + m) * TX + tx];
+    float2& c1 = s[(0 + half) * SB + m * TX + tx];
+    float2& c2 = s[(2 + half) * SB + m * TX + tx];
+    float2& c3 = s[(4 + half) * SB + m * TX + tx];
     float2 a1 = c1, a2 = c2, a3 = c3;
     if (k1i == 0 && k2i == 0 && k3i == 0) {
       a1 = a2 = a3 = make_float2(0.f, 0.f);
@@ -204,12 +211,22 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
     traction_point(C, k1, k2, k3, a1, a2, a3, s1, s2, s3);
     ps1 = cadd(ps1, s1); ps2 = cadd(ps2, s2); ps3 = cadd(ps3, s3);
   }
-  red[0][tid] = ps1; red[1][tid] = ps2; red[2][tid] = ps3;
+  // reduce over the lanes of the wave that hold the same column (fixed
+  // order: bitwise reproducible), then over the waves
+#pragma unroll
+  for (int off = 32; off >= TX; off >>= 1) {
+    ps1.x += __shfl_xor(ps1.x, off); ps1.y += __shfl_xor(ps1.y, off);
+    ps2.x += __shfl_xor(ps2.x, off); ps2.y += __shfl_xor(ps2.y, off);
+    ps3.x += __shfl_xor(ps3.x, off); ps3.y += __shfl_xor(ps3.y, off);
+  }
+  if ((tid & 63) < TX) {
+    red[0][tid >> 6][tx] = ps1; red[1][tid >> 6][tx] = ps2; red[2][tid >> 6][tx] = ps3;
+  }
   __syncthreads();
   if (tid < 3 * TX) {
     const int c = tid / TX, t = tid % TX;
-    float2 acc = make_float2(0.f, 0.f);
-    for (int j = 0; j < NT / TX; ++j) acc = cadd(acc, red[c][t + j * TX]);
+    float2 acc = red[c][0][t];
+    for (int w = 1; w < NW; ++w) acc = cadd(acc, red[c][w][t]);
     const float2 tt = tbase[c * tcs + t];
     // p = t - p / (sx3 dx3)   (green.f90:373-375, :1082-1084)
     pv[c][t] = make_float2(tt.x - acc.x / C.inv_n3dx3, tt.y - acc.y / C.inv_n3dx3);
@@ -221,22 +238,22 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
     for (int e = tid; e < 2 * H3 * TX; e += NT) {
       const int slot = e / TX, half = slot / H3, m = slot % H3;
       const float k3 = wavenumber23f(2 * m + half, n3, C.ik3);
-      float2& c1 = s[((0 + half) * H3 + m) * TX + tx];
-      float2& c2 = s[((2 + half) * H3 + m) * TX + tx];
-      float2& c3 = s[((4 + half) * H3 + m) * TX + tx];
+      float2& c1 = s[(0 + half) * SB + m * TX + tx];
+      float2& c2 = s[(2 + half) * SB + m * TX + tx];
+      float2& c3 = s[(4 + half) * SB + m * TX + tx];
       float2 a1 = c1, a2 = c2, a3 = c3;
       cerruti_point(C, cc, p1, p2, k3, a1, a2, a3);
       c1 = a1; c2 = a2; c3 = a3;
     }
   }
   __syncthreads();
-  lds_fft<H3, 6 * TX, NT, TX, 1, H3 * TX, TX, +1, 2, true>(s, tw3, tid);
+  lds_fft<H3, 6 * TX, NT, TX, 1, SB, TX, +1, 2, true>(s, stw, tid);
   // upper half of the inverse: y[n] = E'[n] + w^-n O'[n]
-  for (int e = tid; e < NE; e += NT) {
+  for (int e = tid; e < 3 * H3 * TX; e += NT) {
     const int t = e % TX, z = (e / TX) % H3, c = e / (TX * H3);
-    const float2 ev = s[((2 * c) * H3 + z) * TX + t];
-    const float2 od = s[((2 * c + 1) * H3 + z) * TX + t];
-    base[c * cs + z * zs + t] = cadd(ev, cmul(od, cconj(tw3[z])));
+    const float2 ev = s[(2 * c) * SB + z * TX + t];
+    const float2 od = s[(2 * c + 1) * SB + z * TX + t];
+    base[c * cs + z * zs + t] = cadd(ev, cmul(od, cconj(stw[z])));
   }
 }
 

@@ -23,7 +23,8 @@ struct relax_model {
   float2* TN = nullptr;    // 3 * sx2: Nyquist bins of the traction spectrum
   StepScalars* sc = nullptr;
   double2* partials = nullptr;
-  long nblk = 0;
+  long nblk = 0;  // stress-kernel workgroups (first-level amplitude partials)
+  long nred = 0;  // second-level partials
   float* tables = nullptr;  // per-depth rheology tables (17 * sx3/2 floats)
   double* ftab = nullptr;   // fault-creep tables: v0, (a-b)sigma, friction, cohesion
 

@@ -65,6 +65,32 @@ __global__ void k_finalize_dt(StepScalars* sc, TimeLine tl, int use_override, do
   sc->it = sc->it + 1;
 }
 
+// First level of the amplitude reduction: workgroup g sums the stress
+// kernel's partials [g*RED_CHUNK, (g+1)*RED_CHUNK) in a fixed order.
+static const int RED_CHUNK = 2048;
+__global__ __launch_bounds__(256) void k_reduce_partials(const double2* __restrict__ partials, long nb,
+                                                         double2* __restrict__ out) {
+  __shared__ double2 red[4];
+  const int tid = threadIdx.x;
+  const long b0 = (long)blockIdx.x * RED_CHUNK;
+  double2 a = make_double2(0.0, 0.0);
+  for (int j = 0; j < RED_CHUNK / 256; ++j) {
+    const long i = b0 + tid + 256L * j;
+    if (i < nb) { a.x += partials[i].x; a.y += partials[i].y; }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a.x += __shfl_xor(a.x, off);
+    a.y += __shfl_xor(a.y, off);
+  }
+  if ((tid & 63) == 0) red[tid >> 6] = a;
+  __syncthreads();
+  if (tid == 0) {
+    double2 r = red[0];
+    for (int w = 1; w < 4; ++w) { r.x += red[w].x; r.y += red[w].y; }
+    out[blockIdx.x] = r;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_end_step(StepScalars* sc, const double2* partials, long nb, double dv,
                                                   TimeLine tl) {
   __shared__ double2 red[256];
@@ -201,8 +227,8 @@ size_t relax_model_footprint(const relax_model_desc_t* d) {
   b += 19 * H * 4;                  // u 3, sig 6, tau 6, gamma 1, work 3
   b += 3 * n2 * h3 * 8;             // Nyquist planes of the work spectrum
   b += 3 * n1 * n2 * 4 + 3 * n2 * 8;  // traction planes and their Nyquist bins
-  b += ((H + 255) / 256) * 16;      // amplitude partials
-  b += 17 * h3 * 4 + 4096;          // tables and scalars
+  b += ((H + 255) / 256 + (H + 255) / 256 / 2048 + 1) * 16;  // amplitude partials, two levels
+  b += 19 * h3 * 4 + 4 * h3 * 8 + 4096;  // rheology and fault-creep tables, scalars
   return b;
 }
 
@@ -252,7 +278,8 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
   alloc((void**)&m->TN, 3L * g.n2 * 8);
   alloc((void**)&m->sc, sizeof(StepScalars));
   m->nblk = stress_blocks(g);
-  alloc((void**)&m->partials, m->nblk * sizeof(double2));
+  m->nred = (m->nblk + RED_CHUNK - 1) / RED_CHUNK;
+  alloc((void**)&m->partials, (m->nblk + m->nred) * sizeof(double2));
   alloc((void**)&m->tables, 19L * g.h3 * 4);
   alloc((void**)&m->ftab, 4L * g.h3 * 8);
   if (ok && hipHostMalloc((void**)&m->host_sc, sizeof(StepScalars) * 64) != hipSuccess) ok = false;
@@ -638,7 +665,8 @@ static int enqueue_step(relax_model_t* m, const double* dt_override) {
     k_frictionadd<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(m->patches_dev, n, m->sc, 0, 0.0);
     mark(m, RELAX_PROF_SMALL);
   }
-  k_end_step<<<1, 256, 0, st>>>(m->sc, m->partials, m->nblk, m->desc.dx1 * m->desc.dx2 * m->desc.dx3, tl);
+  k_reduce_partials<<<(unsigned)m->nred, 256, 0, st>>>(m->partials, m->nblk, m->partials + m->nblk);
+  k_end_step<<<1, 256, 0, st>>>(m->sc, m->partials + m->nblk, m->nred, m->desc.dx1 * m->desc.dx2 * m->desc.dx3, tl);
   mark(m, RELAX_PROF_SMALL);
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }

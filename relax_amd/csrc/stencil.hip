@@ -64,17 +64,41 @@ __global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, RheoParams R, St
     }
   };
 
+  // The tile points (with halo) of one plane that this thread evaluates.
+  // Their stresses for plane z+1 are loaded into registers while plane z is
+  // evaluated and the ring is synchronised, so the load latency overlaps work.
+  constexpr int KP = (HP + NT - 1) / NT;
+  int pj1[KP], pj2[KP];
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    const int e = tid + k * NT;
+    const int hx = e % HX, hy = e / HX;
+    pj1[k] = (x0 + hx - 1 + n1) % n1;
+    pj2[k] = (y0 + hy - 1 + n2) % n2;
+  }
+  T6 pre[KP];
+  auto load_plane = [&](int z) {
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      if (tid + k * NT < HP) pre[k] = load_t6(sig, H, pj1[k] + (long)n1 * (pj2[k] + (long)n2 * z));
+    }
+  };
   const int zs = max(z0 - 1, 0), ze = min(z1, h3 - 1);
+  load_plane(zs);
   for (int z = zs; z <= ze; ++z) {
     float* slot = &ring[z % 3][0][0];
-    for (int e = tid; e < HP; e += NT) {
-      const int hx = e % HX, hy = e / HX;
-      const int j1 = (x0 + hx - 1 + n1) % n1, j2 = (y0 + hy - 1 + n2) % n2;
-      const long o = j1 + (long)n1 * (j2 + (long)n2 * z);
-      const T6 s = load_t6(sig, H, o);
-      const T6 m = RAW ? s : rheology<TM, FRIC>(R, s, j1, j2, z, tmin, nullptr, nullptr, &status);
-      slot[e] = m.s11; slot[HP + e] = m.s12; slot[2 * HP + e] = m.s13;
-      slot[3 * HP + e] = m.s22; slot[4 * HP + e] = m.s23; slot[5 * HP + e] = m.s33;
+    T6 cur[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) cur[k] = pre[k];
+    if (z < ze) load_plane(z + 1);
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int e = tid + k * NT;
+      if (e < HP) {
+        const T6 m = RAW ? cur[k] : rheology<TM, FRIC>(R, cur[k], pj1[k], pj2[k], z, tmin, nullptr, nullptr, &status);
+        slot[e] = m.s11; slot[HP + e] = m.s12; slot[2 * HP + e] = m.s13;
+        slot[3 * HP + e] = m.s22; slot[4 * HP + e] = m.s23; slot[5 * HP + e] = m.s33;
+      }
     }
     __syncthreads();
     const int zo = z - 1;
