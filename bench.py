@@ -40,6 +40,15 @@ def kernel_bytes(sx1, sx2, sx3):
     }
 
 
+# HBM bytes per launch at 512x512x256 measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE, each scaled by its reading for a device copy of known size
+# (profiles/rk2_step_c1_512x512x256_hbm_traffic.md)
+MEASURED_TRAFFIC_C1 = {
+    "green_z": 811.8e6, "fft_x_fwd": 811.7e6, "fft_y": 805.3e6, "fft_x_inv": 1220.7e6,
+    "bodyforce": 3006.7e6, "stress_pred": 4032.0e6, "stress_corr": 5157.1e6,
+}
+
+
 def step_bytes_model(sx1, sx2, sx3):
     """Compulsory bytes of one RK2 step in the unfused accounting of the
     path (404 N + 192 N/sx1 + 96 sx1 sx2, N = sx1 sx2 sx3)."""
@@ -70,6 +79,93 @@ def cpu_baseline(spec, state, nthreads):
     return dt
 
 
+def batched_c3(nmodels=64, nsteps=10):
+    """64 independent 128^3 models (relax_amd.workloads c3<k>: seeded patches,
+    viscosity and stress exponent), 10 RK2 steps each: one batch of shared
+    launches replayed from a hipGraph, against the same models stepped back to
+    back through the single-model path (no host synchronisation between
+    steps).  A solve is the 10 steps of one model from its coseismic state."""
+    import numpy as np
+
+    from relax_amd import workloads
+    from relax_amd.model import Batch
+
+    specs = [workloads.spec("c3%d" % k) for k in range(nmodels)]
+    t0 = time.perf_counter()
+    single = [workloads.build(s) for s in specs]
+    setup = (time.perf_counter() - t0) / nmodels
+    batched = [workloads.build(s) for s in specs]
+    # warm-up of both paths on throw-away models of the same grid
+    w = workloads.build(specs[0])
+    w.step(info=False)
+    w.synchronize()
+    wb = Batch([workloads.build(specs[1]), workloads.build(specs[2])])
+    wb.step(info=False)
+    wb.synchronize()
+    for m in wb.models + [w]:
+        m.close()
+    wb.close()
+    t0 = time.perf_counter()
+    for m in single:
+        for _ in range(nsteps):
+            m.step(info=False)
+    for m in single:
+        m.synchronize()
+    t_single = time.perf_counter() - t0
+    b = Batch(batched)
+    b.step(info=False)  # capture (not timed); one step more than the single path
+    b.synchronize()
+    for m in single:  # bring the single path level
+        m.step(info=False)
+    for m in single:
+        m.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        b.step(info=False)
+    b.synchronize()
+    t_batch = time.perf_counter() - t0
+    same = True
+    for k in (0, nmodels // 3, nmodels - 1):
+        a, c = single[k].download(), batched[k].download()
+        same = same and all(np.array_equal(a[f], c[f]) for f in ("u1", "u2", "u3", "sig", "tau", "gamma"))
+    b.close()
+    for m in single + batched:
+        m.close()
+    return {"workload": "c3: %d models of 128^3, dx=1 km, %d RK2 steps each" % (nmodels, nsteps),
+            "solves_per_s": round(nmodels / t_batch, 2),
+            "single_path_solves_per_s": round(nmodels / t_single, 2),
+            "speedup": round(t_single / t_batch, 2),
+            "setup_s_per_model": round(setup, 4),
+            "identical_to_single_path": bool(same)}
+
+
+def serving_c4(nreq=60, rate=20.0):
+    """The serving stream (relax_amd.serve.request_stream: Poisson arrivals,
+    grids 128x128x64 / 256x256x128 / 512x512x256 in the ratio 6:3:1, 10-30
+    steps, 30 % with an afterslip plane).  Capacity: the whole stream queued
+    at once, continuous batching against the single-model path back to back.
+    Latency: arrivals replayed at their times, p50/p99 time-to-solution
+    (set-up, steps and the surface displacement of each request)."""
+    from relax_amd import serve
+
+    warm = serve.request_stream(6, seed=1, steps=(2, 3))
+    serve.run_back_to_back(warm, realtime=False)
+    serve.Server().run(warm, realtime=False)
+    reqs = serve.request_stream(nreq, rate=rate)
+    cap_single = serve.summarize(serve.run_back_to_back(reqs, realtime=False), "single-model path, back to back")
+    cap_served = serve.summarize(serve.Server().run(reqs, realtime=False), "continuous batching")
+    lat_single = serve.summarize(serve.run_back_to_back(reqs, realtime=True), "single-model path, back to back")
+    lat_served = serve.summarize(serve.Server().run(reqs, realtime=True), "continuous batching")
+    return {"workload": "c4: %d requests, Poisson %g/s, grids 128x128x64:256x256x128:512x512x256 = 6:3:1, "
+                        "10-30 steps, 30%% afterslip" % (nreq, rate),
+            "solves_per_s": cap_served["solves_per_s"],
+            "single_path_solves_per_s": cap_single["solves_per_s"],
+            "speedup": round(cap_served["solves_per_s"] / cap_single["solves_per_s"], 2),
+            "p50_s": lat_served["p50_s"], "p99_s": lat_served["p99_s"],
+            "single_path_p50_s": lat_single["p50_s"], "single_path_p99_s": lat_single["p99_s"],
+            "capacity": [cap_single, cap_served], "at_arrival_rate": [lat_single, lat_served]}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +173,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched 64 x 128^3 measurement")
+    ap.add_argument("--serving", action="store_true", help="also measure the serving stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the RK2 step of one model per GPU "
@@ -135,24 +233,35 @@ def main():
                                "below a 25 km lid, adaptive RK2" % (args.workload, sx1, sx2, sx3, spec["dx"]),
                    "grid": [sx1, sx2, sx3], "dt_last": info["Dt"], "t_last": info["t"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC_C1.get(dom) if args.workload == "c1" else None},
         "step_roofline": {"bytes_per_step_model": B_step, "frac": round(B_step * steps_per_s / 8e12, 4),
                           "fused_bytes_per_step": fused_bytes,
                           "fused_frac": round(fused_bytes * steps_per_s / 8e12, 4)},
         "kernels": kernels,
     }
-    if not args.no_cpu_baseline:
+    state = model.download() if not args.no_cpu_baseline else None
+    model.close()
+    if not args.no_batched:
+        print("# batched: 64 x 128^3 models, 10 steps each", file=sys.stderr, flush=True)
+        try:
+            out["batched"] = batched_c3()
+        except Exception as e:  # keep the headline line
+            out["batched"] = {"error": repr(e)}
+    if args.serving:
+        print("# serving stream", file=sys.stderr, flush=True)
+        try:
+            out["serving"] = serving_c4()
+        except Exception as e:
+            out["serving"] = {"error": repr(e)}
+    if state is not None:
         nthreads = min(16, os.cpu_count() or 1)
         print("# cpu baseline: oracle, one RK2 step of the same state, %d threads" % nthreads, file=sys.stderr,
               flush=True)
-        state = model.download()
-        model.close()
         dt = cpu_baseline(spec, state, nthreads)
         out["cpu_baseline"] = {"value": round(1.0 / dt, 5), "unit": "RK2 steps/s", "cores": nthreads, "kind": "port",
                                "sample": "one RK2 step of the %s state (%dx%dx%d), %.1f s" % (
                                    args.workload, sx1, sx2, sx3, dt)}
-    else:
-        model.close()
     print(json.dumps(out), flush=True)
 
 

@@ -59,6 +59,8 @@ typedef struct {                                                                
 const char* relax_hip_version(void);
 const char* relax_hip_status_string(int status);
 int relax_hip_init(int device); /* selects the device; optional */
+/* Free and total device memory of the current device (for admission). */
+int relax_hip_meminfo(size_t* free_bytes, size_t* total_bytes);
 
 /* ---- fourier.f90 --------------------------------------------------------- */
 /* fft3, fourier.f90:177-201 (FFTW) / :330-355 (ctfft): in-place R2C
@@ -213,8 +215,44 @@ int relax_model_coseismic(relax_model_t* m);
 #define RELAX_PROF_SMALL 7        /* traction planes, Nyquist planes, scalars */
 #define RELAX_PROF_KINDS 8
 int relax_model_profile(relax_model_t* m, int nsteps, double* ms, int* launches);
+/* The surface plane (x3 = 0) of u1..u3 only, each sx2 rows of sx1+2
+ * (align_data=1) or sx1 floats: the slice the reference's exportgrd writes
+ * at each output step for observation depth 0 (relax.f90:955,
+ * export.f90:917). */
+int relax_model_surface(relax_model_t* m, float* u1, float* u2, float* u3, int align_data);
 /* The model's HIP stream (hipStream_t) for callers that order their own work. */
 void* relax_model_stream(relax_model_t* m);
+
+/* ---- batches: many models of one grid stepped by shared launches -------- */
+/* A batch steps all of its models together: every kernel of the RK2 step is
+ * one launch over all members (the model number is part of the launch grid),
+ * and the step is replayed from a captured hipGraph while the membership is
+ * unchanged.  Members must share the device, the grid (sx1, sx2, sx3), the
+ * spacing (dx1, dx2, dx3) and lambda, mu, gamma; rheology, prestress, fault
+ * planes, time line and state are per model.  There is no counterpart in the
+ * reference, whose driver runs one model per process (relax.f90:185-1454);
+ * each member's step is the reference's RK2 step of that model. */
+typedef struct relax_batch relax_batch_t;
+int relax_batch_create(relax_batch_t** out);
+/* Members leave the batch when it is destroyed; the models themselves stay. */
+int relax_batch_destroy(relax_batch_t* b);
+/* RELAX_ERR_ARG if the model is already in a batch or does not match the
+ * members' grid, spacing and elastic constants. */
+int relax_batch_add(relax_batch_t* b, relax_model_t* m);
+int relax_batch_remove(relax_batch_t* b, relax_model_t* m);
+int relax_batch_size(const relax_batch_t* b);
+/* One RK2 step of every member.  infos (NULL, or one entry per member in the
+ * order of addition) receives each member's scalars; the call then waits for
+ * the step and returns the first non-zero member status.  Without infos it
+ * returns after enqueueing.  While a model is a member, its own step and run
+ * calls are refused (RELAX_ERR_ARG); its download, get_patches and
+ * synchronize calls first wait for the batch. */
+int relax_batch_step(relax_batch_t* b, relax_step_info_t* infos);
+int relax_batch_synchronize(relax_batch_t* b);
+/* 1 while the batch still has queued work, 0 when it is idle (no waiting). */
+int relax_batch_busy(relax_batch_t* b);
+/* Each member's current scalars (waits for the queued work). */
+int relax_batch_scalars(relax_batch_t* b, relax_step_info_t* infos);
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,17 @@ SIGNATURES = {
     "relax_model_add_shear": (c_int, [P, P, P, c_int]),
     "relax_model_coseismic": (c_int, [P]),
     "relax_model_profile": (c_int, [P, c_int, P, P]),
+    "relax_hip_meminfo": (c_int, [P, P]),
+    "relax_batch_create": (c_int, [P]),
+    "relax_batch_destroy": (c_int, [P]),
+    "relax_batch_add": (c_int, [P, P]),
+    "relax_batch_remove": (c_int, [P, P]),
+    "relax_batch_size": (c_int, [P]),
+    "relax_batch_step": (c_int, [P, P]),
+    "relax_batch_synchronize": (c_int, [P]),
+    "relax_batch_busy": (c_int, [P]),
+    "relax_batch_scalars": (c_int, [P, P]),
+    "relax_model_surface": (c_int, [P, P, P, P, c_int]),
 }
 
 _LIB = None

@@ -40,6 +40,14 @@ const char* relax_hip_status_string(int s) {
   }
 }
 
+int relax_hip_meminfo(size_t* free_bytes, size_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return RELAX_ERR_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return RELAX_ERR_NODEVICE;
+  HIP_TRY(hipMemGetInfo(free_bytes, total_bytes));
+  return RELAX_OK;
+}
+
 int relax_hip_init(int device) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return RELAX_ERR_NODEVICE;

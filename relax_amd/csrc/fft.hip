@@ -43,75 +43,169 @@ const float2* twiddle_table(int L) {
 }
 
 // ---------------------------------------------------------------- x R2C ----
-template <int N1, int NT>
-__global__ __launch_bounds__(NT) void k_fft_x_r2c(const float* __restrict__ in, long in_pitch, float2* out,
-                                                  long out_pitch, float2* nyq, long nyq_stride, long nrows,
-                                                  float scale, const float2* __restrict__ tw) {
+// With YS (the split y transform of the model path), the rows of a plane of
+// n2 rows are taken in groups of 8, rows j1 + L2*j2 (L2 = n2/8), and after
+// the x transform each column of a group gets the first stage of the y
+// transform: an 8-point DFT over j2 and the twiddle w_n2^(j1*k2), written
+// to row j1 + L2*k2.  The L2-point transforms of the blocks of L2 rows then
+// finish the y transform (k_fft_cols), leaving y frequency 8*k1 + k2 at row
+// L2*k2 + k1 of the plane.  The Nyquist column keeps natural row order.
+template <int N1, bool YS>
+struct XRows {
+  static constexpr int M = N1 / 2;
+  static constexpr int RPB0 = (2048 / M) > 0 ? 2048 / M : 1;
+  static constexpr int RPB = YS ? (RPB0 < 8 ? 8 : RPB0) : RPB0;  // rows per workgroup
+  static constexpr int G = YS ? RPB / 8 : 1;                       // groups of 8 rows
+};
+
+template <int N1, int NT, bool YS>
+__global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long in_pitch, MPtr<float2> out_m,
+                                                  long out_pitch, MPtr<float2> nyq_m, long nyq_stride, long nrows,
+                                                  float scale, const float2* __restrict__ tw, int n2,
+                                                  const float2* __restrict__ twy) {
+  // in and out may be the same buffer (rows are staged in LDS before any write)
+  const float* in = in_m.at(blockIdx.y);
+  float2* out = out_m.at(blockIdx.y);
+  float2* __restrict__ nyq = nyq_m.at(blockIdx.y);
   constexpr int M = N1 / 2;
-  constexpr int RPB = (2048 / M) > 0 ? 2048 / M : 1;
+  constexpr int RPB = XRows<N1, YS>::RPB, G = XRows<N1, YS>::G;
   constexpr int PM = M + 1;  // row pitch in LDS (breaks bank aliasing between rows)
+  constexpr int EPT = (RPB * M + NT - 1) / NT;
   __shared__ float2 s[RPB * PM];
   __shared__ float2 stw[N1];  // twiddles in LDS: off the global-latency path of every stage
+  __shared__ float2 ytw[G][8];
   const int tid = threadIdx.x;
-  const long row0 = (long)blockIdx.x * RPB;
+  const int L2 = n2 / 8;
+  // global row of LDS row b
+  auto rowof = [&](int b) -> long {
+    if (!YS) return (long)blockIdx.x * RPB + b;
+    const long q = (long)blockIdx.x * G + b / 8;
+    return (q / L2) * n2 + (q % L2) + (long)L2 * (b % 8);
+  };
   for (int e = tid; e < N1; e += NT) stw[e] = tw[e];
+  if (YS && tid < G * 8) {
+    const int j1 = (int)(((long)blockIdx.x * G + tid / 8) % L2);
+    ytw[tid / 8][tid % 8] = twy[(j1 * (tid % 8)) % n2];
+  }
   for (int e = tid; e < RPB * M; e += NT) {
     const int b = e / M, j = e % M;
-    const long row = row0 + b;
+    const long row = rowof(b);
     float2 v = make_float2(0.f, 0.f);
     if (row < nrows) v = *reinterpret_cast<const float2*>(in + row * in_pitch + 2 * j);
     s[b * PM + j] = v;
   }
   __syncthreads();
   lds_fft<M, RPB, NT, RPB, PM, 0, 1, -1, 2, false>(s, stw, tid);
-  for (int e = tid; e < RPB * M; e += NT) {
+  float2 X[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) {
+    const int e = tid + i * NT;
+    if (e >= RPB * M) break;
     const int b = e / M, k = e % M;
-    const long row = row0 + b;
-    if (row >= nrows) continue;
     const float2 zk = s[b * PM + k];
     const float2 zr = cconj(s[b * PM + ((M - k) & (M - 1))]);
     const float2 w = stw[k];
     const float2 sum = cadd(zk, zr), dif = csub(zk, zr);
     // X[k] = (Z[k] + conj Z[M-k])/2 - i w^k (Z[k] - conj Z[M-k])/2
     const float2 wd = mul_mi(cmul(w, dif));
-    const float2 X = make_float2(0.5f * (sum.x + wd.x) * scale, 0.5f * (sum.y + wd.y) * scale);
-    out[row * out_pitch + k] = X;
-    if (k == 0) {
-      const float2 z0 = s[b * PM];
-      nyq[row * nyq_stride] = make_float2((z0.x - z0.y) * scale, 0.f);
-    }
+    X[i] = make_float2(0.5f * (sum.x + wd.x) * scale, 0.5f * (sum.y + wd.y) * scale);
+    const long row = rowof(b);
+    if (k == 0 && row < nrows) nyq[row * nyq_stride] = make_float2((zk.x - zk.y) * scale, 0.f);
+    if (!YS && row < nrows) out[row * out_pitch + k] = X[i];
+  }
+  if (!YS) return;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) {
+    const int e = tid + i * NT;
+    if (e >= RPB * M) break;
+    s[(e / M) * PM + (e % M)] = X[i];
+  }
+  __syncthreads();
+  // first stage of the y transform, one column of one group per item
+  for (int e = tid; e < G * M; e += NT) {
+    const int gi = e / M, k = e % M;
+    const long q = (long)blockIdx.x * G + gi;
+    if (q * 8 >= nrows) continue;
+    float2 a[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) a[t] = s[(8 * gi + t) * PM + k];
+    dft8<-1>(a);
+    const long base_row = (q / L2) * n2 + (q % L2);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) out[(base_row + (long)L2 * t) * out_pitch + k] = t ? cmul(a[t], ytw[gi][t]) : a[t];
   }
 }
 
 // ---------------------------------------------------------------- x C2R ----
-template <int N1, int NT, int EPI>
-__global__ __launch_bounds__(NT) void k_fft_x_c2r(const float2* in, long in_pitch, const float2* nyq,
-                                                  long nyq_stride, float* out, long out_pitch, long nrows,
-                                                  float scale, const float2* __restrict__ tw, const float* base,
-                                                  long base_pitch, const StepScalars* __restrict__ sc) {
+// With YS the workgroup first undoes the 8-point y stage of its row groups
+// (the blocks' L2-point inverse transforms ran before, see k_fft_x_r2c).
+template <int N1, int NT, int EPI, bool YS>
+__global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long in_pitch, MPtr<const float2> nyq_m,
+                                                  long nyq_stride, MPtr<float> out_m, long out_pitch, long nrows,
+                                                  float scale, const float2* __restrict__ tw, MPtr<const float> base_m,
+                                                  long base_pitch, MPtr<const StepScalars> sc_m, int n2,
+                                                  const float2* __restrict__ twy) {
+  // in and out may be the same buffer (rows are staged in LDS before any write)
+  const float2* in = in_m.at(blockIdx.y);
+  const float2* __restrict__ nyq = nyq_m.at(blockIdx.y);
+  float* out = out_m.at(blockIdx.y);
+  const float* base = base_m.at(blockIdx.y);
+  const StepScalars* __restrict__ sc = sc_m.at(blockIdx.y);
   constexpr int M = N1 / 2;
-  constexpr int RPB = (2048 / M) > 0 ? 2048 / M : 1;
+  constexpr int RPB = XRows<N1, YS>::RPB, G = XRows<N1, YS>::G;
   constexpr int PM = M + 1;
   constexpr int EPT = (RPB * M + NT - 1) / NT;
   __shared__ float2 s[RPB * PM];
   __shared__ float2 xm[RPB];
   __shared__ float2 stw[N1];
+  __shared__ float2 ytw[G][8];
   const int tid = threadIdx.x;
-  const long row0 = (long)blockIdx.x * RPB;
+  const int L2 = n2 / 8;
+  auto rowof = [&](int b) -> long {
+    if (!YS) return (long)blockIdx.x * RPB + b;
+    const long q = (long)blockIdx.x * G + b / 8;
+    return (q / L2) * n2 + (q % L2) + (long)L2 * (b % 8);
+  };
   for (int e = tid; e < N1; e += NT) stw[e] = tw[e];
+  if (YS && tid < G * 8) {
+    const int j1 = (int)(((long)blockIdx.x * G + tid / 8) % L2);
+    ytw[tid / 8][tid % 8] = twy[(j1 * (tid % 8)) % n2];
+  }
   for (int e = tid; e < RPB * M; e += NT) {
     const int b = e / M, j = e % M;
-    const long row = row0 + b;
+    const long row = rowof(b);
     float2 v = make_float2(0.f, 0.f);
     if (row < nrows) v = in[row * in_pitch + j];
-    if (j == 0) v.y = 0.f;  // a real output takes the real part of the k1=0 bin
+    if (!YS && j == 0) v.y = 0.f;  // a real output takes the real part of the k1=0 bin
     s[b * PM + j] = v;
   }
   for (int b = tid; b < RPB; b += NT) {
-    const long row = row0 + b;
+    const long row = rowof(b);
     xm[b] = row < nrows ? make_float2(nyq[row * nyq_stride].x, 0.f) : make_float2(0.f, 0.f);
   }
   __syncthreads();
+  if (YS) {
+    // y frequencies k2 of a group sit in LDS rows 8*gi + k2; the inverse
+    // 8-point stage leaves rows j1 + L2*j2 in LDS rows 8*gi + j2
+    for (int e = tid; e < G * M; e += NT) {
+      const int gi = e / M, j = e % M;
+      float2 a[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const float2 v = s[(8 * gi + t) * PM + j];
+        a[t] = t ? cmul(v, cconj(ytw[gi][t])) : v;
+      }
+      dft8<+1>(a);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        float2 v = a[t];
+        if (j == 0) v.y = 0.f;  // a real output takes the real part of the k1=0 bin
+        s[(8 * gi + t) * PM + j] = v;
+      }
+    }
+    __syncthreads();
+  }
   float2 z[EPT];
 #pragma unroll
   for (int i = 0; i < EPT; ++i) {
@@ -138,7 +232,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(const float2* in, long in_pitc
   if (EPI == EPI_CORR) coef = (float)sc->Dt;
   for (int e = tid; e < RPB * M; e += NT) {
     const int b = e / M, j = e % M;
-    const long row = row0 + b;
+    const long row = rowof(b);
     if (row >= nrows) continue;
     const float2 v = s[b * PM + j];
     float a0 = v.x * scale, a1 = v.y * scale;
@@ -163,8 +257,9 @@ struct ColTile {
 };
 
 template <int L, int NT, int SIGN>
-__global__ __launch_bounds__(NT) void k_fft_cols(float2* data, long jstride, long pstride, int ncols,
+__global__ __launch_bounds__(NT) void k_fft_cols(MPtr<float2> data_m, long jstride, long pstride, int ncols,
                                                  const float2* __restrict__ tw) {
+  float2* data = data_m.at(blockIdx.y);
   constexpr int TX = ColTile<L>::TX;
   __shared__ float2 s[L * TX];
   __shared__ float2 stw[L];
@@ -190,8 +285,9 @@ __global__ __launch_bounds__(NT) void k_fft_cols(float2* data, long jstride, lon
 
 // ---------------------------------------------------------- contiguous ----
 template <int L, int NT, int SIGN>
-__global__ __launch_bounds__(NT) void k_fft_rows(float2* data, long pitch, long nrows,
+__global__ __launch_bounds__(NT) void k_fft_rows(MPtr<float2> data_m, long pitch, long nrows,
                                                  const float2* __restrict__ tw) {
+  float2* data = data_m.at(blockIdx.y);
   constexpr int RPB = (2048 / L) > 0 ? 2048 / L : 1;
   constexpr int PL = L + 1;
   __shared__ float2 s[RPB * PL];
@@ -216,15 +312,33 @@ __global__ __launch_bounds__(NT) void k_fft_rows(float2* data, long pitch, long 
 // ----------------------------------------------------------- launchers ----
 #define NT_FFT 256
 
-int launch_fft_x_r2c(int n1, const float* in, long in_pitch, float2* out, long out_pitch, float2* nyq,
-                     long nyq_stride, long nrows, float scale, hipStream_t st) {
+// ys_n2 > 0: the split y transform over planes of ys_n2 rows (see
+// k_fft_x_r2c); nrows is then a multiple of ys_n2.
+int launch_fft_x_r2c(int n1, MPtr<const float> in, long in_pitch, MPtr<float2> out, long out_pitch,
+                     MPtr<float2> nyq, long nyq_stride, long nrows, float scale, int nm, hipStream_t st, int ys_n2) {
   const float2* tw = twiddle_table(n1);
   if (!tw) return RELAX_ERR_NOMEM;
-  const int M = n1 / 2;
-  const int rpb = 2048 / M > 0 ? 2048 / M : 1;
-  const dim3 grid((unsigned)((nrows + rpb - 1) / rpb));
+  const float2* twy = nullptr;
+  if (ys_n2) {
+    if (ys_n2 < 64 || nrows % ys_n2) return RELAX_ERR_SIZE;
+    if (!(twy = twiddle_table(ys_n2))) return RELAX_ERR_NOMEM;
+  }
+  long nb = 0;
   switch (n1) {
-#define X(N) case N: k_fft_x_r2c<N, NT_FFT><<<grid, NT_FFT, 0, st>>>(in, in_pitch, out, out_pitch, nyq, nyq_stride, nrows, scale, tw); break;
+#define X(N)                                                                                                \
+  case N:                                                                                                   \
+    if (ys_n2) {                                                                                            \
+      constexpr int G = XRows<N, true>::G;                                                                  \
+      nb = (nrows / 8 + G - 1) / G;                                                                         \
+      k_fft_x_r2c<N, NT_FFT, true><<<dim3((unsigned)nb, (unsigned)nm), NT_FFT, 0, st>>>(                    \
+          in, in_pitch, out, out_pitch, nyq, nyq_stride, nrows, scale, tw, ys_n2, twy);                     \
+    } else {                                                                                                \
+      constexpr int RPB = XRows<N, false>::RPB;                                                             \
+      nb = (nrows + RPB - 1) / RPB;                                                                         \
+      k_fft_x_r2c<N, NT_FFT, false><<<dim3((unsigned)nb, (unsigned)nm), NT_FFT, 0, st>>>(                   \
+          in, in_pitch, out, out_pitch, nyq, nyq_stride, nrows, scale, tw, 0, nullptr);                     \
+    }                                                                                                       \
+    break;
     X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
 #undef X
     default: return RELAX_ERR_SIZE;
@@ -232,25 +346,36 @@ int launch_fft_x_r2c(int n1, const float* in, long in_pitch, float2* out, long o
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }
 
-int launch_fft_x_c2r(int n1, int epi, const float2* in, long in_pitch, const float2* nyq, long nyq_stride,
-                     float* out, long out_pitch, long nrows, float scale, const float* base, long base_pitch,
-                     const StepScalars* sc, hipStream_t st) {
+int launch_fft_x_c2r(int n1, int epi, MPtr<const float2> in, long in_pitch, MPtr<const float2> nyq,
+                     long nyq_stride, MPtr<float> out, long out_pitch, long nrows, float scale,
+                     MPtr<const float> base, long base_pitch, MPtr<const StepScalars> sc, int nm, hipStream_t st,
+                     int ys_n2) {
   const float2* tw = twiddle_table(n1);
   if (!tw) return RELAX_ERR_NOMEM;
-  const int M = n1 / 2;
-  const int rpb = 2048 / M > 0 ? 2048 / M : 1;
-  const dim3 grid((unsigned)((nrows + rpb - 1) / rpb));
-#define X(N, E) case N: k_fft_x_c2r<N, NT_FFT, E><<<grid, NT_FFT, 0, st>>>(in, in_pitch, nyq, nyq_stride, out, out_pitch, nrows, scale, tw, base, base_pitch, sc); break;
+  const float2* twy = nullptr;
+  if (ys_n2) {
+    if (ys_n2 < 64 || nrows % ys_n2) return RELAX_ERR_SIZE;
+    if (!(twy = twiddle_table(ys_n2))) return RELAX_ERR_NOMEM;
+  }
+#define LAUNCH(N, E, YS)                                                                                    \
+  {                                                                                                         \
+    constexpr int RPB = XRows<N, YS>::RPB, G = XRows<N, YS>::G;                                             \
+    const long nb = YS ? (nrows / 8 + G - 1) / G : (nrows + RPB - 1) / RPB;                                 \
+    k_fft_x_c2r<N, NT_FFT, E, YS><<<dim3((unsigned)nb, (unsigned)nm), NT_FFT, 0, st>>>(                     \
+        in, in_pitch, nyq, nyq_stride, out, out_pitch, nrows, scale, tw, base, base_pitch, sc, ys_n2, twy); \
+  }
+#define X(N, E) case N: if (ys_n2) LAUNCH(N, E, true) else LAUNCH(N, E, false) break;
 #define SW(E) switch (n1) { X(16, E) X(32, E) X(64, E) X(128, E) X(256, E) X(512, E) X(1024, E) default: return RELAX_ERR_SIZE; }
   if (epi == EPI_PLAIN) { SW(EPI_PLAIN) }
   else if (epi == EPI_PRED) { SW(EPI_PRED) }
   else { SW(EPI_CORR) }
 #undef SW
 #undef X
+#undef LAUNCH
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }
 
-int launch_fft_cols(int L, int sign, float2* data, long jstride, long pstride, int ncols, int nplanes,
+int launch_fft_cols(int L, int sign, MPtr<float2> data, long jstride, long pstride, int ncols, int nplanes, int nm,
                     hipStream_t st) {
   const float2* tw = twiddle_table(L);
   if (!tw) return RELAX_ERR_NOMEM;
@@ -261,7 +386,7 @@ int launch_fft_cols(int L, int sign, float2* data, long jstride, long pstride, i
 #undef T
     default: return RELAX_ERR_SIZE;
   }
-  const dim3 grid((unsigned)(((ncols + tx - 1) / tx) * (long)nplanes));
+  const dim3 grid((unsigned)(((ncols + tx - 1) / tx) * (long)nplanes), (unsigned)nm);
 #define X(N, S) case N: k_fft_cols<N, NT_FFT, S><<<grid, NT_FFT, 0, st>>>(data, jstride, pstride, ncols, tw); break;
   if (sign < 0) {
     switch (L) { X(16, -1) X(32, -1) X(64, -1) X(128, -1) X(256, -1) X(512, -1) X(1024, -1) default: return RELAX_ERR_SIZE; }
@@ -272,11 +397,11 @@ int launch_fft_cols(int L, int sign, float2* data, long jstride, long pstride, i
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }
 
-int launch_fft_rows(int L, int sign, float2* data, long pitch, long nrows, hipStream_t st) {
+int launch_fft_rows(int L, int sign, MPtr<float2> data, long pitch, long nrows, int nm, hipStream_t st) {
   const float2* tw = twiddle_table(L);
   if (!tw) return RELAX_ERR_NOMEM;
   const int rpb = 2048 / L > 0 ? 2048 / L : 1;
-  const dim3 grid((unsigned)((nrows + rpb - 1) / rpb));
+  const dim3 grid((unsigned)((nrows + rpb - 1) / rpb), (unsigned)nm);
 #define X(N, S) case N: k_fft_rows<N, NT_FFT, S><<<grid, NT_FFT, 0, st>>>(data, pitch, nrows, tw); break;
   if (sign < 0) {
     switch (L) { X(16, -1) X(32, -1) X(64, -1) X(128, -1) X(256, -1) X(512, -1) X(1024, -1) default: return RELAX_ERR_SIZE; }

@@ -140,10 +140,14 @@ __device__ __forceinline__ void cerruti_point(const ColumnConsts& C, const Cerru
 // ---------------------------------------------------- fused column pass ----
 template <int H3, int TX, int NT>
 __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu, double alpha, double gamma,
-                                                      int n1, int n2, int n3, float2* W, float2* NQ,
-                                                      const float2* __restrict__ TM,
-                                                      const float2* __restrict__ TN,
-                                                      const float2* __restrict__ tw3) {
+                                                      int n1, int n2, int n3, MPtr<float2> W_m, MPtr<float2> NQ_m,
+                                                      MPtr<const float2> TM_m, MPtr<const float2> TN_m,
+                                                      const float2* __restrict__ tw3, int l2) {
+  // model blockIdx.y of the launch
+  float2* W = W_m.at(blockIdx.y);
+  float2* NQ = NQ_m.at(blockIdx.y);
+  const float2* __restrict__ TM = TM_m.at(blockIdx.y);
+  const float2* __restrict__ TN = TN_m.at(blockIdx.y);
   static_assert(NT % 64 == 0 && 64 % TX == 0, "a thread keeps one column; whole waves");
   constexpr int NW = NT / 64;
   // LDS: six sequences (component c, half) of H3 x TX complex.  The stride
@@ -166,8 +170,11 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
   long cs, zs, tcs;
   int k1i, k2i;
   if (bid < nmain) {
-    const int k2 = bid / (m1 / TX), k10 = (bid % (m1 / TX)) * TX;
-    base = W + (long)k2 * m1 + k10;
+    // spectrum row r of W; its y frequency is r, or 8*(r % l2) + r / l2 in
+    // the split-y order; the traction planes are in natural order
+    const int r = bid / (m1 / TX), k10 = (bid % (m1 / TX)) * TX;
+    const int k2 = l2 ? 8 * (r % l2) + r / l2 : r;
+    base = W + (long)r * m1 + k10;
     cs = (long)H3 * n2 * m1; zs = (long)n2 * m1;
     tbase = TM + (long)k2 * m1 + k10; tcs = (long)n2 * m1;
     k1i = k10 + tx; k2i = k2;
@@ -262,21 +269,26 @@ struct GreenTile {
   static constexpr int TX = H3 >= 256 ? 4 : (H3 >= 128 ? 8 : 16);
 };
 
-int launch_green_columns(const GreenConst& gc, int h3, float2* W, float2* NQ, const float2* TM, const float2* TN,
-                         hipStream_t st) {
+// threads per workgroup of the column pass (compile-time choice)
+#ifndef RELAX_GREEN_NT
+#define RELAX_GREEN_NT 256
+#endif
+int launch_green_columns(const GreenConst& gc, int h3, MPtr<float2> W, MPtr<float2> NQ, MPtr<const float2> TM,
+                         MPtr<const float2> TN, int nm, hipStream_t st, int l2) {
   const float2* tw3 = twiddle_table(gc.n3);
   if (!tw3) return RELAX_ERR_NOMEM;
   const ColumnConsts C = make_consts(gc);
   const double alpha = (gc.lambda + gc.mu) / (gc.lambda + 2.0 * gc.mu);
   const int m1 = gc.n1 / 2;
-  constexpr int NT = 256;
+  constexpr int NT = RELAX_GREEN_NT;
 #define X(HH)                                                                                          \
   case HH: {                                                                                           \
     constexpr int TX = GreenTile<HH>::TX;                                                              \
     if (m1 % TX || gc.n2 % TX) return RELAX_ERR_SIZE;                                                  \
     const int nb = (m1 / TX) * gc.n2 + gc.n2 / TX;                                                     \
-    k_green_columns<HH, TX, NT><<<nb, NT, 0, st>>>(C, gc.mu, alpha, gc.gamma, gc.n1, gc.n2, gc.n3, W, NQ, \
-                                                   TM, TN, tw3);                                       \
+    k_green_columns<HH, TX, NT><<<dim3(nb, nm), NT, 0, st>>>(C, gc.mu, alpha, gc.gamma, gc.n1, gc.n2, gc.n3, \
+                                                   W, NQ, \
+                                                   TM, TN, tw3, l2);                                   \
     break;                                                                                             \
   }
   switch (h3) {

@@ -1,20 +1,46 @@
 // kernels.h -- host-side launchers of the gfx950 kernels.
 #pragma once
 #include "common.h"
+#include "modeldev.h"
 
 enum { EPI_PLAIN = 0, EPI_PRED = 1, EPI_CORR = 2 };
 
 const float2* twiddle_table(int L);
 
-// fft.hip
-int launch_fft_x_r2c(int n1, const float* in, long in_pitch, float2* out, long out_pitch, float2* nyq,
-                     long nyq_stride, long nrows, float scale, hipStream_t st);
-int launch_fft_x_c2r(int n1, int epi, const float2* in, long in_pitch, const float2* nyq, long nyq_stride,
-                     float* out, long out_pitch, long nrows, float scale, const float* base, long base_pitch,
-                     const StepScalars* sc, hipStream_t st);
-int launch_fft_cols(int L, int sign, float2* data, long jstride, long pstride, int ncols, int nplanes,
+// fft.hip -- each launch covers nm models (grid y); the MPtr arguments give
+// each model's buffer
+// ys_n2 > 0 splits the following y transform of planes of ys_n2 rows: the x
+// kernels do its first radix-8 stage across rows j1 + (ys_n2/8)*j2, and
+// k_fft_cols of length ys_n2/8 over blocks of ys_n2/8 rows completes it,
+// leaving y frequency 8*k1 + k2 at row (ys_n2/8)*k2 + k1 of a plane
+int launch_fft_x_r2c(int n1, MPtr<const float> in, long in_pitch, MPtr<float2> out, long out_pitch,
+                     MPtr<float2> nyq, long nyq_stride, long nrows, float scale, int nm, hipStream_t st,
+                     int ys_n2 = 0);
+int launch_fft_x_c2r(int n1, int epi, MPtr<const float2> in, long in_pitch, MPtr<const float2> nyq,
+                     long nyq_stride, MPtr<float> out, long out_pitch, long nrows, float scale,
+                     MPtr<const float> base, long base_pitch, MPtr<const StepScalars> sc, int nm, hipStream_t st,
+                     int ys_n2 = 0);
+int launch_fft_cols(int L, int sign, MPtr<float2> data, long jstride, long pstride, int ncols, int nplanes, int nm,
                     hipStream_t st);
-int launch_fft_rows(int L, int sign, float2* data, long pitch, long nrows, hipStream_t st);
+int launch_fft_rows(int L, int sign, MPtr<float2> data, long pitch, long nrows, int nm, hipStream_t st);
+// one buffer
+inline int launch_fft_x_r2c(int n1, const float* in, long in_pitch, float2* out, long out_pitch, float2* nyq,
+                            long nyq_stride, long nrows, float scale, hipStream_t st) {
+  return launch_fft_x_r2c(n1, mptr(in), in_pitch, mptr(out), out_pitch, mptr(nyq), nyq_stride, nrows, scale, 1, st);
+}
+inline int launch_fft_x_c2r(int n1, int epi, const float2* in, long in_pitch, const float2* nyq, long nyq_stride,
+                            float* out, long out_pitch, long nrows, float scale, const float* base, long base_pitch,
+                            const StepScalars* sc, hipStream_t st) {
+  return launch_fft_x_c2r(n1, epi, mptr(in), in_pitch, mptr(nyq), nyq_stride, mptr(out), out_pitch, nrows, scale,
+                          mptr(base), base_pitch, mptr(sc), 1, st);
+}
+inline int launch_fft_cols(int L, int sign, float2* data, long jstride, long pstride, int ncols, int nplanes,
+                           hipStream_t st) {
+  return launch_fft_cols(L, sign, mptr(data), jstride, pstride, ncols, nplanes, 1, st);
+}
+inline int launch_fft_rows(int L, int sign, float2* data, long pitch, long nrows, hipStream_t st) {
+  return launch_fft_rows(L, sign, mptr(data), pitch, nrows, 1, st);
+}
 
 // green.hip
 struct GreenConst {
@@ -24,8 +50,10 @@ struct GreenConst {
 };
 // fused pruned column pass: spectra of the half grid (x and y already
 // transformed) -> Green + surface traction + Cerruti -> back to x3 space
-int launch_green_columns(const GreenConst& gc, int h3, float2* W, float2* NQ, const float2* TM, const float2* TN,
-                         hipStream_t st);
+// l2 > 0: the spectrum rows of W are in the split-y order (row l2*k2 + k1
+// holds y frequency 8*k1 + k2); the traction planes TM are in natural order
+int launch_green_columns(const GreenConst& gc, int h3, MPtr<float2> W, MPtr<float2> NQ, MPtr<const float2> TM,
+                         MPtr<const float2> TN, int nm, hipStream_t st, int l2 = 0);
 // per-routine kernels on padded full spectra (row pitch n1/2+1 complex)
 int launch_elastic_padded(const GreenConst& gc, float2* f1, float2* f2, float2* f3, hipStream_t st);
 int launch_surface_padded(const GreenConst& gc, const float2* u1, const float2* u2, const float2* u3,

@@ -40,6 +40,14 @@ struct relax_model {
   RheoParams R{};
   StencilConst S{};
   GreenConst gc{};
+  // the device record the step kernels read (a batch of one), its host
+  // image, and a version that batches compare to refresh their copies
+  ModelDev* md_dev = nullptr;
+  ModelDev md_host{};
+  bool md_dirty = true;
+  unsigned md_version = 0;
+  relax_model* self = nullptr;
+  relax_batch* batch = nullptr;  // the batch the model is a member of
   // phase timers (relax_model_profile)
   bool prof_on = false;
   int prof_n = 0;

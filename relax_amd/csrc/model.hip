@@ -25,6 +25,10 @@
 
 static const double STEP_MAX = 1e7;  // relax.f90:202
 
+// A member of a batch: wait for the batch's queued steps (defined with the
+// batches at the end of the file).
+static int wait_batch(relax_model_t* m);
+
 // ------------------------------------------------------------- kernels ----
 __global__ void k_reset_scalars(StepScalars* sc, double t, int oi, int it) {
   sc->t = t; sc->Dt = 0; sc->tm = 0;
@@ -34,13 +38,14 @@ __global__ void k_reset_scalars(StepScalars* sc, double t, int oi, int it) {
   for (int k = 0; k < 3; ++k) sc->tmin_bits[k] = 0x7f800000u;
 }
 
-struct TimeLine {
-  double interval, odt, tscale, next_event, etime;
-  int skip, has_next;
-  int mech[3];
-};
-
-__global__ void k_finalize_dt(StepScalars* sc, TimeLine tl, int use_override, double dt_override) {
+// One thread per model: the relaxation-time minima of the predictor's
+// rheology pass become tm and Dt (relax.f90:659-677, integrationstep
+// :1420-1452).
+__global__ void k_finalize_dt(const ModelDev* __restrict__ md, int nm, int use_override, double dt_override) {
+  const int mi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mi >= nm) return;
+  StepScalars* sc = md[mi].sc;
+  const TimeLine tl = md[mi].tl;
   double maxwell[3];
   for (int k = 0; k < 3; ++k) {
     const float v = fminf(__uint_as_float(sc->tmin_bits[k]), (float)STEP_MAX);
@@ -67,10 +72,13 @@ __global__ void k_finalize_dt(StepScalars* sc, TimeLine tl, int use_override, do
 
 // First level of the amplitude reduction: workgroup g sums the stress
 // kernel's partials [g*RED_CHUNK, (g+1)*RED_CHUNK) in a fixed order.
+// The second-level partials follow the first nb in the model's buffer;
+// the model is blockIdx.y.
 static const int RED_CHUNK = 2048;
-__global__ __launch_bounds__(256) void k_reduce_partials(const double2* __restrict__ partials, long nb,
-                                                         double2* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_reduce_partials(const ModelDev* __restrict__ md, long nb) {
   __shared__ double2 red[4];
+  const double2* __restrict__ partials = md[blockIdx.y].partials;
+  double2* __restrict__ out = md[blockIdx.y].partials + nb;
   const int tid = threadIdx.x;
   const long b0 = (long)blockIdx.x * RED_CHUNK;
   double2 a = make_double2(0.0, 0.0);
@@ -91,9 +99,16 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double2* __restri
   }
 }
 
-__global__ __launch_bounds__(256) void k_end_step(StepScalars* sc, const double2* partials, long nb, double dv,
-                                                  TimeLine tl) {
+// One workgroup per model (blockIdx.x): power and C:E^i from the nb
+// second-level partials, then t += Dt and the output counter (isoutput,
+// relax.f90:1364-1376).
+__global__ __launch_bounds__(256) void k_end_step(const ModelDev* __restrict__ md, long nfirst, long nb) {
   __shared__ double2 red[256];
+  const ModelDev& D = md[blockIdx.x];
+  StepScalars* sc = D.sc;
+  const double2* partials = D.partials + nfirst;
+  const double dv = D.dv;
+  const TimeLine& tl = D.tl;
   const int tid = threadIdx.x;
   double2 a = make_double2(0.0, 0.0);
   for (long i = tid; i < nb; i += 256) { a.x += partials[i].x; a.y += partials[i].y; }
@@ -235,13 +250,41 @@ size_t relax_model_footprint(const relax_model_desc_t* d) {
 static void free_model(relax_model_t* m) {
   if (!m) return;
   void* ptrs[] = {m->u, m->sig, m->tau, m->gam, m->W, m->NQ, m->T, m->TN, m->sc, m->partials, m->tables, m->ftab,
-                  m->planes_dev, m->patches_dev};
+                  m->planes_dev, m->patches_dev, m->md_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->host_sc) (void)hipHostFree(m->host_sc);
   for (hipEvent_t e : m->prof_ev) (void)hipEventDestroy(e);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
+}
+
+static TimeLine timeline(const relax_model_t* m) {
+  TimeLine tl;
+  tl.interval = m->desc.interval; tl.odt = m->desc.odt; tl.tscale = m->desc.tscale; tl.skip = m->desc.skip;
+  tl.has_next = m->events.size() > 1;
+  tl.next_event = tl.has_next ? m->events[1] : 0.0;
+  tl.etime = m->events.empty() ? 0.0 : m->events[0];
+  for (int k = 0; k < 3; ++k) tl.mech[k] = m->mech[k];
+  return tl;
+}
+
+// Refresh the model's device record after a change of its rheology, planes
+// or time line.  The copy is ordered on the model's stream before any later
+// launch; the host image stays in the model, so it outlives the copy.
+static int sync_md(relax_model_t* m) {
+  if (!m->md_dirty) return RELAX_OK;
+  ModelDev& d = m->md_host;
+  d.u = m->u; d.sig = m->sig; d.tau = m->tau; d.gam = m->gam; d.W = m->W; d.NQ = m->NQ; d.T = m->T; d.TN = m->TN;
+  d.sc = m->sc; d.partials = m->partials;
+  d.dv = m->desc.dx1 * m->desc.dx2 * m->desc.dx3;
+  d.tl = timeline(m);
+  d.R = m->R;
+  HIP_TRY(hipMemcpyAsync(m->md_dev, &d, sizeof(ModelDev), hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  m->md_dirty = false;
+  ++m->md_version;
+  return RELAX_OK;
 }
 
 int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
@@ -282,6 +325,8 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
   alloc((void**)&m->partials, (m->nblk + m->nred) * sizeof(double2));
   alloc((void**)&m->tables, 19L * g.h3 * 4);
   alloc((void**)&m->ftab, 4L * g.h3 * 8);
+  alloc((void**)&m->md_dev, sizeof(ModelDev));
+  m->self = m;
   if (ok && hipHostMalloc((void**)&m->host_sc, sizeof(StepScalars) * 64) != hipSuccess) ok = false;
   m->host_cap = 64;
   if (!ok) { free_model(m); return RELAX_ERR_NOMEM; }
@@ -326,14 +371,21 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
   m->gc.dx1 = d->dx1; m->gc.dx2 = d->dx2; m->gc.dx3 = d->dx3;
   m->gc.n1 = g.n1; m->gc.n2 = g.n2; m->gc.n3 = g.n3;
   // twiddle tables are created outside any capture
-  if (!twiddle_table(g.n1) || !twiddle_table(g.n2) || !twiddle_table(g.n3)) { free_model(m); return RELAX_ERR_NOMEM; }
+  if (!twiddle_table(g.n1) || !twiddle_table(g.n2) || !twiddle_table(g.n3) ||
+      (g.n2 >= 128 && !twiddle_table(g.n2 / 8))) {
+    free_model(m);
+    return RELAX_ERR_NOMEM;
+  }
   if (hipStreamSynchronize(m->stream) != hipSuccess) { free_model(m); return RELAX_ERR_HIP; }
+  m->md_dirty = true;
+  if (sync_md(m) != RELAX_OK) { free_model(m); return RELAX_ERR_HIP; }
   *out = m;
   return RELAX_OK;
 }
 
 int relax_model_destroy(relax_model_t* m) {
   if (!m) return RELAX_ERR_ARG;
+  if (m->batch) (void)relax_batch_remove(m->batch, m);
   (void)hipStreamSynchronize(m->stream);
   free_model(m);
   return RELAX_OK;
@@ -343,6 +395,7 @@ void* relax_model_stream(relax_model_t* m) { return m ? (void*)m->stream : nullp
 
 int relax_model_synchronize(relax_model_t* m) {
   if (!m) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
   HIP_TRY(hipStreamSynchronize(m->stream));
   return RELAX_OK;
 }
@@ -364,7 +417,9 @@ static int update_gdp(relax_model_t* m, int k) {
 
 int relax_model_set_structure(relax_model_t* m, int which, const relax_layer_t* layers, int n) {
   if (!m || which < 0 || which > 2) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
   const int h3 = m->g.h3;
+  m->md_dirty = true;
   if (!layers) {
     if (which < 2) m->R.mech[which].active = 0;
     else m->R.fr.active = 0;
@@ -408,6 +463,7 @@ int relax_model_set_structure(relax_model_t* m, int which, const relax_layer_t* 
 
 int relax_model_set_prestress(relax_model_t* m, const relax_tensor_layer_t* p, int n) {
   if (!m || !p || n != m->g.h3) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
   const int h3 = m->g.h3;
   std::vector<float> sp(6 * h3), taup(h3);
   for (int i = 0; i < h3; ++i) {
@@ -455,6 +511,7 @@ PlaneDev make_plane(const relax_plane_t& p, double beta, double dx1, double dx3)
 
 int relax_model_set_planes(relax_model_t* m, const relax_plane_t* planes, int np, const relax_slippatch_t* patches) {
   if (!m || np < 0 || (np > 0 && !planes)) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
   if (m->planes_dev) { (void)hipFree(m->planes_dev); m->planes_dev = nullptr; }
   if (m->patches_dev) { (void)hipFree(m->patches_dev); m->patches_dev = nullptr; }
   m->host_planes.assign(planes, planes + np);
@@ -477,11 +534,13 @@ int relax_model_set_planes(relax_model_t* m, const relax_plane_t* planes, int np
   m->R.fr.planes = m->planes_dev;
   m->R.fr.np = np;
   m->R.fr.active = (np > 0) && m->mech[2];
+  m->md_dirty = true;
   return RELAX_OK;
 }
 
 int relax_model_get_patches(relax_model_t* m, relax_slippatch_t* patches) {
   if (!m || !patches) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
   const long n = m->patch_off.empty() ? 0 : m->patch_off.back();
   if (n == 0) return RELAX_OK;
   HIP_TRY(hipStreamSynchronize(m->stream));
@@ -501,6 +560,7 @@ int relax_model_upload(relax_model_t* m, const float* u1, const float* u2, const
                        const relax_tensor_t* tau, const relax_tensor_t* sig, const float* gamma, double t, int oi,
                        int i, int align_data) {
   if (!m || !u1 || !u2 || !u3 || !tau) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
   const Grid& g = m->g;
   const long H = g.H;
   int rc;
@@ -524,7 +584,8 @@ int relax_model_upload(relax_model_t* m, const float* u1, const float* u2, const
   if (rc) return rc;
   k_reset_scalars<<<1, 1, 0, m->stream>>>(m->sc, t, oi, i);
   if (!sig) {  // sig = -tau + C:eps(u), relax.f90:369-371
-    rc = launch_stress(STRESS_SET, g, m->R, m->S, m->u, m->sig, m->tau, m->gam, m->sc, m->partials, m->stream);
+    if ((rc = sync_md(m))) return rc;
+    rc = launch_stress(STRESS_SET, g, m->S, m->md_dev, &m->md_host, 1, false, m->stream);
     if (rc) return rc;
   }
   HIP_TRY(hipStreamSynchronize(m->stream));
@@ -534,6 +595,7 @@ int relax_model_upload(relax_model_t* m, const float* u1, const float* u2, const
 int relax_model_download(relax_model_t* m, float* u1, float* u2, float* u3, relax_tensor_t* tau,
                          relax_tensor_t* sig, float* gamma, double* t, int align_data) {
   if (!m) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
   const Grid& g = m->g;
   const long H = g.H;
   int rc;
@@ -568,6 +630,20 @@ int relax_model_download(relax_model_t* m, float* u1, float* u2, float* u3, rela
   return RELAX_OK;
 }
 
+int relax_model_surface(relax_model_t* m, float* u1, float* u2, float* u3, int align_data) {
+  if (!m) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
+  const Grid& g = m->g;
+  const size_t hp = (size_t)(align_data ? g.n1 + 2 : g.n1) * 4, dp = (size_t)g.n1 * 4;
+  float* dst[3] = {u1, u2, u3};
+  for (int c = 0; c < 3; ++c) {
+    if (!dst[c]) continue;
+    HIP_TRY(hipMemcpy2DAsync(dst[c], hp, m->u + c * g.H, dp, dp, (size_t)g.n2, hipMemcpyDeviceToHost, m->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  return RELAX_OK;
+}
+
 // Phase timers: when a profile is being taken, an event is recorded after
 // every launch and the time since the previous event is charged to the
 // launch's kernel (the GPU runs the stream back to back, so the difference is
@@ -585,94 +661,135 @@ static void mark(relax_model_t* m, int kind) {
   ++m->prof_n;
 }
 
+// A launch context: nm models of one grid and one set of elastic constants
+// (those of `lead`), their device records, and the stream.
+struct Ctx {
+  relax_model_t* lead;
+  const ModelDev* md;
+  const ModelDev* one;            // host copy of the record of a single model, else nullptr
+  int nm;
+  bool fric;                      // some model has fault-creep planes
+  hipStream_t st;
+  relax_model_t* const* models;   // host handles, for the per-plane kernels
+  relax_model_t* prof;            // phase timers (single model), or nullptr
+};
+
+static Ctx single(relax_model_t* m) {
+  Ctx c;
+  c.lead = m; c.md = m->md_dev; c.one = &m->md_host; c.nm = 1; c.fric = m->R.fr.active && m->R.fr.np > 0;
+  c.st = m->stream; c.models = &m->self; c.prof = m;
+  return c;
+}
+
+static void cmark(const Ctx& c, int kind) {
+  if (c.prof) mark(c.prof, kind);
+}
+
 // Green's function of the work field W (body force, upper half) with the
-// traction planes T; the last pass writes u + c*G into `out`.
-static int model_green(relax_model_t* m, int epi) {
+// traction planes T of every model; the last pass writes u + c*G into W
+// (predictor) or u (corrector and EPI_PLAIN).
+static int model_green(const Ctx& c, int epi) {
+  const relax_model_t* m = c.lead;
   const Grid& g = m->g;
-  hipStream_t st = m->stream;
-  const int n1 = g.n1, n2 = g.n2, h3 = g.h3, m1 = g.m1;
-  float2* Wc = reinterpret_cast<float2*>(m->W);
-  float2* Tc = reinterpret_cast<float2*>(m->T);
+  hipStream_t st = c.st;
+  const int n1 = g.n1, n2 = g.n2, h3 = g.h3, m1 = g.m1, nm = c.nm;
+  const ModelDev* md = c.md;
   const double dv = m->desc.dx1 * m->desc.dx2 * m->desc.dx3;
+  // the y transform of W is split (radix 8 in the x kernels, n2/8 points in
+  // the column kernel) from n2 = 128 up; the traction planes keep one pass
+  const int ys = n2 >= 128 ? n2 : 0;
+  const int l2 = ys ? n2 / 8 : 0;
+  const int ly = ys ? l2 : n2, nyp = ys ? 3 * h3 * 8 : 3 * h3;
   int rc;
-  if ((rc = launch_fft_x_r2c(n1, m->W, n1, Wc, m1, m->NQ, 1, 3L * n2 * h3, (float)dv, st))) return rc;
-  mark(m, RELAX_PROF_FFT_X_FWD);
-  if ((rc = launch_fft_x_r2c(n1, m->T, n1, Tc, m1, m->TN, 1, 3L * n2, (float)(m->desc.dx1 * m->desc.dx2), st))) return rc;
-  mark(m, RELAX_PROF_SMALL);
-  if ((rc = launch_fft_cols(n2, -1, Wc, m1, (long)n2 * m1, m1, 3 * h3, st))) return rc;
-  mark(m, RELAX_PROF_FFT_Y);
-  if ((rc = launch_fft_cols(n2, -1, Tc, m1, (long)n2 * m1, m1, 3, st))) return rc;
-  mark(m, RELAX_PROF_SMALL);
-  if ((rc = launch_fft_rows(n2, -1, m->NQ, n2, 3L * h3, st))) return rc;
-  mark(m, RELAX_PROF_SMALL);
-  if ((rc = launch_fft_rows(n2, -1, m->TN, n2, 3, st))) return rc;
-  mark(m, RELAX_PROF_SMALL);
-  if ((rc = launch_green_columns(m->gc, h3, Wc, m->NQ, Tc, m->TN, st))) return rc;
-  mark(m, RELAX_PROF_GREEN_Z);
-  if ((rc = launch_fft_cols(n2, +1, Wc, m1, (long)n2 * m1, m1, 3 * h3, st))) return rc;
-  mark(m, RELAX_PROF_FFT_Y);
-  if ((rc = launch_fft_rows(n2, +1, m->NQ, n2, 3L * h3, st))) return rc;
-  mark(m, RELAX_PROF_SMALL);
+  if ((rc = launch_fft_x_r2c(n1, MFIELD(const float, md, W), n1, MFIELD(float2, md, W), m1, MFIELD(float2, md, NQ), 1,
+                             3L * n2 * h3, (float)dv, nm, st, ys)))
+    return rc;
+  cmark(c, RELAX_PROF_FFT_X_FWD);
+  if ((rc = launch_fft_x_r2c(n1, MFIELD(const float, md, T), n1, MFIELD(float2, md, T), m1, MFIELD(float2, md, TN), 1,
+                             3L * n2, (float)(m->desc.dx1 * m->desc.dx2), nm, st)))
+    return rc;
+  cmark(c, RELAX_PROF_SMALL);
+  if ((rc = launch_fft_cols(ly, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st))) return rc;
+  cmark(c, RELAX_PROF_FFT_Y);
+  if ((rc = launch_fft_cols(n2, -1, MFIELD(float2, md, T), m1, (long)n2 * m1, m1, 3, nm, st))) return rc;
+  cmark(c, RELAX_PROF_SMALL);
+  if ((rc = launch_fft_rows(n2, -1, MFIELD(float2, md, NQ), n2, 3L * h3, nm, st))) return rc;
+  cmark(c, RELAX_PROF_SMALL);
+  if ((rc = launch_fft_rows(n2, -1, MFIELD(float2, md, TN), n2, 3, nm, st))) return rc;
+  cmark(c, RELAX_PROF_SMALL);
+  if ((rc = launch_green_columns(m->gc, h3, MFIELD(float2, md, W), MFIELD(float2, md, NQ), MFIELD(const float2, md, T),
+                                 MFIELD(const float2, md, TN), nm, st, l2)))
+    return rc;
+  cmark(c, RELAX_PROF_GREEN_Z);
+  if ((rc = launch_fft_cols(ly, +1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st))) return rc;
+  cmark(c, RELAX_PROF_FFT_Y);
+  if ((rc = launch_fft_rows(n2, +1, MFIELD(float2, md, NQ), n2, 3L * h3, nm, st))) return rc;
+  cmark(c, RELAX_PROF_SMALL);
   const double inv = 1.0 / (n1 * m->desc.dx1 * n2 * m->desc.dx2 * g.n3 * m->desc.dx3);
-  float* out = epi == EPI_PRED ? m->W : m->u;  // EPI_PLAIN writes G into u (coseismic set-up)
-  rc = launch_fft_x_c2r(n1, epi, Wc, m1, m->NQ, 1, out, n1, 3L * n2 * h3, (float)inv, m->u, n1, m->sc, st);
-  mark(m, RELAX_PROF_FFT_X_INV);
+  const MPtr<float> out = epi == EPI_PRED ? MFIELD(float, md, W) : MFIELD(float, md, u);
+  rc = launch_fft_x_c2r(n1, epi, MFIELD(const float2, md, W), m1, MFIELD(const float2, md, NQ), 1, out, n1,
+                        3L * n2 * h3, (float)inv, MFIELD(const float, md, u), n1, MFIELD(const StepScalars, md, sc), nm,
+                        st, ys);
+  cmark(c, RELAX_PROF_FFT_X_INV);
   return rc;
 }
 
-static TimeLine timeline(const relax_model_t* m) {
-  TimeLine tl;
-  tl.interval = m->desc.interval; tl.odt = m->desc.odt; tl.tscale = m->desc.tscale; tl.skip = m->desc.skip;
-  tl.has_next = m->events.size() > 1;
-  tl.next_event = tl.has_next ? m->events[1] : 0.0;
-  tl.etime = m->events.empty() ? 0.0 : m->events[0];
-  for (int k = 0; k < 3; ++k) tl.mech[k] = m->mech[k];
-  return tl;
-}
-
-static int enqueue_step(relax_model_t* m, const double* dt_override) {
-  hipStream_t st = m->stream;
-  const Grid& g = m->g;
-  const TimeLine tl = timeline(m);
+// One RK2 step of every model of the context, enqueued without host
+// synchronisation.
+static int enqueue_steps(const Ctx& c, const double* dt_override) {
+  hipStream_t st = c.st;
+  const Grid& g = c.lead->g;
+  const StencilConst& S = c.lead->S;
+  const int nm = c.nm;
   int rc;
-  mark(m, -1);
+  cmark(c, -1);
   // predictor
-  if ((rc = launch_bodyforce(g, m->R, m->S, m->sig, m->W, m->T, m->sc, BODY_PRED, st))) return rc;
-  mark(m, RELAX_PROF_BODYFORCE);
-  k_finalize_dt<<<1, 1, 0, st>>>(m->sc, tl, dt_override ? 1 : 0, dt_override ? *dt_override : 0.0);
-  mark(m, RELAX_PROF_SMALL);
-  if ((rc = model_green(m, EPI_PRED))) return rc;
-  if ((rc = launch_stress(STRESS_PRED, g, m->R, m->S, m->W, m->sig, m->tau, m->gam, m->sc, m->partials, st))) return rc;
-  mark(m, RELAX_PROF_STRESS_PRED);
+  if ((rc = launch_bodyforce(g, S, c.md, c.one, nm, c.fric, BODY_PRED, st))) return rc;
+  cmark(c, RELAX_PROF_BODYFORCE);
+  k_finalize_dt<<<(nm + 63) / 64, 64, 0, st>>>(c.md, nm, dt_override ? 1 : 0, dt_override ? *dt_override : 0.0);
+  cmark(c, RELAX_PROF_SMALL);
+  if ((rc = model_green(c, EPI_PRED))) return rc;
+  if ((rc = launch_stress(STRESS_PRED, g, S, c.md, c.one, nm, c.fric, st))) return rc;
+  cmark(c, RELAX_PROF_STRESS_PRED);
   // corrector
-  if ((rc = launch_bodyforce(g, m->R, m->S, m->sig, m->W, m->T, m->sc, BODY_CORR, st))) return rc;
-  mark(m, RELAX_PROF_BODYFORCE);
-  if (m->R.fr.active && m->patches_dev) {
+  if ((rc = launch_bodyforce(g, S, c.md, c.one, nm, c.fric, BODY_CORR, st))) return rc;
+  cmark(c, RELAX_PROF_BODYFORCE);
+  for (int i = 0; i < nm; ++i) {
+    relax_model_t* m = c.models[i];
+    if (!(m->R.fr.active && m->patches_dev)) continue;
     for (int k = 0; k < m->R.fr.np; ++k) {
       const relax_plane_t& p = m->host_planes[k];
-      if ((rc = launch_monitorfriction(m->plane_dev_host[k], p.rake, p.px2, p.px3, g, m->desc.dx1, m->desc.dx2,
-                                       m->desc.dx3, g.n3, m->sig, m->R.fr.v0, m->R.fr.tauc, m->R.fr.friction,
+      if ((rc = launch_monitorfriction(m->plane_dev_host[k], p.rake, p.px2, p.px3, m->g, m->desc.dx1, m->desc.dx2,
+                                       m->desc.dx3, m->g.n3, m->sig, m->R.fr.v0, m->R.fr.tauc, m->R.fr.friction,
                                        m->R.fr.cohesion, m->patches_dev + m->patch_off[k], m->sc, st)))
         return rc;
-      mark(m, RELAX_PROF_SMALL);
+      cmark(c, RELAX_PROF_SMALL);
     }
   }
-  if ((rc = model_green(m, EPI_CORR))) return rc;
-  if ((rc = launch_stress(STRESS_CORR, g, m->R, m->S, m->u, m->sig, m->tau, m->gam, m->sc, m->partials, st))) return rc;
-  mark(m, RELAX_PROF_STRESS_CORR);
-  if (m->patches_dev && m->patch_off.back() > 0) {
+  if ((rc = model_green(c, EPI_CORR))) return rc;
+  if ((rc = launch_stress(STRESS_CORR, g, S, c.md, c.one, nm, c.fric, st))) return rc;
+  cmark(c, RELAX_PROF_STRESS_CORR);
+  for (int i = 0; i < nm; ++i) {
+    relax_model_t* m = c.models[i];
+    if (!(m->patches_dev && m->patch_off.back() > 0)) continue;
     const long n = m->patch_off.back();
     k_frictionadd<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(m->patches_dev, n, m->sc, 0, 0.0);
-    mark(m, RELAX_PROF_SMALL);
+    cmark(c, RELAX_PROF_SMALL);
   }
-  k_reduce_partials<<<(unsigned)m->nred, 256, 0, st>>>(m->partials, m->nblk, m->partials + m->nblk);
-  k_end_step<<<1, 256, 0, st>>>(m->sc, m->partials + m->nblk, m->nred, m->desc.dx1 * m->desc.dx2 * m->desc.dx3, tl);
-  mark(m, RELAX_PROF_SMALL);
+  k_reduce_partials<<<dim3((unsigned)c.lead->nred, (unsigned)nm), 256, 0, st>>>(c.md, c.lead->nblk);
+  k_end_step<<<nm, 256, 0, st>>>(c.md, c.lead->nblk, c.lead->nred);
+  cmark(c, RELAX_PROF_SMALL);
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }
 
+static int enqueue_step(relax_model_t* m, const double* dt_override) {
+  int rc = sync_md(m);
+  if (rc) return rc;
+  return enqueue_steps(single(m), dt_override);
+}
+
 int relax_model_profile(relax_model_t* m, int nsteps, double* ms, int* launches) {
-  if (!m || nsteps < 1 || !ms || !launches) return RELAX_ERR_ARG;
+  if (!m || nsteps < 1 || !ms || !launches || m->batch) return RELAX_ERR_ARG;
   for (int k = 0; k < RELAX_PROF_KINDS; ++k) { ms[k] = 0.0; launches[k] = 0; }
   // all steps are enqueued back to back with their markers; one
   // synchronisation at the end, so the timed steps run as they do unprofiled
@@ -701,7 +818,7 @@ static void fill_info(const StepScalars& s, relax_step_info_t* info) {
 }
 
 int relax_model_step(relax_model_t* m, const double* dt_override, relax_step_info_t* info) {
-  if (!m) return RELAX_ERR_ARG;
+  if (!m || m->batch) return RELAX_ERR_ARG;
   int rc = enqueue_step(m, dt_override);
   if (rc) return rc;
   if (info) {
@@ -714,7 +831,7 @@ int relax_model_step(relax_model_t* m, const double* dt_override, relax_step_inf
 }
 
 int relax_model_run(relax_model_t* m, int nsteps, const double* dt_override, relax_step_info_t* info, int* done) {
-  if (!m || nsteps < 0) return RELAX_ERR_ARG;
+  if (!m || nsteps < 0 || m->batch) return RELAX_ERR_ARG;
   int k = 0;
   for (; k < nsteps; ++k) {
     HIP_TRY(hipMemcpyAsync(m->host_sc, m->sc, sizeof(StepScalars), hipMemcpyDeviceToHost, m->stream));
@@ -769,6 +886,7 @@ __global__ void k_add_shear(const PlaneDev* __restrict__ P, const double* __rest
 
 int relax_model_add_shear(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n) {
   if (!m || n < 0 || (n > 0 && (!patches || !slip))) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
   if (n == 0) return RELAX_OK;
   std::vector<PlaneDev> hp(n);
   for (int k = 0; k < n; ++k) hp[k] = make_plane(patches[k], m->desc.beta, m->desc.dx1, m->desc.dx3);
@@ -793,14 +911,212 @@ int relax_model_add_shear(relax_model_t* m, const relax_plane_t* patches, const 
 
 int relax_model_coseismic(relax_model_t* m) {
   if (!m) return RELAX_ERR_ARG;
-  const Grid& g = m->g;
+  if (int rc = wait_batch(m)) return rc;
   int rc;
+  if ((rc = sync_md(m))) return rc;
+  Ctx c = single(m);
+  c.prof = nullptr;
   // body force and surface traction of tau, then u = G[f, t] and
   // sig = -tau + C:eps(u) (relax.f90:318-371 with the FIR divergence)
-  if ((rc = launch_bodyforce(g, m->R, m->S, m->tau, m->W, m->T, m->sc, BODY_RAW, m->stream))) return rc;
-  if ((rc = model_green(m, EPI_PLAIN))) return rc;
-  if ((rc = launch_stress(STRESS_SET, g, m->R, m->S, m->u, m->sig, m->tau, m->gam, m->sc, m->partials, m->stream)))
-    return rc;
+  if ((rc = launch_bodyforce(m->g, m->S, c.md, c.one, 1, false, BODY_RAW, m->stream))) return rc;
+  if ((rc = model_green(c, EPI_PLAIN))) return rc;
+  if ((rc = launch_stress(STRESS_SET, m->g, m->S, c.md, c.one, 1, false, m->stream))) return rc;
   HIP_TRY(hipStreamSynchronize(m->stream));
+  return RELAX_OK;
+}
+
+// ------------------------------------------------------------- batches ----
+struct relax_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<relax_model_t*> models;
+  std::vector<unsigned> versions;   // member record versions in md_dev
+  std::vector<ModelDev> md_host;
+  ModelDev* md_dev = nullptr;       // one record per member
+  StepScalars* gather = nullptr;    // device copies of the members' scalars
+  StepScalars* host_sc = nullptr;   // pinned
+  int cap = 0;
+  bool dirty = true;                // membership or a record changed
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+__global__ void k_gather_scalars(const ModelDev* __restrict__ md, int nm, StepScalars* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nm) out[i] = *md[i].sc;
+}
+
+static void drop_graph(relax_batch_t* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+static bool batch_compatible(const relax_model_t* a, const relax_model_t* m) {
+  const relax_model_desc_t &x = a->desc, &y = m->desc;
+  return a->device == m->device && x.sx1 == y.sx1 && x.sx2 == y.sx2 && x.sx3 == y.sx3 && x.dx1 == y.dx1 &&
+         x.dx2 == y.dx2 && x.dx3 == y.dx3 && x.lambda == y.lambda && x.mu == y.mu && x.gamma == y.gamma;
+}
+
+int relax_batch_create(relax_batch_t** out) {
+  if (!out) return RELAX_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return RELAX_ERR_NODEVICE;
+  relax_batch_t* b = new (std::nothrow) relax_batch_t();
+  if (!b) return RELAX_ERR_NOMEM;
+  if (hipGetDevice(&b->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return RELAX_ERR_HIP;
+  }
+  *out = b;
+  return RELAX_OK;
+}
+
+int relax_batch_destroy(relax_batch_t* b) {
+  if (!b) return RELAX_ERR_ARG;
+  (void)hipStreamSynchronize(b->stream);
+  for (relax_model_t* m : b->models) m->batch = nullptr;
+  drop_graph(b);
+  if (b->md_dev) (void)hipFree(b->md_dev);
+  if (b->gather) (void)hipFree(b->gather);
+  if (b->host_sc) (void)hipHostFree(b->host_sc);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return RELAX_OK;
+}
+
+int relax_batch_add(relax_batch_t* b, relax_model_t* m) {
+  if (!b || !m || m->batch) return RELAX_ERR_ARG;
+  if (m->device != b->device) return RELAX_ERR_ARG;
+  if (!b->models.empty() && !batch_compatible(b->models[0], m)) return RELAX_ERR_ARG;
+  int rc;
+  if ((rc = sync_md(m))) return rc;
+  HIP_TRY(hipStreamSynchronize(m->stream));  // the model's own work is done before the batch's
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->models.push_back(m);
+  m->batch = b;
+  b->dirty = true;
+  return RELAX_OK;
+}
+
+int relax_batch_remove(relax_batch_t* b, relax_model_t* m) {
+  if (!b || !m || m->batch != b) return RELAX_ERR_ARG;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < b->models.size(); ++i)
+    if (b->models[i] == m) { b->models.erase(b->models.begin() + i); break; }
+  m->batch = nullptr;
+  b->dirty = true;
+  return RELAX_OK;
+}
+
+int relax_batch_size(const relax_batch_t* b) { return b ? (int)b->models.size() : 0; }
+
+int relax_batch_synchronize(relax_batch_t* b) {
+  if (!b) return RELAX_ERR_ARG;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return RELAX_OK;
+}
+
+// Bring the device records up to date; a change drops the captured step.
+static int batch_refresh(relax_batch_t* b) {
+  const int nm = (int)b->models.size();
+  bool changed = b->dirty || (int)b->versions.size() != nm;
+  for (int i = 0; i < nm; ++i) {
+    relax_model_t* m = b->models[i];
+    if (m->md_dirty) {
+      int rc = sync_md(m);
+      if (rc) return rc;
+    }
+    if (!changed && b->versions[i] != m->md_version) changed = true;
+  }
+  if (!changed) return RELAX_OK;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  drop_graph(b);
+  if (nm > b->cap) {
+    if (b->md_dev) { (void)hipFree(b->md_dev); b->md_dev = nullptr; }
+    if (b->gather) { (void)hipFree(b->gather); b->gather = nullptr; }
+    if (b->host_sc) { (void)hipHostFree(b->host_sc); b->host_sc = nullptr; }
+    b->cap = 0;
+    HIP_TRY(hipMalloc(&b->md_dev, sizeof(ModelDev) * nm));
+    HIP_TRY(hipMalloc(&b->gather, sizeof(StepScalars) * nm));
+    HIP_TRY(hipHostMalloc((void**)&b->host_sc, sizeof(StepScalars) * nm));
+    b->cap = nm;
+  }
+  b->md_host.resize(nm);
+  b->versions.resize(nm);
+  for (int i = 0; i < nm; ++i) {
+    b->md_host[i] = b->models[i]->md_host;
+    b->versions[i] = b->models[i]->md_version;
+  }
+  HIP_TRY(hipMemcpy(b->md_dev, b->md_host.data(), sizeof(ModelDev) * nm, hipMemcpyHostToDevice));
+  b->dirty = false;
+  return RELAX_OK;
+}
+
+int relax_batch_step(relax_batch_t* b, relax_step_info_t* infos) {
+  if (!b) return RELAX_ERR_ARG;
+  const int nm = (int)b->models.size();
+  if (nm == 0) return RELAX_OK;
+  int rc;
+  if ((rc = batch_refresh(b))) return rc;
+  if (!b->exec) {
+    // capture one step of all members; replayed until the membership or a
+    // member's record changes
+    Ctx c;
+    c.lead = b->models[0]; c.md = b->md_dev; c.one = nullptr; c.nm = nm; c.st = b->stream;
+    c.models = b->models.data();
+    c.prof = nullptr;
+    c.fric = false;
+    for (relax_model_t* m : b->models) c.fric = c.fric || (m->R.fr.active && m->R.fr.np > 0);
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_steps(c, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : RELAX_ERR_HIP;
+    }
+    b->graph = g;
+    if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+      drop_graph(b);
+      return RELAX_ERR_HIP;
+    }
+  }
+  HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  if (!infos) return RELAX_OK;
+  k_gather_scalars<<<(nm + 63) / 64, 64, 0, b->stream>>>(b->md_dev, nm, b->gather);
+  HIP_TRY(hipMemcpyAsync(b->host_sc, b->gather, sizeof(StepScalars) * nm, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  int first = RELAX_OK;
+  for (int i = 0; i < nm; ++i) {
+    fill_info(b->host_sc[i], infos + i);
+    if (!first && b->host_sc[i].status) first = b->host_sc[i].status;
+  }
+  return first;
+}
+
+int relax_batch_busy(relax_batch_t* b) {
+  if (!b) return 0;
+  return hipStreamQuery(b->stream) == hipErrorNotReady ? 1 : 0;
+}
+
+int relax_batch_scalars(relax_batch_t* b, relax_step_info_t* infos) {
+  if (!b || !infos) return RELAX_ERR_ARG;
+  const int nm = (int)b->models.size();
+  if (nm == 0) return RELAX_OK;
+  int rc;
+  if ((rc = batch_refresh(b))) return rc;
+  k_gather_scalars<<<(nm + 63) / 64, 64, 0, b->stream>>>(b->md_dev, nm, b->gather);
+  HIP_TRY(hipMemcpyAsync(b->host_sc, b->gather, sizeof(StepScalars) * nm, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (int i = 0; i < nm; ++i) fill_info(b->host_sc[i], infos + i);
+  return RELAX_OK;
+}
+
+static int wait_batch(relax_model_t* m) {
+  if (m->batch) HIP_TRY(hipStreamSynchronize(m->batch->stream));
   return RELAX_OK;
 }

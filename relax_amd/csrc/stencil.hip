@@ -16,10 +16,23 @@
 #include "rheology.h"
 #include "stencil.h"
 
-template <int TXB, int TYB, bool TM, bool RAW, bool FRIC>
-__global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, RheoParams R, StencilConst S,
-                                                        const float* __restrict__ sig, float* __restrict__ W,
-                                                        float* __restrict__ T, StepScalars* sc, int zc) {
+// BATCH: the models' records are in the device array md (one model per
+// nx*ny*gz tiles); otherwise the one model's record is the argument Dv.
+template <int TXB, int TYB, bool TM, bool RAW, bool FRIC, bool BATCH>
+__global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, ModelDev Dv, const ModelDev* __restrict__ md,
+                                                        StencilConst S, int zc, int gz) {
+  // 1-D grid of tiles (x fastest, then y, depth chunk, model), taken in the
+  // XCD-aware order so that neighbouring tiles, which share halo lines, meet
+  // in one L2
+  const int nx = g.n1 / TXB, ny = g.n2 / TYB;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % nx, by = (tile / nx) % ny, bz = tile / (nx * ny);
+  const ModelDev& D = BATCH ? md[bz / gz] : Dv;
+  const RheoParams& R = D.R;
+  const float* __restrict__ sig = RAW ? D.tau : D.sig;
+  float* __restrict__ W = D.W;
+  float* __restrict__ T = D.T;
+  StepScalars* sc = D.sc;
   constexpr int NT = TXB * TYB;
   constexpr int HX = TXB + 2, HY = TYB + 2, HP = HX * HY;
   __shared__ float ring[3][6][HP];
@@ -28,8 +41,8 @@ __global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, RheoParams R, St
   const int tx = tid % TXB, ty = tid / TXB;
   const int n1 = g.n1, n2 = g.n2, h3 = g.h3;
   const long H = g.H;
-  const int x0 = blockIdx.x * TXB, y0 = blockIdx.y * TYB;
-  const int z0 = blockIdx.z * zc, z1 = min(z0 + zc, h3);
+  const int x0 = bx * TXB, y0 = by * TYB;
+  const int z0 = (bz % gz) * zc, z1 = min(z0 + zc, h3);
   float tmin[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
   int status = 0;
   const int e0 = (ty + 1) * HX + (tx + 1);
@@ -126,10 +139,18 @@ __global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, RheoParams R, St
   }
 }
 
-template <int MODE, bool FRIC>
-__global__ __launch_bounds__(256) void k_stress(Grid g, RheoParams R, StencilConst S, const float* __restrict__ X,
-                                               float* sig, float* tau, float* gam, StepScalars* sc,
-                                               double2* partials) {
+template <int MODE, bool FRIC, bool BATCH>
+__global__ __launch_bounds__(256) void k_stress(Grid g, ModelDev Dv, const ModelDev* __restrict__ md,
+                                               StencilConst S) {
+  // model blockIdx.y of the launch (BATCH) or the argument Dv
+  const ModelDev& D = BATCH ? md[blockIdx.y] : Dv;
+  const RheoParams& R = D.R;
+  const float* __restrict__ X = MODE == STRESS_PRED ? D.W : D.u;
+  float* sig = D.sig;
+  float* tau = D.tau;
+  float* gam = D.gam;
+  StepScalars* sc = D.sc;
+  double2* partials = D.partials;
   __shared__ double2 red[4];
   const int tid = threadIdx.x;
   const long o = (long)blockIdx.x * 256 + tid;
@@ -217,22 +238,25 @@ __global__ __launch_bounds__(256) void k_stress(Grid g, RheoParams R, StencilCon
   if (status) atomicMax(&sc->status, status);
 }
 
-int launch_bodyforce(const Grid& g, const RheoParams& R, const StencilConst& S, const float* sig, float* W,
-                     float* T, StepScalars* sc, int mode, hipStream_t st) {
+int launch_bodyforce(const Grid& g, const StencilConst& S, const ModelDev* md, const ModelDev* one, int nm,
+                     bool fric, int mode, hipStream_t st) {
   const int zc = 16;
-  const unsigned gz = (unsigned)((g.h3 + zc - 1) / zc);
-#define LF(TX, TY, F)                                                                                    \
+  const int gz = (g.h3 + zc - 1) / zc;
+  const bool batch = !(one && nm == 1);
+  const ModelDev Dv = batch ? ModelDev{} : *one;
+#define LB(TX, TY, F, B)                                                                                 \
   {                                                                                                      \
-    const dim3 grid(g.n1 / TX, g.n2 / TY, gz);                                                           \
-    if (mode == BODY_PRED) k_bodyforce<TX, TY, true, false, F><<<grid, TX * TY, 0, st>>>(g, R, S, sig, W, T, sc, zc); \
-    else if (mode == BODY_CORR) k_bodyforce<TX, TY, false, false, F><<<grid, TX * TY, 0, st>>>(g, R, S, sig, W, T, sc, zc); \
-    else k_bodyforce<TX, TY, false, true, false><<<grid, TX * TY, 0, st>>>(g, R, S, sig, W, T, sc, zc);   \
+    const dim3 grid((unsigned)((g.n1 / TX) * (g.n2 / TY) * gz * nm));                                    \
+    if (mode == BODY_PRED) k_bodyforce<TX, TY, true, false, F, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz); \
+    else if (mode == BODY_CORR) k_bodyforce<TX, TY, false, false, F, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz); \
+    else k_bodyforce<TX, TY, false, true, false, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz);   \
   }
+#define LF(TX, TY, F) { if (batch) LB(TX, TY, F, true) else LB(TX, TY, F, false) }
 #define L(TX, TY) { if (fric) LF(TX, TY, true) else LF(TX, TY, false) }
-  const bool fric = R.fr.active && R.fr.np > 0;
   if (g.n1 % 32 == 0 && g.n2 % 16 == 0) L(32, 16)
   else if (g.n1 % 32 == 0 && g.n2 % 8 == 0) L(32, 8)
   else return RELAX_ERR_SIZE;
+#undef LB
 #undef LF
 #undef L
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
@@ -240,14 +264,19 @@ int launch_bodyforce(const Grid& g, const RheoParams& R, const StencilConst& S, 
 
 long stress_blocks(const Grid& g) { return (g.H + 255) / 256; }
 
-int launch_stress(int mode, const Grid& g, const RheoParams& R, const StencilConst& S, const float* X, float* sig,
-                  float* tau, float* gam, StepScalars* sc, double2* partials, hipStream_t st) {
-  const unsigned nb = (unsigned)stress_blocks(g);
-  const bool fric = R.fr.active && R.fr.np > 0;
-#define K(MODE, F) k_stress<MODE, F><<<nb, 256, 0, st>>>(g, R, S, X, sig, tau, gam, sc, partials)
-  if (mode == STRESS_SET) K(STRESS_SET, false);
-  else if (mode == STRESS_PRED) { if (fric) K(STRESS_PRED, true); else K(STRESS_PRED, false); }
-  else { if (fric) K(STRESS_CORR, true); else K(STRESS_CORR, false); }
+int launch_stress(int mode, const Grid& g, const StencilConst& S, const ModelDev* md, const ModelDev* one, int nm,
+                  bool fric, hipStream_t st) {
+  const dim3 grid((unsigned)stress_blocks(g), (unsigned)nm);
+  const bool batch = !(one && nm == 1);
+  const ModelDev Dv = batch ? ModelDev{} : *one;
+#define K(MODE, F)                                                          \
+  {                                                                         \
+    if (batch) k_stress<MODE, F, true><<<grid, 256, 0, st>>>(g, Dv, md, S);  \
+    else k_stress<MODE, F, false><<<grid, 256, 0, st>>>(g, Dv, md, S);       \
+  }
+  if (mode == STRESS_SET) K(STRESS_SET, false)
+  else if (mode == STRESS_PRED) { if (fric) K(STRESS_PRED, true) else K(STRESS_PRED, false) }
+  else { if (fric) K(STRESS_CORR, true) else K(STRESS_CORR, false) }
 #undef K
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }

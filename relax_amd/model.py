@@ -131,6 +131,84 @@ class Model:
     def synchronize(self):
         check(self._lib.relax_model_synchronize(self._h), "synchronize")
 
+    def surface(self, u1, u2, u3):
+        """Copy the surface plane (x3 = 0) of u1..u3 into (sx2, sx1) float32
+        arrays."""
+        for a in (u1, u2, u3):
+            if a.dtype != np.float32 or not a.flags.c_contiguous:
+                raise ValueError("surface arrays must be C-contiguous float32")
+        check(self._lib.relax_model_surface(self._h, ptr(u1), ptr(u2), ptr(u3), 0), "surface")
+
+
+class Batch:
+    """Models of one grid stepped together: every kernel of the RK2 step is
+    one launch over all members, replayed from a captured hipGraph while the
+    membership is unchanged (relax_batch_* of relax_hip.h)."""
+
+    def __init__(self, models=()):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        check(self._lib.relax_batch_create(ctypes.byref(h)), "batch create")
+        self._h = h
+        self.models = []
+        for m in models:
+            self.add(m)
+
+    def add(self, model):
+        check(self._lib.relax_batch_add(self._h, model.handle), "batch add")
+        self.models.append(model)
+
+    def remove(self, model):
+        check(self._lib.relax_batch_remove(self._h, model.handle), "batch remove")
+        self.models.remove(model)
+
+    def __len__(self):
+        return int(self._lib.relax_batch_size(self._h))
+
+    def step(self, info=True):
+        """One RK2 step of every member; returns the members' step records
+        (in the order of addition) or None."""
+        n = len(self.models)
+        if not info:
+            check(self._lib.relax_batch_step(self._h, None), "batch step")
+            return None
+        recs = (StepInfo * max(n, 1))()
+        check(self._lib.relax_batch_step(self._h, recs), "batch step")
+        return [_info(recs[k]) for k in range(n)]
+
+    def synchronize(self):
+        check(self._lib.relax_batch_synchronize(self._h), "batch synchronize")
+
+    def busy(self):
+        """True while the batch has queued work (does not wait)."""
+        return bool(self._lib.relax_batch_busy(self._h))
+
+    def scalars(self):
+        """The members' current step records (waits for queued work)."""
+        n = len(self.models)
+        recs = (StepInfo * max(n, 1))()
+        if n:
+            check(self._lib.relax_batch_scalars(self._h, recs), "batch scalars")
+        return [_info(recs[k]) for k in range(n)]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.relax_batch_destroy(self._h)
+            self._h = None
+            self.models = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
 
 # kernel kinds of relax_model_profile, in the order of RELAX_PROF_* (relax_hip.h)
 PROFILE_KINDS = ("bodyforce", "fft_x_fwd", "fft_y", "green_z", "fft_x_inv", "stress_pred", "stress_corr", "small")

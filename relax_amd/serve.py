@@ -1,0 +1,246 @@
+"""Serving forward-model requests with continuous batching.
+
+A request is one RELAX forward model: a grid, a rheology, coseismic slip
+patches, optionally an afterslip plane, and a number of RK2 steps.  The
+server keeps every admitted model's state (u, sigma, tau, gamma, fault
+patches) on the device from its coseismic set-up to its last step, and steps
+the models of one grid class together as a batch (relax_batch_*: shared
+launches replayed from a hipGraph).  Each grid class has its own batch and
+stream, and a batch takes its next step as soon as its previous one is done,
+so the classes overlap on the device.  Requests are admitted in arrival order
+while their HBM footprint (relax_model_footprint) fits the budget, joining
+their class's batch between two of its steps; finished requests leave, return
+their surface displacement, and free their memory for the queue.
+
+The baseline runs the same requests back to back through the single-model
+path: set-up, steps, surface displacement, release -- one at a time.
+"""
+import ctypes
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+from . import workloads
+from ._lib import RelaxError, check, load
+from .model import Batch, footprint
+
+# grid classes of the request stream: (sx1, sx2, sx3, dx) and their weights
+GRIDS = ((128, 128, 64, 0.5), (256, 256, 128, 1.0), (512, 512, 256, 0.5))
+WEIGHTS = (6, 3, 1)
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float       # seconds after the start of the stream
+    spec: dict           # a workload description (relax_amd.workloads format)
+    nsteps: int
+    grid: int            # index into GRIDS
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    arrival: float
+    start: float
+    finish: float
+    nsteps: int
+    t: float             # model time after the last step
+    status: int
+    surface: tuple       # (u1, u2, u3) at x3 = 0, (sx2, sx1) each
+
+    @property
+    def latency(self):
+        return self.finish - self.arrival
+
+
+def request_spec(rng, gi, afterslip):
+    """One request: rheology as the batched-model workload (log-uniform eta in
+    [1e17, 1e20] Pa s, n in {1, 2, 3, 3.5}, a 15-40 km lid), 1-10 patches of
+    0.5-5 m slip, and on request an afterslip plane with (a-b)sigma = 1 MPa,
+    v0 = 1 m/yr, dip 20, rake 90, away from the slip."""
+    sx1, sx2, sx3, dx = GRIDS[gi]
+    half = 0.5 * sx1 * dx
+    n = [1.0, 2.0, 3.0, 3.5][rng.integers(0, 4)]
+    eta = 10 ** rng.uniform(17, 20)
+    lid = rng.uniform(15, 40)
+    lay = [(lid, workloads.gdot0(eta, n), n, 0.0, 0.6, 0.0)]
+    deck = []
+    for _ in range(int(rng.integers(1, 11))):
+        p = list(workloads._deck_patch(rng, -0.4 * half, 0.0, min(10.0, 0.25 * (sx3 // 2) * dx), 5.0, 5e-3))
+        p[0] = rng.uniform(0.5e-3, 5e-3)
+        deck.append(tuple(p))
+    spec = dict(name="request", sx1=sx1, sx2=sx2, sx3=sx3, dx=dx, linear=lay if n == 1.0 else None,
+                nonlinear=None if n == 1.0 else lay, friction=None, planes=None, deck=deck, interval=1e9, odt=1.0,
+                skip=1, tscale=1.0)
+    if afterslip:
+        L = 0.25 * sx1 * dx
+        W = 0.5 * L
+        dip = 20.0
+        spec["friction"] = [(0.0, 1e-3, 1.0, 0.0, 0.6, 0.0)]
+        spec["planes"] = [dict(x=0.3 * half, y=0.3 * half, z=1.0, length=L, width=W, strike=0.0, dip=dip,
+                               rake=90.0)]
+    return spec
+
+
+def request_stream(n, rate=20.0, seed=20261206, afterslip_fraction=0.3, steps=(10, 30)):
+    """Seeded Poisson arrivals (mean `rate` per second) of n requests over the
+    grid classes in the ratio 6:3:1, with 10-30 RK2 steps each."""
+    rng = np.random.default_rng(seed)
+    p = np.asarray(WEIGHTS, float) / sum(WEIGHTS)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        gi = int(rng.choice(len(GRIDS), p=p))
+        spec = request_spec(rng, gi, rng.uniform() < afterslip_fraction)
+        out.append(Request(i, t, spec, int(rng.integers(steps[0], steps[1] + 1)), gi))
+    return out
+
+
+def meminfo():
+    """(free, total) bytes of device memory."""
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    check(load().relax_hip_meminfo(ctypes.byref(f), ctypes.byref(t)), "meminfo")
+    return f.value, t.value
+
+
+def _surface(m, s):
+    sx1, sx2 = s["sx1"], s["sx2"]
+    u = [np.zeros((sx2, sx1), np.float32) for _ in range(3)]
+    m.surface(*u)
+    return tuple(u)
+
+
+def _footprint(s):
+    return footprint(s["sx1"], s["sx2"], s["sx3"])
+
+
+def run_back_to_back(requests, realtime=True):
+    """Each request alone through the single-model path, in arrival order."""
+    t0 = time.perf_counter()
+    results = []
+    for r in requests:
+        now = time.perf_counter() - t0
+        if realtime and now < r.arrival:
+            time.sleep(r.arrival - now)
+        start = time.perf_counter() - t0
+        m = workloads.build(r.spec)
+        status = 0
+        info = None
+        for _ in range(r.nsteps - 1):
+            m.step(info=False)
+        try:
+            info = m.step()
+        except RelaxError as e:  # a step's error status (e.g. fault creep overflow)
+            status = e.status
+        surf = _surface(m, r.spec)
+        m.close()
+        finish = time.perf_counter() - t0
+        results.append(Result(r.rid, r.arrival if realtime else 0.0, start, finish, r.nsteps,
+                              info["t"] if info else math.nan, info["status"] if info else status, surf))
+    return results
+
+
+class Server:
+    """Continuous batching over grid classes (see the module docstring)."""
+
+    def __init__(self, budget_bytes=None, reserve=0.1, admit_per_pass=2):
+        self.admit_per_pass = admit_per_pass
+        if budget_bytes is None:
+            free, _ = meminfo()
+            budget_bytes = int(free * (1.0 - reserve))
+        self.budget = budget_bytes
+        self.used = 0
+        self.batches = {}   # grid class -> Batch
+        self.members = {}   # grid class -> list of [request, model, steps taken, start]
+
+    def _admit(self, r, now):
+        m = workloads.build(r.spec)
+        b = self.batches.get(r.grid)
+        if b is None:
+            b = self.batches[r.grid] = Batch()
+            self.members[r.grid] = []
+        b.add(m)
+        self.members[r.grid].append([r, m, 0, now])
+        self.used += _footprint(r.spec)
+
+    def run(self, requests, realtime=True, poll=2e-4):
+        pending = sorted(requests, key=lambda r: r.arrival)
+        queue = []
+        results = []
+        t0 = time.perf_counter()
+        n = len(pending)
+        while len(results) < n:
+            now = time.perf_counter() - t0
+            while pending and (not realtime or pending[0].arrival <= now):
+                queue.append(pending.pop(0))
+            worked = False
+            # classes whose previous step is done: retire finished members
+            ready = set()
+            for gi, b in self.batches.items():
+                if b.busy():
+                    continue
+                ready.add(gi)
+                mem = self.members[gi]
+                if not mem:
+                    continue
+                recs = b.scalars()
+                keep = []
+                for (r, m, k, start), rec in zip(mem, recs):
+                    if k >= r.nsteps or rec["status"]:
+                        b.remove(m)
+                        surf = _surface(m, r.spec)
+                        m.close()
+                        self.used -= _footprint(r.spec)
+                        results.append(Result(r.rid, r.arrival if realtime else 0.0, start,
+                                              time.perf_counter() - t0, k, rec["t"], rec["status"], surf))
+                        worked = True
+                    else:
+                        keep.append([r, m, k, start])
+                self.members[gi] = keep
+            # admission in arrival order while the footprint fits, into idle
+            # classes only (a busy class takes its arrivals after its step);
+            # a few per pass, so that set-up on the host overlaps the steps
+            admitted = 0
+            i = 0
+            while i < len(queue) and admitted < self.admit_per_pass:
+                r = queue[i]
+                if r.grid in self.batches and r.grid not in ready:
+                    i += 1
+                    continue
+                if self.used + _footprint(r.spec) > self.budget:
+                    break
+                queue.pop(i)
+                self._admit(r, time.perf_counter() - t0)
+                ready.add(r.grid)
+                admitted += 1
+                worked = True
+            # next step of every idle class
+            for gi in ready:
+                mem = self.members[gi]
+                if mem:
+                    self.batches[gi].step(info=False)
+                    for e in mem:
+                        e[2] += 1
+                    worked = True
+            if not worked:
+                if not any(self.members.values()) and pending and realtime:
+                    time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.05)))
+                else:
+                    time.sleep(poll)
+        for b in self.batches.values():
+            b.close()
+        self.batches.clear()
+        return results
+
+
+def summarize(results, label):
+    lat = np.array([r.latency for r in results])
+    span = max(r.finish for r in results) - min(r.arrival for r in results)
+    ok = sum(1 for r in results if r.status == 0)
+    return {"path": label, "requests": len(results), "ok": ok, "solves_per_s": round(len(results) / span, 3),
+            "makespan_s": round(span, 3), "p50_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_s": round(float(np.percentile(lat, 99)), 4)}

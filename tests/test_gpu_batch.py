@@ -1,0 +1,121 @@
+"""Batches: many models of one grid stepped by shared launches and a captured
+hipGraph (relax_batch_* of relax_hip.h).  Each member's result must be the
+one it gets when stepped alone -- bitwise, since the launches run the same
+per-model work -- whatever the other members are, and across membership
+changes and graph replays with eager work in between."""
+import numpy as np
+import pytest
+
+from relax_amd import RelaxError, workloads
+from relax_amd.model import Batch
+
+from _cases import small_spec
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("u1", "u2", "u3", "tau", "sig", "gamma")
+
+
+def _variant(kind, seed, sx3=64):
+    s = small_spec(kind, sx3=sx3, seed=seed)
+    if kind == "maxwell":
+        # a different viscosity per seed
+        s["linear"] = [(6.0, 0.2 * (1 + seed), 1.0, 0.0, 0.6, 0.0)]
+    return s
+
+
+def _alone(s, nsteps):
+    m = workloads.build(s)
+    recs = [m.step() for _ in range(nsteps)]
+    out = m.download()
+    pat = m.get_patches() if s["planes"] else None
+    m.close()
+    return recs, out, pat
+
+
+def _same(a, b):
+    for k in FIELDS:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_batch_members_match_their_single_runs_bitwise():
+    specs = [_variant("maxwell", 0), _variant("powerlaw", 1), _variant("maxwell", 2)]
+    nsteps = 4
+    ref = [_alone(s, nsteps) for s in specs]
+    models = [workloads.build(s) for s in specs]
+    with Batch(models) as b:
+        assert len(b) == 3
+        for k in range(nsteps):
+            recs = b.step()
+            for i in range(3):
+                assert recs[i] == ref[i][0][k], (i, k)
+            # eager work on one member between replays of the captured step
+            models[1].download()
+    for i, m in enumerate(models):
+        _same(m.download(), ref[i][1])
+        m.close()
+
+
+def test_batch_with_fault_creep_member():
+    """An afterslip member switches the batch to the fault-creep kernels; the
+    viscous members then run those kernels too, so they agree to rounding."""
+    specs = [_variant("afterslip", 0), _variant("maxwell", 1)]
+    nsteps = 3
+    ref = [_alone(s, nsteps) for s in specs]
+    models = [workloads.build(s) for s in specs]
+    with Batch(models) as b:
+        for k in range(nsteps):
+            recs = b.step()
+            for i in range(2):
+                assert abs(recs[i]["t"] - ref[i][0][k]["t"]) <= 1e-12 * abs(ref[i][0][k]["t"])
+                assert abs(recs[i]["power"] / ref[i][0][k]["power"] - 1) < 1e-5
+    for i, m in enumerate(models):
+        out = m.download()
+        for key in FIELDS:
+            a, r = out[key].astype(np.float64), ref[i][1][key].astype(np.float64)
+            nr = np.linalg.norm(r)
+            assert np.linalg.norm(a - r) <= 1e-6 * (nr if nr > 0 else 1.0), (i, key)
+        if i == 0:
+            pat = m.get_patches()
+            np.testing.assert_allclose(pat["slip"], ref[0][2]["slip"], rtol=1e-5, atol=1e-12)
+        m.close()
+
+
+def test_batch_membership_changes():
+    """A member leaves after two steps and continues alone; a new one joins
+    for the last steps.  Each trajectory equals its single run bitwise."""
+    sa, sb, sc = _variant("maxwell", 0), _variant("powerlaw", 1), _variant("maxwell", 3)
+    ref_a = _alone(sa, 4)
+    ref_b = _alone(sb, 4)
+    ref_c = _alone(sc, 2)
+    a, bm, c = workloads.build(sa), workloads.build(sb), workloads.build(sc)
+    with Batch([a, bm]) as b:
+        b.step(info=False)
+        b.step(info=False)
+        b.remove(bm)
+        b.add(c)
+        bm.step()
+        bm.step()
+        b.step(info=False)
+        b.step(info=False)
+        b.synchronize()
+        _same(a.download(), ref_a[1])
+        _same(bm.download(), ref_b[1])
+        _same(c.download(), ref_c[1])
+    for m in (a, bm, c):
+        m.close()
+
+
+def test_batch_refuses_mismatched_models_and_member_steps():
+    a = workloads.build(_variant("maxwell", 0))
+    other = workloads.build(_variant("maxwell", 0, sx3=32))
+    with Batch([a]) as b:
+        with pytest.raises(RelaxError):
+            b.add(other)
+        with pytest.raises(RelaxError):
+            a.step()
+        with pytest.raises(RelaxError):
+            b.add(a)
+    a.step()  # free again once the batch is gone
+    a.close()
+    other.close()
