@@ -164,6 +164,10 @@ typedef struct {
 size_t relax_model_footprint(const relax_model_desc_t* desc);
 int relax_model_create(const relax_model_desc_t* desc, relax_model_t** out);
 int relax_model_destroy(relax_model_t* m);
+/* Back to the state of a new model (zero fields, t = 0, no mechanisms, no
+ * planes) without releasing or allocating device memory, so that a server
+ * can reuse the model for another request of the same grid. */
+int relax_model_reset(relax_model_t* m);
 /* which: 0 = linear viscosity (in%linearstruc), 1 = power law
  * (in%nonlinearstruc), 2 = fault creep (in%faultcreepstruc).  layers are
  * the per-depth samples of viscoelasticstructure (sx3/2 entries), or NULL

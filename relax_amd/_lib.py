@@ -75,6 +75,7 @@ SIGNATURES = {
     "relax_model_footprint": (c_size_t, [P]),
     "relax_model_create": (c_int, [P, P]),
     "relax_model_destroy": (c_int, [P]),
+    "relax_model_reset": (c_int, [P]),
     "relax_model_set_structure": (c_int, [P, c_int, P, c_int]),
     "relax_model_set_prestress": (c_int, [P, P, c_int]),
     "relax_model_set_planes": (c_int, [P, P, c_int, P]),

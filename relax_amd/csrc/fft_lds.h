@@ -1,7 +1,8 @@
 // fft_lds.h -- batched power-of-two complex FFTs held in LDS.
 //
-// Stockham autosort (decimation in frequency) with radix-8 stages and one
-// trailing radix-4 or radix-2 stage (512 = 8*8*8, 256 = 8*8*4, 128 = 8*8*2).
+// Stockham autosort (decimation in frequency) with radix-8 (or radix-16)
+// stages and one trailing smaller stage (512 = 8*8*8, 256 = 8*8*4,
+// 128 = 8*8*2, or with radix 16: 128 = 16*8, 256 = 16*16).
 // The transform runs in place: every thread reads all inputs of its
 // butterflies into registers, the workgroup synchronises, then every thread
 // writes its outputs.
@@ -57,9 +58,42 @@ __device__ __forceinline__ void dft8(float2* a) {
   a[2] = cadd(e2, w2); a[6] = csub(e2, w2);
   a[3] = cadd(e3, w3); a[7] = csub(e3, w3);
 }
+// 16 = 4 x 4: 4-point DFTs over r2 of a[r1 + 4 r2], the twiddles
+// w16^(r1 t1), then 4-point DFTs over r1; output t1 + 4 t2 in natural order.
+template <int SIGN>
+__device__ __forceinline__ void dft16(float2* a) {
+  // cos and sin of 2 pi k / 16, k = 0..9 (the products r1*t1 that occur)
+  constexpr float C[10] = {1.f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f, 0.f,
+                           -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f, -1.f,
+                           -0.92387953251128674f};
+  constexpr float S[10] = {0.f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f, 1.f,
+                           0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f, 0.f,
+                           -0.38268343236508977f};
+  float2 b[4][4];
+#pragma unroll
+  for (int r1 = 0; r1 < 4; ++r1) {
+    b[r1][0] = a[r1]; b[r1][1] = a[r1 + 4]; b[r1][2] = a[r1 + 8]; b[r1][3] = a[r1 + 12];
+    dft4<SIGN>(b[r1][0], b[r1][1], b[r1][2], b[r1][3]);
+#pragma unroll
+    for (int t1 = 1; t1 < 4; ++t1) {
+      if (r1 == 0) continue;
+      const int k = r1 * t1;
+      const float2 w = make_float2(C[k], SIGN * S[k]);
+      b[r1][t1] = cmul(b[r1][t1], w);
+    }
+  }
+#pragma unroll
+  for (int t1 = 0; t1 < 4; ++t1) {
+    float2 c0 = b[0][t1], c1 = b[1][t1], c2 = b[2][t1], c3 = b[3][t1];
+    dft4<SIGN>(c0, c1, c2, c3);
+    a[t1] = c0; a[t1 + 4] = c1; a[t1 + 8] = c2; a[t1 + 12] = c3;
+  }
+}
+
 template <int R, int SIGN>
 __device__ __forceinline__ void dft(float2* a) {
-  if (R == 8) dft8<SIGN>(a);
+  if (R == 16) dft16<SIGN>(a);
+  else if (R == 8) dft8<SIGN>(a);
   else if (R == 4) dft4<SIGN>(a);
   else dft2<SIGN>(a);
 }
@@ -105,22 +139,23 @@ __device__ __forceinline__ void lds_stage(float2* buf, const float2* __restrict_
   __syncthreads();
 }
 
-template <int L, int N, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB>
+template <int L, int N, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB, int RMAX>
 struct LdsStages {
   static __device__ __forceinline__ void run(float2* buf, const float2* __restrict__ tw, int tid) {
-    constexpr int R = N >= 8 ? 8 : N;
+    constexpr int R = N >= RMAX ? RMAX : N;
     lds_stage<L, N, R, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB>(buf, tw, tid);
-    LdsStages<L, N / R, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB>::run(buf, tw, tid);
+    LdsStages<L, N / R, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB, RMAX>::run(buf, tw, tid);
   }
 };
-template <int L, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB>
-struct LdsStages<L, 1, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB> {
+template <int L, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB, int RMAX>
+struct LdsStages<L, 1, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB, RMAX> {
   static __device__ __forceinline__ void run(float2*, const float2* __restrict__, int) {}
 };
 
 // MAPB: consecutive threads take consecutive batches (use when S0 == 1);
 // otherwise consecutive threads take consecutive butterflies of one batch.
-template <int L, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB>
+// RMAX: the largest radix (8, or 16 for fewer stages and LDS passes).
+template <int L, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB, int RMAX = 8>
 __device__ __forceinline__ void lds_fft(float2* buf, const float2* __restrict__ tw, int tid) {
-  LdsStages<L, L, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB>::run(buf, tw, tid);
+  LdsStages<L, L, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB, RMAX>::run(buf, tw, tid);
 }

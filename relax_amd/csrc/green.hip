@@ -138,6 +138,10 @@ __device__ __forceinline__ void cerruti_point(const ColumnConsts& C, const Cerru
 }  // namespace
 
 // ---------------------------------------------------- fused column pass ----
+// largest radix of the column transforms (compile-time choice)
+#ifndef GREEN_RMAX
+#define GREEN_RMAX 8
+#endif
 template <int H3, int TX, int NT>
 __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu, double alpha, double gamma,
                                                       int n1, int n2, int n3, MPtr<float2> W_m, MPtr<float2> NQ_m,
@@ -196,7 +200,7 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
     s[(2 * c + 1) * SB + z * TX + t] = cmul(x, tw3[z]);
   }
   __syncthreads();
-  lds_fft<H3, 6 * TX, NT, TX, 1, SB, TX, -1, 2, true>(s, stw, tid);
+  lds_fft<H3, 6 * TX, NT, TX, 1, SB, TX, -1, 2, true, GREEN_RMAX>(s, stw, tid);
 
   // Green's function and the traction partial sums of this thread's column
   float2 ps1 = make_float2(0.f, 0.f), ps2 = ps1, ps3 = ps1;
@@ -254,7 +258,7 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
     }
   }
   __syncthreads();
-  lds_fft<H3, 6 * TX, NT, TX, 1, SB, TX, +1, 2, true>(s, stw, tid);
+  lds_fft<H3, 6 * TX, NT, TX, 1, SB, TX, +1, 2, true, GREEN_RMAX>(s, stw, tid);
   // upper half of the inverse: y[n] = E'[n] + w^-n O'[n]
   for (int e = tid; e < 3 * H3 * TX; e += NT) {
     const int t = e % TX, z = (e / TX) % H3, c = e / (TX * H3);
@@ -266,12 +270,15 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
 
 template <int H3>
 struct GreenTile {
-  static constexpr int TX = H3 >= 256 ? 4 : (H3 >= 128 ? 8 : 16);
+#ifndef RELAX_GREEN_TX128
+#define RELAX_GREEN_TX128 8
+#endif
+  static constexpr int TX = H3 >= 256 ? 4 : (H3 >= 128 ? RELAX_GREEN_TX128 : 16);
 };
 
 // threads per workgroup of the column pass (compile-time choice)
 #ifndef RELAX_GREEN_NT
-#define RELAX_GREEN_NT 256
+#define RELAX_GREEN_NT 512
 #endif
 int launch_green_columns(const GreenConst& gc, int h3, MPtr<float2> W, MPtr<float2> NQ, MPtr<const float2> TM,
                          MPtr<const float2> TN, int nm, hipStream_t st, int l2) {

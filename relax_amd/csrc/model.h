@@ -30,6 +30,11 @@ struct relax_model {
 
   PlaneDev* planes_dev = nullptr;
   relax_slippatch_t* patches_dev = nullptr;
+  long planes_cap = 0, patches_cap = 0;  // grow-only: a reused model allocates nothing
+  PlaneDev* shear_dev = nullptr;         // add_shear's patch list
+  double* shear_slip = nullptr;
+  long shear_cap = 0;
+  std::vector<PlaneDev> shear_host;
   std::vector<relax_plane_t> host_planes;
   std::vector<PlaneDev> plane_dev_host;
   std::vector<long> patch_off;

@@ -250,7 +250,7 @@ size_t relax_model_footprint(const relax_model_desc_t* d) {
 static void free_model(relax_model_t* m) {
   if (!m) return;
   void* ptrs[] = {m->u, m->sig, m->tau, m->gam, m->W, m->NQ, m->T, m->TN, m->sc, m->partials, m->tables, m->ftab,
-                  m->planes_dev, m->patches_dev, m->md_dev};
+                  m->planes_dev, m->patches_dev, m->md_dev, m->shear_dev, m->shear_slip};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->host_sc) (void)hipHostFree(m->host_sc);
@@ -383,6 +383,34 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
   return RELAX_OK;
 }
 
+int relax_model_reset(relax_model_t* m) {
+  if (!m) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
+  const long H = m->g.H, h3 = m->g.h3;
+  HIP_TRY(hipMemsetAsync(m->u, 0, 3 * H * 4, m->stream));
+  HIP_TRY(hipMemsetAsync(m->sig, 0, 6 * H * 4, m->stream));
+  HIP_TRY(hipMemsetAsync(m->tau, 0, 6 * H * 4, m->stream));
+  HIP_TRY(hipMemsetAsync(m->gam, 0, H * 4, m->stream));
+  HIP_TRY(hipMemsetAsync(m->tables, 0, 19L * h3 * 4, m->stream));
+  HIP_TRY(hipMemsetAsync(m->ftab, 0, 4L * h3 * 8, m->stream));
+  k_reset_scalars<<<1, 1, 0, m->stream>>>(m->sc, 0.0, 1, 0);
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  for (int k = 0; k < 2; ++k) {
+    m->R.mech[k].active = 0;
+    m->hg0[k].clear();
+    m->hn[k].clear();
+  }
+  m->htaup.clear();
+  for (int k = 0; k < 3; ++k) m->mech[k] = 0;
+  m->host_planes.clear();
+  m->plane_dev_host.clear();
+  m->patch_off.assign(1, 0);
+  m->R.fr.np = 0;
+  m->R.fr.active = 0;
+  m->md_dirty = true;
+  return sync_md(m);
+}
+
 int relax_model_destroy(relax_model_t* m) {
   if (!m) return RELAX_ERR_ARG;
   if (m->batch) (void)relax_batch_remove(m->batch, m);
@@ -512,8 +540,6 @@ PlaneDev make_plane(const relax_plane_t& p, double beta, double dx1, double dx3)
 int relax_model_set_planes(relax_model_t* m, const relax_plane_t* planes, int np, const relax_slippatch_t* patches) {
   if (!m || np < 0 || (np > 0 && !planes)) return RELAX_ERR_ARG;
   if (int rc = wait_batch(m)) return rc;
-  if (m->planes_dev) { (void)hipFree(m->planes_dev); m->planes_dev = nullptr; }
-  if (m->patches_dev) { (void)hipFree(m->patches_dev); m->patches_dev = nullptr; }
   m->host_planes.assign(planes, planes + np);
   m->plane_dev_host.clear();
   m->patch_off.assign(np + 1, 0);
@@ -521,16 +547,33 @@ int relax_model_set_planes(relax_model_t* m, const relax_plane_t* planes, int np
     m->plane_dev_host.push_back(make_plane(planes[k], m->desc.beta, m->desc.dx1, m->desc.dx3));
     m->patch_off[k + 1] = m->patch_off[k] + planes[k].px2 * planes[k].px3;
   }
-  if (np > 0) {
+  // buffers only grow, so a model reused for another request allocates
+  // (and frees, which would wait for the whole device) nothing
+  const long npatch = m->patch_off[np];
+  if (np > m->planes_cap) {
+    if (m->planes_dev) (void)hipFree(m->planes_dev);
+    m->planes_dev = nullptr;
+    m->planes_cap = 0;
     HIP_TRY(hipMalloc(&m->planes_dev, sizeof(PlaneDev) * np));
-    HIP_TRY(hipMemcpy(m->planes_dev, m->plane_dev_host.data(), sizeof(PlaneDev) * np, hipMemcpyHostToDevice));
-    const long npatch = m->patch_off[np];
-    if (npatch > 0) {
-      HIP_TRY(hipMalloc(&m->patches_dev, sizeof(relax_slippatch_t) * npatch));
-      if (patches) HIP_TRY(hipMemcpy(m->patches_dev, patches, sizeof(relax_slippatch_t) * npatch, hipMemcpyHostToDevice));
-      else HIP_TRY(hipMemset(m->patches_dev, 0, sizeof(relax_slippatch_t) * npatch));
-    }
+    m->planes_cap = np;
   }
+  if (npatch > m->patches_cap) {
+    if (m->patches_dev) (void)hipFree(m->patches_dev);
+    m->patches_dev = nullptr;
+    m->patches_cap = 0;
+    HIP_TRY(hipMalloc(&m->patches_dev, sizeof(relax_slippatch_t) * npatch));
+    m->patches_cap = npatch;
+  }
+  if (np > 0)
+    HIP_TRY(hipMemcpyAsync(m->planes_dev, m->plane_dev_host.data(), sizeof(PlaneDev) * np, hipMemcpyHostToDevice,
+                           m->stream));
+  if (npatch > 0) {
+    if (patches)
+      HIP_TRY(hipMemcpyAsync(m->patches_dev, patches, sizeof(relax_slippatch_t) * npatch, hipMemcpyHostToDevice,
+                             m->stream));
+    else HIP_TRY(hipMemsetAsync(m->patches_dev, 0, sizeof(relax_slippatch_t) * npatch, m->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(m->stream));
   m->R.fr.planes = m->planes_dev;
   m->R.fr.np = np;
   m->R.fr.active = (np > 0) && m->mech[2];
@@ -888,24 +931,25 @@ int relax_model_add_shear(relax_model_t* m, const relax_plane_t* patches, const 
   if (!m || n < 0 || (n > 0 && (!patches || !slip))) return RELAX_ERR_ARG;
   if (int rc = wait_batch(m)) return rc;
   if (n == 0) return RELAX_OK;
-  std::vector<PlaneDev> hp(n);
+  std::vector<PlaneDev>& hp = m->shear_host;
+  hp.resize(n);
   for (int k = 0; k < n; ++k) hp[k] = make_plane(patches[k], m->desc.beta, m->desc.dx1, m->desc.dx3);
-  PlaneDev* dp = nullptr;
-  double* ds = nullptr;
-  HIP_TRY(hipMalloc(&dp, sizeof(PlaneDev) * n));
-  if (hipMalloc(&ds, sizeof(double) * n) != hipSuccess) { (void)hipFree(dp); return RELAX_ERR_NOMEM; }
-  int rc = RELAX_OK;
-  if (hipMemcpyAsync(dp, hp.data(), sizeof(PlaneDev) * n, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
-      hipMemcpyAsync(ds, slip, sizeof(double) * n, hipMemcpyHostToDevice, m->stream) != hipSuccess)
-    rc = RELAX_ERR_HIP;
-  if (!rc) {
-    k_add_shear<<<grid_1d(m->g.H), 256, 0, m->stream>>>(dp, ds, n, 2.0f * (float)m->desc.mu, m->g, m->desc.dx1,
-                                                        m->desc.dx2, m->desc.dx3, m->tau);
-    if (hipGetLastError() != hipSuccess) rc = RELAX_ERR_HIP;
+  if (n > m->shear_cap) {  // grow-only, as the plane buffers
+    if (m->shear_dev) (void)hipFree(m->shear_dev);
+    if (m->shear_slip) (void)hipFree(m->shear_slip);
+    m->shear_dev = nullptr;
+    m->shear_slip = nullptr;
+    m->shear_cap = 0;
+    HIP_TRY(hipMalloc(&m->shear_dev, sizeof(PlaneDev) * n));
+    HIP_TRY(hipMalloc(&m->shear_slip, sizeof(double) * n));
+    m->shear_cap = n;
   }
+  HIP_TRY(hipMemcpyAsync(m->shear_dev, hp.data(), sizeof(PlaneDev) * n, hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(hipMemcpyAsync(m->shear_slip, slip, sizeof(double) * n, hipMemcpyHostToDevice, m->stream));
+  k_add_shear<<<grid_1d(m->g.H), 256, 0, m->stream>>>(m->shear_dev, m->shear_slip, n, 2.0f * (float)m->desc.mu, m->g,
+                                                      m->desc.dx1, m->desc.dx2, m->desc.dx3, m->tau);
+  int rc = hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
   if (hipStreamSynchronize(m->stream) != hipSuccess && !rc) rc = RELAX_ERR_HIP;
-  (void)hipFree(dp);
-  (void)hipFree(ds);
   return rc;
 }
 
@@ -1040,10 +1084,11 @@ static int batch_refresh(relax_batch_t* b) {
     if (b->gather) { (void)hipFree(b->gather); b->gather = nullptr; }
     if (b->host_sc) { (void)hipHostFree(b->host_sc); b->host_sc = nullptr; }
     b->cap = 0;
-    HIP_TRY(hipMalloc(&b->md_dev, sizeof(ModelDev) * nm));
-    HIP_TRY(hipMalloc(&b->gather, sizeof(StepScalars) * nm));
-    HIP_TRY(hipHostMalloc((void**)&b->host_sc, sizeof(StepScalars) * nm));
-    b->cap = nm;
+    const int cap = nm < 8 ? 8 : 2 * nm;  // room to grow without reallocating
+    HIP_TRY(hipMalloc(&b->md_dev, sizeof(ModelDev) * cap));
+    HIP_TRY(hipMalloc(&b->gather, sizeof(StepScalars) * cap));
+    HIP_TRY(hipHostMalloc((void**)&b->host_sc, sizeof(StepScalars) * cap));
+    b->cap = cap;
   }
   b->md_host.resize(nm);
   b->versions.resize(nm);

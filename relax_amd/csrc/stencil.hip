@@ -16,23 +16,15 @@
 #include "rheology.h"
 #include "stencil.h"
 
-// BATCH: the models' records are in the device array md (one model per
-// nx*ny*gz tiles); otherwise the one model's record is the argument Dv.
-template <int TXB, int TYB, bool TM, bool RAW, bool FRIC, bool BATCH>
-__global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, ModelDev Dv, const ModelDev* __restrict__ md,
-                                                        StencilConst S, int zc, int gz) {
-  // 1-D grid of tiles (x fastest, then y, depth chunk, model), taken in the
-  // XCD-aware order so that neighbouring tiles, which share halo lines, meet
-  // in one L2
-  const int nx = g.n1 / TXB, ny = g.n2 / TYB;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = tile % nx, by = (tile / nx) % ny, bz = tile / (nx * ny);
-  const ModelDev& D = BATCH ? md[bz / gz] : Dv;
-  const RheoParams& R = D.R;
-  const float* __restrict__ sig = RAW ? D.tau : D.sig;
-  float* __restrict__ W = D.W;
-  float* __restrict__ T = D.T;
-  StepScalars* sc = D.sc;
+// The kernel bodies take the model's buffers as __restrict__ parameters:
+// after inlining, the compiler keeps them as alias scopes, so the per-depth
+// rheology tables (read-only) are known not to alias the stores and are
+// read through the scalar cache.
+template <int TXB, int TYB, bool TM, bool RAW, bool FRIC>
+__device__ __forceinline__ void bodyforce_tile(const Grid& g, const RheoParams& R, const StencilConst& S,
+                                               const float* __restrict__ sig, float* __restrict__ W,
+                                               float* __restrict__ T, StepScalars* sc, int bx, int by, int bz,
+                                               int zc, int gz) {
   constexpr int NT = TXB * TYB;
   constexpr int HX = TXB + 2, HY = TYB + 2, HP = HX * HY;
   __shared__ float ring[3][6][HP];
@@ -139,18 +131,26 @@ __global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, ModelDev Dv, con
   }
 }
 
-template <int MODE, bool FRIC, bool BATCH>
-__global__ __launch_bounds__(256) void k_stress(Grid g, ModelDev Dv, const ModelDev* __restrict__ md,
-                                               StencilConst S) {
-  // model blockIdx.y of the launch (BATCH) or the argument Dv
-  const ModelDev& D = BATCH ? md[blockIdx.y] : Dv;
-  const RheoParams& R = D.R;
-  const float* __restrict__ X = MODE == STRESS_PRED ? D.W : D.u;
-  float* sig = D.sig;
-  float* tau = D.tau;
-  float* gam = D.gam;
-  StepScalars* sc = D.sc;
-  double2* partials = D.partials;
+// BATCH: the models' records are in the device array md (one model per
+// nx*ny*gz tiles); otherwise the one model's record is the argument Dv.
+template <int TXB, int TYB, bool TM, bool RAW, bool FRIC, bool BATCH>
+__global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, ModelDev Dv, const ModelDev* __restrict__ md,
+                                                        StencilConst S, int zc, int gz) {
+  // 1-D grid of tiles (x fastest, then y, depth chunk, model), taken in the
+  // XCD-aware order so that neighbouring tiles, which share halo lines, meet
+  // in one L2
+  const int nx = g.n1 / TXB, ny = g.n2 / TYB;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % nx, by = (tile / nx) % ny, bz = tile / (nx * ny);
+  const ModelDev& D = BATCH ? md[bz / gz] : Dv;
+  bodyforce_tile<TXB, TYB, TM, RAW, FRIC>(g, D.R, S, RAW ? D.tau : D.sig, D.W, D.T, D.sc, bx, by, bz, zc, gz);
+}
+
+template <int MODE, bool FRIC>
+__device__ __forceinline__ void stress_block(const Grid& g, const RheoParams& R, const StencilConst& S,
+                                             const float* __restrict__ X, float* __restrict__ sig,
+                                             float* __restrict__ tau, float* __restrict__ gam, StepScalars* sc,
+                                             double2* __restrict__ partials) {
   __shared__ double2 red[4];
   const int tid = threadIdx.x;
   const long o = (long)blockIdx.x * 256 + tid;
@@ -236,6 +236,14 @@ __global__ __launch_bounds__(256) void k_stress(Grid g, ModelDev Dv, const Model
     }
   }
   if (status) atomicMax(&sc->status, status);
+}
+
+template <int MODE, bool FRIC, bool BATCH>
+__global__ __launch_bounds__(256) void k_stress(Grid g, ModelDev Dv, const ModelDev* __restrict__ md,
+                                               StencilConst S) {
+  // model blockIdx.y of the launch (BATCH) or the argument Dv
+  const ModelDev& D = BATCH ? md[blockIdx.y] : Dv;
+  stress_block<MODE, FRIC>(g, D.R, S, MODE == STRESS_PRED ? D.W : D.u, D.sig, D.tau, D.gam, D.sc, D.partials);
 }
 
 int launch_bodyforce(const Grid& g, const StencilConst& S, const ModelDev* md, const ModelDev* one, int nm,

@@ -55,6 +55,10 @@ class Model:
     def handle(self):
         return self._h
 
+    def reset(self):
+        """Back to a new model's state, keeping the device memory."""
+        check(self._lib.relax_model_reset(self._h), "reset")
+
     def set_structure(self, which, layers):
         if layers is None:
             check(self._lib.relax_model_set_structure(self._h, which, None, 0), "set_structure")

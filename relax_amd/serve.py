@@ -29,6 +29,10 @@ from .model import Batch, footprint
 # grid classes of the request stream: (sx1, sx2, sx3, dx) and their weights
 GRIDS = ((128, 128, 64, 0.5), (256, 256, 128, 1.0), (512, 512, 256, 0.5))
 WEIGHTS = (6, 3, 1)
+# A class's batch holds at most this many grid points (over its members):
+# beyond it one batched step already keeps HBM busy, and more members only
+# lengthen every member's step.
+MAX_BATCH_POINTS = 1 << 25
 
 
 @dataclasses.dataclass
@@ -144,8 +148,19 @@ def run_back_to_back(requests, realtime=True):
     return results
 
 
+def batch_cap(gi):
+    sx1, sx2, sx3, _ = GRIDS[gi]
+    return max(1, MAX_BATCH_POINTS // (sx1 * sx2 * sx3))
+
+
 class Server:
-    """Continuous batching over grid classes (see the module docstring)."""
+    """Continuous batching over grid classes (see the module docstring).
+
+    Finished models go back to a pool of their class and are reset for the
+    next request: releasing device memory waits for the whole device, which
+    would stall every class's queued steps.  `used` counts the footprint of
+    every model held, pooled ones included; pooled models of other classes
+    are released when an admission needs their memory."""
 
     def __init__(self, budget_bytes=None, reserve=0.1, admit_per_pass=2):
         self.admit_per_pass = admit_per_pass
@@ -156,16 +171,37 @@ class Server:
         self.used = 0
         self.batches = {}   # grid class -> Batch
         self.members = {}   # grid class -> list of [request, model, steps taken, start]
+        self.pool = {}      # grid class -> idle models
+
+    def _fits(self, r):
+        if self.pool.get(r.grid):
+            return True
+        need = self.used + _footprint(r.spec) - self.budget
+        if need > 0:  # release idle models of other classes
+            for gi, idle in self.pool.items():
+                while idle and need > 0:
+                    m = idle.pop()
+                    m.close()
+                    fp = footprint(*GRIDS[gi][:3])
+                    self.used -= fp
+                    need -= fp
+        return need <= 0
 
     def _admit(self, r, now):
-        m = workloads.build(r.spec)
+        idle = self.pool.get(r.grid)
+        if idle:
+            m = idle.pop()
+            m.reset()
+            workloads.configure(m, r.spec)
+        else:
+            m = workloads.build(r.spec)
+            self.used += _footprint(r.spec)
         b = self.batches.get(r.grid)
         if b is None:
             b = self.batches[r.grid] = Batch()
             self.members[r.grid] = []
         b.add(m)
         self.members[r.grid].append([r, m, 0, now])
-        self.used += _footprint(r.spec)
 
     def run(self, requests, realtime=True, poll=2e-4):
         pending = sorted(requests, key=lambda r: r.arrival)
@@ -193,8 +229,7 @@ class Server:
                     if k >= r.nsteps or rec["status"]:
                         b.remove(m)
                         surf = _surface(m, r.spec)
-                        m.close()
-                        self.used -= _footprint(r.spec)
+                        self.pool.setdefault(r.grid, []).append(m)
                         results.append(Result(r.rid, r.arrival if realtime else 0.0, start,
                                               time.perf_counter() - t0, k, rec["t"], rec["status"], surf))
                         worked = True
@@ -208,10 +243,11 @@ class Server:
             i = 0
             while i < len(queue) and admitted < self.admit_per_pass:
                 r = queue[i]
-                if r.grid in self.batches and r.grid not in ready:
+                if (r.grid in self.batches and r.grid not in ready) or \
+                        len(self.members.get(r.grid, ())) >= batch_cap(r.grid):
                     i += 1
                     continue
-                if self.used + _footprint(r.spec) > self.budget:
+                if not self._fits(r):
                     break
                 queue.pop(i)
                 self._admit(r, time.perf_counter() - t0)
@@ -234,6 +270,11 @@ class Server:
         for b in self.batches.values():
             b.close()
         self.batches.clear()
+        for idle in self.pool.values():
+            for m in idle:
+                m.close()
+        self.pool.clear()
+        self.used = 0
         return results
 
 

@@ -119,6 +119,12 @@ def build(s):
     dx = s["dx"]
     m = Model(s["sx1"], s["sx2"], s["sx3"], dx, dx, dx, LAM, MU, GAM, BETA, interval=s["interval"], odt=s["odt"],
               tscale=s["tscale"], skip=s["skip"])
+    return configure(m, s)
+
+
+def configure(m, s):
+    """Set up a new (or reset) model of the spec's grid: rheology, planes,
+    coseismic state.  The time line (interval, odt, skip) is the model's."""
     t = tables(s)
     m.set_prestress(t["prestress"])
     m.set_structure(LINEAR, t["linear"])

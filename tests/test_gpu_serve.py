@@ -52,3 +52,28 @@ def test_server_respects_the_memory_budget():
     assert len(res) == len(reqs)
     assert max(peak) <= 2 * fp
     assert all(r.status == 0 for r in res)
+
+
+def test_reset_model_matches_a_new_one():
+    """A pooled model, reset and set up for another request, steps exactly
+    as a freshly created one."""
+    from relax_amd import workloads
+
+    reqs = [r for r in _stream(30, seed=5) if r.grid == 1]
+    assert len(reqs) >= 2
+    a, b = reqs[0], reqs[1]
+    fresh = workloads.build(b.spec)
+    for _ in range(3):
+        fresh.step(info=False)
+    ref = fresh.download()
+    fresh.close()
+    m = workloads.build(a.spec)
+    m.step()
+    m.reset()
+    workloads.configure(m, b.spec)
+    for _ in range(3):
+        m.step(info=False)
+    out = m.download()
+    m.close()
+    for k in ("u1", "u2", "u3", "sig", "tau", "gamma"):
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
