@@ -43,6 +43,11 @@ const float2* twiddle_table(int L) {
 }
 
 // ---------------------------------------------------------------- x R2C ----
+// largest radix of the row transforms (compile-time choice)
+#ifndef RELAX_X_RMAX
+#define RELAX_X_RMAX 8
+#endif
+
 // With YS (the split y transform of the model path), the rows of a plane of
 // n2 rows are taken in groups of 8, rows j1 + L2*j2 (L2 = n2/8), and after
 // the x transform each column of a group gets the first stage of the y
@@ -95,7 +100,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
     s[b * PM + j] = v;
   }
   __syncthreads();
-  lds_fft<M, RPB, NT, RPB, PM, 0, 1, -1, 2, false>(s, stw, tid);
+  lds_fft<M, RPB, NT, RPB, PM, 0, 1, -1, 2, false, RELAX_X_RMAX>(s, stw, tid);
   float2 X[EPT];
 #pragma unroll
   for (int i = 0; i < EPT; ++i) {
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long 
     s[(e / M) * PM + (e % M)] = z[i];
   }
   __syncthreads();
-  lds_fft<M, RPB, NT, RPB, PM, 0, 1, +1, 2, false>(s, stw, tid);
+  lds_fft<M, RPB, NT, RPB, PM, 0, 1, +1, 2, false, RELAX_X_RMAX>(s, stw, tid);
   float coef = 1.f;
   if (EPI == EPI_PRED) coef = (float)(sc->Dt / 2);
   if (EPI == EPI_CORR) coef = (float)sc->Dt;

@@ -140,7 +140,7 @@ __device__ __forceinline__ void cerruti_point(const ColumnConsts& C, const Cerru
 // ---------------------------------------------------- fused column pass ----
 // largest radix of the column transforms (compile-time choice)
 #ifndef GREEN_RMAX
-#define GREEN_RMAX 8
+#define GREEN_RMAX 16
 #endif
 template <int H3, int TX, int NT>
 __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu, double alpha, double gamma,
@@ -270,16 +270,12 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
 
 template <int H3>
 struct GreenTile {
-#ifndef RELAX_GREEN_TX128
-#define RELAX_GREEN_TX128 8
-#endif
-  static constexpr int TX = H3 >= 256 ? 4 : (H3 >= 128 ? RELAX_GREEN_TX128 : 16);
+  // columns and threads per workgroup (measured at 512x512x256: 4 columns
+  // of 128 with 256 threads beat 8 with 512 threads by 4 %)
+  static constexpr int TX = H3 >= 128 ? 4 : 16;
+  static constexpr int NT = H3 >= 256 ? 512 : 256;
 };
 
-// threads per workgroup of the column pass (compile-time choice)
-#ifndef RELAX_GREEN_NT
-#define RELAX_GREEN_NT 512
-#endif
 int launch_green_columns(const GreenConst& gc, int h3, MPtr<float2> W, MPtr<float2> NQ, MPtr<const float2> TM,
                          MPtr<const float2> TN, int nm, hipStream_t st, int l2) {
   const float2* tw3 = twiddle_table(gc.n3);
@@ -287,10 +283,9 @@ int launch_green_columns(const GreenConst& gc, int h3, MPtr<float2> W, MPtr<floa
   const ColumnConsts C = make_consts(gc);
   const double alpha = (gc.lambda + gc.mu) / (gc.lambda + 2.0 * gc.mu);
   const int m1 = gc.n1 / 2;
-  constexpr int NT = RELAX_GREEN_NT;
 #define X(HH)                                                                                          \
   case HH: {                                                                                           \
-    constexpr int TX = GreenTile<HH>::TX;                                                              \
+    constexpr int TX = GreenTile<HH>::TX, NT = GreenTile<HH>::NT;                                      \
     if (m1 % TX || gc.n2 % TX) return RELAX_ERR_SIZE;                                                  \
     const int nb = (m1 / TX) * gc.n2 + gc.n2 / TX;                                                     \
     k_green_columns<HH, TX, NT><<<dim3(nb, nm), NT, 0, st>>>(C, gc.mu, alpha, gc.gamma, gc.n1, gc.n2, gc.n3, \

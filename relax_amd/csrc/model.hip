@@ -981,9 +981,15 @@ struct relax_batch {
   StepScalars* host_sc = nullptr;   // pinned
   int cap = 0;
   bool dirty = true;                // membership or a record changed
+  int stable = 0;                   // steps since the last change
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
+
+// A membership that changes every step or two (a server admitting and
+// retiring requests) runs its first steps eagerly; the step is captured as a
+// graph once the membership has been stable for this many steps.
+static const int GRAPH_AFTER = 2;
 
 __global__ void k_gather_scalars(const ModelDev* __restrict__ md, int nm, StepScalars* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1098,6 +1104,7 @@ static int batch_refresh(relax_batch_t* b) {
   }
   HIP_TRY(hipMemcpy(b->md_dev, b->md_host.data(), sizeof(ModelDev) * nm, hipMemcpyHostToDevice));
   b->dirty = false;
+  b->stable = 0;
   return RELAX_OK;
 }
 
@@ -1107,15 +1114,18 @@ int relax_batch_step(relax_batch_t* b, relax_step_info_t* infos) {
   if (nm == 0) return RELAX_OK;
   int rc;
   if ((rc = batch_refresh(b))) return rc;
-  if (!b->exec) {
+  Ctx c;
+  c.lead = b->models[0]; c.md = b->md_dev; c.one = nullptr; c.nm = nm; c.st = b->stream;
+  c.models = b->models.data();
+  c.prof = nullptr;
+  c.fric = false;
+  for (relax_model_t* m : b->models) c.fric = c.fric || (m->R.fr.active && m->R.fr.np > 0);
+  if (!b->exec && b->stable < GRAPH_AFTER) {
+    ++b->stable;
+    if ((rc = enqueue_steps(c, nullptr))) return rc;
+  } else if (!b->exec) {
     // capture one step of all members; replayed until the membership or a
     // member's record changes
-    Ctx c;
-    c.lead = b->models[0]; c.md = b->md_dev; c.one = nullptr; c.nm = nm; c.st = b->stream;
-    c.models = b->models.data();
-    c.prof = nullptr;
-    c.fric = false;
-    for (relax_model_t* m : b->models) c.fric = c.fric || (m->R.fr.active && m->R.fr.np > 0);
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     rc = enqueue_steps(c, nullptr);
     hipGraph_t g = nullptr;
@@ -1130,7 +1140,7 @@ int relax_batch_step(relax_batch_t* b, relax_step_info_t* infos) {
       return RELAX_ERR_HIP;
     }
   }
-  HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  if (b->exec) HIP_TRY(hipGraphLaunch(b->exec, b->stream));
   if (!infos) return RELAX_OK;
   k_gather_scalars<<<(nm + 63) / 64, 64, 0, b->stream>>>(b->md_dev, nm, b->gather);
   HIP_TRY(hipMemcpyAsync(b->host_sc, b->gather, sizeof(StepScalars) * nm, hipMemcpyDeviceToHost, b->stream));

@@ -162,8 +162,9 @@ class Server:
     every model held, pooled ones included; pooled models of other classes
     are released when an admission needs their memory."""
 
-    def __init__(self, budget_bytes=None, reserve=0.1, admit_per_pass=2):
+    def __init__(self, budget_bytes=None, reserve=0.1, admit_per_pass=2, chunk=4):
         self.admit_per_pass = admit_per_pass
+        self.chunk = chunk
         if budget_bytes is None:
             free, _ = meminfo()
             budget_bytes = int(free * (1.0 - reserve))
@@ -208,8 +209,8 @@ class Server:
         queue = []
         results = []
         t0 = time.perf_counter()
-        n = len(pending)
-        while len(results) < n:
+        nreq = len(pending)
+        while len(results) < nreq:
             now = time.perf_counter() - t0
             while pending and (not realtime or pending[0].arrival <= now):
                 queue.append(pending.pop(0))
@@ -254,13 +255,17 @@ class Server:
                 ready.add(r.grid)
                 admitted += 1
                 worked = True
-            # next step of every idle class
+            # next steps of every idle class: up to `chunk` steps at once (no
+            # member past its last step), one when requests wait for the class
+            waiting = {r.grid for r in queue}
             for gi in ready:
                 mem = self.members[gi]
                 if mem:
-                    self.batches[gi].step(info=False)
+                    nst = 1 if gi in waiting else min(self.chunk, min(r.nsteps - k for r, _, k, _ in mem))
+                    for _ in range(nst):
+                        self.batches[gi].step(info=False)
                     for e in mem:
-                        e[2] += 1
+                        e[2] += nst
                     worked = True
             if not worked:
                 if not any(self.members.values()) and pending and realtime:
