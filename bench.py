@@ -189,6 +189,7 @@ def main():
     sx1, sx2, sx3 = spec["sx1"], spec["sx2"], spec["sx3"]
     print("# building %s (%dx%dx%d) on the device" % (args.workload, sx1, sx2, sx3), file=sys.stderr, flush=True)
     model = workloads.build(spec)
+    tuned = model.autotune(times=True)  # launch shapes of the tunable kernels, timed on this grid
     for _ in range(args.warmup):
         model.step(info=False)
     model.synchronize()
@@ -231,7 +232,7 @@ def main():
         "data": "synthetic (seeded slip patches, relax_amd.workloads.%s)" % args.workload,
         "config": {"workload": "%s: %dx%dx%d FP32, dx=%g km, 40 rectangular 5x5 km slip patches, power law n=3 "
                                "below a 25 km lid, adaptive RK2" % (args.workload, sx1, sx2, sx3, spec["dx"]),
-                   "grid": [sx1, sx2, sx3], "dt_last": info["Dt"], "t_last": info["t"]},
+                   "grid": [sx1, sx2, sx3], "dt_last": info["Dt"], "t_last": info["t"], "autotune": tuned},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC_C1.get(dom) if args.workload == "c1" else None},

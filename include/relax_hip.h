@@ -54,6 +54,10 @@ typedef struct {                                                                
   double x, y, z, width, length, strike, dip, rake;
   int px2, px3; /* the patch array is passed separately */
 } relax_plane_t;
+typedef struct {                                                                 /* types.f90:46-50 */
+  double dgammadot0, x, y, z, width, length, thickness, strike, dip;  /* angles in radians */
+  relax_tensor_t e;
+} relax_weak_t;
 
 /* ---- library ------------------------------------------------------------ */
 const char* relax_hip_version(void);
@@ -119,6 +123,19 @@ int relax_viscouseigenstress(double mu, const relax_layer_t* structure, int nstr
                              const relax_tensor_layer_t* prestress, int nprestress, const relax_tensor_t* sig,
                              int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, relax_tensor_t* moment,
                              double* maxwelltime, const float* dgammadot0, float* gamma, int align_data);
+/* transienteigenstress, viscoelastic3d.f90:123-202 (Burgers transient
+ * creep; structure Gk is the Kelvin shear modulus).  epsilonik and
+ * epsilonikdot may be the same array, as in the corrector's call
+ * (relax.f90:768-790).  maxwelltime and dgammadot0 are optional (NULL).  A
+ * structure whose size is not sx3 is ignored (returns RELAX_OK). */
+int relax_transienteigenstress(double mu, const relax_layer_t* structure, int nstructure,
+                               const relax_tensor_t* sig, const relax_tensor_t* epsilonik, int sx1, int sx2,
+                               int sx3, double dx1, double dx2, double dx3, relax_tensor_t* moment,
+                               relax_tensor_t* epsilonikdot, double* maxwelltime, const float* dgammadot0);
+/* builddgammadot0, viscoelastic3d.f90:320-435: the fluidity perturbation
+ * (sx1, sx2, sx3) of nz weak zones, tapered by beta. */
+int relax_builddgammadot0(int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, double beta, int nz,
+                          const relax_weak_t* zones, float* dgammadot0);
 /* frictioneigenstress, friction3d.f90:82-241 (CPU path semantics) */
 int relax_frictioneigenstress(double x, double y, double z, double L, double W, double strike, double dip,
                               double rake, double beta, const relax_tensor_t* sig,
@@ -152,7 +169,8 @@ typedef struct {
 typedef struct {
   int i;              /* iteration index after the step (relax.f90:555) */
   double Dt, tm;      /* time step and relaxation time of the step */
-  double maxwell[3];  /* linear, power-law, fault creep (STEP_MAX when inactive) */
+  double maxwell[5];  /* linear, power-law, fault creep, linear and power-law
+                         transient creep (STEP_MAX when inactive), relax.f90:565 */
   double power;       /* tensoramplitude(moment), relax.f90:1027-1046 */
   double cei;         /* tensoramplitude(tau) */
   double t;           /* time after the step */
@@ -169,10 +187,28 @@ int relax_model_destroy(relax_model_t* m);
  * can reuse the model for another request of the same grid. */
 int relax_model_reset(relax_model_t* m);
 /* which: 0 = linear viscosity (in%linearstruc), 1 = power law
- * (in%nonlinearstruc), 2 = fault creep (in%faultcreepstruc).  layers are
- * the per-depth samples of viscoelasticstructure (sx3/2 entries), or NULL
- * to remove the mechanism. */
+ * (in%nonlinearstruc), 2 = fault creep (in%faultcreepstruc), 3 = linear
+ * transient creep (in%ltransientstruc), 4 = power-law transient creep
+ * (in%nltransientstruc; relax.f90:614-650).  layers are the per-depth
+ * samples of viscoelasticstructure (sx3/2 entries), or NULL to remove the
+ * mechanism.  The first transient structure allocates the transient strain
+ * epsilonik (48 bytes per point with its mid-step copy), set to zero. */
+#define RELAX_LINEAR 0
+#define RELAX_POWERLAW 1
+#define RELAX_FAULTCREEP 2
+#define RELAX_LTRANSIENT 3
+#define RELAX_NLTRANSIENT 4
 int relax_model_set_structure(relax_model_t* m, int which, const relax_layer_t* layers, int n);
+/* Weak zones of mechanism which (0, 1, 3 or 4): the model adds the field
+ * builddgammadot0 makes of them (taper 0, as relax.f90:487-541 passes) to
+ * the mechanism's depth-dependent fluidity, as the DGAMMADOT0 argument does
+ * (viscoelastic3d.f90:277-280, :164-167).  The field takes 4 bytes per
+ * point; nz = 0 removes it. */
+int relax_model_set_weakzones(relax_model_t* m, int which, const relax_weak_t* zones, int nz);
+/* The transient strain epsilonik (TYPE(TENSOR) (sx1, sx2, sx3/2)): set or
+ * read.  RELAX_ERR_ARG without a transient structure. */
+int relax_model_set_transient_strain(relax_model_t* m, const relax_tensor_t* epsilonik);
+int relax_model_get_transient_strain(relax_model_t* m, relax_tensor_t* epsilonik);
 int relax_model_set_prestress(relax_model_t* m, const relax_tensor_layer_t* prestress, int n);
 /* friction planes (in%n), each with its px2*px3 patches (may be NULL) */
 int relax_model_set_planes(relax_model_t* m, const relax_plane_t* planes, int np,
@@ -224,6 +260,17 @@ int relax_model_profile(relax_model_t* m, int nsteps, double* ms, int* launches)
  * at each output step for observation depth 0 (relax.f90:955,
  * export.f90:917). */
 int relax_model_surface(relax_model_t* m, float* u1, float* u2, float* u3, int align_data);
+/* Autotuner: times the launch-shape variants of the tunable kernels (columns
+ * per workgroup of the y transform, columns/threads/radix of the Green
+ * column pass, divergence tile) on this model's scratch buffers and keeps the
+ * fastest for this device and grid; later models of the grid in the process
+ * use the choice.  choice (3 ints, may be NULL) receives the variants;
+ * ms (may be NULL) the average time of one launch of each variant in
+ * milliseconds, 3 rows of RELAX_TUNE_VARIANTS (columns, Green, divergence),
+ * with -1 for a variant that does not apply to the grid.  The model's state
+ * is not changed.  Not on a member of a batch. */
+#define RELAX_TUNE_VARIANTS 4
+int relax_model_autotune(relax_model_t* m, int* choice, float* ms);
 /* The model's HIP stream (hipStream_t) for callers that order their own work. */
 void* relax_model_stream(relax_model_t* m);
 

@@ -27,6 +27,9 @@ PATCH = np.dtype([("x1", "f8"), ("x2", "f8"), ("x3", "f8"), ("lx", "f8"), ("lz",
                   ("vds", "f8"), ("taus", "f8"), ("sig", "f4", (6,))])
 PLANE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("width", "f8"), ("length", "f8"),
                   ("strike", "f8"), ("dip", "f8"), ("rake", "f8"), ("px2", "i4"), ("px3", "i4")])
+# WEAK_STRUCT, types.f90:46-50
+WEAK = np.dtype([("dgammadot0", "f8"), ("x", "f8"), ("y", "f8"), ("z", "f8"), ("width", "f8"),
+                 ("length", "f8"), ("thickness", "f8"), ("strike", "f8"), ("dip", "f8"), ("e", "f4", (6,))])
 
 
 class OrcModel(ctypes.Structure):
@@ -38,7 +41,9 @@ class OrcModel(ctypes.Structure):
                 ("faultcreep", ctypes.c_void_p), ("prestress", ctypes.c_void_p),
                 ("np", ctypes.c_int), ("planes", ctypes.c_void_p), ("patches", ctypes.c_void_p),
                 ("interval", ctypes.c_double), ("odt", ctypes.c_double), ("tscale", ctypes.c_double),
-                ("skip", ctypes.c_int), ("ne", ctypes.c_int), ("event_times", ctypes.c_void_p)]
+                ("skip", ctypes.c_int), ("ne", ctypes.c_int), ("event_times", ctypes.c_void_p),
+                ("ltransient", ctypes.c_void_p), ("nltransient", ctypes.c_void_p),
+                ("dg0", ctypes.c_void_p * 4)]
 
 
 def lib():
@@ -52,6 +57,7 @@ def lib():
         _LIB.orc_tensoramplitude_d.restype = ctypes.c_double
         assert _LIB.orc_sizeof_patch() == PATCH.itemsize
         assert _LIB.orc_sizeof_model() == ctypes.sizeof(OrcModel)
+        assert _LIB.orc_sizeof_weak() == WEAK.itemsize
     return _LIB
 
 
@@ -147,6 +153,25 @@ def viscouseigenstress(mu, structure, prestress, sig, moment, maxwelltime=None, 
     return None if mw is None else float(mw[0])
 
 
+def transienteigenstress(mu, structure, sig, epsilonik, moment, epsilonikdot, maxwelltime=None, dgammadot0=None):
+    """viscoelastic3d.f90:123-202; epsilonik may be epsilonikdot itself."""
+    sx3, sx2, sx1, _ = sig.shape
+    mw = None if maxwelltime is None else np.array([maxwelltime], np.float64)
+    _chk(getattr(lib(), "orc_transienteigenstress_" + _sfx(sig))(
+        _d(mu), _p(structure), _p(sig), _p(epsilonik), sx1, sx2, sx3, _p(moment), _p(epsilonikdot), _p(mw),
+        _p(dgammadot0)), "transienteigenstress")
+    return None if mw is None else float(mw[0])
+
+
+def builddgammadot0(sx1, sx2, sx3, dx1, dx2, dx3, beta, zones, dtype=np.float32):
+    """viscoelastic3d.f90:320-435: the weak-zone fluidity field (sx3, sx2, sx1)."""
+    zones = np.ascontiguousarray(zones, WEAK)
+    out = np.zeros((sx3, sx2, sx1), dtype)
+    getattr(lib(), "orc_builddgammadot0_" + ("f" if out.dtype == np.float32 else "d"))(
+        sx1, sx2, sx3, _d(dx1), _d(dx2), _d(dx3), _d(beta), len(zones), _p(zones), _p(out))
+    return out
+
+
 def frictioneigenstress(x, y, z, L, W, strike, dip, rake, beta, sig, mu, structure, dx1, dx2, dx3, moment,
                         maxwelltime=None, vel=None):
     sx3, sx2, sx1, _ = sig.shape
@@ -177,8 +202,11 @@ def rk2_run(model, state, nsteps, dt_replay=None):
     `model` is a dict: sx1,sx2,sx3,dx1,dx2,dx3,lam,mu,gam,beta, linear/nonlinear/
     faultcreep (LAYER arrays of sx3/2 or None), prestress (TENSOR_LAYER sx3/2),
     planes (PLANE array or None), patches (PATCH array or None), interval, odt,
-    tscale, skip, event_times.  Returns an (nsteps, 8) array of
-    Dt, tm, maxwell(1:3), power, C:E^i, t.
+    tscale, skip, event_times; optional ltransient/nltransient (LAYER arrays)
+    with state["epsilonik"] (tensor field), and weak-zone fields dg_linear,
+    dg_nonlinear, dg_ltransient, dg_nltransient ((sx3/2, sx2, sx1) arrays of
+    the state's dtype).  Returns an (nsteps, 10) array of
+    Dt, tm, maxwell(1:3), power, C:E^i, t, maxwell(4:5).
     """
     keep = []
 
@@ -196,16 +224,21 @@ def rk2_run(model, state, nsteps, dt_replay=None):
                  ptr(model["prestress"]), 0 if model.get("planes") is None else len(model["planes"]),
                  ptr(model.get("planes")), None if model.get("patches") is None else model["patches"].ctypes.data,
                  model["interval"], model.get("odt", 1.0), model.get("tscale", 1.0), model.get("skip", 1),
-                 len(ev), ptr(ev))
+                 len(ev), ptr(ev), ptr(model.get("ltransient")), ptr(model.get("nltransient")))
+    for k, key in enumerate(("dg_linear", "dg_nonlinear", "dg_ltransient", "dg_nltransient")):
+        a = model.get(key)
+        if a is not None:
+            assert a.dtype == state["u1"].dtype and a.size == state["tau"].size // 6, key
+        m.dg0[k] = ptr(a)
     sfx = _sfx(state["u1"])
     t = ctypes.c_double(state.get("t", 0.0))
     oi = ctypes.c_int(state.get("oi", 1))
     it = ctypes.c_int(state.get("i", 0))
-    info = np.zeros((nsteps, 8), np.float64)
+    info = np.zeros((nsteps, 10), np.float64)
     dtr = None if dt_replay is None else np.ascontiguousarray(dt_replay, np.float64)
     rc = getattr(lib(), "orc_rk2_run_" + sfx)(
         ctypes.byref(m), _p(state["u1"]), _p(state["u2"]), _p(state["u3"]), _p(state["tau"]), _p(state["sig"]),
-        _p(state["gamma"]), ctypes.byref(t), ctypes.byref(oi), ctypes.byref(it), nsteps, _p(dtr), _p(info))
+        _p(state["gamma"]), _p(state.get("epsilonik")), ctypes.byref(t), ctypes.byref(oi), ctypes.byref(it), nsteps, _p(dtr), _p(info))
     _chk(rc, "rk2_run")
     done = it.value - state.get("i", 0)
     state["t"], state["oi"], state["i"] = t.value, oi.value, it.value

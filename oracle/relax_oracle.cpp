@@ -603,6 +603,115 @@ static inline int fix(double number) {
 }
 
 // ---------------------------------------------------------------------------
+// transienteigenstress, viscoelastic3d.f90:123-202 (Burgers transient creep):
+//   Q = s' - 2 Gk E,  gammadot = gammadot0 (|Q|/mu)^(n-1),
+//   Edot = gammadot/mu Q,  moment += 2 gammadot Q,  tm = min 1/gammadot.
+// epsilonik and epsilonikdot may be the same array (the corrector passes
+// epsilonikdot for both, relax.f90:768-790): each point reads its E before
+// it writes its Edot.
+// ---------------------------------------------------------------------------
+template <class R>
+static int transienteigenstress(double mu, const orc_layer* structure, const R* sig, const R* epsilonik, int sx1,
+                                int sx2, int sx3, R* moment, R* epsilonikdot, double* maxwelltime,
+                                const R* dgammadot0) {
+  const T6<R>* S = reinterpret_cast<const T6<R>*>(sig);
+  const T6<R>* E = reinterpret_cast<const T6<R>*>(epsilonik);
+  T6<R>* M = reinterpret_cast<T6<R>*>(moment);
+  T6<R>* Ed = reinterpret_cast<T6<R>*>(epsilonikdot);
+  for (int i3 = 0; i3 < sx3; ++i3)
+    if (structure[i3].stressexponent < 0.999999) return 1;
+  float tm = maxwelltime ? (float)*maxwelltime : 1e30f;
+#pragma omp parallel for reduction(min : tm)
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    const double power = structure[i3].stressexponent;
+    const double muk = structure[i3].Gk;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        const long o = i1 + (long)sx1 * (i2 + (long)sx2 * i3);
+        double gammadot0 = structure[i3].gammadot0;
+        if (dgammadot0) gammadot0 = gammadot0 + dgammadot0[o];
+        if (1.0e-20 > gammadot0) {
+          Ed[o] = T6<R>{0, 0, 0, 0, 0, 0};
+          continue;
+        }
+        const T6<R> s = tensordeviatoric(S[o]);
+        const R k2 = (R)(MUT(14) ? muk : 2.0 * muk);
+        const T6<R> e = E[o];
+        const T6<R> Q = {s.s11 - k2 * e.s11, s.s12 - k2 * e.s12, s.s13 - k2 * e.s13,
+                         s.s22 - k2 * e.s22, s.s23 - k2 * e.s23, s.s33 - k2 * e.s33};
+        const double nq = tensornorm(Q);
+        const double gammadot = gammadot0 * std::pow(nq / mu, MUT(15) ? power : power - 1);
+        const R a = (R)(gammadot / mu);
+        Ed[o] = T6<R>{a * Q.s11, a * Q.s12, a * Q.s13, a * Q.s22, a * Q.s23, a * Q.s33};
+        const R b = (R)(2.0 * gammadot);
+        T6<R>& m = M[o];
+        m.s11 = m.s11 + b * Q.s11; m.s12 = m.s12 + b * Q.s12; m.s13 = m.s13 + b * Q.s13;
+        m.s22 = m.s22 + b * Q.s22; m.s23 = m.s23 + b * Q.s23; m.s33 = m.s33 + b * Q.s33;
+        tm = std::min(tm, (float)(1.0 / gammadot));
+      }
+  }
+  if (maxwelltime) *maxwelltime = std::min((double)tm, *maxwelltime);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// builddgammadot0, viscoelastic3d.f90:320-435 (fdgammadot0 :351-433): the
+// fluidity perturbation of weak zones, tapered by omega; boxes, spheres and
+// the three cylinder orientations chosen by the signs of D, W, L.
+// ---------------------------------------------------------------------------
+extern "C" {
+struct orc_weak {
+  double dgammadot0, x, y, z, width, length, thickness, strike, dip;
+  float e[6];
+};
+}
+
+static double fdgammadot0(const orc_weak* zones, int n, double x1, double x2, double x3, double beta) {
+  double f = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const orc_weak& Z = zones[i];
+    const double dg = Z.dgammadot0, x = Z.x, y = Z.y, z = Z.z;
+    // the zone's "length" is the x2 extent W and its "width" the x3 extent L
+    const double W = Z.length, L = Z.width, D = Z.thickness;
+    const double Wp = W * (1.0 + 2.0 * beta) / 2.0, Lp = L * (1.0 + 2.0 * beta) / 2.0,
+                 Dp = D * (1.0 + 2.0 * beta) / 2.0;
+    const double LM = std::max(Wp, std::max(Lp, Dp));
+    if ((std::fabs(x3 - z) > LM) || (std::fabs(x1 - x) > LM) || (std::fabs(x2 - y) > LM)) continue;
+    const double cs = std::cos(Z.strike), ss = std::sin(Z.strike), cd_ = std::cos(Z.dip), sd = std::sin(Z.dip);
+    double x2r = cs * x - ss * y;
+    const double xr = cd_ * x2r - sd * z, yr = ss * x + cs * y, zr = sd * x2r + cd_ * z;
+    x2r = cs * x1 - ss * x2;
+    const double x1s = cd_ * x2r - sd * x3, x2s = ss * x1 + cs * x2, x3s = sd * x2r + cd_ * x3;
+    const double a = (x1s - xr) / D, b = (x2s - yr) / W, c = (x3s - zr) / L;
+    if (D >= 0 && W >= 0 && L >= 0) {
+      f += omega(a, beta) * omega(b, beta) * omega(MUT(16) ? (x3s + zr) / L : c, beta) * dg;
+    } else if (D < 0 && W < 0 && L < 0) {
+      f += omega(std::sqrt(a * a + b * b + c * c), beta) * dg;
+    } else if (D < 0 && W < 0) {
+      f += omega(std::sqrt(a * a + b * b), beta) * omega(c, beta) * dg;
+    } else if (D < 0 && L < 0) {
+      f += omega(std::sqrt(a * a + c * c), beta) * omega(b, beta) * dg;
+    } else if (W < 0 && L < 0) {
+      f += omega(std::sqrt(b * b + c * c), beta) * omega(a, beta) * dg;
+    }
+  }
+  return f;
+}
+
+template <class R>
+static void builddgammadot0(int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, double beta, int nz,
+                            const orc_weak* zones, R* dg) {
+#pragma omp parallel for
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    const double x3 = (double)i3 * dx3;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1)
+        dg[i1 + (long)sx1 * (i2 + (long)sx2 * i3)] =
+            (R)fdgammadot0(zones, nz, shifted1(i1, sx1, dx1), shifted1(i2, sx2, dx2), x3, beta);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // frictioneigenstress, friction3d.f90:82-241 (CPU path)
 // ---------------------------------------------------------------------------
 template <class R>
@@ -810,9 +919,10 @@ static void momentdensityshear(double mu, double slip, double x, double y, doubl
 // ---------------------------------------------------------------------------
 // RK2 loop: relax.f90:555-890 with integrationstep :1420-1452 and the
 // isoutput counter :1364-1376 (oi advances whenever isoutput holds).
-// Mechanisms: linear and power-law viscosity and fault creep; no transient
-// creep, no interseismic loading, no time-dependent surface loads and no
-// coseismic events after t=0 (event times only truncate Dt).
+// Mechanisms: linear and power-law viscosity (with weak zones), fault creep,
+// and linear and power-law transient creep (with weak zones); no
+// interseismic loading, no time-dependent surface loads and no coseismic
+// events after t=0 (event times only truncate Dt).
 // ---------------------------------------------------------------------------
 extern "C" {
 struct orc_model {
@@ -829,18 +939,32 @@ struct orc_model {
   int skip;
   int ne;
   const double* event_times;
+  // transient creep (relax.f90:614-650, :764-800): linear and power-law
+  // Burgers structures, sx3/2 entries or NULL; either one needs the
+  // epsilonik state of rk2_run
+  const orc_layer* ltransient;
+  const orc_layer* nltransient;
+  // weak zones (builddgammadot0 fields, sx1*sx2*sx3/2 values of the storage
+  // type, or NULL): linear, power-law, linear transient, power-law transient
+  const void* dg0[4];
 };
 }
 
 static const double STEP_MAX = 1e7;
 
 template <class R>
-static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* gamma, double* t_io, int* oi_io,
-                   int* i_io, int nsteps, const double* dt_replay, double* info) {
+static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* gamma, R* epsilonik, double* t_io,
+                   int* oi_io, int* i_io, int nsteps, const double* dt_replay, double* info) {
   const int sx1 = m->sx1, sx2 = m->sx2, sx3 = m->sx3, h3 = sx3 / 2;
   const long pitch = sx1 + 2;
   const long nv = pitch * sx2 * sx3, nu = pitch * sx2 * h3, nt = pitch * sx2, npts = (long)sx1 * sx2 * h3;
   std::vector<R> v1(nv), v2(nv), v3(nv), t1(nt), t2(nt), t3(nt), moment(6 * npts, 0);
+  const bool transient = m->ltransient || m->nltransient;
+  if (transient && !epsilonik) return 10;
+  std::vector<R> epsilonikdot(transient ? 6 * npts : 0, (R)0);
+  const R* dg[4];
+  for (int k = 0; k < 4; ++k) dg[k] = static_cast<const R*>(m->dg0[k]);
+  const orc_layer* trans[2] = {m->ltransient, m->nltransient};
   double t = *t_io;
   int oi = *oi_io, it = *i_io;
   const int e = 1;  // single event at t=0
@@ -852,12 +976,12 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
     std::fill(moment.begin(), moment.end(), (R)0);
     int rc;
     if (m->linear) {
-      rc = viscouseigenstress(m->mu, m->linear, m->prestress, sig, sx1, sx2, h3, moment.data(), &maxwell[0], (const R*)0, (R*)0, pitch);
+      rc = viscouseigenstress(m->mu, m->linear, m->prestress, sig, sx1, sx2, h3, moment.data(), &maxwell[0], dg[0], (R*)0, pitch);
       if (rc) return rc;
       mech[0] = 1;
     }
     if (m->nonlinear) {
-      rc = viscouseigenstress(m->mu, m->nonlinear, m->prestress, sig, sx1, sx2, h3, moment.data(), &maxwell[1], (const R*)0, (R*)0, pitch);
+      rc = viscouseigenstress(m->mu, m->nonlinear, m->prestress, sig, sx1, sx2, h3, moment.data(), &maxwell[1], dg[1], (R*)0, pitch);
       if (rc) return rc;
       mech[1] = 1;
     }
@@ -870,6 +994,14 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
         if (rc) return rc;
       }
       mech[2] = 1;
+    }
+    // transient creep reads epsilonik and writes epsilonikdot
+    for (int k = 0; k < 2; ++k) {
+      if (!trans[k]) continue;
+      rc = transienteigenstress(m->mu, trans[k], sig, epsilonik, sx1, sx2, h3, moment.data(), epsilonikdot.data(),
+                                &maxwell[3 + k], dg[2 + k]);
+      if (rc) return rc;
+      mech[3 + k] = 1;
     }
     double tm = 1.0 / (mech[0] / maxwell[0] + mech[1] / maxwell[1] + mech[2] / maxwell[2] + mech[3] / maxwell[3] +
                        mech[4] / maxwell[4]);
@@ -901,6 +1033,9 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
       v2[o] = hdt * v2[o] + u2[o];
       v3[o] = hdt * v3[o] + u3[o];
     }
+    // epsilonikdot <- Dt/2 epsilonikdot + epsilonik (the mid-step strain)
+    if (transient)
+      for (long o = 0; o < 6 * npts; ++o) epsilonikdot[o] = hdt * epsilonikdot[o] + (R)1 * epsilonik[o];
     for (long o = 0; o < 6 * npts; ++o) sig[o] = (-hdt) * sig[o] + (R)(-1) * tau[o];
     // corrector
     stressupdate(v1.data(), v2.data(), v3.data(), m->lambda, m->mu, m->dx1, m->dx2, m->dx3, sx1, sx2, h3, sig, pitch);
@@ -909,7 +1044,7 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
     for (int k = 0; k < 2; ++k) {
       if (!visc[k]) continue;
       std::fill(v1.begin(), v1.end(), (R)0);
-      rc = viscouseigenstress(m->mu, visc[k], m->prestress, sig, sx1, sx2, h3, moment.data(), (double*)0, (const R*)0, v1.data(), pitch);
+      rc = viscouseigenstress(m->mu, visc[k], m->prestress, sig, sx1, sx2, h3, moment.data(), (double*)0, dg[k], v1.data(), pitch);
       if (rc) return rc;
       const R cdt = (R)Dt;
       for (long o = 0; o < nu; ++o) gamma[o] = gamma[o] + cdt * v1[o];
@@ -929,6 +1064,18 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
           off += (long)P.px2 * P.px3;
         }
       }
+    }
+    if (transient) {
+      // the corrector passes epsilonikdot as both the strain and the rate
+      for (int k = 0; k < 2; ++k) {
+        if (!trans[k]) continue;
+        rc = transienteigenstress(m->mu, trans[k], sig, epsilonikdot.data(), sx1, sx2, h3, moment.data(),
+                                  epsilonikdot.data(), (double*)0, dg[2 + k]);
+        if (rc) return rc;
+      }
+      const R cdt = (R)Dt;
+      for (long o = 0; o < 6 * npts; ++o) epsilonik[o] = epsilonik[o] + cdt * epsilonikdot[o];
+      std::fill(epsilonikdot.begin(), epsilonikdot.end(), (R)0);
     }
     std::fill(v1.begin(), v1.end(), (R)0); std::fill(v2.begin(), v2.end(), (R)0); std::fill(v3.begin(), v3.end(), (R)0);
     std::fill(t1.begin(), t1.end(), (R)0); std::fill(t2.begin(), t2.end(), (R)0); std::fill(t3.begin(), t3.end(), (R)0);
@@ -960,11 +1107,11 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
     bool out = ((0 == m->skip) && (std::fabs(t - oi * m->odt) < 1e-6 * m->odt)) ||
                ((0 < m->skip) && ((it - 1) % m->skip == 0)) || (std::fabs(t - etime) < 1e-6);
     if (info) {
-      double* I = info + 8 * step;
+      double* I = info + 10 * step;
       I[0] = Dt; I[1] = tm; I[2] = maxwell[0]; I[3] = maxwell[1]; I[4] = maxwell[2];
       I[5] = tensoramplitude(moment.data(), npts, m->dx1, m->dx2, m->dx3);
       I[6] = tensoramplitude(tau, npts, m->dx1, m->dx2, m->dx3);
-      I[7] = t;
+      I[7] = t; I[8] = maxwell[3]; I[9] = maxwell[4];
     }
     if (out) ++oi;
   }
@@ -1040,9 +1187,18 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
     momentdensityshear(mu, slip, x, y, z, L, W, strike, dip, rake, beta, sx1, sx2, sx3, dx1, dx2, dx3, sig);         \
   }                                                                                                                  \
   ORC_EXPORT int orc_rk2_run_##SUFFIX(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* gamma,           \
-                                      double* t, int* oi, int* it, int nsteps, const double* dt_replay,             \
-                                      double* info) {                                                                \
-    return rk2_run(m, u1, u2, u3, tau, sig, gamma, t, oi, it, nsteps, dt_replay, info);                              \
+                                      R* epsilonik, double* t, int* oi, int* it, int nsteps,                        \
+                                      const double* dt_replay, double* info) {                                       \
+    return rk2_run(m, u1, u2, u3, tau, sig, gamma, epsilonik, t, oi, it, nsteps, dt_replay, info);                   \
+  }                                                                                                                  \
+  ORC_EXPORT int orc_transienteigenstress_##SUFFIX(double mu, const orc_layer* st, const R* sig, const R* eik,     \
+                                                   int sx1, int sx2, int sx3, R* moment, R* eikdot,                 \
+                                                   double* maxwelltime, const R* dg0) {                              \
+    return transienteigenstress(mu, st, sig, eik, sx1, sx2, sx3, moment, eikdot, maxwelltime, dg0);                  \
+  }                                                                                                                  \
+  ORC_EXPORT void orc_builddgammadot0_##SUFFIX(int sx1, int sx2, int sx3, double dx1, double dx2, double dx3,      \
+                                               double beta, int nz, const orc_weak* zones, R* dg) {                 \
+    builddgammadot0(sx1, sx2, sx3, dx1, dx2, dx3, beta, nz, zones, dg);                                              \
   }
 
 ORC_BOTH(f, float)
@@ -1052,3 +1208,4 @@ ORC_EXPORT void orc_frictionadd(orc_patch* patch, long npatch, double dt) { fric
 ORC_EXPORT void orc_set_mutation(int k) { g_mut = k; }
 ORC_EXPORT int orc_sizeof_patch(void) { return (int)sizeof(orc_patch); }
 ORC_EXPORT int orc_sizeof_model(void) { return (int)sizeof(orc_model); }
+ORC_EXPORT int orc_sizeof_weak(void) { return (int)sizeof(orc_weak); }

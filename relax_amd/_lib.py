@@ -25,6 +25,8 @@ PATCH = np.dtype([("x1", "f8"), ("x2", "f8"), ("x3", "f8"), ("lx", "f8"), ("lz",
                   ("vds", "f8"), ("taus", "f8"), ("sig", "f4", (6,))])
 PLANE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("width", "f8"), ("length", "f8"),
                   ("strike", "f8"), ("dip", "f8"), ("rake", "f8"), ("px2", "i4"), ("px3", "i4")])
+WEAK = np.dtype([("dgammadot0", "f8"), ("x", "f8"), ("y", "f8"), ("z", "f8"), ("width", "f8"),
+                 ("length", "f8"), ("thickness", "f8"), ("strike", "f8"), ("dip", "f8"), ("e", "f4", (6,))])
 
 
 class ModelDesc(ctypes.Structure):
@@ -36,7 +38,7 @@ class ModelDesc(ctypes.Structure):
 
 
 class StepInfo(ctypes.Structure):
-    _fields_ = [("i", c_int), ("Dt", c_double), ("tm", c_double), ("maxwell", c_double * 3),
+    _fields_ = [("i", c_int), ("Dt", c_double), ("tm", c_double), ("maxwell", c_double * 5),
                 ("power", c_double), ("cei", c_double), ("t", c_double), ("oi", c_int), ("status", c_int)]
 
 
@@ -67,6 +69,9 @@ SIGNATURES = {
                                          c_double, c_double, P]),
     "relax_viscouseigenstress": (c_int, [c_double, P, c_int, P, c_int, P, c_int, c_int, c_int, c_double, c_double,
                                          c_double, P, P, P, P, c_int]),
+    "relax_transienteigenstress": (c_int, [c_double, P, c_int, P, P, c_int, c_int, c_int, c_double, c_double,
+                                           c_double, P, P, P, P]),
+    "relax_builddgammadot0": (c_int, [c_int, c_int, c_int, c_double, c_double, c_double, c_double, c_int, P, P]),
     "relax_frictioneigenstress": (c_int, [c_double] * 9 + [P, P, c_double, P, c_int, c_int, c_int, c_double,
                                                            c_double, c_double, P, P, P, c_int]),
     "relax_monitorfriction": (c_int, [c_double] * 9 + [c_int, c_int, c_int, c_int, c_double, c_double, c_double,
@@ -78,6 +83,9 @@ SIGNATURES = {
     "relax_model_reset": (c_int, [P]),
     "relax_model_set_structure": (c_int, [P, c_int, P, c_int]),
     "relax_model_set_prestress": (c_int, [P, P, c_int]),
+    "relax_model_set_weakzones": (c_int, [P, c_int, P, c_int]),
+    "relax_model_set_transient_strain": (c_int, [P, P]),
+    "relax_model_get_transient_strain": (c_int, [P, P]),
     "relax_model_set_planes": (c_int, [P, P, c_int, P]),
     "relax_model_get_patches": (c_int, [P, P]),
     "relax_model_upload": (c_int, [P, P, P, P, P, P, P, c_double, c_int, c_int, c_int]),
@@ -89,6 +97,7 @@ SIGNATURES = {
     "relax_model_add_shear": (c_int, [P, P, P, c_int]),
     "relax_model_coseismic": (c_int, [P]),
     "relax_model_profile": (c_int, [P, c_int, P, P]),
+    "relax_model_autotune": (c_int, [P, P, P]),
     "relax_hip_meminfo": (c_int, [P, P]),
     "relax_batch_create": (c_int, [P]),
     "relax_batch_destroy": (c_int, [P]),

@@ -47,8 +47,9 @@ def build(verbose=False):
 
 
 def build_fortran():
-    """Compile MODULE relax_hip and the Fortran drop-in test driver with
-    flang (skipped when the image has no flang)."""
+    """Compile MODULE relax_hip and the Fortran test drivers (per-routine
+    drop-in, RK2 model loop) with flang (skipped when the image has no
+    flang)."""
     import shutil
     flang = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
     if not os.path.exists(flang):
@@ -56,14 +57,21 @@ def build_fortran():
     root = os.path.dirname(HERE)
     bdir = os.path.join(root, "tests", "fortran", "_bin")
     os.makedirs(bdir, exist_ok=True)
-    exe = os.path.join(bdir, "drop_in")
-    cmd = [flang, "-cpp", "-O2", "-J", bdir, os.path.join(HERE, "fortran", "relax_hip.f90"),
-           os.path.join(root, "tests", "fortran", "drop_in.f90"), "-L" + HERE, "-lrelax_hip",
-           "-Wl,-rpath,$ORIGIN/../../../relax_amd", "-o", exe]
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    mod_obj = os.path.join(bdir, "relax_hip.o")
+    r = subprocess.run([flang, "-cpp", "-O2", "-J", bdir, "-c", os.path.join(HERE, "fortran", "relax_hip.f90"),
+                        "-o", mod_obj], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
-    return exe
+    exes = []
+    for name in ("drop_in", "model_loop"):
+        exe = os.path.join(bdir, name)
+        cmd = [flang, "-cpp", "-O2", "-J", bdir, mod_obj, os.path.join(root, "tests", "fortran", name + ".f90"),
+               "-L" + HERE, "-lrelax_hip", "-Wl,-rpath,$ORIGIN/../../../relax_amd", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
+        exes.append(exe)
+    return exes
 
 
 if __name__ == "__main__":

@@ -74,8 +74,10 @@ __device__ __forceinline__ void store_t6(float* __restrict__ t, long H, long o, 
 struct Mechanism {
   const float* gammadot0;  // h3
   const float* power;      // h3 (stress exponent n)
-  const float* cohesion;   // h3
+  const float* cohesion;   // h3 (viscous); Gk (transient)
   const float* gdp;        // h3: gammadot0 (taup/mu)^(n-1), the prestress rate
+  const float* rp;         // h3: (taup/mu)^(n-1), for points with a weak zone
+  const float* dg;         // H: weak-zone fluidity perturbation, or NULL
   int active;
 };
 
@@ -104,11 +106,14 @@ struct FrictionTables {
 };
 
 // Device-side scalars of one model's time line.
+// Relaxation mechanisms of the time step (relax.f90:565-660): linear and
+// power-law viscosity, fault creep, linear and power-law transient creep.
+constexpr int NMECH = 5;
 struct StepScalars {
   double t, Dt, tm;
-  double maxwell[3];
+  double maxwell[NMECH];
   double power, cei;
   int oi, it, status, pad;
-  unsigned int tmin_bits[3];  // running minima (positive floats as ordered ints)
-  unsigned int pad2;
+  unsigned int tmin_bits[NMECH];  // running minima (positive floats as ordered ints)
+  unsigned int pad2[3];
 };

@@ -5,6 +5,7 @@
 //
 // Stencils here take any FIR length chosen by optimalfilter
 // (elastic3d.f90:846-907): fir1 for 15+ samples, fir7 for 5..14.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -243,6 +244,67 @@ __global__ void k_viscous_generic(Mechanism M, Prestress P, float mu, int n1, in
   }
   for (int off = 32; off > 0; off >>= 1) tmin = fminf(tmin, __shfl_xor(tmin, off));
   if ((threadIdx.x & 63) == 0 && tmin < __builtin_inff()) atomicMin(tmin_bits, __float_as_uint(tmin));
+}
+
+// transienteigenstress, viscoelastic3d.f90:123-202.  E and Ed may be one
+// buffer (the corrector's call): each point reads its strain before it
+// writes its rate.
+__global__ void k_transient_generic(Mechanism M, float mu, int n1, int n2, int n3, const float* __restrict__ sig,
+                                    const float* E, float* moment, float* Ed, const float* __restrict__ dg0,
+                                    unsigned int* tmin_bits) {
+  const long N = (long)n1 * n2 * n3;
+  float tmin = __builtin_inff();
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < N; o += (long)gridDim.x * blockDim.x) {
+    const int i3 = o / ((long)n1 * n2);
+    float g0 = M.gammadot0[i3];
+    if (dg0) g0 = g0 + dg0[o];
+    const float* s = sig + 6 * o;
+    const float* e = E + 6 * o;
+    const T6 ev{e[0], e[1], e[2], e[3], e[4], e[5]};
+    T6 m{0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, r;
+    const float t = transient_point(M, mu, i3, g0, deviatoric(T6{s[0], s[1], s[2], s[3], s[4], s[5]}), ev, m, r);
+    float* d = Ed + 6 * o;
+    d[0] = r.s11; d[1] = r.s12; d[2] = r.s13; d[3] = r.s22; d[4] = r.s23; d[5] = r.s33;
+    if (t < __builtin_inff()) {
+      float* mo = moment + 6 * o;
+      mo[0] = mo[0] + m.s11; mo[1] = mo[1] + m.s12; mo[2] = mo[2] + m.s13;
+      mo[3] = mo[3] + m.s22; mo[4] = mo[4] + m.s23; mo[5] = mo[5] + m.s33;
+    }
+    tmin = fminf(tmin, t);
+  }
+  for (int off = 32; off > 0; off >>= 1) tmin = fminf(tmin, __shfl_xor(tmin, off));
+  if ((threadIdx.x & 63) == 0 && tmin < __builtin_inff()) atomicMin(tmin_bits, __float_as_uint(tmin));
+}
+
+// builddgammadot0, viscoelastic3d.f90:320-435 (fdgammadot0 :351-433): one
+// thread per point, the zones' contributions summed in their order.  The
+// gate |x_i - c_i| <= LM uses the largest tapered half extent, so zones
+// with all three extents negative (spheres) add nothing, as in the
+// reference.  No contraction into fused multiply-adds: a grid point on a
+// zone face then lands on the same side as in the reference's arithmetic.
+__global__ void k_builddgammadot0(int n1, int n2, int n3, double dx1, double dx2, double dx3, double beta, int nz,
+                                  const WeakDev* __restrict__ zones, float* dg) {
+#pragma clang fp contract(off)
+  const long N = (long)n1 * n2 * n3;
+  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (o >= N) return;
+  const int i1 = o % n1, i2 = (o / n1) % n2, i3 = o / ((long)n1 * n2);
+  const double x1 = d_shifted(i1, n1, dx1), x2 = d_shifted(i2, n2, dx2), x3 = (double)i3 * dx3;
+  double f = 0.0;
+  for (int i = 0; i < nz; ++i) {
+    const WeakDev& Z = zones[i];
+    if ((fabs(x3 - Z.z) > Z.LM) || (fabs(x1 - Z.x) > Z.LM) || (fabs(x2 - Z.y) > Z.LM)) continue;
+    const double x2r = Z.cs * x1 - Z.ss * x2;
+    const double x1s = Z.cd * x2r - Z.sd * x3, x2s = Z.ss * x1 + Z.cs * x2, x3s = Z.sd * x2r + Z.cd * x3;
+    const double a = (x1s - Z.xr) / Z.D, b = (x2s - Z.yr) / Z.W, c = (x3s - Z.zr) / Z.L;
+    const double D = Z.D, W = Z.W, L = Z.L;
+    if (D >= 0 && W >= 0 && L >= 0) f = f + d_omega(a, beta) * d_omega(b, beta) * d_omega(c, beta) * Z.dg;
+    else if (D < 0 && W < 0 && L < 0) f = f + d_omega(sqrt(a * a + b * b + c * c), beta) * Z.dg;
+    else if (D < 0 && W < 0) f = f + d_omega(sqrt(a * a + b * b), beta) * d_omega(c, beta) * Z.dg;
+    else if (D < 0 && L < 0) f = f + d_omega(sqrt(a * a + c * c), beta) * d_omega(b, beta) * Z.dg;
+    else if (W < 0 && L < 0) f = f + d_omega(sqrt(b * b + c * c), beta) * d_omega(a, beta) * Z.dg;
+  }
+  dg[o] = (float)f;
 }
 
 __global__ void k_friction_generic(RheoParams R, int plane, int n3, const float* __restrict__ sig, float* moment,
@@ -743,6 +805,105 @@ int relax_viscouseigenstress(double mu, const relax_layer_t* structure, int nstr
     tm = std::fmin(tm, (float)*maxwelltime);
     *maxwelltime = std::fmin((double)tm, *maxwelltime);
   }
+  return RELAX_OK;
+}
+
+WeakDev make_weak(const relax_weak_t& z, double beta) {
+  WeakDev w;
+  w.dg = z.dgammadot0; w.x = z.x; w.y = z.y; w.z = z.z;
+  // the zone's length is its x2 extent W, its width the x3 extent L
+  w.W = z.length; w.L = z.width; w.D = z.thickness;
+  const double Wp = w.W * (1.0 + 2.0 * beta) / 2.0, Lp = w.L * (1.0 + 2.0 * beta) / 2.0,
+               Dp = w.D * (1.0 + 2.0 * beta) / 2.0;
+  w.LM = std::max(Wp, std::max(Lp, Dp));
+  w.cs = std::cos(z.strike); w.ss = std::sin(z.strike);
+  w.cd = std::cos(z.dip); w.sd = std::sin(z.dip);
+  const double x2r = w.cs * z.x - w.ss * z.y;
+  w.xr = w.cd * x2r - w.sd * z.z;
+  w.yr = w.ss * z.x + w.cs * z.y;
+  w.zr = w.sd * x2r + w.cd * z.z;
+  return w;
+}
+
+int launch_builddgammadot0(int n1, int n2, int n3, double dx1, double dx2, double dx3, double beta, int nz,
+                           const WeakDev* zones, float* dg, hipStream_t st) {
+  const long N = (long)n1 * n2 * n3;
+  k_builddgammadot0<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(n1, n2, n3, dx1, dx2, dx3, beta, nz, zones, dg);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
+int relax_transienteigenstress(double mu, const relax_layer_t* structure, int nstructure,
+                               const relax_tensor_t* sig, const relax_tensor_t* epsilonik, int sx1, int sx2,
+                               int sx3, double dx1, double dx2, double dx3, relax_tensor_t* moment,
+                               relax_tensor_t* epsilonikdot, double* maxwelltime, const float* dgammadot0) {
+  (void)dx1; (void)dx2; (void)dx3;
+  if (!structure || !sig || !epsilonik || !moment || !epsilonikdot) return RELAX_ERR_ARG;
+  // viscoelastic3d.f90:144: a structure of the wrong size is ignored
+  if (nstructure != sx3) return RELAX_OK;
+  for (int i = 0; i < sx3; ++i)
+    if (structure[i].stressexponent < 0.999999) return RELAX_ERR_POWER;
+  int rc;
+  if ((rc = has_device())) return rc;
+  const long N = (long)sx1 * sx2 * sx3;
+  Buf tab, s, e, m, ed, dg, tb;
+  std::vector<float> h(3 * sx3);
+  for (int i = 0; i < sx3; ++i) {
+    h[i] = (float)structure[i].gammadot0;
+    h[sx3 + i] = (float)structure[i].stressexponent;
+    h[2 * sx3 + i] = (float)structure[i].Gk;
+  }
+  if ((rc = tab.alloc(h.size() * 4)) || (rc = s.alloc(N * 24)) || (rc = e.alloc(N * 24)) ||
+      (rc = m.alloc(N * 24)) || (rc = tb.alloc(4)))
+    return rc;
+  const bool alias = epsilonik == epsilonikdot;
+  if (!alias && (rc = ed.alloc(N * 24))) return rc;
+  HIP_TRY(hipMemcpy(tab.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s.p, sig, N * 24, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(e.p, epsilonik, N * 24, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(m.p, moment, N * 24, hipMemcpyHostToDevice));
+  const unsigned int inf_bits = 0x7f800000u;
+  HIP_TRY(hipMemcpy(tb.p, &inf_bits, 4, hipMemcpyHostToDevice));
+  if (dgammadot0) {
+    if ((rc = dg.alloc(N * 4))) return rc;
+    HIP_TRY(hipMemcpy(dg.p, dgammadot0, N * 4, hipMemcpyHostToDevice));
+  }
+  Mechanism M{};
+  M.gammadot0 = tab.f(); M.power = tab.f() + sx3; M.cohesion = tab.f() + 2 * sx3; M.active = 1;
+  float* edp = alias ? e.f() : ed.f();
+  k_transient_generic<<<g1d(N), 256>>>(M, (float)mu, sx1, sx2, sx3, s.f(), e.f(), m.f(), edp,
+                                       dgammadot0 ? dg.f() : nullptr, (unsigned int*)tb.p);
+  if ((rc = sync_status(RELAX_OK))) return rc;
+  HIP_TRY(hipMemcpy(moment, m.p, N * 24, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(epsilonikdot, edp, N * 24, hipMemcpyDeviceToHost));
+  if (maxwelltime) {
+    unsigned int bits;
+    HIP_TRY(hipMemcpy(&bits, tb.p, 4, hipMemcpyDeviceToHost));
+    float tm;
+    std::memcpy(&tm, &bits, 4);
+    tm = std::fmin(tm, (float)*maxwelltime);
+    *maxwelltime = std::fmin((double)tm, *maxwelltime);
+  }
+  return RELAX_OK;
+}
+
+int relax_builddgammadot0(int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, double beta, int nz,
+                          const relax_weak_t* zones, float* dgammadot0) {
+  if (!dgammadot0 || nz < 0 || (nz > 0 && !zones) || sx1 < 1 || sx2 < 1 || sx3 < 1) return RELAX_ERR_ARG;
+  int rc;
+  if ((rc = has_device())) return rc;
+  const long N = (long)sx1 * sx2 * sx3;
+  Buf z, out;
+  if ((rc = out.alloc(N * 4))) return rc;
+  if (nz > 0) {
+    std::vector<WeakDev> wz(nz);
+    for (int i = 0; i < nz; ++i) wz[i] = make_weak(zones[i], beta);
+    if ((rc = z.alloc((size_t)nz * sizeof(WeakDev)))) return rc;
+    HIP_TRY(hipMemcpy(z.p, wz.data(), (size_t)nz * sizeof(WeakDev), hipMemcpyHostToDevice));
+  }
+  if ((rc = launch_builddgammadot0(sx1, sx2, sx3, dx1, dx2, dx3, beta, nz, (const WeakDev*)z.p, out.f(), 0)))
+    return rc;
+  if ((rc = sync_status(RELAX_OK))) return rc;
+  HIP_TRY(hipMemcpy(dgammadot0, out.p, N * 4, hipMemcpyDeviceToHost));
   return RELAX_OK;
 }
 

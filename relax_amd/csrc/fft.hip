@@ -251,21 +251,21 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long 
 }
 
 // --------------------------------------------------------- strided axis ----
-// Columns per workgroup of the strided transforms (compile-time tile choice;
-// RELAX_COLS_TX512 overrides the 512-point tile).
-#ifndef RELAX_COLS_TX512
-#define RELAX_COLS_TX512 8
-#endif
-template <int L>
+// Columns per workgroup of the strided transforms: variant 0 is the
+// default, 1 half as many, 2 twice as many (within 64 columns and 64 KB of
+// LDS); the autotuner picks one per grid.
+template <int L, int V>
 struct ColTile {
-  static constexpr int TX = L >= 1024 ? 8 : (L == 512 ? RELAX_COLS_TX512 : (L >= 128 ? 16 : (2048 / L > 64 ? 64 : 2048 / L)));
+  static constexpr int T0 = L >= 512 ? 8 : (L >= 128 ? 16 : (2048 / L > 64 ? 64 : 2048 / L));
+  static constexpr int T1 = T0 / 2 < 4 ? 4 : T0 / 2;
+  static constexpr int T2 = (T0 * 2 > 64 || L * T0 * 2 * 8 > 65536) ? T0 : T0 * 2;
+  static constexpr int TX = V == 0 ? T0 : (V == 1 ? T1 : T2);
 };
 
-template <int L, int NT, int SIGN>
+template <int L, int NT, int SIGN, int TX>
 __global__ __launch_bounds__(NT) void k_fft_cols(MPtr<float2> data_m, long jstride, long pstride, int ncols,
                                                  const float2* __restrict__ tw) {
   float2* data = data_m.at(blockIdx.y);
-  constexpr int TX = ColTile<L>::TX;
   __shared__ float2 s[L * TX];
   __shared__ float2 stw[L];
   const int tid = threadIdx.x;
@@ -380,25 +380,31 @@ int launch_fft_x_c2r(int n1, int epi, MPtr<const float2> in, long in_pitch, MPtr
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }
 
+template <int L, int SIGN, int V>
+static void cols_launch(MPtr<float2> data, long jstride, long pstride, int ncols, int nplanes, int nm,
+                        const float2* tw, hipStream_t st) {
+  constexpr int TX = ColTile<L, V>::TX;
+  const dim3 grid((unsigned)(((ncols + TX - 1) / TX) * (long)nplanes), (unsigned)nm);
+  k_fft_cols<L, NT_FFT, SIGN, TX><<<grid, NT_FFT, 0, st>>>(data, jstride, pstride, ncols, tw);
+}
+
 int launch_fft_cols(int L, int sign, MPtr<float2> data, long jstride, long pstride, int ncols, int nplanes, int nm,
-                    hipStream_t st) {
+                    hipStream_t st, int variant) {
   const float2* tw = twiddle_table(L);
   if (!tw) return RELAX_ERR_NOMEM;
-  int tx = 0;
-  switch (L) {
-#define T(N) case N: tx = ColTile<N>::TX; break;
-    T(16) T(32) T(64) T(128) T(256) T(512) T(1024)
-#undef T
-    default: return RELAX_ERR_SIZE;
-  }
-  const dim3 grid((unsigned)(((ncols + tx - 1) / tx) * (long)nplanes), (unsigned)nm);
-#define X(N, S) case N: k_fft_cols<N, NT_FFT, S><<<grid, NT_FFT, 0, st>>>(data, jstride, pstride, ncols, tw); break;
+  if (variant < 0 || variant >= COLS_VARIANTS) return RELAX_ERR_ARG;
+#define V(N, S)                                                                                 \
+  case N:                                                                                       \
+    if (variant == 0) cols_launch<N, S, 0>(data, jstride, pstride, ncols, nplanes, nm, tw, st); \
+    else if (variant == 1) cols_launch<N, S, 1>(data, jstride, pstride, ncols, nplanes, nm, tw, st); \
+    else cols_launch<N, S, 2>(data, jstride, pstride, ncols, nplanes, nm, tw, st);               \
+    break;
   if (sign < 0) {
-    switch (L) { X(16, -1) X(32, -1) X(64, -1) X(128, -1) X(256, -1) X(512, -1) X(1024, -1) default: return RELAX_ERR_SIZE; }
+    switch (L) { V(16, -1) V(32, -1) V(64, -1) V(128, -1) V(256, -1) V(512, -1) V(1024, -1) default: return RELAX_ERR_SIZE; }
   } else {
-    switch (L) { X(16, 1) X(32, 1) X(64, 1) X(128, 1) X(256, 1) X(512, 1) X(1024, 1) default: return RELAX_ERR_SIZE; }
+    switch (L) { V(16, 1) V(32, 1) V(64, 1) V(128, 1) V(256, 1) V(512, 1) V(1024, 1) default: return RELAX_ERR_SIZE; }
   }
-#undef X
+#undef V
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }
 

@@ -138,11 +138,43 @@ __device__ __forceinline__ void cerruti_point(const ColumnConsts& C, const Cerru
 }  // namespace
 
 // ---------------------------------------------------- fused column pass ----
-// largest radix of the column transforms (compile-time choice)
-#ifndef GREEN_RMAX
-#define GREEN_RMAX 16
-#endif
-template <int H3, int TX, int NT>
+// Addresses and wavenumbers of one tile: TX columns of the spectrum (or of
+// its Nyquist column), all depths, all three components.
+struct GreenTileAddr {
+  float2* base;
+  const float2* tbase;
+  long cs, zs, tcs;
+  int k1i, k2i;
+};
+
+template <int TX>
+__device__ __forceinline__ GreenTileAddr green_tile(int bid, int tx, int n1, int n2, int h3, int l2, float2* W,
+                                                    float2* NQ, const float2* TM, const float2* TN) {
+  GreenTileAddr A;
+  const int m1 = n1 / 2;
+  const int nmain = (m1 / TX) * n2;
+  if (bid < nmain) {
+    // spectrum row r of W; its y frequency is r, or 8*(r % l2) + r / l2 in
+    // the split-y order; the traction planes are in natural order
+    const int r = bid / (m1 / TX), k10 = (bid % (m1 / TX)) * TX;
+    const int k2 = l2 ? 8 * (r % l2) + r / l2 : r;
+    A.base = W + (long)r * m1 + k10;
+    A.cs = (long)h3 * n2 * m1; A.zs = (long)n2 * m1;
+    A.tbase = TM + (long)k2 * m1 + k10; A.tcs = (long)n2 * m1;
+    A.k1i = k10 + tx; A.k2i = k2;
+  } else {
+    const int k20 = (bid - nmain) * TX;
+    A.base = NQ + k20;
+    A.cs = (long)h3 * n2; A.zs = n2;
+    A.tbase = TN + k20; A.tcs = n2;
+    A.k1i = m1; A.k2i = k20 + tx;
+  }
+  return A;
+}
+
+// RM: the largest radix of the column transforms (8 or 16).  One tile per
+// workgroup.
+template <int H3, int TX, int NT, int RM>
 __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu, double alpha, double gamma,
                                                       int n1, int n2, int n3, MPtr<float2> W_m, MPtr<float2> NQ_m,
                                                       MPtr<const float2> TM_m, MPtr<const float2> TN_m,
@@ -154,6 +186,7 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
   const float2* __restrict__ TN = TN_m.at(blockIdx.y);
   static_assert(NT % 64 == 0 && 64 % TX == 0, "a thread keeps one column; whole waves");
   constexpr int NW = NT / 64;
+  constexpr int NE = 3 * H3 * TX;  // input elements of a tile
   // LDS: six sequences (component c, half) of H3 x TX complex.  The stride
   // between sequences is padded by 12 float2 so that the lanes of a wave,
   // which walk batches across sequences in the FFT stages, spread over the
@@ -165,139 +198,139 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
   __shared__ float2 stw[2 * H3];  // the n3-point twiddles, staged in LDS
   const int tid = threadIdx.x;
   const int tx = tid % TX;
+  const int h3 = H3;
   for (int e = tid; e < 2 * H3; e += NT) stw[e] = tw3[e];
-  const int m1 = n1 / 2;
-  const int nmain = (m1 / TX) * n2;
-  const int bid = xcd_tile(blockIdx.x, gridDim.x);
-  float2* base;
-  const float2* tbase;
-  long cs, zs, tcs;
-  int k1i, k2i;
-  if (bid < nmain) {
-    // spectrum row r of W; its y frequency is r, or 8*(r % l2) + r / l2 in
-    // the split-y order; the traction planes are in natural order
-    const int r = bid / (m1 / TX), k10 = (bid % (m1 / TX)) * TX;
-    const int k2 = l2 ? 8 * (r % l2) + r / l2 : r;
-    base = W + (long)r * m1 + k10;
-    cs = (long)H3 * n2 * m1; zs = (long)n2 * m1;
-    tbase = TM + (long)k2 * m1 + k10; tcs = (long)n2 * m1;
-    k1i = k10 + tx; k2i = k2;
-  } else {
-    const int k20 = (bid - nmain) * TX;
-    base = NQ + k20;
-    cs = (long)H3 * n2; zs = n2;
-    tbase = TN + k20; tcs = n2;
-    k1i = m1; k2i = k20 + tx;
-  }
-  const float k1 = (float)k1i * C.ik1;
-  const float k2 = wavenumber23f(k2i, n2, C.ik2);
 
-  // load the upper half and its w^i3-shifted copy
-  for (int e = tid; e < 3 * H3 * TX; e += NT) {
-    const int t = e % TX, z = (e / TX) % H3, c = e / (TX * H3);
-    const float2 x = base[c * cs + z * zs + t];
-    s[(2 * c) * SB + z * TX + t] = x;
-    s[(2 * c + 1) * SB + z * TX + t] = cmul(x, tw3[z]);
-  }
-  __syncthreads();
-  lds_fft<H3, 6 * TX, NT, TX, 1, SB, TX, -1, 2, true, GREEN_RMAX>(s, stw, tid);
+  // transform, Green + traction, p = t - p, Cerruti, inverse, store
+  auto body = [&](const GreenTileAddr& A) {
+    const float k1 = (float)A.k1i * C.ik1;
+    const float k2 = wavenumber23f(A.k2i, n2, C.ik2);
+    lds_fft<H3, 6 * TX, NT, TX, 1, SB, TX, -1, 2, true, RM>(s, stw, tid);
 
-  // Green's function and the traction partial sums of this thread's column
-  float2 ps1 = make_float2(0.f, 0.f), ps2 = ps1, ps3 = ps1;
-  for (int e = tid; e < 2 * H3 * TX; e += NT) {
-    const int slot = e / TX, half = slot / H3, m = slot % H3;
-    const int k3i = 2 * m + half;
-    const float k3 = wavenumber23f(k3i, n3, C.ik3);
-    float2& c1 = s[(0 + half) * SB + m * TX + tx];
-    float2& c2 = s[(2 + half) * SB + m * TX + tx];
-    float2& c3 = s[(4 + half) * SB + m * TX + tx];
-    float2 a1 = c1, a2 = c2, a3 = c3;
-    if (k1i == 0 && k2i == 0 && k3i == 0) {
-      a1 = a2 = a3 = make_float2(0.f, 0.f);
-    } else {
-      elastic_point(C, k1, k2, k3, a1, a2, a3);
-    }
-    c1 = a1; c2 = a2; c3 = a3;
-    float2 s1, s2, s3;
-    traction_point(C, k1, k2, k3, a1, a2, a3, s1, s2, s3);
-    ps1 = cadd(ps1, s1); ps2 = cadd(ps2, s2); ps3 = cadd(ps3, s3);
-  }
-  // reduce over the lanes of the wave that hold the same column (fixed
-  // order: bitwise reproducible), then over the waves
-#pragma unroll
-  for (int off = 32; off >= TX; off >>= 1) {
-    ps1.x += __shfl_xor(ps1.x, off); ps1.y += __shfl_xor(ps1.y, off);
-    ps2.x += __shfl_xor(ps2.x, off); ps2.y += __shfl_xor(ps2.y, off);
-    ps3.x += __shfl_xor(ps3.x, off); ps3.y += __shfl_xor(ps3.y, off);
-  }
-  if ((tid & 63) < TX) {
-    red[0][tid >> 6][tx] = ps1; red[1][tid >> 6][tx] = ps2; red[2][tid >> 6][tx] = ps3;
-  }
-  __syncthreads();
-  if (tid < 3 * TX) {
-    const int c = tid / TX, t = tid % TX;
-    float2 acc = red[c][0][t];
-    for (int w = 1; w < NW; ++w) acc = cadd(acc, red[c][w][t]);
-    const float2 tt = tbase[c * tcs + t];
-    // p = t - p / (sx3 dx3)   (green.f90:373-375, :1082-1084)
-    pv[c][t] = make_float2(tt.x - acc.x / C.inv_n3dx3, tt.y - acc.y / C.inv_n3dx3);
-  }
-  __syncthreads();
-  {
-    const float2 p1 = pv[0][tx], p2 = pv[1][tx], p3 = pv[2][tx];
-    const CerrutiCol cc = cerruti_column(C, mu, alpha, gamma, k1, k2, p1, p2, p3);
+    // Green's function and the traction partial sums of this thread's column
+    float2 ps1 = make_float2(0.f, 0.f), ps2 = ps1, ps3 = ps1;
     for (int e = tid; e < 2 * H3 * TX; e += NT) {
       const int slot = e / TX, half = slot / H3, m = slot % H3;
-      const float k3 = wavenumber23f(2 * m + half, n3, C.ik3);
+      const int k3i = 2 * m + half;
+      const float k3 = wavenumber23f(k3i, n3, C.ik3);
       float2& c1 = s[(0 + half) * SB + m * TX + tx];
       float2& c2 = s[(2 + half) * SB + m * TX + tx];
       float2& c3 = s[(4 + half) * SB + m * TX + tx];
       float2 a1 = c1, a2 = c2, a3 = c3;
-      cerruti_point(C, cc, p1, p2, k3, a1, a2, a3);
+      if (A.k1i == 0 && A.k2i == 0 && k3i == 0) {
+        a1 = a2 = a3 = make_float2(0.f, 0.f);
+      } else {
+        elastic_point(C, k1, k2, k3, a1, a2, a3);
+      }
       c1 = a1; c2 = a2; c3 = a3;
+      float2 s1, s2, s3;
+      traction_point(C, k1, k2, k3, a1, a2, a3, s1, s2, s3);
+      ps1 = cadd(ps1, s1); ps2 = cadd(ps2, s2); ps3 = cadd(ps3, s3);
     }
+    // reduce over the lanes of the wave that hold the same column (fixed
+    // order: bitwise reproducible), then over the waves
+#pragma unroll
+    for (int off = 32; off >= TX; off >>= 1) {
+      ps1.x += __shfl_xor(ps1.x, off); ps1.y += __shfl_xor(ps1.y, off);
+      ps2.x += __shfl_xor(ps2.x, off); ps2.y += __shfl_xor(ps2.y, off);
+      ps3.x += __shfl_xor(ps3.x, off); ps3.y += __shfl_xor(ps3.y, off);
+    }
+    if ((tid & 63) < TX) {
+      red[0][tid >> 6][tx] = ps1; red[1][tid >> 6][tx] = ps2; red[2][tid >> 6][tx] = ps3;
+    }
+    __syncthreads();
+    if (tid < 3 * TX) {
+      const int c = tid / TX, t = tid % TX;
+      float2 acc = red[c][0][t];
+      for (int w = 1; w < NW; ++w) acc = cadd(acc, red[c][w][t]);
+      const float2 tt = A.tbase[c * A.tcs + t];
+      // p = t - p / (sx3 dx3)   (green.f90:373-375, :1082-1084)
+      pv[c][t] = make_float2(tt.x - acc.x / C.inv_n3dx3, tt.y - acc.y / C.inv_n3dx3);
+    }
+    __syncthreads();
+    {
+      const float2 p1 = pv[0][tx], p2 = pv[1][tx], p3 = pv[2][tx];
+      const CerrutiCol cc = cerruti_column(C, mu, alpha, gamma, k1, k2, p1, p2, p3);
+      for (int e = tid; e < 2 * H3 * TX; e += NT) {
+        const int slot = e / TX, half = slot / H3, m = slot % H3;
+        const float k3 = wavenumber23f(2 * m + half, n3, C.ik3);
+        float2& c1 = s[(0 + half) * SB + m * TX + tx];
+        float2& c2 = s[(2 + half) * SB + m * TX + tx];
+        float2& c3 = s[(4 + half) * SB + m * TX + tx];
+        float2 a1 = c1, a2 = c2, a3 = c3;
+        cerruti_point(C, cc, p1, p2, k3, a1, a2, a3);
+        c1 = a1; c2 = a2; c3 = a3;
+      }
+    }
+    __syncthreads();
+    lds_fft<H3, 6 * TX, NT, TX, 1, SB, TX, +1, 2, true, RM>(s, stw, tid);
+    // upper half of the inverse: y[n] = E'[n] + w^-n O'[n]
+    for (int e = tid; e < NE; e += NT) {
+      const int t = e % TX, z = (e / TX) % H3, c = e / (TX * H3);
+      const float2 ev = s[(2 * c) * SB + z * TX + t];
+      const float2 od = s[(2 * c + 1) * SB + z * TX + t];
+      A.base[c * A.cs + z * A.zs + t] = cadd(ev, cmul(od, cconj(stw[z])));
+    }
+  };
+
+  const GreenTileAddr A = green_tile<TX>(xcd_tile(blockIdx.x, gridDim.x), tx, n1, n2, h3, l2, W, NQ, TM, TN);
+  // load the upper half and its w^i3-shifted copy
+  for (int e = tid; e < NE; e += NT) {
+    const int t = e % TX, z = (e / TX) % H3, c = e / (TX * H3);
+    const float2 x = A.base[c * A.cs + z * A.zs + t];
+    s[(2 * c) * SB + z * TX + t] = x;
+    s[(2 * c + 1) * SB + z * TX + t] = cmul(x, tw3[z]);
   }
   __syncthreads();
-  lds_fft<H3, 6 * TX, NT, TX, 1, SB, TX, +1, 2, true, GREEN_RMAX>(s, stw, tid);
-  // upper half of the inverse: y[n] = E'[n] + w^-n O'[n]
-  for (int e = tid; e < 3 * H3 * TX; e += NT) {
-    const int t = e % TX, z = (e / TX) % H3, c = e / (TX * H3);
-    const float2 ev = s[(2 * c) * SB + z * TX + t];
-    const float2 od = s[(2 * c + 1) * SB + z * TX + t];
-    base[c * cs + z * zs + t] = cadd(ev, cmul(od, cconj(stw[z])));
-  }
+  body(A);
 }
 
-template <int H3>
-struct GreenTile {
-  // columns and threads per workgroup (measured at 512x512x256: 4 columns
-  // of 128 with 256 threads beat 8 with 512 threads by 4 %)
-  static constexpr int TX = H3 >= 128 ? 4 : 16;
-  static constexpr int NT = H3 >= 256 ? 512 : 256;
+// Launch shapes of the column pass: columns per workgroup, threads, largest
+// radix.  Variant 0 is the default (measured at 512x512x256: 4 columns of
+// 128 with 256 threads and radix 16 is 4 % faster than 8 columns with 512
+// threads, 25 % faster than radix 8); the autotuner may pick another.
+template <int H3, int V>
+struct GreenVariant {
+  static constexpr int TX0 = H3 >= 128 ? 4 : 16;
+  static constexpr int NT0 = H3 >= 256 ? 512 : 256;
+  static constexpr int TX = V == 1 ? (H3 >= 256 ? 4 : TX0 * 2 > 16 ? 16 : TX0 * 2) : TX0;
+  static constexpr int NT = V == 1 ? (H3 >= 256 ? 256 : 512) : NT0;
+  static constexpr int RM = V == 2 ? 8 : 16;
 };
 
+template <int HH, int V>
+static int green_launch(const GreenConst& gc, const ColumnConsts& C, double alpha, MPtr<float2> W, MPtr<float2> NQ,
+                        MPtr<const float2> TM, MPtr<const float2> TN, int nm, const float2* tw3, int l2,
+                        hipStream_t st) {
+  constexpr int TX = GreenVariant<HH, V>::TX, NT = GreenVariant<HH, V>::NT, RM = GreenVariant<HH, V>::RM;
+  const int m1 = gc.n1 / 2;
+  if (m1 % TX || gc.n2 % TX) return RELAX_ERR_SIZE;
+  const int nb = (m1 / TX) * gc.n2 + gc.n2 / TX;
+  k_green_columns<HH, TX, NT, RM><<<dim3(nb, nm), NT, 0, st>>>(C, gc.mu, alpha, gc.gamma, gc.n1, gc.n2, gc.n3, W,
+                                                               NQ, TM, TN, tw3, l2);
+  return RELAX_OK;
+}
+
 int launch_green_columns(const GreenConst& gc, int h3, MPtr<float2> W, MPtr<float2> NQ, MPtr<const float2> TM,
-                         MPtr<const float2> TN, int nm, hipStream_t st, int l2) {
+                         MPtr<const float2> TN, int nm, hipStream_t st, int l2, int variant) {
   const float2* tw3 = twiddle_table(gc.n3);
   if (!tw3) return RELAX_ERR_NOMEM;
+  if (variant < 0 || variant >= GREEN_VARIANTS) return RELAX_ERR_ARG;
   const ColumnConsts C = make_consts(gc);
   const double alpha = (gc.lambda + gc.mu) / (gc.lambda + 2.0 * gc.mu);
-  const int m1 = gc.n1 / 2;
+  int rc = RELAX_OK;
 #define X(HH)                                                                                          \
-  case HH: {                                                                                           \
-    constexpr int TX = GreenTile<HH>::TX, NT = GreenTile<HH>::NT;                                      \
-    if (m1 % TX || gc.n2 % TX) return RELAX_ERR_SIZE;                                                  \
-    const int nb = (m1 / TX) * gc.n2 + gc.n2 / TX;                                                     \
-    k_green_columns<HH, TX, NT><<<dim3(nb, nm), NT, 0, st>>>(C, gc.mu, alpha, gc.gamma, gc.n1, gc.n2, gc.n3, \
-                                                   W, NQ, \
-                                                   TM, TN, tw3, l2);                                   \
-    break;                                                                                             \
-  }
+  case HH:                                                                                             \
+    if (variant == 0) rc = green_launch<HH, 0>(gc, C, alpha, W, NQ, TM, TN, nm, tw3, l2, st);          \
+    else if (variant == 1) rc = green_launch<HH, 1>(gc, C, alpha, W, NQ, TM, TN, nm, tw3, l2, st);     \
+    else rc = green_launch<HH, 2>(gc, C, alpha, W, NQ, TM, TN, nm, tw3, l2, st);                       \
+    break;
   switch (h3) {
     X(8) X(16) X(32) X(64) X(128) X(256)
     default: return RELAX_ERR_SIZE;
   }
 #undef X
+  if (rc) return rc;
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }
 

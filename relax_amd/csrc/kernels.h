@@ -20,8 +20,11 @@ int launch_fft_x_c2r(int n1, int epi, MPtr<const float2> in, long in_pitch, MPtr
                      long nyq_stride, MPtr<float> out, long out_pitch, long nrows, float scale,
                      MPtr<const float> base, long base_pitch, MPtr<const StepScalars> sc, int nm, hipStream_t st,
                      int ys_n2 = 0);
+// launch-shape variants of the tunable kernels (0 = default)
+constexpr int COLS_VARIANTS = 3, GREEN_VARIANTS = 3, BODY_VARIANTS = 2;
+
 int launch_fft_cols(int L, int sign, MPtr<float2> data, long jstride, long pstride, int ncols, int nplanes, int nm,
-                    hipStream_t st);
+                    hipStream_t st, int variant = 0);
 int launch_fft_rows(int L, int sign, MPtr<float2> data, long pitch, long nrows, int nm, hipStream_t st);
 // one buffer
 inline int launch_fft_x_r2c(int n1, const float* in, long in_pitch, float2* out, long out_pitch, float2* nyq,
@@ -53,7 +56,7 @@ struct GreenConst {
 // l2 > 0: the spectrum rows of W are in the split-y order (row l2*k2 + k1
 // holds y frequency 8*k1 + k2); the traction planes TM are in natural order
 int launch_green_columns(const GreenConst& gc, int h3, MPtr<float2> W, MPtr<float2> NQ, MPtr<const float2> TM,
-                         MPtr<const float2> TN, int nm, hipStream_t st, int l2 = 0);
+                         MPtr<const float2> TN, int nm, hipStream_t st, int l2 = 0, int variant = 0);
 // per-routine kernels on padded full spectra (row pitch n1/2+1 complex)
 int launch_elastic_padded(const GreenConst& gc, float2* f1, float2* f2, float2* f3, hipStream_t st);
 int launch_surface_padded(const GreenConst& gc, const float2* u1, const float2* u2, const float2* u3,
@@ -61,3 +64,18 @@ int launch_surface_padded(const GreenConst& gc, const float2* u1, const float2* 
                           float2* p3, hipStream_t st);
 int launch_cerruti_padded(const GreenConst& gc, const float2* p1, const float2* p2, const float2* p3,
                           float2* u1, float2* u2, float2* u3, hipStream_t st);
+
+// compat.hip: the weak-zone fluidity field of builddgammadot0
+// (viscoelastic3d.f90:320-435) on the device.  A zone in device form holds
+// what depends on the zone only, computed on the host in double as the
+// reference does (its trigonometry from the host libm), so the kernel is
+// left with additions, products and quotients in the reference's order.
+struct WeakDev {
+  double dg, x, y, z;       // dgammadot0 and centre
+  double W, L, D, LM;       // x2, x3, x1 extents and the gate half-width
+  double cs, ss, cd, sd;    // cos/sin of strike and dip
+  double xr, yr, zr;        // rotated centre
+};
+WeakDev make_weak(const relax_weak_t& z, double beta);
+int launch_builddgammadot0(int n1, int n2, int n3, double dx1, double dx2, double dx3, double beta, int nz,
+                           const WeakDev* zones, float* dg, hipStream_t st);

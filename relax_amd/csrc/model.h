@@ -6,6 +6,12 @@
 #include "rheology.h"
 #include "stencil.h"
 
+// Rows of the per-depth table block (each sx3/2 floats): viscous mechanism k
+// at 3k (gammadot0, power, cohesion), deviatoric prestress at 10 (six rows),
+// its norm at 16, prestress rates at 17+k, (taup/mu)^(n-1) at 19+k,
+// transient mechanism k at 21+3k (gammadot0, power, Gk).
+static const long TABLE_ROWS = 27;
+
 struct relax_model {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -25,7 +31,12 @@ struct relax_model {
   double2* partials = nullptr;
   long nblk = 0;  // stress-kernel workgroups (first-level amplitude partials)
   long nred = 0;  // second-level partials
-  float* tables = nullptr;  // per-depth rheology tables (17 * sx3/2 floats)
+  float* E = nullptr;       // 6 planes of H: transient strain (with a transient structure)
+  float* Em = nullptr;      // 6 planes of H: its mid-step value
+  float* dg[4] = {nullptr, nullptr, nullptr, nullptr};  // weak-zone fields (H each)
+  WeakDev* zones_dev = nullptr;  // staging of set_weakzones' zones
+  int zones_cap = 0;
+  float* tables = nullptr;  // per-depth rheology tables (TABLE_ROWS * sx3/2 floats)
   double* ftab = nullptr;   // fault-creep tables: v0, (a-b)sigma, friction, cohesion
 
   PlaneDev* planes_dev = nullptr;
@@ -40,7 +51,7 @@ struct relax_model {
   std::vector<long> patch_off;
   StepScalars* host_sc = nullptr;  // pinned
   int host_cap = 0;
-  int mech[3] = {0, 0, 0};
+  int mech[NMECH] = {0, 0, 0, 0, 0};
   std::vector<double> hg0[2], hn[2], htaup;  // host copies for the prestress-rate table
   RheoParams R{};
   StencilConst S{};
@@ -53,6 +64,9 @@ struct relax_model {
   unsigned md_version = 0;
   relax_model* self = nullptr;
   relax_batch* batch = nullptr;  // the batch the model is a member of
+  // launch-shape variants of the tunable kernels (relax_model_autotune):
+  // y transform columns, Green column pass, divergence tiles
+  int tune_cols = 0, tune_green = 0, tune_body = 0;
   // phase timers (relax_model_profile)
   bool prof_on = false;
   int prof_n = 0;

@@ -12,9 +12,11 @@
 //   corrector  k_bodyforce(sig) -> monitorfriction -> Green(W) -> u += Dt G
 //              -> k_stress: tau += Dt m(sig), gamma += Dt gdot, sig = -tau + C:eps(u)
 //              -> frictionadd -> k_end_step (t, oi, power, C:E^i)
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -32,10 +34,10 @@ static int wait_batch(relax_model_t* m);
 // ------------------------------------------------------------- kernels ----
 __global__ void k_reset_scalars(StepScalars* sc, double t, int oi, int it) {
   sc->t = t; sc->Dt = 0; sc->tm = 0;
-  sc->maxwell[0] = sc->maxwell[1] = sc->maxwell[2] = STEP_MAX;
+  for (int k = 0; k < NMECH; ++k) sc->maxwell[k] = STEP_MAX;
   sc->power = sc->cei = 0;
   sc->oi = oi; sc->it = it; sc->status = 0;
-  for (int k = 0; k < 3; ++k) sc->tmin_bits[k] = 0x7f800000u;
+  for (int k = 0; k < NMECH; ++k) sc->tmin_bits[k] = 0x7f800000u;
 }
 
 // One thread per model: the relaxation-time minima of the predictor's
@@ -46,14 +48,14 @@ __global__ void k_finalize_dt(const ModelDev* __restrict__ md, int nm, int use_o
   if (mi >= nm) return;
   StepScalars* sc = md[mi].sc;
   const TimeLine tl = md[mi].tl;
-  double maxwell[3];
-  for (int k = 0; k < 3; ++k) {
+  double maxwell[NMECH], rate = 0.0;
+  for (int k = 0; k < NMECH; ++k) {
     const float v = fminf(__uint_as_float(sc->tmin_bits[k]), (float)STEP_MAX);
     maxwell[k] = fmin((double)v, STEP_MAX);
     if (!tl.mech[k]) maxwell[k] = STEP_MAX;
   }
-  double tm = 1.0 / ((double)tl.mech[0] / maxwell[0] + (double)tl.mech[1] / maxwell[1] +
-                     (double)tl.mech[2] / maxwell[2]);
+  for (int k = 0; k < NMECH; ++k) rate = rate + (double)tl.mech[k] / maxwell[k];
+  double tm = 1.0 / rate;
   tm = fmin(tm, STEP_MAX);
   const double t = sc->t;
   double Dt = tm / 10.0;
@@ -66,7 +68,7 @@ __global__ void k_finalize_dt(const ModelDev* __restrict__ md, int nm, int use_o
   if (use_override) Dt = dt_override;
   sc->Dt = Dt;
   sc->tm = tm;
-  for (int k = 0; k < 3; ++k) sc->maxwell[k] = maxwell[k];
+  for (int k = 0; k < NMECH; ++k) sc->maxwell[k] = maxwell[k];
   sc->it = sc->it + 1;
 }
 
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(256) void k_end_step(const ModelDev* __restrict__ m
     const bool out = ((0 == tl.skip) && (fabs(t - oi * tl.odt) < 1e-6 * tl.odt)) ||
                      ((0 < tl.skip) && ((it - 1) % tl.skip == 0)) || (fabs(t - tl.etime) < 1e-6);
     if (out) sc->oi = oi + 1;
-    for (int k = 0; k < 3; ++k) sc->tmin_bits[k] = 0x7f800000u;
+    for (int k = 0; k < NMECH; ++k) sc->tmin_bits[k] = 0x7f800000u;
   }
 }
 
@@ -250,7 +252,8 @@ size_t relax_model_footprint(const relax_model_desc_t* d) {
 static void free_model(relax_model_t* m) {
   if (!m) return;
   void* ptrs[] = {m->u, m->sig, m->tau, m->gam, m->W, m->NQ, m->T, m->TN, m->sc, m->partials, m->tables, m->ftab,
-                  m->planes_dev, m->patches_dev, m->md_dev, m->shear_dev, m->shear_slip};
+                  m->planes_dev, m->patches_dev, m->md_dev, m->shear_dev, m->shear_slip, m->E, m->Em,
+                  m->dg[0], m->dg[1], m->dg[2], m->dg[3], m->zones_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->host_sc) (void)hipHostFree(m->host_sc);
@@ -259,14 +262,46 @@ static void free_model(relax_model_t* m) {
   delete m;
 }
 
+// --------------------------------------------------------- tuning table ----
+// Launch-shape choices per device and grid, found by relax_model_autotune
+// and used by every later model of that grid in the process.
+struct TuneKey {
+  int dev, n1, n2, n3;
+  bool operator<(const TuneKey& o) const {
+    return dev != o.dev ? dev < o.dev : n1 != o.n1 ? n1 < o.n1 : n2 != o.n2 ? n2 < o.n2 : n3 < o.n3;
+  }
+};
+struct TuneChoice {
+  int cols, green, body;
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TuneChoice> g_tune;
+
+static void tune_lookup(relax_model_t* m) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(TuneKey{m->device, m->g.n1, m->g.n2, m->g.n3});
+  if (it == g_tune.end()) return;
+  m->tune_cols = it->second.cols;
+  m->tune_green = it->second.green;
+  m->tune_body = it->second.body;
+}
+
 static TimeLine timeline(const relax_model_t* m) {
   TimeLine tl;
   tl.interval = m->desc.interval; tl.odt = m->desc.odt; tl.tscale = m->desc.tscale; tl.skip = m->desc.skip;
   tl.has_next = m->events.size() > 1;
   tl.next_event = tl.has_next ? m->events[1] : 0.0;
   tl.etime = m->events.empty() ? 0.0 : m->events[0];
-  for (int k = 0; k < 3; ++k) tl.mech[k] = m->mech[k];
+  for (int k = 0; k < NMECH; ++k) tl.mech[k] = m->mech[k];
   return tl;
+}
+
+// The kernels a model needs: the extended rheology with weak zones or
+// transient creep, the fault-creep kernels with afterslip planes.
+static int rheo_class(const relax_model_t* m) {
+  const RheoParams& R = m->R;
+  if (R.tr[0].active || R.tr[1].active || R.mech[0].dg || R.mech[1].dg) return RHEO_EXT;
+  return (R.fr.active && R.fr.np > 0) ? RHEO_CREEP : RHEO_VISCOUS;
 }
 
 // Refresh the model's device record after a change of its rheology, planes
@@ -276,6 +311,7 @@ static int sync_md(relax_model_t* m) {
   if (!m->md_dirty) return RELAX_OK;
   ModelDev& d = m->md_host;
   d.u = m->u; d.sig = m->sig; d.tau = m->tau; d.gam = m->gam; d.W = m->W; d.NQ = m->NQ; d.T = m->T; d.TN = m->TN;
+  d.E = m->E; d.Em = m->Em;
   d.sc = m->sc; d.partials = m->partials;
   d.dv = m->desc.dx1 * m->desc.dx2 * m->desc.dx3;
   d.tl = timeline(m);
@@ -323,7 +359,7 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
   m->nblk = stress_blocks(g);
   m->nred = (m->nblk + RED_CHUNK - 1) / RED_CHUNK;
   alloc((void**)&m->partials, (m->nblk + m->nred) * sizeof(double2));
-  alloc((void**)&m->tables, 19L * g.h3 * 4);
+  alloc((void**)&m->tables, TABLE_ROWS * g.h3 * 4);
   alloc((void**)&m->ftab, 4L * g.h3 * 8);
   alloc((void**)&m->md_dev, sizeof(ModelDev));
   m->self = m;
@@ -335,7 +371,7 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
       hipMemsetAsync(m->sig, 0, 6 * H * 4, m->stream) != hipSuccess ||
       hipMemsetAsync(m->tau, 0, 6 * H * 4, m->stream) != hipSuccess ||
       hipMemsetAsync(m->gam, 0, H * 4, m->stream) != hipSuccess ||
-      hipMemsetAsync(m->tables, 0, 19L * g.h3 * 4, m->stream) != hipSuccess ||
+      hipMemsetAsync(m->tables, 0, TABLE_ROWS * g.h3 * 4, m->stream) != hipSuccess ||
       hipMemsetAsync(m->ftab, 0, 4L * g.h3 * 8, m->stream) != hipSuccess) {
     free_model(m);
     return RELAX_ERR_HIP;
@@ -351,7 +387,17 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
     R.mech[k].power = m->tables + (3 * k + 1) * h3;
     R.mech[k].cohesion = m->tables + (3 * k + 2) * h3;
     R.mech[k].gdp = m->tables + (17 + k) * h3;
+    R.mech[k].rp = m->tables + (19 + k) * h3;
+    R.mech[k].dg = nullptr;
     R.mech[k].active = 0;
+    // transient: gammadot0, power, Gk (in the cohesion slot)
+    R.tr[k].gammadot0 = m->tables + (21 + 3 * k) * h3;
+    R.tr[k].power = m->tables + (22 + 3 * k) * h3;
+    R.tr[k].cohesion = m->tables + (23 + 3 * k) * h3;
+    R.tr[k].gdp = nullptr;
+    R.tr[k].rp = nullptr;
+    R.tr[k].dg = nullptr;
+    R.tr[k].active = 0;
   }
   R.fr.v0 = m->ftab;
   R.fr.tauc = m->ftab + h3;
@@ -379,6 +425,7 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
   if (hipStreamSynchronize(m->stream) != hipSuccess) { free_model(m); return RELAX_ERR_HIP; }
   m->md_dirty = true;
   if (sync_md(m) != RELAX_OK) { free_model(m); return RELAX_ERR_HIP; }
+  tune_lookup(m);
   *out = m;
   return RELAX_OK;
 }
@@ -391,17 +438,24 @@ int relax_model_reset(relax_model_t* m) {
   HIP_TRY(hipMemsetAsync(m->sig, 0, 6 * H * 4, m->stream));
   HIP_TRY(hipMemsetAsync(m->tau, 0, 6 * H * 4, m->stream));
   HIP_TRY(hipMemsetAsync(m->gam, 0, H * 4, m->stream));
-  HIP_TRY(hipMemsetAsync(m->tables, 0, 19L * h3 * 4, m->stream));
+  HIP_TRY(hipMemsetAsync(m->tables, 0, TABLE_ROWS * h3 * 4, m->stream));
+  if (m->E) {
+    HIP_TRY(hipMemsetAsync(m->E, 0, 6 * H * 4, m->stream));
+    HIP_TRY(hipMemsetAsync(m->Em, 0, 6 * H * 4, m->stream));
+  }
   HIP_TRY(hipMemsetAsync(m->ftab, 0, 4L * h3 * 8, m->stream));
   k_reset_scalars<<<1, 1, 0, m->stream>>>(m->sc, 0.0, 1, 0);
   HIP_TRY(hipStreamSynchronize(m->stream));
   for (int k = 0; k < 2; ++k) {
     m->R.mech[k].active = 0;
+    m->R.mech[k].dg = nullptr;
+    m->R.tr[k].active = 0;
+    m->R.tr[k].dg = nullptr;
     m->hg0[k].clear();
     m->hn[k].clear();
   }
   m->htaup.clear();
-  for (int k = 0; k < 3; ++k) m->mech[k] = 0;
+  for (int k = 0; k < NMECH; ++k) m->mech[k] = 0;
   m->host_planes.clear();
   m->plane_dev_host.clear();
   m->patch_off.assign(1, 0);
@@ -433,27 +487,122 @@ int relax_model_synchronize(relax_model_t* m) {
 static int update_gdp(relax_model_t* m, int k) {
   const int h3 = m->g.h3;
   if (m->hg0[k].empty()) return RELAX_OK;
-  std::vector<float> g(h3);
+  std::vector<float> g(h3), r(h3);
   for (int i = 0; i < h3; ++i) {
     const double taup = m->htaup.empty() ? 0.0 : m->htaup[i];
-    g[i] = (float)(m->hg0[k][i] * std::pow(taup / m->desc.mu, m->hn[k][i] - 1.0));
+    const double ratio = std::pow(taup / m->desc.mu, m->hn[k][i] - 1.0);
+    g[i] = (float)(m->hg0[k][i] * ratio);
+    r[i] = (float)ratio;
   }
   HIP_TRY(hipMemcpyAsync(m->tables + (17L + k) * h3, g.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(hipMemcpyAsync(m->tables + (19L + k) * h3, r.data(), h3 * 4, hipMemcpyHostToDevice, m->stream));
   HIP_TRY(hipStreamSynchronize(m->stream));
   return RELAX_OK;
 }
 
+// Transient structure k (0: linear, 1: power law): fluidity, exponent and Gk
+// per depth; the first one allocates the transient strain and its mid-step
+// copy, both zero (relax.f90:299-303 allocates epsilonik).
+static int set_transient(relax_model_t* m, int k, const relax_layer_t* layers, int n) {
+  const int h3 = m->g.h3;
+  const long H = m->g.H;
+  // transienteigenstress returns without effect when the structure does
+  // not match the grid (viscoelastic3d.f90:144); here that is an error
+  if (n != h3) return RELAX_ERR_ARG;
+  std::vector<float> t(3 * h3);
+  for (int i = 0; i < h3; ++i) {
+    if (layers[i].stressexponent < 0.999999) return RELAX_ERR_POWER;
+    t[i] = (float)layers[i].gammadot0;
+    t[h3 + i] = (float)layers[i].stressexponent;
+    t[2 * h3 + i] = (float)layers[i].Gk;
+  }
+  if (!m->E) {
+    if (hipMalloc((void**)&m->E, 6 * H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
+    if (hipMalloc((void**)&m->Em, 6 * H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
+    HIP_TRY(hipMemsetAsync(m->E, 0, 6 * H * 4, m->stream));
+    HIP_TRY(hipMemsetAsync(m->Em, 0, 6 * H * 4, m->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(m->tables + (21L + 3 * k) * h3, t.data(), 3L * h3 * 4, hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  m->R.tr[k].active = 1;
+  m->mech[3 + k] = 1;
+  m->md_dirty = true;
+  return RELAX_OK;
+}
+
+int relax_model_set_weakzones(relax_model_t* m, int which, const relax_weak_t* zones, int nz) {
+  if (!m || nz < 0 || (nz > 0 && !zones) || !(which == 0 || which == 1 || which == 3 || which == 4))
+    return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
+  const int slot = which < 2 ? which : which - 1;  // dg[0..3]: linear, power law, two transients
+  Mechanism& M = which < 2 ? m->R.mech[which] : m->R.tr[which - 3];
+  m->md_dirty = true;
+  if (nz == 0) {
+    M.dg = nullptr;
+    return RELAX_OK;
+  }
+  const Grid& g = m->g;
+  if (!m->dg[slot] && hipMalloc((void**)&m->dg[slot], g.H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
+  if (nz > m->zones_cap) {
+    if (m->zones_dev) HIP_TRY(hipFree(m->zones_dev));
+    m->zones_dev = nullptr;
+    m->zones_cap = 0;
+    if (hipMalloc((void**)&m->zones_dev, (size_t)nz * sizeof(WeakDev)) != hipSuccess) return RELAX_ERR_NOMEM;
+    m->zones_cap = nz;
+  }
+  // the taper of the reference's calls is 0 (relax.f90:487, :502, :525, :540)
+  std::vector<WeakDev> wz(nz);
+  for (int i = 0; i < nz; ++i) wz[i] = make_weak(zones[i], 0.0);
+  HIP_TRY(hipMemcpyAsync(m->zones_dev, wz.data(), (size_t)nz * sizeof(WeakDev), hipMemcpyHostToDevice, m->stream));
+  int rc = launch_builddgammadot0(g.n1, g.n2, g.h3, m->desc.dx1, m->desc.dx2, m->desc.dx3, 0.0, nz, m->zones_dev,
+                                  m->dg[slot], m->stream);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  M.dg = m->dg[slot];
+  return RELAX_OK;
+}
+
+int relax_model_set_transient_strain(relax_model_t* m, const relax_tensor_t* epsilonik) {
+  if (!m || !epsilonik || !m->E) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
+  const long H = m->g.H;
+  float* tmp = nullptr;
+  if (hipMalloc((void**)&tmp, 6 * H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
+  int rc = hipMemcpyAsync(tmp, epsilonik, 6 * H * 4, hipMemcpyHostToDevice, m->stream) == hipSuccess
+               ? launch_aos_soa(tmp, m->E, H, true, m->stream)
+               : RELAX_ERR_HIP;
+  if (hipStreamSynchronize(m->stream) != hipSuccess && !rc) rc = RELAX_ERR_HIP;
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int relax_model_get_transient_strain(relax_model_t* m, relax_tensor_t* epsilonik) {
+  if (!m || !epsilonik || !m->E) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
+  const long H = m->g.H;
+  float* tmp = nullptr;
+  if (hipMalloc((void**)&tmp, 6 * H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
+  int rc = launch_aos_soa(tmp, m->E, H, false, m->stream);
+  if (!rc && hipMemcpyAsync(epsilonik, tmp, 6 * H * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess)
+    rc = RELAX_ERR_HIP;
+  if (hipStreamSynchronize(m->stream) != hipSuccess && !rc) rc = RELAX_ERR_HIP;
+  (void)hipFree(tmp);
+  return rc;
+}
+
 int relax_model_set_structure(relax_model_t* m, int which, const relax_layer_t* layers, int n) {
-  if (!m || which < 0 || which > 2) return RELAX_ERR_ARG;
+  if (!m || which < 0 || which >= NMECH) return RELAX_ERR_ARG;
   if (int rc = wait_batch(m)) return rc;
   const int h3 = m->g.h3;
   m->md_dirty = true;
   if (!layers) {
     if (which < 2) m->R.mech[which].active = 0;
-    else m->R.fr.active = 0;
+    else if (which == 2) m->R.fr.active = 0;
+    else m->R.tr[which - 3].active = 0;
     m->mech[which] = 0;
     return RELAX_OK;
   }
+  if (which >= 3) return set_transient(m, which - 3, layers, n);
   // viscouseigenstress returns without effect when the structure does not
   // match the grid (viscoelastic3d.f90:240-241); here that is an error
   if (n != h3) return RELAX_ERR_ARG;
@@ -628,7 +777,7 @@ int relax_model_upload(relax_model_t* m, const float* u1, const float* u2, const
   k_reset_scalars<<<1, 1, 0, m->stream>>>(m->sc, t, oi, i);
   if (!sig) {  // sig = -tau + C:eps(u), relax.f90:369-371
     if ((rc = sync_md(m))) return rc;
-    rc = launch_stress(STRESS_SET, g, m->S, m->md_dev, &m->md_host, 1, false, m->stream);
+    rc = launch_stress(STRESS_SET, g, m->S, m->md_dev, &m->md_host, 1, RHEO_VISCOUS, m->stream);
     if (rc) return rc;
   }
   HIP_TRY(hipStreamSynchronize(m->stream));
@@ -711,7 +860,7 @@ struct Ctx {
   const ModelDev* md;
   const ModelDev* one;            // host copy of the record of a single model, else nullptr
   int nm;
-  bool fric;                      // some model has fault-creep planes
+  int rclass;                     // rheology class of the launches (RHEO_*)
   hipStream_t st;
   relax_model_t* const* models;   // host handles, for the per-plane kernels
   relax_model_t* prof;            // phase timers (single model), or nullptr
@@ -719,7 +868,7 @@ struct Ctx {
 
 static Ctx single(relax_model_t* m) {
   Ctx c;
-  c.lead = m; c.md = m->md_dev; c.one = &m->md_host; c.nm = 1; c.fric = m->R.fr.active && m->R.fr.np > 0;
+  c.lead = m; c.md = m->md_dev; c.one = &m->md_host; c.nm = 1; c.rclass = rheo_class(m);
   c.st = m->stream; c.models = &m->self; c.prof = m;
   return c;
 }
@@ -752,7 +901,8 @@ static int model_green(const Ctx& c, int epi) {
                              3L * n2, (float)(m->desc.dx1 * m->desc.dx2), nm, st)))
     return rc;
   cmark(c, RELAX_PROF_SMALL);
-  if ((rc = launch_fft_cols(ly, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st))) return rc;
+  if ((rc = launch_fft_cols(ly, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st, m->tune_cols)))
+    return rc;
   cmark(c, RELAX_PROF_FFT_Y);
   if ((rc = launch_fft_cols(n2, -1, MFIELD(float2, md, T), m1, (long)n2 * m1, m1, 3, nm, st))) return rc;
   cmark(c, RELAX_PROF_SMALL);
@@ -761,10 +911,11 @@ static int model_green(const Ctx& c, int epi) {
   if ((rc = launch_fft_rows(n2, -1, MFIELD(float2, md, TN), n2, 3, nm, st))) return rc;
   cmark(c, RELAX_PROF_SMALL);
   if ((rc = launch_green_columns(m->gc, h3, MFIELD(float2, md, W), MFIELD(float2, md, NQ), MFIELD(const float2, md, T),
-                                 MFIELD(const float2, md, TN), nm, st, l2)))
+                                 MFIELD(const float2, md, TN), nm, st, l2, m->tune_green)))
     return rc;
   cmark(c, RELAX_PROF_GREEN_Z);
-  if ((rc = launch_fft_cols(ly, +1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st))) return rc;
+  if ((rc = launch_fft_cols(ly, +1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st, m->tune_cols)))
+    return rc;
   cmark(c, RELAX_PROF_FFT_Y);
   if ((rc = launch_fft_rows(n2, +1, MFIELD(float2, md, NQ), n2, 3L * h3, nm, st))) return rc;
   cmark(c, RELAX_PROF_SMALL);
@@ -787,15 +938,15 @@ static int enqueue_steps(const Ctx& c, const double* dt_override) {
   int rc;
   cmark(c, -1);
   // predictor
-  if ((rc = launch_bodyforce(g, S, c.md, c.one, nm, c.fric, BODY_PRED, st))) return rc;
+  if ((rc = launch_bodyforce(g, S, c.md, c.one, nm, c.rclass, BODY_PRED, st, c.lead->tune_body))) return rc;
   cmark(c, RELAX_PROF_BODYFORCE);
   k_finalize_dt<<<(nm + 63) / 64, 64, 0, st>>>(c.md, nm, dt_override ? 1 : 0, dt_override ? *dt_override : 0.0);
   cmark(c, RELAX_PROF_SMALL);
   if ((rc = model_green(c, EPI_PRED))) return rc;
-  if ((rc = launch_stress(STRESS_PRED, g, S, c.md, c.one, nm, c.fric, st))) return rc;
+  if ((rc = launch_stress(STRESS_PRED, g, S, c.md, c.one, nm, c.rclass, st))) return rc;
   cmark(c, RELAX_PROF_STRESS_PRED);
   // corrector
-  if ((rc = launch_bodyforce(g, S, c.md, c.one, nm, c.fric, BODY_CORR, st))) return rc;
+  if ((rc = launch_bodyforce(g, S, c.md, c.one, nm, c.rclass, BODY_CORR, st, c.lead->tune_body))) return rc;
   cmark(c, RELAX_PROF_BODYFORCE);
   for (int i = 0; i < nm; ++i) {
     relax_model_t* m = c.models[i];
@@ -810,7 +961,7 @@ static int enqueue_steps(const Ctx& c, const double* dt_override) {
     }
   }
   if ((rc = model_green(c, EPI_CORR))) return rc;
-  if ((rc = launch_stress(STRESS_CORR, g, S, c.md, c.one, nm, c.fric, st))) return rc;
+  if ((rc = launch_stress(STRESS_CORR, g, S, c.md, c.one, nm, c.rclass, st))) return rc;
   cmark(c, RELAX_PROF_STRESS_CORR);
   for (int i = 0; i < nm; ++i) {
     relax_model_t* m = c.models[i];
@@ -856,7 +1007,7 @@ int relax_model_profile(relax_model_t* m, int nsteps, double* ms, int* launches)
 
 static void fill_info(const StepScalars& s, relax_step_info_t* info) {
   info->i = s.it; info->Dt = s.Dt; info->tm = s.tm;
-  for (int k = 0; k < 3; ++k) info->maxwell[k] = s.maxwell[k];
+  for (int k = 0; k < NMECH; ++k) info->maxwell[k] = s.maxwell[k];
   info->power = s.power; info->cei = s.cei; info->t = s.t; info->oi = s.oi; info->status = s.status;
 }
 
@@ -962,9 +1113,9 @@ int relax_model_coseismic(relax_model_t* m) {
   c.prof = nullptr;
   // body force and surface traction of tau, then u = G[f, t] and
   // sig = -tau + C:eps(u) (relax.f90:318-371 with the FIR divergence)
-  if ((rc = launch_bodyforce(m->g, m->S, c.md, c.one, 1, false, BODY_RAW, m->stream))) return rc;
+  if ((rc = launch_bodyforce(m->g, m->S, c.md, c.one, 1, RHEO_VISCOUS, BODY_RAW, m->stream))) return rc;
   if ((rc = model_green(c, EPI_PLAIN))) return rc;
-  if ((rc = launch_stress(STRESS_SET, m->g, m->S, c.md, c.one, 1, false, m->stream))) return rc;
+  if ((rc = launch_stress(STRESS_SET, m->g, m->S, c.md, c.one, 1, RHEO_VISCOUS, m->stream))) return rc;
   HIP_TRY(hipStreamSynchronize(m->stream));
   return RELAX_OK;
 }
@@ -1118,8 +1269,8 @@ int relax_batch_step(relax_batch_t* b, relax_step_info_t* infos) {
   c.lead = b->models[0]; c.md = b->md_dev; c.one = nullptr; c.nm = nm; c.st = b->stream;
   c.models = b->models.data();
   c.prof = nullptr;
-  c.fric = false;
-  for (relax_model_t* m : b->models) c.fric = c.fric || (m->R.fr.active && m->R.fr.np > 0);
+  c.rclass = RHEO_VISCOUS;
+  for (relax_model_t* m : b->models) c.rclass = std::max(c.rclass, rheo_class(m));
   if (!b->exec && b->stable < GRAPH_AFTER) {
     ++b->stable;
     if ((rc = enqueue_steps(c, nullptr))) return rc;
@@ -1173,5 +1324,76 @@ int relax_batch_scalars(relax_batch_t* b, relax_step_info_t* infos) {
 
 static int wait_batch(relax_model_t* m) {
   if (m->batch) HIP_TRY(hipStreamSynchronize(m->batch->stream));
+  return RELAX_OK;
+}
+
+// ------------------------------------------------------------ autotuner ----
+// Times every variant of the tunable kernels on the model's own work
+// buffers (W, NQ, T, TN are scratch between steps; the divergence pass in its
+// corrector form only reads the stress and writes W and T) and keeps the
+// fastest per device and grid.
+int relax_model_autotune(relax_model_t* m, int* choice, float* ms) {
+  if (!m || m->batch) return RELAX_ERR_ARG;
+  static_assert(COLS_VARIANTS <= RELAX_TUNE_VARIANTS && GREEN_VARIANTS <= RELAX_TUNE_VARIANTS &&
+                    BODY_VARIANTS <= RELAX_TUNE_VARIANTS,
+                "the timing table holds every variant");
+  int rc;
+  if ((rc = sync_md(m))) return rc;
+  const Grid& g = m->g;
+  hipStream_t st = m->stream;
+  const int n2 = g.n2, h3 = g.h3, m1 = g.m1;
+  const int ys = n2 >= 128 ? n2 : 0, l2 = ys ? n2 / 8 : 0;
+  const int ly = ys ? l2 : n2, nyp = ys ? 3 * h3 * 8 : 3 * h3;
+  const ModelDev* md = m->md_dev;
+  const int rclass = rheo_class(m);
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return RELAX_ERR_HIP; }
+  auto run = [&](int kind, int v) -> int {
+    if (kind == 0)
+      return launch_fft_cols(ly, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, 1, st, v);
+    if (kind == 1)
+      return launch_green_columns(m->gc, h3, MFIELD(float2, md, W), MFIELD(float2, md, NQ),
+                                  MFIELD(const float2, md, T), MFIELD(const float2, md, TN), 1, st, l2, v);
+    return launch_bodyforce(g, m->S, md, &m->md_host, 1, rclass, BODY_CORR, st, v);
+  };
+  const int nvar[3] = {COLS_VARIANTS, GREEN_VARIANTS, BODY_VARIANTS};
+  constexpr int NREP = 3;
+  int best[3] = {0, 0, 0};
+  float times[3][RELAX_TUNE_VARIANTS];
+  for (auto& row : times)
+    for (float& t : row) t = -1.f;
+  rc = RELAX_OK;
+  for (int kind = 0; kind < 3 && !rc; ++kind) {
+    float tbest = -1.f;
+    for (int v = 0; v < nvar[kind] && !rc; ++v) {
+      rc = run(kind, v);  // warm-up (and the shape check)
+      if (rc == RELAX_ERR_SIZE) { rc = RELAX_OK; continue; }  // does not fit this grid
+      if (rc) break;
+      if (hipEventRecord(e0, st) != hipSuccess) { rc = RELAX_ERR_HIP; break; }
+      for (int r = 0; r < NREP && !rc; ++r) rc = run(kind, v);
+      if (rc) break;
+      if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = RELAX_ERR_HIP; break; }
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, e0, e1);
+      times[kind][v] = t / NREP;
+      if (tbest < 0.f || t < tbest) { tbest = t; best[kind] = v; }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  m->tune_cols = best[0];
+  m->tune_green = best[1];
+  m->tune_body = best[2];
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[TuneKey{m->device, g.n1, g.n2, g.n3}] = TuneChoice{best[0], best[1], best[2]};
+  }
+  if (ms)
+    for (int k = 0; k < 3; ++k)
+      for (int v = 0; v < RELAX_TUNE_VARIANTS; ++v) ms[k * RELAX_TUNE_VARIANTS + v] = times[k][v];
+  if (choice) { choice[0] = best[0]; choice[1] = best[1]; choice[2] = best[2]; }
   return RELAX_OK;
 }

@@ -16,7 +16,7 @@
 struct TimeLine {
   double interval, odt, tscale, next_event, etime;
   int skip, has_next;
-  int mech[3];
+  int mech[NMECH];
 };
 
 struct ModelDev {
@@ -24,6 +24,8 @@ struct ModelDev {
   float* sig;        // 6 planes of H
   float* tau;        // 6 planes of H
   float* gam;        // H
+  float* E;          // 6 planes of H: transient strain epsilonik (or NULL)
+  float* Em;         // 6 planes of H: its mid-step value (predictor output)
   float* W;          // 3H work field / spectrum
   float2* NQ;        // Nyquist bins of the work spectrum
   float* T;          // surface traction planes

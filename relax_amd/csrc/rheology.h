@@ -2,12 +2,17 @@
 // kernels so the moment-density field never round-trips through HBM.
 //   viscous power law: viscouseigenstress, viscoelastic3d.f90:217-314
 //   fault creep:       frictioneigenstress, friction3d.f90:82-241 (CPU path)
+//   transient creep:   transienteigenstress, viscoelastic3d.f90:123-202
+//   weak zones:        the dgammadot0 fields of builddgammadot0, :320-435
 #pragma once
 #include "common.h"
 
 struct RheoParams {
   float mu;
   Mechanism mech[2];  // 0: linear (in%linearstruc), 1: power law (in%nonlinearstruc)
+  // transient (Burgers) creep: 0: in%ltransientstruc, 1: in%nltransientstruc;
+  // their cohesion table holds Gk
+  Mechanism tr[2];
   Prestress pre;
   FrictionTables fr;
   int n1, n2;
@@ -65,12 +70,12 @@ __device__ __forceinline__ float pow_exponent(float x, float e) {
 }
 
 // Viscous contribution of one mechanism; returns 1/gammadot (or +inf when
-// the point is skipped) and adds gammadot*tauc/mu to *grate.  The prestress
-// rate gammadot0 (taup/mu)^(n-1) depends on depth only and comes from a
-// table (M.gdp).
+// the point is skipped) and adds gammadot*tauc/mu to *grate.  g0 is the
+// fluidity gammadot0 of the point and gdp its prestress rate
+// gammadot0 (taup/mu)^(n-1); without weak zones both depend on depth only
+// and come from tables.
 __device__ __forceinline__ float viscous_point(const Mechanism& M, const Prestress& P, float mu, int i3,
-                                               const T6& sd, T6& m, float* grate) {
-  const float g0 = M.gammadot0[i3];
+                                               float g0, float gdp, const T6& sd, T6& m, float* grate) {
   if (1.0e-20 > (double)g0) return __builtin_inff();
   const float* sp = P.sp + 6 * i3;
   const T6 ss{sd.s11 + sp[0], sd.s12 + sp[1], sd.s13 + sp[2], sd.s22 + sp[3], sd.s23 + sp[4], sd.s33 + sp[5]};
@@ -79,7 +84,6 @@ __device__ __forceinline__ float viscous_point(const Mechanism& M, const Prestre
   if ((double)tauc <= 1.0e-20) return __builtin_inff();
   const float ex = M.power[i3] - 1.f;
   const float gd = g0 * pow_exponent(tauc / mu, ex);
-  const float gdp = M.gdp[i3];
   const float a = 2.f * gd, b = 2.f * gdp;
   m.s11 += a * sd.s11 - b * sp[0];
   m.s12 += a * sd.s12 - b * sp[1];
@@ -152,15 +156,96 @@ __device__ __forceinline__ T6 rheology(const RheoParams& R, const T6& s, int i1,
   if (R.mech[0].active || R.mech[1].active) {
     const T6 sd = deviatoric(s);
     if (R.mech[0].active) {
-      const float t = viscous_point(R.mech[0], R.pre, R.mu, i3, sd, m, grate_lin);
+      const Mechanism& M = R.mech[0];
+      const float t = viscous_point(M, R.pre, R.mu, i3, M.gammadot0[i3], M.gdp[i3], sd, m, grate_lin);
       if (TM) tmin[0] = fminf(tmin[0], t);
     }
     if (R.mech[1].active) {
-      const float t = viscous_point(R.mech[1], R.pre, R.mu, i3, sd, m, grate_nl);
+      const Mechanism& M = R.mech[1];
+      const float t = viscous_point(M, R.pre, R.mu, i3, M.gammadot0[i3], M.gdp[i3], sd, m, grate_nl);
       if (TM) tmin[1] = fminf(tmin[1], t);
     }
   }
   if (FRIC && R.fr.active) {
+    for (int k = 0; k < R.fr.np; ++k) {
+      const float t = friction_point(R, R.fr.planes[k], i1, i2, i3, s, m, nullptr, status);
+      if (TM) tmin[2] = fminf(tmin[2], t);
+    }
+  }
+  return m;
+}
+
+// Transient contribution of one mechanism at a point of fluidity g0
+// (transienteigenstress, viscoelastic3d.f90:170-196): Q = s' - 2 Gk E,
+// gammadot = g0 (|Q|/mu)^(n-1), Edot = gammadot/mu Q, moment += 2 gammadot Q.
+// Returns 1/gammadot; a point with g0 < 1e-20 gets Edot = 0 and +inf.
+__device__ __forceinline__ float transient_point(const Mechanism& M, float mu, int i3, float g0, const T6& sd,
+                                                 const T6& e, T6& m, T6& ed) {
+  if (1.0e-20 > (double)g0) {
+    ed = T6{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    return __builtin_inff();
+  }
+  const float k2 = 2.f * M.cohesion[i3];  // REAL(2 Gk)
+  const T6 q{sd.s11 - k2 * e.s11, sd.s12 - k2 * e.s12, sd.s13 - k2 * e.s13,
+             sd.s22 - k2 * e.s22, sd.s23 - k2 * e.s23, sd.s33 - k2 * e.s33};
+  const float nq = tnorm(q);
+  const float gd = g0 * pow_exponent(nq / mu, M.power[i3] - 1.f);
+  const float a = gd / mu, b = 2.f * gd;
+  ed = T6{a * q.s11, a * q.s12, a * q.s13, a * q.s22, a * q.s23, a * q.s33};
+  m.s11 += b * q.s11; m.s12 += b * q.s12; m.s13 += b * q.s13;
+  m.s22 += b * q.s22; m.s23 += b * q.s23; m.s33 += b * q.s33;
+  return 1.f / gd;
+}
+
+// Point fluidity of a mechanism: the depth table plus the weak-zone field.
+__device__ __forceinline__ float point_fluidity(const Mechanism& M, int i3, long o) {
+  return M.dg ? M.gammadot0[i3] + M.dg[o] : M.gammadot0[i3];
+}
+
+// Moment-density rate of all mechanisms at point o = (i1,i2,i3), weak zones
+// and transient creep included (relax.f90:568-650 predictor, :704-793
+// corrector).  e is the transient strain the first transient mechanism
+// reads: epsilonik in the predictor, the mid-step strain in the corrector.
+// *ed receives the final epsilonikdot.  CORR reproduces the corrector's
+// calls, which pass epsilonikdot as both the strain and the rate: the
+// second mechanism reads the first one's rate.  In the predictor both read
+// epsilonik and the second one's rate replaces the first one's.
+template <bool TM, bool CORR>
+__device__ __forceinline__ T6 rheology_ext(const RheoParams& R, const T6& s, int i1, int i2, int i3, long o,
+                                           const T6& e, T6* ed, float* tmin, float* grate_lin, float* grate_nl,
+                                           int* status) {
+  T6 m{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const bool trans = R.tr[0].active || R.tr[1].active;
+  if (R.mech[0].active || R.mech[1].active || trans) {
+    const T6 sd = deviatoric(s);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const Mechanism& M = R.mech[k];
+      if (!M.active) continue;
+      const float g0 = point_fluidity(M, i3, o);
+      const float gdp = M.dg ? g0 * M.rp[i3] : M.gdp[i3];
+      const float t = viscous_point(M, R.pre, R.mu, i3, g0, gdp, sd, m, k ? grate_nl : grate_lin);
+      if (TM) tmin[k] = fminf(tmin[k], t);
+    }
+    if (R.fr.active) {
+      for (int k = 0; k < R.fr.np; ++k) {
+        const float t = friction_point(R, R.fr.planes[k], i1, i2, i3, s, m, nullptr, status);
+        if (TM) tmin[2] = fminf(tmin[2], t);
+      }
+    }
+    if (trans) {
+      T6 ein = e, r{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const Mechanism& M = R.tr[k];
+        if (!M.active) continue;
+        const float t = transient_point(M, R.mu, i3, point_fluidity(M, i3, o), sd, ein, m, r);
+        if (TM) tmin[3 + k] = fminf(tmin[3 + k], t);
+        if (CORR) ein = r;
+      }
+      *ed = r;
+    }
+  } else if (R.fr.active) {
     for (int k = 0; k < R.fr.np; ++k) {
       const float t = friction_point(R, R.fr.planes[k], i1, i2, i3, s, m, nullptr, status);
       if (TM) tmin[2] = fminf(tmin[2], t);

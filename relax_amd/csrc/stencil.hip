@@ -20,23 +20,32 @@
 // after inlining, the compiler keeps them as alias scopes, so the per-depth
 // rheology tables (read-only) are known not to alias the stores and are
 // read through the scalar cache.
-template <int TXB, int TYB, bool TM, bool RAW, bool FRIC>
+//
+// RC is the rheology class: RHEO_VISCOUS, RHEO_CREEP (with fault creep) or
+// RHEO_EXT (also weak zones and transient creep, which reads the transient
+// strain E: epsilonik in the predictor, the mid-step strain in the
+// corrector).  TM marks the predictor pass.
+template <int TXB, int TYB, bool TM, bool RAW, int RC>
 __device__ __forceinline__ void bodyforce_tile(const Grid& g, const RheoParams& R, const StencilConst& S,
-                                               const float* __restrict__ sig, float* __restrict__ W,
-                                               float* __restrict__ T, StepScalars* sc, int bx, int by, int bz,
-                                               int zc, int gz) {
+                                               const float* __restrict__ sig, const float* __restrict__ E,
+                                               float* __restrict__ W, float* __restrict__ T, StepScalars* sc,
+                                               int bx, int by, int bz, int zc, int gz) {
   constexpr int NT = TXB * TYB;
   constexpr int HX = TXB + 2, HY = TYB + 2, HP = HX * HY;
+  constexpr int NTM = RC == RHEO_EXT ? NMECH : 3;  // mechanisms whose minima are reduced
   __shared__ float ring[3][6][HP];
-  __shared__ float rmin[3][NT / 64];
+  __shared__ float rmin[NTM][NT / 64];
   const int tid = threadIdx.x;
   const int tx = tid % TXB, ty = tid / TXB;
   const int n1 = g.n1, n2 = g.n2, h3 = g.h3;
   const long H = g.H;
   const int x0 = bx * TXB, y0 = by * TYB;
   const int z0 = (bz % gz) * zc, z1 = min(z0 + zc, h3);
-  float tmin[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+  float tmin[NTM];
+#pragma unroll
+  for (int k = 0; k < NTM; ++k) tmin[k] = __builtin_inff();
   int status = 0;
+  const bool trans = RC == RHEO_EXT && (R.tr[0].active || R.tr[1].active);
   const int e0 = (ty + 1) * HX + (tx + 1);
   const int i1 = x0 + tx, i2 = y0 + ty;
 
@@ -81,26 +90,43 @@ __device__ __forceinline__ void bodyforce_tile(const Grid& g, const RheoParams& 
     pj1[k] = (x0 + hx - 1 + n1) % n1;
     pj2[k] = (y0 + hy - 1 + n2) % n2;
   }
-  T6 pre[KP];
+  T6 pre[KP], pre_e[RC == RHEO_EXT ? KP : 1];
   auto load_plane = [&](int z) {
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
-      if (tid + k * NT < HP) pre[k] = load_t6(sig, H, pj1[k] + (long)n1 * (pj2[k] + (long)n2 * z));
+      if (tid + k * NT < HP) {
+        const long o = pj1[k] + (long)n1 * (pj2[k] + (long)n2 * z);
+        pre[k] = load_t6(sig, H, o);
+        if (RC == RHEO_EXT) pre_e[k] = trans ? load_t6(E, H, o) : T6{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      }
     }
   };
   const int zs = max(z0 - 1, 0), ze = min(z1, h3 - 1);
   load_plane(zs);
   for (int z = zs; z <= ze; ++z) {
     float* slot = &ring[z % 3][0][0];
-    T6 cur[KP];
+    T6 cur[KP], cur_e[RC == RHEO_EXT ? KP : 1];
 #pragma unroll
-    for (int k = 0; k < KP; ++k) cur[k] = pre[k];
+    for (int k = 0; k < KP; ++k) {
+      cur[k] = pre[k];
+      if (RC == RHEO_EXT) cur_e[k] = pre_e[k];
+    }
     if (z < ze) load_plane(z + 1);
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
       const int e = tid + k * NT;
       if (e < HP) {
-        const T6 m = RAW ? cur[k] : rheology<TM, FRIC>(R, cur[k], pj1[k], pj2[k], z, tmin, nullptr, nullptr, &status);
+        T6 m;
+        if (RAW) {
+          m = cur[k];
+        } else if (RC == RHEO_EXT) {
+          T6 ed;
+          const long o = pj1[k] + (long)n1 * (pj2[k] + (long)n2 * z);
+          m = rheology_ext<TM, !TM>(R, cur[k], pj1[k], pj2[k], z, o, cur_e[RC == RHEO_EXT ? k : 0], &ed, tmin,
+                                    nullptr, nullptr, &status);
+        } else {
+          m = rheology<TM, RC == RHEO_CREEP>(R, cur[k], pj1[k], pj2[k], z, tmin, nullptr, nullptr, &status);
+        }
         slot[e] = m.s11; slot[HP + e] = m.s12; slot[2 * HP + e] = m.s13;
         slot[3 * HP + e] = m.s22; slot[4 * HP + e] = m.s23; slot[5 * HP + e] = m.s33;
       }
@@ -117,13 +143,13 @@ __device__ __forceinline__ void bodyforce_tile(const Grid& g, const RheoParams& 
     // wavefront minima, then one atomic per block (positive floats order as
     // unsigned integers, so the result does not depend on the order)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < NTM; ++k) {
       float v = tmin[k];
       for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
       if ((tid & 63) == 0) rmin[k][tid >> 6] = v;
     }
     __syncthreads();
-    if (tid < 3) {
+    if (tid < NTM) {
       float v = rmin[tid][0];
       for (int w = 1; w < NT / 64; ++w) v = fminf(v, rmin[tid][w]);
       if (v < __builtin_inff()) atomicMin(&sc->tmin_bits[tid], __float_as_uint(v));
@@ -133,7 +159,7 @@ __device__ __forceinline__ void bodyforce_tile(const Grid& g, const RheoParams& 
 
 // BATCH: the models' records are in the device array md (one model per
 // nx*ny*gz tiles); otherwise the one model's record is the argument Dv.
-template <int TXB, int TYB, bool TM, bool RAW, bool FRIC, bool BATCH>
+template <int TXB, int TYB, bool TM, bool RAW, int RC, bool BATCH>
 __global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, ModelDev Dv, const ModelDev* __restrict__ md,
                                                         StencilConst S, int zc, int gz) {
   // 1-D grid of tiles (x fastest, then y, depth chunk, model), taken in the
@@ -143,13 +169,18 @@ __global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, ModelDev Dv, con
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = tile % nx, by = (tile / nx) % ny, bz = tile / (nx * ny);
   const ModelDev& D = BATCH ? md[bz / gz] : Dv;
-  bodyforce_tile<TXB, TYB, TM, RAW, FRIC>(g, D.R, S, RAW ? D.tau : D.sig, D.W, D.T, D.sc, bx, by, bz, zc, gz);
+  bodyforce_tile<TXB, TYB, TM, RAW, RC>(g, D.R, S, RAW ? D.tau : D.sig, TM ? D.E : D.Em, D.W, D.T, D.sc, bx, by,
+                                         bz, zc, gz);
 }
 
-template <int MODE, bool FRIC>
+// With transient creep (RC == RHEO_EXT) the predictor also writes the
+// mid-step strain Em = E + Dt/2 Edot (relax.f90:690-692) and the corrector
+// advances E += Dt Edot (:792-793), Edot from the mid-step strain.
+template <int MODE, int RC>
 __device__ __forceinline__ void stress_block(const Grid& g, const RheoParams& R, const StencilConst& S,
                                              const float* __restrict__ X, float* __restrict__ sig,
-                                             float* __restrict__ tau, float* __restrict__ gam, StepScalars* sc,
+                                             float* __restrict__ tau, float* __restrict__ gam,
+                                             float* __restrict__ E, float* __restrict__ Em, StepScalars* sc,
                                              double2* __restrict__ partials) {
   __shared__ double2 red[4];
   const int tid = threadIdx.x;
@@ -197,15 +228,41 @@ __device__ __forceinline__ void stress_block(const Grid& g, const RheoParams& R,
       out = T6{-t.s11 + e.s11, -t.s12 + e.s12, -t.s13 + e.s13, -t.s22 + e.s22, -t.s23 + e.s23, -t.s33 + e.s33};
     } else if (MODE == STRESS_PRED) {
       const T6 s = load_t6(sig, H, o);
-      const T6 m = rheology<false, FRIC>(R, s, i1, i2, i3, nullptr, nullptr, nullptr, &status);
+      T6 m;
+      if (RC == RHEO_EXT) {
+        const bool trans = R.tr[0].active || R.tr[1].active;
+        const T6 e = trans ? load_t6(E, H, o) : T6{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        T6 ed;
+        m = rheology_ext<false, false>(R, s, i1, i2, i3, o, e, &ed, nullptr, nullptr, nullptr, &status);
+        if (trans) {
+          const float h = (float)(sc->Dt / 2);
+          store_t6(Em, H, o, T6{h * ed.s11 + e.s11, h * ed.s12 + e.s12, h * ed.s13 + e.s13,
+                                h * ed.s22 + e.s22, h * ed.s23 + e.s23, h * ed.s33 + e.s33});
+        }
+      } else {
+        m = rheology<false, RC == RHEO_CREEP>(R, s, i1, i2, i3, nullptr, nullptr, nullptr, &status);
+      }
       const float c = -(float)(sc->Dt / 2);
       out = T6{(c * m.s11 - t.s11) + e.s11, (c * m.s12 - t.s12) + e.s12, (c * m.s13 - t.s13) + e.s13,
                (c * m.s22 - t.s22) + e.s22, (c * m.s23 - t.s23) + e.s23, (c * m.s33 - t.s33) + e.s33};
     } else {
       const T6 s = load_t6(sig, H, o);
       float gl = 0.f, gn = 0.f;
-      const T6 m = rheology<false, FRIC>(R, s, i1, i2, i3, nullptr, &gl, &gn, &status);
       const float dt = (float)sc->Dt;
+      T6 m;
+      if (RC == RHEO_EXT) {
+        const bool trans = R.tr[0].active || R.tr[1].active;
+        const T6 em = trans ? load_t6(Em, H, o) : T6{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        T6 ed;
+        m = rheology_ext<false, true>(R, s, i1, i2, i3, o, em, &ed, nullptr, &gl, &gn, &status);
+        if (trans) {
+          const T6 e = load_t6(E, H, o);
+          store_t6(E, H, o, T6{e.s11 + dt * ed.s11, e.s12 + dt * ed.s12, e.s13 + dt * ed.s13,
+                               e.s22 + dt * ed.s22, e.s23 + dt * ed.s23, e.s33 + dt * ed.s33});
+        }
+      } else {
+        m = rheology<false, RC == RHEO_CREEP>(R, s, i1, i2, i3, nullptr, &gl, &gn, &status);
+      }
       const T6 tn{t.s11 + dt * m.s11, t.s12 + dt * m.s12, t.s13 + dt * m.s13,
                   t.s22 + dt * m.s22, t.s23 + dt * m.s23, t.s33 + dt * m.s33};
       store_t6(tau, H, o, tn);
@@ -238,30 +295,39 @@ __device__ __forceinline__ void stress_block(const Grid& g, const RheoParams& R,
   if (status) atomicMax(&sc->status, status);
 }
 
-template <int MODE, bool FRIC, bool BATCH>
+template <int MODE, int RC, bool BATCH>
 __global__ __launch_bounds__(256) void k_stress(Grid g, ModelDev Dv, const ModelDev* __restrict__ md,
                                                StencilConst S) {
   // model blockIdx.y of the launch (BATCH) or the argument Dv
   const ModelDev& D = BATCH ? md[blockIdx.y] : Dv;
-  stress_block<MODE, FRIC>(g, D.R, S, MODE == STRESS_PRED ? D.W : D.u, D.sig, D.tau, D.gam, D.sc, D.partials);
+  stress_block<MODE, RC>(g, D.R, S, MODE == STRESS_PRED ? D.W : D.u, D.sig, D.tau, D.gam, D.E, D.Em, D.sc,
+                         D.partials);
 }
 
 int launch_bodyforce(const Grid& g, const StencilConst& S, const ModelDev* md, const ModelDev* one, int nm,
-                     bool fric, int mode, hipStream_t st) {
+                     int rclass, int mode, hipStream_t st, int variant) {
   const int zc = 16;
   const int gz = (g.h3 + zc - 1) / zc;
   const bool batch = !(one && nm == 1);
   const ModelDev Dv = batch ? ModelDev{} : *one;
+  if (rclass < RHEO_VISCOUS || rclass > RHEO_EXT) return RELAX_ERR_ARG;
 #define LB(TX, TY, F, B)                                                                                 \
   {                                                                                                      \
     const dim3 grid((unsigned)((g.n1 / TX) * (g.n2 / TY) * gz * nm));                                    \
     if (mode == BODY_PRED) k_bodyforce<TX, TY, true, false, F, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz); \
     else if (mode == BODY_CORR) k_bodyforce<TX, TY, false, false, F, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz); \
-    else k_bodyforce<TX, TY, false, true, false, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz);   \
+    else k_bodyforce<TX, TY, false, true, RHEO_VISCOUS, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz); \
   }
 #define LF(TX, TY, F) { if (batch) LB(TX, TY, F, true) else LB(TX, TY, F, false) }
-#define L(TX, TY) { if (fric) LF(TX, TY, true) else LF(TX, TY, false) }
-  if (g.n1 % 32 == 0 && g.n2 % 16 == 0) L(32, 16)
+#define L(TX, TY)                                   \
+  {                                                 \
+    if (rclass == RHEO_EXT) LF(TX, TY, RHEO_EXT)    \
+    else if (rclass == RHEO_CREEP) LF(TX, TY, RHEO_CREEP) \
+    else LF(TX, TY, RHEO_VISCOUS)                   \
+  }
+  // variant 0: 32x16 tiles, 1: 32x8 tiles
+  if (variant < 0 || variant >= BODY_VARIANTS) return RELAX_ERR_ARG;
+  if (variant == 0 && g.n1 % 32 == 0 && g.n2 % 16 == 0) L(32, 16)
   else if (g.n1 % 32 == 0 && g.n2 % 8 == 0) L(32, 8)
   else return RELAX_ERR_SIZE;
 #undef LB
@@ -273,18 +339,26 @@ int launch_bodyforce(const Grid& g, const StencilConst& S, const ModelDev* md, c
 long stress_blocks(const Grid& g) { return (g.H + 255) / 256; }
 
 int launch_stress(int mode, const Grid& g, const StencilConst& S, const ModelDev* md, const ModelDev* one, int nm,
-                  bool fric, hipStream_t st) {
+                  int rclass, hipStream_t st) {
   const dim3 grid((unsigned)stress_blocks(g), (unsigned)nm);
   const bool batch = !(one && nm == 1);
   const ModelDev Dv = batch ? ModelDev{} : *one;
+  if (rclass < RHEO_VISCOUS || rclass > RHEO_EXT) return RELAX_ERR_ARG;
 #define K(MODE, F)                                                          \
   {                                                                         \
     if (batch) k_stress<MODE, F, true><<<grid, 256, 0, st>>>(g, Dv, md, S);  \
     else k_stress<MODE, F, false><<<grid, 256, 0, st>>>(g, Dv, md, S);       \
   }
-  if (mode == STRESS_SET) K(STRESS_SET, false)
-  else if (mode == STRESS_PRED) { if (fric) K(STRESS_PRED, true) else K(STRESS_PRED, false) }
-  else { if (fric) K(STRESS_CORR, true) else K(STRESS_CORR, false) }
+#define KM(MODE)                                          \
+  {                                                       \
+    if (rclass == RHEO_EXT) K(MODE, RHEO_EXT)             \
+    else if (rclass == RHEO_CREEP) K(MODE, RHEO_CREEP)    \
+    else K(MODE, RHEO_VISCOUS)                            \
+  }
+  if (mode == STRESS_SET) K(STRESS_SET, RHEO_VISCOUS)
+  else if (mode == STRESS_PRED) KM(STRESS_PRED)
+  else KM(STRESS_CORR)
 #undef K
+#undef KM
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }

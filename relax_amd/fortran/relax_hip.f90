@@ -13,6 +13,8 @@
 !   fieldadd, tensorfieldadd           elastic3d.f90:3774-3959
 !   tensoramplitude                    elastic3d.f90:446-466
 !   viscouseigenstress                 viscoelastic3d.f90:217-314
+!   transienteigenstress               viscoelastic3d.f90:123-202
+!   builddgammadot0                    viscoelastic3d.f90:320-435
 !   frictioneigenstress                friction3d.f90:82-241
 !   monitorfriction, frictionadd       friction3d.f90:263-453
 ! A STOP of the reference becomes a non-zero relax_hip_status and a message
@@ -53,6 +55,11 @@ MODULE relax_hip_types
      INTEGER :: px2,px3
      TYPE(SLIPPATCH_STRUCT), DIMENSION(:,:), ALLOCATABLE :: patch
   END TYPE PLANE_STRUCT
+  TYPE WEAK_STRUCT
+     SEQUENCE
+     REAL*8 :: dgammadot0,x,y,z,width,length,thickness,strike,dip
+     TYPE(TENSOR) :: e
+  END TYPE WEAK_STRUCT
 END MODULE relax_hip_types
 
 MODULE relax_hip
@@ -71,6 +78,31 @@ MODULE relax_hip
   !> status of the last call (0 = success, RELAX_ERR_* of relax_hip.h)
   INTEGER, PUBLIC :: relax_hip_status = 0
   INTEGER, PARAMETER, PUBLIC :: FFT_FORWARD=-1, FFT_INVERSE=1
+
+  !> device-resident model: the body of the time loop of relax.f90:555-890
+  !! (relax_model_* of relax_hip.h); the layouts are those of the C structs
+  TYPE, BIND(C), PUBLIC :: relax_model_desc
+     INTEGER(c_int) :: sx1,sx2,sx3
+     REAL(c_double) :: dx1,dx2,dx3,lambda,mu,gamma,beta,interval,odt,tscale
+     INTEGER(c_int) :: skip,ne
+     TYPE(c_ptr) :: event_times=C_NULL_PTR
+  END TYPE relax_model_desc
+  TYPE, BIND(C), PUBLIC :: relax_step_info
+     INTEGER(c_int) :: i
+     REAL(c_double) :: Dt,tm,maxwell(5),power,cei,t
+     INTEGER(c_int) :: oi,status
+  END TYPE relax_step_info
+  !> a fault plane or slip patch in the internal convention (angles in
+  !! radians): PLANE_STRUCT without its patch array
+  TYPE, BIND(C), PUBLIC :: relax_plane_c
+     REAL(c_double) :: x,y,z,width,length,strike,dip,rake
+     INTEGER(c_int) :: px2,px3
+  END TYPE relax_plane_c
+  TYPE, PUBLIC :: relax_model
+     TYPE(c_ptr) :: h=C_NULL_PTR
+  END TYPE relax_model
+  INTEGER, PARAMETER, PUBLIC :: RELAX_LINEAR=0, RELAX_POWERLAW=1, RELAX_FAULTCREEP=2, &
+       RELAX_LTRANSIENT=3, RELAX_NLTRANSIENT=4
 
   INTERFACE
      INTEGER(c_int) FUNCTION c_fft3(d,sx1,sx2,sx3,dx1,dx2,dx3,dir,al) BIND(C,name="relax_fft3")
@@ -150,6 +182,29 @@ MODULE relax_hip
        TYPE(c_ptr), VALUE :: st,pre,sig,mom,mw,dg,gam
        INTEGER(c_int), VALUE :: nst,npre,sx1,sx2,sx3,al
      END FUNCTION
+     INTEGER(c_int) FUNCTION c_transient(mu,st,nst,sig,eik,sx1,sx2,sx3,dx1,dx2,dx3,mom,eikdot,mw,dg) &
+          BIND(C,name="relax_transienteigenstress")
+       IMPORT :: c_int, c_double, c_ptr
+       REAL(c_double), VALUE :: mu,dx1,dx2,dx3
+       TYPE(c_ptr), VALUE :: st,sig,eik,mom,eikdot,mw,dg
+       INTEGER(c_int), VALUE :: nst,sx1,sx2,sx3
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_builddg(sx1,sx2,sx3,dx1,dx2,dx3,beta,nz,zones,dg) &
+          BIND(C,name="relax_builddgammadot0")
+       IMPORT :: c_int, c_double, c_ptr
+       INTEGER(c_int), VALUE :: sx1,sx2,sx3,nz
+       REAL(c_double), VALUE :: dx1,dx2,dx3,beta
+       TYPE(c_ptr), VALUE :: zones,dg
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_set_weakzones(h,which,zones,nz) BIND(C,name="relax_model_set_weakzones")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h,zones
+       INTEGER(c_int), VALUE :: which,nz
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_transient_strain(h,eik) BIND(C,name="relax_model_get_transient_strain")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h,eik
+     END FUNCTION
      INTEGER(c_int) FUNCTION c_friction(x,y,z,L,W,strike,dip,rake,beta,sig,pre,mu,st,sx1,sx2,sx3, &
           dx1,dx2,dx3,mom,mw,vel,al) BIND(C,name="relax_frictioneigenstress")
        IMPORT :: c_int, c_double, c_ptr
@@ -174,6 +229,55 @@ MODULE relax_hip
        IMPORT :: c_int, c_ptr
        INTEGER(c_int), VALUE :: s
        TYPE(c_ptr) :: c_status_string
+     END FUNCTION
+     ! device-resident model (relax_model_* of relax_hip.h)
+     INTEGER(c_int) FUNCTION c_model_create(d,h) BIND(C,name="relax_model_create")
+       IMPORT :: c_int, c_ptr, relax_model_desc
+       TYPE(relax_model_desc), INTENT(IN) :: d
+       TYPE(c_ptr), INTENT(OUT) :: h
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_destroy(h) BIND(C,name="relax_model_destroy")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_set_structure(h,which,layers,n) BIND(C,name="relax_model_set_structure")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h, layers
+       INTEGER(c_int), VALUE :: which, n
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_set_prestress(h,p,n) BIND(C,name="relax_model_set_prestress")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h, p
+       INTEGER(c_int), VALUE :: n
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_set_planes(h,planes,np,patches) BIND(C,name="relax_model_set_planes")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h, planes, patches
+       INTEGER(c_int), VALUE :: np
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_upload(h,u1,u2,u3,tau,sig,gam,t,oi,i,al) BIND(C,name="relax_model_upload")
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: h, u1, u2, u3, tau, sig, gam
+       REAL(c_double), VALUE :: t
+       INTEGER(c_int), VALUE :: oi, i, al
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_download(h,u1,u2,u3,tau,sig,gam,t,al) BIND(C,name="relax_model_download")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h, u1, u2, u3, tau, sig, gam, t
+       INTEGER(c_int), VALUE :: al
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_step(h,dt,info) BIND(C,name="relax_model_step")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h, dt, info
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_add_shear(h,patches,slip,n) BIND(C,name="relax_model_add_shear")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h, patches, slip
+       INTEGER(c_int), VALUE :: n
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_coseismic(h) BIND(C,name="relax_model_coseismic")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h
      END FUNCTION
   END INTERFACE
 
@@ -311,6 +415,38 @@ CONTAINS
          sx1,sx2,sx3,dx1,dx2,dx3,C_LOC(moment),pm,pd,pg,relax_hip_align),"viscouseigenstress")
   END SUBROUTINE viscouseigenstress
 
+  !> epsilonik and epsilonikdot may be the same array (the corrector's call,
+  !! relax.f90:768-790)
+  SUBROUTINE transienteigenstress(mu,structure,sig,epsilonik,sx1,sx2,sx3, &
+       dx1,dx2,dx3,moment,epsilonikdot,maxwelltime,dgammadot0)
+    INTEGER, INTENT(IN) :: sx1,sx2,sx3
+    REAL*4, DIMENSION(sx1,sx2,sx3), INTENT(IN), OPTIONAL, TARGET :: dgammadot0
+    REAL*8, INTENT(IN) :: mu,dx1,dx2,dx3
+    TYPE(LAYER_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: structure
+    TYPE(TENSOR), INTENT(IN), DIMENSION(sx1,sx2,sx3), TARGET :: sig
+    TYPE(TENSOR), INTENT(INOUT), DIMENSION(sx1,sx2,sx3), TARGET :: moment
+    REAL*8, OPTIONAL, INTENT(INOUT), TARGET :: maxwelltime
+    TYPE(TENSOR), INTENT(IN), DIMENSION(sx1,sx2,sx3), TARGET :: epsilonik
+    TYPE(TENSOR), INTENT(INOUT), DIMENSION(sx1,sx2,sx3), TARGET :: epsilonikdot
+    TYPE(c_ptr) :: pm,pd
+    pm=C_NULL_PTR; pd=C_NULL_PTR
+    IF (PRESENT(maxwelltime)) pm=C_LOC(maxwelltime)
+    IF (PRESENT(dgammadot0)) pd=C_LOC(dgammadot0)
+    CALL check(c_transient(mu,C_LOC(structure),SIZE(structure),C_LOC(sig),C_LOC(epsilonik), &
+         sx1,sx2,sx3,dx1,dx2,dx3,C_LOC(moment),C_LOC(epsilonikdot),pm,pd),"transienteigenstress")
+  END SUBROUTINE transienteigenstress
+
+  SUBROUTINE builddgammadot0(sx1,sx2,sx3,dx1,dx2,dx3,beta,nz,ductilezones,dgammadot0)
+    INTEGER, INTENT(IN) :: sx1,sx2,sx3,nz
+    REAL*4, DIMENSION(sx1,sx2,sx3), INTENT(OUT), TARGET :: dgammadot0
+    TYPE(WEAK_STRUCT), DIMENSION(nz), INTENT(IN), TARGET :: ductilezones
+    REAL*8, INTENT(IN) :: dx1,dx2,dx3,beta
+    TYPE(c_ptr) :: pz
+    pz=C_NULL_PTR
+    IF (nz .GT. 0) pz=C_LOC(ductilezones)
+    CALL check(c_builddg(sx1,sx2,sx3,dx1,dx2,dx3,beta,nz,pz,C_LOC(dgammadot0)),"builddgammadot0")
+  END SUBROUTINE builddgammadot0
+
   SUBROUTINE frictioneigenstress(x,y,z,L,W,strike,dip,rake,beta, &
        sig,prestress,mu,structure,sx1,sx2,sx3,dx1,dx2,dx3,moment,maxwelltime,vel)
     INTEGER, INTENT(IN) :: sx1,sx2,sx3
@@ -353,5 +489,127 @@ CONTAINS
        CALL check(c_frictionadd(C_LOC(n(k)%patch),INT(SIZE(n(k)%patch),c_long),dt),"frictionadd")
     END DO
   END SUBROUTINE frictionadd
+
+  !------------------------------------------------ device-resident model --
+  SUBROUTINE relax_hip_model_create(desc,model)
+    TYPE(relax_model_desc), INTENT(IN) :: desc
+    TYPE(relax_model), INTENT(OUT) :: model
+    CALL check(c_model_create(desc,model%h),"relax_model_create")
+  END SUBROUTINE relax_hip_model_create
+
+  SUBROUTINE relax_hip_model_destroy(model)
+    TYPE(relax_model), INTENT(INOUT) :: model
+    CALL check(c_model_destroy(model%h),"relax_model_destroy")
+    model%h=C_NULL_PTR
+  END SUBROUTINE relax_hip_model_destroy
+
+  !> which: RELAX_LINEAR (in%linearstruc), RELAX_POWERLAW (in%nonlinearstruc)
+  !! or RELAX_FAULTCREEP (in%faultcreepstruc), sampled by viscoelasticstructure
+  SUBROUTINE relax_hip_model_set_structure(model,which,structure)
+    TYPE(relax_model), INTENT(IN) :: model
+    INTEGER, INTENT(IN) :: which
+    TYPE(LAYER_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: structure
+    CALL check(c_model_set_structure(model%h,INT(which,c_int),C_LOC(structure),INT(SIZE(structure),c_int)), &
+         "relax_model_set_structure")
+  END SUBROUTINE relax_hip_model_set_structure
+
+  !> the weak zones of a mechanism (RELAX_LINEAR, RELAX_POWERLAW,
+  !! RELAX_LTRANSIENT, RELAX_NLTRANSIENT): in%linearweakzone, ...
+  SUBROUTINE relax_hip_model_set_weakzones(model,which,zones)
+    TYPE(relax_model), INTENT(IN) :: model
+    INTEGER, INTENT(IN) :: which
+    TYPE(WEAK_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: zones
+    TYPE(c_ptr) :: pz
+    pz=C_NULL_PTR
+    IF (SIZE(zones) .GT. 0) pz=C_LOC(zones)
+    CALL check(c_model_set_weakzones(model%h,INT(which,c_int),pz,INT(SIZE(zones),c_int)), &
+         "relax_model_set_weakzones")
+  END SUBROUTINE relax_hip_model_set_weakzones
+
+  !> the transient strain epsilonik (sx1,sx2,sx3/2) of a model with transient creep
+  SUBROUTINE relax_hip_model_transient_strain(model,epsilonik)
+    TYPE(relax_model), INTENT(IN) :: model
+    TYPE(TENSOR), DIMENSION(:,:,:), INTENT(OUT), TARGET, CONTIGUOUS :: epsilonik
+    CALL check(c_model_transient_strain(model%h,C_LOC(epsilonik)),"relax_model_get_transient_strain")
+  END SUBROUTINE relax_hip_model_transient_strain
+
+  SUBROUTINE relax_hip_model_set_prestress(model,prestress)
+    TYPE(relax_model), INTENT(IN) :: model
+    TYPE(TENSOR_LAYER_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: prestress
+    CALL check(c_model_set_prestress(model%h,C_LOC(prestress),INT(SIZE(prestress),c_int)), &
+         "relax_model_set_prestress")
+  END SUBROUTINE relax_hip_model_set_prestress
+
+  !> the driver's friction planes (in%n), geometry in the internal
+  !! convention as the input routines leave it; their patches start at zero
+  SUBROUTINE relax_hip_model_set_planes(model,n)
+    TYPE(relax_model), INTENT(IN) :: model
+    TYPE(PLANE_STRUCT), DIMENSION(:), INTENT(IN) :: n
+    TYPE(relax_plane_c), DIMENSION(:), ALLOCATABLE, TARGET :: p
+    INTEGER :: k
+    ALLOCATE(p(MAX(1,SIZE(n))))
+    DO k=1,SIZE(n)
+       p(k)=relax_plane_c(n(k)%x,n(k)%y,n(k)%z,n(k)%width,n(k)%length,n(k)%strike,n(k)%dip,n(k)%rake, &
+            INT(n(k)%px2,c_int),INT(n(k)%px3,c_int))
+    END DO
+    CALL check(c_model_set_planes(model%h,C_LOC(p),INT(SIZE(n),c_int),C_NULL_PTR),"relax_model_set_planes")
+    DEALLOCATE(p)
+  END SUBROUTINE relax_hip_model_set_planes
+
+  !> fields in the driver's layout (relax_hip_align); sig may be omitted: it
+  !! is then -tau + C:eps(u) (relax.f90:369-371)
+  SUBROUTINE relax_hip_model_upload(model,u1,u2,u3,tau,t,oi,i,sig,gamma)
+    TYPE(relax_model), INTENT(IN) :: model
+    REAL*4, DIMENSION(*), INTENT(IN), TARGET :: u1,u2,u3
+    TYPE(TENSOR), DIMENSION(*), INTENT(IN), TARGET :: tau
+    REAL*8, INTENT(IN) :: t
+    INTEGER, INTENT(IN) :: oi,i
+    TYPE(TENSOR), DIMENSION(*), INTENT(IN), OPTIONAL, TARGET :: sig
+    REAL*4, DIMENSION(*), INTENT(IN), OPTIONAL, TARGET :: gamma
+    TYPE(c_ptr) :: ps,pg
+    ps=C_NULL_PTR; pg=C_NULL_PTR
+    IF (PRESENT(sig)) ps=C_LOC(sig)
+    IF (PRESENT(gamma)) pg=C_LOC(gamma)
+    CALL check(c_model_upload(model%h,C_LOC(u1),C_LOC(u2),C_LOC(u3),C_LOC(tau),ps,pg,t,INT(oi,c_int), &
+         INT(i,c_int),INT(relax_hip_align,c_int)),"relax_model_upload")
+  END SUBROUTINE relax_hip_model_upload
+
+  SUBROUTINE relax_hip_model_download(model,u1,u2,u3,tau,sig,gamma,t)
+    TYPE(relax_model), INTENT(IN) :: model
+    REAL*4, DIMENSION(*), INTENT(OUT), TARGET :: u1,u2,u3
+    TYPE(TENSOR), DIMENSION(*), INTENT(OUT), OPTIONAL, TARGET :: tau,sig
+    REAL*4, DIMENSION(*), INTENT(OUT), OPTIONAL, TARGET :: gamma
+    REAL*8, INTENT(OUT), OPTIONAL, TARGET :: t
+    TYPE(c_ptr) :: pt,ps,pg,pp
+    pt=C_NULL_PTR; ps=C_NULL_PTR; pg=C_NULL_PTR; pp=C_NULL_PTR
+    IF (PRESENT(tau)) pt=C_LOC(tau)
+    IF (PRESENT(sig)) ps=C_LOC(sig)
+    IF (PRESENT(gamma)) pg=C_LOC(gamma)
+    IF (PRESENT(t)) pp=C_LOC(t)
+    CALL check(c_model_download(model%h,C_LOC(u1),C_LOC(u2),C_LOC(u3),pt,ps,pg,pp,INT(relax_hip_align,c_int)), &
+         "relax_model_download")
+  END SUBROUTINE relax_hip_model_download
+
+  !> one RK2 step (relax.f90:555-890); dt replaces the adaptive step when given
+  SUBROUTINE relax_hip_model_step(model,info,dt)
+    TYPE(relax_model), INTENT(IN) :: model
+    TYPE(relax_step_info), INTENT(OUT), TARGET :: info
+    REAL*8, INTENT(IN), OPTIONAL, TARGET :: dt
+    TYPE(c_ptr) :: pd
+    pd=C_NULL_PTR
+    IF (PRESENT(dt)) pd=C_LOC(dt)
+    CALL check(c_model_step(model%h,pd,C_LOC(info)),"relax_model_step")
+  END SUBROUTINE relax_hip_model_step
+
+  !> momentdensityshear of each patch (internal convention) into tau, as
+  !! dislocations does (relax.f90:1199-1206), then the coseismic u and sigma
+  SUBROUTINE relax_hip_model_coseismic(model,patches,slip)
+    TYPE(relax_model), INTENT(IN) :: model
+    TYPE(relax_plane_c), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: patches
+    REAL*8, DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: slip
+    CALL check(c_model_add_shear(model%h,C_LOC(patches),C_LOC(slip),INT(SIZE(patches),c_int)), &
+         "relax_model_add_shear")
+    CALL check(c_model_coseismic(model%h),"relax_model_coseismic")
+  END SUBROUTINE relax_hip_model_coseismic
 
 END MODULE relax_hip

@@ -3,9 +3,11 @@ import ctypes
 
 import numpy as np
 
-from ._lib import LAYER, PATCH, PLANE, TENSOR_LAYER, ModelDesc, StepInfo, check, load, ptr
+from ._lib import LAYER, PATCH, PLANE, TENSOR_LAYER, WEAK, ModelDesc, StepInfo, check, load, ptr
 
-LINEAR, POWERLAW, FAULTCREEP = 0, 1, 2
+# mechanisms of set_structure (relax.f90:565-650): linear and power-law
+# viscosity, fault creep, linear and power-law transient creep
+LINEAR, POWERLAW, FAULTCREEP, LTRANSIENT, NLTRANSIENT = 0, 1, 2, 3, 4
 
 
 def footprint(sx1, sx2, sx3):
@@ -65,6 +67,27 @@ class Model:
             return
         layers = np.ascontiguousarray(layers, LAYER)
         check(self._lib.relax_model_set_structure(self._h, which, ptr(layers), len(layers)), "set_structure")
+
+    def set_weakzones(self, which, zones):
+        """Weak zones (WEAK array) of mechanism LINEAR, POWERLAW, LTRANSIENT
+        or NLTRANSIENT; an empty array removes them."""
+        zones = np.ascontiguousarray(zones, WEAK)
+        check(self._lib.relax_model_set_weakzones(self._h, which, ptr(zones) if len(zones) else None, len(zones)),
+              "set_weakzones")
+
+    def set_transient_strain(self, epsilonik):
+        """epsilonik as (sx3/2, sx2, sx1, 6) float32 (needs a transient structure)."""
+        e = np.ascontiguousarray(epsilonik, np.float32)
+        sx1, sx2, sx3 = self.shape
+        if e.shape != (sx3 // 2, sx2, sx1, 6):
+            raise ValueError("epsilonik must be (sx3/2, sx2, sx1, 6)")
+        check(self._lib.relax_model_set_transient_strain(self._h, ptr(e)), "set_transient_strain")
+
+    def transient_strain(self):
+        sx1, sx2, sx3 = self.shape
+        e = np.zeros((sx3 // 2, sx2, sx1, 6), np.float32)
+        check(self._lib.relax_model_get_transient_strain(self._h, ptr(e)), "get_transient_strain")
+        return e
 
     def set_prestress(self, prestress):
         prestress = np.ascontiguousarray(prestress, TENSOR_LAYER)
@@ -134,6 +157,22 @@ class Model:
 
     def synchronize(self):
         check(self._lib.relax_model_synchronize(self._h), "synchronize")
+
+    TUNE_VARIANTS = 4
+
+    def autotune(self, times=False):
+        """Choose the launch shapes of the tunable kernels for this grid by
+        timing them (relax_model_autotune); returns the chosen variants, and
+        with times=True also the ms per launch of every variant (None where
+        a variant does not apply)."""
+        c = np.zeros(3, np.int32)
+        ms = np.zeros((3, self.TUNE_VARIANTS), np.float32)
+        check(self._lib.relax_model_autotune(self._h, ptr(c), ptr(ms)), "autotune")
+        out = dict(fft_cols=int(c[0]), green=int(c[1]), bodyforce=int(c[2]))
+        if times:
+            out["ms"] = {k: [None if v < 0 else round(float(v), 4) for v in ms[i]]
+                         for i, k in enumerate(("fft_cols", "green", "bodyforce"))}
+        return out
 
     def surface(self, u1, u2, u3):
         """Copy the surface plane (x3 = 0) of u1..u3 into (sx2, sx1) float32

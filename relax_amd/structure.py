@@ -4,7 +4,7 @@ import math
 
 import numpy as np
 
-from ._lib import LAYER, PLANE
+from ._lib import LAYER, PLANE, WEAK
 
 DEG2RAD = 0.01745329251994329547437168059786927
 
@@ -98,3 +98,18 @@ def friction_plane(x, y, z, length, width, strike, dip, rake, dx2, dx3):
     p = np.zeros(1, PLANE)
     p[0] = (x2, y2, z2, width, length, st2, dp2, rk2, fix(length / dx2), fix(width / dx3))
     return p
+
+
+def weakzones(deck):
+    """Convert deck weak-zone lines (dgammadot0, x, y, z, length, width,
+    thickness, strike, dip) to a WEAK array in the internal convention, as
+    the parser does (input.f90:550-583: Wang's convention of the position,
+    strike and dip; the thickness is kept)."""
+    out = np.zeros(len(deck), WEAK)
+    for k, (dg, x, y, z, L, W, T, st, dp) in enumerate(deck):
+        _, x2, y2, z2, st2, dp2, _ = wangconvention(0.0, x, y, z, L, W, st, dp, 0.0)
+        out[k]["dgammadot0"] = dg
+        out[k]["x"], out[k]["y"], out[k]["z"] = x2, y2, z2
+        out[k]["length"], out[k]["width"], out[k]["thickness"] = L, W, T
+        out[k]["strike"], out[k]["dip"] = st2, dp2
+    return out

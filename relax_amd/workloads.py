@@ -22,7 +22,7 @@ import numpy as np
 
 from . import structure
 from ._lib import LAYER, TENSOR_LAYER
-from .model import FAULTCREEP, LINEAR, POWERLAW, Model
+from .model import FAULTCREEP, LINEAR, LTRANSIENT, NLTRANSIENT, POWERLAW, Model
 
 MU = LAM = 3.0e4          # MPa
 GAM = 8.33e-4             # Cowling coefficient, 1/km
@@ -99,10 +99,21 @@ def tables(s):
     """Per-depth tables (sx3/2 entries) of the mechanisms of a spec."""
     h3 = s["sx3"] // 2
     out = {}
-    for key in ("linear", "nonlinear", "friction"):
-        out[key] = None if s[key] is None else structure.viscoelasticstructure(s[key], h3, s["dx"])
+    for key in ("linear", "nonlinear", "friction", "ltransient", "nltransient"):
+        lay = s.get(key)
+        out[key] = None if lay is None else structure.viscoelasticstructure(lay, h3, s["dx"])
     out["prestress"] = np.zeros(h3, TENSOR_LAYER)
     return out
+
+
+# weak-zone deck keys of a spec and the mechanism each perturbs
+WEAK_KEYS = (("weak_linear", LINEAR), ("weak_nonlinear", POWERLAW), ("weak_ltransient", LTRANSIENT),
+             ("weak_nltransient", NLTRANSIENT))
+
+
+def weak_zones(s):
+    """{mechanism: WEAK array} of the spec's weak zones."""
+    return {mech: structure.weakzones(s[key]) for key, mech in WEAK_KEYS if s.get(key)}
 
 
 def planes(s):
@@ -129,6 +140,12 @@ def configure(m, s):
     m.set_prestress(t["prestress"])
     m.set_structure(LINEAR, t["linear"])
     m.set_structure(POWERLAW, t["nonlinear"])
+    if t["ltransient"] is not None:
+        m.set_structure(LTRANSIENT, t["ltransient"])
+    if t["nltransient"] is not None:
+        m.set_structure(NLTRANSIENT, t["nltransient"])
+    for mech, zones in weak_zones(s).items():
+        m.set_weakzones(mech, zones)
     pl = planes(s)
     if pl is not None:
         m.set_planes(pl)

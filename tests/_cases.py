@@ -3,6 +3,7 @@ import numpy as np
 
 import oracle as O
 from relax_amd import structure, workloads
+from relax_amd.model import LINEAR, LTRANSIENT, NLTRANSIENT, POWERLAW
 
 
 def rel_l2(a, b):
@@ -34,6 +35,23 @@ def small_spec(kind, sx=64, sx3=64, dx=1.0, seed=0):
         return dict(name="afterslip", sx1=sx, sx2=sx, sx3=sx3, dx=dx, linear=None,
                     nonlinear=[(12.0, 4e7, 3.5, 0.0, 0.6, 0.0)], friction=[(0.0, 1e-3, 1.0, 0.0, 0.6, 0.0)],
                     planes=[plane], deck=deck, interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=4)
+    if kind == "transient":
+        # linear Maxwell below 20 km, linear Burgers transient below 6 km and
+        # a power-law Burgers transient below 12 km, with weak zones on the
+        # Maxwell and the linear transient mechanisms
+        deck = [workloads._deck_patch(rng, -10, 10, 5.0, 5.0, 2e-3) for _ in range(3)]
+        return dict(name="transient", sx1=sx, sx2=sx, sx3=sx3, dx=dx, linear=[(20.0, 0.05, 1.0, 0.0, 0.6, 0.0)],
+                    nonlinear=None, friction=None, planes=None, deck=deck, interval=1e9, odt=1.0, skip=1,
+                    tscale=1.0, nsteps=4,
+                    ltransient=[(6.0, 0.4, 1.0, 0.0, 0.6, 1.5e4)],
+                    nltransient=[(12.0, 3e7, 3.0, 0.0, 0.6, 2.0e4)],
+                    weak_linear=[(0.3, -4.0, -2.0, 4.0, 8.0, 6.0, 5.0, 30.0, 60.0)],
+                    weak_ltransient=[(0.8, 3.0, 2.0, 2.0, 6.0, 4.0, 6.0, 0.0, 90.0)])
+    if kind == "ltransient":
+        deck = [workloads._deck_patch(rng, -10, 10, 5.0, 5.0, 2e-3) for _ in range(3)]
+        return dict(name="ltransient", sx1=sx, sx2=sx, sx3=sx3, dx=dx, linear=None, nonlinear=None,
+                    friction=None, planes=None, deck=deck, interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=4,
+                    ltransient=[(4.0, 0.5, 1.0, 0.0, 0.6, 2.0e4)])
     raise KeyError(kind)
 
 
@@ -43,7 +61,15 @@ def oracle_model(s):
     m = dict(sx1=s["sx1"], sx2=s["sx2"], sx3=s["sx3"], dx1=s["dx"], dx2=s["dx"], dx3=s["dx"], lam=workloads.LAM,
              mu=workloads.MU, gam=workloads.GAM, beta=workloads.BETA, linear=t["linear"], nonlinear=t["nonlinear"],
              faultcreep=t["friction"], prestress=t["prestress"], interval=s["interval"], odt=s["odt"],
-             tscale=s["tscale"], skip=s["skip"], event_times=[0.0])
+             tscale=s["tscale"], skip=s["skip"], event_times=[0.0], ltransient=t["ltransient"],
+             nltransient=t["nltransient"])
+    h3 = s["sx3"] // 2
+    keys = {LINEAR: "dg_linear", POWERLAW: "dg_nonlinear", LTRANSIENT: "dg_ltransient",
+            NLTRANSIENT: "dg_nltransient"}
+    for mech, zones in workloads.weak_zones(s).items():
+        # the model builds the field with taper 0, as relax.f90:487-541 does
+        m[keys[mech]] = O.builddgammadot0(s["sx1"], s["sx2"], h3, s["dx"], s["dx"], s["dx"], 0.0,
+                                          zones.astype(O.WEAK), np.float32)
     if pl is not None:
         m["planes"] = np.ascontiguousarray(pl.astype(O.PLANE))
         m["patches"] = np.zeros(int(sum(int(p["px2"]) * int(p["px3"]) for p in pl)), O.PATCH)
@@ -70,4 +96,7 @@ def oracle_initial_state(s, dtype=np.float32):
     sig = -tau
     sig = np.ascontiguousarray(sig)
     O.stressupdate(u[0], u[1], u[2], workloads.LAM, workloads.MU, dx, dx, dx, sig)
-    return dict(u1=u[0], u2=u[1], u3=u[2], tau=tau, sig=sig, gamma=np.zeros_like(u[0]), t=0.0, oi=1, i=0)
+    st = dict(u1=u[0], u2=u[1], u3=u[2], tau=tau, sig=sig, gamma=np.zeros_like(u[0]), t=0.0, oi=1, i=0)
+    if s.get("ltransient") or s.get("nltransient"):
+        st["epsilonik"] = np.zeros((h3, sx2, sx1, 6), dtype)
+    return st

@@ -119,3 +119,35 @@ def test_batch_refuses_mismatched_models_and_member_steps():
     a.step()  # free again once the batch is gone
     a.close()
     other.close()
+
+
+def test_batch_with_transient_member():
+    """A member with transient creep and weak zones switches the batch to the
+    extended rheology kernels; each member agrees with its single run to
+    rounding, and the transient strain follows."""
+    specs = [small_spec("transient", seed=0), _variant("maxwell", 1)]
+    nsteps = 3
+    ref = []
+    for s in specs:
+        m = workloads.build(s)
+        recs = [m.step() for _ in range(nsteps)]
+        out = m.download()
+        if s.get("ltransient"):
+            out["epsilonik"] = m.transient_strain()
+        m.close()
+        ref.append((recs, out))
+    models = [workloads.build(s) for s in specs]
+    with Batch(models) as b:
+        for k in range(nsteps):
+            recs = b.step()
+            for i in range(2):
+                assert abs(recs[i]["Dt"] / ref[i][0][k]["Dt"] - 1) < 1e-6
+    for i, m in enumerate(models):
+        out = m.download()
+        if i == 0:
+            out["epsilonik"] = m.transient_strain()
+        for key in FIELDS + (("epsilonik",) if i == 0 else ()):
+            a, r = out[key].astype(np.float64), ref[i][1][key].astype(np.float64)
+            nr = np.linalg.norm(r)
+            assert np.linalg.norm(a - r) <= 1e-6 * (nr if nr > 0 else 1.0), (i, key)
+        m.close()

@@ -10,8 +10,8 @@ import pytest
 
 import oracle as O
 from _cases import oracle_initial_state, oracle_model, rel_l2, small_spec
-from relax_amd import (LAYER, PATCH, TENSOR_LAYER, RelaxError, elastic3d, fourier, friction3d, green,
-                       viscoelastic3d, workloads)
+from relax_amd import (LAYER, PATCH, TENSOR_LAYER, WEAK, RelaxError, elastic3d, fourier, friction3d, green,
+                       structure, viscoelastic3d, workloads)
 from relax_amd.model import Model
 
 pytestmark = pytest.mark.gpu
@@ -215,6 +215,68 @@ def test_viscous_errors_and_ignored_structure():
     assert np.abs(m).max() == 0
 
 
+@pytest.mark.parametrize("n", [1.0, 3.0])
+def test_transienteigenstress(n):
+    """Rate, moment and relaxation time of the Burgers element with a weak
+    zone field; the corrector's call with one array as strain and rate."""
+    rng = np.random.default_rng(23)
+    h3, sx2, sx1 = 12, 16, 32
+    sig = rnd(rng, h3, sx2, sx1, 6) * (50 if n == 1 else 2e3)
+    e = rnd(rng, h3, sx2, sx1, 6) * 1e-3
+    st = layers(h3, 0.3 if n == 1 else 4e6, n)
+    st["Gk"] = np.linspace(1e4, 3e4, h3)
+    st["gammadot0"][3] = 0.0
+    dg = (rng.uniform(size=(h3, sx2, sx1)) * (0.01 if n == 1 else 1e5)).astype(np.float32)
+    dg[3] = 0.0
+    m, ed = rnd(rng, h3, sx2, sx1, 6), np.full_like(sig, 5.0)
+    rm, red = m.copy(), ed.copy()
+    tm = viscoelastic3d.transienteigenstress(MU, st, sig, e, m, ed, 1e7, dg)
+    rtm = O.transienteigenstress(MU, st, sig, e, rm, red, 1e7, dg)
+    assert np.all(ed[3] == 0) and np.abs(ed).max() > 0
+    assert rel_l2(m, rm) < 1e-5 and rel_l2(ed, red) < 1e-5
+    assert abs(tm / rtm - 1) < 1e-5
+    both, rboth = e.copy(), e.copy()
+    m2, rm2 = np.zeros_like(sig), np.zeros_like(sig)
+    viscoelastic3d.transienteigenstress(MU, st, sig, both, m2, both)
+    O.transienteigenstress(MU, st, sig, rboth, rm2, rboth)
+    assert rel_l2(both, rboth) < 1e-5 and rel_l2(m2, rm2) < 1e-5
+    with pytest.raises(RelaxError) as err:
+        viscoelastic3d.transienteigenstress(MU, layers(h3, n=0.5), sig, e, m, ed)
+    assert err.value.status == 1
+    m3 = np.zeros_like(sig)  # a structure of the wrong size is ignored (viscoelastic3d.f90:144)
+    viscoelastic3d.transienteigenstress(MU, layers(h3 - 1), sig, e, m3, ed)
+    assert np.abs(m3).max() == 0
+
+
+def test_builddgammadot0():
+    """Boxes (tapered, rotated), cylinders and a sphere: the device field
+    equals the oracle's to FP32 rounding, and with no taper the box
+    indicator agrees point for point."""
+    zones = np.zeros(5, WEAK)
+    zones[0] = (0.5, 1.0, -2.0, 5.0, 4.0, 6.0, 3.0, 0.4, 0.3, (0,) * 6)
+    zones[1] = (0.2, -6.0, 4.0, 8.0, 5.0, 7.0, -4.0, 1.2, 0.0, (0,) * 6)
+    zones[2] = (1.5, 3.0, 3.0, 6.0, -6.0, 4.0, -5.0, 0.0, 0.7, (0,) * 6)
+    zones[3] = (0.7, 0.0, 0.0, 4.0, -6.0, -6.0, -6.0, 0.0, 0.0, (0,) * 6)
+    zones[4] = (2.0, -3.0, -5.0, 10.0, 6.0, 3.0, 5.0, 2.5, 1.1, (0,) * 6)
+    sx1, sx2, sx3, dx = 64, 32, 16, (0.5, 0.75, 1.0)
+    for beta in (0.2, 0.0):
+        f = viscoelastic3d.builddgammadot0(sx1, sx2, sx3, *dx, beta, zones)
+        r = O.builddgammadot0(sx1, sx2, sx3, *dx, beta, zones.astype(O.WEAK), np.float32)
+        assert np.abs(r).max() > 0
+        np.testing.assert_allclose(f, r, rtol=1e-6, atol=1e-7)
+    # the transient workload's deck zones, as the model builds them
+    s = small_spec("transient")
+    for key, mech in workloads.WEAK_KEYS:
+        if not s.get(key):
+            continue
+        z = structure.weakzones(s[key])
+        f = viscoelastic3d.builddgammadot0(s["sx1"], s["sx2"], s["sx3"] // 2, s["dx"], s["dx"], s["dx"], 0.0, z)
+        r = O.builddgammadot0(s["sx1"], s["sx2"], s["sx3"] // 2, s["dx"], s["dx"], s["dx"], 0.0,
+                              z.astype(O.WEAK), np.float32)
+        assert (r > 0).sum() > 50
+        np.testing.assert_array_equal(f, r)
+
+
 def friction_layers(h3):
     st = np.zeros(h3, LAYER)
     st["gammadot0"] = 1e-3
@@ -277,12 +339,15 @@ def run_pair(s, nsteps, replay=True):
     g = workloads.build(s)
     recs = g.run(nsteps, dt_replay=info[:, 0] if replay else None)
     out = g.download()
+    if "epsilonik" in st:
+        out["epsilonik"] = g.transient_strain()
     patches = g.get_patches() if om.get("patches") is not None else None
     g.close()
     return st, ost, info, out, recs, om, patches
 
 
-@pytest.mark.parametrize("kind,nsteps", [("maxwell", 6), ("powerlaw", 4), ("afterslip", 4)])
+@pytest.mark.parametrize("kind,nsteps", [("maxwell", 6), ("powerlaw", 4), ("afterslip", 4), ("ltransient", 4),
+                                         ("transient", 4)])
 def test_rk2_trajectory_matches_oracle(kind, nsteps):
     s = small_spec(kind)
     st, ost, info, out, recs, om, patches = run_pair(s, nsteps)
@@ -293,6 +358,9 @@ def test_rk2_trajectory_matches_oracle(kind, nsteps):
         assert rel_l2(out[k], ost[k]) < 1e-4, k
     if np.abs(ost["gamma"]).max() > 0:
         assert rel_l2(out["gamma"][..., :s["sx1"]], ost["gamma"][..., :s["sx1"]]) < 1e-4
+    if "epsilonik" in ost:
+        assert np.abs(ost["epsilonik"]).max() > 0
+        assert rel_l2(out["epsilonik"], ost["epsilonik"]) < 1e-4
     for k, r in enumerate(recs):
         assert abs(r["power"] / info[k, 5] - 1) < 1e-4
         assert abs(r["cei"] / info[k, 6] - 1) < 1e-4
@@ -304,16 +372,16 @@ def test_rk2_trajectory_matches_oracle(kind, nsteps):
         assert rel_l2(patches["v"], ref["v"]) < 1e-4
 
 
-@pytest.mark.parametrize("kind", ["maxwell", "powerlaw", "afterslip"])
+@pytest.mark.parametrize("kind", ["maxwell", "powerlaw", "afterslip", "transient"])
 def test_rk2_adaptive_step_matches_oracle(kind):
-    """Without replay the device-side Dt (min 1/gammadot, integrationstep)
-    follows the oracle's to FP32 accuracy."""
+    """Without replay the device-side Dt (min 1/gammadot over the five
+    mechanisms, integrationstep) follows the oracle's to FP32 accuracy."""
     s = small_spec(kind)
     st, ost, info, out, recs, om, patches = run_pair(s, 3, replay=False)
     for k, r in enumerate(recs):
         assert abs(r["Dt"] / info[k, 0] - 1) < 1e-5
-        for j in range(3):
-            assert abs(r["maxwell"][j] / info[k, 2 + j] - 1) < 1e-5
+        for j, col in enumerate((2, 3, 4, 8, 9)):
+            assert abs(r["maxwell"][j] / info[k, col] - 1) < 1e-5, (k, j)
     assert rel_l2(out["sig"], ost["sig"]) < 1e-4
 
 

@@ -1,0 +1,37 @@
+"""The autotuner (relax_model_autotune) picks launch shapes by timing them on
+the model's scratch buffers: the model's state must not change, and a model
+stepped with the chosen shapes agrees with the default shapes to FP32
+rounding (the variants differ in FFT radix and tile, not in the algorithm)."""
+import numpy as np
+import pytest
+
+from relax_amd import workloads
+
+from _cases import rel_l2, small_spec
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_keeps_state_and_results():
+    s = small_spec("powerlaw", sx=128, sx3=128)
+    ref = workloads.build(s)
+    before = ref.download()
+    recs_ref = [ref.step() for _ in range(3)]
+    out_ref = ref.download()
+    ref.close()
+
+    m = workloads.build(s)
+    choice = m.autotune()
+    assert 0 <= choice["fft_cols"] < 3 and 0 <= choice["green"] < 3 and 0 <= choice["bodyforce"] < 2
+    after = m.download()
+    for k in ("u1", "u2", "u3", "sig", "tau", "gamma"):
+        np.testing.assert_array_equal(after[k], before[k], err_msg=k)
+    recs = [m.step() for _ in range(3)]
+    out = m.download()
+    m.close()
+    for k in ("u1", "u2", "u3"):
+        assert rel_l2(out[k][..., :128], out_ref[k][..., :128]) < 1e-5, k
+    for k in ("sig", "tau"):
+        assert rel_l2(out[k], out_ref[k]) < 1e-5, k
+    for a, b in zip(recs, recs_ref):
+        assert abs(a["t"] - b["t"]) <= 1e-6 * abs(b["t"])

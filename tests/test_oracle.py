@@ -22,10 +22,19 @@ Parity status per function:
                               equation + free-surface traction reduction
   stressupdate / equivalentbodyforce ... pinned (FIR closed forms)
   viscouseigenstress ........ pinned (power-law closed forms)
+  transienteigenstress ...... pinned (Kelvin-element closed forms: the
+                              linear exponential approach to s'/(2 Gk) and
+                              the n = 3 algebraic decay; aliasing of the
+                              strain and rate arrays)
+  builddgammadot0 ........... pinned (brute-force point counts of boxes and
+                              spheres, rotation by the strike, linearity
+                              in the zones)
   frictioneigenstress ....... pinned (invariants, delta normalisation with image)
   monitorfriction/frictionadd pinned (invariants, closed form)
   tensoramplitude ........... pinned (closed form)
-  RK2 loop .................. pinned (uniform-stress closed-form decay)
+  RK2 loop .................. pinned (uniform-stress closed-form decay,
+                              Burgers transient midpoint recurrence, weak
+                              zones equal to the same fluidity in layers)
 """
 import numpy as np
 import pytest
@@ -709,3 +718,239 @@ def test_fp32_oracle_tracks_fp64():
     for k in ("sig", "tau", "u1", "u3"):
         a, b = out[np.float32][k].astype(np.float64), out[np.float64][k]
         assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-5, k
+
+
+# ------------------------------------------------- transient creep (Burgers) ---
+def _transient_layers(h3, gd0, n, gk):
+    st = _layers(h3, gd0, n)
+    st["Gk"] = gk
+    return st
+
+
+def _shear_field(shape, s12, dtype=np.float64):
+    t = np.zeros(shape + (6,), dtype)
+    t[..., 1] = s12
+    return t
+
+
+def _integrate_transient(st, s12, e0, h, nsteps):
+    """Classical RK4 in time of dE/dt = rate(E) at fixed stress, the rate
+    taken from the oracle routine (one point)."""
+    sig = _shear_field((1, 1, 1), s12)
+    e = _shear_field((1, 1, 1), e0)
+
+    def rate(ev):
+        ed = np.zeros_like(ev)
+        mom = np.zeros_like(ev)
+        O.transienteigenstress(MU, st, sig, ev, mom, ed)
+        return ed
+
+    for _ in range(nsteps):
+        k1 = rate(e)
+        k2 = rate(e + h / 2 * k1)
+        k3 = rate(e + h / 2 * k2)
+        k4 = rate(e + h * k3)
+        e = e + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    return e[0, 0, 0, 1]
+
+
+@pytest.mark.parametrize("mut", [0, 14, 15])
+def test_transient_linear_kelvin_closed_form(mut, mutation):
+    """n = 1 at constant shear stress s: E(t) = s/(2 Gk) (1 - exp(-2 Gk gd0 t / mu))."""
+    gd0, gk, s0 = 0.3, 2.0e4, 5.0
+    st = _transient_layers(1, gd0, 1.0, gk)
+    t_end, nsteps = 40.0, 400
+    mutation(mut)
+    e = _integrate_transient(st, s0, 0.0, t_end / nsteps, nsteps)
+    mutation(0)
+    ref = s0 / (2 * gk) * (1 - np.exp(-2 * gk * gd0 * t_end / MU))
+    if mut:
+        assert abs(e / ref - 1) > 1e-3
+    else:
+        np.testing.assert_allclose(e, ref, rtol=1e-9)
+
+
+@pytest.mark.parametrize("mut", [0, 15])
+def test_transient_power_law_closed_form(mut, mutation):
+    """n = 3: q = s - 2 Gk E obeys dq/dt = -(2 Gk gd0 / mu^3) q^3, so
+    q(t) = q0 / sqrt(1 + 2 (2 Gk gd0 / mu^3) q0^2 t)."""
+    gd0, gk, s0 = 2.0, 1.0e4, 3.0e3
+    st = _transient_layers(1, gd0, 3.0, gk)
+    t_end, nsteps = 30.0, 600
+    mutation(mut)
+    e = _integrate_transient(st, s0, 0.0, t_end / nsteps, nsteps)
+    mutation(0)
+    c = 2 * gk * gd0 / MU ** 3
+    q = s0 / np.sqrt(1 + 2 * c * s0 ** 2 * t_end)
+    ref = (s0 - q) / (2 * gk)
+    if mut:
+        assert abs(e / ref - 1) > 1e-3
+    else:
+        np.testing.assert_allclose(e, ref, rtol=1e-8)
+
+
+def test_transient_rate_moment_and_alias():
+    """moment = 2 mu Edot; a point with gammadot0 < 1e-20 gets Edot = 0 and
+    no moment; passing one array as strain and rate (the corrector's call)
+    equals passing a copy; a power exponent below 1 is status 1."""
+    rng = np.random.default_rng(3)
+    h3, sx = 3, 4
+    st = _transient_layers(h3, 0.4, 3.0, 1.5e4)
+    st["gammadot0"][1] = 0.0
+    sig = rng.standard_normal((h3, sx, sx, 6)) * 1e3
+    e = rng.standard_normal((h3, sx, sx, 6)) * 1e-2
+    mom = np.zeros_like(sig)
+    ed = np.full_like(sig, 7.0)
+    tm = O.transienteigenstress(MU, st, sig, e, mom, ed, maxwelltime=1e7)
+    np.testing.assert_allclose(mom, 2 * MU * ed, rtol=1e-12)
+    assert np.all(ed[1] == 0) and np.all(mom[1] == 0)
+    assert np.all(np.abs(ed[0]) > 0)
+    # 1/gammadot of every active point, gammadot = gd0 (|Q|/mu)^2
+    sd = sig.copy()
+    tr = (sd[..., 0] + sd[..., 3] + sd[..., 5]) / 3
+    for k in (0, 3, 5):
+        sd[..., k] -= tr
+    q = sd - 2 * 1.5e4 * e
+    nq = np.sqrt((q[..., 0] ** 2 + q[..., 3] ** 2 + q[..., 5] ** 2
+                  + 2 * (q[..., 1] ** 2 + q[..., 2] ** 2 + q[..., 4] ** 2)) / 2)
+    gd = 0.4 * (nq / MU) ** 2
+    np.testing.assert_allclose(tm, np.float32(1 / gd[[0, 2]].max()), rtol=1e-6)
+    both = e.copy()
+    mom2 = np.zeros_like(sig)
+    O.transienteigenstress(MU, st, sig, both, mom2, both)
+    np.testing.assert_array_equal(both, ed)
+    np.testing.assert_array_equal(mom2, mom)
+    bad = st.copy()
+    bad["stressexponent"][2] = 0.5
+    with pytest.raises(RuntimeError):
+        O.transienteigenstress(MU, bad, sig, e, mom, ed)
+
+
+# ------------------------------------------------------------- weak zones ---
+def _weak(**kw):
+    z = np.zeros(1, O.WEAK)
+    for k, v in kw.items():
+        z[k] = v
+    return z
+
+
+def _coords(sx, dx):
+    i = np.arange(sx)
+    return np.where(i < sx // 2, i, i - sx) * dx
+
+
+@pytest.mark.parametrize("mut", [0, 16])
+def test_weak_box_counts_points(mut, mutation):
+    """beta = 0: a box zone is the indicator of |x - c| <= extent/2 along each
+    of its axes (x1: thickness, x2: length, x3: width), times dgammadot0."""
+    sx1, sx2, sx3, dx = 32, 32, 16, 0.5
+    z = _weak(dgammadot0=2.5, x=1.0, y=-2.0, z=3.0, thickness=3.0, length=5.0, width=2.0)
+    mutation(mut)
+    f = O.builddgammadot0(sx1, sx2, sx3, dx, dx, dx, 0.0, z, np.float64)
+    mutation(0)
+    x1, x2, x3 = _coords(sx1, dx), _coords(sx2, dx), np.arange(sx3) * dx
+    ref = (np.abs(x3 - 3.0) <= 1.0)[:, None, None] * (np.abs(x2 + 2.0) <= 2.5)[None, :, None] \
+        * (np.abs(x1 - 1.0) <= 1.5)[None, None, :] * 2.5
+    if mut:
+        assert np.abs(f - ref).max() > 1
+    else:
+        np.testing.assert_array_equal(f, ref)
+
+
+def test_weak_sphere_and_cylinder_count_points():
+    """Negative extents select spheres and cylinders, but every point is
+    first gated by |x_i - c_i| <= LM, LM the largest tapered half extent
+    (viscoelastic3d.f90:381-387).  A sphere (all three negative) has LM < 0
+    and adds nothing; a cylinder with D, W < 0 and L > 0 is the disc
+    rho <= 1/2 in x1-x2 times |x3 - z| <= L/2, clipped to the LM box."""
+    sx, dx = 32, 0.5
+    x1, x2, x3 = _coords(sx, dx), _coords(sx, dx), np.arange(sx // 2) * dx
+    X3, X2, X1 = np.meshgrid(x3, x2, x1, indexing="ij")
+    z = _weak(dgammadot0=1.0, x=0.5, y=1.0, z=4.0, thickness=-4.0, length=-4.0, width=-4.0)
+    f = O.builddgammadot0(sx, sx, sx // 2, dx, dx, dx, 0.0, z, np.float64)
+    assert np.all(f == 0)
+    z = _weak(dgammadot0=1.0, x=0.5, y=1.0, z=4.0, thickness=-4.0, length=-4.0, width=3.0)
+    f = O.builddgammadot0(sx, sx, sx // 2, dx, dx, dx, 0.0, z, np.float64)
+    rho = np.sqrt(((X1 - 0.5) / 4) ** 2 + ((X2 - 1.0) / 4) ** 2)
+    lm = 1.5
+    gate = (np.abs(X1 - 0.5) <= lm) & (np.abs(X2 - 1.0) <= lm) & (np.abs(X3 - 4.0) <= lm)
+    ref = (rho <= 0.5) * (np.abs(X3 - 4.0) <= 1.5) * gate * 1.0
+    assert ref.sum() > 20
+    np.testing.assert_array_equal(f, ref)
+
+
+def test_weak_strike_rotates_and_zones_add():
+    """strike = 90 degrees maps the zone's x1 axis onto -x2: the field is
+    the unrotated one with x1 and x2 exchanged (centred zone, symmetric
+    grid range); the field of two zones is the sum of their fields; the
+    taper keeps dgammadot0 at the centre and decreases outwards."""
+    sx, dx, beta = 32, 0.5, 0.25
+    a = _weak(dgammadot0=1.0, x=0.0, y=0.0, z=3.0, thickness=2.0, length=5.0, width=2.0)
+    b = a.copy()
+    b["strike"] = np.pi / 2
+    fa = O.builddgammadot0(sx, sx, 16, dx, dx, dx, beta, a, np.float64)
+    fb = O.builddgammadot0(sx, sx, 16, dx, dx, dx, beta, b, np.float64)
+    # indices 1..sx-1 hold the symmetric range -(sx/2-1)..(sx/2-1)
+    sym = np.r_[np.arange(1, sx // 2), np.arange(sx // 2 + 1, sx)]
+    ca = fa[:, sym][:, :, sym]
+    cb = fb[:, sym][:, :, sym]
+    np.testing.assert_allclose(cb, np.swapaxes(ca, 1, 2), atol=1e-12)
+    c = _weak(dgammadot0=0.5, x=-3.0, y=2.0, z=5.0, thickness=3.0, length=3.0, width=3.0, dip=0.4)
+    fc = O.builddgammadot0(sx, sx, 16, dx, dx, dx, beta, c, np.float64)
+    fac = O.builddgammadot0(sx, sx, 16, dx, dx, dx, beta, np.concatenate([a, c]), np.float64)
+    np.testing.assert_allclose(fac, fa + fc, rtol=0, atol=1e-15)
+    assert fa[6, 0, 0] == 1.0  # centre (x3 = 3 at i3 = 6)
+    line = fa[6, 0, : sx // 2]
+    assert np.all(np.diff(line) <= 0) and line[-1] == 0.0
+
+
+# ------------------------------------------------ RK2 with transient creep ---
+def test_rk2_transient_midpoint_recurrence():
+    """Uniform shear s12 over a linear Burgers (Kelvin) element, no Maxwell
+    mechanism: u stays 0 and (sigma, E) follow the midpoint scheme of
+    dsigma/dt = -2 gd0 q, dE/dt = gd0/mu q, q = sigma - 2 Gk E, with
+    Dt = tm/10 = 1/(10 gd0); the scheme converges to the exact solution."""
+    sx, sx3 = 8, 16
+    h3 = sx3 // 2
+    gd0, gk, s0 = 0.2, 1.5e4, 4.0
+    m = _uniform_model(sx=sx, sx3=sx3)
+    m["linear"] = None
+    m["ltransient"] = _transient_layers(h3, gd0, 1.0, gk)
+    s = _uniform_state(sx, sx3, s0)
+    s["epsilonik"] = np.zeros((h3, sx, sx, 6))
+    info = O.rk2_run(m, s, 6)
+    sig, e = s0, 0.0
+    dt = np.float64(np.float32(1 / gd0)) / 10
+    for k in range(6):
+        q = sig - 2 * gk * e
+        em = e + dt / 2 * gd0 / MU * q
+        sp = sig - dt / 2 * 2 * gd0 * q
+        qc = sp - 2 * gk * em
+        e = e + dt * gd0 / MU * qc
+        sig = sig - dt * 2 * gd0 * qc
+        np.testing.assert_allclose(info[k, 0], dt, rtol=1e-12)
+        np.testing.assert_allclose(info[k, 8], np.float32(1 / gd0), rtol=1e-7)
+    np.testing.assert_allclose(s["sig"][..., 1], sig, rtol=1e-12)
+    np.testing.assert_allclose(s["epsilonik"][..., 1], e, rtol=1e-12)
+    assert np.abs(s["u1"]).max() == 0 and np.abs(s["u3"]).max() == 0
+    lam = 2 * gd0 * (1 + gk / MU)
+    t = info[:, 7][-1]
+    exact = s0 - 2 * gd0 * s0 * (1 - np.exp(-lam * t)) / lam
+    # second-order scheme at lambda Dt = 0.3: about 1e-2 after six steps
+    assert abs(sig / exact - 1) < 2e-2
+
+
+def test_rk2_weak_zone_equals_layered_fluidity():
+    """A linear structure with no fluidity plus a weak-zone field that holds
+    0.2 everywhere steps exactly as the structure with gammadot0 = 0.2."""
+    sx, sx3 = 8, 16
+    h3 = sx3 // 2
+    m = _uniform_model(sx=sx, sx3=sx3)
+    s = _uniform_state(sx, sx3, 3.0)
+    ref = O.rk2_run(m, s, 3)
+    m2 = _uniform_model(sx=sx, sx3=sx3, gd0=0.0)
+    m2["dg_linear"] = np.full((h3, sx, sx), 0.2)
+    s2 = _uniform_state(sx, sx3, 3.0)
+    got = O.rk2_run(m2, s2, 3)
+    np.testing.assert_array_equal(got, ref)
+    np.testing.assert_array_equal(s2["sig"], s["sig"])
