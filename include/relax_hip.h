@@ -271,6 +271,11 @@ int relax_model_surface(relax_model_t* m, float* u1, float* u2, float* u3, int a
  * is not changed.  Not on a member of a batch. */
 #define RELAX_TUNE_VARIANTS 4
 int relax_model_autotune(relax_model_t* m, int* choice, float* ms);
+/* Sets the three variants (as relax_model_autotune reports them) without
+ * timing, e.g. a choice kept from an earlier run; RELAX_ERR_ARG when out of
+ * range.  A variant that does not fit the grid makes the step return
+ * RELAX_ERR_SIZE. */
+int relax_model_set_tuning(relax_model_t* m, const int* choice);
 /* The model's HIP stream (hipStream_t) for callers that order their own work. */
 void* relax_model_stream(relax_model_t* m);
 

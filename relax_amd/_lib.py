@@ -98,6 +98,7 @@ SIGNATURES = {
     "relax_model_coseismic": (c_int, [P]),
     "relax_model_profile": (c_int, [P, c_int, P, P]),
     "relax_model_autotune": (c_int, [P, P, P]),
+    "relax_model_set_tuning": (c_int, [P, P]),
     "relax_hip_meminfo": (c_int, [P, P]),
     "relax_batch_create": (c_int, [P]),
     "relax_batch_destroy": (c_int, [P]),

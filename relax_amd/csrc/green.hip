@@ -285,6 +285,188 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
   body(A);
 }
 
+// Register-resident form of the column pass for H3 = 16 Q (Q = 2, 4, 8):
+// 16 columns per workgroup, so each depth row of a tile is 128 contiguous
+// bytes.  The H3-point transforms are split as 16 x Q:
+//   z = Q m + j,  k = k1 + 16 k2:
+//   step A (thread c, j, column): DFT16 over m in registers, twiddle
+//     w_H3^(j k1), through LDS ([c][j][k1][column]);
+//   step B (thread half, k1, column): DFT_Q over j for the three components
+//     in registers -> X[k1 + 16 k2].
+// The even (E) and odd (O: x w_n3^z) halves go through the one 48 KB LDS
+// buffer in turn; E ends in threads 0..255 and O in 256..511, which then
+// apply the Green's function and the traction sums to their own 3 Q
+// frequencies, reduce the sums per column, and run the inverse the same way
+// back (B^-1 then A^-1, E then O) to y = E' + w^-z O'.
+template <int H3>
+__global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu, double alpha, double gamma, int n1,
+                                                   int n2, int n3, MPtr<float2> W_m, MPtr<float2> NQ_m,
+                                                   MPtr<const float2> TM_m, MPtr<const float2> TN_m,
+                                                   const float2* __restrict__ tw3, int l2) {
+  constexpr int TX = 16, NT = 512, Q = H3 / 16;
+  static_assert(Q == 2 || Q == 4 || Q == 8, "H3 = 32, 64 or 128");
+  constexpr int NA = 3 * Q * TX;  // step-A threads
+  constexpr int NB = 16 * TX;     // step-B threads per half
+  constexpr int RS = 16 * TX;     // float2 per (c, j) group of the transpose buffer
+  constexpr int NW = NT / 64;
+  float2* W = W_m.at(blockIdx.y);
+  float2* NQ = NQ_m.at(blockIdx.y);
+  const float2* __restrict__ TM = TM_m.at(blockIdx.y);
+  const float2* __restrict__ TN = TN_m.at(blockIdx.y);
+  __shared__ float2 buf[3 * Q * RS];
+  __shared__ float2 stw[2 * H3];
+  // the per-column reductions reuse the transpose buffer
+  float2(*red)[NW][TX] = reinterpret_cast<float2(*)[NW][TX]>(buf);
+  float2(*pv)[TX] = reinterpret_cast<float2(*)[TX]>(buf + 3 * NW * TX);
+  static_assert(3 * NW * TX + 3 * TX <= 3 * Q * RS, "reductions fit the buffer");
+
+  const int tid = threadIdx.x;
+  const int acol = tid % TX, aj = (tid / TX) % Q, ac = tid / (TX * Q);  // step A
+  const int bcol = tid % TX, bk1 = (tid / TX) % 16, half = tid / NB;    // step B
+  const bool isA = tid < NA;
+  const GreenTileAddr A = green_tile<TX>(xcd_tile(blockIdx.x, gridDim.x), bcol, n1, n2, H3, l2, W, NQ, TM, TN);
+
+  // step A of one half: DFT16, twiddle, into the buffer.  The odd half
+  // loads the column again (from L2) rather than keeping it in registers.
+  float2 x[16];
+  auto load = [&]() {
+#pragma unroll
+    for (int m = 0; m < 16; ++m) x[m] = A.base[ac * A.cs + (Q * m + aj) * A.zs + acol];
+  };
+  if (isA) load();
+  for (int e = tid; e < 2 * H3; e += NT) stw[e] = tw3[e];
+  __syncthreads();
+
+  auto stepA = [&](bool odd) {
+    if (!isA) return;
+    float2* y = x;
+    if (odd) {
+      load();
+#pragma unroll
+      for (int m = 0; m < 16; ++m) y[m] = cmul(x[m], stw[Q * m + aj]);
+    }
+    dft16<-1>(y);
+    float2* row = buf + (ac * Q + aj) * RS + acol;
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) row[k1 * TX] = k1 && aj ? cmul(y[k1], stw[(2 * aj * k1) % (2 * H3)]) : y[k1];
+  };
+  // step B: this thread's 3 Q frequencies of its half
+  float2 X[3][Q];
+  auto stepB = [&]() {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int j = 0; j < Q; ++j) X[c][j] = buf[(c * Q + j) * RS + bk1 * TX + bcol];
+      dft<Q, -1>(X[c]);
+    }
+  };
+  stepA(false);
+  __syncthreads();
+  if (half == 0) stepB();
+  __syncthreads();
+  stepA(true);
+  __syncthreads();
+  if (half == 1) stepB();
+
+  // Green's function and the traction partial sums
+  const float k1w = (float)A.k1i * C.ik1;
+  const float k2w = wavenumber23f(A.k2i, n2, C.ik2);
+  float2 ps1 = make_float2(0.f, 0.f), ps2 = ps1, ps3 = ps1;
+#pragma unroll
+  for (int k2 = 0; k2 < Q; ++k2) {
+    const int k3i = 2 * (bk1 + 16 * k2) + half;
+    const float k3 = wavenumber23f(k3i, n3, C.ik3);
+    float2 a1 = X[0][k2], a2 = X[1][k2], a3 = X[2][k2];
+    if (A.k1i == 0 && A.k2i == 0 && k3i == 0) {
+      a1 = a2 = a3 = make_float2(0.f, 0.f);
+    } else {
+      elastic_point(C, k1w, k2w, k3, a1, a2, a3);
+    }
+    X[0][k2] = a1; X[1][k2] = a2; X[2][k2] = a3;
+    float2 s1, s2, s3;
+    traction_point(C, k1w, k2w, k3, a1, a2, a3, s1, s2, s3);
+    ps1 = cadd(ps1, s1); ps2 = cadd(ps2, s2); ps3 = cadd(ps3, s3);
+  }
+  // reduce over the threads of a column (fixed order): lanes 16 apart in a
+  // wave, then the waves
+#pragma unroll
+  for (int off = 32; off >= TX; off >>= 1) {
+    ps1.x += __shfl_xor(ps1.x, off); ps1.y += __shfl_xor(ps1.y, off);
+    ps2.x += __shfl_xor(ps2.x, off); ps2.y += __shfl_xor(ps2.y, off);
+    ps3.x += __shfl_xor(ps3.x, off); ps3.y += __shfl_xor(ps3.y, off);
+  }
+  __syncthreads();  // the last step B has read the buffer
+  if ((tid & 63) < TX) {
+    red[0][tid >> 6][bcol] = ps1; red[1][tid >> 6][bcol] = ps2; red[2][tid >> 6][bcol] = ps3;
+  }
+  __syncthreads();
+  if (tid < 3 * TX) {
+    const int c = tid / TX, t = tid % TX;
+    float2 acc = red[c][0][t];
+    for (int w = 1; w < NW; ++w) acc = cadd(acc, red[c][w][t]);
+    const float2 tt = A.tbase[c * A.tcs + t];
+    // p = t - p / (sx3 dx3)   (green.f90:373-375, :1082-1084)
+    pv[c][t] = make_float2(tt.x - acc.x / C.inv_n3dx3, tt.y - acc.y / C.inv_n3dx3);
+  }
+  __syncthreads();
+  {
+    const float2 p1 = pv[0][bcol], p2 = pv[1][bcol], p3 = pv[2][bcol];
+    const CerrutiCol cc = cerruti_column(C, mu, alpha, gamma, k1w, k2w, p1, p2, p3);
+#pragma unroll
+    for (int k2 = 0; k2 < Q; ++k2) {
+      const float k3 = wavenumber23f(2 * (bk1 + 16 * k2) + half, n3, C.ik3);
+      cerruti_point(C, cc, p1, p2, k3, X[0][k2], X[1][k2], X[2][k2]);
+    }
+  }
+  __syncthreads();  // pv is read before the buffer is refilled
+
+  // inverse: B^-1 into the buffer, A^-1 out of it
+  auto stepBinv = [&]() {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      dft<Q, +1>(X[c]);
+#pragma unroll
+      for (int j = 0; j < Q; ++j)
+        buf[(c * Q + j) * RS + bk1 * TX + bcol] =
+            j && bk1 ? cmul(X[c][j], cconj(stw[(2 * j * bk1) % (2 * H3)])) : X[c][j];
+    }
+  };
+  auto stepAinv = [&](float2* y) {
+    const float2* row = buf + (ac * Q + aj) * RS + acol;
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) y[k1] = row[k1 * TX];
+    dft16<+1>(y);
+  };
+  if (half == 0) stepBinv();
+  __syncthreads();
+  if (isA) stepAinv(x);  // E'
+  __syncthreads();
+  if (half == 1) stepBinv();
+  __syncthreads();
+  if (isA) {
+    float2 o[16];
+    stepAinv(o);  // O'
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int z = Q * m + aj;
+      A.base[ac * A.cs + z * A.zs + acol] = cadd(x[m], cmul(o[m], cconj(stw[z])));
+    }
+  }
+}
+
+template <int HH>
+static int green_reg_launch(const GreenConst& gc, const ColumnConsts& C, double alpha, MPtr<float2> W,
+                            MPtr<float2> NQ, MPtr<const float2> TM, MPtr<const float2> TN, int nm,
+                            const float2* tw3, int l2, hipStream_t st) {
+  constexpr int TX = 16;
+  const int m1 = gc.n1 / 2;
+  if (HH < 32 || HH > 128 || m1 % TX || gc.n2 % TX) return RELAX_ERR_SIZE;
+  const int nb = (m1 / TX) * gc.n2 + gc.n2 / TX;
+  k_green_reg<(HH < 32 ? 32 : HH > 128 ? 128 : HH)><<<dim3(nb, nm), 512, 0, st>>>(
+      C, gc.mu, alpha, gc.gamma, gc.n1, gc.n2, gc.n3, W, NQ, TM, TN, tw3, l2);
+  return RELAX_OK;
+}
+
 // Launch shapes of the column pass: columns per workgroup, threads, largest
 // radix.  Variant 0 is the default (measured at 512x512x256: 4 columns of
 // 128 with 256 threads and radix 16 is 4 % faster than 8 columns with 512
@@ -323,7 +505,8 @@ int launch_green_columns(const GreenConst& gc, int h3, MPtr<float2> W, MPtr<floa
   case HH:                                                                                             \
     if (variant == 0) rc = green_launch<HH, 0>(gc, C, alpha, W, NQ, TM, TN, nm, tw3, l2, st);          \
     else if (variant == 1) rc = green_launch<HH, 1>(gc, C, alpha, W, NQ, TM, TN, nm, tw3, l2, st);     \
-    else rc = green_launch<HH, 2>(gc, C, alpha, W, NQ, TM, TN, nm, tw3, l2, st);                       \
+    else if (variant == 2) rc = green_launch<HH, 2>(gc, C, alpha, W, NQ, TM, TN, nm, tw3, l2, st);     \
+    else rc = green_reg_launch<HH>(gc, C, alpha, W, NQ, TM, TN, nm, tw3, l2, st);                      \
     break;
   switch (h3) {
     X(8) X(16) X(32) X(64) X(128) X(256)

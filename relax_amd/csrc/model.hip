@@ -1332,6 +1332,18 @@ static int wait_batch(relax_model_t* m) {
 // buffers (W, NQ, T, TN are scratch between steps; the divergence pass in its
 // corrector form only reads the stress and writes W and T) and keeps the
 // fastest per device and grid.
+int relax_model_set_tuning(relax_model_t* m, const int* choice) {
+  if (!m || !choice) return RELAX_ERR_ARG;
+  if (choice[0] < 0 || choice[0] >= COLS_VARIANTS || choice[1] < 0 || choice[1] >= GREEN_VARIANTS ||
+      choice[2] < 0 || choice[2] >= BODY_VARIANTS)
+    return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
+  m->tune_cols = choice[0];
+  m->tune_green = choice[1];
+  m->tune_body = choice[2];
+  return RELAX_OK;
+}
+
 int relax_model_autotune(relax_model_t* m, int* choice, float* ms) {
   if (!m || m->batch) return RELAX_ERR_ARG;
   static_assert(COLS_VARIANTS <= RELAX_TUNE_VARIANTS && GREEN_VARIANTS <= RELAX_TUNE_VARIANTS &&

@@ -174,6 +174,11 @@ class Model:
                          for i, k in enumerate(("fft_cols", "green", "bodyforce"))}
         return out
 
+    def set_tuning(self, fft_cols=0, green=0, bodyforce=0):
+        """Use the given launch-shape variants (relax_model_set_tuning)."""
+        c = np.array([fft_cols, green, bodyforce], np.int32)
+        check(self._lib.relax_model_set_tuning(self._h, ptr(c)), "set_tuning")
+
     def surface(self, u1, u2, u3):
         """Copy the surface plane (x3 = 0) of u1..u3 into (sx2, sx1) float32
         arrays."""

@@ -22,7 +22,7 @@ def test_autotune_keeps_state_and_results():
 
     m = workloads.build(s)
     choice = m.autotune()
-    assert 0 <= choice["fft_cols"] < 3 and 0 <= choice["green"] < 3 and 0 <= choice["bodyforce"] < 2
+    assert 0 <= choice["fft_cols"] < 3 and 0 <= choice["green"] < 4 and 0 <= choice["bodyforce"] < 2
     after = m.download()
     for k in ("u1", "u2", "u3", "sig", "tau", "gamma"):
         np.testing.assert_array_equal(after[k], before[k], err_msg=k)
@@ -35,3 +35,24 @@ def test_autotune_keeps_state_and_results():
         assert rel_l2(out[k], out_ref[k]) < 1e-5, k
     for a, b in zip(recs, recs_ref):
         assert abs(a["t"] - b["t"]) <= 1e-6 * abs(b["t"])
+
+
+@pytest.mark.parametrize("sx3", [64, 128, 256])
+@pytest.mark.parametrize("green", [1, 2, 3])
+def test_every_green_variant_matches_the_default(green, sx3):
+    """Each launch shape of the Green column pass (4 or 8 columns through LDS
+    stages of radix 16 or 8, or 16 columns with register transforms) gives
+    the default's trajectory to FP32 rounding."""
+    s = small_spec("powerlaw", sx=64, sx3=sx3)
+    out = {}
+    for v in (0, green):
+        m = workloads.build(s)
+        m.set_tuning(green=v)
+        for _ in range(3):
+            m.step(info=False)
+        out[v] = m.download()
+        m.close()
+    for k in ("u1", "u2", "u3"):
+        assert rel_l2(out[green][k][..., :64], out[0][k][..., :64]) < 1e-5, k
+    for k in ("sig", "tau"):
+        assert rel_l2(out[green][k], out[0][k]) < 1e-5, k
