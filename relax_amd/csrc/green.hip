@@ -307,7 +307,7 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
   static_assert(Q == 2 || Q == 4 || Q == 8, "H3 = 32, 64 or 128");
   constexpr int NA = 3 * Q * TX;  // step-A threads
   constexpr int NB = 16 * TX;     // step-B threads per half
-  constexpr int RS = 16 * TX;     // float2 per (c, j) group of the transpose buffer
+  constexpr int RS = 16 * TX + 16;  // float2 per (c, j) group (padded: j groups on alternate banks)
   constexpr int NW = NT / 64;
   float2* W = W_m.at(blockIdx.y);
   float2* NQ = NQ_m.at(blockIdx.y);
