@@ -260,6 +260,14 @@ int relax_model_profile(relax_model_t* m, int nsteps, double* ms, int* launches)
  * at each output step for observation depth 0 (relax.f90:955,
  * export.f90:917). */
 int relax_model_surface(relax_model_t* m, float* u1, float* u2, float* u3, int align_data);
+/* Observation points (exportpoints, util.f90:59-139): u and sig at the
+ * shiftedindex (elastic3d.f90:4063-4101) of each point of x (n x 3, x1 x2 x3
+ * in the computational frame, over the half grid as relax.f90:894 passes
+ * it).  u (n x 3) is turned into the user frame by rotation(v1, v2, -rot),
+ * rot in degrees (input.f90:1809-1822); sig (n, may be NULL) is returned as
+ * it is.  RELAX_ERR_RANGE for a point outside the grid, where the reference
+ * stops. */
+int relax_model_points(relax_model_t* m, const double* x, int n, double rot, double* u, relax_tensor_t* sig);
 /* Autotuner: times the launch-shape variants of the tunable kernels (columns
  * per workgroup of the y transform, columns/threads/radix of the Green
  * column pass, divergence tile) on this model's scratch buffers and keeps the

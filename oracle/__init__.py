@@ -191,6 +191,25 @@ def monitorfriction(x, y, z, L, W, strike, dip, rake, beta, sx3_full, dx1, dx2, 
         _d(dx1), _d(dx2), _d(dx3), _p(sig), _p(structure), _p(patch), px2, px3), "monitorfriction")
 
 
+def shiftedindex(x1, x2, x3, sx1, sx2, sx3, dx1, dx2, dx3):
+    """elastic3d.f90:4063-4101, 0-based (i1, i2, i3); None out of range."""
+    idx = np.zeros(3, np.int32)
+    rc = lib().orc_shiftedindex(_d(x1), _d(x2), _d(x3), sx1, sx2, sx3, _d(dx1), _d(dx2), _d(dx3), _p(idx))
+    return None if rc else tuple(int(v) for v in idx)
+
+
+def exportpoints(u1, u2, u3, sig, dx1, dx2, dx3, pts, rot=0.0):
+    """util.f90:59-139: (n, 9) array of v1 v2 v3 (user frame) and sig at the
+    observation points pts (n, 3) of the internal frame."""
+    sx3, sx2, sx1, _ = sig.shape
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    out = np.zeros((len(pts), 9), np.float64)
+    _chk(getattr(lib(), "orc_exportpoints_" + _sfx(sig))(
+        _p(u1), _p(u2), _p(u3), _p(sig), sx1, sx2, sx3, _d(dx1), _d(dx2), _d(dx3), _p(pts), len(pts), _d(rot),
+        _p(out)), "exportpoints")
+    return out
+
+
 def frictionadd(patch, dt):
     lib().orc_frictionadd(_p(patch), ctypes.c_long(patch.size), _d(dt))
 

@@ -794,6 +794,47 @@ static int frictioneigenstress(double x, double y, double z, double L, double W,
 }
 
 // ---------------------------------------------------------------------------
+// shiftedindex, elastic3d.f90:4063-4101 (0-based indices; status 15 where the
+// reference stops on a coordinate out of range)
+// ---------------------------------------------------------------------------
+static int shiftedindex(double x1, double x2, double x3, int sx1, int sx2, int sx3, double dx1, double dx2,
+                        double dx3, int* i1, int* i2, int* i3) {
+  if (x1 > (double)(sx1 / 2 - 1) * dx1 || x1 < -(double)(sx1 / 2) * dx1) return 15;
+  if (x2 > (double)(sx2 / 2 - 1) * dx2 || x2 < -(double)(sx2 / 2) * dx2) return 15;
+  if (x3 > (double)(sx3 - 1) * dx3 || x3 < 0.0) return 15;
+  *i1 = (sx1 + fix(x1 / dx1)) % sx1;
+  *i2 = (sx2 + fix(x2 / dx2)) % sx2;
+  *i3 = MUT(17) ? (int)std::lround(x3 / dx3) : fix(x3 / dx3);
+  return 0;
+}
+
+// exportpoints, util.f90:59-139 (the values it writes): u and sig at the
+// shiftedindex of each observation point, u turned into the user frame by
+// rotation(v1, v2, -rot) (input.f90:1809-1822).  out: n x 9 (v1 v2 v3 s11
+// s12 s13 s22 s23 s33).
+template <class R>
+static int exportpoints(const R* u1, const R* u2, const R* u3, const R* sig, int sx1, int sx2, int sx3, double dx1,
+                        double dx2, double dx3, const double* pts, int n, double rot, double* out) {
+  const long pitch = sx1 + 2;
+  const T6<R>* S = reinterpret_cast<const T6<R>*>(sig);
+  for (int k = 0; k < n; ++k) {
+    int i1, i2, i3;
+    if (int rc = shiftedindex(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2], sx1, sx2, sx3, dx1, dx2, dx3, &i1, &i2,
+                              &i3))
+      return rc;
+    const long o = i1 + pitch * (i2 + (long)sx2 * i3);
+    double v1 = (double)u1[o], v2 = (double)u2[o];
+    const double a = -rot * DEG2RAD, c = std::cos(a), s_ = std::sin(a);
+    const double w1 = +v1 * c + v2 * s_, w2 = -v1 * s_ + v2 * c;
+    const T6<R>& t = S[i1 + (long)sx1 * (i2 + (long)sx2 * i3)];
+    double* r = out + 9 * k;
+    r[0] = w1; r[1] = w2; r[2] = (double)u3[o];
+    r[3] = t.s11; r[4] = t.s12; r[5] = t.s13; r[6] = t.s22; r[7] = t.s23; r[8] = t.s33;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // monitorfriction, friction3d.f90:263-416; shiftedindex elastic3d.f90:4063-4101
 // ---------------------------------------------------------------------------
 template <class R>
@@ -1186,6 +1227,11 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
                                                   R* sig) {                                                          \
     momentdensityshear(mu, slip, x, y, z, L, W, strike, dip, rake, beta, sx1, sx2, sx3, dx1, dx2, dx3, sig);         \
   }                                                                                                                  \
+  ORC_EXPORT int orc_exportpoints_##SUFFIX(const R* u1, const R* u2, const R* u3, const R* sig, int sx1, int sx2,  \
+                                           int sx3, double dx1, double dx2, double dx3, const double* pts, int n,   \
+                                           double rot, double* out) {                                               \
+    return exportpoints(u1, u2, u3, sig, sx1, sx2, sx3, dx1, dx2, dx3, pts, n, rot, out);                           \
+  }                                                                                                                  \
   ORC_EXPORT int orc_rk2_run_##SUFFIX(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* gamma,           \
                                       R* epsilonik, double* t, int* oi, int* it, int nsteps,                        \
                                       const double* dt_replay, double* info) {                                       \
@@ -1209,3 +1255,7 @@ ORC_EXPORT void orc_set_mutation(int k) { g_mut = k; }
 ORC_EXPORT int orc_sizeof_patch(void) { return (int)sizeof(orc_patch); }
 ORC_EXPORT int orc_sizeof_model(void) { return (int)sizeof(orc_model); }
 ORC_EXPORT int orc_sizeof_weak(void) { return (int)sizeof(orc_weak); }
+ORC_EXPORT int orc_shiftedindex(double x1, double x2, double x3, int sx1, int sx2, int sx3, double dx1, double dx2,
+                                double dx3, int* idx) {
+  return shiftedindex(x1, x2, x3, sx1, sx2, sx3, dx1, dx2, dx3, idx, idx + 1, idx + 2);
+}

@@ -9,6 +9,7 @@
 
 #define RELAX_PI 3.141592653589793115997963468544185161
 #define RELAX_PI2 6.28318530717958623199592693708837032318
+#define RELAX_DEG2RAD 0.01745329251994329547437168059786927  // DEG2RAD of input.f90:28
 
 // Return a status from a HIP call.
 #define HIP_TRY(expr)                                     \

@@ -35,6 +35,8 @@ struct relax_model {
   float* Em = nullptr;      // 6 planes of H: its mid-step value
   float* dg[4] = {nullptr, nullptr, nullptr, nullptr};  // weak-zone fields (H each)
   WeakDev* zones_dev = nullptr;  // staging of set_weakzones' zones
+  void* pts_dev = nullptr;        // staging of relax_model_points (grow-only)
+  size_t pts_cap = 0;
   int zones_cap = 0;
   float* tables = nullptr;  // per-depth rheology tables (TABLE_ROWS * sx3/2 floats)
   double* ftab = nullptr;   // fault-creep tables: v0, (a-b)sigma, friction, cohesion

@@ -133,9 +133,13 @@ __global__ __launch_bounds__(256) void k_end_step(const ModelDev* __restrict__ m
   }
 }
 
-// fix, elastic3d.f90:70-83
+// fix, elastic3d.f90:70-83 (rounds half down)
 __device__ __forceinline__ int d_fix(double number) {
   const double f = floor(number), c = ceil(number);
+  return (number - f > 0.5) ? (int)c : (int)f;
+}
+static int host_fix(double number) {
+  const double f = std::floor(number), c = std::ceil(number);
   return (number - f > 0.5) ? (int)c : (int)f;
 }
 
@@ -253,7 +257,7 @@ static void free_model(relax_model_t* m) {
   if (!m) return;
   void* ptrs[] = {m->u, m->sig, m->tau, m->gam, m->W, m->NQ, m->T, m->TN, m->sc, m->partials, m->tables, m->ftab,
                   m->planes_dev, m->patches_dev, m->md_dev, m->shear_dev, m->shear_slip, m->E, m->Em,
-                  m->dg[0], m->dg[1], m->dg[2], m->dg[3], m->zones_dev};
+                  m->dg[0], m->dg[1], m->dg[2], m->dg[3], m->zones_dev, m->pts_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->host_sc) (void)hipHostFree(m->host_sc);
@@ -833,6 +837,69 @@ int relax_model_surface(relax_model_t* m, float* u1, float* u2, float* u3, int a
     HIP_TRY(hipMemcpy2DAsync(dst[c], hp, m->u + c * g.H, dp, dp, (size_t)g.n2, hipMemcpyDeviceToHost, m->stream));
   }
   HIP_TRY(hipStreamSynchronize(m->stream));
+  return RELAX_OK;
+}
+
+// Gather of u (3 planes) and sig (6 planes) at nq point offsets: out holds
+// 9 floats per point.
+__global__ void k_gather_points(const float* __restrict__ u, const float* __restrict__ sig, long H,
+                                const long* __restrict__ off, int nq, float* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nq) return;
+  const long o = off[k];
+  float* r = out + 9 * k;
+  for (int c = 0; c < 3; ++c) r[c] = u[c * H + o];
+  for (int c = 0; c < 6; ++c) r[3 + c] = sig[c * H + o];
+}
+
+int relax_model_points(relax_model_t* m, const double* x, int n, double rot, double* u, relax_tensor_t* sig) {
+  if (!m || n < 0 || (n > 0 && (!x || !u))) return RELAX_ERR_ARG;
+  if (n == 0) return RELAX_OK;
+  if (int rc = wait_batch(m)) return rc;
+  const Grid& g = m->g;
+  const double dx1 = m->desc.dx1, dx2 = m->desc.dx2, dx3 = m->desc.dx3;
+  // shiftedindex (elastic3d.f90:4063-4101) over the half grid, as
+  // relax.f90:894 passes sx3/2; out of range is an error, not a STOP
+  std::vector<long> off(n);
+  for (int k = 0; k < n; ++k) {
+    const double x1 = x[3 * k], x2 = x[3 * k + 1], x3 = x[3 * k + 2];
+    if (x1 > (double)(g.n1 / 2 - 1) * dx1 || x1 < -(double)(g.n1 / 2) * dx1 ||
+        x2 > (double)(g.n2 / 2 - 1) * dx2 || x2 < -(double)(g.n2 / 2) * dx2 || x3 > (double)(g.h3 - 1) * dx3 ||
+        x3 < 0.0)
+      return RELAX_ERR_RANGE;
+    const int i1 = (g.n1 + host_fix(x1 / dx1)) % g.n1, i2 = (g.n2 + host_fix(x2 / dx2)) % g.n2;
+    const int i3 = host_fix(x3 / dx3);
+    off[k] = i1 + (long)g.n1 * (i2 + (long)g.n2 * i3);
+  }
+  const size_t need = (size_t)n * (sizeof(long) + 9 * sizeof(float));
+  if (need > m->pts_cap) {
+    if (m->pts_dev) HIP_TRY(hipFree(m->pts_dev));
+    m->pts_dev = nullptr;
+    m->pts_cap = 0;
+    if (hipMalloc(&m->pts_dev, need) != hipSuccess) return RELAX_ERR_NOMEM;
+    m->pts_cap = need;
+  }
+  long* doff = static_cast<long*>(m->pts_dev);
+  float* dout = reinterpret_cast<float*>(doff + n);
+  std::vector<float> h(9L * n);
+  HIP_TRY(hipMemcpyAsync(doff, off.data(), n * sizeof(long), hipMemcpyHostToDevice, m->stream));
+  k_gather_points<<<(n + 255) / 256, 256, 0, m->stream>>>(m->u, m->sig, g.H, doff, n, dout);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(h.data(), dout, 9L * n * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  // exportpoints writes u in the user frame: rotation(v1, v2, -rot)
+  // (util.f90:113-116, input.f90:1809-1822); sig as it is
+  const double a = -rot * RELAX_DEG2RAD, c = std::cos(a), sn = std::sin(a);
+  for (int k = 0; k < n; ++k) {
+    const double v1 = h[9 * k], v2 = h[9 * k + 1];
+    u[3 * k] = +v1 * c + v2 * sn;
+    u[3 * k + 1] = -v1 * sn + v2 * c;
+    u[3 * k + 2] = h[9 * k + 2];
+    if (sig) {
+      const float* t = &h[9 * k + 3];
+      sig[k] = relax_tensor_t{t[0], t[1], t[2], t[3], t[4], t[5]};
+    }
+  }
   return RELAX_OK;
 }
 

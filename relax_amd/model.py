@@ -179,6 +179,16 @@ class Model:
         c = np.array([fft_cols, green, bodyforce], np.int32)
         check(self._lib.relax_model_set_tuning(self._h, ptr(c)), "set_tuning")
 
+    def points(self, xyz, rot=0.0):
+        """u and sig at observation points (exportpoints, util.f90:59-139):
+        xyz (n, 3) in the computational frame; returns u (n, 3) float64 in
+        the user frame (rotation by -rot degrees) and sig (n, 6) float32."""
+        x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        u = np.zeros((len(x), 3), np.float64)
+        sig = np.zeros((len(x), 6), np.float32)
+        check(self._lib.relax_model_points(self._h, ptr(x), len(x), float(rot), ptr(u), ptr(sig)), "points")
+        return u, sig
+
     def surface(self, u1, u2, u3):
         """Copy the surface plane (x3 = 0) of u1..u3 into (sx2, sx1) float32
         arrays."""

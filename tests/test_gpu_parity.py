@@ -453,3 +453,27 @@ def test_model_rejects_bad_input():
         with pytest.raises(RelaxError) as e:
             g.set_structure(1, layers(32, n=0.5))
         assert e.value.status == 1
+
+
+def test_observation_points_match_oracle():
+    """relax_model_points samples u and sig where exportpoints does, and
+    turns u into the user frame; a point outside the grid is an error."""
+    s = small_spec("powerlaw")
+    g = workloads.build(s)
+    g.run(2)
+    out = g.download()
+    rng = np.random.default_rng(31)
+    dx, sx, h3 = s["dx"], s["sx1"], s["sx3"] // 2
+    pts = np.column_stack([rng.uniform(-sx / 2 * dx, (sx / 2 - 1) * dx, 40),
+                           rng.uniform(-sx / 2 * dx, (sx / 2 - 1) * dx, 40),
+                           rng.uniform(0, (h3 - 1) * dx, 40)])
+    pts[0] = (0.25 * dx, -0.5 * dx, 1.5 * dx)  # ties in x2 and x3
+    for rot in (0.0, 30.0):
+        u, sig = g.points(pts, rot)
+        ref = O.exportpoints(out["u1"], out["u2"], out["u3"], out["sig"], dx, dx, dx, pts, rot)
+        np.testing.assert_array_equal(sig, ref[:, 3:].astype(np.float32))
+        np.testing.assert_allclose(u, ref[:, :3], rtol=1e-15, atol=0)
+    with pytest.raises(RelaxError) as e:
+        g.points([[0.0, 0.0, h3 * dx]])
+    assert e.value.status == 15
+    g.close()

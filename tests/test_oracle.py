@@ -32,6 +32,9 @@ Parity status per function:
   frictioneigenstress ....... pinned (invariants, delta normalisation with image)
   monitorfriction/frictionadd pinned (invariants, closed form)
   tensoramplitude ........... pinned (closed form)
+  shiftedindex, exportpoints  pinned (grid points map to themselves, ties
+                              round to the lower index, range; the user
+                              frame at 90 degrees swaps the components)
   RK2 loop .................. pinned (uniform-stress closed-form decay,
                               Burgers transient midpoint recurrence, weak
                               zones equal to the same fluidity in layers)
@@ -954,3 +957,50 @@ def test_rk2_weak_zone_equals_layered_fluidity():
     got = O.rk2_run(m2, s2, 3)
     np.testing.assert_array_equal(got, ref)
     np.testing.assert_array_equal(s2["sig"], s["sig"])
+
+
+# ------------------------------------------------ observation points ---
+@pytest.mark.parametrize("mut", [0, 17])
+def test_shiftedindex_nearest_point_ties_down(mut, mutation):
+    """Coordinates on grid points map to their index; between two points to
+    the nearer one, ties to the lower index (fix, elastic3d.f90:70-83), with
+    the x1/x2 wrap of the shifted coordinates; out of range is refused."""
+    sx1, sx2, sx3, dx = 32, 16, 8, 0.5
+    mutation(mut)
+    got = []
+    for k3 in range(sx3 - 1):
+        for f in (0.0, 0.25, 0.5, 0.75):
+            got.append((O.shiftedindex(0.0, 0.0, (k3 + f) * dx, sx1, sx2, sx3, dx, dx, dx)[2],
+                        int(np.ceil(k3 + f - 0.5))))
+    mutation(0)
+    bad = [(a, b) for a, b in got if a != b]
+    if mut:
+        assert bad
+        return
+    assert not bad
+    for i1 in range(sx1):
+        x1 = (i1 if i1 < sx1 // 2 else i1 - sx1) * dx
+        assert O.shiftedindex(x1, 0.0, 0.0, sx1, sx2, sx3, dx, dx, dx)[0] == i1
+    assert O.shiftedindex(-0.25 * dx, 0.0, 0.0, sx1, sx2, sx3, dx, dx, dx)[0] == 0
+    assert O.shiftedindex(-0.75 * dx, 0.0, 0.0, sx1, sx2, sx3, dx, dx, dx)[0] == sx1 - 1
+    assert O.shiftedindex(sx1 // 2 * dx, 0.0, 0.0, sx1, sx2, sx3, dx, dx, dx) is None
+    assert O.shiftedindex(0.0, 0.0, -0.1, sx1, sx2, sx3, dx, dx, dx) is None
+
+
+def test_exportpoints_user_frame():
+    """At rot = 90 degrees the user frame is (v1, v2) = (-u2, u1); at 0 it is
+    the computational frame; sig is sampled as it is."""
+    rng = np.random.default_rng(4)
+    h3, sx = 4, 8
+    u = [rng.standard_normal((h3, sx, sx + 2)) for _ in range(3)]
+    sig = rng.standard_normal((h3, sx, sx, 6))
+    pts = np.array([[0.5, -1.0, 0.5], [1.5, 1.0, 1.0]])
+    a = O.exportpoints(*u, sig, 0.5, 0.5, 0.5, pts, 0.0)
+    b = O.exportpoints(*u, sig, 0.5, 0.5, 0.5, pts, 90.0)
+    i = [O.shiftedindex(*p, sx, sx, h3, 0.5, 0.5, 0.5) for p in pts]
+    for k, (i1, i2, i3) in enumerate(i):
+        np.testing.assert_array_equal(a[k, :3], [u[0][i3, i2, i1], u[1][i3, i2, i1], u[2][i3, i2, i1]])
+        np.testing.assert_array_equal(a[k, 3:], sig[i3, i2, i1])
+    np.testing.assert_allclose(b[:, 0], -a[:, 1], atol=1e-15)
+    np.testing.assert_allclose(b[:, 1], a[:, 0], atol=1e-15)
+    np.testing.assert_array_equal(b[:, 2:], a[:, 2:])
