@@ -176,6 +176,8 @@ typedef struct {
   double t;           /* time after the step */
   int oi;             /* output counter after the step */
   int status;         /* RELAX_OK or the error of the step */
+  int e;              /* index of the last event reached (0: the one at t = 0); it
+                         advances when t reaches the next event time (relax.f90:858-863) */
 } relax_step_info_t;
 
 /* Bytes of HBM one model of this size keeps resident (for admission). */
@@ -228,8 +230,10 @@ int relax_model_download(relax_model_t* m, float* u1, float* u2, float* u3, rela
  * info may be NULL.  The step is asynchronous on the model's stream unless
  * info is requested. */
 int relax_model_step(relax_model_t* m, const double* dt_override, relax_step_info_t* info);
-/* Run up to nsteps steps or until t >= interval; info holds nsteps entries
- * or is NULL.  *done receives the number of steps taken. */
+/* Run up to nsteps steps, until t >= interval, or through the step that
+ * reaches the next event time (info.e advances: the caller applies the
+ * event with relax_model_event and runs on); info holds nsteps entries or
+ * is NULL.  *done receives the number of steps taken. */
 int relax_model_run(relax_model_t* m, int nsteps, const double* dt_override, relax_step_info_t* info, int* done);
 int relax_model_synchronize(relax_model_t* m);
 /* Coseismic set-up on the device.  add_shear adds momentdensityshear
@@ -241,6 +245,11 @@ int relax_model_synchronize(relax_model_t* m);
  * place of the analytic source() body force). */
 int relax_model_add_shear(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n);
 int relax_model_coseismic(relax_model_t* m);
+/* A later coseismic event (the event switch of relax.f90:858-889), once a
+ * step has reached its time (info.e): tau += dtau, the momentdensityshear of
+ * the event's patches; u += greenfunctioncowling(equivalentbodyforce(dtau));
+ * sig = -tau + C:eps(u).  Not on a member of a batch. */
+int relax_model_event(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n);
 /* Phase timers (the HIP-event counterpart of the reference's PAPI phase
  * timers, papi_prof.c): runs nsteps steps with an event after every launch
  * and returns, per kernel kind, the summed milliseconds and the number of

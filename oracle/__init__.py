@@ -43,7 +43,8 @@ class OrcModel(ctypes.Structure):
                 ("interval", ctypes.c_double), ("odt", ctypes.c_double), ("tscale", ctypes.c_double),
                 ("skip", ctypes.c_int), ("ne", ctypes.c_int), ("event_times", ctypes.c_void_p),
                 ("ltransient", ctypes.c_void_p), ("nltransient", ctypes.c_void_p),
-                ("dg0", ctypes.c_void_p * 4)]
+                ("dg0", ctypes.c_void_p * 4),
+                ("ev_planes", ctypes.c_void_p), ("ev_slip", ctypes.c_void_p), ("ev_off", ctypes.c_void_p)]
 
 
 def lib():
@@ -224,7 +225,10 @@ def rk2_run(model, state, nsteps, dt_replay=None):
     tscale, skip, event_times; optional ltransient/nltransient (LAYER arrays)
     with state["epsilonik"] (tensor field), and weak-zone fields dg_linear,
     dg_nonlinear, dg_ltransient, dg_nltransient ((sx3/2, sx2, sx1) arrays of
-    the state's dtype).  Returns an (nsteps, 10) array of
+    the state's dtype), and the later events' shear patches: event_patches, a
+    list (one entry per event after the first) of (PLANE array, slips).
+    state["e"] is the current event (1-based, default 1).  Returns an
+    (nsteps, 10) array of
     Dt, tm, maxwell(1:3), power, C:E^i, t, maxwell(4:5).
     """
     keep = []
@@ -244,6 +248,15 @@ def rk2_run(model, state, nsteps, dt_replay=None):
                  ptr(model.get("planes")), None if model.get("patches") is None else model["patches"].ctypes.data,
                  model["interval"], model.get("odt", 1.0), model.get("tscale", 1.0), model.get("skip", 1),
                  len(ev), ptr(ev), ptr(model.get("ltransient")), ptr(model.get("nltransient")))
+    evp = model.get("event_patches")
+    if evp:
+        planes = [np.zeros(0, PLANE)] + [np.asarray(p, PLANE) for p, _ in evp]
+        slips = [np.zeros(0)] + [np.asarray(sl, np.float64) for _, sl in evp]
+        off = np.zeros(len(planes) + 1, np.int32)
+        off[1:] = np.cumsum([len(p) for p in planes])
+        m.ev_planes = ptr(np.concatenate(planes).astype(PLANE))
+        m.ev_slip = ptr(np.concatenate(slips))
+        m.ev_off = ptr(off)
     for k, key in enumerate(("dg_linear", "dg_nonlinear", "dg_ltransient", "dg_nltransient")):
         a = model.get(key)
         if a is not None:
@@ -253,12 +266,13 @@ def rk2_run(model, state, nsteps, dt_replay=None):
     t = ctypes.c_double(state.get("t", 0.0))
     oi = ctypes.c_int(state.get("oi", 1))
     it = ctypes.c_int(state.get("i", 0))
+    ev = ctypes.c_int(state.get("e", 1))
     info = np.zeros((nsteps, 10), np.float64)
     dtr = None if dt_replay is None else np.ascontiguousarray(dt_replay, np.float64)
     rc = getattr(lib(), "orc_rk2_run_" + sfx)(
         ctypes.byref(m), _p(state["u1"]), _p(state["u2"]), _p(state["u3"]), _p(state["tau"]), _p(state["sig"]),
-        _p(state["gamma"]), _p(state.get("epsilonik")), ctypes.byref(t), ctypes.byref(oi), ctypes.byref(it), nsteps, _p(dtr), _p(info))
+        _p(state["gamma"]), _p(state.get("epsilonik")), ctypes.byref(t), ctypes.byref(oi), ctypes.byref(it), ctypes.byref(ev), nsteps, _p(dtr), _p(info))
     _chk(rc, "rk2_run")
     done = it.value - state.get("i", 0)
-    state["t"], state["oi"], state["i"] = t.value, oi.value, it.value
+    state["t"], state["oi"], state["i"], state["e"] = t.value, oi.value, it.value, ev.value
     return info[:done]

@@ -961,9 +961,10 @@ static void momentdensityshear(double mu, double slip, double x, double y, doubl
 // RK2 loop: relax.f90:555-890 with integrationstep :1420-1452 and the
 // isoutput counter :1364-1376 (oi advances whenever isoutput holds).
 // Mechanisms: linear and power-law viscosity (with weak zones), fault creep,
-// and linear and power-law transient creep (with weak zones); no
-// interseismic loading, no time-dependent surface loads and no coseismic
-// events after t=0 (event times only truncate Dt).
+// and linear and power-law transient creep (with weak zones); later events'
+// shear patches through the event switch (:858-889, the body force of their
+// moment density by the FIR divergence as in the set-up); no interseismic
+// loading and no time-dependent surface loads.
 // ---------------------------------------------------------------------------
 extern "C" {
 struct orc_model {
@@ -988,6 +989,11 @@ struct orc_model {
   // weak zones (builddgammadot0 fields, sx1*sx2*sx3/2 values of the storage
   // type, or NULL): linear, power-law, linear transient, power-law transient
   const void* dg0[4];
+  // shear patches of the later events: event k (1..ne-1) owns
+  // ev_planes[ev_off[k] .. ev_off[k+1]) with slips ev_slip (NULL: none)
+  const orc_plane* ev_planes;
+  const double* ev_slip;
+  const int* ev_off;
 };
 }
 
@@ -995,7 +1001,7 @@ static const double STEP_MAX = 1e7;
 
 template <class R>
 static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* gamma, R* epsilonik, double* t_io,
-                   int* oi_io, int* i_io, int nsteps, const double* dt_replay, double* info) {
+                   int* oi_io, int* i_io, int* e_io, int nsteps, const double* dt_replay, double* info) {
   const int sx1 = m->sx1, sx2 = m->sx2, sx3 = m->sx3, h3 = sx3 / 2;
   const long pitch = sx1 + 2;
   const long nv = pitch * sx2 * sx3, nu = pitch * sx2 * h3, nt = pitch * sx2, npts = (long)sx1 * sx2 * h3;
@@ -1008,7 +1014,7 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
   const orc_layer* trans[2] = {m->ltransient, m->nltransient};
   double t = *t_io;
   int oi = *oi_io, it = *i_io;
-  const int e = 1;  // single event at t=0
+  int e = *e_io;  // current event, 1-based as in relax.f90 (1: the set-up at t = 0)
   for (int step = 0; step < nsteps; ++step) {
     if (t >= m->interval) break;
     ++it;
@@ -1141,10 +1147,38 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
       }
     }
     t = t + Dt;
+    // the event switch (relax.f90:858-886)
+    if (e < m->ne && std::fabs(t - m->event_times[e]) < 1e-6) {
+      e = e + 1;
+      if (m->ev_off && m->ev_off[e] > m->ev_off[e - 1]) {
+        std::vector<R> dtau(6 * npts, (R)0);
+        for (int k = m->ev_off[e - 1]; k < m->ev_off[e]; ++k) {
+          const orc_plane& P = m->ev_planes[k];
+          momentdensityshear(m->mu, m->ev_slip[k], P.x, P.y, P.z, P.width, P.length, P.strike, P.dip, P.rake,
+                             m->beta, sx1, sx2, h3, m->dx1, m->dx2, m->dx3, dtau.data());
+        }
+        std::fill(v1.begin(), v1.end(), (R)0); std::fill(v2.begin(), v2.end(), (R)0);
+        std::fill(v3.begin(), v3.end(), (R)0); std::fill(t1.begin(), t1.end(), (R)0);
+        std::fill(t2.begin(), t2.end(), (R)0); std::fill(t3.begin(), t3.end(), (R)0);
+        rc = equivalentbodyforce(dtau.data(), m->dx1, m->dx2, m->dx3, sx1, sx2, h3, v1.data(), v2.data(),
+                                 v3.data(), t1.data(), t2.data(), t3.data(), pitch, (const R*)0);
+        if (rc) return rc;
+        rc = greenfunctioncowling(v1.data(), v2.data(), v3.data(), t1.data(), t2.data(), t3.data(), sx1, sx2, sx3,
+                                  m->dx1, m->dx2, m->dx3, m->lambda, m->mu, m->gam);
+        if (rc) return rc;
+        for (long o = 0; o < nu; ++o) {
+          u1[o] = u1[o] + v1[o];
+          u2[o] = u2[o] + v2[o];
+          u3[o] = u3[o] + v3[o];
+        }
+        if (!MUT(18))
+          for (long o = 0; o < 6 * npts; ++o) tau[o] = tau[o] + dtau[o];
+      }
+    }
     for (long o = 0; o < 6 * npts; ++o) sig[o] = (R)(-1) * tau[o];
     stressupdate(u1, u2, u3, m->lambda, m->mu, m->dx1, m->dx2, m->dx3, sx1, sx2, h3, sig, pitch);
     // isoutput, relax.f90:1364-1376, advances the output counter
-    double etime = m->ne > 0 ? m->event_times[e - 1] : 0.0;
+    const double etime = m->ne > 0 ? m->event_times[e - 1] : 0.0;
     bool out = ((0 == m->skip) && (std::fabs(t - oi * m->odt) < 1e-6 * m->odt)) ||
                ((0 < m->skip) && ((it - 1) % m->skip == 0)) || (std::fabs(t - etime) < 1e-6);
     if (info) {
@@ -1156,7 +1190,7 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
     }
     if (out) ++oi;
   }
-  *t_io = t; *oi_io = oi; *i_io = it;
+  *t_io = t; *oi_io = oi; *i_io = it; *e_io = e;
   return 0;
 }
 
@@ -1233,9 +1267,9 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
     return exportpoints(u1, u2, u3, sig, sx1, sx2, sx3, dx1, dx2, dx3, pts, n, rot, out);                           \
   }                                                                                                                  \
   ORC_EXPORT int orc_rk2_run_##SUFFIX(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* gamma,           \
-                                      R* epsilonik, double* t, int* oi, int* it, int nsteps,                        \
+                                      R* epsilonik, double* t, int* oi, int* it, int* e, int nsteps,                \
                                       const double* dt_replay, double* info) {                                       \
-    return rk2_run(m, u1, u2, u3, tau, sig, gamma, epsilonik, t, oi, it, nsteps, dt_replay, info);                   \
+    return rk2_run(m, u1, u2, u3, tau, sig, gamma, epsilonik, t, oi, it, e, nsteps, dt_replay, info);                \
   }                                                                                                                  \
   ORC_EXPORT int orc_transienteigenstress_##SUFFIX(double mu, const orc_layer* st, const R* sig, const R* eik,     \
                                                    int sx1, int sx2, int sx3, R* moment, R* eikdot,                 \

@@ -39,7 +39,8 @@ class ModelDesc(ctypes.Structure):
 
 class StepInfo(ctypes.Structure):
     _fields_ = [("i", c_int), ("Dt", c_double), ("tm", c_double), ("maxwell", c_double * 5),
-                ("power", c_double), ("cei", c_double), ("t", c_double), ("oi", c_int), ("status", c_int)]
+                ("power", c_double), ("cei", c_double), ("t", c_double), ("oi", c_int), ("status", c_int),
+                ("e", c_int)]
 
 
 # name -> (restype, argtypes); the list is the ABI contract checked by the tests
@@ -96,6 +97,7 @@ SIGNATURES = {
     "relax_model_stream": (c_void_p, [P]),
     "relax_model_add_shear": (c_int, [P, P, P, c_int]),
     "relax_model_coseismic": (c_int, [P]),
+    "relax_model_event": (c_int, [P, P, P, c_int]),
     "relax_model_profile": (c_int, [P, c_int, P, P]),
     "relax_model_autotune": (c_int, [P, P, P]),
     "relax_model_set_tuning": (c_int, [P, P]),

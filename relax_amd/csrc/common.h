@@ -114,7 +114,7 @@ struct StepScalars {
   double t, Dt, tm;
   double maxwell[NMECH];
   double power, cei;
-  int oi, it, status, pad;
+  int oi, it, status, e;          // e: index of the last event (0 at t = 0)
   unsigned int tmin_bits[NMECH];  // running minima (positive floats as ordered ints)
   unsigned int pad2[3];
 };

@@ -373,6 +373,7 @@ int launch_fft_x_c2r(int n1, int epi, MPtr<const float2> in, long in_pitch, MPtr
 #define SW(E) switch (n1) { X(16, E) X(32, E) X(64, E) X(128, E) X(256, E) X(512, E) X(1024, E) default: return RELAX_ERR_SIZE; }
   if (epi == EPI_PLAIN) { SW(EPI_PLAIN) }
   else if (epi == EPI_PRED) { SW(EPI_PRED) }
+  else if (epi == EPI_ADD) { SW(EPI_ADD) }
   else { SW(EPI_CORR) }
 #undef SW
 #undef X

@@ -3,7 +3,7 @@
 #include "common.h"
 #include "modeldev.h"
 
-enum { EPI_PLAIN = 0, EPI_PRED = 1, EPI_CORR = 2 };
+enum { EPI_PLAIN = 0, EPI_PRED = 1, EPI_CORR = 2, EPI_ADD = 3 };
 
 const float2* twiddle_table(int L);
 

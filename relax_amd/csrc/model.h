@@ -36,6 +36,7 @@ struct relax_model {
   float* dg[4] = {nullptr, nullptr, nullptr, nullptr};  // weak-zone fields (H each)
   WeakDev* zones_dev = nullptr;  // staging of set_weakzones' zones
   void* pts_dev = nullptr;        // staging of relax_model_points (grow-only)
+  double* ev_dev = nullptr;       // event times on the device
   size_t pts_cap = 0;
   int zones_cap = 0;
   float* tables = nullptr;  // per-depth rheology tables (TABLE_ROWS * sx3/2 floats)

@@ -33,7 +33,7 @@ static int wait_batch(relax_model_t* m);
 
 // ------------------------------------------------------------- kernels ----
 __global__ void k_reset_scalars(StepScalars* sc, double t, int oi, int it) {
-  sc->t = t; sc->Dt = 0; sc->tm = 0;
+  sc->t = t; sc->Dt = 0; sc->tm = 0; sc->e = 0;
   for (int k = 0; k < NMECH; ++k) sc->maxwell[k] = STEP_MAX;
   sc->power = sc->cei = 0;
   sc->oi = oi; sc->it = it; sc->status = 0;
@@ -63,7 +63,9 @@ __global__ void k_finalize_dt(const ModelDev* __restrict__ md, int nm, int use_o
     if ((t + Dt) >= ((double)sc->oi * tl.odt) - Dt * 0.04) Dt = (double)sc->oi * tl.odt - t;
   } else {
     Dt = Dt * tl.tscale;
-    if (tl.has_next && (t + Dt - tl.next_event) >= 0.0) Dt = tl.next_event - t;
+    // reduce the step so that t + Dt is the next event's time
+    const int e = sc->e;
+    if (e + 1 < tl.ne && (t + Dt - tl.ev[e + 1]) >= 0.0) Dt = tl.ev[e + 1] - t;
   }
   if (use_override) Dt = dt_override;
   sc->Dt = Dt;
@@ -126,8 +128,13 @@ __global__ __launch_bounds__(256) void k_end_step(const ModelDev* __restrict__ m
     const double t = sc->t + sc->Dt;
     sc->t = t;
     const int it = sc->it, oi = sc->oi;
+    // the event switch (relax.f90:858-863): the next event is reached; its
+    // slip is applied by relax_model_event
+    int e = sc->e;
+    if (e + 1 < tl.ne && fabs(t - tl.ev[e + 1]) < 1e-6) sc->e = ++e;
+    const double etime = tl.ne > 0 ? tl.ev[e] : 0.0;
     const bool out = ((0 == tl.skip) && (fabs(t - oi * tl.odt) < 1e-6 * tl.odt)) ||
-                     ((0 < tl.skip) && ((it - 1) % tl.skip == 0)) || (fabs(t - tl.etime) < 1e-6);
+                     ((0 < tl.skip) && ((it - 1) % tl.skip == 0)) || (fabs(t - etime) < 1e-6);
     if (out) sc->oi = oi + 1;
     for (int k = 0; k < NMECH; ++k) sc->tmin_bits[k] = 0x7f800000u;
   }
@@ -257,7 +264,7 @@ static void free_model(relax_model_t* m) {
   if (!m) return;
   void* ptrs[] = {m->u, m->sig, m->tau, m->gam, m->W, m->NQ, m->T, m->TN, m->sc, m->partials, m->tables, m->ftab,
                   m->planes_dev, m->patches_dev, m->md_dev, m->shear_dev, m->shear_slip, m->E, m->Em,
-                  m->dg[0], m->dg[1], m->dg[2], m->dg[3], m->zones_dev, m->pts_dev};
+                  m->dg[0], m->dg[1], m->dg[2], m->dg[3], m->zones_dev, m->pts_dev, m->ev_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->host_sc) (void)hipHostFree(m->host_sc);
@@ -293,9 +300,8 @@ static void tune_lookup(relax_model_t* m) {
 static TimeLine timeline(const relax_model_t* m) {
   TimeLine tl;
   tl.interval = m->desc.interval; tl.odt = m->desc.odt; tl.tscale = m->desc.tscale; tl.skip = m->desc.skip;
-  tl.has_next = m->events.size() > 1;
-  tl.next_event = tl.has_next ? m->events[1] : 0.0;
-  tl.etime = m->events.empty() ? 0.0 : m->events[0];
+  tl.ev = m->ev_dev;
+  tl.ne = (int)m->events.size();
   for (int k = 0; k < NMECH; ++k) tl.mech[k] = m->mech[k];
   return tl;
 }
@@ -343,6 +349,14 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
   HIP_TRY(hipGetDevice(&m->device));
   m->desc = *d;
   m->events.assign(d->event_times, d->event_times + (d->event_times ? d->ne : 0));
+  if (!m->events.empty()) {
+    if (hipMalloc((void**)&m->ev_dev, m->events.size() * sizeof(double)) != hipSuccess ||
+        hipMemcpy(m->ev_dev, m->events.data(), m->events.size() * sizeof(double), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+      free_model(m);
+      return RELAX_ERR_NOMEM;
+    }
+  }
   m->desc.event_times = nullptr;
   Grid& g = m->g;
   g.n1 = d->sx1; g.n2 = d->sx2; g.n3 = d->sx3; g.h3 = d->sx3 / 2; g.m1 = d->sx1 / 2;
@@ -1076,6 +1090,7 @@ static void fill_info(const StepScalars& s, relax_step_info_t* info) {
   info->i = s.it; info->Dt = s.Dt; info->tm = s.tm;
   for (int k = 0; k < NMECH; ++k) info->maxwell[k] = s.maxwell[k];
   info->power = s.power; info->cei = s.cei; info->t = s.t; info->oi = s.oi; info->status = s.status;
+  info->e = s.e;
 }
 
 int relax_model_step(relax_model_t* m, const double* dt_override, relax_step_info_t* info) {
@@ -1099,9 +1114,13 @@ int relax_model_run(relax_model_t* m, int nsteps, const double* dt_override, rel
     HIP_TRY(hipStreamSynchronize(m->stream));
     if (m->host_sc->status) { if (done) *done = k; return m->host_sc->status; }
     if (m->host_sc->t >= m->desc.interval) break;
+    const int e0 = m->host_sc->e;
     relax_step_info_t tmp;
-    const int rc = relax_model_step(m, dt_override ? dt_override + k : nullptr, info ? info + k : &tmp);
+    relax_step_info_t* cur = info ? info + k : &tmp;
+    const int rc = relax_model_step(m, dt_override ? dt_override + k : nullptr, cur);
     if (rc) { if (done) *done = k; return rc; }
+    // an event was reached: the caller applies its slip before going on
+    if (cur->e != e0) { ++k; break; }
   }
   if (done) *done = k;
   return RELAX_OK;
@@ -1145,9 +1164,8 @@ __global__ void k_add_shear(const PlaneDev* __restrict__ P, const double* __rest
   }
 }
 
-int relax_model_add_shear(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n) {
-  if (!m || n < 0 || (n > 0 && (!patches || !slip))) return RELAX_ERR_ARG;
-  if (int rc = wait_batch(m)) return rc;
+// momentdensityshear of n patches added to the tensor planes `target`
+static int shear_into(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n, float* target) {
   if (n == 0) return RELAX_OK;
   std::vector<PlaneDev>& hp = m->shear_host;
   hp.resize(n);
@@ -1165,10 +1183,47 @@ int relax_model_add_shear(relax_model_t* m, const relax_plane_t* patches, const 
   HIP_TRY(hipMemcpyAsync(m->shear_dev, hp.data(), sizeof(PlaneDev) * n, hipMemcpyHostToDevice, m->stream));
   HIP_TRY(hipMemcpyAsync(m->shear_slip, slip, sizeof(double) * n, hipMemcpyHostToDevice, m->stream));
   k_add_shear<<<grid_1d(m->g.H), 256, 0, m->stream>>>(m->shear_dev, m->shear_slip, n, 2.0f * (float)m->desc.mu, m->g,
-                                                      m->desc.dx1, m->desc.dx2, m->desc.dx3, m->tau);
-  int rc = hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+                                                      m->desc.dx1, m->desc.dx2, m->desc.dx3, target);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
+int relax_model_add_shear(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n) {
+  if (!m || n < 0 || (n > 0 && (!patches || !slip))) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
+  int rc = shear_into(m, patches, slip, n, m->tau);
   if (hipStreamSynchronize(m->stream) != hipSuccess && !rc) rc = RELAX_ERR_HIP;
   return rc;
+}
+
+// tau += dtau over the 6 planes of H
+__global__ void k_add_planes(float* __restrict__ tau, const float* __restrict__ d, long n) {
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < n; o += (long)gridDim.x * blockDim.x)
+    tau[o] = tau[o] + d[o];
+}
+
+int relax_model_event(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n) {
+  if (!m || n < 0 || (n > 0 && (!patches || !slip))) return RELAX_ERR_ARG;
+  if (int rc = wait_batch(m)) return rc;
+  int rc;
+  if ((rc = sync_md(m))) return rc;
+  const long H = m->g.H;
+  // the event's moment density into sig, which is recomputed at the end
+  HIP_TRY(hipMemsetAsync(m->sig, 0, 6 * H * 4, m->stream));
+  if ((rc = shear_into(m, patches, slip, n, m->sig))) return rc;
+  // its body force and surface traction, u += G[f, t] (relax.f90:868-880,
+  // with the FIR divergence as in the set-up), tau += dtau, then
+  // sig = -tau + C:eps(u) (:887-889)
+  ModelDev tmp = m->md_host;
+  tmp.tau = m->sig;
+  Ctx c = single(m);
+  c.prof = nullptr;
+  if ((rc = launch_bodyforce(m->g, m->S, c.md, &tmp, 1, RHEO_VISCOUS, BODY_RAW, m->stream))) return rc;
+  if ((rc = model_green(c, EPI_ADD))) return rc;
+  k_add_planes<<<grid_1d(6 * H), 256, 0, m->stream>>>(m->tau, m->sig, 6 * H);
+  HIP_TRY(hipGetLastError());
+  if ((rc = launch_stress(STRESS_SET, m->g, m->S, c.md, c.one, 1, RHEO_VISCOUS, m->stream))) return rc;
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  return RELAX_OK;
 }
 
 int relax_model_coseismic(relax_model_t* m) {

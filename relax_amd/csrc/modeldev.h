@@ -14,8 +14,10 @@
 // Time line of one model (relax.f90:659-677 step size, :1364-1376 isoutput,
 // :1420-1452 integrationstep).
 struct TimeLine {
-  double interval, odt, tscale, next_event, etime;
-  int skip, has_next;
+  double interval, odt, tscale;
+  const double* ev;  // device array of the ne event times (event 0 at t = 0)
+  int ne;
+  int skip;
   int mech[NMECH];
 };
 

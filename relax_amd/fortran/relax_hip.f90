@@ -90,7 +90,7 @@ MODULE relax_hip
   TYPE, BIND(C), PUBLIC :: relax_step_info
      INTEGER(c_int) :: i
      REAL(c_double) :: Dt,tm,maxwell(5),power,cei,t
-     INTEGER(c_int) :: oi,status
+     INTEGER(c_int) :: oi,status,e
   END TYPE relax_step_info
   !> a fault plane or slip patch in the internal convention (angles in
   !! radians): PLANE_STRUCT without its patch array
@@ -273,6 +273,11 @@ MODULE relax_hip
      INTEGER(c_int) FUNCTION c_model_add_shear(h,patches,slip,n) BIND(C,name="relax_model_add_shear")
        IMPORT :: c_int, c_ptr
        TYPE(c_ptr), VALUE :: h, patches, slip
+       INTEGER(c_int), VALUE :: n
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_event(h,patches,slip,n) BIND(C,name="relax_model_event")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h,patches,slip
        INTEGER(c_int), VALUE :: n
      END FUNCTION
      INTEGER(c_int) FUNCTION c_model_coseismic(h) BIND(C,name="relax_model_coseismic")
@@ -611,5 +616,15 @@ CONTAINS
          "relax_model_add_shear")
     CALL check(c_model_coseismic(model%h),"relax_model_coseismic")
   END SUBROUTINE relax_hip_model_coseismic
+
+  !> a later event, once a step has reached its time (info%e advanced): the
+  !! event switch of relax.f90:858-889
+  SUBROUTINE relax_hip_model_event(model,patches,slip)
+    TYPE(relax_model), INTENT(IN) :: model
+    TYPE(relax_plane_c), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: patches
+    REAL*8, DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: slip
+    CALL check(c_model_event(model%h,C_LOC(patches),C_LOC(slip),INT(SIZE(patches),c_int)), &
+         "relax_model_event")
+  END SUBROUTINE relax_hip_model_event
 
 END MODULE relax_hip

@@ -147,6 +147,14 @@ class Model:
         """u = Green(body force of tau), sig = -tau + C:eps(u)."""
         check(self._lib.relax_model_coseismic(self._h), "coseismic")
 
+    def event(self, patches, slip):
+        """A later coseismic event, once a step has reached its time (the
+        step's "e" advanced): tau += dtau, u += Green(body force of dtau),
+        sig = -tau + C:eps(u)."""
+        patches = np.ascontiguousarray(patches, PLANE)
+        slip = np.ascontiguousarray(slip, np.float64)
+        check(self._lib.relax_model_event(self._h, ptr(patches), ptr(slip), len(patches)), "event")
+
     def profile(self, nsteps):
         """Run nsteps steps with HIP-event phase timers; returns
         {kind: (total_ms, launches)} for the kernel kinds of PROFILE_KINDS."""
@@ -274,4 +282,4 @@ PROFILE_KINDS = ("bodyforce", "fft_x_fwd", "fft_y", "green_z", "fft_x_inv", "str
 
 def _info(r):
     return dict(i=r.i, Dt=r.Dt, tm=r.tm, maxwell=list(r.maxwell), power=r.power, cei=r.cei, t=r.t, oi=r.oi,
-                status=r.status)
+                status=r.status, e=r.e)

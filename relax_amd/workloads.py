@@ -125,12 +125,36 @@ def planes(s):
     return np.concatenate(ps)
 
 
+def event_times(s):
+    """The spec's event times: the set-up at t = 0, then its later events
+    (s["events"]: a list of (time, deck))."""
+    return (0.0,) + tuple(float(t) for t, _ in s.get("events", ()))
+
+
 def build(s):
     """Create the model of a spec on the GPU with its coseismic state."""
     dx = s["dx"]
     m = Model(s["sx1"], s["sx2"], s["sx3"], dx, dx, dx, LAM, MU, GAM, BETA, interval=s["interval"], odt=s["odt"],
-              tscale=s["tscale"], skip=s["skip"])
+              tscale=s["tscale"], skip=s["skip"], event_times=event_times(s))
     return configure(m, s)
+
+
+def run(m, s, nsteps, dt_replay=None):
+    """Up to nsteps RK2 steps of a model of spec s, applying each later event
+    of the spec when a step reaches its time (the driver's side of the event
+    switch, relax.f90:858-886).  Returns the step records."""
+    recs = []
+    while len(recs) < nsteps:
+        k = len(recs)
+        got = m.run(nsteps - k, dt_replay=None if dt_replay is None else dt_replay[k:])
+        if not got:
+            break
+        e0 = recs[-1]["e"] if recs else 0
+        recs.extend(got)
+        if got[-1]["e"] != e0:
+            patches, slips = structure.patches(s["events"][got[-1]["e"] - 1][1])
+            m.event(patches, slips)
+    return recs
 
 
 def configure(m, s):

@@ -61,8 +61,13 @@ def oracle_model(s):
     m = dict(sx1=s["sx1"], sx2=s["sx2"], sx3=s["sx3"], dx1=s["dx"], dx2=s["dx"], dx3=s["dx"], lam=workloads.LAM,
              mu=workloads.MU, gam=workloads.GAM, beta=workloads.BETA, linear=t["linear"], nonlinear=t["nonlinear"],
              faultcreep=t["friction"], prestress=t["prestress"], interval=s["interval"], odt=s["odt"],
-             tscale=s["tscale"], skip=s["skip"], event_times=[0.0], ltransient=t["ltransient"],
-             nltransient=t["nltransient"])
+             tscale=s["tscale"], skip=s["skip"], event_times=list(workloads.event_times(s)),
+             ltransient=t["ltransient"], nltransient=t["nltransient"])
+    if s.get("events"):
+        m["event_patches"] = []
+        for _, deck in s["events"]:
+            pa, sl = structure.patches(deck)
+            m["event_patches"].append((pa.astype(O.PLANE), sl))
     h3 = s["sx3"] // 2
     keys = {LINEAR: "dg_linear", POWERLAW: "dg_nonlinear", LTRANSIENT: "dg_ltransient",
             NLTRANSIENT: "dg_nltransient"}

@@ -477,3 +477,32 @@ def test_observation_points_match_oracle():
         g.points([[0.0, 0.0, h3 * dx]])
     assert e.value.status == 15
     g.close()
+
+
+def test_event_switch_matches_oracle():
+    """A second coseismic event at t = 1.3: integrationstep cuts Dt to reach
+    it, the step's event index advances, relax_model_run stops there, and
+    relax_model_event applies the slip (tau += dtau, u += G[f(dtau)]); the
+    trajectory through and after the event follows the oracle."""
+    s = small_spec("maxwell")
+    rng = np.random.default_rng(77)
+    s["events"] = [(1.3, [workloads._deck_patch(rng, -8, 8, 5.0, 4.0, 2e-3) for _ in range(2)])]
+    nsteps = 6
+    st = oracle_initial_state(s)
+    om = oracle_model(s)
+    info = O.rk2_run(om, st, nsteps)
+    assert st["e"] == 2
+    g = workloads.build(s)
+    first = g.run(nsteps)
+    assert first[-1]["e"] == 1 and len(first) < nsteps  # stopped at the event
+    g.close()
+    g = workloads.build(s)
+    recs = workloads.run(g, s, nsteps, dt_replay=info[:, 0])
+    out = g.download()
+    g.close()
+    assert [r["e"] for r in recs] == [0 if r["t"] < 1.3 - 1e-9 else 1 for r in recs]
+    np.testing.assert_allclose([r["t"] for r in recs], info[:, 7], rtol=1e-12)
+    for k in ("u1", "u2", "u3"):
+        assert rel_l2(out[k][..., :s["sx1"]], st[k][..., :s["sx1"]]) < 1e-4, k
+    for k in ("sig", "tau"):
+        assert rel_l2(out[k], st[k]) < 1e-4, k

@@ -35,6 +35,9 @@ Parity status per function:
   shiftedindex, exportpoints  pinned (grid points map to themselves, ties
                               round to the lower index, range; the user
                               frame at 90 degrees swaps the components)
+  event switch .............. pinned (elastic superposition: an event of
+                              slip s1 at t1 on top of a set-up of slip s0
+                              equals a set-up of slip s0 + s1)
   RK2 loop .................. pinned (uniform-stress closed-form decay,
                               Burgers transient midpoint recurrence, weak
                               zones equal to the same fluidity in layers)
@@ -1004,3 +1007,55 @@ def test_exportpoints_user_frame():
     np.testing.assert_allclose(b[:, 0], -a[:, 1], atol=1e-15)
     np.testing.assert_allclose(b[:, 1], a[:, 0], atol=1e-15)
     np.testing.assert_array_equal(b[:, 2:], a[:, 2:])
+
+
+# --------------------------------------------------------- event switch ---
+def rel_l2(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _setup(sx, sx3, dx, patches, slips, dtype=np.float64):
+    """Coseismic set-up: tau from momentdensityshear, u = G[f(tau)],
+    sig = -tau + C:eps(u)."""
+    h3 = sx3 // 2
+    tau = np.zeros((h3, sx, sx, 6), dtype)
+    for p, sl in zip(patches, slips):
+        O.momentdensityshear(MU, sl, p["x"], p["y"], p["z"], p["width"], p["length"], p["strike"], p["dip"],
+                             p["rake"], 0.2, dx, dx, dx, tau)
+    c = [np.zeros((sx3, sx, sx + 2), dtype) for _ in range(3)]
+    t = [np.zeros((sx, sx + 2), dtype) for _ in range(3)]
+    O.equivalentbodyforce(tau, dx, dx, dx, c[0][:h3], c[1][:h3], c[2][:h3], t[0], t[1], t[2])
+    O.greenfunctioncowling(c[0], c[1], c[2], t[0], t[1], t[2], dx, dx, dx, LAM, MU, 0.0)
+    u = [np.ascontiguousarray(a[:h3]) for a in c]
+    sig = np.ascontiguousarray(-tau)
+    O.stressupdate(u[0], u[1], u[2], LAM, MU, dx, dx, dx, sig)
+    return dict(u1=u[0], u2=u[1], u3=u[2], tau=tau, sig=sig, gamma=np.zeros_like(u[0]), t=0.0, oi=1, i=0)
+
+
+@pytest.mark.parametrize("mut", [0, 18])
+def test_event_switch_superposes_elastically(mut, mutation):
+    """Without relaxation, a second event of slip s1 on the first event's
+    patches, reached by the Dt truncation of integrationstep, leaves the
+    state of a single set-up of slip s0 + s1; the output counter advances at
+    the event."""
+    sx, sx3, dx = 16, 32, 1.0
+    patches = np.zeros(2, O.PLANE)
+    patches[0] = (0.0, -1.0, 3.0, 4.0, 6.0, 0.3, 0.5, 1.0, 0, 0)
+    patches[1] = (2.0, 2.0, 5.0, 3.0, 3.0, 1.2, 0.9, -0.4, 0, 0)
+    s0, s1 = np.array([1.0, 0.5]), np.array([-0.3, 2.0])
+    h3 = sx3 // 2
+    m = dict(sx1=sx, sx2=sx, sx3=sx3, dx1=dx, dx2=dx, dx3=dx, lam=LAM, mu=MU, gam=0.0, beta=0.2,
+             prestress=np.zeros(h3, O.TENSOR_LAYER), interval=100.0, odt=1.0, tscale=1.0, skip=1,
+             event_times=[0.0, 2.5], event_patches=[(patches, s1)])
+    st = _setup(sx, sx3, dx, patches, s0)
+    mutation(mut)
+    info = O.rk2_run(m, st, 1)
+    mutation(0)
+    ref = _setup(sx, sx3, dx, patches, s0 + s1)
+    assert info[0, 0] == 2.5 and st["t"] == 2.5 and st["e"] == 2 and st["oi"] == 2
+    err = max(rel_l2(st[k], ref[k]) for k in ("u1", "u2", "u3", "tau", "sig"))
+    if mut:
+        assert err > 1e-3
+    else:
+        assert err < 1e-12
