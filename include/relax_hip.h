@@ -58,6 +58,11 @@ typedef struct {                                                                
   double dgammadot0, x, y, z, width, length, thickness, strike, dip;  /* angles in radians */
   relax_tensor_t e;
 } relax_weak_t;
+typedef struct {                                                                 /* types.f90:26-29 */
+  /* a shear patch or a surface load of an event, as input.f90 leaves it
+   * (angles in radians, the internal convention of wangconvention) */
+  double slip, x, y, z, width, length, strike, dip, rake, period, phase, beta, opening;
+} relax_source_t;
 
 /* ---- library ------------------------------------------------------------ */
 const char* relax_hip_version(void);
@@ -115,6 +120,19 @@ int relax_tensoramplitude(const relax_tensor_t* t, long n, double dx1, double dx
 int relax_momentdensityshear(double mu, double slip, double x, double y, double z, double L, double W,
                              double strike, double dip, double rake, double beta, int sx1, int sx2, int sx3,
                              double dx1, double dx2, double dx3, relax_tensor_t* sig);
+
+/* source, elastic3d.f90:2064-2252: the body force of a tapered shear
+ * dislocation (L down dip, W along strike) added to the top sx3/2 planes of
+ * f1..f3 ((sx1+2*align) x sx2 x sx3 arrays) and its surface traction to
+ * t1..t3 ((sx1+2*align) x sx2). */
+int relax_source(double mu, double slip, double x, double y, double z, double L, double W, double strike,
+                 double dip, double rake, double beta, int sx1, int sx2, int sx3, double dx1, double dx2,
+                 double dx3, float* f1, float* f2, float* f3, float* t1, float* t2, float* t3, int align);
+/* traction, elastic3d.f90:2517-2590: the surface loads l[0..nl) of an event
+ * subtracted from t3 ((sx1+2*align) x sx2) at time t, or their rate over
+ * [t, t+Dt] when rate is nonzero. */
+int relax_traction(double mu, const relax_source_t* l, int nl, int sx1, int sx2, double dx1, double dx2, double t,
+                   double Dt, float* t3, int rate, int align);
 
 /* ---- viscoelastic3d.f90 / friction3d.f90 --------------------------------- */
 /* viscouseigenstress, viscoelastic3d.f90:217-314.  maxwelltime, dgammadot0
@@ -236,20 +254,27 @@ int relax_model_step(relax_model_t* m, const double* dt_override, relax_step_inf
  * is NULL.  *done receives the number of steps taken. */
 int relax_model_run(relax_model_t* m, int nsteps, const double* dt_override, relax_step_info_t* info, int* done);
 int relax_model_synchronize(relax_model_t* m);
-/* Coseismic set-up on the device.  add_shear adds momentdensityshear
+/* A coseismic event: the first one at t = 0 on a new model (relax.f90:
+ * 316-371) or a later one once a step has reached its time (info.e, the
+ * event switch of relax.f90:858-889).  As dislocations (relax.f90:
+ * 1116-1289) and traction (elastic3d.f90:2517-2590) followed by the Green
+ * solve: f, t = the source() body force and traction of the shear patches
+ * s[k] longer or wider than nyquist*min(dx1,dx2,dx3), minus the surface
+ * loads l[k] at the model's time; u += greenfunctioncowling(f, t);
+ * tau += the momentdensityshear of every patch; sig = -tau + C:eps(u).
+ * Patches below that size get no displacement (the reference adds Okada's
+ * solution for them at t = 0 only, relax.f90:357-360).  Not on a member of
+ * a batch. */
+int relax_model_event(relax_model_t* m, const relax_source_t* s, int ns, const relax_source_t* l, int nl,
+                      double nyquist);
+/* Utilities of the same set-up: add_shear adds momentdensityshear
  * (elastic3d.f90:2661-2750) of n rectangular patches (centre x,y,z, width,
- * length, strike, dip, rake in the internal convention, input.f90:1780-1800)
- * with slip[k] to tau, as dislocations does (relax.f90:1199-1206).
- * coseismic then sets u = greenfunctioncowling(equivalentbodyforce(tau)) and
- * sig = -tau + C:eps(u) (relax.f90:318-371 with the FIR divergence of tau in
- * place of the analytic source() body force). */
+ * length, strike, dip, rake in the internal convention) with slip[k] to tau
+ * (taper beta of the model); coseismic then sets u = greenfunctioncowling(
+ * equivalentbodyforce(tau)) and sig = -tau + C:eps(u), the elastic state of
+ * a given tau with the FIR divergence as body force. */
 int relax_model_add_shear(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n);
 int relax_model_coseismic(relax_model_t* m);
-/* A later coseismic event (the event switch of relax.f90:858-889), once a
- * step has reached its time (info.e): tau += dtau, the momentdensityshear of
- * the event's patches; u += greenfunctioncowling(equivalentbodyforce(dtau));
- * sig = -tau + C:eps(u).  Not on a member of a batch. */
-int relax_model_event(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n);
 /* Phase timers (the HIP-event counterpart of the reference's PAPI phase
  * timers, papi_prof.c): runs nsteps steps with an event after every launch
  * and returns, per kernel kind, the summed milliseconds and the number of

@@ -30,6 +30,9 @@ PLANE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("width", "f8"), ("leng
 # WEAK_STRUCT, types.f90:46-50
 WEAK = np.dtype([("dgammadot0", "f8"), ("x", "f8"), ("y", "f8"), ("z", "f8"), ("width", "f8"),
                  ("length", "f8"), ("thickness", "f8"), ("strike", "f8"), ("dip", "f8"), ("e", "f4", (6,))])
+# SOURCE_STRUCT, types.f90:26-29 (shear patches and surface loads of an event)
+SOURCE = np.dtype([(k, "f8") for k in ("slip", "x", "y", "z", "width", "length", "strike", "dip", "rake", "period",
+                                       "phase", "beta", "opening")])
 
 
 class OrcModel(ctypes.Structure):
@@ -44,7 +47,8 @@ class OrcModel(ctypes.Structure):
                 ("skip", ctypes.c_int), ("ne", ctypes.c_int), ("event_times", ctypes.c_void_p),
                 ("ltransient", ctypes.c_void_p), ("nltransient", ctypes.c_void_p),
                 ("dg0", ctypes.c_void_p * 4),
-                ("ev_planes", ctypes.c_void_p), ("ev_slip", ctypes.c_void_p), ("ev_off", ctypes.c_void_p)]
+                ("ev_src", ctypes.c_void_p), ("ev_off", ctypes.c_void_p), ("ev_loads", ctypes.c_void_p),
+                ("ev_loff", ctypes.c_void_p), ("nyquist", ctypes.c_double)]
 
 
 def lib():
@@ -58,6 +62,7 @@ def lib():
         _LIB.orc_tensoramplitude_d.restype = ctypes.c_double
         assert _LIB.orc_sizeof_patch() == PATCH.itemsize
         assert _LIB.orc_sizeof_model() == ctypes.sizeof(OrcModel)
+        assert _LIB.orc_sizeof_source() == SOURCE.itemsize
         assert _LIB.orc_sizeof_weak() == WEAK.itemsize
     return _LIB
 
@@ -192,6 +197,38 @@ def monitorfriction(x, y, z, L, W, strike, dip, rake, beta, sx3_full, dx1, dx2, 
         _d(dx1), _d(dx2), _d(dx3), _p(sig), _p(structure), _p(patch), px2, px3), "monitorfriction")
 
 
+def source(mu, slip, x, y, z, L, W, strike, dip, rake, beta, dx1, dx2, dx3, f1, f2, f3, t1, t2, t3):
+    """elastic3d.f90:2064-2252: adds the body force of a shear dislocation to
+    the padded fields f (sx3, sx2, sx1+2) (top half) and its surface
+    traction to the padded planes t (sx2, sx1+2)."""
+    sx3, sx2, p1 = f1.shape
+    getattr(lib(), "orc_source_" + _sfx(f1))(
+        _d(mu), _d(slip), _d(x), _d(y), _d(z), _d(L), _d(W), _d(strike), _d(dip), _d(rake), _d(beta), p1 - 2, sx2,
+        sx3, _d(dx1), _d(dx2), _d(dx3), _p(f1), _p(f2), _p(f3), _p(t1), _p(t2), _p(t3))
+
+
+def traction(loads, dx1, dx2, t, t3, Dt=0.0, rate=False):
+    """elastic3d.f90:2517-2590: subtracts the surface loads (SOURCE array) at
+    time t (or their rate over [t, t+Dt]) from the padded plane t3."""
+    sx2, p1 = t3.shape
+    loads = np.ascontiguousarray(loads, SOURCE)
+    getattr(lib(), "orc_traction_" + _sfx(t3))(_p(loads), len(loads), p1 - 2, sx2, _d(dx1), _d(dx2), _d(t), _d(Dt),
+                                               _p(t3), int(rate))
+
+
+def dislocations(sources, loads, mu, nyquist, t, dx1, dx2, dx3, v, tr, tau):
+    """relax.f90:1116-1289 then traction: the source() body force of the
+    shear patches (SOURCE array) longer or wider than nyquist*min(dx) into
+    the padded fields v (3 arrays (sx3, sx2, sx1+2)) and planes tr, every
+    patch's moment density into tau, the surface loads into tr[2]."""
+    sx3, sx2, p1 = v[0].shape
+    s = np.ascontiguousarray(sources, SOURCE)
+    ld = np.ascontiguousarray(np.zeros(0, SOURCE) if loads is None else loads, SOURCE)
+    getattr(lib(), "orc_dislocations_" + _sfx(tau))(
+        _p(s), len(s), _p(ld), len(ld), _d(mu), _d(nyquist), _d(t), p1 - 2, sx2, sx3, _d(dx1), _d(dx2), _d(dx3),
+        _p(v[0]), _p(v[1]), _p(v[2]), _p(tr[0]), _p(tr[1]), _p(tr[2]), _p(tau))
+
+
 def shiftedindex(x1, x2, x3, sx1, sx2, sx3, dx1, dx2, dx3):
     """elastic3d.f90:4063-4101, 0-based (i1, i2, i3); None out of range."""
     idx = np.zeros(3, np.int32)
@@ -225,8 +262,9 @@ def rk2_run(model, state, nsteps, dt_replay=None):
     tscale, skip, event_times; optional ltransient/nltransient (LAYER arrays)
     with state["epsilonik"] (tensor field), and weak-zone fields dg_linear,
     dg_nonlinear, dg_ltransient, dg_nltransient ((sx3/2, sx2, sx1) arrays of
-    the state's dtype), and the later events' shear patches: event_patches, a
-    list (one entry per event after the first) of (PLANE array, slips).
+    the state's dtype), and the later events: event_sources, a list (one
+    entry per event after the first) of SOURCE arrays, event_loads likewise
+    (surface loads, optional), and nyquist (dislocations' size gate).
     state["e"] is the current event (1-based, default 1).  Returns an
     (nsteps, 10) array of
     Dt, tm, maxwell(1:3), power, C:E^i, t, maxwell(4:5).
@@ -248,15 +286,15 @@ def rk2_run(model, state, nsteps, dt_replay=None):
                  ptr(model.get("planes")), None if model.get("patches") is None else model["patches"].ctypes.data,
                  model["interval"], model.get("odt", 1.0), model.get("tscale", 1.0), model.get("skip", 1),
                  len(ev), ptr(ev), ptr(model.get("ltransient")), ptr(model.get("nltransient")))
-    evp = model.get("event_patches")
-    if evp:
-        planes = [np.zeros(0, PLANE)] + [np.asarray(p, PLANE) for p, _ in evp]
-        slips = [np.zeros(0)] + [np.asarray(sl, np.float64) for _, sl in evp]
-        off = np.zeros(len(planes) + 1, np.int32)
-        off[1:] = np.cumsum([len(p) for p in planes])
-        m.ev_planes = ptr(np.concatenate(planes).astype(PLANE))
-        m.ev_slip = ptr(np.concatenate(slips))
-        m.ev_off = ptr(off)
+    for key, fs, fo in (("event_sources", "ev_src", "ev_off"), ("event_loads", "ev_loads", "ev_loff")):
+        evs = model.get(key)
+        if evs:
+            lists = [np.zeros(0, SOURCE)] + [np.asarray(a, SOURCE) for a in evs]
+            off = np.zeros(len(lists) + 1, np.int32)
+            off[1:] = np.cumsum([len(a) for a in lists])
+            setattr(m, fs, ptr(np.concatenate(lists).astype(SOURCE)))
+            setattr(m, fo, ptr(off))
+    m.nyquist = model.get("nyquist", 1.0)
     for k, key in enumerate(("dg_linear", "dg_nonlinear", "dg_ltransient", "dg_nltransient")):
         a = model.get(key)
         if a is not None:

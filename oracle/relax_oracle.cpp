@@ -48,6 +48,7 @@ struct orc_patch {
   float sig[6];
 };
 struct orc_plane { double x, y, z, width, length, strike, dip, rake; int px2, px3; };
+struct orc_source { double slip, x, y, z, width, length, strike, dip, rake, period, phase, beta, opening; };
 }
 
 template <class R> struct T6 { R s11, s12, s13, s22, s23, s33; };
@@ -958,6 +959,153 @@ static void momentdensityshear(double mu, double slip, double x, double y, doubl
 }
 
 // ---------------------------------------------------------------------------
+// source, elastic3d.f90:2064-2252: the equivalent body force of a tapered
+// shear dislocation in the space domain (x1 derivative of the Gaussian
+// aperture, x2 and x3 derivatives of the raised-cosine tapers), with the
+// surface image for near-vertical planes, and its surface traction.  L is
+// the down-dip extent and W the along-strike one (the driver passes width,
+// length).  f1..f3 are padded (sx1+2, sx2, sx3) fields of which the top half
+// is written; t1..t3 padded planes.
+// ---------------------------------------------------------------------------
+static inline double gaussp(double x, double sigma) {
+  return -x * std::exp(-0.5 * (x / sigma) * (x / sigma)) / std::sqrt(PI2) / (sigma * sigma * sigma);
+}
+static inline double omegap(double x, double beta) {
+  const double ax = std::fabs(x);
+  if (ax > (1.0 - 2.0 * beta) / (1.0 - beta) / 2.0 && ax < 1.0 / (1 - beta) / 2.0)
+    return -std::copysign(1.0, x) * PI * (1.0 - beta) / 2.0 / beta *
+           std::sin(PI * ((1.0 - beta) * ax - 0.5 + beta) / beta);
+  return 0.0;
+}
+
+template <class R>
+static void source(double mu, double s, double x, double y, double z, double L, double W, double strike, double dip,
+                   double rake, double beta, int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, R* f1,
+                   R* f2, R* f3, R* t1, R* t2, R* t3) {
+  const long pitch = sx1 + 2;
+  const double cs = std::cos(strike), ss = std::sin(strike), cd = std::cos(dip), sd = std::sin(dip);
+  const double cr = std::cos(rake), sr = std::sin(rake);
+  const double scale = -mu * s;
+  const double Wp = W * (1.0 + 2.0 * beta) / 2.0, Lp = L * (1.0 + 2.0 * beta) / 2.0;
+  const double wl = std::max(Lp, Wp);
+  double x2r = cs * x - ss * y;
+  const double xr = cd * x2r - sd * z, yr = ss * x + cs * y, zr = sd * x2r + cd * z;
+  // surface traction of the moment density at x3 = 0
+  for (int i2 = 0; i2 < sx2; ++i2)
+    for (int i1 = 0; i1 < sx1; ++i1) {
+      const double x1 = shifted1(i1, sx1, dx1), x2 = shifted1(i2, sx2, dx2), x3 = 0.0;
+      if ((std::fabs(x1 - x) > wl) || (std::fabs(x2 - y) > wl)) continue;
+      const double xx = cs * x1 - ss * x2;
+      const double x1s = cd * xx - sd * x3, x1i = cd * xx + sd * x3;
+      if ((std::fabs(x1s - xr) > 7.01 * dx1) && (std::fabs(x1i - xr) > 7.01 * dx1)) continue;
+      const double x2s = ss * x1 + cs * x2, x3s = sd * xx + cd * x3, x3i = -sd * xx + cd * x3;
+      const double om2 = omega((x2s - yr) / W, beta);
+      const double ap = gauss(x1s - xr, dx1) * om2 * omega((x3s - zr) / L, beta) +
+                        gauss(x1i - xr, dx1) * om2 * omega((x3i + zr) / L, beta);
+      const double n[3] = {cd * cs * ap, -cd * ss * ap, -sd * ap};
+      const double b[3] = {mu * s * (ss * cr + cs * sd * sr), mu * s * (cs * cr - ss * sd * sr), mu * s * (cd * sr)};
+      const long q = i1 + pitch * i2;
+      t1[q] = t1[q] + (R)((n[0] * b[2] + n[2] * b[0]) / 2.0);
+      t2[q] = t2[q] + (R)((n[1] * b[2] + n[2] * b[1]) / 2.0);
+      t3[q] = t3[q] + (R)(n[2] * b[2]);
+    }
+  // body force in the top half
+  const bool vertical = 2.01 * DEG2RAD > dip;
+  for (int i3 = 0; i3 < sx3 / 2; ++i3) {
+    const double x3 = (double)i3 * dx3;
+    if ((std::fabs(x3 - z) > Lp) && (std::fabs(x3 + z) > Lp)) continue;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        const double x1 = shifted1(i1, sx1, dx1), x2 = shifted1(i2, sx2, dx2);
+        if ((std::fabs(x1 - x) > wl) || (std::fabs(x2 - y) > wl)) continue;
+        const double xx = cs * x1 - ss * x2;
+        const double x1s = cd * xx - sd * x3, x1i = cd * xx + sd * x3;
+        if ((std::fabs(x1s - xr) > 7.01 * dx1) && (std::fabs(x1i - xr) > 7.01 * dx1)) continue;
+        const double x2s = ss * x1 + cs * x2, x3s = sd * xx + cd * x3, x3i = -sd * xx + cd * x3;
+        double g = gauss(x1s - xr, dx1);
+        const double om2 = omega((x2s - yr) / W, beta), om2p = omegap((x2s - yr) / W, beta);
+        double om3 = omega((x3s - zr) / L, beta);
+        const double sourc = scale * gaussp(x1s - xr, dx1) * om2 * om3;
+        const double dblcp = scale / W * g * om2p * om3;
+        const double dipcs = scale / L * g * om2 * omegap((x3s - zr) / L, beta);
+        g = gauss(x1i - xr, dx1);
+        om3 = omega((x3i + zr) / L, beta);
+        const double image = scale * gaussp(x1i - xr, dx1) * om2 * om3;
+        const double cplei = scale / W * g * om2p * om3;
+        const long o = i1 + pitch * (i2 + (long)sx2 * i3);
+        if (vertical && !MUT(19)) {
+          f1[o] = (R)((double)f1[o] + (cr * ss * (sourc + image) + cr * cd * cs * (dblcp + cplei)));
+          f2[o] = (R)((double)f2[o] + (cr * cs * (sourc + image) - cr * cd * ss * (dblcp + cplei)));
+          f3[o] = (R)((double)f3[o] + (-cr * sd * (dblcp - cplei)));
+        } else {
+          f1[o] = (R)((double)f1[o] + (cr * ss * sourc + cr * cd * cs * dblcp));
+          f2[o] = (R)((double)f2[o] + (cr * cs * sourc - cr * cd * ss * dblcp));
+          f3[o] = (R)((double)f3[o] + (-cr * sd * dblcp));
+        }
+        // dip-slip component
+        f1[o] = (R)((double)f1[o] + (cd * sr * cs * dipcs + sd * sr * cs * sourc));
+        f2[o] = (R)((double)f2[o] + (-cd * sr * ss * dipcs - sd * sr * ss * sourc));
+        f3[o] = (R)((double)f3[o] + (cd * sr * sourc - sd * sr * dipcs));
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// traction, elastic3d.f90:2517-2590: the surface loads l of an event
+// subtracted from t3 (padded plane) at time t, or their rate over [t, t+Dt]
+// ---------------------------------------------------------------------------
+template <class R>
+static void traction(const orc_source* l, int nl, int sx1, int sx2, double dx1, double dx2, double t, double Dt,
+                     R* t3, bool rate) {
+  const long pitch = sx1 + 2;
+  for (int k = 0; k < nl; ++k) {
+    const orc_source& L = l[k];
+    const double Lp = L.length * (1.0 + 2.0 * L.beta) / 2.0, Wp = L.width * (1.0 + 2.0 * L.beta) / 2.0;
+    if (0 == Lp || 0 == Wp) continue;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        const double x1 = shifted1(i1, sx1, dx1), x2 = shifted1(i2, sx2, dx2);
+        if ((std::fabs(x1 - L.x) > std::max(Lp, Wp)) || (std::fabs(x2 - L.y) > std::max(Lp, Wp))) continue;
+        const double amp = omega((x1 - L.x) / L.length, L.beta) * omega((x2 - L.y) / L.width, L.beta) * L.slip;
+        R& v = t3[i1 + pitch * i2];
+        const double sg = MUT(21) ? -1.0 : 1.0;
+        if (rate) {
+          if (0 != L.period)
+            v = (R)((double)v - sg * amp * (std::sin(2 * PI * (t + Dt) / L.period + L.phase) -
+                                            std::sin(2 * PI * t / L.period + L.phase)) / Dt);
+        } else if (L.period == 0) {
+          v = (R)((double)v - sg * amp);
+        } else {
+          v = (R)((double)v - sg * amp * std::sin(2 * PI * t / L.period + L.phase));
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dislocations, relax.f90:1116-1289 (shear patches), then traction
+// (relax.f90:319, :874): the source() body force of the patches longer or
+// wider than nyquist*min(dx) into v (padded, sx3 deep) and t, the moment
+// density of every patch into tau, and the surface loads into t3.
+// ---------------------------------------------------------------------------
+template <class R>
+static void dislocations(const orc_source* s, int ns, const orc_source* l, int nl, double mu, double nyquist,
+                         double t, int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, R* v1, R* v2,
+                         R* v3, R* t1, R* t2, R* t3, R* tau) {
+  const double dmin = nyquist * std::min(dx1, std::min(dx2, dx3));
+  for (int k = 0; k < ns; ++k) {
+    const bool big = dmin < s[k].length || dmin < s[k].width;
+    if (MUT(20) ? !big : big)
+      source(mu, s[k].slip, s[k].x, s[k].y, s[k].z, s[k].width, s[k].length, s[k].strike, s[k].dip, s[k].rake,
+             s[k].beta, sx1, sx2, sx3, dx1, dx2, dx3, v1, v2, v3, t1, t2, t3);
+  }
+  for (int k = 0; k < ns; ++k)
+    momentdensityshear(mu, s[k].slip, s[k].x, s[k].y, s[k].z, s[k].width, s[k].length, s[k].strike, s[k].dip,
+                       s[k].rake, s[k].beta, sx1, sx2, sx3 / 2, dx1, dx2, dx3, tau);
+  traction(l, nl, sx1, sx2, dx1, dx2, t, 0.0, t3, false);
+}
+
+// ---------------------------------------------------------------------------
 // RK2 loop: relax.f90:555-890 with integrationstep :1420-1452 and the
 // isoutput counter :1364-1376 (oi advances whenever isoutput holds).
 // Mechanisms: linear and power-law viscosity (with weak zones), fault creep,
@@ -989,11 +1137,14 @@ struct orc_model {
   // weak zones (builddgammadot0 fields, sx1*sx2*sx3/2 values of the storage
   // type, or NULL): linear, power-law, linear transient, power-law transient
   const void* dg0[4];
-  // shear patches of the later events: event k (1..ne-1) owns
-  // ev_planes[ev_off[k] .. ev_off[k+1]) with slips ev_slip (NULL: none)
-  const orc_plane* ev_planes;
-  const double* ev_slip;
+  // the later events (NULL: none): event k (1..ne-1) owns the shear patches
+  // ev_src[ev_off[k] .. ev_off[k+1]) and the surface loads
+  // ev_loads[ev_loff[k] .. ev_loff[k+1]); dislocations' nyquist
+  const orc_source* ev_src;
   const int* ev_off;
+  const orc_source* ev_loads;
+  const int* ev_loff;
+  double nyquist;
 };
 }
 
@@ -1150,19 +1301,16 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
     // the event switch (relax.f90:858-886)
     if (e < m->ne && std::fabs(t - m->event_times[e]) < 1e-6) {
       e = e + 1;
-      if (m->ev_off && m->ev_off[e] > m->ev_off[e - 1]) {
-        std::vector<R> dtau(6 * npts, (R)0);
-        for (int k = m->ev_off[e - 1]; k < m->ev_off[e]; ++k) {
-          const orc_plane& P = m->ev_planes[k];
-          momentdensityshear(m->mu, m->ev_slip[k], P.x, P.y, P.z, P.width, P.length, P.strike, P.dip, P.rake,
-                             m->beta, sx1, sx2, h3, m->dx1, m->dx2, m->dx3, dtau.data());
-        }
+      const int s0 = m->ev_off ? m->ev_off[e - 1] : 0, s1 = m->ev_off ? m->ev_off[e] : 0;
+      const int l0 = m->ev_loff ? m->ev_loff[e - 1] : 0, l1 = m->ev_loff ? m->ev_loff[e] : 0;
+      if (s1 > s0 || l1 > l0) {
         std::fill(v1.begin(), v1.end(), (R)0); std::fill(v2.begin(), v2.end(), (R)0);
         std::fill(v3.begin(), v3.end(), (R)0); std::fill(t1.begin(), t1.end(), (R)0);
         std::fill(t2.begin(), t2.end(), (R)0); std::fill(t3.begin(), t3.end(), (R)0);
-        rc = equivalentbodyforce(dtau.data(), m->dx1, m->dx2, m->dx3, sx1, sx2, h3, v1.data(), v2.data(),
-                                 v3.data(), t1.data(), t2.data(), t3.data(), pitch, (const R*)0);
-        if (rc) return rc;
+        std::vector<R> lost(MUT(18) ? 6 * npts : 0, (R)0);
+        dislocations(m->ev_src + s0, s1 - s0, m->ev_loads ? m->ev_loads + l0 : nullptr, l1 - l0, m->mu, m->nyquist,
+                     t, sx1, sx2, sx3, m->dx1, m->dx2, m->dx3, v1.data(), v2.data(), v3.data(), t1.data(),
+                     t2.data(), t3.data(), MUT(18) ? lost.data() : tau);
         rc = greenfunctioncowling(v1.data(), v2.data(), v3.data(), t1.data(), t2.data(), t3.data(), sx1, sx2, sx3,
                                   m->dx1, m->dx2, m->dx3, m->lambda, m->mu, m->gam);
         if (rc) return rc;
@@ -1171,8 +1319,6 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
           u2[o] = u2[o] + v2[o];
           u3[o] = u3[o] + v3[o];
         }
-        if (!MUT(18))
-          for (long o = 0; o < 6 * npts; ++o) tau[o] = tau[o] + dtau[o];
       }
     }
     for (long o = 0; o < 6 * npts; ++o) sig[o] = (R)(-1) * tau[o];
@@ -1261,6 +1407,22 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
                                                   R* sig) {                                                          \
     momentdensityshear(mu, slip, x, y, z, L, W, strike, dip, rake, beta, sx1, sx2, sx3, dx1, dx2, dx3, sig);         \
   }                                                                                                                  \
+  ORC_EXPORT void orc_source_##SUFFIX(double mu, double s, double x, double y, double z, double L, double W,       \
+                                      double strike, double dip, double rake, double beta, int sx1, int sx2,        \
+                                      int sx3, double dx1, double dx2, double dx3, R* f1, R* f2, R* f3, R* t1,      \
+                                      R* t2, R* t3) {                                                                \
+    source(mu, s, x, y, z, L, W, strike, dip, rake, beta, sx1, sx2, sx3, dx1, dx2, dx3, f1, f2, f3, t1, t2, t3);     \
+  }                                                                                                                  \
+  ORC_EXPORT void orc_traction_##SUFFIX(const orc_source* l, int nl, int sx1, int sx2, double dx1, double dx2,      \
+                                        double t, double Dt, R* t3, int rate) {                                     \
+    traction(l, nl, sx1, sx2, dx1, dx2, t, Dt, t3, rate != 0);                                                       \
+  }                                                                                                                  \
+  ORC_EXPORT void orc_dislocations_##SUFFIX(const orc_source* s, int ns, const orc_source* l, int nl, double mu,     \
+                                            double nyquist, double t, int sx1, int sx2, int sx3, double dx1,        \
+                                            double dx2, double dx3, R* v1, R* v2, R* v3, R* t1, R* t2, R* t3,       \
+                                            R* tau) {                                                                \
+    dislocations(s, ns, l, nl, mu, nyquist, t, sx1, sx2, sx3, dx1, dx2, dx3, v1, v2, v3, t1, t2, t3, tau);         \
+  }                                                                                                                  \
   ORC_EXPORT int orc_exportpoints_##SUFFIX(const R* u1, const R* u2, const R* u3, const R* sig, int sx1, int sx2,  \
                                            int sx3, double dx1, double dx2, double dx3, const double* pts, int n,   \
                                            double rot, double* out) {                                               \
@@ -1289,6 +1451,7 @@ ORC_EXPORT void orc_set_mutation(int k) { g_mut = k; }
 ORC_EXPORT int orc_sizeof_patch(void) { return (int)sizeof(orc_patch); }
 ORC_EXPORT int orc_sizeof_model(void) { return (int)sizeof(orc_model); }
 ORC_EXPORT int orc_sizeof_weak(void) { return (int)sizeof(orc_weak); }
+ORC_EXPORT int orc_sizeof_source(void) { return (int)sizeof(orc_source); }
 ORC_EXPORT int orc_shiftedindex(double x1, double x2, double x3, int sx1, int sx2, int sx3, double dx1, double dx2,
                                 double dx3, int* idx) {
   return shiftedindex(x1, x2, x3, sx1, sx2, sx3, dx1, dx2, dx3, idx, idx + 1, idx + 2);

@@ -15,6 +15,6 @@ Arrays are numpy float32 in the reference's memory order (C-order shapes
 (sx3, sx2, sx1+2) for vector fields, (sx3, sx2, sx1, 6) for TENSOR fields).
 Everything runs through librelax_hip.so; there is no CPU fallback.
 """
-from ._lib import LAYER, PATCH, PLANE, TENSOR_LAYER, WEAK, RelaxError, load  # noqa: F401
+from ._lib import LAYER, PATCH, PLANE, SOURCE, TENSOR_LAYER, WEAK, RelaxError, load  # noqa: F401
 
-__all__ = ["LAYER", "PATCH", "PLANE", "TENSOR_LAYER", "WEAK", "RelaxError", "load"]
+__all__ = ["LAYER", "PATCH", "PLANE", "SOURCE", "TENSOR_LAYER", "WEAK", "RelaxError", "load"]

@@ -27,6 +27,8 @@ PLANE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("width", "f8"), ("leng
                   ("strike", "f8"), ("dip", "f8"), ("rake", "f8"), ("px2", "i4"), ("px3", "i4")])
 WEAK = np.dtype([("dgammadot0", "f8"), ("x", "f8"), ("y", "f8"), ("z", "f8"), ("width", "f8"),
                  ("length", "f8"), ("thickness", "f8"), ("strike", "f8"), ("dip", "f8"), ("e", "f4", (6,))])
+SOURCE = np.dtype([(k, "f8") for k in ("slip", "x", "y", "z", "width", "length", "strike", "dip", "rake", "period",
+                                       "phase", "beta", "opening")])
 
 
 class ModelDesc(ctypes.Structure):
@@ -65,6 +67,10 @@ SIGNATURES = {
     "relax_fieldadd": (c_int, [P, P, c_long, P, P]),
     "relax_tensorfieldadd": (c_int, [P, P, c_long, P, P]),
     "relax_tensoramplitude": (c_int, [P, c_long, c_double, c_double, c_double, P]),
+    "relax_source": (c_int, [c_double] * 11 + [c_int, c_int, c_int, c_double, c_double, c_double] + [P] * 6
+                     + [c_int]),
+    "relax_traction": (c_int, [c_double, P, c_int, c_int, c_int, c_double, c_double, c_double, c_double, P, c_int,
+                               c_int]),
     "relax_momentdensityshear": (c_int, [c_double, c_double, c_double, c_double, c_double, c_double, c_double,
                                          c_double, c_double, c_double, c_double, c_int, c_int, c_int, c_double,
                                          c_double, c_double, P]),
@@ -97,7 +103,7 @@ SIGNATURES = {
     "relax_model_stream": (c_void_p, [P]),
     "relax_model_add_shear": (c_int, [P, P, P, c_int]),
     "relax_model_coseismic": (c_int, [P]),
-    "relax_model_event": (c_int, [P, P, P, c_int]),
+    "relax_model_event": (c_int, [P, P, c_int, P, c_int, c_double]),
     "relax_model_profile": (c_int, [P, c_int, P, P]),
     "relax_model_autotune": (c_int, [P, P, P]),
     "relax_model_set_tuning": (c_int, [P, P]),

@@ -90,7 +90,7 @@ struct Prestress {
 // Friction planes in device form (friction3d.f90:82-241); angles in radians.
 struct PlaneDev {
   double x, y, z, L, W, beta;
-  double cstrike, sstrike, cdip, sdip, rake;
+  double cstrike, sstrike, cdip, sdip, rake, dip;
   double Wp, Lp, xr, yr, zr;
   double n[3], r[3];
   double minlw;  // min(L,W)/sqrt(dx1*dx3) of the relaxation-time estimate

@@ -729,6 +729,59 @@ int relax_momentdensityshear(double mu, double slip, double x, double y, double 
   return RELAX_OK;
 }
 
+int relax_source(double mu, double slip, double x, double y, double z, double L, double W, double strike,
+                 double dip, double rake, double beta, int sx1, int sx2, int sx3, double dx1, double dx2,
+                 double dx3, float* f1, float* f2, float* f3, float* t1, float* t2, float* t3, int align) {
+  if (!f1 || !f2 || !f3 || !t1 || !t2 || !t3 || sx1 < 1 || sx2 < 1 || sx3 < 2) return RELAX_ERR_ARG;
+  int rc;
+  if ((rc = has_device())) return rc;
+  // the top sx3/2 planes are the first ld*sx2*(sx3/2) values of each array
+  const long ld = sx1 + (align ? 2 : 0), fs = ld * sx2 * (sx3 / 2), ts = ld * sx2;
+  Buf f, t, pl, sl;
+  if ((rc = f.alloc(3 * fs * 4)) || (rc = t.alloc(3 * ts * 4)) || (rc = pl.alloc(sizeof(PlaneDev))) ||
+      (rc = sl.alloc(sizeof(double))))
+    return rc;
+  float* fh[3] = {f1, f2, f3};
+  float* th[3] = {t1, t2, t3};
+  for (int c = 0; c < 3; ++c) {
+    HIP_TRY(hipMemcpy(f.f() + c * fs, fh[c], fs * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.f() + c * ts, th[c], ts * 4, hipMemcpyHostToDevice));
+  }
+  // the driver passes (width, length) as (L, W)
+  const relax_plane_t p{x, y, z, /*width*/ L, /*length*/ W, strike, dip, rake, 0, 0};
+  const PlaneDev P = make_plane(p, beta, dx1, dx3);
+  HIP_TRY(hipMemcpy(pl.p, &P, sizeof(PlaneDev), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(sl.p, &slip, sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = launch_source((const PlaneDev*)pl.p, (const double*)sl.p, 1, mu, sx1, sx2, sx3 / 2, dx1, dx2, dx3,
+                          f.f(), ld, fs, t.f(), ts, 0)))
+    return rc;
+  if ((rc = sync_status(RELAX_OK))) return rc;
+  for (int c = 0; c < 3; ++c) {
+    HIP_TRY(hipMemcpy(fh[c], f.f() + c * fs, fs * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(th[c], t.f() + c * ts, ts * 4, hipMemcpyDeviceToHost));
+  }
+  return RELAX_OK;
+}
+
+int relax_traction(double mu, const relax_source_t* l, int nl, int sx1, int sx2, double dx1, double dx2, double t,
+                   double Dt, float* t3, int rate, int align) {
+  (void)mu;  // the reference's argument list carries mu; the loads do not use it
+  if (!t3 || nl < 0 || (nl > 0 && !l) || sx1 < 1 || sx2 < 1) return RELAX_ERR_ARG;
+  if (rate && Dt == 0.0) return RELAX_ERR_ARG;
+  int rc;
+  if ((rc = has_device())) return rc;
+  const long ld = sx1 + (align ? 2 : 0), ts = ld * sx2;
+  Buf tb, lb;
+  if ((rc = tb.alloc(ts * 4)) || (rc = lb.alloc(sizeof(relax_source_t) * (nl ? nl : 1)))) return rc;
+  HIP_TRY(hipMemcpy(tb.p, t3, ts * 4, hipMemcpyHostToDevice));
+  if (nl) HIP_TRY(hipMemcpy(lb.p, l, sizeof(relax_source_t) * nl, hipMemcpyHostToDevice));
+  if ((rc = launch_traction((const relax_source_t*)lb.p, nl, sx1, sx2, dx1, dx2, t, Dt, rate, tb.f(), ld, 0)))
+    return rc;
+  if ((rc = sync_status(RELAX_OK))) return rc;
+  HIP_TRY(hipMemcpy(t3, tb.p, ts * 4, hipMemcpyDeviceToHost));
+  return RELAX_OK;
+}
+
 // ============================================ viscoelastic3d / friction3d ==
 namespace {
 // per-depth tables of one mechanism and the deviatoric prestress

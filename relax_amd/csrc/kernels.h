@@ -65,6 +65,19 @@ int launch_surface_padded(const GreenConst& gc, const float2* u1, const float2* 
 int launch_cerruti_padded(const GreenConst& gc, const float2* p1, const float2* p2, const float2* p3,
                           float2* u1, float2* u2, float2* u3, hipStream_t st);
 
+// model.hip: the body force of source() (elastic3d.f90:2064-2252) of the
+// np planes P (slips slip) added to the top half of f (3 planes of
+// n1*n2*h3, packed rows of ld floats, plane stride fs) and their surface
+// traction to t (3 planes of n1*n2, row stride ld, plane stride ts).
+struct PlaneDev;
+int launch_source(const PlaneDev* P, const double* slip, int np, double mu, int n1, int n2, int h3, double dx1,
+                  double dx2, double dx3, float* f, long ld, long fs, float* t, long ts, hipStream_t st);
+
+// model.hip: traction() of the surface loads l subtracted from t3 (n1*n2,
+// row stride ld) at time t, or their rate over [t, t+Dt] when rate != 0
+int launch_traction(const relax_source_t* l, int nl, int n1, int n2, double dx1, double dx2, double t, double Dt,
+                    int rate, float* t3, long ld, hipStream_t st);
+
 // compat.hip: the weak-zone fluidity field of builddgammadot0
 // (viscoelastic3d.f90:320-435) on the device.  A zone in device form holds
 // what depends on the zone only, computed on the host in double as the

@@ -49,6 +49,8 @@ struct relax_model {
   double* shear_slip = nullptr;
   long shear_cap = 0;
   std::vector<PlaneDev> shear_host;
+  relax_source_t* loads_dev = nullptr;  // an event's surface loads
+  long loads_cap = 0;
   std::vector<relax_plane_t> host_planes;
   std::vector<PlaneDev> plane_dev_host;
   std::vector<long> patch_off;

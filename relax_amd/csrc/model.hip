@@ -263,7 +263,7 @@ size_t relax_model_footprint(const relax_model_desc_t* d) {
 static void free_model(relax_model_t* m) {
   if (!m) return;
   void* ptrs[] = {m->u, m->sig, m->tau, m->gam, m->W, m->NQ, m->T, m->TN, m->sc, m->partials, m->tables, m->ftab,
-                  m->planes_dev, m->patches_dev, m->md_dev, m->shear_dev, m->shear_slip, m->E, m->Em,
+                  m->planes_dev, m->patches_dev, m->md_dev, m->shear_dev, m->shear_slip, m->loads_dev, m->E, m->Em,
                   m->dg[0], m->dg[1], m->dg[2], m->dg[3], m->zones_dev, m->pts_dev, m->ev_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -689,6 +689,7 @@ PlaneDev make_plane(const relax_plane_t& p, double beta, double dx1, double dx3)
   P.cstrike = std::cos(p.strike); P.sstrike = std::sin(p.strike);
   P.cdip = std::cos(p.dip); P.sdip = std::sin(p.dip);
   P.rake = p.rake;
+  P.dip = p.dip;
   const double cr = std::cos(p.rake), sr = std::sin(p.rake);
   P.Wp = P.W * (1.0 + 2.0 * beta) / 2.0;
   P.Lp = P.L * (1.0 + 2.0 * beta) / 2.0;
@@ -1164,13 +1165,10 @@ __global__ void k_add_shear(const PlaneDev* __restrict__ P, const double* __rest
   }
 }
 
-// momentdensityshear of n patches added to the tensor planes `target`
-static int shear_into(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n, float* target) {
-  if (n == 0) return RELAX_OK;
-  std::vector<PlaneDev>& hp = m->shear_host;
-  hp.resize(n);
-  for (int k = 0; k < n; ++k) hp[k] = make_plane(patches[k], m->desc.beta, m->desc.dx1, m->desc.dx3);
-  if (n > m->shear_cap) {  // grow-only, as the plane buffers
+// The patch list m->shear_host and its slips in device form (grow-only)
+static int stage_planes(relax_model_t* m, const double* slip) {
+  const long n = (long)m->shear_host.size();
+  if (n > m->shear_cap) {
     if (m->shear_dev) (void)hipFree(m->shear_dev);
     if (m->shear_slip) (void)hipFree(m->shear_slip);
     m->shear_dev = nullptr;
@@ -1180,48 +1178,205 @@ static int shear_into(relax_model_t* m, const relax_plane_t* patches, const doub
     HIP_TRY(hipMalloc(&m->shear_slip, sizeof(double) * n));
     m->shear_cap = n;
   }
-  HIP_TRY(hipMemcpyAsync(m->shear_dev, hp.data(), sizeof(PlaneDev) * n, hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(hipMemcpyAsync(m->shear_dev, m->shear_host.data(), sizeof(PlaneDev) * n, hipMemcpyHostToDevice,
+                         m->stream));
   HIP_TRY(hipMemcpyAsync(m->shear_slip, slip, sizeof(double) * n, hipMemcpyHostToDevice, m->stream));
-  k_add_shear<<<grid_1d(m->g.H), 256, 0, m->stream>>>(m->shear_dev, m->shear_slip, n, 2.0f * (float)m->desc.mu, m->g,
-                                                      m->desc.dx1, m->desc.dx2, m->desc.dx3, target);
+  return RELAX_OK;
+}
+
+// momentdensityshear of the staged patches [k0, k0+n) added to tau
+static int add_shear(relax_model_t* m, int k0, int n) {
+  if (n == 0) return RELAX_OK;
+  k_add_shear<<<grid_1d(m->g.H), 256, 0, m->stream>>>(m->shear_dev + k0, m->shear_slip + k0, n,
+                                                      2.0f * (float)m->desc.mu, m->g, m->desc.dx1, m->desc.dx2,
+                                                      m->desc.dx3, m->tau);
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }
 
 int relax_model_add_shear(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n) {
   if (!m || n < 0 || (n > 0 && (!patches || !slip))) return RELAX_ERR_ARG;
   if (int rc = wait_batch(m)) return rc;
-  int rc = shear_into(m, patches, slip, n, m->tau);
+  if (n == 0) return RELAX_OK;
+  m->shear_host.resize(n);
+  for (int k = 0; k < n; ++k) m->shear_host[k] = make_plane(patches[k], m->desc.beta, m->desc.dx1, m->desc.dx3);
+  int rc = stage_planes(m, slip);
+  if (!rc) rc = add_shear(m, 0, n);
   if (hipStreamSynchronize(m->stream) != hipSuccess && !rc) rc = RELAX_ERR_HIP;
   return rc;
 }
 
-// tau += dtau over the 6 planes of H
-__global__ void k_add_planes(float* __restrict__ tau, const float* __restrict__ d, long n) {
-  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < n; o += (long)gridDim.x * blockDim.x)
-    tau[o] = tau[o] + d[o];
+// source, elastic3d.f90:2064-2252: one thread per point of the top half,
+// the planes' contributions added in their order with the reference's
+// rounding (two FP32 roundings per plane and component); threads of the top
+// plane also add the surface traction (n sdyad mu s b) e3.
+__global__ void k_source(const PlaneDev* __restrict__ P, const double* __restrict__ slip, int np, double mu,
+                         int n1, int n2, int h3, double dx1, double dx2, double dx3, float* f, long ld, long fs,
+                         float* t, long ts) {
+  const long N = (long)n1 * n2 * h3;
+  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (o >= N) return;
+  const int i1 = o % n1, i2 = (o / n1) % n2, i3 = o / ((long)n1 * n2);
+  const double x1 = d_shifted(i1, n1, dx1), x2 = d_shifted(i2, n2, dx2), x3 = (double)i3 * dx3;
+  const long of = i1 + ld * (i2 + (long)n2 * i3), ot = i1 + ld * i2;
+  float f1 = f[of], f2 = f[fs + of], f3 = f[2 * fs + of];
+  float t1 = 0.f, t2 = 0.f, t3 = 0.f;
+  if (i3 == 0) { t1 = t[ot]; t2 = t[ts + ot]; t3 = t[2 * ts + ot]; }
+  for (int k = 0; k < np; ++k) {
+    const PlaneDev& p = P[k];
+    const double wl = fmax(p.Lp, p.Wp);
+    if ((fabs(x1 - p.x) > wl) || (fabs(x2 - p.y) > wl)) continue;
+    const double cs = p.cstrike, ss = p.sstrike, cd = p.cdip, sd = p.sdip;
+    const double xx = cs * x1 - ss * x2;
+    const double x1s = cd * xx - sd * x3, x1i = cd * xx + sd * x3;
+    if ((fabs(x1s - p.xr) > 7.01 * dx1) && (fabs(x1i - p.xr) > 7.01 * dx1)) continue;
+    const double x2s = ss * x1 + cs * x2, x3s = sd * xx + cd * x3, x3i = -sd * xx + cd * x3;
+    const double cr = cos(p.rake), sr = sin(p.rake), s = slip[k];
+    const double om2 = d_omega((x2s - p.yr) / p.W, p.beta);
+    if (i3 == 0) {
+      // the moment density at the surface (source and image apertures)
+      const double ap = d_gauss(x1s - p.xr, dx1) * om2 * d_omega((x3s - p.zr) / p.L, p.beta) +
+                        d_gauss(x1i - p.xr, dx1) * om2 * d_omega((x3i + p.zr) / p.L, p.beta);
+      const double n[3] = {p.n[0] * ap, p.n[1] * ap, p.n[2] * ap};
+      const double b[3] = {mu * s * p.r[0], mu * s * p.r[1], mu * s * p.r[2]};
+      t1 = t1 + (float)((n[0] * b[2] + n[2] * b[0]) / 2.0);
+      t2 = t2 + (float)((n[1] * b[2] + n[2] * b[1]) / 2.0);
+      t3 = t3 + (float)(n[2] * b[2]);
+    }
+    if ((fabs(x3 - p.z) > p.Lp) && (fabs(x3 + p.z) > p.Lp)) continue;
+    const double scale = -mu * s;
+    const double om2p = d_omegap((x2s - p.yr) / p.W, p.beta);
+    double g = d_gauss(x1s - p.xr, dx1);
+    double om3 = d_omega((x3s - p.zr) / p.L, p.beta);
+    const double sourc = scale * d_gaussp(x1s - p.xr, dx1) * om2 * om3;
+    const double dblcp = scale / p.W * g * om2p * om3;
+    const double dipcs = scale / p.L * g * om2 * d_omegap((x3s - p.zr) / p.L, p.beta);
+    g = d_gauss(x1i - p.xr, dx1);
+    om3 = d_omega((x3i + p.zr) / p.L, p.beta);
+    const double image = scale * d_gaussp(x1i - p.xr, dx1) * om2 * om3;
+    const double cplei = scale / p.W * g * om2p * om3;
+    if (2.01 * RELAX_DEG2RAD > p.dip) {
+      // near-vertical planes (dip measured from the vertical) use the image
+      f1 = (float)((double)f1 + (cr * ss * (sourc + image) + cr * cd * cs * (dblcp + cplei)));
+      f2 = (float)((double)f2 + (cr * cs * (sourc + image) - cr * cd * ss * (dblcp + cplei)));
+      f3 = (float)((double)f3 + (-cr * sd * (dblcp - cplei)));
+    } else {
+      f1 = (float)((double)f1 + (cr * ss * sourc + cr * cd * cs * dblcp));
+      f2 = (float)((double)f2 + (cr * cs * sourc - cr * cd * ss * dblcp));
+      f3 = (float)((double)f3 + (-cr * sd * dblcp));
+    }
+    f1 = (float)((double)f1 + (cd * sr * cs * dipcs + sd * sr * cs * sourc));
+    f2 = (float)((double)f2 + (-cd * sr * ss * dipcs - sd * sr * ss * sourc));
+    f3 = (float)((double)f3 + (cd * sr * sourc - sd * sr * dipcs));
+  }
+  f[of] = f1; f[fs + of] = f2; f[2 * fs + of] = f3;
+  if (i3 == 0) { t[ot] = t1; t[ts + ot] = t2; t[2 * ts + ot] = t3; }
 }
 
-int relax_model_event(relax_model_t* m, const relax_plane_t* patches, const double* slip, int n) {
-  if (!m || n < 0 || (n > 0 && (!patches || !slip))) return RELAX_ERR_ARG;
+int launch_source(const PlaneDev* P, const double* slip, int np, double mu, int n1, int n2, int h3, double dx1,
+                  double dx2, double dx3, float* f, long ld, long fs, float* t, long ts, hipStream_t st) {
+  const long N = (long)n1 * n2 * h3;
+  k_source<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(P, slip, np, mu, n1, n2, h3, dx1, dx2, dx3, f, ld, fs, t,
+                                                        ts);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
+// traction, elastic3d.f90:2517-2590: one thread per surface point, the
+// loads subtracted in their order, each rounded to FP32 as the reference's
+// REAL() does
+__global__ void k_traction(const relax_source_t* __restrict__ l, int nl, int n1, int n2, double dx1, double dx2,
+                           double t, double Dt, int rate, float* t3, long ld) {
+  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (o >= (long)n1 * n2) return;
+  const int i1 = o % n1, i2 = o / n1;
+  const double x1 = d_shifted(i1, n1, dx1), x2 = d_shifted(i2, n2, dx2);
+  float v = t3[i1 + ld * i2];
+  for (int k = 0; k < nl; ++k) {
+    const relax_source_t& L = l[k];
+    const double Lp = L.length * (1.0 + 2.0 * L.beta) / 2.0, Wp = L.width * (1.0 + 2.0 * L.beta) / 2.0;
+    if (Lp == 0.0 || Wp == 0.0) continue;
+    if ((fabs(x1 - L.x) > fmax(Lp, Wp)) || (fabs(x2 - L.y) > fmax(Lp, Wp))) continue;
+    const double amp = d_omega((x1 - L.x) / L.length, L.beta) * d_omega((x2 - L.y) / L.width, L.beta) * L.slip;
+    if (rate) {
+      if (L.period != 0.0)
+        v = (float)((double)v - amp * (sin(2 * RELAX_PI * (t + Dt) / L.period + L.phase) -
+                                       sin(2 * RELAX_PI * t / L.period + L.phase)) / Dt);
+    } else if (L.period == 0.0) {
+      v = (float)((double)v - amp);
+    } else {
+      v = (float)((double)v - amp * sin(2 * RELAX_PI * t / L.period + L.phase));
+    }
+  }
+  t3[i1 + ld * i2] = v;
+}
+
+int launch_traction(const relax_source_t* l, int nl, int n1, int n2, double dx1, double dx2, double t, double Dt,
+                    int rate, float* t3, long ld, hipStream_t st) {
+  const long N = (long)n1 * n2;
+  k_traction<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(l, nl, n1, n2, dx1, dx2, t, Dt, rate, t3, ld);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
+// A SOURCE_STRUCT patch in device form (its own taper beta)
+static PlaneDev source_plane(const relax_source_t& s, double dx1, double dx3) {
+  const relax_plane_t p{s.x, s.y, s.z, s.width, s.length, s.strike, s.dip, s.rake, 0, 0};
+  return make_plane(p, s.beta, dx1, dx3);
+}
+
+int relax_model_event(relax_model_t* m, const relax_source_t* s, int ns, const relax_source_t* l, int nl,
+                      double nyquist) {
+  if (!m || ns < 0 || nl < 0 || (ns > 0 && !s) || (nl > 0 && !l)) return RELAX_ERR_ARG;
   if (int rc = wait_batch(m)) return rc;
   int rc;
   if ((rc = sync_md(m))) return rc;
-  const long H = m->g.H;
-  // the event's moment density into sig, which is recomputed at the end
-  HIP_TRY(hipMemsetAsync(m->sig, 0, 6 * H * 4, m->stream));
-  if ((rc = shear_into(m, patches, slip, n, m->sig))) return rc;
-  // its body force and surface traction, u += G[f, t] (relax.f90:868-880,
-  // with the FIR divergence as in the set-up), tau += dtau, then
-  // sig = -tau + C:eps(u) (:887-889)
-  ModelDev tmp = m->md_host;
-  tmp.tau = m->sig;
+  const Grid& g = m->g;
+  const long H = g.H;
+  // dislocations (relax.f90:1116-1289): the source() body force of the
+  // patches longer or wider than nyquist*min(dx) (:1140-1150), then every
+  // patch's moment density into tau (:1199-1206).  The staged list holds the
+  // large patches first and then all of them.
+  const double dmin = std::min(m->desc.dx1, std::min(m->desc.dx2, m->desc.dx3)) * nyquist;
+  std::vector<PlaneDev>& hp = m->shear_host;
+  std::vector<double> slip;
+  hp.clear();
+  for (int k = 0; k < ns; ++k)
+    if (dmin < s[k].length || dmin < s[k].width) {
+      hp.push_back(source_plane(s[k], m->desc.dx1, m->desc.dx3));
+      slip.push_back(s[k].slip);
+    }
+  const int nbig = (int)hp.size();
+  for (int k = 0; k < ns; ++k) {
+    hp.push_back(source_plane(s[k], m->desc.dx1, m->desc.dx3));
+    slip.push_back(s[k].slip);
+  }
+  HIP_TRY(hipMemsetAsync(m->W, 0, 3 * H * 4, m->stream));
+  HIP_TRY(hipMemsetAsync(m->T, 0, 3L * g.n1 * g.n2 * 4, m->stream));
+  if (!hp.empty() && (rc = stage_planes(m, slip.data()))) return rc;
+  if (nbig && (rc = launch_source(m->shear_dev, m->shear_slip, nbig, m->desc.mu, g.n1, g.n2, g.h3, m->desc.dx1,
+                                  m->desc.dx2, m->desc.dx3, m->W, g.n1, H, m->T, (long)g.n1 * g.n2, m->stream)))
+    return rc;
+  // the surface loads at the model's time (relax.f90:319, :874)
+  if (nl > 0) {
+    if (nl > m->loads_cap) {
+      if (m->loads_dev) (void)hipFree(m->loads_dev);
+      m->loads_dev = nullptr;
+      m->loads_cap = 0;
+      HIP_TRY(hipMalloc(&m->loads_dev, sizeof(relax_source_t) * nl));
+      m->loads_cap = nl;
+    }
+    HIP_TRY(hipMemcpyAsync(m->loads_dev, l, sizeof(relax_source_t) * nl, hipMemcpyHostToDevice, m->stream));
+    double t = 0.0;
+    HIP_TRY(hipMemcpyAsync(&t, &m->sc->t, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    if ((rc = launch_traction(m->loads_dev, nl, g.n1, g.n2, m->desc.dx1, m->desc.dx2, t, 0.0, 0,
+                              m->T + 2L * g.n1 * g.n2, g.n1, m->stream)))
+      return rc;
+  }
+  // u += G[f, t] (relax.f90:362-371, :877-884), tau, sig = -tau + C:eps(u)
   Ctx c = single(m);
   c.prof = nullptr;
-  if ((rc = launch_bodyforce(m->g, m->S, c.md, &tmp, 1, RHEO_VISCOUS, BODY_RAW, m->stream))) return rc;
   if ((rc = model_green(c, EPI_ADD))) return rc;
-  k_add_planes<<<grid_1d(6 * H), 256, 0, m->stream>>>(m->tau, m->sig, 6 * H);
-  HIP_TRY(hipGetLastError());
-  if ((rc = launch_stress(STRESS_SET, m->g, m->S, c.md, c.one, 1, RHEO_VISCOUS, m->stream))) return rc;
+  if ((rc = add_shear(m, nbig, ns))) return rc;
+  if ((rc = launch_stress(STRESS_SET, g, m->S, c.md, c.one, 1, RHEO_VISCOUS, m->stream))) return rc;
   HIP_TRY(hipStreamSynchronize(m->stream));
   return RELAX_OK;
 }

@@ -48,6 +48,18 @@ __device__ __forceinline__ double d_omega(double x, double beta) {
   }
   return 0.0;
 }
+// the derivatives gaussp and omegap, elastic3d.f90:537-542, :570-581
+__device__ __forceinline__ double d_gaussp(double x, double sigma) {
+  const double r = x / sigma;
+  return -x * exp(-0.5 * r * r) / sqrt(RELAX_PI2) / (sigma * sigma * sigma);
+}
+__device__ __forceinline__ double d_omegap(double x, double beta) {
+  const double ax = fabs(x);
+  if (ax > (1.0 - 2.0 * beta) / (1.0 - beta) / 2.0 && ax < 1.0 / (1 - beta) / 2.0)
+    return -copysign(1.0, x) * RELAX_PI * (1.0 - beta) / 2.0 / beta *
+           sin(RELAX_PI * ((1.0 - beta) * ax - 0.5 + beta) / beta);
+  return 0.0;
+}
 __device__ __forceinline__ double d_my_sinh(double x) {
   if (fabs(x) > 11.0) return copysign(sinh(11.0), x);
   return sinh(x);

@@ -3,7 +3,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, f32, load, ptr
+from ._lib import SOURCE, check, f32, load, ptr
 
 
 def stressupdate(v1, v2, v3, lam, mu, dx1, dx2, dx3, sig, align_data=1):
@@ -44,6 +44,30 @@ def tensoramplitude(t, dx1, dx2, dx3):
     out = np.zeros(1, np.float64)
     check(load().relax_tensoramplitude(ptr(f32(t, "t")), t.size // 6, dx1, dx2, dx3, ptr(out)), "tensoramplitude")
     return float(out[0])
+
+
+def source(mu, slip, x, y, z, L, W, strike, dip, rake, beta, dx1, dx2, dx3, f1, f2, f3, t1, t2, t3, align_data=1):
+    """source, elastic3d.f90:2064-2252: adds the body force of a tapered shear
+    dislocation to the top half of f1..f3 ((sx3, sx2, sx1+2) with
+    align_data=1, else (sx3, sx2, sx1)) and its surface traction to t1..t3
+    ((sx2, sx1+2) or (sx2, sx1)), in place.  L is the down-dip extent (the
+    patch width) and W the along-strike one (its length)."""
+    sx3, sx2, p1 = f1.shape
+    sx1 = p1 - 2 if align_data else p1
+    check(load().relax_source(mu, slip, x, y, z, L, W, strike, dip, rake, beta, sx1, sx2, sx3, dx1, dx2, dx3,
+                              ptr(f32(f1, "f1")), ptr(f32(f2, "f2")), ptr(f32(f3, "f3")), ptr(f32(t1, "t1")),
+                              ptr(f32(t2, "t2")), ptr(f32(t3, "t3")), int(bool(align_data))), "source")
+
+
+def traction(mu, loads, dx1, dx2, t, Dt, t3, rate=False, align_data=1):
+    """traction, elastic3d.f90:2517-2590: subtracts the surface loads (SOURCE
+    array) at time t, or their rate over [t, t+Dt], from t3 ((sx2, sx1+2)
+    with align_data=1, else (sx2, sx1)), in place."""
+    sx2, p1 = t3.shape
+    sx1 = p1 - 2 if align_data else p1
+    ld = np.ascontiguousarray(loads, SOURCE)
+    check(load().relax_traction(mu, ptr(ld), len(ld), sx1, sx2, dx1, dx2, t, Dt, ptr(f32(t3, "t3")), int(rate),
+                                int(bool(align_data))), "traction")
 
 
 def momentdensityshear(mu, slip, x, y, z, L, W, strike, dip, rake, beta, dx1, dx2, dx3, sig):

@@ -15,6 +15,7 @@
 !   viscouseigenstress                 viscoelastic3d.f90:217-314
 !   transienteigenstress               viscoelastic3d.f90:123-202
 !   builddgammadot0                    viscoelastic3d.f90:320-435
+!   source, traction                   elastic3d.f90:2064-2252, :2517-2590
 !   frictioneigenstress                friction3d.f90:82-241
 !   monitorfriction, frictionadd       friction3d.f90:263-453
 ! A STOP of the reference becomes a non-zero relax_hip_status and a message
@@ -27,6 +28,10 @@
 !-----------------------------------------------------------------------
 MODULE relax_hip_types
   IMPLICIT NONE
+  TYPE SOURCE_STRUCT
+     SEQUENCE
+     REAL*8 :: slip,x,y,z,width,length,strike,dip,rake,period,phase,beta,opening
+  END TYPE SOURCE_STRUCT
   TYPE LAYER_STRUCT
      SEQUENCE
      REAL*8 :: z,gammadot0,stressexponent,cohesion,friction,Gk
@@ -189,6 +194,19 @@ MODULE relax_hip
        TYPE(c_ptr), VALUE :: st,sig,eik,mom,eikdot,mw,dg
        INTEGER(c_int), VALUE :: nst,sx1,sx2,sx3
      END FUNCTION
+     INTEGER(c_int) FUNCTION c_source(mu,s,x,y,z,L,W,strike,dip,rake,beta,sx1,sx2,sx3,dx1,dx2,dx3, &
+          f1,f2,f3,t1,t2,t3,al) BIND(C,name="relax_source")
+       IMPORT :: c_int, c_double, c_ptr
+       REAL(c_double), VALUE :: mu,s,x,y,z,L,W,strike,dip,rake,beta,dx1,dx2,dx3
+       INTEGER(c_int), VALUE :: sx1,sx2,sx3,al
+       TYPE(c_ptr), VALUE :: f1,f2,f3,t1,t2,t3
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_traction(mu,l,nl,sx1,sx2,dx1,dx2,t,Dt,t3,rate,al) BIND(C,name="relax_traction")
+       IMPORT :: c_int, c_double, c_ptr
+       REAL(c_double), VALUE :: mu,dx1,dx2,t,Dt
+       INTEGER(c_int), VALUE :: nl,sx1,sx2,rate,al
+       TYPE(c_ptr), VALUE :: l,t3
+     END FUNCTION
      INTEGER(c_int) FUNCTION c_builddg(sx1,sx2,sx3,dx1,dx2,dx3,beta,nz,zones,dg) &
           BIND(C,name="relax_builddgammadot0")
        IMPORT :: c_int, c_double, c_ptr
@@ -275,10 +293,11 @@ MODULE relax_hip
        TYPE(c_ptr), VALUE :: h, patches, slip
        INTEGER(c_int), VALUE :: n
      END FUNCTION
-     INTEGER(c_int) FUNCTION c_model_event(h,patches,slip,n) BIND(C,name="relax_model_event")
-       IMPORT :: c_int, c_ptr
-       TYPE(c_ptr), VALUE :: h,patches,slip
-       INTEGER(c_int), VALUE :: n
+     INTEGER(c_int) FUNCTION c_model_event(h,s,ns,l,nl,nyquist) BIND(C,name="relax_model_event")
+       IMPORT :: c_int, c_double, c_ptr
+       TYPE(c_ptr), VALUE :: h,s,l
+       INTEGER(c_int), VALUE :: ns,nl
+       REAL(c_double), VALUE :: nyquist
      END FUNCTION
      INTEGER(c_int) FUNCTION c_model_coseismic(h) BIND(C,name="relax_model_coseismic")
        IMPORT :: c_int, c_ptr
@@ -452,6 +471,44 @@ CONTAINS
     CALL check(c_builddg(sx1,sx2,sx3,dx1,dx2,dx3,beta,nz,pz,C_LOC(dgammadot0)),"builddgammadot0")
   END SUBROUTINE builddgammadot0
 
+  SUBROUTINE source(mu,s,x,y,z,L,W,strike,dip,rake,beta,sx1,sx2,sx3,dx1,dx2,dx3,f1,f2,f3,t1,t2,t3)
+    INTEGER, INTENT(IN) :: sx1,sx2,sx3
+    REAL*8, INTENT(IN) :: mu,s,x,y,z,L,W,strike,dip,rake,beta,dx1,dx2,dx3
+    REAL*4, INTENT(INOUT), DIMENSION(*), TARGET :: f1,f2,f3,t1,t2,t3
+    CALL check(c_source(mu,s,x,y,z,L,W,strike,dip,rake,beta,sx1,sx2,sx3,dx1,dx2,dx3, &
+         C_LOC(f1),C_LOC(f2),C_LOC(f3),C_LOC(t1),C_LOC(t2),C_LOC(t3),relax_hip_align),"source")
+  END SUBROUTINE source
+
+  !> traction (elastic3d.f90:2517-2590) over a list of surface loads
+  SUBROUTINE relax_hip_traction(mu,l,sx1,sx2,dx1,dx2,t,Dt,t3,rate)
+    TYPE(SOURCE_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: l
+    INTEGER, INTENT(IN) :: sx1,sx2
+    REAL*8, INTENT(IN) :: mu,dx1,dx2,t,Dt
+    REAL*4, INTENT(INOUT), DIMENSION(*), TARGET :: t3
+    LOGICAL, INTENT(IN), OPTIONAL :: rate
+    INTEGER(c_int) :: r
+    TYPE(c_ptr) :: pl
+    r=0
+    IF (PRESENT(rate)) THEN
+       IF (rate) r=1
+    END IF
+    pl=C_NULL_PTR
+    IF (SIZE(l) .GT. 0) pl=C_LOC(l)
+    CALL check(c_traction(mu,pl,INT(SIZE(l),c_int),sx1,sx2,dx1,dx2,t,Dt,C_LOC(t3),r,relax_hip_align),"traction")
+  END SUBROUTINE relax_hip_traction
+
+#ifdef RELAX_REFERENCE_TYPES
+  SUBROUTINE traction(mu,e,sx1,sx2,dx1,dx2,t,Dt,t3,rate)
+    TYPE(EVENT_STRUC), INTENT(IN) :: e
+    INTEGER, INTENT(IN) :: sx1,sx2
+    REAL*8, INTENT(IN) :: mu,dx1,dx2,t,Dt
+    REAL*4, INTENT(INOUT), DIMENSION(*), TARGET :: t3
+    LOGICAL, INTENT(IN), OPTIONAL :: rate
+    IF (e%nl .LE. 0) RETURN
+    CALL relax_hip_traction(mu,e%l(1:e%nl),sx1,sx2,dx1,dx2,t,Dt,t3,rate)
+  END SUBROUTINE traction
+#endif
+
   SUBROUTINE frictioneigenstress(x,y,z,L,W,strike,dip,rake,beta, &
        sig,prestress,mu,structure,sx1,sx2,sx3,dx1,dx2,dx3,moment,maxwelltime,vel)
     INTEGER, INTENT(IN) :: sx1,sx2,sx3
@@ -606,8 +663,8 @@ CONTAINS
     CALL check(c_model_step(model%h,pd,C_LOC(info)),"relax_model_step")
   END SUBROUTINE relax_hip_model_step
 
-  !> momentdensityshear of each patch (internal convention) into tau, as
-  !! dislocations does (relax.f90:1199-1206), then the coseismic u and sigma
+  !> momentdensityshear of each patch (internal convention) into tau, then
+  !! the elastic u and sigma of that tau (FIR divergence as body force)
   SUBROUTINE relax_hip_model_coseismic(model,patches,slip)
     TYPE(relax_model), INTENT(IN) :: model
     TYPE(relax_plane_c), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: patches
@@ -617,14 +674,18 @@ CONTAINS
     CALL check(c_model_coseismic(model%h),"relax_model_coseismic")
   END SUBROUTINE relax_hip_model_coseismic
 
-  !> a later event, once a step has reached its time (info%e advanced): the
-  !! event switch of relax.f90:858-889
-  SUBROUTINE relax_hip_model_event(model,patches,slip)
+  !> a coseismic event: the first at t = 0 (relax.f90:316-371) or a later
+  !! one once a step has reached its time (info%e advanced, the event switch
+  !! of relax.f90:858-889): dislocations, traction, Green, stress
+  SUBROUTINE relax_hip_model_event(model,s,l,nyquist)
     TYPE(relax_model), INTENT(IN) :: model
-    TYPE(relax_plane_c), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: patches
-    REAL*8, DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: slip
-    CALL check(c_model_event(model%h,C_LOC(patches),C_LOC(slip),INT(SIZE(patches),c_int)), &
-         "relax_model_event")
+    TYPE(SOURCE_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: s,l
+    REAL*8, INTENT(IN) :: nyquist
+    TYPE(c_ptr) :: ps,pl
+    ps=C_NULL_PTR; pl=C_NULL_PTR
+    IF (SIZE(s) .GT. 0) ps=C_LOC(s)
+    IF (SIZE(l) .GT. 0) pl=C_LOC(l)
+    CALL check(c_model_event(model%h,ps,INT(SIZE(s),c_int),pl,INT(SIZE(l),c_int),nyquist),"relax_model_event")
   END SUBROUTINE relax_hip_model_event
 
 END MODULE relax_hip

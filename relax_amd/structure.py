@@ -4,7 +4,7 @@ import math
 
 import numpy as np
 
-from ._lib import LAYER, PLANE, WEAK
+from ._lib import LAYER, PLANE, SOURCE, WEAK
 
 DEG2RAD = 0.01745329251994329547437168059786927
 
@@ -87,6 +87,27 @@ def patches(deck):
         out[k] = (x2, y2, z2, W, L, st2, dp2, rk2, 0, 0)
         slips[k] = s2
     return out, slips
+
+
+def sources(deck, beta):
+    """Convert deck fault lines (slip, x, y, z, length, width, strike, dip,
+    rake) to the SOURCE array of an event as the parser leaves it
+    (input.f90:1297-1370: Wang's convention, the default taper beta)."""
+    out = np.zeros(len(deck), SOURCE)
+    for k, (s, x, y, z, L, W, st, dp, rk) in enumerate(deck):
+        s2, x2, y2, z2, st2, dp2, rk2 = wangconvention(s, x, y, z, L, W, st, dp, rk)
+        out[k] = (s2, x2, y2, z2, W, L, st2, dp2, rk2, 0.0, 0.0, beta, 0.0)
+    return out
+
+
+def loads(deck, beta):
+    """Convert deck surface-load lines (x, y, length, width, t3, period,
+    phase) to the SOURCE array of an event (input.f90:1686-1716, no frame
+    rotation, the default taper beta)."""
+    out = np.zeros(len(deck or ()), SOURCE)
+    for k, (x, y, L, W, amp, period, phase) in enumerate(deck or ()):
+        out[k] = (amp, x, y, 0.0, W, L, 0.0, 0.0, 0.0, period, phase, beta, 0.0)
+    return out
 
 
 def friction_plane(x, y, z, length, width, strike, dip, rake, dx2, dx3):

@@ -27,6 +27,7 @@ from .model import FAULTCREEP, LINEAR, LTRANSIENT, NLTRANSIENT, POWERLAW, Model
 MU = LAM = 3.0e4          # MPa
 GAM = 8.33e-4             # Cowling coefficient, 1/km
 BETA = 0.2                # taper width
+NYQUIST = 1.0             # patches up to nyquist*dx get no source() body force
 SEC_PER_YR = 3.15576e7
 SEED0 = 20261202
 
@@ -73,9 +74,11 @@ def spec(name):
         ytop = -16 * math.cos(math.radians(dip))
         plane = dict(x=-32.0, y=ytop, z=ztop, length=64.0, width=32.0, strike=0.0, dip=dip, rake=90.0)
         nonlin = [(30.0, gdot0(1e18, 3.5), 3.5, 0.0, 0.6, 0.0)]
+        # nyquist 0.5: the 1 km patches on the 1 km grid are given the
+        # source() body force
         return dict(name=name, sx1=256, sx2=256, sx3=128, dx=1.0, linear=None, nonlinear=nonlin,
                     friction=[(0.0, 1e-3, 1.0, 0.0, 0.6, 0.0)], planes=[plane], deck=deck,
-                    interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=30)
+                    interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=30, nyquist=0.5)
     if name.startswith("c3"):
         idx = int(name[3:]) if len(name) > 3 else 0
         rng = np.random.default_rng(SEED0 + 3 + 1000 * idx)
@@ -127,8 +130,8 @@ def planes(s):
 
 def event_times(s):
     """The spec's event times: the set-up at t = 0, then its later events
-    (s["events"]: a list of (time, deck))."""
-    return (0.0,) + tuple(float(t) for t, _ in s.get("events", ()))
+    (s["events"]: a list of (time, deck) or (time, deck, load deck))."""
+    return (0.0,) + tuple(float(ev[0]) for ev in s.get("events", ()))
 
 
 def build(s):
@@ -152,9 +155,20 @@ def run(m, s, nsteps, dt_replay=None):
         e0 = recs[-1]["e"] if recs else 0
         recs.extend(got)
         if got[-1]["e"] != e0:
-            patches, slips = structure.patches(s["events"][got[-1]["e"] - 1][1])
-            m.event(patches, slips)
+            # the step's event index is 0-based; event_sources counts from 1
+            m.event(*event_sources(s, got[-1]["e"] + 1), nyquist=s.get("nyquist", NYQUIST))
     return recs
+
+
+def event_sources(s, e):
+    """(shear patches, surface loads) of event e (1 = the set-up at t = 0) as
+    SOURCE arrays."""
+    if e == 1:
+        deck, loads = s["deck"], s.get("loads")
+    else:
+        ev = s["events"][e - 2]
+        deck, loads = ev[1], ev[2] if len(ev) > 2 else None
+    return structure.sources(deck, BETA), structure.loads(loads, BETA)
 
 
 def configure(m, s):
@@ -174,7 +188,6 @@ def configure(m, s):
     if pl is not None:
         m.set_planes(pl)
     m.set_structure(FAULTCREEP, t["friction"])
-    patches, slips = structure.patches(s["deck"])
-    m.add_shear(patches, slips)
-    m.coseismic()
+    # the first event at t = 0 (relax.f90:316-371)
+    m.event(*event_sources(s, 1), nyquist=s.get("nyquist", NYQUIST))
     return m

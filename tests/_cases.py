@@ -2,7 +2,7 @@
 import numpy as np
 
 import oracle as O
-from relax_amd import structure, workloads
+from relax_amd import workloads
 from relax_amd.model import LINEAR, LTRANSIENT, NLTRANSIENT, POWERLAW
 
 
@@ -34,7 +34,7 @@ def small_spec(kind, sx=64, sx3=64, dx=1.0, seed=0):
                      width=12.0, strike=0.0, dip=dip, rake=90.0)
         return dict(name="afterslip", sx1=sx, sx2=sx, sx3=sx3, dx=dx, linear=None,
                     nonlinear=[(12.0, 4e7, 3.5, 0.0, 0.6, 0.0)], friction=[(0.0, 1e-3, 1.0, 0.0, 0.6, 0.0)],
-                    planes=[plane], deck=deck, interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=4)
+                    planes=[plane], deck=deck, interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=4, nyquist=0.5)
     if kind == "transient":
         # linear Maxwell below 20 km, linear Burgers transient below 6 km and
         # a power-law Burgers transient below 12 km, with weak zones on the
@@ -63,11 +63,11 @@ def oracle_model(s):
              faultcreep=t["friction"], prestress=t["prestress"], interval=s["interval"], odt=s["odt"],
              tscale=s["tscale"], skip=s["skip"], event_times=list(workloads.event_times(s)),
              ltransient=t["ltransient"], nltransient=t["nltransient"])
+    m["nyquist"] = s.get("nyquist", workloads.NYQUIST)
     if s.get("events"):
-        m["event_patches"] = []
-        for _, deck in s["events"]:
-            pa, sl = structure.patches(deck)
-            m["event_patches"].append((pa.astype(O.PLANE), sl))
+        srcs = [workloads.event_sources(s, e) for e in range(2, len(s["events"]) + 2)]
+        m["event_sources"] = [a.astype(O.SOURCE) for a, _ in srcs]
+        m["event_loads"] = [b.astype(O.SOURCE) for _, b in srcs]
     h3 = s["sx3"] // 2
     keys = {LINEAR: "dg_linear", POWERLAW: "dg_nonlinear", LTRANSIENT: "dg_ltransient",
             NLTRANSIENT: "dg_nltransient"}
@@ -82,19 +82,18 @@ def oracle_model(s):
 
 
 def oracle_initial_state(s, dtype=np.float32):
-    """tau from momentdensityshear of the deck patches, u from the Green's
-    function of its body force, sig = -tau + C:eps(u) -- the same set-up the
-    model's add_shear/coseismic perform on the device."""
+    """The first event at t = 0 (relax.f90:316-371): dislocations and traction
+    of the deck's patches and loads, u from the Green's function of their
+    body force, sig = -tau + C:eps(u) -- the set-up the model's event performs
+    on the device."""
     sx1, sx2, sx3, dx = s["sx1"], s["sx2"], s["sx3"], s["dx"]
     h3 = sx3 // 2
     tau = np.zeros((h3, sx2, sx1, 6), dtype)
-    pa, slips = structure.patches(s["deck"])
-    for p, sl in zip(pa, slips):
-        O.momentdensityshear(workloads.MU, sl, p["x"], p["y"], p["z"], p["width"], p["length"], p["strike"],
-                             p["dip"], p["rake"], workloads.BETA, dx, dx, dx, tau)
     c = [np.zeros((sx3, sx2, sx1 + 2), dtype) for _ in range(3)]
     t = [np.zeros((sx2, sx1 + 2), dtype) for _ in range(3)]
-    O.equivalentbodyforce(tau, dx, dx, dx, c[0][:h3], c[1][:h3], c[2][:h3], t[0], t[1], t[2])
+    src, loads = workloads.event_sources(s, 1)
+    O.dislocations(src.astype(O.SOURCE), loads.astype(O.SOURCE), workloads.MU, s.get("nyquist", workloads.NYQUIST),
+                   0.0, dx, dx, dx, c, t, tau)
     O.greenfunctioncowling(c[0], c[1], c[2], t[0], t[1], t[2], dx, dx, dx, workloads.LAM, workloads.MU,
                            workloads.GAM)
     u = [np.ascontiguousarray(a[:h3]) for a in c]

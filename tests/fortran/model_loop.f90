@@ -1,7 +1,7 @@
 ! Fortran driver of the model-loop test: sets up a model through the
 ! device-resident part of MODULE relax_hip as relax.f90 would (structure,
-! prestress, coseismic slip), runs RK2 steps, and writes the step scalars and
-! the fields to a stream file.
+! prestress, the first event: one shear patch and one surface load), runs RK2
+! steps, and writes the step scalars and the fields to a stream file.
 PROGRAM model_loop
   USE relax_hip
   IMPLICIT NONE
@@ -12,8 +12,7 @@ PROGRAM model_loop
   TYPE(relax_model) :: m
   TYPE(LAYER_STRUCT), TARGET :: lin(h3)
   TYPE(TENSOR_LAYER_STRUCT), TARGET :: pre(h3)
-  TYPE(relax_plane_c), TARGET :: pat(1)
-  REAL*8, TARGET :: slip(1)
+  TYPE(SOURCE_STRUCT), TARGET :: pat(1), load(1)
   TYPE(relax_step_info) :: info(nsteps)
   REAL*4, ALLOCATABLE :: u1(:,:,:),u2(:,:,:),u3(:,:,:)
   TYPE(TENSOR), ALLOCATABLE :: tau(:,:,:)
@@ -31,10 +30,11 @@ PROGRAM model_loop
   CALL relax_hip_model_set_prestress(m,pre)
   CALL relax_hip_model_set_structure(m,RELAX_LINEAR,lin)
   ! one vertical strike-slip patch, 10 km long and 6 km wide, top at 1 km
-  ! (internal convention: dip measured from the vertical, radians)
-  pat(1)=relax_plane_c(0.d0,0.d0,1.d0,6.d0,10.d0,0.d0,0.d0,0.d0,0,0)
-  slip(1)=1.d-3
-  CALL relax_hip_model_coseismic(m,pat,slip)
+  ! (internal convention: dip measured from the vertical, radians), and a
+  ! 4 x 3 km surface load of 0.5 MPa
+  pat(1)=SOURCE_STRUCT(1.d-3,0.d0,0.d0,1.d0,6.d0,10.d0,0.d0,0.d0,0.d0,0.d0,0.d0,0.2d0,0.d0)
+  load(1)=SOURCE_STRUCT(0.5d0,3.d0,-2.d0,0.d0,3.d0,4.d0,0.d0,0.d0,0.d0,0.d0,0.d0,0.2d0,0.d0)
+  CALL relax_hip_model_event(m,pat,load,1.d0)
   DO k=1,nsteps
      CALL relax_hip_model_step(m,info(k))
      IF (relax_hip_status .NE. 0) STOP 1

@@ -44,12 +44,12 @@ def test_fortran_driver_green_and_stress(tmp_path):
 @pytest.mark.gpu
 def test_fortran_model_loop_matches_python_driver(tmp_path):
     """The device-resident model driven from Fortran (create, structure,
-    coseismic patch, 3 RK2 steps, download) gives bitwise the fields and
+    first event with a patch and a load, 3 RK2 steps, download) gives bitwise the fields and
     scalars of the same model driven through the Python binding."""
     exe = os.path.join(ROOT, "tests", "fortran", "_bin", "model_loop")
     if not os.path.exists(exe):
         pytest.skip("the Fortran driver is built by __graft_entry__.build() when flang is present")
-    from relax_amd import LAYER, PLANE, TENSOR_LAYER
+    from relax_amd import LAYER, SOURCE, TENSOR_LAYER
     from relax_amd.model import LINEAR, Model
 
     out = tmp_path / "model_loop.bin"
@@ -76,10 +76,11 @@ def test_fortran_model_loop_matches_python_driver(tmp_path):
         pre["z"] = np.arange(h3)
         m.set_prestress(pre)
         m.set_structure(LINEAR, lin)
-        pat = np.zeros(1, PLANE)
-        pat[0] = (0.0, 0.0, 1.0, 6.0, 10.0, 0.0, 0.0, 0.0, 0, 0)
-        m.add_shear(pat, np.array([1e-3]))
-        m.coseismic()
+        pat = np.zeros(1, SOURCE)
+        pat[0] = (1e-3, 0.0, 0.0, 1.0, 6.0, 10.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.2, 0.0)
+        load = np.zeros(1, SOURCE)
+        load[0] = (0.5, 3.0, -2.0, 0.0, 3.0, 4.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.2, 0.0)
+        m.event(pat, load, nyquist=1.0)
         recs = [m.step() for _ in range(nsteps)]
         ref = m.download()
     np.testing.assert_array_equal(scal, [[r["Dt"], r["t"], r["power"], r["cei"]] for r in recs])

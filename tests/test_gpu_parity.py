@@ -177,6 +177,50 @@ def test_momentdensityshear():
     assert np.abs(ref).max() > 0 and rel_l2(sig, ref) < 1e-6
 
 
+@pytest.mark.parametrize("case", ["dipping", "vertical", "shallow"])
+@pytest.mark.parametrize("align", [1, 0])
+def test_source(case, align):
+    """source(): body force of the top half and surface traction, added to
+    nonzero fields; the padding columns stay as they were."""
+    sx1, sx2, sx3, dx = 64, 48, 48, 0.5
+    kw = dict(dipping=(1.3e-3, 0.4, -0.8, 6.0, 4.0, 6.0, 0.5, 0.6, 1.0),
+              vertical=(2e-3, -1.0, 0.5, 3.0, 5.0, 7.0, 1.2, 0.01, 0.3),
+              shallow=(1e-3, 0.0, 0.0, 1.0, 3.0, 5.0, -0.7, 0.9, -1.4))[case]
+    slip, x, y, z, L, W, strike, dip, rake = kw
+    rng = np.random.default_rng(5)
+    p1 = sx1 + 2 if align else sx1
+    f = [rnd(rng, sx3, sx2, p1) * 1e-3 for _ in range(3)]
+    t = [rnd(rng, sx2, p1) * 1e-3 for _ in range(3)]
+    fo = [np.zeros((sx3, sx2, sx1 + 2), np.float32) for _ in range(3)]
+    to = [np.zeros((sx2, sx1 + 2), np.float32) for _ in range(3)]
+    for a, b in zip(fo + to, f + t):
+        a[..., :sx1] = b[..., :sx1]
+    f0 = [a.copy() for a in f + t]
+    elastic3d.source(MU, slip, x, y, z, L, W, strike, dip, rake, 0.25, dx, dx, dx, *f, *t, align_data=align)
+    O.source(MU, slip, x, y, z, L, W, strike, dip, rake, 0.25, dx, dx, dx, *fo, *to)
+    for a, b, c in zip(f + t, fo + to, f0):
+        assert np.abs(b[..., :sx1] - c[..., :sx1]).max() > 0 or b.ndim == 2
+        assert rel_l2(a[..., :sx1], b[..., :sx1]) < 1e-6
+        if align:
+            np.testing.assert_array_equal(a[..., sx1:], c[..., sx1:])
+    np.testing.assert_array_equal(f[0][sx3 // 2:], f0[0][sx3 // 2:])  # the bottom half is not touched
+
+
+@pytest.mark.parametrize("rate", [False, True])
+def test_traction(rate):
+    """traction(): constant and periodic surface loads, the load and its rate."""
+    sx1, sx2, dx = 96, 64, 0.5
+    loads = structure.loads([(1.0, -2.0, 8.0, 5.0, 0.4, 0.0, 0.0), (-6.0, 3.0, 4.0, 6.0, -0.2, 2.5, 0.3),
+                             (20.0, 14.0, 6.0, 6.0, 0.1, 0.7, -1.0)], 0.3)
+    rng = np.random.default_rng(9)
+    t3 = rnd(rng, sx2, sx1 + 2)
+    ref = t3.copy()
+    elastic3d.traction(MU, loads, dx, dx, 1.7, 0.2, t3, rate=rate)
+    O.traction(loads.astype(O.SOURCE), dx, dx, 1.7, ref, Dt=0.2, rate=rate)
+    assert np.abs(ref - rnd(np.random.default_rng(9), sx2, sx1 + 2)).max() > 0.01
+    assert rel_l2(t3, ref) < 1e-6
+
+
 # ------------------------------------------------------------ rheology ----
 def layers(h3, g0=0.2, n=3.0, coh=0.1):
     st = np.zeros(h3, LAYER)
@@ -385,8 +429,15 @@ def test_rk2_adaptive_step_matches_oracle(kind):
     assert rel_l2(out["sig"], ost["sig"]) < 1e-4
 
 
-def test_coseismic_setup_matches_oracle():
+@pytest.mark.parametrize("kind", ["powerlaw", "loads"])
+def test_coseismic_setup_matches_oracle(kind):
+    """The first event on the device (dislocations, traction, Green, stress)
+    against the oracle's; "loads" adds a patch below the nyquist size (moment
+    density only) and two surface loads."""
     s = small_spec("powerlaw")
+    if kind == "loads":
+        s["deck"] = s["deck"] + [(2e-3, 3.0, -4.0, 2.0, 0.8, 0.9, 40.0, 60.0, 20.0)]
+        s["loads"] = [(2.0, 1.0, 10.0, 6.0, 0.3, 0.0, 0.0), (-8.0, -5.0, 6.0, 6.0, -0.1, 4.0, 0.5)]
     st = oracle_initial_state(s)
     g = workloads.build(s)
     out = g.download()
@@ -482,11 +533,13 @@ def test_observation_points_match_oracle():
 def test_event_switch_matches_oracle():
     """A second coseismic event at t = 1.3: integrationstep cuts Dt to reach
     it, the step's event index advances, relax_model_run stops there, and
-    relax_model_event applies the slip (tau += dtau, u += G[f(dtau)]); the
-    trajectory through and after the event follows the oracle."""
+    relax_model_event applies its patches and surface load (dislocations,
+    traction, u += G[f, t]); the trajectory through and after the event
+    follows the oracle."""
     s = small_spec("maxwell")
     rng = np.random.default_rng(77)
-    s["events"] = [(1.3, [workloads._deck_patch(rng, -8, 8, 5.0, 4.0, 2e-3) for _ in range(2)])]
+    s["events"] = [(1.3, [workloads._deck_patch(rng, -8, 8, 5.0, 4.0, 2e-3) for _ in range(2)],
+                    [(3.0, -2.0, 8.0, 6.0, 0.2, 5.0, 0.3)])]
     nsteps = 6
     st = oracle_initial_state(s)
     om = oracle_model(s)

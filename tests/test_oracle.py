@@ -35,6 +35,16 @@ Parity status per function:
   shiftedindex, exportpoints  pinned (grid points map to themselves, ties
                               round to the lower index, range; the user
                               frame at 90 degrees swaps the components)
+  source .................... pinned (first moment of the body force is the
+                              seismic moment mu s A (n b + b n); the
+                              surface traction is half the moment density's
+                              traction; the image makes a vertical
+                              strike-slip source even in its depth)
+  traction .................. pinned (plateau = -load; each taper integrates
+                              to its length; periodic load and its rate)
+  dislocations .............. pinned (the Nyquist gate: small patches add
+                              only their moment density, large ones exactly
+                              source()'s force; tau = momentdensityshear)
   event switch .............. pinned (elastic superposition: an event of
                               slip s1 at t1 on top of a set-up of slip s0
                               equals a set-up of slip s0 + s1)
@@ -1015,17 +1025,21 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
+def _sources(patches, slips):
+    out = np.zeros(len(patches), O.SOURCE)
+    for k, (p, sl) in enumerate(zip(patches, slips)):
+        out[k] = (sl, p["x"], p["y"], p["z"], p["width"], p["length"], p["strike"], p["dip"], p["rake"], 0, 0, 0.2, 0)
+    return out
+
+
 def _setup(sx, sx3, dx, patches, slips, dtype=np.float64):
-    """Coseismic set-up: tau from momentdensityshear, u = G[f(tau)],
-    sig = -tau + C:eps(u)."""
+    """The first event: dislocations (source() body force, moment density),
+    u = G[f, t], sig = -tau + C:eps(u)."""
     h3 = sx3 // 2
     tau = np.zeros((h3, sx, sx, 6), dtype)
-    for p, sl in zip(patches, slips):
-        O.momentdensityshear(MU, sl, p["x"], p["y"], p["z"], p["width"], p["length"], p["strike"], p["dip"],
-                             p["rake"], 0.2, dx, dx, dx, tau)
     c = [np.zeros((sx3, sx, sx + 2), dtype) for _ in range(3)]
     t = [np.zeros((sx, sx + 2), dtype) for _ in range(3)]
-    O.equivalentbodyforce(tau, dx, dx, dx, c[0][:h3], c[1][:h3], c[2][:h3], t[0], t[1], t[2])
+    O.dislocations(_sources(patches, slips), None, MU, 1.0, 0.0, dx, dx, dx, c, t, tau)
     O.greenfunctioncowling(c[0], c[1], c[2], t[0], t[1], t[2], dx, dx, dx, LAM, MU, 0.0)
     u = [np.ascontiguousarray(a[:h3]) for a in c]
     sig = np.ascontiguousarray(-tau)
@@ -1047,7 +1061,7 @@ def test_event_switch_superposes_elastically(mut, mutation):
     h3 = sx3 // 2
     m = dict(sx1=sx, sx2=sx, sx3=sx3, dx1=dx, dx2=dx, dx3=dx, lam=LAM, mu=MU, gam=0.0, beta=0.2,
              prestress=np.zeros(h3, O.TENSOR_LAYER), interval=100.0, odt=1.0, tscale=1.0, skip=1,
-             event_times=[0.0, 2.5], event_patches=[(patches, s1)])
+             event_times=[0.0, 2.5], event_sources=[_sources(patches, s1)], nyquist=1.0)
     st = _setup(sx, sx3, dx, patches, s0)
     mutation(mut)
     info = O.rk2_run(m, st, 1)
@@ -1059,3 +1073,156 @@ def test_event_switch_superposes_elastically(mut, mutation):
         assert err > 1e-3
     else:
         assert err < 1e-12
+
+
+# ------------------------------------------------------------- source() ---
+def _source(mut, mutation, sx=64, sx3=64, dx=0.5, **kw):
+    a = dict(slip=1.3, x=0.3, y=-0.7, z=7.0, L=4.0, W=6.0, strike=np.radians(30), dip=np.radians(40),
+             rake=np.radians(60), beta=0.3)
+    a.update(kw)
+    f = [np.zeros((sx3, sx, sx + 2)) for _ in range(3)]
+    t = [np.zeros((sx, sx + 2)) for _ in range(3)]
+    mutation(mut)
+    O.source(MU, a["slip"], a["x"], a["y"], a["z"], a["L"], a["W"], a["strike"], a["dip"], a["rake"], a["beta"],
+             dx, dx, dx, *f, *t)
+    mutation(0)
+    return a, f, t
+
+
+@pytest.mark.parametrize("case", ["dipping", "vertical"])
+def test_source_first_moment_is_the_seismic_moment(case, mutation):
+    """For a buried patch, sum x_j f_i dV over the grid is the seismic moment
+    tensor mu s L W (n b + b n) (f = -div M, integrated by parts) and the
+    net force vanishes; the residual is the quadrature of the tapers."""
+    kw = dict(dip=np.radians(40)) if case == "dipping" else dict(dip=0.0, L=8.0, z=16.0, beta=0.45)
+    sx3 = 64 if case == "dipping" else 128
+    dx = 0.5
+    a, f, t = _source(0, mutation, sx3=sx3, **kw)
+    h3, sx = sx3 // 2, 64
+    F = np.array([c[:h3, :, :sx] for c in f])
+    X1 = (np.arange(sx) - sx * (np.arange(sx) >= sx // 2)) * dx
+    xx = [X1[None, None, :], X1[None, :, None], (np.arange(h3) * dx)[:, None, None]]
+    Mf = np.array([[(xx[j] * F[i]).sum() * dx ** 3 for j in range(3)] for i in range(3)])
+    cs, ss, cd, sd = np.cos(a["strike"]), np.sin(a["strike"]), np.cos(a["dip"]), np.sin(a["dip"])
+    cr, sr = np.cos(a["rake"]), np.sin(a["rake"])
+    n = np.array([cd * cs, -cd * ss, -sd])
+    b = np.array([ss * cr + cs * sd * sr, cs * cr - ss * sd * sr, cd * sr])
+    M0 = MU * a["slip"] * a["L"] * a["W"] * (np.outer(n, b) + np.outer(b, n))
+    assert np.abs(Mf - M0).max() / np.abs(M0).max() < (1e-3 if case == "dipping" else 2e-3)
+    assert np.abs(F.sum(axis=(1, 2, 3)) * dx ** 3).max() < 1e-3 * np.abs(M0).max() / a["L"]
+    assert all(np.abs(c).max() == 0 for c in t)  # buried: no surface traction
+
+
+def test_source_surface_traction_is_half_the_moment_traction(mutation):
+    """t = (n sdyad mu s b) e3 at x3 = 0: half of momentdensityshear's
+    M e3 there (the same aperture, source and image)."""
+    sx, sx3, dx = 64, 64, 0.5
+    a, f, t = _source(0, mutation, z=2.0, dip=np.radians(30))
+    M = np.zeros((sx3 // 2, sx, sx, 6))
+    O.momentdensityshear(MU, a["slip"], a["x"], a["y"], a["z"], a["L"], a["W"], a["strike"], a["dip"], a["rake"],
+                         a["beta"], dx, dx, dx, M)
+    for i, k in enumerate((2, 4, 5)):
+        assert np.abs(M[0, :, :, k]).max() > 0
+        np.testing.assert_allclose(t[i][:, :sx], M[0, :, :, k] / 2, rtol=0, atol=1e-12 * np.abs(M).max())
+
+
+@pytest.mark.parametrize("mut", [0, 19])
+def test_source_vertical_image_symmetry(mut, mutation):
+    """A vertical strike-slip patch (dip < 2.01 degrees) near the surface adds
+    its image at -z: the strike-slip body force of depth z then equals that
+    of depth -z exactly (source and image swap)."""
+    kw = dict(z=2.0, dip=0.0, rake=0.0, L=4.0)
+    _, f1, _ = _source(mut, mutation, **kw)
+    _, f2, _ = _source(mut, mutation, **dict(kw, z=-2.0))
+    diff = max(np.abs(a - b).max() for a, b in zip(f1, f2))
+    if mut:
+        assert diff > 1e-3 * max(np.abs(a).max() for a in f1)
+    else:
+        assert diff == 0 and max(np.abs(a).max() for a in f1) > 0
+
+
+# ------------------------------------------------- traction, dislocations ---
+def _load(**kw):
+    a = dict(slip=0.7, x=1.5, y=-2.0, z=0.0, width=6.0, length=10.0, strike=0, dip=0, rake=0, period=0.0, phase=0.0,
+             beta=0.25, opening=0)
+    a.update(kw)
+    out = np.zeros(1, O.SOURCE)
+    out[0] = tuple(a[k] for k in O.SOURCE.names)
+    return out
+
+
+@pytest.mark.parametrize("mut", [0, 21])
+def test_traction_plateau_and_load_integral(mut, mutation):
+    """A constant load presses down: t3 = -amplitude on the plateau of the
+    tapers, and its sum is -amplitude L W (each raised-cosine taper
+    integrates to its length)."""
+    sx, dx = 64, 0.5
+    l = _load()
+    t3 = np.zeros((sx, sx + 2))
+    mutation(mut)
+    O.traction(l, dx, dx, 0.0, t3)
+    mutation(0)
+    i1, i2 = int(round(1.5 / dx)), sx - int(round(2.0 / dx))
+    if mut:
+        assert t3[i2, i1] == pytest.approx(0.7)
+        return
+    assert t3[i2, i1] == -0.7
+    assert t3.sum() * dx * dx == pytest.approx(-0.7 * 10.0 * 6.0, rel=1e-6)
+    assert np.all(t3[:, sx:] == 0)
+
+
+def test_traction_periodic_load_and_rate():
+    """A periodic load is amp sin(2 pi t / T + phi); the rate form is the
+    difference quotient over [t, t + Dt]."""
+    sx, dx = 32, 1.0
+    l = _load(period=3.0, phase=0.4, x=0.0, y=0.0)
+    t, Dt = 1.1, 0.25
+    a = np.zeros((sx, sx + 2))
+    O.traction(l, dx, dx, t, a)
+    b = np.zeros((sx, sx + 2))
+    O.traction(l, dx, dx, t + Dt, b)
+    r = np.zeros((sx, sx + 2))
+    O.traction(l, dx, dx, t, r, Dt=Dt, rate=True)
+    assert a[0, 0] == pytest.approx(-0.7 * np.sin(2 * np.pi * t / 3.0 + 0.4), rel=1e-14)
+    np.testing.assert_allclose(r, (b - a) / Dt, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("mut", [0, 20])
+def test_dislocations_nyquist_gate(mut, mutation):
+    """Patches up to nyquist*min(dx) in length and width get no source()
+    body force (only their moment density); larger ones get exactly
+    source()'s force and traction."""
+    sx, sx3, dx = 32, 32, 1.0
+    big = np.zeros(1, O.PLANE)
+    big[0] = (0.0, 1.0, 5.0, 4.0, 6.0, 0.3, 0.5, 1.0, 0, 0)
+    small = np.zeros(1, O.PLANE)
+    small[0] = (-3.0, -2.0, 4.0, 1.5, 2.0, 1.0, 0.2, 0.4, 0, 0)
+    h3 = sx3 // 2
+    res = {}
+    for name, pa in (("big", big), ("small", small)):
+        tau = np.zeros((h3, sx, sx, 6))
+        v = [np.zeros((sx3, sx, sx + 2)) for _ in range(3)]
+        t = [np.zeros((sx, sx + 2)) for _ in range(3)]
+        mutation(mut)
+        O.dislocations(_sources(pa, [1.0]), None, MU, 2.0, 0.0, dx, dx, dx, v, t, tau)
+        mutation(0)
+        res[name] = (v, t, tau)
+        M = np.zeros_like(tau)
+        p = pa[0]
+        O.momentdensityshear(MU, 1.0, p["x"], p["y"], p["z"], p["width"], p["length"], p["strike"], p["dip"],
+                             p["rake"], 0.2, dx, dx, dx, M)
+        np.testing.assert_array_equal(tau, M)
+    v, t, _ = res["small"]
+    if mut:
+        assert max(np.abs(a).max() for a in v) > 0
+        return
+    assert max(np.abs(a).max() for a in v + t) == 0
+    v, t, _ = res["big"]
+    f = [np.zeros((sx3, sx, sx + 2)) for _ in range(3)]
+    tt = [np.zeros((sx, sx + 2)) for _ in range(3)]
+    p = big[0]
+    O.source(MU, 1.0, p["x"], p["y"], p["z"], p["width"], p["length"], p["strike"], p["dip"], p["rake"], 0.2, dx, dx,
+             dx, *f, *tt)
+    for a, b in zip(v + t, f + tt):
+        np.testing.assert_array_equal(a, b)
+    assert max(np.abs(a).max() for a in v) > 0
