@@ -70,6 +70,19 @@ __device__ __forceinline__ void store_t6(float* __restrict__ t, long H, long o, 
   t[o] = v.s11; t[H + o] = v.s12; t[2 * H + o] = v.s13;
   t[3 * H + o] = v.s22; t[4 * H + o] = v.s23; t[5 * H + o] = v.s33;
 }
+__device__ __forceinline__ T6 load_t6_nt(const float* __restrict__ t, long H, long o) {
+  return T6{__builtin_nontemporal_load(t + o), __builtin_nontemporal_load(t + H + o),
+            __builtin_nontemporal_load(t + 2 * H + o), __builtin_nontemporal_load(t + 3 * H + o),
+            __builtin_nontemporal_load(t + 4 * H + o), __builtin_nontemporal_load(t + 5 * H + o)};
+}
+__device__ __forceinline__ void store_t6_nt(float* __restrict__ t, long H, long o, const T6& v) {
+  __builtin_nontemporal_store(v.s11, t + o);
+  __builtin_nontemporal_store(v.s12, t + H + o);
+  __builtin_nontemporal_store(v.s13, t + 2 * H + o);
+  __builtin_nontemporal_store(v.s22, t + 3 * H + o);
+  __builtin_nontemporal_store(v.s23, t + 4 * H + o);
+  __builtin_nontemporal_store(v.s33, t + 5 * H + o);
+}
 
 // Device tables of the rheology, one entry per depth of the half grid.
 struct Mechanism {

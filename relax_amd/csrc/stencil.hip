@@ -179,6 +179,11 @@ __global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, ModelDev Dv, con
 // Em = E + Dt/2 Edot (relax.f90:690-692) and the corrector advances
 // E += Dt Edot (:792-793), Edot from the mid-step strain.  pw and ce collect
 // |moment| and |tau| of the corrector.
+// The pointwise streams (tau, sig, gamma, the transient strains) are read
+// and written once per pass with non-temporal accesses, which leaves the
+// caches to the stencil's neighbour reads of u or v (0.83 -> 0.75 ms for
+// the corrector at 512x512x256).  The divergence pass keeps cached loads of
+// the stress: its tiles re-read their halo rows.
 template <int MODE, int RC>
 __device__ __forceinline__ void stress_point(const RheoParams& R, const StencilConst& S, long H, long o, int i1,
                                              int i2, int i3, T6 e, float* __restrict__ sig, float* __restrict__ tau,
@@ -193,21 +198,21 @@ __device__ __forceinline__ void stress_point(const RheoParams& R, const StencilC
   e.s22 = e.s22 * S.mu2 + S.lam * tr;
   e.s23 *= S.mu2;
   e.s33 = e.s33 * S.mu2 + S.lam * tr;
-  const T6 t = load_t6(tau, H, o);
+  const T6 t = load_t6_nt(tau, H, o);
   T6 out;
   if (MODE == STRESS_SET) {
     out = T6{-t.s11 + e.s11, -t.s12 + e.s12, -t.s13 + e.s13, -t.s22 + e.s22, -t.s23 + e.s23, -t.s33 + e.s33};
   } else if (MODE == STRESS_PRED) {
-    const T6 s = load_t6(sig, H, o);
+    const T6 s = load_t6_nt(sig, H, o);
     T6 m;
     if (RC == RHEO_EXT) {
       const bool trans = R.tr[0].active || R.tr[1].active;
-      const T6 ev = trans ? load_t6(E, H, o) : T6{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      const T6 ev = trans ? load_t6_nt(E, H, o) : T6{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       T6 ed;
       m = rheology_ext<false, false>(R, s, i1, i2, i3, o, ev, &ed, nullptr, nullptr, nullptr, &status);
       if (trans) {
         const float h = (float)(sc->Dt / 2);
-        store_t6(Em, H, o, T6{h * ed.s11 + ev.s11, h * ed.s12 + ev.s12, h * ed.s13 + ev.s13,
+        store_t6_nt(Em, H, o, T6{h * ed.s11 + ev.s11, h * ed.s12 + ev.s12, h * ed.s13 + ev.s13,
                               h * ed.s22 + ev.s22, h * ed.s23 + ev.s23, h * ed.s33 + ev.s33});
       }
     } else {
@@ -217,18 +222,18 @@ __device__ __forceinline__ void stress_point(const RheoParams& R, const StencilC
     out = T6{(c * m.s11 - t.s11) + e.s11, (c * m.s12 - t.s12) + e.s12, (c * m.s13 - t.s13) + e.s13,
              (c * m.s22 - t.s22) + e.s22, (c * m.s23 - t.s23) + e.s23, (c * m.s33 - t.s33) + e.s33};
   } else {
-    const T6 s = load_t6(sig, H, o);
+    const T6 s = load_t6_nt(sig, H, o);
     float gl = 0.f, gn = 0.f;
     const float dt = (float)sc->Dt;
     T6 m;
     if (RC == RHEO_EXT) {
       const bool trans = R.tr[0].active || R.tr[1].active;
-      const T6 em = trans ? load_t6(Em, H, o) : T6{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      const T6 em = trans ? load_t6_nt(Em, H, o) : T6{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       T6 ed;
       m = rheology_ext<false, true>(R, s, i1, i2, i3, o, em, &ed, nullptr, &gl, &gn, &status);
       if (trans) {
-        const T6 ev = load_t6(E, H, o);
-        store_t6(E, H, o, T6{ev.s11 + dt * ed.s11, ev.s12 + dt * ed.s12, ev.s13 + dt * ed.s13,
+        const T6 ev = load_t6_nt(E, H, o);
+        store_t6_nt(E, H, o, T6{ev.s11 + dt * ed.s11, ev.s12 + dt * ed.s12, ev.s13 + dt * ed.s13,
                              ev.s22 + dt * ed.s22, ev.s23 + dt * ed.s23, ev.s33 + dt * ed.s33});
       }
     } else {
@@ -236,19 +241,19 @@ __device__ __forceinline__ void stress_point(const RheoParams& R, const StencilC
     }
     const T6 tn{t.s11 + dt * m.s11, t.s12 + dt * m.s12, t.s13 + dt * m.s13,
                 t.s22 + dt * m.s22, t.s23 + dt * m.s23, t.s33 + dt * m.s33};
-    store_t6(tau, H, o, tn);
+    store_t6_nt(tau, H, o, tn);
     if (R.mech[0].active || R.mech[1].active) {
-      float gv = gam[o];
+      float gv = __builtin_nontemporal_load(gam + o);
       if (R.mech[0].active) gv = gv + dt * gl;
       if (R.mech[1].active) gv = gv + dt * gn;
-      gam[o] = gv;
+      __builtin_nontemporal_store(gv, gam + o);
     }
     out = T6{-tn.s11 + e.s11, -tn.s12 + e.s12, -tn.s13 + e.s13, -tn.s22 + e.s22, -tn.s23 + e.s23,
              -tn.s33 + e.s33};
     pw += tnorm_d(m);
     ce += tnorm_d(tn);
   }
-  store_t6(sig, H, o, out);
+  store_t6_nt(sig, H, o, out);
 }
 
 // The corrector's |moment| and |tau| sums of a workgroup, in a fixed order.
@@ -272,7 +277,9 @@ __device__ __forceinline__ void block_partials(double pw, double ce, double2* __
 // One point per thread.  (A z-marching tile form, reading X once per plane
 // through LDS, fetched 1.6x less but ran 10 % slower at 512x512x256: its
 // lower occupancy cost more than the re-reads, which the last-level cache
-// serves.)
+// serves.  64x4 tiles marched down 8 or 16 planes with the x3 neighbours in
+// registers lost the same way: 104 VGPRs against 68 in the predictor form,
+// 0.63 ms against 0.56 ms.)
 template <int MODE, int RC>
 __device__ __forceinline__ void stress_block(const Grid& g, const RheoParams& R, const StencilConst& S,
                                              const float* __restrict__ X, float* __restrict__ sig,

@@ -27,7 +27,7 @@ from .model import FAULTCREEP, LINEAR, LTRANSIENT, NLTRANSIENT, POWERLAW, Model
 MU = LAM = 3.0e4          # MPa
 GAM = 8.33e-4             # Cowling coefficient, 1/km
 BETA = 0.2                # taper width
-NYQUIST = 1.0             # patches up to nyquist*dx get no source() body force
+NYQUIST = 2.0             # patches up to nyquist*dx get no source() body force (run1.input:5)
 SEC_PER_YR = 3.15576e7
 SEED0 = 20261202
 
