@@ -21,7 +21,7 @@
 namespace {
 
 struct ColumnConsts {
-  float ratio1, ratio2;       // elasticresponse: (l+m)/(l+2m)/m/(2pi)^2, m/(l+m)
+  float ratio1, ratio2p1;     // elasticresponse: (l+m)/(l+2m)/m/(2pi)^2, 1 + m/(l+m)
   float mu2pi, mod2pi, lam2pi, grav;  // traction: 2 pi mu, 2 pi (l+2m), 2 pi l, 2 mu alpha gamma
   float alpha, inv_alpha;
   float inv_n3dx3;
@@ -33,7 +33,7 @@ __host__ ColumnConsts make_consts(const GreenConst& gc) {
   const double l = gc.lambda, m = gc.mu;
   const double alpha = (l + m) / (l + 2.0 * m);
   c.ratio1 = (float)((l + m) / (l + 2.0 * m) / m / (RELAX_PI2 * RELAX_PI2));
-  c.ratio2 = (float)(m / (l + m));
+  c.ratio2p1 = (float)(1.0 + m / (l + m));
   c.mu2pi = (float)(RELAX_PI2 * m);
   c.mod2pi = (float)(RELAX_PI2 * (l + 2.0 * m));
   c.lam2pi = (float)(RELAX_PI2 * l);
@@ -47,21 +47,18 @@ __host__ ColumnConsts make_consts(const GreenConst& gc) {
   return c;
 }
 
-// elasticresponse at one wavenumber (green.f90:63-86)
+// elasticresponse at one wavenumber (green.f90:63-86).  The reference's
+// matrix [(|k|^2 (1 + m/(l+m)) - k_i^2) delta_ij - k_i k_j (1 - delta_ij)]
+// is applied as s c - k (k.c) with s = |k|^2 (1 + m/(l+m)).
 __device__ __forceinline__ void elastic_point(const ColumnConsts& C, float k1, float k2, float k3, float2& c1,
                                               float2& c2, float2& c3) {
   const float r2 = k1 * k1 + k2 * k2 + k3 * k3;
   const float denom = C.ratio1 / (r2 * r2);
-  const float a1 = k2 * k2 + k3 * k3 + C.ratio2 * r2;
-  const float a2 = k1 * k1 + k3 * k3 + C.ratio2 * r2;
-  const float a3 = k1 * k1 + k2 * k2 + C.ratio2 * r2;
-  const float2 d1 = make_float2(k2 * c2.x + k3 * c3.x, k2 * c2.y + k3 * c3.y);
-  const float2 d2 = make_float2(k1 * c1.x + k3 * c3.x, k1 * c1.y + k3 * c3.y);
-  const float2 d3 = make_float2(k1 * c1.x + k2 * c2.x, k1 * c1.y + k2 * c2.y);
-  const float2 b1 = make_float2((a1 * c1.x - k1 * d1.x) * denom, (a1 * c1.y - k1 * d1.y) * denom);
-  const float2 b2 = make_float2((a2 * c2.x - k2 * d2.x) * denom, (a2 * c2.y - k2 * d2.y) * denom);
-  const float2 b3 = make_float2((a3 * c3.x - k3 * d3.x) * denom, (a3 * c3.y - k3 * d3.y) * denom);
-  c1 = b1; c2 = b2; c3 = b3;
+  const float s = r2 * C.ratio2p1;
+  const float2 kc = make_float2(k1 * c1.x + k2 * c2.x + k3 * c3.x, k1 * c1.y + k2 * c2.y + k3 * c3.y);
+  c1 = make_float2((s * c1.x - k1 * kc.x) * denom, (s * c1.y - k1 * kc.y) * denom);
+  c2 = make_float2((s * c2.x - k2 * kc.x) * denom, (s * c2.y - k2 * kc.y) * denom);
+  c3 = make_float2((s * c3.x - k3 * kc.x) * denom, (s * c3.y - k3 * kc.y) * denom);
 }
 
 // summand of surfacetractioncowling (green.f90:357-359)
@@ -78,21 +75,33 @@ __device__ __forceinline__ void traction_point(const ColumnConsts& C, float k1, 
   s3 = make_float2(-d.y - C.grav * c3.x, d.x - C.grav * c3.y);
 }
 
-// Per-column part of cerrutisolmodified (green.f90:571-598): everything that
-// does not depend on k3, in FP64 then rounded.
+// Per-column part of cerrutisolmodified (green.f90:571-606).  With
+// fi = 2 beta / ((2 pi k3)^2 + beta^2) and kf = 2 pi k3 fi, every term the
+// reference adds to u (:599-606) is a column constant times fi or fi kf:
+//   u_c += fi A_c + fi kf B_c,
+// with b3 and P_c = p_c / (2 mu beta^3) (B1,2 of :596-598 without fi), and
+// BK = k1 P1 + k2 P2:
+//   A_c = i k_c a b 2pi (1 - 1/a) b3 - 2 b^2 P_c + k_c a (2pi)^2 BK   (c = 1, 2)
+//   B_c = k_c a b 2pi b3 - i k_c a (2pi)^2 BK
+//   A_3 = -b^2 b3,   B_3 = i a b^2 b3 + a b 2pi BK
+// (a = alpha, b = beta).  beta, b3 and 1/(2 mu beta^3) are formed in FP64
+// and rounded; the constants are then combined in FP32 (few live registers
+// while the column's spectrum is held in them).
 struct CerrutiCol {
-  float beta, beta2, cb1;  // cb1 = 1/(2 mu beta^3)
-  float2 b3;               // k3-independent b3
-  float k1, k2;
+  float beta, beta2;
+  float2 A[3], B[3];
   int zero;                // k1 == k2 == 0: no correction
 };
 
 __device__ __forceinline__ CerrutiCol cerruti_column(const ColumnConsts& C, double mu, double alpha, double gamma,
                                                      float k1, float k2, float2 p1, float2 p2, float2 p3) {
   CerrutiCol cc;
-  cc.k1 = k1; cc.k2 = k2;
   cc.zero = (k1 == 0.f && k2 == 0.f);
-  if (cc.zero) { cc.beta = cc.beta2 = cc.cb1 = 0.f; cc.b3 = make_float2(0.f, 0.f); return cc; }
+  if (cc.zero) {
+    cc.beta = cc.beta2 = 0.f;
+    for (int c = 0; c < 3; ++c) cc.A[c] = cc.B[c] = make_float2(0.f, 0.f);
+    return cc;
+  }
   const double dk1 = k1, dk2 = k2;
   const double beta = RELAX_PI2 * sqrt(dk1 * dk1 + dk2 * dk2);
   const double h = gamma / beta;
@@ -101,38 +110,42 @@ __device__ __forceinline__ CerrutiCol cerruti_column(const ColumnConsts& C, doub
   const double qx = RELAX_PI2 * (dk1 * p1.x + dk2 * p2.x), qy = RELAX_PI2 * (dk1 * p1.y + dk2 * p2.y);
   const double b3x = (beta * p3.x + (1.0 - alpha) * qy) * den;
   const double b3y = (beta * p3.y - (1.0 - alpha) * qx) * den;
-  cc.beta = (float)beta;
-  cc.beta2 = (float)(beta * beta);
-  cc.cb1 = (float)(1.0 / (2.0 * mu * beta * beta * beta));
-  cc.b3 = make_float2((float)b3x, (float)b3y);
+  const float cb1 = (float)(1.0 / (2.0 * mu * beta * beta * beta));
+  const float fb3x = (float)b3x, fb3y = (float)b3y;
+  const float fbeta = (float)beta, b2 = (float)(beta * beta);
+  const float P1x = p1.x * cb1, P1y = p1.y * cb1, P2x = p2.x * cb1, P2y = p2.y * cb1;
+  const float BKx = k1 * P1x + k2 * P2x, BKy = k1 * P1y + k2 * P2y;
+  const float ab2pi = C.alpha * fbeta * (float)RELAX_PI2, a4pi2 = C.alpha * (float)(RELAX_PI2 * RELAX_PI2);
+  const float r = 1.f - C.inv_alpha;
+  const float kk[2] = {k1, k2}, Px[2] = {P1x, P2x}, Py[2] = {P1y, P2y};
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    // i k ab2pi r b3 = k ab2pi r (-b3y, b3x)
+    const float kr = kk[c] * ab2pi * r, ka = kk[c] * a4pi2, kb = kk[c] * ab2pi;
+    cc.A[c] = make_float2(-kr * fb3y - 2.f * b2 * Px[c] + ka * BKx, kr * fb3x - 2.f * b2 * Py[c] + ka * BKy);
+    // -i k a4pi2 BK = k a4pi2 (BKy, -BKx)
+    cc.B[c] = make_float2(kb * fb3x + ka * BKy, kb * fb3y - ka * BKx);
+  }
+  cc.A[2] = make_float2(-b2 * fb3x, -b2 * fb3y);
+  // i alpha b^2 b3 + ab2pi BK
+  const float ab2 = C.alpha * b2;
+  cc.B[2] = make_float2(-ab2 * fb3y + ab2pi * BKx, ab2 * fb3x + ab2pi * BKy);
+  cc.beta = fbeta;
+  cc.beta2 = b2;
   return cc;
 }
 
 // The k3-dependent part of cerrutisolmodified (green.f90:596-606).
 __device__ __forceinline__ void cerruti_point(const ColumnConsts& C, const CerrutiCol& cc, float2 p1, float2 p2,
                                               float k3, float2& u1, float2& u2, float2& u3) {
+  (void)C; (void)p1; (void)p2;
   if (cc.zero) return;
   const float tk3 = (float)RELAX_PI2 * k3;
   const float fi = 2.f * cc.beta / (tk3 * tk3 + cc.beta2);
-  const float t = cc.cb1 * fi;
-  const float2 b1 = cscale(p1, t), b2 = cscale(p2, t);
-  const float2 b3 = cc.b3;
-  const float kf = tk3 * fi;  // 2 pi k3 fi
-  // tmp = alpha i beta 2pi b3 (1 - 1/alpha - i kf)
-  const float2 g = make_float2(1.f - C.inv_alpha, -kf);
-  const float2 tmp = cscale(mul_i(cmul(b3, g)), C.alpha * cc.beta * (float)RELAX_PI2);
-  const float2 v1 = cscale(tmp, cc.k1 * fi), v2 = cscale(tmp, cc.k2 * fi);
-  // v3 = -alpha beta^2 b3 (1/alpha - i kf)
-  const float2 v3 = cscale(cmul(b3, make_float2(C.inv_alpha, -kf)), -C.alpha * cc.beta2);
-  // common: alpha (2pi)^2 (b1 k1 + b2 k2) (1 - i kf)
-  const float2 bk = make_float2(b1.x * cc.k1 + b2.x * cc.k2, b1.y * cc.k1 + b2.y * cc.k2);
-  const float2 com = cscale(cmul(bk, make_float2(1.f, -kf)), C.alpha * (float)(RELAX_PI2 * RELAX_PI2));
-  const float tb2 = 2.f * cc.beta2;
-  u1 = cadd(u1, cadd(v1, make_float2(-tb2 * b1.x + cc.k1 * com.x, -tb2 * b1.y + cc.k1 * com.y)));
-  u2 = cadd(u2, cadd(v2, make_float2(-tb2 * b2.x + cc.k2 * com.x, -tb2 * b2.y + cc.k2 * com.y)));
-  // u3 = v3 fi + alpha beta 2pi (k1 b1 + k2 b2) 2pi k3 fi
-  const float s = C.alpha * cc.beta * (float)RELAX_PI2 * kf;
-  u3 = cadd(u3, make_float2(v3.x * fi + s * bk.x, v3.y * fi + s * bk.y));
+  const float fk = fi * (tk3 * fi);  // fi kf
+  u1 = make_float2(u1.x + (fi * cc.A[0].x + fk * cc.B[0].x), u1.y + (fi * cc.A[0].y + fk * cc.B[0].y));
+  u2 = make_float2(u2.x + (fi * cc.A[1].x + fk * cc.B[1].x), u2.y + (fi * cc.A[1].y + fk * cc.B[1].y));
+  u3 = make_float2(u3.x + (fi * cc.A[2].x + fk * cc.B[2].x), u3.y + (fi * cc.A[2].y + fk * cc.B[2].y));
 }
 
 }  // namespace
