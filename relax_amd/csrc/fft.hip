@@ -96,7 +96,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
     const int b = e / M, j = e % M;
     const long row = rowof(b);
     float2 v = make_float2(0.f, 0.f);
-    if (row < nrows) v = *reinterpret_cast<const float2*>(in + row * in_pitch + 2 * j);
+    if (row < nrows) v = gload(reinterpret_cast<const float2*>(in + row * in_pitch + 2 * j));
     s[b * PM + j] = v;
   }
   __syncthreads();
@@ -115,8 +115,8 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
     const float2 wd = mul_mi(cmul(w, dif));
     X[i] = make_float2(0.5f * (sum.x + wd.x) * scale, 0.5f * (sum.y + wd.y) * scale);
     const long row = rowof(b);
-    if (k == 0 && row < nrows) nyq[row * nyq_stride] = make_float2((zk.x - zk.y) * scale, 0.f);
-    if (!YS && row < nrows) out[row * out_pitch + k] = X[i];
+    if (k == 0 && row < nrows) gstore(nyq + row * nyq_stride, make_float2((zk.x - zk.y) * scale, 0.f));
+    if (!YS && row < nrows) gstore(out + row * out_pitch + k, X[i]);
   }
   if (!YS) return;
   __syncthreads();
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
     dft8<-1>(a);
     const long base_row = (q / L2) * n2 + (q % L2);
 #pragma unroll
-    for (int t = 0; t < 8; ++t) out[(base_row + (long)L2 * t) * out_pitch + k] = t ? cmul(a[t], ytw[gi][t]) : a[t];
+    for (int t = 0; t < 8; ++t) gstore(out + (base_row + (long)L2 * t) * out_pitch + k, t ? cmul(a[t], ytw[gi][t]) : a[t]);
   }
 }
 
@@ -181,13 +181,13 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long 
     const int b = e / M, j = e % M;
     const long row = rowof(b);
     float2 v = make_float2(0.f, 0.f);
-    if (row < nrows) v = in[row * in_pitch + j];
+    if (row < nrows) v = gload(in + row * in_pitch + j);
     if (!YS && j == 0) v.y = 0.f;  // a real output takes the real part of the k1=0 bin
     s[b * PM + j] = v;
   }
   for (int b = tid; b < RPB; b += NT) {
     const long row = rowof(b);
-    xm[b] = row < nrows ? make_float2(nyq[row * nyq_stride].x, 0.f) : make_float2(0.f, 0.f);
+    xm[b] = row < nrows ? make_float2(gload(nyq + row * nyq_stride).x, 0.f) : make_float2(0.f, 0.f);
   }
   __syncthreads();
   if (YS) {
@@ -242,11 +242,11 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long 
     const float2 v = s[b * PM + j];
     float a0 = v.x * scale, a1 = v.y * scale;
     if (EPI != EPI_PLAIN) {
-      const float2 u = *reinterpret_cast<const float2*>(base + row * base_pitch + 2 * j);
+      const float2 u = gload(reinterpret_cast<const float2*>(base + row * base_pitch + 2 * j));
       a0 = coef * a0 + u.x;
       a1 = coef * a1 + u.y;
     }
-    *reinterpret_cast<float2*>(out + row * out_pitch + 2 * j) = make_float2(a0, a1);
+    gstore(reinterpret_cast<float2*>(out + row * out_pitch + 2 * j), make_float2(a0, a1));
   }
 }
 
@@ -277,14 +277,14 @@ __global__ __launch_bounds__(NT) void k_fft_cols(MPtr<float2> data_m, long jstri
   for (int e = tid; e < L * TX; e += NT) {
     const int b = e % TX, j = e / TX;
     const int col = col0 + b;
-    s[e] = col < ncols ? base[j * jstride + col] : make_float2(0.f, 0.f);
+    s[e] = col < ncols ? gload(base + j * jstride + col) : make_float2(0.f, 0.f);
   }
   __syncthreads();
   lds_fft<L, TX, NT, TX, 1, 0, TX, SIGN, 1, true>(s, stw, tid);
   for (int e = tid; e < L * TX; e += NT) {
     const int b = e % TX, j = e / TX;
     const int col = col0 + b;
-    if (col < ncols) base[j * jstride + col] = s[e];
+    if (col < ncols) gstore(base + j * jstride + col, s[e]);
   }
 }
 
@@ -303,14 +303,14 @@ __global__ __launch_bounds__(NT) void k_fft_rows(MPtr<float2> data_m, long pitch
   for (int e = tid; e < RPB * L; e += NT) {
     const int b = e / L, j = e % L;
     const long row = row0 + b;
-    s[b * PL + j] = row < nrows ? data[row * pitch + j] : make_float2(0.f, 0.f);
+    s[b * PL + j] = row < nrows ? gload(data + row * pitch + j) : make_float2(0.f, 0.f);
   }
   __syncthreads();
   lds_fft<L, RPB, NT, RPB, PL, 0, 1, SIGN, 1, false>(s, stw, tid);
   for (int e = tid; e < RPB * L; e += NT) {
     const int b = e / L, j = e % L;
     const long row = row0 + b;
-    if (row < nrows) data[row * pitch + j] = s[b * PL + j];
+    if (row < nrows) gstore(data + row * pitch + j, s[b * PL + j]);
   }
 }
 

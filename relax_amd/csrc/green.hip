@@ -256,7 +256,7 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
       const int c = tid / TX, t = tid % TX;
       float2 acc = red[c][0][t];
       for (int w = 1; w < NW; ++w) acc = cadd(acc, red[c][w][t]);
-      const float2 tt = A.tbase[c * A.tcs + t];
+      const float2 tt = gload(A.tbase + c * A.tcs + t);
       // p = t - p / (sx3 dx3)   (green.f90:373-375, :1082-1084)
       pv[c][t] = make_float2(tt.x - acc.x / C.inv_n3dx3, tt.y - acc.y / C.inv_n3dx3);
     }
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
       const int t = e % TX, z = (e / TX) % H3, c = e / (TX * H3);
       const float2 ev = s[(2 * c) * SB + z * TX + t];
       const float2 od = s[(2 * c + 1) * SB + z * TX + t];
-      A.base[c * A.cs + z * A.zs + t] = cadd(ev, cmul(od, cconj(stw[z])));
+      gstore(A.base + c * A.cs + z * A.zs + t, cadd(ev, cmul(od, cconj(stw[z]))));
     }
   };
 
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
   // load the upper half and its w^i3-shifted copy
   for (int e = tid; e < NE; e += NT) {
     const int t = e % TX, z = (e / TX) % H3, c = e / (TX * H3);
-    const float2 x = A.base[c * A.cs + z * A.zs + t];
+    const float2 x = gload(A.base + c * A.cs + z * A.zs + t);
     s[(2 * c) * SB + z * TX + t] = x;
     s[(2 * c + 1) * SB + z * TX + t] = cmul(x, tw3[z]);
   }
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
   float2 x[16];
   auto load = [&]() {
 #pragma unroll
-    for (int m = 0; m < 16; ++m) x[m] = A.base[ac * A.cs + (Q * m + aj) * A.zs + acol];
+    for (int m = 0; m < 16; ++m) x[m] = gload(A.base + ac * A.cs + (Q * m + aj) * A.zs + acol);
   };
   if (isA) load();
   for (int e = tid; e < 2 * H3; e += NT) stw[e] = tw3[e];
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
     const int c = tid / TX, t = tid % TX;
     float2 acc = red[c][0][t];
     for (int w = 1; w < NW; ++w) acc = cadd(acc, red[c][w][t]);
-    const float2 tt = A.tbase[c * A.tcs + t];
+    const float2 tt = gload(A.tbase + c * A.tcs + t);
     // p = t - p / (sx3 dx3)   (green.f90:373-375, :1082-1084)
     pv[c][t] = make_float2(tt.x - acc.x / C.inv_n3dx3, tt.y - acc.y / C.inv_n3dx3);
   }
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
       const int z = Q * m + aj;
-      A.base[ac * A.cs + z * A.zs + acol] = cadd(x[m], cmul(o[m], cconj(stw[z])));
+      gstore(A.base + ac * A.cs + z * A.zs + acol, cadd(x[m], cmul(o[m], cconj(stw[z]))));
     }
   }
 }

@@ -51,6 +51,21 @@ struct MPtr {
   }
 };
 
+// Loads and stores through the global address space: pointers taken from
+// records in memory are otherwise generic, and the compiler emits flat
+// accesses, which also count against the LDS wait counter.
+typedef float relax_f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 gload(const float2* p) {
+  const relax_f2v v = *(const __attribute__((address_space(1))) relax_f2v*)p;
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void gstore(float2* p, float2 v) {
+  relax_f2v w;
+  w.x = v.x;
+  w.y = v.y;
+  *(__attribute__((address_space(1))) relax_f2v*)p = w;
+}
+
 template <class T>
 inline MPtr<T> mptr(T* p) {
   MPtr<T> r;
