@@ -921,12 +921,14 @@ int relax_model_points(relax_model_t* m, const double* x, int n, double rot, dou
 // Phase timers: when a profile is being taken, an event is recorded after
 // every launch and the time since the previous event is charged to the
 // launch's kernel (the GPU runs the stream back to back, so the difference is
-// the kernel's duration plus a negligible launch gap).
+// the kernel's duration plus a negligible launch gap).  The events only time:
+// they are created without the system-scope fence, whose cache write-back
+// and invalidation would otherwise slow the kernels that follow.
 static void mark(relax_model_t* m, int kind) {
   if (!m->prof_on) return;
   if (m->prof_n >= (int)m->prof_ev.size()) {
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return;
     m->prof_ev.push_back(e);
   }
   (void)hipEventRecord(m->prof_ev[m->prof_n], m->stream);
