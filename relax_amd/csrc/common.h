@@ -63,25 +63,30 @@ struct Grid {
 struct T6 {
   float s11, s12, s13, s22, s23, s33;
 };
+// Accesses through the global address space.  Pointers read from a record in
+// device memory (a batch member's fields) are generic, and the compiler would
+// emit flat accesses for them, which also wait on the LDS counter.
+typedef __attribute__((address_space(1))) float relax_gfloat;
+__device__ __forceinline__ float gld(const float* p) { return *(const relax_gfloat*)p; }
+__device__ __forceinline__ void gst(float* p, float v) { *(relax_gfloat*)p = v; }
+__device__ __forceinline__ float gld_nt(const float* p) { return __builtin_nontemporal_load((const relax_gfloat*)p); }
+__device__ __forceinline__ void gst_nt(float* p, float v) { __builtin_nontemporal_store(v, (relax_gfloat*)p); }
+
 __device__ __forceinline__ T6 load_t6(const float* __restrict__ t, long H, long o) {
-  return T6{t[o], t[H + o], t[2 * H + o], t[3 * H + o], t[4 * H + o], t[5 * H + o]};
+  return T6{gld(t + o), gld(t + H + o), gld(t + 2 * H + o), gld(t + 3 * H + o), gld(t + 4 * H + o),
+            gld(t + 5 * H + o)};
 }
 __device__ __forceinline__ void store_t6(float* __restrict__ t, long H, long o, const T6& v) {
-  t[o] = v.s11; t[H + o] = v.s12; t[2 * H + o] = v.s13;
-  t[3 * H + o] = v.s22; t[4 * H + o] = v.s23; t[5 * H + o] = v.s33;
+  gst(t + o, v.s11); gst(t + H + o, v.s12); gst(t + 2 * H + o, v.s13);
+  gst(t + 3 * H + o, v.s22); gst(t + 4 * H + o, v.s23); gst(t + 5 * H + o, v.s33);
 }
 __device__ __forceinline__ T6 load_t6_nt(const float* __restrict__ t, long H, long o) {
-  return T6{__builtin_nontemporal_load(t + o), __builtin_nontemporal_load(t + H + o),
-            __builtin_nontemporal_load(t + 2 * H + o), __builtin_nontemporal_load(t + 3 * H + o),
-            __builtin_nontemporal_load(t + 4 * H + o), __builtin_nontemporal_load(t + 5 * H + o)};
+  return T6{gld_nt(t + o), gld_nt(t + H + o), gld_nt(t + 2 * H + o), gld_nt(t + 3 * H + o),
+            gld_nt(t + 4 * H + o), gld_nt(t + 5 * H + o)};
 }
 __device__ __forceinline__ void store_t6_nt(float* __restrict__ t, long H, long o, const T6& v) {
-  __builtin_nontemporal_store(v.s11, t + o);
-  __builtin_nontemporal_store(v.s12, t + H + o);
-  __builtin_nontemporal_store(v.s13, t + 2 * H + o);
-  __builtin_nontemporal_store(v.s22, t + 3 * H + o);
-  __builtin_nontemporal_store(v.s23, t + 4 * H + o);
-  __builtin_nontemporal_store(v.s33, t + 5 * H + o);
+  gst_nt(t + o, v.s11); gst_nt(t + H + o, v.s12); gst_nt(t + 2 * H + o, v.s13);
+  gst_nt(t + 3 * H + o, v.s22); gst_nt(t + 4 * H + o, v.s23); gst_nt(t + 5 * H + o, v.s33);
 }
 
 // Device tables of the rheology, one entry per depth of the half grid.

@@ -67,14 +67,14 @@ __device__ __forceinline__ void bodyforce_tile(const Grid& g, const RheoParams& 
     const float f2 = (s12[e0 + 1] - s12[e0 - 1]) * S.k1h + (s22[e0 + HX] - s22[e0 - HX]) * S.k2h + v[1];
     const float f3 = (s13[e0 + 1] - s13[e0 - 1]) * S.k1h + (s23[e0 + HX] - s23[e0 - HX]) * S.k2h + v[2];
     const long o = i1 + (long)n1 * (i2 + (long)n2 * zo);
-    W[o] = -f1;
-    W[H + o] = -f2;
-    W[2 * H + o] = -f3;
+    gst(W + o, -f1);
+    gst(W + H + o, -f2);
+    gst(W + 2 * H + o, -f3);
     if (zo == 0) {
       const long q = i1 + (long)n1 * i2, P = (long)n1 * n2;
-      T[q] = s13[e0];
-      T[P + q] = s23[e0];
-      T[2 * P + q] = cur[5 * HP + e0];
+      gst(T + q, s13[e0]);
+      gst(T + P + q, s23[e0]);
+      gst(T + 2 * P + q, cur[5 * HP + e0]);
     }
   };
 
@@ -243,10 +243,10 @@ __device__ __forceinline__ void stress_point(const RheoParams& R, const StencilC
                 t.s22 + dt * m.s22, t.s23 + dt * m.s23, t.s33 + dt * m.s33};
     store_t6_nt(tau, H, o, tn);
     if (R.mech[0].active || R.mech[1].active) {
-      float gv = __builtin_nontemporal_load(gam + o);
+      float gv = gld_nt(gam + o);
       if (R.mech[0].active) gv = gv + dt * gl;
       if (R.mech[1].active) gv = gv + dt * gn;
-      __builtin_nontemporal_store(gv, gam + o);
+      gst_nt(gam + o, gv);
     }
     out = T6{-tn.s11 + e.s11, -tn.s12 + e.s12, -tn.s13 + e.s13, -tn.s22 + e.s22, -tn.s23 + e.s23,
              -tn.s33 + e.s33};
@@ -311,15 +311,15 @@ __device__ __forceinline__ void stress_block(const Grid& g, const RheoParams& R,
     const float* X1 = X + H;
     const float* X2 = X + 2 * H;
     T6 e;
-    e.s11 = (X0[xp] - X0[xm]) * S.k1h;
-    e.s12 = (X1[xp] - X1[xm]) * S.k1h;
-    e.s13 = (X2[xp] - X2[xm]) * S.k1h;
-    e.s12 = e.s12 + (X0[yp] - X0[ym]) * S.k2h;
-    e.s22 = (X1[yp] - X1[ym]) * S.k2h;
-    e.s23 = (X2[yp] - X2[ym]) * S.k2h;
-    e.s13 = e.s13 + (X0[zp] - X0[zm]) * wz;
-    e.s23 = e.s23 + (X1[zp] - X1[zm]) * wz;
-    e.s33 = (X2[zp] - X2[zm]) * wz;
+    e.s11 = (gld(X0 + xp) - gld(X0 + xm)) * S.k1h;
+    e.s12 = (gld(X1 + xp) - gld(X1 + xm)) * S.k1h;
+    e.s13 = (gld(X2 + xp) - gld(X2 + xm)) * S.k1h;
+    e.s12 = e.s12 + (gld(X0 + yp) - gld(X0 + ym)) * S.k2h;
+    e.s22 = (gld(X1 + yp) - gld(X1 + ym)) * S.k2h;
+    e.s23 = (gld(X2 + yp) - gld(X2 + ym)) * S.k2h;
+    e.s13 = e.s13 + (gld(X0 + zp) - gld(X0 + zm)) * wz;
+    e.s23 = e.s23 + (gld(X1 + zp) - gld(X1 + zm)) * wz;
+    e.s33 = (gld(X2 + zp) - gld(X2 + zm)) * wz;
     stress_point<MODE, RC>(R, S, H, o, i1, i2, i3, e, sig, tau, gam, E, Em, sc, pw, ce, status);
   }
   if (MODE == STRESS_CORR) block_partials<256>(pw, ce, partials, blk);
