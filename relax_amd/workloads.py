@@ -8,7 +8,10 @@
        strain rate growing exponentially with depth (e-fold 10 km); 20 steps
   c2   256x256x128, dx 1: rate-strengthening afterslip plane (64 x 32 km,
        64 x 32 patches, dip 20, rake 90) loaded by a 32 x 16 km coseismic
-       thrust up dip of it, power law n = 3.5 below 30 km (eta0 1e18); 30 steps
+       thrust up dip of it (slip 0-1 m), power law n = 3.5 below 30 km
+       (eta0 1e18); 30 steps.  With 3 m of slip the stress on the adjacent
+       friction plane exceeds 20 (a-b)sigma, where the reference stops
+       (friction3d.f90:200-215).
   c3   one of 64 batched models on 128^3, dx 1: 1-10 patches, log-uniform
        eta in [1e17, 1e20], n in {1, 2, 3, 3.5}, 15-40 km lid; 10 steps
 
@@ -69,7 +72,7 @@ def spec(name):
             for j2 in range(32):
                 z = j3 * math.sin(math.radians(dip))
                 y = -(j3 + 0.0) * math.cos(math.radians(dip))
-                deck.append((rng.uniform(0, 3e-3), -16.0 + j2, y, z, 1.0, 1.0, 0.0, dip, 90.0))
+                deck.append((rng.uniform(0, 1e-3), -16.0 + j2, y, z, 1.0, 1.0, 0.0, dip, 90.0))
         ztop = 16 * math.sin(math.radians(dip))
         ytop = -16 * math.cos(math.radians(dip))
         plane = dict(x=-32.0, y=ytop, z=ztop, length=64.0, width=32.0, strike=0.0, dip=dip, rake=90.0)

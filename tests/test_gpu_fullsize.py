@@ -1,7 +1,8 @@
-"""The 512x512x256 power-law workload (c1) at its full size: one adaptive RK2
-step of the device from its coseismic state against the oracle stepping the
-same state (FP32 storage, relative L2), and the step size the oracle chooses
-for that state."""
+"""The benchmark workloads at their full sizes -- c1, 512x512x256 with a
+power-law mantle, and c2, 256x256x128 with afterslip on a friction plane and
+a power-law mantle: one adaptive RK2 step of the device from its coseismic
+state against the oracle stepping the same state (FP32 storage, relative L2),
+and the relaxation time and amplitudes the oracle finds for that state."""
 import time
 
 import numpy as np
@@ -14,8 +15,9 @@ from relax_amd import workloads
 pytestmark = pytest.mark.gpu
 
 
-def test_c1_step_matches_oracle():
-    s = workloads.spec("c1")
+@pytest.mark.parametrize("name", ["c1", "c2"])
+def test_full_size_step_matches_oracle(name):
+    s = workloads.spec(name)
     t0 = time.perf_counter()
     g = workloads.build(s)
     st0 = g.download()
