@@ -1326,8 +1326,7 @@ static PlaneDev source_plane(const relax_source_t& s, double dx1, double dx3) {
 
 int relax_model_event(relax_model_t* m, const relax_source_t* s, int ns, const relax_source_t* l, int nl,
                       double nyquist) {
-  if (!m || ns < 0 || nl < 0 || (ns > 0 && !s) || (nl > 0 && !l)) return RELAX_ERR_ARG;
-  if (int rc = wait_batch(m)) return rc;
+  if (!m || m->batch || ns < 0 || nl < 0 || (ns > 0 && !s) || (nl > 0 && !l)) return RELAX_ERR_ARG;
   int rc;
   if ((rc = sync_md(m))) return rc;
   const Grid& g = m->g;
