@@ -63,6 +63,13 @@ typedef struct {                                                                
    * (angles in radians, the internal convention of wangconvention) */
   double slip, x, y, z, width, length, strike, dip, rake, period, phase, beta, opening;
 } relax_source_t;
+typedef struct {                                                                 /* types.f90:99-106 */
+  /* the sources of one event (EVENT_STRUC without its time and copies) */
+  const relax_source_t* s;  int ns;   /* shear patches (slip) */
+  const relax_source_t* ts; int nt;   /* tensile cracks (opening) */
+  const relax_source_t* m;  int nm;   /* point dilatation sources (slip: volume change) */
+  const relax_source_t* l;  int nl;   /* surface loads */
+} relax_event_t;
 
 /* ---- library ------------------------------------------------------------ */
 const char* relax_hip_version(void);
@@ -133,6 +140,22 @@ int relax_source(double mu, double slip, double x, double y, double z, double L,
  * [t, t+Dt] when rate is nonzero. */
 int relax_traction(double mu, const relax_source_t* l, int nl, int sx1, int sx2, double dx1, double dx2, double t,
                    double Dt, float* t3, int rate, int align);
+/* tensilesource, elastic3d.f90:2315-2411, and mogisource, :2458-2509: the
+ * body force of a tapered opening crack (opening s, L down dip, W along
+ * strike) or of a point dilatation source (volume change o) added to the
+ * whole depth of f1..f3 ((sx1+2*align) x sx2 x sx3). */
+int relax_tensilesource(double lambda, double mu, double s, double x, double y, double z, double L, double W,
+                        double strike, double dip, double beta, int sx1, int sx2, int sx3, double dx1, double dx2,
+                        double dx3, float* f1, float* f2, float* f3, int align);
+int relax_mogisource(double lambda, double mu, double o, double xs, double ys, double zs, int sx1, int sx2, int sx3,
+                     double dx1, double dx2, double dx3, float* f1, float* f2, float* f3, int align);
+/* momentdensitytensile, elastic3d.f90:3137-3215, and momentdensitymogi,
+ * :3240-3286: their moment densities added to sig (sx1 x sx2 x sx3). */
+int relax_momentdensitytensile(double lambda, double mu, double s, double x, double y, double z, double L, double W,
+                               double strike, double dip, double beta, int sx1, int sx2, int sx3, double dx1,
+                               double dx2, double dx3, relax_tensor_t* sig);
+int relax_momentdensitymogi(double lambda, double mu, double o, double xs, double ys, double zs, int sx1, int sx2,
+                            int sx3, double dx1, double dx2, double dx3, relax_tensor_t* sig);
 
 /* ---- viscoelastic3d.f90 / friction3d.f90 --------------------------------- */
 /* viscouseigenstress, viscoelastic3d.f90:217-314.  maxwelltime, dgammadot0
@@ -259,14 +282,17 @@ int relax_model_synchronize(relax_model_t* m);
  * event switch of relax.f90:858-889).  As dislocations (relax.f90:
  * 1116-1289) and traction (elastic3d.f90:2517-2590) followed by the Green
  * solve: f, t = the source() body force and traction of the shear patches
- * s[k] longer or wider than nyquist*min(dx1,dx2,dx3), minus the surface
- * loads l[k] at the model's time; u += greenfunctioncowling(f, t);
- * tau += the momentdensityshear of every patch; sig = -tau + C:eps(u).
- * Patches below that size get no displacement (the reference adds Okada's
- * solution for them at t = 0 only, relax.f90:357-360).  Not on a member of
- * a batch. */
-int relax_model_event(relax_model_t* m, const relax_source_t* s, int ns, const relax_source_t* l, int nl,
-                      double nyquist);
+ * and the tensilesource() body force of the tensile cracks longer or wider
+ * than nyquist*min(dx1,dx2,dx3), the mogisource() body force of every point
+ * dilatation source, minus the surface loads at the model's time;
+ * u += greenfunctioncowling(f, t); tau += the moment density of every
+ * source; sig = -tau + C:eps(u).  Tensile cracks and point sources reach
+ * above the surface (their images), so such an event takes the full-depth
+ * Green solve on temporary buffers; shear patches and loads alone take the
+ * step's pruned solve.  Sources below the Nyquist size get no displacement
+ * (the reference adds Okada's solution for shear patches at t = 0 only,
+ * relax.f90:357-360).  Not on a member of a batch. */
+int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist);
 /* Utilities of the same set-up: add_shear adds momentdensityshear
  * (elastic3d.f90:2661-2750) of n rectangular patches (centre x,y,z, width,
  * length, strike, dip, rake in the internal convention) with slip[k] to tau

@@ -48,7 +48,9 @@ class OrcModel(ctypes.Structure):
                 ("ltransient", ctypes.c_void_p), ("nltransient", ctypes.c_void_p),
                 ("dg0", ctypes.c_void_p * 4),
                 ("ev_src", ctypes.c_void_p), ("ev_off", ctypes.c_void_p), ("ev_loads", ctypes.c_void_p),
-                ("ev_loff", ctypes.c_void_p), ("nyquist", ctypes.c_double)]
+                ("ev_loff", ctypes.c_void_p), ("nyquist", ctypes.c_double),
+                ("ev_ts", ctypes.c_void_p), ("ev_tsoff", ctypes.c_void_p), ("ev_mg", ctypes.c_void_p),
+                ("ev_mgoff", ctypes.c_void_p)]
 
 
 def lib():
@@ -216,17 +218,53 @@ def traction(loads, dx1, dx2, t, t3, Dt=0.0, rate=False):
                                                _p(t3), int(rate))
 
 
-def dislocations(sources, loads, mu, nyquist, t, dx1, dx2, dx3, v, tr, tau):
+def dislocations(sources, loads, mu, nyquist, t, dx1, dx2, dx3, v, tr, tau, lam=None, beta=0.2, tensile=None,
+                 mogi=None):
     """relax.f90:1116-1289 then traction: the source() body force of the
     shear patches (SOURCE array) longer or wider than nyquist*min(dx) into
     the padded fields v (3 arrays (sx3, sx2, sx1+2)) and planes tr, every
-    patch's moment density into tau, the surface loads into tr[2]."""
+    patch's moment density into tau; the same for the tensile cracks
+    (SOURCE, opening) and the point dilatation sources (SOURCE, slip = volume
+    change), whose body forces fill the full depth of v; the surface loads
+    into tr[2].  lam defaults to mu."""
     sx3, sx2, p1 = v[0].shape
-    s = np.ascontiguousarray(sources, SOURCE)
-    ld = np.ascontiguousarray(np.zeros(0, SOURCE) if loads is None else loads, SOURCE)
+    arr = [np.ascontiguousarray(np.zeros(0, SOURCE) if a is None else a, SOURCE)
+           for a in (sources, tensile, mogi, loads)]
     getattr(lib(), "orc_dislocations_" + _sfx(tau))(
-        _p(s), len(s), _p(ld), len(ld), _d(mu), _d(nyquist), _d(t), p1 - 2, sx2, sx3, _d(dx1), _d(dx2), _d(dx3),
-        _p(v[0]), _p(v[1]), _p(v[2]), _p(tr[0]), _p(tr[1]), _p(tr[2]), _p(tau))
+        _p(arr[0]), len(arr[0]), _p(arr[1]), len(arr[1]), _p(arr[2]), len(arr[2]), _p(arr[3]), len(arr[3]),
+        _d(mu if lam is None else lam), _d(mu), _d(beta), _d(nyquist), _d(t), p1 - 2, sx2, sx3, _d(dx1), _d(dx2),
+        _d(dx3), _p(v[0]), _p(v[1]), _p(v[2]), _p(tr[0]), _p(tr[1]), _p(tr[2]), _p(tau))
+
+
+def tensilesource(lam, mu, s, x, y, z, L, W, strike, dip, beta, dx1, dx2, dx3, f1, f2, f3):
+    """elastic3d.f90:2315-2411 on padded full-depth fields (sx3, sx2, sx1+2)."""
+    sx3, sx2, p1 = f1.shape
+    getattr(lib(), "orc_tensilesource_" + _sfx(f1))(
+        _d(lam), _d(mu), _d(s), _d(x), _d(y), _d(z), _d(L), _d(W), _d(strike), _d(dip), _d(beta), p1 - 2, sx2, sx3,
+        _d(dx1), _d(dx2), _d(dx3), _p(f1), _p(f2), _p(f3))
+
+
+def mogisource(lam, mu, o, x, y, z, dx1, dx2, dx3, f1, f2, f3):
+    """elastic3d.f90:2458-2509 on padded full-depth fields (sx3, sx2, sx1+2)."""
+    sx3, sx2, p1 = f1.shape
+    getattr(lib(), "orc_mogisource_" + _sfx(f1))(
+        _d(lam), _d(mu), _d(o), _d(x), _d(y), _d(z), p1 - 2, sx2, sx3, _d(dx1), _d(dx2), _d(dx3), _p(f1), _p(f2),
+        _p(f3))
+
+
+def momentdensitytensile(lam, mu, s, x, y, z, L, W, strike, dip, beta, dx1, dx2, dx3, sig):
+    """elastic3d.f90:3137-3215, added to the tensor field sig (sx3, sx2, sx1, 6)."""
+    sx3, sx2, sx1, _ = sig.shape
+    getattr(lib(), "orc_momentdensitytensile_" + _sfx(sig))(
+        _d(lam), _d(mu), _d(s), _d(x), _d(y), _d(z), _d(L), _d(W), _d(strike), _d(dip), _d(beta), sx1, sx2, sx3,
+        _d(dx1), _d(dx2), _d(dx3), _p(sig))
+
+
+def momentdensitymogi(lam, mu, o, x, y, z, dx1, dx2, dx3, sig):
+    """elastic3d.f90:3240-3286, added to the tensor field sig (sx3, sx2, sx1, 6)."""
+    sx3, sx2, sx1, _ = sig.shape
+    getattr(lib(), "orc_momentdensitymogi_" + _sfx(sig))(
+        _d(lam), _d(mu), _d(o), _d(x), _d(y), _d(z), sx1, sx2, sx3, _d(dx1), _d(dx2), _d(dx3), _p(sig))
 
 
 def shiftedindex(x1, x2, x3, sx1, sx2, sx3, dx1, dx2, dx3):
@@ -263,8 +301,10 @@ def rk2_run(model, state, nsteps, dt_replay=None):
     with state["epsilonik"] (tensor field), and weak-zone fields dg_linear,
     dg_nonlinear, dg_ltransient, dg_nltransient ((sx3/2, sx2, sx1) arrays of
     the state's dtype), and the later events: event_sources, a list (one
-    entry per event after the first) of SOURCE arrays, event_loads likewise
-    (surface loads, optional), and nyquist (dislocations' size gate).
+    entry per event after the first) of SOURCE arrays, event_loads,
+    event_tensile and event_mogi likewise (surface loads, tensile cracks,
+    point dilatation sources; optional), and nyquist (dislocations' size
+    gate).
     state["e"] is the current event (1-based, default 1).  Returns an
     (nsteps, 10) array of
     Dt, tm, maxwell(1:3), power, C:E^i, t, maxwell(4:5).
@@ -286,7 +326,8 @@ def rk2_run(model, state, nsteps, dt_replay=None):
                  ptr(model.get("planes")), None if model.get("patches") is None else model["patches"].ctypes.data,
                  model["interval"], model.get("odt", 1.0), model.get("tscale", 1.0), model.get("skip", 1),
                  len(ev), ptr(ev), ptr(model.get("ltransient")), ptr(model.get("nltransient")))
-    for key, fs, fo in (("event_sources", "ev_src", "ev_off"), ("event_loads", "ev_loads", "ev_loff")):
+    for key, fs, fo in (("event_sources", "ev_src", "ev_off"), ("event_loads", "ev_loads", "ev_loff"),
+                        ("event_tensile", "ev_ts", "ev_tsoff"), ("event_mogi", "ev_mg", "ev_mgoff")):
         evs = model.get(key)
         if evs:
             lists = [np.zeros(0, SOURCE)] + [np.asarray(a, SOURCE) for a in evs]

@@ -1083,13 +1083,157 @@ static void traction(const orc_source* l, int nl, int sx1, int sx2, double dx1, 
 }
 
 // ---------------------------------------------------------------------------
+// tensilesource, elastic3d.f90:2315-2411: the body force of a tapered
+// opening crack over the full depth of f (x3 from shiftedcoordinates, so the
+// image above the surface lands in the lower half), one rounding per point.
+// ---------------------------------------------------------------------------
+template <class R>
+static void tensilesource(double lambda, double mu, double s, double x, double y, double z, double L, double W,
+                          double strike, double dip, double beta, int sx1, int sx2, int sx3, double dx1, double dx2,
+                          double dx3, R* f1, R* f2, R* f3) {
+  const long pitch = sx1 + 2;
+  const double cs = std::cos(strike), ss = std::sin(strike), cd = std::cos(dip), sd = std::sin(dip);
+  const double Wp = W * (1.0 + 2.0 * beta) / 2.0, Lp = L * (1.0 + 2.0 * beta) / 2.0;
+  const double x2r = cs * x - ss * y;
+  const double xr = cd * x2r - sd * z, yr = ss * x + cs * y, zr = sd * x2r + cd * z;
+  const double scale1 = -s * (lambda + 2.0 * mu), scale2 = -s * lambda;
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    const double x3 = shifted1(i3, sx3, dx3);
+    if ((std::fabs(x3 - z) > Lp) && (std::fabs(x3 + z) > Lp)) continue;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        const double x1 = shifted1(i1, sx1, dx1), x2 = shifted1(i2, sx2, dx2);
+        if ((std::fabs(x1 - x) > Wp) || (std::fabs(x2 - y) > Wp)) continue;
+        const double xx = cs * x1 - ss * x2;
+        const double x1s = cd * xx - sd * x3, x1i = cd * xx + sd * x3;
+        if ((std::fabs(x1s - xr) > 7.01 * dx1) && (std::fabs(x1i - xr) > 7.01 * dx1)) continue;
+        const double x2s = ss * x1 + cs * x2, x3s = sd * xx + cd * x3, x3i = -sd * xx + cd * x3;
+        double t1 = gauss(x1s - xr, dx1);
+        const double t2 = omega((x2s - yr) / W, beta);
+        double t3 = omega((x3s - zr) / L, beta);
+        const double sourc = scale1 * gaussp(x1s - xr, dx1) * t2 * t3;
+        const double dblcp = scale2 / W * t1 * omegap((x2s - yr) / W, beta) * t3;
+        const double dipcs = scale2 / L * t1 * t2 * omegap((x3s - zr) / L, beta);
+        t1 = gauss(x1i - xr, dx1);
+        t3 = omega((x3i + zr) / L, beta);
+        const double image = scale1 * gaussp(x1i - xr, dx1) * t2 * t3;
+        const double cplei = scale2 / W * t1 * omegap((x2s - yr) / W, beta) * t3;
+        const double dipci = scale2 / L * t1 * t2 * omegap((x3i + zr) / L, beta);
+        const long o = i1 + pitch * (i2 + (long)sx2 * i3);
+        f1[o] = (R)((double)f1[o] + cs * cd * (sourc + image) + ss * (dblcp + cplei) + cs * sd * (dipcs + dipci));
+        f2[o] = (R)((double)f2[o] - ss * cd * (sourc + image) + cs * (dblcp + cplei) - ss * sd * (dipcs + dipci));
+        f3[o] = (R)((double)f3[o] - sd * (sourc - image) + cd * (MUT(22) ? dipcs + dipci : dipcs - dipci));
+      }
+  }
+}
+
+// mogisource, elastic3d.f90:2458-2509: the body force of a point dilatation
+// source (volume change o) and its image, over the full depth of f
+template <class R>
+static void mogisource(double lambda, double mu, double o, double xs, double ys, double zs, int sx1, int sx2, int sx3,
+                       double dx1, double dx2, double dx3, R* f1, R* f2, R* f3) {
+  const long pitch = sx1 + 2;
+  const double scale = -(lambda + 2.0 * mu / 3.0) * o;
+  const double Wp = 6.0 * std::max(dx1, std::max(dx2, dx3)), Lp = Wp;
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    const double x3 = shifted1(i3, sx3, dx3);
+    if ((std::fabs(x3 - zs) > Lp) && (std::fabs(x3 + zs) > Lp)) continue;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        const double x1 = shifted1(i1, sx1, dx1), x2 = shifted1(i2, sx2, dx2);
+        if ((std::fabs(x1 - xs) > Wp) || (std::fabs(x2 - ys) > Wp)) continue;
+        const double t1 = gauss(x1 - xs, dx1), t2 = gauss(x2 - ys, dx2);
+        double t3 = gauss(x3 - zs, dx3);
+        const double s1 = scale * gaussp(x1 - xs, dx1) * t2 * t3, s2 = scale * t1 * gaussp(x2 - ys, dx2) * t3;
+        const double s3 = scale * t1 * t2 * gaussp(x3 - zs, dx3);
+        t3 = gauss(x3 + zs, dx3);
+        const double m1 = scale * gaussp(x1 - xs, dx1) * t2 * t3, m2 = scale * t1 * gaussp(x2 - ys, dx2) * t3;
+        const double m3 = scale * t1 * t2 * gaussp(x3 + zs, dx3);
+        const long q = i1 + pitch * (i2 + (long)sx2 * i3);
+        f1[q] = (R)((double)f1[q] + (s1 + m1));
+        f2[q] = (R)((double)f2[q] + (s2 + m2));
+        f3[q] = (R)((double)f3[q] + (MUT(24) ? s3 + m3 : s3 - m3));
+      }
+  }
+}
+
+// momentdensitytensile, elastic3d.f90:3137-3215: Hooke's law of the
+// eigenstrain n sdyad (slip n) of an opening crack (n carries the aperture,
+// so the eigenstrain goes with its square), added to sig (sx3 = the depth
+// of sig, x3 unshifted)
+template <class R>
+static void momentdensitytensile(double lambda, double mu, double slip, double x, double y, double z, double L,
+                                 double W, double strike, double dip, double beta, int sx1, int sx2, int sx3,
+                                 double dx1, double dx2, double dx3, R* sig) {
+  T6<R>* S = reinterpret_cast<T6<R>*>(sig);
+  const double cs = std::cos(strike), ss = std::sin(strike), cd = std::cos(dip), sd = std::sin(dip);
+  const double Wp = W * (1.0 + 2.0 * beta) / 2.0, Lp = L * (1.0 + 2.0 * beta) / 2.0;
+  const double x2r = cs * x - ss * y;
+  const double xr = cd * x2r - sd * z, yr = ss * x + cs * y, zr = sd * x2r + cd * z;
+  const R m2 = (R)(2.0 * mu);
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    const double x3 = (double)i3 * dx3;
+    if (std::fabs(x3 - z) > Lp) continue;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        const double x1 = shifted1(i1, sx1, dx1), x2 = shifted1(i2, sx2, dx2);
+        if ((std::fabs(x1 - x) > Wp) || (std::fabs(x2 - y) > Wp)) continue;
+        const double xx = cs * x1 - ss * x2;
+        const double x1s = cd * xx - sd * x3, x1i = cd * xx + sd * x3;
+        if ((std::fabs(x1s - xr) > 7.01 * dx1) && (std::fabs(x1i - xr) > 7.01 * dx1)) continue;
+        const double x2s = ss * x1 + cs * x2, x3s = sd * xx + cd * x3, x3i = -sd * xx + cd * x3;
+        const double t2 = omega((x2s - yr) / W, beta);
+        double ap = gauss(x1s - xr, dx1) * t2 * omega((x3s - zr) / L, beta);
+        ap = ap + gauss(x1i - xr, dx1) * t2 * omega((x3i + zr) / L, beta);
+        const double n[3] = {+cd * cs * ap, -cd * ss * ap, -sd * ap};
+        const double b[3] = {slip * n[0], slip * n[1], slip * n[2]};
+        R e[6] = {(R)(n[0] * b[0]), (R)((n[0] * b[1] + n[1] * b[0]) / 2.0), (R)((n[0] * b[2] + n[2] * b[0]) / 2.0),
+                  (R)(n[1] * b[1]), (R)((n[1] * b[2] + n[2] * b[1]) / 2.0), (R)(n[2] * b[2])};
+        // isotropicstressstrain, elastic3d.f90:129-142
+        const double tr = (double)(e[0] + e[3] + e[5]);
+        for (R& v : e) v = m2 * v;
+        e[0] = (R)((double)e[0] + lambda * tr);
+        e[3] = (R)((double)e[3] + lambda * tr);
+        e[5] = (R)((double)e[5] + (MUT(23) ? 0.0 : lambda * tr));
+        T6<R>& q = S[i1 + (long)sx1 * (i2 + (long)sx2 * i3)];
+        q.s11 += e[0]; q.s12 += e[1]; q.s13 += e[2]; q.s22 += e[3]; q.s23 += e[4]; q.s33 += e[5];
+      }
+  }
+}
+
+// momentdensitymogi, elastic3d.f90:3240-3286: the isotropic moment density
+// kappa o G of a point dilatation source (without its image), x3 unshifted
+template <class R>
+static void momentdensitymogi(double lambda, double mu, double o, double xs, double ys, double zs, int sx1, int sx2,
+                              int sx3, double dx1, double dx2, double dx3, R* sig) {
+  T6<R>* S = reinterpret_cast<T6<R>*>(sig);
+  const double kappa = lambda + 2.0 / 3.0 * mu;
+  const double Wp = 6.0 * std::max(dx1, std::max(dx2, dx3)), Lp = Wp;
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    const double x3 = (double)i3 * dx3;
+    if (std::fabs(x3 - zs) > Lp) continue;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        const double x1 = shifted1(i1, sx1, dx1), x2 = shifted1(i2, sx2, dx2);
+        if ((std::fabs(x1 - xs) > Wp) || (std::fabs(x2 - ys) > Wp)) continue;
+        double g = o * gauss(x1 - xs, dx1) * gauss(x2 - ys, dx2) * gauss(x3 - zs, dx3);
+        g = kappa * g;
+        const R m = (R)g;
+        T6<R>& q = S[i1 + (long)sx1 * (i2 + (long)sx2 * i3)];
+        q.s11 += m; q.s22 += m; q.s33 += m;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // dislocations, relax.f90:1116-1289 (shear patches), then traction
 // (relax.f90:319, :874): the source() body force of the patches longer or
 // wider than nyquist*min(dx) into v (padded, sx3 deep) and t, the moment
 // density of every patch into tau, and the surface loads into t3.
 // ---------------------------------------------------------------------------
 template <class R>
-static void dislocations(const orc_source* s, int ns, const orc_source* l, int nl, double mu, double nyquist,
+static void dislocations(const orc_source* s, int ns, const orc_source* ts, int nt, const orc_source* mg, int nm,
+                         const orc_source* l, int nl, double lambda, double mu, double beta, double nyquist,
                          double t, int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, R* v1, R* v2,
                          R* v3, R* t1, R* t2, R* t3, R* tau) {
   const double dmin = nyquist * std::min(dx1, std::min(dx2, dx3));
@@ -1102,6 +1246,20 @@ static void dislocations(const orc_source* s, int ns, const orc_source* l, int n
   for (int k = 0; k < ns; ++k)
     momentdensityshear(mu, s[k].slip, s[k].x, s[k].y, s[k].z, s[k].width, s[k].length, s[k].strike, s[k].dip,
                        s[k].rake, s[k].beta, sx1, sx2, sx3 / 2, dx1, dx2, dx3, tau);
+  // tensile cracks (:1225-1254): the body force of the large ones with the
+  // model's taper, every crack's moment density
+  for (int k = 0; k < nt; ++k)
+    if (dmin < ts[k].length || dmin < ts[k].width)
+      tensilesource(lambda, mu, ts[k].opening, ts[k].x, ts[k].y, ts[k].z, ts[k].width, ts[k].length, ts[k].strike,
+                    ts[k].dip, beta, sx1, sx2, sx3, dx1, dx2, dx3, v1, v2, v3);
+  for (int k = 0; k < nt; ++k)
+    momentdensitytensile(lambda, mu, ts[k].opening, ts[k].x, ts[k].y, ts[k].z, ts[k].width, ts[k].length,
+                         ts[k].strike, ts[k].dip, beta, sx1, sx2, sx3 / 2, dx1, dx2, dx3, tau);
+  // point dilatation sources (:1256-1287): slip is the volume change
+  for (int k = 0; k < nm; ++k)
+    mogisource(lambda, mu, mg[k].slip, mg[k].x, mg[k].y, mg[k].z, sx1, sx2, sx3, dx1, dx2, dx3, v1, v2, v3);
+  for (int k = 0; k < nm; ++k)
+    momentdensitymogi(lambda, mu, mg[k].slip, mg[k].x, mg[k].y, mg[k].z, sx1, sx2, sx3 / 2, dx1, dx2, dx3, tau);
   traction(l, nl, sx1, sx2, dx1, dx2, t, 0.0, t3, false);
 }
 
@@ -1145,6 +1303,12 @@ struct orc_model {
   const orc_source* ev_loads;
   const int* ev_loff;
   double nyquist;
+  // tensile cracks and point dilatation sources of the later events, listed
+  // as the shear patches
+  const orc_source* ev_ts;
+  const int* ev_tsoff;
+  const orc_source* ev_mg;
+  const int* ev_mgoff;
 };
 }
 
@@ -1303,14 +1467,17 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
       e = e + 1;
       const int s0 = m->ev_off ? m->ev_off[e - 1] : 0, s1 = m->ev_off ? m->ev_off[e] : 0;
       const int l0 = m->ev_loff ? m->ev_loff[e - 1] : 0, l1 = m->ev_loff ? m->ev_loff[e] : 0;
-      if (s1 > s0 || l1 > l0) {
+      const int a0 = m->ev_tsoff ? m->ev_tsoff[e - 1] : 0, a1 = m->ev_tsoff ? m->ev_tsoff[e] : 0;
+      const int b0 = m->ev_mgoff ? m->ev_mgoff[e - 1] : 0, b1 = m->ev_mgoff ? m->ev_mgoff[e] : 0;
+      if (s1 > s0 || l1 > l0 || a1 > a0 || b1 > b0) {
         std::fill(v1.begin(), v1.end(), (R)0); std::fill(v2.begin(), v2.end(), (R)0);
         std::fill(v3.begin(), v3.end(), (R)0); std::fill(t1.begin(), t1.end(), (R)0);
         std::fill(t2.begin(), t2.end(), (R)0); std::fill(t3.begin(), t3.end(), (R)0);
         std::vector<R> lost(MUT(18) ? 6 * npts : 0, (R)0);
-        dislocations(m->ev_src + s0, s1 - s0, m->ev_loads ? m->ev_loads + l0 : nullptr, l1 - l0, m->mu, m->nyquist,
-                     t, sx1, sx2, sx3, m->dx1, m->dx2, m->dx3, v1.data(), v2.data(), v3.data(), t1.data(),
-                     t2.data(), t3.data(), MUT(18) ? lost.data() : tau);
+        dislocations(m->ev_src ? m->ev_src + s0 : nullptr, s1 - s0, m->ev_ts ? m->ev_ts + a0 : nullptr, a1 - a0,
+                     m->ev_mg ? m->ev_mg + b0 : nullptr, b1 - b0, m->ev_loads ? m->ev_loads + l0 : nullptr, l1 - l0,
+                     m->lambda, m->mu, m->beta, m->nyquist, t, sx1, sx2, sx3, m->dx1, m->dx2, m->dx3, v1.data(),
+                     v2.data(), v3.data(), t1.data(), t2.data(), t3.data(), MUT(18) ? lost.data() : tau);
         rc = greenfunctioncowling(v1.data(), v2.data(), v3.data(), t1.data(), t2.data(), t3.data(), sx1, sx2, sx3,
                                   m->dx1, m->dx2, m->dx3, m->lambda, m->mu, m->gam);
         if (rc) return rc;
@@ -1417,11 +1584,35 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
                                         double t, double Dt, R* t3, int rate) {                                     \
     traction(l, nl, sx1, sx2, dx1, dx2, t, Dt, t3, rate != 0);                                                       \
   }                                                                                                                  \
-  ORC_EXPORT void orc_dislocations_##SUFFIX(const orc_source* s, int ns, const orc_source* l, int nl, double mu,     \
-                                            double nyquist, double t, int sx1, int sx2, int sx3, double dx1,        \
-                                            double dx2, double dx3, R* v1, R* v2, R* v3, R* t1, R* t2, R* t3,       \
-                                            R* tau) {                                                                \
-    dislocations(s, ns, l, nl, mu, nyquist, t, sx1, sx2, sx3, dx1, dx2, dx3, v1, v2, v3, t1, t2, t3, tau);         \
+  ORC_EXPORT void orc_dislocations_##SUFFIX(const orc_source* s, int ns, const orc_source* ts, int nt,              \
+                                            const orc_source* mg, int nm, const orc_source* l, int nl,              \
+                                            double lambda, double mu, double beta, double nyquist, double t,        \
+                                            int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, R* v1,   \
+                                            R* v2, R* v3, R* t1, R* t2, R* t3, R* tau) {                            \
+    dislocations(s, ns, ts, nt, mg, nm, l, nl, lambda, mu, beta, nyquist, t, sx1, sx2, sx3, dx1, dx2, dx3, v1, v2,  \
+                 v3, t1, t2, t3, tau);                                                                               \
+  }                                                                                                                  \
+  ORC_EXPORT void orc_tensilesource_##SUFFIX(double lambda, double mu, double s, double x, double y, double z,       \
+                                             double L, double W, double strike, double dip, double beta, int sx1,  \
+                                             int sx2, int sx3, double dx1, double dx2, double dx3, R* f1, R* f2,   \
+                                             R* f3) {                                                                \
+    tensilesource(lambda, mu, s, x, y, z, L, W, strike, dip, beta, sx1, sx2, sx3, dx1, dx2, dx3, f1, f2, f3);      \
+  }                                                                                                                  \
+  ORC_EXPORT void orc_mogisource_##SUFFIX(double lambda, double mu, double o, double x, double y, double z, int sx1, \
+                                          int sx2, int sx3, double dx1, double dx2, double dx3, R* f1, R* f2,       \
+                                          R* f3) {                                                                   \
+    mogisource(lambda, mu, o, x, y, z, sx1, sx2, sx3, dx1, dx2, dx3, f1, f2, f3);                                  \
+  }                                                                                                                  \
+  ORC_EXPORT void orc_momentdensitytensile_##SUFFIX(double lambda, double mu, double s, double x, double y,         \
+                                                    double z, double L, double W, double strike, double dip,        \
+                                                    double beta, int sx1, int sx2, int sx3, double dx1, double dx2, \
+                                                    double dx3, R* sig) {                                            \
+    momentdensitytensile(lambda, mu, s, x, y, z, L, W, strike, dip, beta, sx1, sx2, sx3, dx1, dx2, dx3, sig);      \
+  }                                                                                                                  \
+  ORC_EXPORT void orc_momentdensitymogi_##SUFFIX(double lambda, double mu, double o, double x, double y, double z,  \
+                                                 int sx1, int sx2, int sx3, double dx1, double dx2, double dx3,    \
+                                                 R* sig) {                                                           \
+    momentdensitymogi(lambda, mu, o, x, y, z, sx1, sx2, sx3, dx1, dx2, dx3, sig);                                  \
   }                                                                                                                  \
   ORC_EXPORT int orc_exportpoints_##SUFFIX(const R* u1, const R* u2, const R* u3, const R* sig, int sx1, int sx2,  \
                                            int sx3, double dx1, double dx2, double dx3, const double* pts, int n,   \

@@ -31,6 +31,12 @@ SOURCE = np.dtype([(k, "f8") for k in ("slip", "x", "y", "z", "width", "length",
                                        "phase", "beta", "opening")])
 
 
+class EventDesc(ctypes.Structure):
+    """relax_event_t: the source lists of one event (SOURCE arrays)."""
+    _fields_ = [("s", c_void_p), ("ns", c_int), ("ts", c_void_p), ("nt", c_int), ("m", c_void_p), ("nm", c_int),
+                ("l", c_void_p), ("nl", c_int)]
+
+
 class ModelDesc(ctypes.Structure):
     _fields_ = [("sx1", c_int), ("sx2", c_int), ("sx3", c_int),
                 ("dx1", c_double), ("dx2", c_double), ("dx3", c_double),
@@ -103,7 +109,13 @@ SIGNATURES = {
     "relax_model_stream": (c_void_p, [P]),
     "relax_model_add_shear": (c_int, [P, P, P, c_int]),
     "relax_model_coseismic": (c_int, [P]),
-    "relax_model_event": (c_int, [P, P, c_int, P, c_int, c_double]),
+    "relax_model_event": (c_int, [P, P, c_double]),
+    "relax_tensilesource": (c_int, [c_double] * 11 + [c_int, c_int, c_int, c_double, c_double, c_double] + [P] * 3
+                            + [c_int]),
+    "relax_mogisource": (c_int, [c_double] * 6 + [c_int, c_int, c_int, c_double, c_double, c_double] + [P] * 3
+                         + [c_int]),
+    "relax_momentdensitytensile": (c_int, [c_double] * 11 + [c_int, c_int, c_int, c_double, c_double, c_double, P]),
+    "relax_momentdensitymogi": (c_int, [c_double] * 6 + [c_int, c_int, c_int, c_double, c_double, c_double, P]),
     "relax_model_profile": (c_int, [P, c_int, P, P]),
     "relax_model_autotune": (c_int, [P, P, P]),
     "relax_model_set_tuning": (c_int, [P, P]),

@@ -6,6 +6,7 @@
 // Stencils here take any FIR length chosen by optimalfilter
 // (elastic3d.f90:846-907): fir1 for 15+ samples, fir7 for 5..14.
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -562,6 +563,30 @@ int relax_cerrutimodified(const float* p1, const float* p2, const float* p3, dou
   return RELAX_OK;
 }
 
+// greenfunctioncowling on device buffers in the padded layout: v, three
+// fields of (sx1+2) sx2 sx3 floats over the full depth; t, three planes of
+// (sx1+2) sx2; p, three planes of scratch.  Synchronous, on the null stream.
+int green_full_device(float* const v[3], float* const t[3], float* const p[3], int sx1, int sx2, int sx3,
+                      double dx1, double dx2, double dx3, double lambda, double mu, double gamma) {
+  int rc;
+  const GreenConst gc = green_const(sx1, sx2, sx3, dx1, dx2, dx3, lambda, mu, gamma);
+  float2* vc[3] = {(float2*)v[0], (float2*)v[1], (float2*)v[2]};
+  float2* tc[3] = {(float2*)t[0], (float2*)t[1], (float2*)t[2]};
+  float2* pc[3] = {(float2*)p[0], (float2*)p[1], (float2*)p[2]};
+  for (int k = 0; k < 3; ++k) {
+    if ((rc = fftn_padded(v[k], sx1, sx2, sx3, dx1, dx2, dx3, -1))) return rc;
+    if ((rc = fftn_padded(t[k], sx1, sx2, 1, dx1, dx2, dx3, -1))) return rc;
+  }
+  if ((rc = launch_elastic_padded(gc, vc[0], vc[1], vc[2], 0))) return rc;
+  if ((rc = launch_surface_padded(gc, vc[0], vc[1], vc[2], tc[0], tc[1], tc[2], pc[0], pc[1], pc[2], 0))) return rc;
+  if ((rc = launch_cerruti_padded(gc, pc[0], pc[1], pc[2], vc[0], vc[1], vc[2], 0))) return rc;
+  for (int k = 0; k < 3; ++k) {
+    if ((rc = fftn_padded(v[k], sx1, sx2, sx3, dx1, dx2, dx3, +1))) return rc;
+    if ((rc = fftn_padded(t[k], sx1, sx2, 1, dx1, dx2, dx3, +1))) return rc;
+  }
+  return sync_status(RELAX_OK);
+}
+
 int relax_greenfunctioncowling(float* c1, float* c2, float* c3, float* t1, float* t2, float* t3, int sx1,
                                int sx2, int sx3, double dx1, double dx2, double dx3, double lambda, double mu,
                                double gamma, int align_data) {
@@ -580,21 +605,10 @@ int relax_greenfunctioncowling(float* c1, float* c2, float* c3, float* t1, float
     if ((rc = put_real(v[k].f(), hc[k], sx1, rows, align_data))) return rc;
     if ((rc = put_real(t[k].f(), ht[k], sx1, sx2, align_data))) return rc;
   }
-  const GreenConst gc = green_const(sx1, sx2, sx3, dx1, dx2, dx3, lambda, mu, gamma);
-  for (int k = 0; k < 3; ++k) {
-    if ((rc = fftn_padded(v[k].f(), sx1, sx2, sx3, dx1, dx2, dx3, -1))) return rc;
-    if ((rc = fftn_padded(t[k].f(), sx1, sx2, 1, dx1, dx2, dx3, -1))) return rc;
-  }
-  if ((rc = launch_elastic_padded(gc, v[0].c(), v[1].c(), v[2].c(), 0))) return rc;
-  if ((rc = launch_surface_padded(gc, v[0].c(), v[1].c(), v[2].c(), t[0].c(), t[1].c(), t[2].c(), p[0].c(),
-                                  p[1].c(), p[2].c(), 0)))
-    return rc;
-  if ((rc = launch_cerruti_padded(gc, p[0].c(), p[1].c(), p[2].c(), v[0].c(), v[1].c(), v[2].c(), 0))) return rc;
-  for (int k = 0; k < 3; ++k) {
-    if ((rc = fftn_padded(v[k].f(), sx1, sx2, sx3, dx1, dx2, dx3, +1))) return rc;
-    if ((rc = fftn_padded(t[k].f(), sx1, sx2, 1, dx1, dx2, dx3, +1))) return rc;
-  }
-  if ((rc = sync_status(RELAX_OK))) return rc;
+  float* vd[3] = {v[0].f(), v[1].f(), v[2].f()};
+  float* td[3] = {t[0].f(), t[1].f(), t[2].f()};
+  float* pd[3] = {p[0].f(), p[1].f(), p[2].f()};
+  if ((rc = green_full_device(vd, td, pd, sx1, sx2, sx3, dx1, dx2, dx3, lambda, mu, gamma))) return rc;
   for (int k = 0; k < 3; ++k) {
     if ((rc = get_real(hc[k], v[k].f(), sx1, rows, align_data))) return rc;
     if ((rc = get_real(ht[k], t[k].f(), sx1, sx2, align_data))) return rc;
@@ -761,6 +775,103 @@ int relax_source(double mu, double slip, double x, double y, double z, double L,
     HIP_TRY(hipMemcpy(th[c], t.f() + c * ts, ts * 4, hipMemcpyDeviceToHost));
   }
   return RELAX_OK;
+}
+
+int relax_tensilesource(double lambda, double mu, double s, double x, double y, double z, double L, double W,
+                        double strike, double dip, double beta, int sx1, int sx2, int sx3, double dx1, double dx2,
+                        double dx3, float* f1, float* f2, float* f3, int align) {
+  if (!f1 || !f2 || !f3 || sx1 < 1 || sx2 < 1 || sx3 < 2) return RELAX_ERR_ARG;
+  int rc;
+  if ((rc = has_device())) return rc;
+  const long ld = sx1 + (align ? 2 : 0), fs = ld * sx2 * sx3;
+  Buf f, pl, op;
+  if ((rc = f.alloc(3 * fs * 4)) || (rc = pl.alloc(sizeof(PlaneDev))) || (rc = op.alloc(sizeof(double)))) return rc;
+  float* fh[3] = {f1, f2, f3};
+  for (int c = 0; c < 3; ++c) HIP_TRY(hipMemcpy(f.f() + c * fs, fh[c], fs * 4, hipMemcpyHostToDevice));
+  const relax_plane_t p{x, y, z, /*width*/ L, /*length*/ W, strike, dip, 0.0, 0, 0};
+  const PlaneDev P = make_plane(p, beta, dx1, dx3);
+  HIP_TRY(hipMemcpy(pl.p, &P, sizeof(PlaneDev), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(op.p, &s, sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = launch_tensile_source((const PlaneDev*)pl.p, (const double*)op.p, 1, lambda, mu, sx1, sx2, sx3, dx1, dx2,
+                                  dx3, f.f(), ld, fs, 0)))
+    return rc;
+  if ((rc = sync_status(RELAX_OK))) return rc;
+  for (int c = 0; c < 3; ++c) HIP_TRY(hipMemcpy(fh[c], f.f() + c * fs, fs * 4, hipMemcpyDeviceToHost));
+  return RELAX_OK;
+}
+
+int relax_mogisource(double lambda, double mu, double o, double xs, double ys, double zs, int sx1, int sx2, int sx3,
+                     double dx1, double dx2, double dx3, float* f1, float* f2, float* f3, int align) {
+  if (!f1 || !f2 || !f3 || sx1 < 1 || sx2 < 1 || sx3 < 2) return RELAX_ERR_ARG;
+  int rc;
+  if ((rc = has_device())) return rc;
+  const long ld = sx1 + (align ? 2 : 0), fs = ld * sx2 * sx3;
+  Buf f, ms;
+  if ((rc = f.alloc(3 * fs * 4)) || (rc = ms.alloc(sizeof(relax_source_t)))) return rc;
+  float* fh[3] = {f1, f2, f3};
+  for (int c = 0; c < 3; ++c) HIP_TRY(hipMemcpy(f.f() + c * fs, fh[c], fs * 4, hipMemcpyHostToDevice));
+  relax_source_t M{};
+  M.slip = o; M.x = xs; M.y = ys; M.z = zs;
+  HIP_TRY(hipMemcpy(ms.p, &M, sizeof(M), hipMemcpyHostToDevice));
+  if ((rc = launch_mogi_source((const relax_source_t*)ms.p, 1, lambda, mu, sx1, sx2, sx3, dx1, dx2, dx3, f.f(), ld,
+                               fs, 0)))
+    return rc;
+  if ((rc = sync_status(RELAX_OK))) return rc;
+  for (int c = 0; c < 3; ++c) HIP_TRY(hipMemcpy(fh[c], f.f() + c * fs, fs * 4, hipMemcpyDeviceToHost));
+  return RELAX_OK;
+}
+
+// a TENSOR field (AoS) through the device's six planes
+static int tensor_roundtrip(relax_tensor_t* sig, long N, const std::function<int(float*)>& body) {
+  int rc;
+  Buf s;
+  if ((rc = s.alloc(N * 24))) return rc;
+  std::vector<float> h(6 * N);
+  for (long i = 0; i < N; ++i) {
+    const relax_tensor_t& t = sig[i];
+    h[i] = t.s11; h[N + i] = t.s12; h[2 * N + i] = t.s13; h[3 * N + i] = t.s22; h[4 * N + i] = t.s23; h[5 * N + i] = t.s33;
+  }
+  HIP_TRY(hipMemcpy(s.p, h.data(), N * 24, hipMemcpyHostToDevice));
+  if ((rc = body(s.f()))) return rc;
+  if ((rc = sync_status(RELAX_OK))) return rc;
+  HIP_TRY(hipMemcpy(h.data(), s.p, N * 24, hipMemcpyDeviceToHost));
+  for (long i = 0; i < N; ++i) sig[i] = relax_tensor_t{h[i], h[N + i], h[2 * N + i], h[3 * N + i], h[4 * N + i], h[5 * N + i]};
+  return RELAX_OK;
+}
+
+int relax_momentdensitytensile(double lambda, double mu, double s, double x, double y, double z, double L, double W,
+                               double strike, double dip, double beta, int sx1, int sx2, int sx3, double dx1,
+                               double dx2, double dx3, relax_tensor_t* sig) {
+  if (!sig || sx1 < 1 || sx2 < 1 || sx3 < 1) return RELAX_ERR_ARG;
+  int rc;
+  if ((rc = has_device())) return rc;
+  Buf pl, op;
+  if ((rc = pl.alloc(sizeof(PlaneDev))) || (rc = op.alloc(sizeof(double)))) return rc;
+  const relax_plane_t p{x, y, z, /*width*/ L, /*length*/ W, strike, dip, 0.0, 0, 0};
+  const PlaneDev P = make_plane(p, beta, dx1, dx3);
+  HIP_TRY(hipMemcpy(pl.p, &P, sizeof(PlaneDev), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(op.p, &s, sizeof(double), hipMemcpyHostToDevice));
+  const long N = (long)sx1 * sx2 * sx3;
+  return tensor_roundtrip(sig, N, [&](float* d) {
+    return launch_add_tensile((const PlaneDev*)pl.p, (const double*)op.p, 1, lambda, mu, sx1, sx2, sx3, dx1, dx2, dx3,
+                              d, N, 0);
+  });
+}
+
+int relax_momentdensitymogi(double lambda, double mu, double o, double xs, double ys, double zs, int sx1, int sx2,
+                            int sx3, double dx1, double dx2, double dx3, relax_tensor_t* sig) {
+  if (!sig || sx1 < 1 || sx2 < 1 || sx3 < 1) return RELAX_ERR_ARG;
+  int rc;
+  if ((rc = has_device())) return rc;
+  Buf ms;
+  if ((rc = ms.alloc(sizeof(relax_source_t)))) return rc;
+  relax_source_t M{};
+  M.slip = o; M.x = xs; M.y = ys; M.z = zs;
+  HIP_TRY(hipMemcpy(ms.p, &M, sizeof(M), hipMemcpyHostToDevice));
+  const long N = (long)sx1 * sx2 * sx3;
+  return tensor_roundtrip(sig, N, [&](float* d) {
+    return launch_add_mogi((const relax_source_t*)ms.p, 1, lambda, mu, sx1, sx2, sx3, dx1, dx2, dx3, d, N, 0);
+  });
 }
 
 int relax_traction(double mu, const relax_source_t* l, int nl, int sx1, int sx2, double dx1, double dx2, double t,

@@ -78,6 +78,26 @@ int launch_source(const PlaneDev* P, const double* slip, int np, double mu, int 
 int launch_traction(const relax_source_t* l, int nl, int n1, int n2, double dx1, double dx2, double t, double Dt,
                     int rate, float* t3, long ld, hipStream_t st);
 
+// compat.hip: greenfunctioncowling over the full depth on device buffers in
+// the padded layout (v: fields of (sx1+2) sx2 sx3, t and p: planes of
+// (sx1+2) sx2), synchronous on the null stream
+int green_full_device(float* const v[3], float* const t[3], float* const p[3], int sx1, int sx2, int sx3,
+                      double dx1, double dx2, double dx3, double lambda, double mu, double gamma);
+
+// model.hip: tensilesource (elastic3d.f90:2315-2411) and mogisource
+// (:2458-2509) of np cracks / point sources into f (3 fields of n1 n2 n3
+// floats in rows of ld, component stride fs) over the full depth, and their
+// moment densities (:3137-3215, :3240-3286) into a tensor field of the
+// top half (6 planes of H)
+int launch_tensile_source(const PlaneDev* P, const double* opening, int np, double lambda, double mu, int n1, int n2,
+                          int n3, double dx1, double dx2, double dx3, float* f, long ld, long fs, hipStream_t st);
+int launch_mogi_source(const relax_source_t* M, int nm, double lambda, double mu, int n1, int n2, int n3,
+                       double dx1, double dx2, double dx3, float* f, long ld, long fs, hipStream_t st);
+int launch_add_tensile(const PlaneDev* P, const double* opening, int np, double lambda, double mu, int n1, int n2,
+                       int h3, double dx1, double dx2, double dx3, float* sig, long H, hipStream_t st);
+int launch_add_mogi(const relax_source_t* M, int nm, double lambda, double mu, int n1, int n2, int h3, double dx1,
+                    double dx2, double dx3, float* sig, long H, hipStream_t st);
+
 // compat.hip: the weak-zone fluidity field of builddgammadot0
 // (viscoelastic3d.f90:320-435) on the device.  A zone in device form holds
 // what depends on the zone only, computed on the host in double as the

@@ -1282,6 +1282,176 @@ int launch_source(const PlaneDev* P, const double* slip, int np, double mu, int 
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }
 
+// tensilesource, elastic3d.f90:2315-2411: one thread per point of the full
+// depth (x3 from the shifted coordinates, so that the images above the
+// surface land in the lower half of the grid), the cracks added in their
+// order with one rounding each
+__global__ void k_tensile_source(const PlaneDev* __restrict__ P, const double* __restrict__ opening, int np,
+                                 double lambda, double mu, int n1, int n2, int n3, double dx1, double dx2, double dx3,
+                                 float* f, long ld, long fs) {
+  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (o >= (long)n1 * n2 * n3) return;
+  const int i1 = o % n1, i2 = (o / n1) % n2, i3 = o / ((long)n1 * n2);
+  const double x1 = d_shifted(i1, n1, dx1), x2 = d_shifted(i2, n2, dx2), x3 = d_shifted(i3, n3, dx3);
+  const long of = i1 + ld * (i2 + (long)n2 * i3);
+  float f1 = f[of], f2 = f[fs + of], f3 = f[2 * fs + of];
+  bool any = false;
+  for (int k = 0; k < np; ++k) {
+    const PlaneDev& p = P[k];
+    if ((fabs(x3 - p.z) > p.Lp) && (fabs(x3 + p.z) > p.Lp)) continue;
+    if ((fabs(x1 - p.x) > p.Wp) || (fabs(x2 - p.y) > p.Wp)) continue;
+    const double cs = p.cstrike, ss = p.sstrike, cd = p.cdip, sd = p.sdip;
+    const double xx = cs * x1 - ss * x2;
+    const double x1s = cd * xx - sd * x3, x1i = cd * xx + sd * x3;
+    if ((fabs(x1s - p.xr) > 7.01 * dx1) && (fabs(x1i - p.xr) > 7.01 * dx1)) continue;
+    const double x2s = ss * x1 + cs * x2, x3s = sd * xx + cd * x3, x3i = -sd * xx + cd * x3;
+    const double scale1 = -opening[k] * (lambda + 2.0 * mu), scale2 = -opening[k] * lambda;
+    double t1 = d_gauss(x1s - p.xr, dx1);
+    const double t2 = d_omega((x2s - p.yr) / p.W, p.beta);
+    double t3 = d_omega((x3s - p.zr) / p.L, p.beta);
+    const double sourc = scale1 * d_gaussp(x1s - p.xr, dx1) * t2 * t3;
+    const double dblcp = scale2 / p.W * t1 * d_omegap((x2s - p.yr) / p.W, p.beta) * t3;
+    const double dipcs = scale2 / p.L * t1 * t2 * d_omegap((x3s - p.zr) / p.L, p.beta);
+    t1 = d_gauss(x1i - p.xr, dx1);
+    t3 = d_omega((x3i + p.zr) / p.L, p.beta);
+    const double image = scale1 * d_gaussp(x1i - p.xr, dx1) * t2 * t3;
+    const double cplei = scale2 / p.W * t1 * d_omegap((x2s - p.yr) / p.W, p.beta) * t3;
+    const double dipci = scale2 / p.L * t1 * t2 * d_omegap((x3i + p.zr) / p.L, p.beta);
+    f1 = (float)((double)f1 + cs * cd * (sourc + image) + ss * (dblcp + cplei) + cs * sd * (dipcs + dipci));
+    f2 = (float)((double)f2 - ss * cd * (sourc + image) + cs * (dblcp + cplei) - ss * sd * (dipcs + dipci));
+    f3 = (float)((double)f3 - sd * (sourc - image) + cd * (dipcs - dipci));
+    any = true;
+  }
+  if (any) { f[of] = f1; f[fs + of] = f2; f[2 * fs + of] = f3; }
+}
+
+// mogisource, elastic3d.f90:2458-2509: point dilatation sources (volume
+// change in slip) and their images over the full depth
+__global__ void k_mogi_source(const relax_source_t* __restrict__ M, int nm, double lambda, double mu, int n1, int n2,
+                              int n3, double dx1, double dx2, double dx3, float* f, long ld, long fs) {
+  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (o >= (long)n1 * n2 * n3) return;
+  const int i1 = o % n1, i2 = (o / n1) % n2, i3 = o / ((long)n1 * n2);
+  const double x1 = d_shifted(i1, n1, dx1), x2 = d_shifted(i2, n2, dx2), x3 = d_shifted(i3, n3, dx3);
+  const double Wp = 6.0 * fmax(dx1, fmax(dx2, dx3)), Lp = Wp;
+  const long of = i1 + ld * (i2 + (long)n2 * i3);
+  float f1 = f[of], f2 = f[fs + of], f3 = f[2 * fs + of];
+  bool any = false;
+  for (int k = 0; k < nm; ++k) {
+    const relax_source_t& s = M[k];
+    if ((fabs(x3 - s.z) > Lp) && (fabs(x3 + s.z) > Lp)) continue;
+    if ((fabs(x1 - s.x) > Wp) || (fabs(x2 - s.y) > Wp)) continue;
+    const double scale = -(lambda + 2.0 * mu / 3.0) * s.slip;
+    const double t1 = d_gauss(x1 - s.x, dx1), t2 = d_gauss(x2 - s.y, dx2);
+    double t3 = d_gauss(x3 - s.z, dx3);
+    const double s1 = scale * d_gaussp(x1 - s.x, dx1) * t2 * t3, s2 = scale * t1 * d_gaussp(x2 - s.y, dx2) * t3;
+    const double s3 = scale * t1 * t2 * d_gaussp(x3 - s.z, dx3);
+    t3 = d_gauss(x3 + s.z, dx3);
+    const double m1 = scale * d_gaussp(x1 - s.x, dx1) * t2 * t3, m2 = scale * t1 * d_gaussp(x2 - s.y, dx2) * t3;
+    const double m3 = scale * t1 * t2 * d_gaussp(x3 + s.z, dx3);
+    f1 = (float)((double)f1 + (s1 + m1));
+    f2 = (float)((double)f2 + (s2 + m2));
+    f3 = (float)((double)f3 + (s3 - m3));
+    any = true;
+  }
+  if (any) { f[of] = f1; f[fs + of] = f2; f[2 * fs + of] = f3; }
+}
+
+// momentdensitytensile, elastic3d.f90:3137-3215: Hooke's law of
+// n sdyad (opening n), n carrying the aperture; each crack adds in turn
+__global__ void k_add_tensile(const PlaneDev* __restrict__ P, const double* __restrict__ opening, int np,
+                              double lambda, double mu, int n1, int n2, int h3, double dx1, double dx2, double dx3,
+                              float* sig, long H) {
+  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (o >= (long)n1 * n2 * h3) return;
+  const int i1 = o % n1, i2 = (o / n1) % n2, i3 = o / ((long)n1 * n2);
+  const double x1 = d_shifted(i1, n1, dx1), x2 = d_shifted(i2, n2, dx2), x3 = (double)i3 * dx3;
+  const float m2 = (float)(2.0 * mu);
+  float acc[6];
+  bool any = false;
+  for (int k = 0; k < np; ++k) {
+    const PlaneDev& p = P[k];
+    if (fabs(x3 - p.z) > p.Lp) continue;
+    if ((fabs(x1 - p.x) > p.Wp) || (fabs(x2 - p.y) > p.Wp)) continue;
+    const double xx = p.cstrike * x1 - p.sstrike * x2;
+    const double x1s = p.cdip * xx - p.sdip * x3, x1i = p.cdip * xx + p.sdip * x3;
+    if ((fabs(x1s - p.xr) > 7.01 * dx1) && (fabs(x1i - p.xr) > 7.01 * dx1)) continue;
+    const double x2s = p.sstrike * x1 + p.cstrike * x2;
+    const double x3s = p.sdip * xx + p.cdip * x3, x3i = -p.sdip * xx + p.cdip * x3;
+    const double t2 = d_omega((x2s - p.yr) / p.W, p.beta);
+    double ap = d_gauss(x1s - p.xr, dx1) * t2 * d_omega((x3s - p.zr) / p.L, p.beta);
+    ap = ap + d_gauss(x1i - p.xr, dx1) * t2 * d_omega((x3i + p.zr) / p.L, p.beta);
+    const double n[3] = {p.n[0] * ap, p.n[1] * ap, p.n[2] * ap};
+    const double b[3] = {opening[k] * n[0], opening[k] * n[1], opening[k] * n[2]};
+    float e[6] = {(float)(n[0] * b[0]), (float)((n[0] * b[1] + n[1] * b[0]) / 2.0),
+                  (float)((n[0] * b[2] + n[2] * b[0]) / 2.0), (float)(n[1] * b[1]),
+                  (float)((n[1] * b[2] + n[2] * b[1]) / 2.0), (float)(n[2] * b[2])};
+    // isotropicstressstrain, elastic3d.f90:129-142
+    const double tr = (double)(e[0] + e[3] + e[5]);
+    for (float& v : e) v = m2 * v;
+    e[0] = (float)((double)e[0] + lambda * tr);
+    e[3] = (float)((double)e[3] + lambda * tr);
+    e[5] = (float)((double)e[5] + lambda * tr);
+    if (!any)
+      for (int c = 0; c < 6; ++c) acc[c] = sig[c * H + o];
+    for (int c = 0; c < 6; ++c) acc[c] = acc[c] + e[c];
+    any = true;
+  }
+  if (any)
+    for (int c = 0; c < 6; ++c) sig[c * H + o] = acc[c];
+}
+
+// momentdensitymogi, elastic3d.f90:3240-3286: kappa o G on the diagonal
+__global__ void k_add_mogi(const relax_source_t* __restrict__ M, int nm, double lambda, double mu, int n1, int n2,
+                           int h3, double dx1, double dx2, double dx3, float* sig, long H) {
+  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (o >= (long)n1 * n2 * h3) return;
+  const int i1 = o % n1, i2 = (o / n1) % n2, i3 = o / ((long)n1 * n2);
+  const double x1 = d_shifted(i1, n1, dx1), x2 = d_shifted(i2, n2, dx2), x3 = (double)i3 * dx3;
+  const double kappa = lambda + 2.0 / 3.0 * mu;
+  const double Wp = 6.0 * fmax(dx1, fmax(dx2, dx3)), Lp = Wp;
+  float a11 = 0.f, a22 = 0.f, a33 = 0.f;
+  bool any = false;
+  for (int k = 0; k < nm; ++k) {
+    const relax_source_t& s = M[k];
+    if (fabs(x3 - s.z) > Lp) continue;
+    if ((fabs(x1 - s.x) > Wp) || (fabs(x2 - s.y) > Wp)) continue;
+    double g = s.slip * d_gauss(x1 - s.x, dx1) * d_gauss(x2 - s.y, dx2) * d_gauss(x3 - s.z, dx3);
+    g = kappa * g;
+    const float m = (float)g;
+    if (!any) { a11 = sig[o]; a22 = sig[3 * H + o]; a33 = sig[5 * H + o]; }
+    a11 = a11 + m; a22 = a22 + m; a33 = a33 + m;
+    any = true;
+  }
+  if (any) { sig[o] = a11; sig[3 * H + o] = a22; sig[5 * H + o] = a33; }
+}
+
+static unsigned blocks_of(long n) { return (unsigned)((n + 255) / 256); }
+
+int launch_tensile_source(const PlaneDev* P, const double* opening, int np, double lambda, double mu, int n1, int n2,
+                          int n3, double dx1, double dx2, double dx3, float* f, long ld, long fs, hipStream_t st) {
+  k_tensile_source<<<blocks_of((long)n1 * n2 * n3), 256, 0, st>>>(P, opening, np, lambda, mu, n1, n2, n3, dx1, dx2,
+                                                                  dx3, f, ld, fs);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+int launch_mogi_source(const relax_source_t* M, int nm, double lambda, double mu, int n1, int n2, int n3,
+                       double dx1, double dx2, double dx3, float* f, long ld, long fs, hipStream_t st) {
+  k_mogi_source<<<blocks_of((long)n1 * n2 * n3), 256, 0, st>>>(M, nm, lambda, mu, n1, n2, n3, dx1, dx2, dx3, f, ld,
+                                                               fs);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+int launch_add_tensile(const PlaneDev* P, const double* opening, int np, double lambda, double mu, int n1, int n2,
+                       int h3, double dx1, double dx2, double dx3, float* sig, long H, hipStream_t st) {
+  k_add_tensile<<<blocks_of((long)n1 * n2 * h3), 256, 0, st>>>(P, opening, np, lambda, mu, n1, n2, h3, dx1, dx2, dx3,
+                                                               sig, H);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+int launch_add_mogi(const relax_source_t* M, int nm, double lambda, double mu, int n1, int n2, int h3, double dx1,
+                    double dx2, double dx3, float* sig, long H, hipStream_t st) {
+  k_add_mogi<<<blocks_of((long)n1 * n2 * h3), 256, 0, st>>>(M, nm, lambda, mu, n1, n2, h3, dx1, dx2, dx3, sig, H);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
 // traction, elastic3d.f90:2517-2590: one thread per surface point, the
 // loads subtracted in their order, each rounded to FP32 as the reference's
 // REAL() does
@@ -1324,59 +1494,128 @@ static PlaneDev source_plane(const relax_source_t& s, double dx1, double dx3) {
   return make_plane(p, s.beta, dx1, dx3);
 }
 
-int relax_model_event(relax_model_t* m, const relax_source_t* s, int ns, const relax_source_t* l, int nl,
-                      double nyquist) {
-  if (!m || m->batch || ns < 0 || nl < 0 || (ns > 0 && !s) || (nl > 0 && !l)) return RELAX_ERR_ARG;
+// u += the top half of a full-depth field in the padded layout
+__global__ void k_add_padded_top(float* __restrict__ u, const float* __restrict__ v, int n1, int n2, int h3, long H,
+                                 long vs) {
+  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (o >= H) return;
+  const int i1 = o % n1;
+  const long r = o / n1;  // row (i2, i3)
+  const long q = i1 + (long)(n1 + 2) * r;
+  for (int c = 0; c < 3; ++c) u[c * H + o] = u[c * H + o] + v[c * vs + q];
+}
+
+int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist) {
+  if (!m || m->batch || !ev || ev->ns < 0 || ev->nt < 0 || ev->nm < 0 || ev->nl < 0 || (ev->ns > 0 && !ev->s) ||
+      (ev->nt > 0 && !ev->ts) || (ev->nm > 0 && !ev->m) || (ev->nl > 0 && !ev->l))
+    return RELAX_ERR_ARG;
   int rc;
   if ((rc = sync_md(m))) return rc;
   const Grid& g = m->g;
   const long H = g.H;
-  // dislocations (relax.f90:1116-1289): the source() body force of the
-  // patches longer or wider than nyquist*min(dx) (:1140-1150), then every
-  // patch's moment density into tau (:1199-1206).  The staged list holds the
-  // large patches first and then all of them.
+  const int ns = ev->ns, nt = ev->nt, nm = ev->nm, nl = ev->nl;
+  // dislocations (relax.f90:1116-1289): the body forces of the shear
+  // patches (source(), :1140-1150) and tensile cracks (tensilesource(),
+  // :1228-1239) longer or wider than nyquist*min(dx) and of every point
+  // dilatation source (mogisource(), :1262-1268); every source's moment
+  // density into tau.  The staged plane list holds the large shear patches,
+  // all shear patches, the large cracks and all cracks, in that order.
   const double dmin = std::min(m->desc.dx1, std::min(m->desc.dx2, m->desc.dx3)) * nyquist;
   std::vector<PlaneDev>& hp = m->shear_host;
-  std::vector<double> slip;
+  std::vector<double> amp;
   hp.clear();
+  auto push = [&](const relax_source_t& x, double beta, double a) {
+    relax_source_t y = x;
+    y.beta = beta;
+    hp.push_back(source_plane(y, m->desc.dx1, m->desc.dx3));
+    amp.push_back(a);
+  };
   for (int k = 0; k < ns; ++k)
-    if (dmin < s[k].length || dmin < s[k].width) {
-      hp.push_back(source_plane(s[k], m->desc.dx1, m->desc.dx3));
-      slip.push_back(s[k].slip);
-    }
+    if (dmin < ev->s[k].length || dmin < ev->s[k].width) push(ev->s[k], ev->s[k].beta, ev->s[k].slip);
   const int nbig = (int)hp.size();
-  for (int k = 0; k < ns; ++k) {
-    hp.push_back(source_plane(s[k], m->desc.dx1, m->desc.dx3));
-    slip.push_back(s[k].slip);
-  }
-  HIP_TRY(hipMemsetAsync(m->W, 0, 3 * H * 4, m->stream));
-  HIP_TRY(hipMemsetAsync(m->T, 0, 3L * g.n1 * g.n2 * 4, m->stream));
-  if (!hp.empty() && (rc = stage_planes(m, slip.data()))) return rc;
-  if (nbig && (rc = launch_source(m->shear_dev, m->shear_slip, nbig, m->desc.mu, g.n1, g.n2, g.h3, m->desc.dx1,
-                                  m->desc.dx2, m->desc.dx3, m->W, g.n1, H, m->T, (long)g.n1 * g.n2, m->stream)))
-    return rc;
-  // the surface loads at the model's time (relax.f90:319, :874)
-  if (nl > 0) {
-    if (nl > m->loads_cap) {
+  for (int k = 0; k < ns; ++k) push(ev->s[k], ev->s[k].beta, ev->s[k].slip);
+  // tensile cracks take the model's taper (dislocations passes its beta)
+  const int t0 = (int)hp.size();
+  for (int k = 0; k < nt; ++k)
+    if (dmin < ev->ts[k].length || dmin < ev->ts[k].width) push(ev->ts[k], m->desc.beta, ev->ts[k].opening);
+  const int ntbig = (int)hp.size() - t0;
+  for (int k = 0; k < nt; ++k) push(ev->ts[k], m->desc.beta, ev->ts[k].opening);
+  if (!hp.empty() && (rc = stage_planes(m, amp.data()))) return rc;
+  // the surface loads and the point sources, staged together
+  if (nl + nm > 0) {
+    if (nl + nm > m->loads_cap) {
       if (m->loads_dev) (void)hipFree(m->loads_dev);
       m->loads_dev = nullptr;
       m->loads_cap = 0;
-      HIP_TRY(hipMalloc(&m->loads_dev, sizeof(relax_source_t) * nl));
-      m->loads_cap = nl;
+      HIP_TRY(hipMalloc(&m->loads_dev, sizeof(relax_source_t) * (nl + nm)));
+      m->loads_cap = nl + nm;
     }
-    HIP_TRY(hipMemcpyAsync(m->loads_dev, l, sizeof(relax_source_t) * nl, hipMemcpyHostToDevice, m->stream));
-    double t = 0.0;
-    HIP_TRY(hipMemcpyAsync(&t, &m->sc->t, sizeof(double), hipMemcpyDeviceToHost, m->stream));
-    HIP_TRY(hipStreamSynchronize(m->stream));
-    if ((rc = launch_traction(m->loads_dev, nl, g.n1, g.n2, m->desc.dx1, m->desc.dx2, t, 0.0, 0,
-                              m->T + 2L * g.n1 * g.n2, g.n1, m->stream)))
-      return rc;
+    if (nl) HIP_TRY(hipMemcpyAsync(m->loads_dev, ev->l, sizeof(relax_source_t) * nl, hipMemcpyHostToDevice, m->stream));
+    if (nm)
+      HIP_TRY(hipMemcpyAsync(m->loads_dev + nl, ev->m, sizeof(relax_source_t) * nm, hipMemcpyHostToDevice, m->stream));
   }
-  // u += G[f, t] (relax.f90:362-371, :877-884), tau, sig = -tau + C:eps(u)
+  double t = 0.0;  // the model's time, for the loads (relax.f90:319, :874)
+  HIP_TRY(hipMemcpyAsync(&t, &m->sc->t, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  const double dx1 = m->desc.dx1, dx2 = m->desc.dx2, dx3 = m->desc.dx3, lam = m->desc.lambda, mu = m->desc.mu;
+  if (nt == 0 && nm == 0) {
+    // body forces in the top half only: the pruned Green solve of the step,
+    // adding to u (relax.f90:362-371, :877-884)
+    HIP_TRY(hipMemsetAsync(m->W, 0, 3 * H * 4, m->stream));
+    HIP_TRY(hipMemsetAsync(m->T, 0, 3L * g.n1 * g.n2 * 4, m->stream));
+    if (nbig && (rc = launch_source(m->shear_dev, m->shear_slip, nbig, mu, g.n1, g.n2, g.h3, dx1, dx2, dx3, m->W,
+                                    g.n1, H, m->T, (long)g.n1 * g.n2, m->stream)))
+      return rc;
+    if (nl && (rc = launch_traction(m->loads_dev, nl, g.n1, g.n2, dx1, dx2, t, 0.0, 0, m->T + 2L * g.n1 * g.n2, g.n1,
+                                    m->stream)))
+      return rc;
+    Ctx c = single(m);
+    c.prof = nullptr;
+    if ((rc = model_green(c, EPI_ADD))) return rc;
+  } else {
+    // tensile cracks and point sources put images above the surface, in the
+    // lower half of the grid: the full-depth Green solve on scratch fields
+    // in the padded layout
+    const long ld = g.n1 + 2, vs = ld * g.n2 * g.n3, ts = ld * g.n2;
+    float* scratch = nullptr;
+    const size_t bytes = (size_t)(3 * vs + 6 * ts) * 4;
+    HIP_TRY(hipMalloc(&scratch, bytes));
+    float* v[3] = {scratch, scratch + vs, scratch + 2 * vs};
+    float* tr[3] = {scratch + 3 * vs, scratch + 3 * vs + ts, scratch + 3 * vs + 2 * ts};
+    float* pp[3] = {scratch + 3 * vs + 3 * ts, scratch + 3 * vs + 4 * ts, scratch + 3 * vs + 5 * ts};
+    rc = hipMemsetAsync(scratch, 0, bytes, m->stream) == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+    if (!rc && nbig)
+      rc = launch_source(m->shear_dev, m->shear_slip, nbig, mu, g.n1, g.n2, g.h3, dx1, dx2, dx3, v[0], ld, vs, tr[0],
+                         ts, m->stream);
+    if (!rc && ntbig)
+      rc = launch_tensile_source(m->shear_dev + t0, m->shear_slip + t0, ntbig, lam, mu, g.n1, g.n2, g.n3, dx1, dx2,
+                                 dx3, v[0], ld, vs, m->stream);
+    if (!rc && nm)
+      rc = launch_mogi_source(m->loads_dev + nl, nm, lam, mu, g.n1, g.n2, g.n3, dx1, dx2, dx3, v[0], ld, vs,
+                              m->stream);
+    if (!rc && nl)
+      rc = launch_traction(m->loads_dev, nl, g.n1, g.n2, dx1, dx2, t, 0.0, 0, tr[2], ld, m->stream);
+    if (!rc && hipStreamSynchronize(m->stream) != hipSuccess) rc = RELAX_ERR_HIP;
+    if (!rc) rc = green_full_device(v, tr, pp, g.n1, g.n2, g.n3, dx1, dx2, dx3, lam, mu, m->desc.gamma);
+    if (!rc) {
+      k_add_padded_top<<<blocks_of(H), 256, 0, m->stream>>>(m->u, scratch, g.n1, g.n2, g.h3, H, vs);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(m->stream) != hipSuccess) rc = RELAX_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(m->stream);
+    (void)hipFree(scratch);
+    if (rc) return rc;
+  }
+  // the moment densities of every source into tau, then
+  // sig = -tau + C:eps(u) (relax.f90:887-889)
+  if ((rc = add_shear(m, nbig, ns))) return rc;
+  if (nt && (rc = launch_add_tensile(m->shear_dev + t0 + ntbig, m->shear_slip + t0 + ntbig, nt, lam, mu, g.n1, g.n2,
+                                     g.h3, dx1, dx2, dx3, m->tau, H, m->stream)))
+    return rc;
+  if (nm && (rc = launch_add_mogi(m->loads_dev + nl, nm, lam, mu, g.n1, g.n2, g.h3, dx1, dx2, dx3, m->tau, H,
+                                  m->stream)))
+    return rc;
   Ctx c = single(m);
   c.prof = nullptr;
-  if ((rc = model_green(c, EPI_ADD))) return rc;
-  if ((rc = add_shear(m, nbig, ns))) return rc;
   if ((rc = launch_stress(STRESS_SET, g, m->S, c.md, c.one, 1, RHEO_VISCOUS, m->stream))) return rc;
   HIP_TRY(hipStreamSynchronize(m->stream));
   return RELAX_OK;

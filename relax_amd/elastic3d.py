@@ -70,6 +70,40 @@ def traction(mu, loads, dx1, dx2, t, Dt, t3, rate=False, align_data=1):
                                 int(bool(align_data))), "traction")
 
 
+def tensilesource(lam, mu, s, x, y, z, L, W, strike, dip, beta, dx1, dx2, dx3, f1, f2, f3, align_data=1):
+    """tensilesource, elastic3d.f90:2315-2411: adds the body force of an
+    opening crack to the whole depth of f1..f3 ((sx3, sx2, sx1+2) with
+    align_data=1, else (sx3, sx2, sx1)), in place."""
+    sx3, sx2, p1 = f1.shape
+    sx1 = p1 - 2 if align_data else p1
+    check(load().relax_tensilesource(lam, mu, s, x, y, z, L, W, strike, dip, beta, sx1, sx2, sx3, dx1, dx2, dx3,
+                                     ptr(f32(f1, "f1")), ptr(f32(f2, "f2")), ptr(f32(f3, "f3")),
+                                     int(bool(align_data))), "tensilesource")
+
+
+def mogisource(lam, mu, o, x, y, z, dx1, dx2, dx3, f1, f2, f3, align_data=1):
+    """mogisource, elastic3d.f90:2458-2509: adds the body force of a point
+    dilatation source (volume change o) to the whole depth of f1..f3."""
+    sx3, sx2, p1 = f1.shape
+    sx1 = p1 - 2 if align_data else p1
+    check(load().relax_mogisource(lam, mu, o, x, y, z, sx1, sx2, sx3, dx1, dx2, dx3, ptr(f32(f1, "f1")),
+                                  ptr(f32(f2, "f2")), ptr(f32(f3, "f3")), int(bool(align_data))), "mogisource")
+
+
+def momentdensitytensile(lam, mu, s, x, y, z, L, W, strike, dip, beta, dx1, dx2, dx3, sig):
+    """momentdensitytensile, elastic3d.f90:3137-3215 (adds to sig in place)."""
+    sx3, sx2, sx1, _ = sig.shape
+    check(load().relax_momentdensitytensile(lam, mu, s, x, y, z, L, W, strike, dip, beta, sx1, sx2, sx3, dx1, dx2,
+                                            dx3, ptr(f32(sig, "sig"))), "momentdensitytensile")
+
+
+def momentdensitymogi(lam, mu, o, x, y, z, dx1, dx2, dx3, sig):
+    """momentdensitymogi, elastic3d.f90:3240-3286 (adds to sig in place)."""
+    sx3, sx2, sx1, _ = sig.shape
+    check(load().relax_momentdensitymogi(lam, mu, o, x, y, z, sx1, sx2, sx3, dx1, dx2, dx3, ptr(f32(sig, "sig"))),
+          "momentdensitymogi")
+
+
 def momentdensityshear(mu, slip, x, y, z, L, W, strike, dip, rake, beta, dx1, dx2, dx3, sig):
     """momentdensityshear, elastic3d.f90:2661-2750 (adds to sig in place)."""
     sx3, sx2, sx1, _ = sig.shape

@@ -16,6 +16,8 @@
 !   transienteigenstress               viscoelastic3d.f90:123-202
 !   builddgammadot0                    viscoelastic3d.f90:320-435
 !   source, traction                   elastic3d.f90:2064-2252, :2517-2590
+!   tensilesource, mogisource          elastic3d.f90:2315-2411, :2458-2509
+!   momentdensitytensile/-mogi         elastic3d.f90:3137-3215, :3240-3286
 !   frictioneigenstress                friction3d.f90:82-241
 !   monitorfriction, frictionadd       friction3d.f90:263-453
 ! A STOP of the reference becomes a non-zero relax_hip_status and a message
@@ -106,6 +108,17 @@ MODULE relax_hip
   TYPE, PUBLIC :: relax_model
      TYPE(c_ptr) :: h=C_NULL_PTR
   END TYPE relax_model
+  !> the source lists of one event (relax_event_t)
+  TYPE, BIND(C), PUBLIC :: relax_event_c
+     TYPE(c_ptr) :: s=C_NULL_PTR
+     INTEGER(c_int) :: ns=0
+     TYPE(c_ptr) :: ts=C_NULL_PTR
+     INTEGER(c_int) :: nt=0
+     TYPE(c_ptr) :: m=C_NULL_PTR
+     INTEGER(c_int) :: nm=0
+     TYPE(c_ptr) :: l=C_NULL_PTR
+     INTEGER(c_int) :: nl=0
+  END TYPE relax_event_c
   INTEGER, PARAMETER, PUBLIC :: RELAX_LINEAR=0, RELAX_POWERLAW=1, RELAX_FAULTCREEP=2, &
        RELAX_LTRANSIENT=3, RELAX_NLTRANSIENT=4
 
@@ -293,11 +306,38 @@ MODULE relax_hip
        TYPE(c_ptr), VALUE :: h, patches, slip
        INTEGER(c_int), VALUE :: n
      END FUNCTION
-     INTEGER(c_int) FUNCTION c_model_event(h,s,ns,l,nl,nyquist) BIND(C,name="relax_model_event")
+     INTEGER(c_int) FUNCTION c_model_event(h,ev,nyquist) BIND(C,name="relax_model_event")
        IMPORT :: c_int, c_double, c_ptr
-       TYPE(c_ptr), VALUE :: h,s,l
-       INTEGER(c_int), VALUE :: ns,nl
+       TYPE(c_ptr), VALUE :: h,ev
        REAL(c_double), VALUE :: nyquist
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_tensilesource(lambda,mu,s,x,y,z,L,W,strike,dip,beta,sx1,sx2,sx3,dx1,dx2,dx3, &
+          f1,f2,f3,al) BIND(C,name="relax_tensilesource")
+       IMPORT :: c_int, c_double, c_ptr
+       REAL(c_double), VALUE :: lambda,mu,s,x,y,z,L,W,strike,dip,beta,dx1,dx2,dx3
+       INTEGER(c_int), VALUE :: sx1,sx2,sx3,al
+       TYPE(c_ptr), VALUE :: f1,f2,f3
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_mogisource(lambda,mu,o,x,y,z,sx1,sx2,sx3,dx1,dx2,dx3,f1,f2,f3,al) &
+          BIND(C,name="relax_mogisource")
+       IMPORT :: c_int, c_double, c_ptr
+       REAL(c_double), VALUE :: lambda,mu,o,x,y,z,dx1,dx2,dx3
+       INTEGER(c_int), VALUE :: sx1,sx2,sx3,al
+       TYPE(c_ptr), VALUE :: f1,f2,f3
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_mdtensile(lambda,mu,s,x,y,z,L,W,strike,dip,beta,sx1,sx2,sx3,dx1,dx2,dx3,sig) &
+          BIND(C,name="relax_momentdensitytensile")
+       IMPORT :: c_int, c_double, c_ptr
+       REAL(c_double), VALUE :: lambda,mu,s,x,y,z,L,W,strike,dip,beta,dx1,dx2,dx3
+       INTEGER(c_int), VALUE :: sx1,sx2,sx3
+       TYPE(c_ptr), VALUE :: sig
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_mdmogi(lambda,mu,o,x,y,z,sx1,sx2,sx3,dx1,dx2,dx3,sig) &
+          BIND(C,name="relax_momentdensitymogi")
+       IMPORT :: c_int, c_double, c_ptr
+       REAL(c_double), VALUE :: lambda,mu,o,x,y,z,dx1,dx2,dx3
+       INTEGER(c_int), VALUE :: sx1,sx2,sx3
+       TYPE(c_ptr), VALUE :: sig
      END FUNCTION
      INTEGER(c_int) FUNCTION c_model_coseismic(h) BIND(C,name="relax_model_coseismic")
        IMPORT :: c_int, c_ptr
@@ -478,6 +518,37 @@ CONTAINS
     CALL check(c_source(mu,s,x,y,z,L,W,strike,dip,rake,beta,sx1,sx2,sx3,dx1,dx2,dx3, &
          C_LOC(f1),C_LOC(f2),C_LOC(f3),C_LOC(t1),C_LOC(t2),C_LOC(t3),relax_hip_align),"source")
   END SUBROUTINE source
+
+  SUBROUTINE tensilesource(lambda,mu,s,x,y,z,L,W,strike,dip,beta,sx1,sx2,sx3,dx1,dx2,dx3,f1,f2,f3)
+    INTEGER, INTENT(IN) :: sx1,sx2,sx3
+    REAL*8, INTENT(IN) :: lambda,mu,s,x,y,z,L,W,strike,dip,beta,dx1,dx2,dx3
+    REAL*4, INTENT(INOUT), DIMENSION(*), TARGET :: f1,f2,f3
+    CALL check(c_tensilesource(lambda,mu,s,x,y,z,L,W,strike,dip,beta,sx1,sx2,sx3,dx1,dx2,dx3, &
+         C_LOC(f1),C_LOC(f2),C_LOC(f3),relax_hip_align),"tensilesource")
+  END SUBROUTINE tensilesource
+
+  SUBROUTINE mogisource(lambda,mu,o,xs,ys,zs,sx1,sx2,sx3,dx1,dx2,dx3,f1,f2,f3)
+    INTEGER, INTENT(IN) :: sx1,sx2,sx3
+    REAL*8, INTENT(IN) :: lambda,mu,o,xs,ys,zs,dx1,dx2,dx3
+    REAL*4, INTENT(INOUT), DIMENSION(*), TARGET :: f1,f2,f3
+    CALL check(c_mogisource(lambda,mu,o,xs,ys,zs,sx1,sx2,sx3,dx1,dx2,dx3,C_LOC(f1),C_LOC(f2),C_LOC(f3), &
+         relax_hip_align),"mogisource")
+  END SUBROUTINE mogisource
+
+  SUBROUTINE momentdensitytensile(lambda,mu,slip,x,y,z,L,W,strike,dip,rake,beta,sx1,sx2,sx3,dx1,dx2,dx3,sig)
+    INTEGER, INTENT(IN) :: sx1,sx2,sx3
+    REAL*8, INTENT(IN) :: lambda,mu,slip,x,y,z,L,W,strike,dip,rake,beta,dx1,dx2,dx3
+    TYPE(TENSOR), INTENT(INOUT), DIMENSION(sx1,sx2,sx3), TARGET :: sig
+    CALL check(c_mdtensile(lambda,mu,slip,x,y,z,L,W,strike,dip,beta,sx1,sx2,sx3,dx1,dx2,dx3,C_LOC(sig)), &
+         "momentdensitytensile")
+  END SUBROUTINE momentdensitytensile
+
+  SUBROUTINE momentdensitymogi(lambda,mu,o,xs,ys,zs,sx1,sx2,sx3,dx1,dx2,dx3,sig)
+    INTEGER, INTENT(IN) :: sx1,sx2,sx3
+    REAL*8, INTENT(IN) :: lambda,mu,o,xs,ys,zs,dx1,dx2,dx3
+    TYPE(TENSOR), INTENT(INOUT), DIMENSION(sx1,sx2,sx3), TARGET :: sig
+    CALL check(c_mdmogi(lambda,mu,o,xs,ys,zs,sx1,sx2,sx3,dx1,dx2,dx3,C_LOC(sig)),"momentdensitymogi")
+  END SUBROUTINE momentdensitymogi
 
   !> traction (elastic3d.f90:2517-2590) over a list of surface loads
   SUBROUTINE relax_hip_traction(mu,l,sx1,sx2,dx1,dx2,t,Dt,t3,rate)
@@ -677,15 +748,27 @@ CONTAINS
   !> a coseismic event: the first at t = 0 (relax.f90:316-371) or a later
   !! one once a step has reached its time (info%e advanced, the event switch
   !! of relax.f90:858-889): dislocations, traction, Green, stress
-  SUBROUTINE relax_hip_model_event(model,s,l,nyquist)
+  !! (shear patches s, surface loads l; optional tensile cracks ts and point
+  !! dilatation sources m)
+  SUBROUTINE relax_hip_model_event(model,s,l,nyquist,ts,m)
     TYPE(relax_model), INTENT(IN) :: model
     TYPE(SOURCE_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: s,l
     REAL*8, INTENT(IN) :: nyquist
-    TYPE(c_ptr) :: ps,pl
-    ps=C_NULL_PTR; pl=C_NULL_PTR
-    IF (SIZE(s) .GT. 0) ps=C_LOC(s)
-    IF (SIZE(l) .GT. 0) pl=C_LOC(l)
-    CALL check(c_model_event(model%h,ps,INT(SIZE(s),c_int),pl,INT(SIZE(l),c_int),nyquist),"relax_model_event")
+    TYPE(SOURCE_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS, OPTIONAL :: ts,m
+    TYPE(relax_event_c), TARGET :: ev
+    ev%ns=INT(SIZE(s),c_int)
+    IF (ev%ns .GT. 0) ev%s=C_LOC(s)
+    ev%nl=INT(SIZE(l),c_int)
+    IF (ev%nl .GT. 0) ev%l=C_LOC(l)
+    IF (PRESENT(ts)) THEN
+       ev%nt=INT(SIZE(ts),c_int)
+       IF (ev%nt .GT. 0) ev%ts=C_LOC(ts)
+    END IF
+    IF (PRESENT(m)) THEN
+       ev%nm=INT(SIZE(m),c_int)
+       IF (ev%nm .GT. 0) ev%m=C_LOC(m)
+    END IF
+    CALL check(c_model_event(model%h,C_LOC(ev),nyquist),"relax_model_event")
   END SUBROUTINE relax_hip_model_event
 
 END MODULE relax_hip

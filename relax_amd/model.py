@@ -3,7 +3,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import LAYER, PATCH, PLANE, SOURCE, TENSOR_LAYER, WEAK, ModelDesc, StepInfo, check, load, ptr
+from ._lib import LAYER, PATCH, PLANE, SOURCE, TENSOR_LAYER, WEAK, EventDesc, ModelDesc, StepInfo, check, load, ptr
 
 # mechanisms of set_structure (relax.f90:565-650): linear and power-law
 # viscosity, fault creep, linear and power-law transient creep
@@ -147,16 +147,19 @@ class Model:
         """u = Green(body force of tau), sig = -tau + C:eps(u)."""
         check(self._lib.relax_model_coseismic(self._h), "coseismic")
 
-    def event(self, sources, loads=None, nyquist=1.0):
+    def event(self, sources=None, loads=None, nyquist=1.0, tensile=None, mogi=None):
         """A coseismic event (dislocations, relax.f90:1116-1289, and traction):
         the first at t = 0 on a new model, a later one once a step has reached
-        its time (the step's "e" advanced).  sources: SOURCE array of shear
-        patches; loads: SOURCE array of surface loads or None.  u += Green(the
-        source() body force of the patches larger than nyquist*min(dx), the
-        loads), tau += their moment density, sig = -tau + C:eps(u)."""
-        s = np.ascontiguousarray(sources, SOURCE)
-        ld = np.ascontiguousarray(np.zeros(0, SOURCE) if loads is None else loads, SOURCE)
-        check(self._lib.relax_model_event(self._h, ptr(s), len(s), ptr(ld), len(ld), float(nyquist)), "event")
+        its time (the step's "e" advanced).  SOURCE arrays (or None) of shear
+        patches, surface loads, tensile cracks (opening) and point dilatation
+        sources (slip = volume change).  u += Green(the body forces of the
+        sources larger than nyquist*min(dx), of every point source, and the
+        loads), tau += their moment densities, sig = -tau + C:eps(u)."""
+        arr = [np.ascontiguousarray(np.zeros(0, SOURCE) if a is None else a, SOURCE)
+               for a in (sources, tensile, mogi, loads)]
+        ev = EventDesc(ptr(arr[0]), len(arr[0]), ptr(arr[1]), len(arr[1]), ptr(arr[2]), len(arr[2]), ptr(arr[3]),
+                       len(arr[3]))
+        check(self._lib.relax_model_event(self._h, ctypes.byref(ev), float(nyquist)), "event")
 
     def profile(self, nsteps):
         """Run nsteps steps with HIP-event phase timers; returns

@@ -110,6 +110,27 @@ def loads(deck, beta):
     return out
 
 
+def tensile(deck, beta):
+    """Convert deck tensile-crack lines (opening, x, y, z, length, width,
+    strike, dip) to a SOURCE array as the parser leaves them
+    (input.f90:1398-1442: Wang's convention of the geometry; slip holds the
+    negated opening, the opening is kept)."""
+    out = np.zeros(len(deck or ()), SOURCE)
+    for k, (op, x, y, z, L, W, st, dp) in enumerate(deck or ()):
+        s2, x2, y2, z2, st2, dp2, _ = wangconvention(op, x, y, z, L, W, st, dp, 0.0)
+        out[k] = (s2, x2, y2, z2, W, L, st2, dp2, 0.0, 0.0, 0.0, beta, op)
+    return out
+
+
+def mogi(deck):
+    """Convert deck point-source lines (volume change, x, y, z) to a SOURCE
+    array (input.f90:1470-1490, no frame rotation)."""
+    out = np.zeros(len(deck or ()), SOURCE)
+    for k, (o, x, y, z) in enumerate(deck or ()):
+        out[k]["slip"], out[k]["x"], out[k]["y"], out[k]["z"] = o, x, y, z
+    return out
+
+
 def friction_plane(x, y, z, length, width, strike, dip, rake, dx2, dx3):
     """A fault-creep plane as the parser builds it (input.f90:1079-1093): the
     rake is turned by 180 degrees, the geometry follows Wang's convention and

@@ -133,7 +133,8 @@ def planes(s):
 
 def event_times(s):
     """The spec's event times: the set-up at t = 0, then its later events
-    (s["events"]: a list of (time, deck) or (time, deck, load deck))."""
+    (s["events"]: a list of (time, deck[, load deck[, tensile deck[, point
+    source deck]]]))."""
     return (0.0,) + tuple(float(ev[0]) for ev in s.get("events", ()))
 
 
@@ -159,19 +160,21 @@ def run(m, s, nsteps, dt_replay=None):
         recs.extend(got)
         if got[-1]["e"] != e0:
             # the step's event index is 0-based; event_sources counts from 1
-            m.event(*event_sources(s, got[-1]["e"] + 1), nyquist=s.get("nyquist", NYQUIST))
+            m.event(nyquist=s.get("nyquist", NYQUIST), **event_sources(s, got[-1]["e"] + 1))
     return recs
 
 
 def event_sources(s, e):
-    """(shear patches, surface loads) of event e (1 = the set-up at t = 0) as
-    SOURCE arrays."""
+    """The source lists of event e (1 = the set-up at t = 0) as SOURCE arrays:
+    {sources, loads, tensile, mogi} (shear patches, surface loads, tensile
+    cracks, point dilatation sources)."""
     if e == 1:
-        deck, loads = s["deck"], s.get("loads")
+        decks = (s["deck"], s.get("loads"), s.get("tensile"), s.get("mogi"))
     else:
-        ev = s["events"][e - 2]
-        deck, loads = ev[1], ev[2] if len(ev) > 2 else None
-    return structure.sources(deck, BETA), structure.loads(loads, BETA)
+        ev = tuple(s["events"][e - 2][1:])
+        decks = ev + (None,) * (4 - len(ev))
+    return dict(sources=structure.sources(decks[0], BETA), loads=structure.loads(decks[1], BETA),
+                tensile=structure.tensile(decks[2], BETA), mogi=structure.mogi(decks[3]))
 
 
 def configure(m, s):
@@ -192,5 +195,5 @@ def configure(m, s):
         m.set_planes(pl)
     m.set_structure(FAULTCREEP, t["friction"])
     # the first event at t = 0 (relax.f90:316-371)
-    m.event(*event_sources(s, 1), nyquist=s.get("nyquist", NYQUIST))
+    m.event(nyquist=s.get("nyquist", NYQUIST), **event_sources(s, 1))
     return m

@@ -66,8 +66,9 @@ def oracle_model(s):
     m["nyquist"] = s.get("nyquist", workloads.NYQUIST)
     if s.get("events"):
         srcs = [workloads.event_sources(s, e) for e in range(2, len(s["events"]) + 2)]
-        m["event_sources"] = [a.astype(O.SOURCE) for a, _ in srcs]
-        m["event_loads"] = [b.astype(O.SOURCE) for _, b in srcs]
+        for key, name in (("event_sources", "sources"), ("event_loads", "loads"), ("event_tensile", "tensile"),
+                          ("event_mogi", "mogi")):
+            m[key] = [d[name].astype(O.SOURCE) for d in srcs]
     h3 = s["sx3"] // 2
     keys = {LINEAR: "dg_linear", POWERLAW: "dg_nonlinear", LTRANSIENT: "dg_ltransient",
             NLTRANSIENT: "dg_nltransient"}
@@ -91,9 +92,9 @@ def oracle_initial_state(s, dtype=np.float32):
     tau = np.zeros((h3, sx2, sx1, 6), dtype)
     c = [np.zeros((sx3, sx2, sx1 + 2), dtype) for _ in range(3)]
     t = [np.zeros((sx2, sx1 + 2), dtype) for _ in range(3)]
-    src, loads = workloads.event_sources(s, 1)
-    O.dislocations(src.astype(O.SOURCE), loads.astype(O.SOURCE), workloads.MU, s.get("nyquist", workloads.NYQUIST),
-                   0.0, dx, dx, dx, c, t, tau)
+    ev = {k: v.astype(O.SOURCE) for k, v in workloads.event_sources(s, 1).items()}
+    O.dislocations(ev["sources"], ev["loads"], workloads.MU, s.get("nyquist", workloads.NYQUIST), 0.0, dx, dx, dx, c,
+                   t, tau, lam=workloads.LAM, beta=workloads.BETA, tensile=ev["tensile"], mogi=ev["mogi"])
     O.greenfunctioncowling(c[0], c[1], c[2], t[0], t[1], t[2], dx, dx, dx, workloads.LAM, workloads.MU,
                            workloads.GAM)
     u = [np.ascontiguousarray(a[:h3]) for a in c]

@@ -80,7 +80,7 @@ def test_fortran_model_loop_matches_python_driver(tmp_path):
         pat[0] = (1e-3, 0.0, 0.0, 1.0, 6.0, 10.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.2, 0.0)
         load = np.zeros(1, SOURCE)
         load[0] = (0.5, 3.0, -2.0, 0.0, 3.0, 4.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.2, 0.0)
-        m.event(pat, load, nyquist=1.0)
+        m.event(pat, load, nyquist=1.0)  # shear patch and load
         recs = [m.step() for _ in range(nsteps)]
         ref = m.download()
     np.testing.assert_array_equal(scal, [[r["Dt"], r["t"], r["power"], r["cei"]] for r in recs])

@@ -117,7 +117,7 @@ def test_batch_refuses_mismatched_models_and_member_steps():
         with pytest.raises(RelaxError):
             b.add(a)
         with pytest.raises(RelaxError):  # events are applied to models outside a batch
-            a.event(*workloads.event_sources(_variant("maxwell", 0), 1))
+            a.event(**workloads.event_sources(_variant("maxwell", 0), 1))
     a.step()  # free again once the batch is gone
     a.close()
     other.close()

@@ -206,6 +206,40 @@ def test_source(case, align):
     np.testing.assert_array_equal(f[0][sx3 // 2:], f0[0][sx3 // 2:])  # the bottom half is not touched
 
 
+@pytest.mark.parametrize("align", [1, 0])
+def test_tensile_and_mogi_sources(align):
+    """tensilesource and mogisource over the whole depth (their images land
+    in the lower half of the grid), added to nonzero fields."""
+    sx1, sx2, sx3, dx = 64, 48, 48, 0.5
+    rng = np.random.default_rng(11)
+    p1 = sx1 + 2 if align else sx1
+    f = [rnd(rng, sx3, sx2, p1) * 1e-3 for _ in range(3)]
+    fo = [np.zeros((sx3, sx2, sx1 + 2), np.float32) for _ in range(3)]
+    for a, b in zip(fo, f):
+        a[..., :sx1] = b[..., :sx1]
+    f0 = [a.copy() for a in f]
+    elastic3d.tensilesource(2.0e4, MU, 1.2e-3, 0.4, -0.8, 3.0, 4.0, 6.0, 0.5, 0.6, 0.25, dx, dx, dx, *f,
+                            align_data=align)
+    O.tensilesource(2.0e4, MU, 1.2e-3, 0.4, -0.8, 3.0, 4.0, 6.0, 0.5, 0.6, 0.25, dx, dx, dx, *fo)
+    elastic3d.mogisource(2.0e4, MU, 2e-3, -1.0, 2.0, 2.5, dx, dx, dx, *f, align_data=align)
+    O.mogisource(2.0e4, MU, 2e-3, -1.0, 2.0, 2.5, dx, dx, dx, *fo)
+    for a, b, c in zip(f, fo, f0):
+        assert np.abs(b[sx3 // 2:, :, :sx1] - c[sx3 // 2:, :, :sx1]).max() > 0  # the images
+        assert rel_l2(a[..., :sx1], b[..., :sx1]) < 1e-6
+        if align:
+            np.testing.assert_array_equal(a[..., sx1:], c[..., sx1:])
+
+
+def test_tensile_and_mogi_moment_densities():
+    sig = np.zeros((24, 64, 64, 6), np.float32)
+    ref = sig.copy()
+    elastic3d.momentdensitytensile(2.0e4, MU, 2e-3, 1.0, -2.0, 4.0, 6.0, 8.0, 0.4, 0.3, 0.2, 0.5, 0.5, 0.5, sig)
+    O.momentdensitytensile(2.0e4, MU, 2e-3, 1.0, -2.0, 4.0, 6.0, 8.0, 0.4, 0.3, 0.2, 0.5, 0.5, 0.5, ref)
+    elastic3d.momentdensitymogi(2.0e4, MU, 3e-3, -2.0, 1.0, 5.0, 0.5, 0.5, 0.5, sig)
+    O.momentdensitymogi(2.0e4, MU, 3e-3, -2.0, 1.0, 5.0, 0.5, 0.5, 0.5, ref)
+    assert np.abs(ref).max() > 0 and rel_l2(sig, ref) < 1e-6
+
+
 @pytest.mark.parametrize("rate", [False, True])
 def test_traction(rate):
     """traction(): constant and periodic surface loads, the load and its rate."""
@@ -429,15 +463,20 @@ def test_rk2_adaptive_step_matches_oracle(kind):
     assert rel_l2(out["sig"], ost["sig"]) < 1e-4
 
 
-@pytest.mark.parametrize("kind", ["powerlaw", "loads"])
+@pytest.mark.parametrize("kind", ["powerlaw", "loads", "tensile"])
 def test_coseismic_setup_matches_oracle(kind):
     """The first event on the device (dislocations, traction, Green, stress)
     against the oracle's; "loads" adds a patch below the nyquist size (moment
-    density only) and two surface loads."""
+    density only) and two surface loads; "tensile" adds two tensile cracks
+    (one below the nyquist size) and two point dilatation sources, which
+    take the full-depth Green solve."""
     s = small_spec("powerlaw")
-    if kind == "loads":
+    if kind in ("loads", "tensile"):
         s["deck"] = s["deck"] + [(2e-3, 3.0, -4.0, 2.0, 0.8, 0.9, 40.0, 60.0, 20.0)]
         s["loads"] = [(2.0, 1.0, 10.0, 6.0, 0.3, 0.0, 0.0), (-8.0, -5.0, 6.0, 6.0, -0.1, 4.0, 0.5)]
+    if kind == "tensile":
+        s["tensile"] = [(1.5e-3, -6.0, 4.0, 1.0, 6.0, 4.0, 30.0, 70.0), (1e-3, 5.0, 5.0, 3.0, 0.9, 0.8, 0.0, 90.0)]
+        s["mogi"] = [(4e-3, 8.0, -6.0, 4.0), (-2e-3, -9.0, -9.0, 1.5)]
     st = oracle_initial_state(s)
     g = workloads.build(s)
     out = g.download()
@@ -530,7 +569,8 @@ def test_observation_points_match_oracle():
     g.close()
 
 
-def test_event_switch_matches_oracle():
+@pytest.mark.parametrize("kind", ["shear", "tensile"])
+def test_event_switch_matches_oracle(kind):
     """A second coseismic event at t = 1.3: integrationstep cuts Dt to reach
     it, the step's event index advances, relax_model_run stops there, and
     relax_model_event applies its patches and surface load (dislocations,
@@ -540,6 +580,8 @@ def test_event_switch_matches_oracle():
     rng = np.random.default_rng(77)
     s["events"] = [(1.3, [workloads._deck_patch(rng, -8, 8, 5.0, 4.0, 2e-3) for _ in range(2)],
                     [(3.0, -2.0, 8.0, 6.0, 0.2, 5.0, 0.3)])]
+    if kind == "tensile":  # a crack and a point source as well: the full-depth solve
+        s["events"][0] += ([(1e-3, 4.0, -3.0, 2.0, 5.0, 3.0, 10.0, 80.0)], [(3e-3, -5.0, 6.0, 3.0)])
     nsteps = 6
     st = oracle_initial_state(s)
     om = oracle_model(s)

@@ -45,6 +45,13 @@ Parity status per function:
   dislocations .............. pinned (the Nyquist gate: small patches add
                               only their moment density, large ones exactly
                               source()'s force; tau = momentdensityshear)
+  tensilesource, mogisource . pinned (first moment of the body force is the
+                              moment tensor s L W (lambda I + 2 mu n n), or
+                              kappa o I; the image half is the mirror of the
+                              source half)
+  momentdensitytensile/mogi . pinned (the integral of the moment density:
+                              closed forms of the Gaussian and taper
+                              integrals; kappa o for a point source)
   event switch .............. pinned (elastic superposition: an event of
                               slip s1 at t1 on top of a set-up of slip s0
                               equals a set-up of slip s0 + s1)
@@ -1226,3 +1233,115 @@ def test_dislocations_nyquist_gate(mut, mutation):
     for a, b in zip(v + t, f + tt):
         np.testing.assert_array_equal(a, b)
     assert max(np.abs(a).max() for a in v) > 0
+
+
+# ------------------------------------------------- tensile cracks, Mogi ---
+def _first_moment(F, sx, h3, dx):
+    X = (np.arange(sx) - sx * (np.arange(sx) >= sx // 2)) * dx
+    xx = [X[None, None, :], X[None, :, None], (np.arange(h3) * dx)[:, None, None]]
+    return np.array([[(xx[j] * F[i]).sum() * dx ** 3 for j in range(3)] for i in range(3)])
+
+
+@pytest.mark.parametrize("mut", [0, 24])
+def test_mogisource_moment_and_mirror_image(mut, mutation):
+    """A deep point dilatation source: the first moment of its body force in
+    the lower half-space is (lambda + 2 mu / 3) o I, and its image fills the
+    upper half of the grid (x3 < 0) as the mirror of the source."""
+    sx, sx3, dx, lam, o, zs = 48, 64, 0.5, 2.0e4, 2.5e-3, 7.0
+    f = [np.zeros((sx3, sx, sx + 2)) for _ in range(3)]
+    mutation(mut)
+    O.mogisource(lam, MU, o, 0.5, -1.0, zs, dx, dx, dx, *f)
+    mutation(0)
+    h3 = sx3 // 2
+    F = np.array([c[:h3, :, :sx] for c in f])
+    Mf = _first_moment(F, sx, h3, dx)
+    # the source sits at (0.5, -1.0): shift the moment to its centre
+    assert np.abs(F.sum(axis=(1, 2, 3))).max() * dx ** 3 < 1e-9
+    kappa = lam + 2.0 * MU / 3.0
+    np.testing.assert_allclose(np.diag(Mf), [kappa * o] * 3, rtol=1e-6)
+    assert np.abs(Mf - np.diag(np.diag(Mf))).max() < 1e-9 * kappa * o
+    mirror = [np.concatenate([c[:1], c[:0:-1]]) for c in f]  # x3 -> -x3
+    diff = max(np.abs(a[h3 + 1:] - b[h3 + 1:]).max() for a, b in zip(f, mirror))
+    if mut:
+        assert diff > 1e-3 * np.abs(F).max()
+    else:
+        assert diff == 0
+
+
+def test_momentdensitymogi_integral():
+    """The isotropic moment density of a point source integrates to
+    kappa o on the diagonal (normalised Gaussians of width dx)."""
+    sx, h3, dx, lam, o = 48, 32, 0.5, 2.0e4, 2.5e-3
+    sig = np.zeros((h3, sx, sx, 6))
+    O.momentdensitymogi(lam, MU, o, 0.5, -1.0, 7.0, dx, dx, dx, sig)
+    tot = sig.sum(axis=(0, 1, 2)) * dx ** 3
+    kappa = lam + 2.0 / 3.0 * MU
+    np.testing.assert_allclose(tot[[0, 3, 5]], kappa * o, rtol=1e-7)
+    assert np.abs(tot[[1, 2, 4]]).max() == 0
+
+
+@pytest.mark.parametrize("case", ["dipping", "vertical"])
+def test_tensilesource_first_moment(case, mutation):
+    """A deep opening crack: the first moment of its body force in the lower
+    half-space is the moment tensor s L W (lambda I + 2 mu n n) (to the
+    quadrature of the tapers)."""
+    sx, sx3, dx, lam, s = 64, 128, 0.5, 2.0e4, 1.5e-3
+    dip = np.radians(35) if case == "dipping" else 0.0
+    # a vertical crack's x3 taper is sampled along its width: a longer, softer
+    # taper keeps the quadrature error of its derivative small
+    L, W, strike, z, beta = (4.0, 6.0, np.radians(25), 14.0, 0.3) if case == "dipping" else \
+        (8.0, 6.0, np.radians(25), 16.0, 0.45)
+    f = [np.zeros((sx3, sx, sx + 2)) for _ in range(3)]
+    O.tensilesource(lam, MU, s, 0.3, -0.7, z, L, W, strike, dip, beta, dx, dx, dx, *f)
+    h3 = sx3 // 2
+    F = np.array([c[:h3, :, :sx] for c in f])
+    Mf = _first_moment(F, sx, h3, dx)
+    cs, ss, cd, sd = np.cos(strike), np.sin(strike), np.cos(dip), np.sin(dip)
+    n = np.array([cd * cs, -cd * ss, -sd])
+    M0 = s * L * W * (lam * np.eye(3) + 2 * MU * np.outer(n, n))
+    assert np.abs(Mf - M0).max() / np.abs(M0).max() < 2e-3
+    assert np.abs(F.sum(axis=(1, 2, 3)) * dx ** 3).max() < 1e-3 * np.abs(M0).max() / L
+
+
+@pytest.mark.parametrize("mut", [0, 22])
+def test_tensilesource_vertical_mirror_image(mut, mutation):
+    """A deep vertical crack's image in the upper half of the grid (x3 < 0)
+    is the mirror of its body force."""
+    sx, sx3, dx = 48, 64, 0.5
+    f = [np.zeros((sx3, sx, sx + 2)) for _ in range(3)]
+    mutation(mut)
+    O.tensilesource(2.0e4, MU, 1e-3, 0.0, 0.0, 8.0, 4.0, 5.0, np.radians(40), 0.0, 0.25, dx, dx, dx, *f)
+    mutation(0)
+    h3 = sx3 // 2
+    mirror = [np.concatenate([c[:1], c[:0:-1]]) for c in f]
+    diff = max(np.abs(a[h3 + 1:] - b[h3 + 1:]).max() for a, b in zip(f, mirror))
+    scale = max(np.abs(c[:h3]).max() for c in f)
+    if mut:
+        assert diff > 1e-3 * scale
+    else:
+        assert diff <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("mut", [0, 23])
+def test_momentdensitytensile_integral(mut, mutation):
+    """Hooke's law of s (n a)(n a) for a deep crack, a = the aperture:
+    integrates to s (2 mu n n + lambda I) times the integral of a^2, i.e.
+    1/(2 dx sqrt(pi)) for the Gaussian and W (1 - 1.25 beta)/(1 - beta),
+    L (1 - 1.25 beta)/(1 - beta) for the squared tapers."""
+    sx, h3, dx, lam, s, beta = 64, 64, 0.25, 2.0e4, 1.5e-3, 0.3
+    L, W, strike, dip, z = 3.0, 4.0, np.radians(25), np.radians(35), 8.0
+    sig = np.zeros((h3, sx, sx, 6))
+    mutation(mut)
+    O.momentdensitytensile(lam, MU, s, 0.3, -0.7, z, L, W, strike, dip, beta, dx, dx, dx, sig)
+    mutation(0)
+    tot = sig.sum(axis=(0, 1, 2)) * dx ** 3
+    cs, ss, cd, sd = np.cos(strike), np.sin(strike), np.cos(dip), np.sin(dip)
+    n = np.array([cd * cs, -cd * ss, -sd])
+    a2 = 1.0 / (2 * dx * np.sqrt(np.pi)) * W * (1 - 1.25 * beta) / (1 - beta) * L * (1 - 1.25 * beta) / (1 - beta)
+    M = s * a2 * (2 * MU * np.outer(n, n) + lam * np.eye(3))
+    ref = M[[0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2]]
+    err = np.abs(tot - ref).max() / np.abs(ref).max()
+    if mut:
+        assert err > 1e-2
+    else:
+        assert err < 2e-3
