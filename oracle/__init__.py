@@ -303,8 +303,9 @@ def rk2_run(model, state, nsteps, dt_replay=None):
     the state's dtype), and the later events: event_sources, a list (one
     entry per event after the first) of SOURCE arrays, event_loads,
     event_tensile and event_mogi likewise (surface loads, tensile cracks,
-    point dilatation sources; optional), and nyquist (dislocations' size
-    gate).
+    point dilatation sources; optional), loads0 (the first event's surface
+    loads, whose rate the steps add until the next event), and nyquist
+    (dislocations' size gate).
     state["e"] is the current event (1-based, default 1).  Returns an
     (nsteps, 10) array of
     Dt, tm, maxwell(1:3), power, C:E^i, t, maxwell(4:5).
@@ -328,9 +329,12 @@ def rk2_run(model, state, nsteps, dt_replay=None):
                  len(ev), ptr(ev), ptr(model.get("ltransient")), ptr(model.get("nltransient")))
     for key, fs, fo in (("event_sources", "ev_src", "ev_off"), ("event_loads", "ev_loads", "ev_loff"),
                         ("event_tensile", "ev_ts", "ev_tsoff"), ("event_mogi", "ev_mg", "ev_mgoff")):
-        evs = model.get(key)
-        if evs:
-            lists = [np.zeros(0, SOURCE)] + [np.asarray(a, SOURCE) for a in evs]
+        evs = model.get(key) or []
+        first = model.get("loads0") if key == "event_loads" else None
+        if evs or first is not None:
+            # entry 0: the first event (only its loads are used, for their rate)
+            lists = [np.asarray(np.zeros(0, SOURCE) if first is None else first, SOURCE)]
+            lists += [np.asarray(a, SOURCE) for a in evs]
             off = np.zeros(len(lists) + 1, np.int32)
             off[1:] = np.cumsum([len(a) for a in lists])
             setattr(m, fs, ptr(np.concatenate(lists).astype(SOURCE)))

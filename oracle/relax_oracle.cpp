@@ -1267,10 +1267,10 @@ static void dislocations(const orc_source* s, int ns, const orc_source* ts, int 
 // RK2 loop: relax.f90:555-890 with integrationstep :1420-1452 and the
 // isoutput counter :1364-1376 (oi advances whenever isoutput holds).
 // Mechanisms: linear and power-law viscosity (with weak zones), fault creep,
-// and linear and power-law transient creep (with weak zones); later events'
-// shear patches through the event switch (:858-889, the body force of their
-// moment density by the FIR divergence as in the set-up); no interseismic
-// loading and no time-dependent surface loads.
+// and linear and power-law transient creep (with weak zones); later events
+// through the event switch (:858-889, dislocations and traction); the rate of
+// the current event's time-dependent surface loads in both stages (:684-685,
+// :805-806); no interseismic loading.
 // ---------------------------------------------------------------------------
 extern "C" {
 struct orc_model {
@@ -1297,7 +1297,9 @@ struct orc_model {
   const void* dg0[4];
   // the later events (NULL: none): event k (1..ne-1) owns the shear patches
   // ev_src[ev_off[k] .. ev_off[k+1]) and the surface loads
-  // ev_loads[ev_loff[k] .. ev_loff[k+1]); dislocations' nyquist
+  // ev_loads[ev_loff[k] .. ev_loff[k+1]); dislocations' nyquist.  Entry 0
+  // of the loads is the first event's, whose rate the steps before the
+  // second event add.
   const orc_source* ev_src;
   const int* ev_off;
   const orc_source* ev_loads;
@@ -1386,6 +1388,11 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
     rc = equivalentbodyforce(sig, m->dx1, m->dx2, m->dx3, sx1, sx2, h3, v1.data(), v2.data(), v3.data(), t1.data(),
                              t2.data(), t3.data(), pitch, (const R*)0);
     if (rc) return rc;
+    // the rate of the current event's time-dependent surface loads
+    // (relax.f90:684-685; the reference's step there is Dt/2e8)
+    if (m->ev_loff && m->ev_loff[e] > m->ev_loff[e - 1])
+      traction(m->ev_loads + m->ev_loff[e - 1], m->ev_loff[e] - m->ev_loff[e - 1], sx1, sx2, m->dx1, m->dx2, t,
+               Dt / 2.e8, t3.data(), true);
     rc = greenfunctioncowling(v1.data(), v2.data(), v3.data(), t1.data(), t2.data(), t3.data(), sx1, sx2, sx3, m->dx1,
                               m->dx2, m->dx3, m->lambda, m->mu, m->gam);
     if (rc) return rc;
@@ -1444,6 +1451,10 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
     rc = equivalentbodyforce(moment.data(), m->dx1, m->dx2, m->dx3, sx1, sx2, h3, v1.data(), v2.data(), v3.data(),
                              t1.data(), t2.data(), t3.data(), pitch, (const R*)0);
     if (rc) return rc;
+    // the rate of the surface loads over the step (relax.f90:805-806)
+    if (m->ev_loff && m->ev_loff[e] > m->ev_loff[e - 1])
+      traction(m->ev_loads + m->ev_loff[e - 1], m->ev_loff[e] - m->ev_loff[e - 1], sx1, sx2, m->dx1, m->dx2, t, Dt,
+               t3.data(), true);
     rc = greenfunctioncowling(v1.data(), v2.data(), v3.data(), t1.data(), t2.data(), t3.data(), sx1, sx2, sx3, m->dx1,
                               m->dx2, m->dx3, m->lambda, m->mu, m->gam);
     if (rc) return rc;

@@ -49,8 +49,9 @@ struct relax_model {
   double* shear_slip = nullptr;
   long shear_cap = 0;
   std::vector<PlaneDev> shear_host;
-  relax_source_t* loads_dev = nullptr;  // an event's surface loads
+  relax_source_t* loads_dev = nullptr;  // an event's surface loads, then its point sources
   long loads_cap = 0;
+  int cur_nloads = 0;  // the current event's loads (loads_dev[0..)), whose rate each step adds
   std::vector<relax_plane_t> host_planes;
   std::vector<PlaneDev> plane_dev_host;
   std::vector<long> patch_off;

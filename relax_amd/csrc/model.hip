@@ -473,6 +473,7 @@ int relax_model_reset(relax_model_t* m) {
     m->hn[k].clear();
   }
   m->htaup.clear();
+  m->cur_nloads = 0;
   for (int k = 0; k < NMECH; ++k) m->mech[k] = 0;
   m->host_planes.clear();
   m->plane_dev_host.clear();
@@ -1012,6 +1013,23 @@ static int model_green(const Ctx& c, int epi) {
   return rc;
 }
 
+__global__ void k_traction_rate(const relax_source_t* __restrict__ l, int nl, int n1, int n2, double dx1, double dx2,
+                                const StepScalars* __restrict__ sc, int pred, float* t3);
+
+// The surface-load rates of the models whose current event has loads
+static int traction_rates(const Ctx& c, int pred) {
+  for (int i = 0; i < c.nm; ++i) {
+    relax_model_t* m = c.models[i];
+    if (m->cur_nloads == 0) continue;
+    const long P = (long)m->g.n1 * m->g.n2;
+    k_traction_rate<<<(unsigned)((P + 255) / 256), 256, 0, c.st>>>(m->loads_dev, m->cur_nloads, m->g.n1, m->g.n2,
+                                                                   m->desc.dx1, m->desc.dx2, m->sc, pred, m->T + 2 * P);
+    if (hipGetLastError() != hipSuccess) return RELAX_ERR_HIP;
+    cmark(c, RELAX_PROF_SMALL);
+  }
+  return RELAX_OK;
+}
+
 // One RK2 step of every model of the context, enqueued without host
 // synchronisation.
 static int enqueue_steps(const Ctx& c, const double* dt_override) {
@@ -1026,6 +1044,7 @@ static int enqueue_steps(const Ctx& c, const double* dt_override) {
   cmark(c, RELAX_PROF_BODYFORCE);
   k_finalize_dt<<<(nm + 63) / 64, 64, 0, st>>>(c.md, nm, dt_override ? 1 : 0, dt_override ? *dt_override : 0.0);
   cmark(c, RELAX_PROF_SMALL);
+  if ((rc = traction_rates(c, 1))) return rc;
   if ((rc = model_green(c, EPI_PRED))) return rc;
   if ((rc = launch_stress(STRESS_PRED, g, S, c.md, c.one, nm, c.rclass, st))) return rc;
   cmark(c, RELAX_PROF_STRESS_PRED);
@@ -1044,6 +1063,7 @@ static int enqueue_steps(const Ctx& c, const double* dt_override) {
       cmark(c, RELAX_PROF_SMALL);
     }
   }
+  if ((rc = traction_rates(c, 0))) return rc;
   if ((rc = model_green(c, EPI_CORR))) return rc;
   if ((rc = launch_stress(STRESS_CORR, g, S, c.md, c.one, nm, c.rclass, st))) return rc;
   cmark(c, RELAX_PROF_STRESS_CORR);
@@ -1455,13 +1475,8 @@ int launch_add_mogi(const relax_source_t* M, int nm, double lambda, double mu, i
 // traction, elastic3d.f90:2517-2590: one thread per surface point, the
 // loads subtracted in their order, each rounded to FP32 as the reference's
 // REAL() does
-__global__ void k_traction(const relax_source_t* __restrict__ l, int nl, int n1, int n2, double dx1, double dx2,
-                           double t, double Dt, int rate, float* t3, long ld) {
-  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (o >= (long)n1 * n2) return;
-  const int i1 = o % n1, i2 = o / n1;
-  const double x1 = d_shifted(i1, n1, dx1), x2 = d_shifted(i2, n2, dx2);
-  float v = t3[i1 + ld * i2];
+__device__ __forceinline__ float traction_point(const relax_source_t* __restrict__ l, int nl, double x1, double x2,
+                                                double t, double Dt, int rate, float v) {
   for (int k = 0; k < nl; ++k) {
     const relax_source_t& L = l[k];
     const double Lp = L.length * (1.0 + 2.0 * L.beta) / 2.0, Wp = L.width * (1.0 + 2.0 * L.beta) / 2.0;
@@ -1478,7 +1493,29 @@ __global__ void k_traction(const relax_source_t* __restrict__ l, int nl, int n1,
       v = (float)((double)v - amp * sin(2 * RELAX_PI * t / L.period + L.phase));
     }
   }
-  t3[i1 + ld * i2] = v;
+  return v;
+}
+
+__global__ void k_traction(const relax_source_t* __restrict__ l, int nl, int n1, int n2, double dx1, double dx2,
+                           double t, double Dt, int rate, float* t3, long ld) {
+  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (o >= (long)n1 * n2) return;
+  const int i1 = o % n1, i2 = o / n1;
+  t3[i1 + ld * i2] = traction_point(l, nl, d_shifted(i1, n1, dx1), d_shifted(i2, n2, dx2), t, Dt, rate,
+                                    t3[i1 + ld * i2]);
+}
+
+// The rate of a model's time-dependent surface loads in a stage of the RK2
+// step (relax.f90:684-685, :805-806): over [t, t + Dt] in the corrector,
+// over [t, t + Dt/2e8] in the predictor (the reference's step there), with
+// t and Dt of the step from the device
+__global__ void k_traction_rate(const relax_source_t* __restrict__ l, int nl, int n1, int n2, double dx1, double dx2,
+                                const StepScalars* __restrict__ sc, int pred, float* t3) {
+  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (o >= (long)n1 * n2) return;
+  const int i1 = o % n1, i2 = o / n1;
+  const double dt = pred ? sc->Dt / 2.e8 : sc->Dt;
+  t3[o] = traction_point(l, nl, d_shifted(i1, n1, dx1), d_shifted(i2, n2, dx2), sc->t, dt, 1, t3[o]);
 }
 
 int launch_traction(const relax_source_t* l, int nl, int n1, int n2, double dx1, double dx2, double t, double Dt,
@@ -1542,6 +1579,7 @@ int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist)
   for (int k = 0; k < nt; ++k) push(ev->ts[k], m->desc.beta, ev->ts[k].opening);
   if (!hp.empty() && (rc = stage_planes(m, amp.data()))) return rc;
   // the surface loads and the point sources, staged together
+  m->cur_nloads = 0;
   if (nl + nm > 0) {
     if (nl + nm > m->loads_cap) {
       if (m->loads_dev) (void)hipFree(m->loads_dev);
@@ -1551,6 +1589,8 @@ int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist)
       m->loads_cap = nl + nm;
     }
     if (nl) HIP_TRY(hipMemcpyAsync(m->loads_dev, ev->l, sizeof(relax_source_t) * nl, hipMemcpyHostToDevice, m->stream));
+    // from now on the steps add the rate of this event's loads
+    m->cur_nloads = nl;
     if (nm)
       HIP_TRY(hipMemcpyAsync(m->loads_dev + nl, ev->m, sizeof(relax_source_t) * nm, hipMemcpyHostToDevice, m->stream));
   }

@@ -64,6 +64,7 @@ def oracle_model(s):
              tscale=s["tscale"], skip=s["skip"], event_times=list(workloads.event_times(s)),
              ltransient=t["ltransient"], nltransient=t["nltransient"])
     m["nyquist"] = s.get("nyquist", workloads.NYQUIST)
+    m["loads0"] = workloads.event_sources(s, 1)["loads"].astype(O.SOURCE)
     if s.get("events"):
         srcs = [workloads.event_sources(s, e) for e in range(2, len(s["events"]) + 2)]
         for key, name in (("event_sources", "sources"), ("event_loads", "loads"), ("event_tensile", "tensile"),

@@ -487,6 +487,35 @@ def test_coseismic_setup_matches_oracle(kind):
     assert rel_l2(out["sig"], st["sig"]) < 1e-5
 
 
+def test_periodic_load_trajectory_matches_oracle():
+    """Time-dependent surface loads: each stage of the step adds the rate of
+    the current event's periodic loads to the surface traction (relax.f90:
+    684-685, 805-806); the trajectory with a Maxwell layer follows the
+    oracle's."""
+    s = small_spec("maxwell")
+    s["loads"] = [(2.0, -1.0, 12.0, 8.0, 0.5, 2.5, 0.3), (-6.0, 5.0, 6.0, 6.0, 0.2, 0.0, 0.0)]
+    nsteps = 5
+    st = oracle_initial_state(s)
+    om = oracle_model(s)
+    info = O.rk2_run(om, st, nsteps)
+    g = workloads.build(s)
+    recs = g.run(nsteps, dt_replay=info[:, 0])
+    out = g.download()
+    g.close()
+    np.testing.assert_allclose([r["t"] for r in recs], info[:, 7], rtol=1e-12)
+    for k in ("u1", "u2", "u3"):
+        assert rel_l2(out[k][..., :s["sx1"]], st[k][..., :s["sx1"]]) < 1e-4, k
+    for k in ("sig", "tau"):
+        assert rel_l2(out[k], st[k]) < 1e-4, k
+    # the load's rate moved the surface: the same model without it differs
+    s2 = dict(s, loads=[s["loads"][1]])
+    g = workloads.build(s2)
+    g.run(nsteps, dt_replay=info[:, 0])
+    u3 = g.download()["u3"]
+    g.close()
+    assert rel_l2(u3[..., :s["sx1"]], st["u3"][..., :s["sx1"]]) > 1e-3
+
+
 def test_output_time_truncation_c0():
     """c0 literally: the Maxwell layer lies below the grid, so tm = STEP_MAX
     and Dt lands on the output times: 8 steps of 1.25 yr to 10 yr."""

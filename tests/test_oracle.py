@@ -1345,3 +1345,33 @@ def test_momentdensitytensile_integral(mut, mutation):
         assert err > 1e-2
     else:
         assert err < 2e-3
+
+
+def test_rk2_periodic_load_rate_telescopes():
+    """Time-dependent surface loads in the RK2 loop (relax.f90:805-806): with
+    no relaxation, the corrector's u += Dt G[(load(t+Dt) - load(t))/Dt]
+    telescopes, so after any step sequence u is the elastic response to
+    load(t_n) - load(0)."""
+    sx, sx3, dx = 32, 32, 1.0
+    h3 = sx3 // 2
+    load = _load(period=3.0, phase=0.4, x=1.0, y=-2.0, width=6.0, length=8.0, slip=0.5)
+    m = dict(sx1=sx, sx2=sx, sx3=sx3, dx1=dx, dx2=dx, dx3=dx, lam=LAM, mu=MU, gam=8.33e-4, beta=0.2,
+             prestress=np.zeros(h3, O.TENSOR_LAYER), interval=100.0, odt=1.0, tscale=1.0, skip=1,
+             event_times=[0.0], loads0=load)
+    st = dict(u1=np.zeros((h3, sx, sx + 2)), u2=np.zeros((h3, sx, sx + 2)), u3=np.zeros((h3, sx, sx + 2)),
+              tau=np.zeros((h3, sx, sx, 6)), sig=np.zeros((h3, sx, sx, 6)), gamma=np.zeros((h3, sx, sx + 2)),
+              t=0.0, oi=1, i=0)
+    dts = [0.3, 0.7, 0.45, 1.1]
+    O.rk2_run(m, st, len(dts), dt_replay=dts)
+    tn = sum(dts)
+    c = [np.zeros((sx3, sx, sx + 2)) for _ in range(3)]
+    t = [np.zeros((sx, sx + 2)) for _ in range(3)]
+    a, b = np.zeros((sx, sx + 2)), np.zeros((sx, sx + 2))
+    O.traction(load, dx, dx, tn, a)
+    O.traction(load, dx, dx, 0.0, b)
+    t[2][:] = a - b
+    O.greenfunctioncowling(c[0], c[1], c[2], t[0], t[1], t[2], dx, dx, dx, LAM, MU, 8.33e-4)
+    assert np.abs(c[2][:h3, :, :sx]).max() > 0
+    for k in range(3):
+        np.testing.assert_allclose(st["u%d" % (k + 1)][..., :sx], c[k][:h3, :, :sx], rtol=0,
+                                   atol=1e-9 * np.abs(c[2][:h3]).max())
