@@ -291,7 +291,10 @@ int relax_model_synchronize(relax_model_t* m);
  * Green solve on temporary buffers; shear patches and loads alone take the
  * step's pruned solve.  Sources below the Nyquist size get no displacement
  * (the reference adds Okada's solution for shear patches at t = 0 only,
- * relax.f90:357-360).  Not on a member of a batch. */
+ * relax.f90:357-360).  The event's surface loads stay with the model: the
+ * following steps add the rate of the periodic ones to the surface traction
+ * in both stages (relax.f90:684-685, :805-806) until the next event.  Not
+ * on a member of a batch. */
 int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist);
 /* Utilities of the same set-up: add_shear adds momentdensityshear
  * (elastic3d.f90:2661-2750) of n rectangular patches (centre x,y,z, width,
