@@ -53,7 +53,9 @@ __host__ ColumnConsts make_consts(const GreenConst& gc) {
 __device__ __forceinline__ void elastic_point(const ColumnConsts& C, float k1, float k2, float k3, float2& c1,
                                               float2& c2, float2& c3) {
   const float r2 = k1 * k1 + k2 * k2 + k3 * k3;
-  const float denom = C.ratio1 / (r2 * r2);
+  // the hardware reciprocal (1 ulp) in place of a correctly rounded
+  // division: the column pass is bound by its FP32 issue rate
+  const float denom = C.ratio1 * __builtin_amdgcn_rcpf(r2 * r2);
   const float s = r2 * C.ratio2p1;
   const float2 kc = make_float2(k1 * c1.x + k2 * c2.x + k3 * c3.x, k1 * c1.y + k2 * c2.y + k3 * c3.y);
   c1 = make_float2((s * c1.x - k1 * kc.x) * denom, (s * c1.y - k1 * kc.y) * denom);
@@ -141,7 +143,7 @@ __device__ __forceinline__ void cerruti_point(const ColumnConsts& C, const Cerru
   (void)C; (void)p1; (void)p2;
   if (cc.zero) return;
   const float tk3 = (float)RELAX_PI2 * k3;
-  const float fi = 2.f * cc.beta / (tk3 * tk3 + cc.beta2);
+  const float fi = 2.f * cc.beta * __builtin_amdgcn_rcpf(tk3 * tk3 + cc.beta2);
   const float fk = fi * (tk3 * fi);  // fi kf
   u1 = make_float2(u1.x + (fi * cc.A[0].x + fk * cc.B[0].x), u1.y + (fi * cc.A[0].y + fk * cc.B[0].y));
   u2 = make_float2(u2.x + (fi * cc.A[1].x + fk * cc.B[1].x), u2.y + (fi * cc.A[1].y + fk * cc.B[1].y));
