@@ -333,7 +333,10 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
   // the per-column reductions reuse the transpose buffer
   float2(*red)[NW][TX] = reinterpret_cast<float2(*)[NW][TX]>(buf);
   float2(*pv)[TX] = reinterpret_cast<float2(*)[TX]>(buf + 3 * NW * TX);
-  static_assert(3 * NW * TX + 3 * TX <= 3 * Q * RS, "reductions fit the buffer");
+  // the Cerruti constants of the tile's columns follow pv
+  CerrutiCol* ccs = reinterpret_cast<CerrutiCol*>(buf + 3 * NW * TX + 3 * TX);
+  static_assert((3 * NW * TX + 3 * TX) * sizeof(float2) + TX * sizeof(CerrutiCol) <= 3 * Q * RS * sizeof(float2),
+                "reductions and column constants fit the buffer");
 
   const int tid = threadIdx.x;
   const int acol = tid % TX, aj = (tid / TX) % Q, ac = tid / (TX * Q);  // step A
@@ -424,16 +427,20 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
     pv[c][t] = make_float2(tt.x - acc.x / C.inv_n3dx3, tt.y - acc.y / C.inv_n3dx3);
   }
   __syncthreads();
+  // one thread per column forms its constants (FP64 divisions and square
+  // root), the others read them
+  if (tid < TX) ccs[tid] = cerruti_column(C, mu, alpha, gamma, k1w, k2w, pv[0][tid], pv[1][tid], pv[2][tid]);
+  __syncthreads();
   {
-    const float2 p1 = pv[0][bcol], p2 = pv[1][bcol], p3 = pv[2][bcol];
-    const CerrutiCol cc = cerruti_column(C, mu, alpha, gamma, k1w, k2w, p1, p2, p3);
+    const CerrutiCol cc = ccs[bcol];
+    const float2 p1 = pv[0][bcol], p2 = pv[1][bcol];
 #pragma unroll
     for (int k2 = 0; k2 < Q; ++k2) {
       const float k3 = wavenumber23f(2 * (bk1 + 16 * k2) + half, n3, C.ik3);
       cerruti_point(C, cc, p1, p2, k3, X[0][k2], X[1][k2], X[2][k2]);
     }
   }
-  __syncthreads();  // pv is read before the buffer is refilled
+  __syncthreads();  // pv and the constants are read before the buffer is refilled
 
   // inverse: B^-1 into the buffer, A^-1 out of it
   auto stepBinv = [&]() {
