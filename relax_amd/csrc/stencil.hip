@@ -337,7 +337,8 @@ __global__ __launch_bounds__(256) void k_stress(Grid g, ModelDev Dv, const Model
 
 int launch_bodyforce(const Grid& g, const StencilConst& S, const ModelDev* md, const ModelDev* one, int nm,
                      int rclass, int mode, hipStream_t st, int variant) {
-  const int zc = 16;
+  // depth chunk of one tile: each chunk re-reads the plane above and below it
+  const int zc = variant == 3 ? 32 : 16;
   const int gz = (g.h3 + zc - 1) / zc;
   const bool batch = !(one && nm == 1);
   const ModelDev Dv = batch ? ModelDev{} : *one;
@@ -356,10 +357,11 @@ int launch_bodyforce(const Grid& g, const StencilConst& S, const ModelDev* md, c
     else if (rclass == RHEO_CREEP) LF(TX, TY, RHEO_CREEP) \
     else LF(TX, TY, RHEO_VISCOUS)                   \
   }
-  // variant 0: 32x16 tiles, 1: 32x8 tiles, 2: 64x8 tiles
+  // variant 0: 32x16 tiles, 1: 32x8 tiles, 2: 64x8 tiles, 3: 64x8 tiles
+  // 32 planes deep (16 for the others)
   if (variant < 0 || variant >= BODY_VARIANTS) return RELAX_ERR_ARG;
   if (variant == 0 && g.n1 % 32 == 0 && g.n2 % 16 == 0) L(32, 16)
-  else if (variant == 2 && g.n1 % 64 == 0 && g.n2 % 8 == 0) L(64, 8)
+  else if (variant >= 2 && g.n1 % 64 == 0 && g.n2 % 8 == 0) L(64, 8)
   else if (variant != 2 && g.n1 % 32 == 0 && g.n2 % 8 == 0) L(32, 8)
   else return RELAX_ERR_SIZE;
 #undef LB

@@ -22,7 +22,7 @@ def test_autotune_keeps_state_and_results():
 
     m = workloads.build(s)
     choice = m.autotune()
-    assert 0 <= choice["fft_cols"] < 3 and 0 <= choice["green"] < 4 and 0 <= choice["bodyforce"] < 3
+    assert 0 <= choice["fft_cols"] < 3 and 0 <= choice["green"] < 4 and 0 <= choice["bodyforce"] < 4
     after = m.download()
     for k in ("u1", "u2", "u3", "sig", "tau", "gamma"):
         np.testing.assert_array_equal(after[k], before[k], err_msg=k)
@@ -58,10 +58,12 @@ def test_every_green_variant_matches_the_default(green, sx3):
         assert rel_l2(out[green][k], out[0][k]) < 1e-5, k
 
 
-@pytest.mark.parametrize("body", [1, 2])
-def test_every_bodyforce_tile_matches_the_default(body):
-    """The divergence tiles (32x16, 32x8, 64x8) give the same trajectory."""
-    s = small_spec("afterslip", sx=128, sx3=64)
+@pytest.mark.parametrize("body,sx3", [(1, 64), (2, 64), (3, 32), (3, 64), (3, 128)])
+def test_every_bodyforce_tile_matches_the_default(body, sx3):
+    """The divergence tiles (32x16, 32x8, 64x8 16 or 32 planes deep) give the
+    same trajectory, with the 32-plane chunk cut short (16 planes), whole, and
+    repeated."""
+    s = small_spec("afterslip", sx=128, sx3=sx3)
     out = {}
     for v in (0, body):
         m = workloads.build(s)
