@@ -44,8 +44,8 @@ def kernel_bytes(sx1, sx2, sx3):
 # WRITE_SIZE, each scaled by its reading for a device copy of known size
 # (profiles/rk2_step_c1_512x512x256_hbm_traffic.md)
 MEASURED_TRAFFIC_C1 = {
-    "green_z": 816.3e6, "fft_x_fwd": 811.7e6, "fft_y": 805.4e6, "fft_x_inv": 1220.8e6,
-    "bodyforce": 1595.4e6, "stress_pred": 3725.0e6, "stress_corr": 4802.7e6,
+    "green_z": 816.0e6, "fft_x_fwd": 811.7e6, "fft_y": 805.3e6, "fft_x_inv": 1220.7e6,
+    "bodyforce": 1779.6e6, "stress_pred": 3727.2e6, "stress_corr": 4803.0e6,
 }
 
 
