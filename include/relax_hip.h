@@ -333,7 +333,7 @@ int relax_model_surface(relax_model_t* m, float* u1, float* u2, float* u3, int a
 int relax_model_points(relax_model_t* m, const double* x, int n, double rot, double* u, relax_tensor_t* sig);
 /* Autotuner: times the launch-shape variants of the tunable kernels (columns
  * per workgroup of the y transform, columns/threads/radix of the Green
- * column pass, divergence tile) on this model's scratch buffers and keeps the
+ * column pass, divergence tile and depth chunk) on this model's scratch buffers and keeps the
  * fastest for this device and grid; later models of the grid in the process
  * use the choice.  choice (3 ints, may be NULL) receives the variants;
  * ms (may be NULL) the average time of one launch of each variant in
