@@ -6,6 +6,7 @@ over the 512x512x256 workload (40 slip patches, power law n = 3 below a 25 km
 lid; relax_amd.workloads "c1"), with the whole state resident in HBM.
 
   python bench.py [--steps K] [--warmup W] [--workload c1] [--no-cpu-baseline]
+                  [--no-batched] [--no-serving]
 
 Prints one JSON line.  The timed region is K back-to-back steps on the
 model's stream with HIP-event phase timers (one synchronisation at the end),
@@ -40,13 +41,30 @@ def kernel_bytes(sx1, sx2, sx3):
     }
 
 
-# HBM bytes per launch at 512x512x256 measured with rocprofv3 FETCH_SIZE and
-# WRITE_SIZE, each scaled by its reading for a device copy of known size
-# (profiles/rk2_step_c1_512x512x256_hbm_traffic.md)
-MEASURED_TRAFFIC_C1 = {
-    "green_z": 816.0e6, "fft_x_fwd": 811.7e6, "fft_y": 805.3e6, "fft_x_inv": 1220.7e6,
-    "bodyforce": 1779.6e6, "stress_pred": 3727.2e6, "stress_corr": 4803.0e6,
-}
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "rk2_step_c1_512x512x256_hbm_traffic.md")
+
+
+def measured_traffic(tuning):
+    """HBM bytes per launch of each kernel kind at 512x512x256, read from the
+    committed rocprofv3 summary (FETCH_SIZE and WRITE_SIZE, each scaled by its
+    reading for a device copy of known size).  The summary names the launch
+    shapes it was measured with; when the autotuner picks others now, the
+    figures do not describe the timed kernels and None is returned."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    rec = json.loads(line.split(":", 1)[1])
+                    break
+            else:
+                return None, "no traffic line in the profile"
+    except OSError as e:
+        return None, "profile not readable: %s" % e
+    want = {k: tuning[k] for k in ("fft_cols", "green", "bodyforce")}
+    if rec.get("tuning") != want:
+        return None, "profile measured with tuning %s, this run uses %s" % (rec.get("tuning"), want)
+    return rec["bytes"], None
 
 
 def step_bytes_model(sx1, sx2, sx3):
@@ -56,27 +74,26 @@ def step_bytes_model(sx1, sx2, sx3):
     return 404 * N + 192 * N / sx1 + 96 * sx1 * sx2
 
 
-def cpu_baseline(spec, state, nthreads):
-    """The oracle (oracle/, FP32 storage) on one RK2 step of the GPU's state."""
-    import numpy as np
-
+def cpu_baseline(spec, state, nthreads, nsteps=2):
+    """The oracle (oracle/, FP32 storage, OpenMP over nthreads host threads)
+    on nsteps RK2 steps from the GPU's state; returns the seconds taken."""
     ROOT = os.path.dirname(os.path.abspath(__file__))
     sys.path.insert(0, ROOT)
-    os.environ.setdefault("OMP_NUM_THREADS", str(nthreads))
+    os.environ["OMP_NUM_THREADS"] = str(nthreads)
     import oracle as O
     from relax_amd import workloads
 
     t = workloads.tables(spec)
-    m = dict(sx1=spec["sx1"], sx2=spec["sx2"], sx3=spec["sx3"], dx1=spec["dx"], dx2=spec["dx"], dx3=spec["dx"],
+    dx1, dx2, dx3 = workloads.spacing(spec)
+    m = dict(sx1=spec["sx1"], sx2=spec["sx2"], sx3=spec["sx3"], dx1=dx1, dx2=dx2, dx3=dx3,
              lam=workloads.LAM, mu=workloads.MU, gam=workloads.GAM, beta=workloads.BETA, linear=t["linear"],
              nonlinear=t["nonlinear"], faultcreep=t["friction"], prestress=t["prestress"], interval=1e30,
              odt=1.0, tscale=1.0, skip=1, event_times=[0.0])
     st = {k: state[k] for k in ("u1", "u2", "u3", "tau", "sig", "gamma")}
     st.update(t=0.0, oi=1, i=0)
     t0 = time.perf_counter()
-    O.rk2_run(m, st, 1)
-    dt = time.perf_counter() - t0
-    return dt
+    O.rk2_run(m, st, nsteps)
+    return time.perf_counter() - t0
 
 
 def batched_c3(nmodels=64, nsteps=10):
@@ -174,7 +191,7 @@ def main():
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 64 x 128^3 measurement")
-    ap.add_argument("--serving", action="store_true", help="also measure the serving stream")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the RK2 step of one model per GPU "
@@ -216,6 +233,7 @@ def main():
     dom = max((k for k in kernels if k in kb), key=lambda k: prof[k][0])
     achieved = kernels[dom]["achieved_GBs"]
     B_step = step_bytes_model(sx1, sx2, sx3)
+    traffic, why = measured_traffic(tuned) if args.workload == "c1" else (None, "profiled at c1 only")
     fused_bytes = sum(kb[k] * kernels[k]["launches_per_step"] for k in kb if k in kernels)
     out = {
         "metric": METRIC,
@@ -231,16 +249,22 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded slip patches, relax_amd.workloads.%s)" % args.workload,
         "config": {"workload": "%s: %dx%dx%d FP32, dx=%g km, 40 rectangular 5x5 km slip patches, power law n=3 "
-                               "below a 25 km lid, adaptive RK2" % (args.workload, sx1, sx2, sx3, spec["dx"]),
+                               "below a 25 km lid, adaptive RK2" % (args.workload, sx1, sx2, sx3,
+                                                                    workloads.spacing(spec)[0]),
                    "grid": [sx1, sx2, sx3], "dt_last": info["Dt"], "t_last": info["t"], "autotune": tuned},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC_C1.get(dom) if args.workload == "c1" else None},
+                     "traffic": traffic.get(dom) if traffic else None},
+        # frac: the unfused model bytes of the path (404 N + ...) over the step
+        # time, bytes this implementation never moves; fused_frac: the bytes
+        # the fused kernels must move
         "step_roofline": {"bytes_per_step_model": B_step, "frac": round(B_step * steps_per_s / 8e12, 4),
                           "fused_bytes_per_step": fused_bytes,
                           "fused_frac": round(fused_bytes * steps_per_s / 8e12, 4)},
         "kernels": kernels,
     }
+    if why:
+        out["roofline"]["traffic_note"] = why
     state = model.download() if not args.no_cpu_baseline else None
     model.close()
     if not args.no_batched:
@@ -249,20 +273,26 @@ def main():
             out["batched"] = batched_c3()
         except Exception as e:  # keep the headline line
             out["batched"] = {"error": repr(e)}
-    if args.serving:
+    if not args.no_serving:
         print("# serving stream", file=sys.stderr, flush=True)
         try:
             out["serving"] = serving_c4()
         except Exception as e:
             out["serving"] = {"error": repr(e)}
     if state is not None:
-        nthreads = min(16, os.cpu_count() or 1)
-        print("# cpu baseline: oracle, one RK2 step of the same state, %d threads" % nthreads, file=sys.stderr,
-              flush=True)
-        dt = cpu_baseline(spec, state, nthreads)
-        out["cpu_baseline"] = {"value": round(1.0 / dt, 5), "unit": "RK2 steps/s", "cores": nthreads, "kind": "port",
-                               "sample": "one RK2 step of the %s state (%dx%dx%d), %.1f s" % (
-                                   args.workload, sx1, sx2, sx3, dt)}
+        # the host threads this process may use: the box's CPU share for
+        # its GPU (OMP_NUM_THREADS), not every core the host reports
+        nproc = os.cpu_count() or 1
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", nproc))
+        nsteps_cpu = 2
+        print("# cpu baseline: oracle, %d RK2 steps of the same state, %d threads" % (nsteps_cpu, nthreads),
+              file=sys.stderr, flush=True)
+        dt = cpu_baseline(spec, state, nthreads, nsteps_cpu)
+        out["cpu_baseline"] = {"value": round(nsteps_cpu / dt, 5), "unit": "RK2 steps/s", "cores": nthreads,
+                               "kind": "port",
+                               "sample": "%d RK2 steps of the %s state (%dx%dx%d) in %.1f s on %d OpenMP threads "
+                                         "(host reports %d cores; OMP_NUM_THREADS is this GPU's CPU share)" % (
+                                             nsteps_cpu, args.workload, sx1, sx2, sx3, dt, nthreads, nproc)}
     print(json.dumps(out), flush=True)
 
 

@@ -61,24 +61,69 @@ template <class R> struct T6 { R s11, s12, s13, s22, s23, s33; };
 // ---------------------------------------------------------------------------
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-static void fft1d(cd* a, int n, long stride, int sign) {
-  // iterative radix-2 decimation in time, bit-reversed input reorder
+// Twiddles w[k] = exp(-2 pi i k / n), k < n/2, evaluated in long double.
+static std::vector<cd> twiddle_table(int n) {
+  std::vector<cd> w(std::max(1, n / 2));
+  for (int k = 0; k < n / 2; ++k) {
+    const long double a = -2.0L * 3.141592653589793238462643383279502884L * k / n;
+    w[k] = cd((double)cosl(a), (double)sinl(a));
+  }
+  return w;
+}
+
+// In-place iterative radix-2 DFT of a contiguous sequence: bit-reversed
+// reorder, then butterflies with the table twiddles (conjugated for sign +1).
+static void fft1d(cd* a, int n, const cd* w, int sign) {
   for (int i = 1, j = 0; i < n; ++i) {
     int bit = n >> 1;
     for (; j & bit; bit >>= 1) j ^= bit;
     j ^= bit;
-    if (i < j) std::swap(a[i * stride], a[j * stride]);
+    if (i < j) std::swap(a[i], a[j]);
   }
+  // butterflies in real arithmetic (std::complex's operator* takes the
+  // library's NaN-recovery path)
+  double* d = reinterpret_cast<double*>(a);
+  const double* wd = reinterpret_cast<const double*>(w);
+  const double sg = sign < 0 ? 1.0 : -1.0;
   for (int len = 2; len <= n; len <<= 1) {
-    for (int k = 0; k < len / 2; ++k) {
-      double ang = sign * PI2 * k / len;
-      cd w(std::cos(ang), std::sin(ang));
-      for (int i = 0; i < n; i += len) {
-        cd u = a[(i + k) * stride];
-        cd v = a[(i + k + len / 2) * stride] * w;
-        a[(i + k) * stride] = u + v;
-        a[(i + k + len / 2) * stride] = u - v;
+    const int half = len / 2, step = n / len;
+    for (int i = 0; i < n; i += len) {
+      for (int k = 0; k < half; ++k) {
+        const double wr = wd[2 * (k * step)], wi = sg * wd[2 * (k * step) + 1];
+        double* p = d + 2 * (i + k);
+        double* q = d + 2 * (i + k + half);
+        const double vr = q[0] * wr - q[1] * wi, vi = q[0] * wi + q[1] * wr;
+        const double ur = p[0], ui = p[1];
+        p[0] = ur + vr; p[1] = ui + vi;
+        q[0] = ur - vr; q[1] = ui - vi;
       }
+    }
+  }
+}
+
+// DFT along one strided axis of a complex array: ncol sequences of length n,
+// element j of sequence c at base(c) + j*stride, where the sequences of
+// neighbouring c are contiguous in memory (columns c0..c0+B-1 share rows).
+// Blocks of B neighbouring sequences are gathered into a scratch buffer, so
+// each row read is B consecutive values.
+static void fft_axis(cd* data, long ncol, long ncol_inner, long plane_stride, int n, long stride, int sign) {
+  const std::vector<cd> w = twiddle_table(n);
+  const int B = 16;
+  const long nblk_inner = (ncol_inner + B - 1) / B;
+  const long nouter = ncol / ncol_inner;
+#pragma omp parallel
+  {
+    std::vector<cd> buf((size_t)B * n);
+#pragma omp for schedule(static)
+    for (long blk = 0; blk < nouter * nblk_inner; ++blk) {
+      const long o = blk / nblk_inner, c0 = (blk % nblk_inner) * B;
+      const int nb = (int)std::min<long>(B, ncol_inner - c0);
+      cd* base = data + o * plane_stride + c0;
+      for (int j = 0; j < n; ++j)
+        for (int b = 0; b < nb; ++b) buf[(size_t)b * n + j] = base[j * stride + b];
+      for (int b = 0; b < nb; ++b) fft1d(&buf[(size_t)b * n], n, w.data(), sign);
+      for (int j = 0; j < n; ++j)
+        for (int b = 0; b < nb; ++b) base[j * stride + b] = buf[(size_t)b * n + j];
     }
   }
 }
@@ -90,28 +135,21 @@ static int fftn_forward(R* data, int sx1, int sx2, int sx3, double scale) {
   const int h1 = sx1 / 2 + 1;
   const long pitch = sx1 + 2;
   std::vector<cd> spec((size_t)h1 * sx2 * sx3);
+  const std::vector<cd> w1 = twiddle_table(sx1);
 #pragma omp parallel
   {
     std::vector<cd> row(sx1);
 #pragma omp for
     for (long r = 0; r < (long)sx2 * sx3; ++r) {
       for (int i = 0; i < sx1; ++i) row[i] = cd((double)data[r * pitch + i], 0.0);
-      fft1d(row.data(), sx1, 1, -1);
+      fft1d(row.data(), sx1, w1.data(), -1);
       for (int k = 0; k < h1; ++k) spec[r * h1 + k] = row[k];
     }
   }
-  if (sx2 > 1) {
-#pragma omp parallel for
-    for (long c = 0; c < (long)h1 * sx3; ++c) {
-      long i3 = c / h1, k1 = c % h1;
-      fft1d(&spec[i3 * h1 * sx2 + k1], sx2, h1, -1);
-    }
-  }
-  if (sx3 > 1) {
-#pragma omp parallel for
-    for (long c = 0; c < (long)h1 * sx2; ++c)
-      fft1d(&spec[c], sx3, (long)h1 * sx2, -1);
-  }
+  // x2: in each x3 plane, h1 columns of sx2 at stride h1
+  if (sx2 > 1) fft_axis(spec.data(), (long)h1 * sx3, h1, (long)h1 * sx2, sx2, h1, -1);
+  // x3: h1*sx2 columns of sx3 at stride h1*sx2
+  if (sx3 > 1) fft_axis(spec.data(), (long)h1 * sx2, (long)h1 * sx2, 0, sx3, (long)h1 * sx2, -1);
 #pragma omp parallel for
   for (long r = 0; r < (long)sx2 * sx3; ++r)
     for (int k = 0; k < h1; ++k) {
@@ -128,22 +166,13 @@ static int fftn_inverse(R* data, int sx1, int sx2, int sx3, double scale) {
   const int h1 = sx1 / 2 + 1;
   const long pitch = sx1 + 2;
   std::vector<cd> spec((size_t)h1 * sx2 * sx3);
+  const std::vector<cd> w1 = twiddle_table(sx1);
 #pragma omp parallel for
   for (long r = 0; r < (long)sx2 * sx3; ++r)
     for (int k = 0; k < h1; ++k)
       spec[r * h1 + k] = cd((double)data[r * pitch + 2 * k], (double)data[r * pitch + 2 * k + 1]);
-  if (sx3 > 1) {
-#pragma omp parallel for
-    for (long c = 0; c < (long)h1 * sx2; ++c)
-      fft1d(&spec[c], sx3, (long)h1 * sx2, +1);
-  }
-  if (sx2 > 1) {
-#pragma omp parallel for
-    for (long c = 0; c < (long)h1 * sx3; ++c) {
-      long i3 = c / h1, k1 = c % h1;
-      fft1d(&spec[i3 * h1 * sx2 + k1], sx2, h1, +1);
-    }
-  }
+  if (sx3 > 1) fft_axis(spec.data(), (long)h1 * sx2, (long)h1 * sx2, 0, sx3, (long)h1 * sx2, +1);
+  if (sx2 > 1) fft_axis(spec.data(), (long)h1 * sx3, h1, (long)h1 * sx2, sx2, h1, +1);
 #pragma omp parallel
   {
     std::vector<cd> row(sx1);
@@ -155,7 +184,7 @@ static int fftn_inverse(R* data, int sx1, int sx2, int sx3, double scale) {
       row[0] = cd(row[0].real(), 0.0);
       row[sx1 / 2] = cd(row[sx1 / 2].real(), 0.0);
       for (int k = 1; k < sx1 / 2; ++k) row[sx1 - k] = std::conj(row[k]);
-      fft1d(row.data(), sx1, 1, +1);
+      fft1d(row.data(), sx1, w1.data(), +1);
       for (int i = 0; i < sx1; ++i) data[r * pitch + i] = (R)(row[i].real() * scale);
       data[r * pitch + sx1] = 0;
       data[r * pitch + sx1 + 1] = 0;
@@ -425,7 +454,7 @@ static int stressupdate(const R* v1, const R* v2, const R* v3, double lambda, do
           e.s33 = rnd<R>((double)(V(v3, i1, i2, i3p) - V(v3, i1, i2, i3m)) / px3);
         } else {
           for (int l = 1; l <= len3; ++l) {
-            int m = i3 - l, p = i3 + l;
+            int m = MUT(25) ? i3 : i3 - l, p = i3 + l;
             e.s13 = rnd<R>(e.s13 + (double)(V(v1, i1, i2, p) - V(v1, i1, i2, m)) * ker3[l - 1]);
             e.s23 = rnd<R>(e.s23 + (double)(V(v2, i1, i2, p) - V(v2, i1, i2, m)) * ker3[l - 1]);
             e.s33 = rnd<R>(e.s33 + (double)(V(v3, i1, i2, p) - V(v3, i1, i2, m)) * ker3[l - 1]);
@@ -497,7 +526,7 @@ static int equivalentbodyforce(const R* sig, double dx1, double dx2, double dx3,
           f3 += (double)(at(i1, i2, i3p).s33 - at(i1, i2, i3m).s33) / px3;
         } else {
           for (int l = 1; l <= len3; ++l) {
-            int m = i3 - l, p = i3 + l;
+            int m = i3 - l, p = MUT(26) ? i3 : i3 + l;
             f1 += (double)(at(i1, i2, p).s13 - at(i1, i2, m).s13) * ker3[l - 1];
             f2 += (double)(at(i1, i2, p).s23 - at(i1, i2, m).s23) * ker3[l - 1];
             f3 += (double)(at(i1, i2, p).s33 - at(i1, i2, m).s33) * ker3[l - 1];
@@ -772,7 +801,7 @@ static int frictioneigenstress(double x, double y, double z, double L, double W,
         if ((tau / tauc) > 20) { status = 5; continue; }
         ts[0] /= taus; ts[1] /= taus; ts[2] /= taus;
         double gammadot = vo * 2.0 * my_sinh(tau / tauc);
-        float cand = (float)(tau / mu / gammadot * (std::min(L, W) / std::sqrt(dx1 * dx3)));
+        float cand = (float)(tau / mu / gammadot * (std::min(L, W) / std::sqrt(dx1 * (MUT(27) ? dx2 : dx3))));
         if (cand < tm) tm = cand;  // MIN ignores a NaN candidate
         if (vel) {
           long vo2 = i1 + vpitch * (i2 + (long)sx2 * i3);
@@ -860,12 +889,12 @@ static int monitorfriction(double x, double y, double z, double L, double W, dou
       double yr = sstrike * x + cstrike * y;
       double zr = sdip * x2 + cdip * z;
       // yr=REAL(yr)+REAL(...): a single-precision sum stored in REAL*8 (friction3d.f90:335-336)
-      yr = (double)((float)yr + (float)(((double)j2 - (double)px2 / 2.0 - 1.0) * dx2));
-      zr = (double)((float)zr + (float)(((double)j3 - (double)px3 / 2.0 - 1.0) * dx3));
+      yr = (double)((float)yr + (float)(((double)j2 - (double)px2 / 2.0 - 1.0) * (MUT(28) ? dx3 : dx2)));
+      zr = (double)((float)zr + (float)(((double)j3 - (double)px3 / 2.0 - 1.0) * (MUT(28) ? dx2 : dx3)));
       double x2p = cdip * xr + sdip * zr;
       double X1 = cstrike * x2p + sstrike * yr;
       double X2 = -sstrike * x2p + cstrike * yr;
-      double X3 = -sdip * xr + cdip * zr;
+      double X3 = (MUT(30) ? sdip : -sdip) * xr + cdip * zr;
       P.x1 = X1; P.x2 = X2; P.x3 = X3; P.lx = yr; P.lz = zr;
       if ((X1 > (double)(sx1 / 2 - 1) * dx1) || (X1 < -(double)(sx1 / 2) * dx1) ||
           (X2 > (double)(sx2 / 2 - 1) * dx2) || (X2 < -(double)(sx2 / 2) * dx2) ||
@@ -885,7 +914,7 @@ static int monitorfriction(double x, double y, double z, double L, double W, dou
       double taun = t[0] * n[0] + t[1] * n[1] + t[2] * n[2];
       double ts[3] = {t[0] - taun * n[0], t[1] - taun * n[1], t[2] - taun * n[2]};
       double taus = std::sqrt(ts[0] * ts[0] + ts[1] * ts[1] + ts[2] * ts[2]);
-      double tau = std::max(0.0, taus + friction * taun - cohesion);
+      double tau = MUT(29) ? taus : std::max(0.0, taus + friction * taun - cohesion);
       if ((ts[0] * r[0] + ts[1] * r[1] + ts[2] * r[2]) < 0.0 && std::fabs(rake) < PI2 * 1.5) continue;
       P.taus = taus;
       P.v = vo * 2.0 * my_sinh(tau / tauc);

@@ -101,13 +101,23 @@ def spec(name):
     raise KeyError(name)
 
 
+def spacing(s):
+    """(dx1, dx2, dx3) of a spec, whose "dx" is one spacing or three."""
+    dx = s["dx"]
+    if np.ndim(dx) == 0:
+        return float(dx), float(dx), float(dx)
+    dx1, dx2, dx3 = (float(v) for v in dx)
+    return dx1, dx2, dx3
+
+
 def tables(s):
     """Per-depth tables (sx3/2 entries) of the mechanisms of a spec."""
     h3 = s["sx3"] // 2
+    dx3 = spacing(s)[2]
     out = {}
     for key in ("linear", "nonlinear", "friction", "ltransient", "nltransient"):
         lay = s.get(key)
-        out[key] = None if lay is None else structure.viscoelasticstructure(lay, h3, s["dx"])
+        out[key] = None if lay is None else structure.viscoelasticstructure(lay, h3, dx3)
     out["prestress"] = np.zeros(h3, TENSOR_LAYER)
     return out
 
@@ -126,8 +136,9 @@ def planes(s):
     """Fault-creep planes of a spec (PLANE array) or None."""
     if not s["planes"]:
         return None
+    _, dx2, dx3 = spacing(s)
     ps = [structure.friction_plane(p["x"], p["y"], p["z"], p["length"], p["width"], p["strike"], p["dip"],
-                                   p["rake"], s["dx"], s["dx"]) for p in s["planes"]]
+                                   p["rake"], dx2, dx3) for p in s["planes"]]
     return np.concatenate(ps)
 
 
@@ -140,8 +151,8 @@ def event_times(s):
 
 def build(s):
     """Create the model of a spec on the GPU with its coseismic state."""
-    dx = s["dx"]
-    m = Model(s["sx1"], s["sx2"], s["sx3"], dx, dx, dx, LAM, MU, GAM, BETA, interval=s["interval"], odt=s["odt"],
+    dx1, dx2, dx3 = spacing(s)
+    m = Model(s["sx1"], s["sx2"], s["sx3"], dx1, dx2, dx3, LAM, MU, GAM, BETA, interval=s["interval"], odt=s["odt"],
               tscale=s["tscale"], skip=s["skip"], event_times=event_times(s))
     return configure(m, s)
 

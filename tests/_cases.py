@@ -58,7 +58,8 @@ def small_spec(kind, sx=64, sx3=64, dx=1.0, seed=0):
 def oracle_model(s):
     t = workloads.tables(s)
     pl = workloads.planes(s)
-    m = dict(sx1=s["sx1"], sx2=s["sx2"], sx3=s["sx3"], dx1=s["dx"], dx2=s["dx"], dx3=s["dx"], lam=workloads.LAM,
+    dx1, dx2, dx3 = workloads.spacing(s)
+    m = dict(sx1=s["sx1"], sx2=s["sx2"], sx3=s["sx3"], dx1=dx1, dx2=dx2, dx3=dx3, lam=workloads.LAM,
              mu=workloads.MU, gam=workloads.GAM, beta=workloads.BETA, linear=t["linear"], nonlinear=t["nonlinear"],
              faultcreep=t["friction"], prestress=t["prestress"], interval=s["interval"], odt=s["odt"],
              tscale=s["tscale"], skip=s["skip"], event_times=list(workloads.event_times(s)),
@@ -75,7 +76,7 @@ def oracle_model(s):
             NLTRANSIENT: "dg_nltransient"}
     for mech, zones in workloads.weak_zones(s).items():
         # the model builds the field with taper 0, as relax.f90:487-541 does
-        m[keys[mech]] = O.builddgammadot0(s["sx1"], s["sx2"], h3, s["dx"], s["dx"], s["dx"], 0.0,
+        m[keys[mech]] = O.builddgammadot0(s["sx1"], s["sx2"], h3, dx1, dx2, dx3, 0.0,
                                           zones.astype(O.WEAK), np.float32)
     if pl is not None:
         m["planes"] = np.ascontiguousarray(pl.astype(O.PLANE))
@@ -88,20 +89,21 @@ def oracle_initial_state(s, dtype=np.float32):
     of the deck's patches and loads, u from the Green's function of their
     body force, sig = -tau + C:eps(u) -- the set-up the model's event performs
     on the device."""
-    sx1, sx2, sx3, dx = s["sx1"], s["sx2"], s["sx3"], s["dx"]
+    sx1, sx2, sx3 = s["sx1"], s["sx2"], s["sx3"]
+    dx1, dx2, dx3 = workloads.spacing(s)
     h3 = sx3 // 2
     tau = np.zeros((h3, sx2, sx1, 6), dtype)
     c = [np.zeros((sx3, sx2, sx1 + 2), dtype) for _ in range(3)]
     t = [np.zeros((sx2, sx1 + 2), dtype) for _ in range(3)]
     ev = {k: v.astype(O.SOURCE) for k, v in workloads.event_sources(s, 1).items()}
-    O.dislocations(ev["sources"], ev["loads"], workloads.MU, s.get("nyquist", workloads.NYQUIST), 0.0, dx, dx, dx, c,
+    O.dislocations(ev["sources"], ev["loads"], workloads.MU, s.get("nyquist", workloads.NYQUIST), 0.0, dx1, dx2, dx3, c,
                    t, tau, lam=workloads.LAM, beta=workloads.BETA, tensile=ev["tensile"], mogi=ev["mogi"])
-    O.greenfunctioncowling(c[0], c[1], c[2], t[0], t[1], t[2], dx, dx, dx, workloads.LAM, workloads.MU,
+    O.greenfunctioncowling(c[0], c[1], c[2], t[0], t[1], t[2], dx1, dx2, dx3, workloads.LAM, workloads.MU,
                            workloads.GAM)
     u = [np.ascontiguousarray(a[:h3]) for a in c]
     sig = -tau
     sig = np.ascontiguousarray(sig)
-    O.stressupdate(u[0], u[1], u[2], workloads.LAM, workloads.MU, dx, dx, dx, sig)
+    O.stressupdate(u[0], u[1], u[2], workloads.LAM, workloads.MU, dx1, dx2, dx3, sig)
     st = dict(u1=u[0], u2=u[1], u3=u[2], tau=tau, sig=sig, gamma=np.zeros_like(u[0]), t=0.0, oi=1, i=0)
     if s.get("ltransient") or s.get("nltransient"):
         st["epsilonik"] = np.zeros((h3, sx2, sx1, 6), dtype)

@@ -450,6 +450,29 @@ def test_rk2_trajectory_matches_oracle(kind, nsteps):
         assert rel_l2(patches["v"], ref["v"]) < 1e-4
 
 
+@pytest.mark.parametrize("kind,dx", [("powerlaw", (0.8, 1.0, 0.6)), ("afterslip", (1.0, 0.75, 0.5))])
+def test_rk2_trajectory_with_unequal_spacing(kind, dx):
+    """dx1, dx2 and dx3 all different: the strain and divergence stencils,
+    the wavenumbers, the per-depth tables (sampled every dx3), the friction
+    plane's patch grid (px2 = fix(L/dx2), px3 = fix(W/dx3)) and the fault
+    creep relaxation time (sqrt(dx1 dx3), friction3d.f90:223) each see their
+    own axis; the adaptive trajectory follows the oracle's."""
+    s = small_spec(kind, dx=dx)
+    st, ost, info, out, recs, om, patches = run_pair(s, 4, replay=False)
+    assert len(recs) == 4
+    for k, r in enumerate(recs):
+        assert abs(r["Dt"] / info[k, 0] - 1) < 1e-5, k
+    for k in ("u1", "u2", "u3"):
+        assert rel_l2(out[k][..., :s["sx1"]], ost[k][..., :s["sx1"]]) < 1e-4, k
+    for k in ("sig", "tau", "gamma"):
+        assert rel_l2(out[k][..., :s["sx1"]] if k == "gamma" else out[k],
+                      ost[k][..., :s["sx1"]] if k == "gamma" else ost[k]) < 1e-4, k
+    if patches is not None:
+        ref = om["patches"]
+        assert np.abs(ref["v"]).max() > 0
+        assert rel_l2(patches["v"], ref["v"]) < 1e-4
+
+
 @pytest.mark.parametrize("kind", ["maxwell", "powerlaw", "afterslip", "transient"])
 def test_rk2_adaptive_step_matches_oracle(kind):
     """Without replay the device-side Dt (min 1/gammadot over the five
@@ -491,14 +514,17 @@ def test_periodic_load_trajectory_matches_oracle():
     """Time-dependent surface loads: each stage of the step adds the rate of
     the current event's periodic loads to the surface traction (relax.f90:
     684-685, 805-806); the trajectory with a Maxwell layer follows the
-    oracle's."""
+    oracle's.  The periodic load has phase 0, so it adds nothing at t = 0:
+    the same model without it has the same coseismic state, and its
+    trajectory differs only through the rate terms of the steps."""
     s = small_spec("maxwell")
-    s["loads"] = [(2.0, -1.0, 12.0, 8.0, 0.5, 2.5, 0.3), (-6.0, 5.0, 6.0, 6.0, 0.2, 0.0, 0.0)]
+    s["loads"] = [(2.0, -1.0, 12.0, 8.0, 0.5, 7.0, 0.0), (-6.0, 5.0, 6.0, 6.0, 0.2, 0.0, 0.0)]
     nsteps = 5
     st = oracle_initial_state(s)
     om = oracle_model(s)
     info = O.rk2_run(om, st, nsteps)
     g = workloads.build(s)
+    setup = g.download()
     recs = g.run(nsteps, dt_replay=info[:, 0])
     out = g.download()
     g.close()
@@ -507,13 +533,19 @@ def test_periodic_load_trajectory_matches_oracle():
         assert rel_l2(out[k][..., :s["sx1"]], st[k][..., :s["sx1"]]) < 1e-4, k
     for k in ("sig", "tau"):
         assert rel_l2(out[k], st[k]) < 1e-4, k
-    # the load's rate moved the surface: the same model without it differs
+    # without the periodic load: the same set-up, bit for bit ...
     s2 = dict(s, loads=[s["loads"][1]])
     g = workloads.build(s2)
+    setup2 = g.download()
+    for k in ("u1", "u2", "u3", "sig", "tau"):
+        np.testing.assert_array_equal(setup2[k], setup[k], err_msg=k)
+    # ... and a different trajectory: the load's rate moved the surface
     g.run(nsteps, dt_replay=info[:, 0])
     u3 = g.download()["u3"]
     g.close()
-    assert rel_l2(u3[..., :s["sx1"]], st["u3"][..., :s["sx1"]]) > 1e-3
+    du = out["u3"][..., :s["sx1"]] - setup["u3"][..., :s["sx1"]]
+    du2 = u3[..., :s["sx1"]] - setup["u3"][..., :s["sx1"]]
+    assert rel_l2(du2, du) > 1e-2
 
 
 def test_output_time_truncation_c0():

@@ -1375,3 +1375,208 @@ def test_rk2_periodic_load_rate_telescopes():
     for k in range(3):
         np.testing.assert_allclose(st["u%d" % (k + 1)][..., :sx], c[k][:h3, :, :sx], rtol=0,
                                    atol=1e-9 * np.abs(c[2][:h3]).max())
+
+
+# ------------------------------------------- depth stencils on quadratics --
+def _depth_derivative(b, c, z, dx3):
+    """What the reference's depth differences give for f = a + b z + c z^2
+    (elastic3d.f90:977-1002, :1259-1270, fir1 in the interior): centred,
+    exact, inside; forward at the top layer (b + c dx3); backward at the
+    bottom layer (b + 2 c z - c dx3)."""
+    d = b + 2 * c * z
+    d[0] = b + c * dx3
+    d[-1] = b + 2 * c * z[-1] - c * dx3
+    return d
+
+
+@pytest.mark.parametrize("mut", [0, 8, 25])
+def test_stressupdate_quadratic_in_depth(mut, mutation):
+    """Quadratic depth profiles tell a centred difference from a one-sided
+    one: the interior must be exact and the two boundary layers off by
+    exactly +-c dx3."""
+    sx1, sx2, h3 = 8, 16, 24
+    dx1, dx2, dx3 = 0.5, 0.75, 1.25
+    z = np.arange(h3) * dx3
+    coef = [(0.3, -0.2, 0.05), (-1.0, 0.4, -0.03), (0.5, 0.7, 0.02)]  # (a, b, c) of u1, u2, u3
+    u = [np.broadcast_to((a + b * z + c * z * z)[:, None, None], (h3, sx2, sx1)) for a, b, c in coef]
+    D = [_depth_derivative(b, c, z, dx3) for _, b, c in coef]
+    e13, e23, e33 = 0.5 * D[0], 0.5 * D[1], D[2]
+    ref = np.zeros((h3, 6))
+    ref[:, 0] = LAM * e33
+    ref[:, 2] = 2 * MU * e13
+    ref[:, 3] = LAM * e33
+    ref[:, 4] = 2 * MU * e23
+    ref[:, 5] = (2 * MU + LAM) * e33
+    sig = np.zeros((h3, sx2, sx1, 6))
+    mutation(mut)
+    O.stressupdate(*(padded(np.ascontiguousarray(a)) for a in u), LAM, MU, dx1, dx2, dx3, sig)
+    err = np.abs(sig - ref[:, None, None, :]).max() / np.abs(ref).max()
+    if mut == 0:
+        assert err < 1e-12
+    else:
+        assert err > 1e-3
+
+
+@pytest.mark.parametrize("mut", [0, 26])
+def test_equivalentbodyforce_quadratic_in_depth(mut, mutation):
+    sx1, sx2, h3 = 16, 8, 20
+    dx1, dx2, dx3 = 0.5, 0.75, 1.25
+    z = np.arange(h3) * dx3
+    s = np.zeros((h3, sx2, sx1, 6))
+    coef = {2: (0.1, 0.5, -0.04), 4: (-0.3, 0.2, 0.06), 5: (1.0, -0.8, 0.03)}  # s13, s23, s33
+    for comp, (a, b, c) in coef.items():
+        s[..., comp] = (a + b * z + c * z * z)[:, None, None]
+    f = [_depth_derivative(b, c, z, dx3) for _, b, c in (coef[2], coef[4], coef[5])]
+    cc = [np.zeros((h3, sx2, sx1 + 2)) for _ in range(3)]
+    t = [np.zeros((sx2, sx1 + 2)) for _ in range(3)]
+    mutation(mut)
+    O.equivalentbodyforce(s, dx1, dx2, dx3, cc[0], cc[1], cc[2], t[0], t[1], t[2])
+    err = max(np.abs(cc[k][..., :sx1] + f[k][:, None, None]).max() for k in range(3))
+    if mut == 0:
+        assert err < 1e-12
+        for k, comp in enumerate((2, 4, 5)):
+            np.testing.assert_allclose(t[k][:, :sx1], coef[comp][0], rtol=1e-14)
+    else:
+        assert err > 1e-3
+
+
+# ------------------------------------------------ fault creep closed forms --
+def _plane_frame(strike, dip):
+    """Unit vectors of a fault plane in the internal convention, from the
+    geometry alone: the strike direction is horizontal, the normal is
+    perpendicular to it and tilted by the dip from the horizontal, and the
+    down-dip direction completes the frame with a positive x3 (depth)
+    component for |dip| < 90 degrees."""
+    s = np.array([np.sin(strike), np.cos(strike), 0.0])
+    h = np.array([np.cos(strike), -np.sin(strike), 0.0])  # horizontal, perpendicular to the strike
+    n = np.cos(dip) * h + np.sin(dip) * np.array([0.0, 0.0, -1.0])
+    d = np.cross(n, s)
+    if d[2] < 0:
+        d = -d
+    return s, n, d
+
+
+def _uniform_traction_stress(n, e, tau0, p):
+    """sigma = tau0 (n e^T + e n^T) + p I: on the plane of normal n (|n| = 1,
+    e in the plane, |e| = 1) the traction is tau0 e + p n."""
+    S = tau0 * (np.outer(n, e) + np.outer(e, n)) + p * np.eye(3)
+    return np.array([S[0, 0], S[0, 1], S[0, 2], S[1, 1], S[1, 2], S[2, 2]])
+
+
+@pytest.mark.parametrize("mut", [0, 27])
+def test_friction_relaxation_time_unequal_spacing(mut, mutation):
+    """tm = tau / mu / gammadot * min(L, W) / sqrt(dx1 dx3) (friction3d.f90:
+    223) with dx1, dx2, dx3 all different, for a uniform shear traction
+    along the rake: every point near the plane gives the same candidate."""
+    h3, sx = 32, 48
+    dx1, dx2, dx3 = 0.5, 0.8, 0.3
+    st = np.zeros(h3, O.LAYER)
+    st["gammadot0"], st["stressexponent"], st["friction"], st["cohesion"] = 2e-3, 1.5, 0.4, 0.05
+    strike, dip, rake = 0.3, 0.2, 0.0
+    s, n, d = _plane_frame(strike, dip)
+    tau0, p = 0.9, -0.25
+    sig = np.zeros((h3, sx, sx, 6))
+    sig[...] = _uniform_traction_stress(n, s, tau0, p)
+    mom = np.zeros_like(sig)
+    L, W = 4.0, 6.0
+    mutation(mut)
+    tm = O.frictioneigenstress(0.0, 0.0, 3.0, L, W, strike, dip, rake, 0.2, sig, MU, st, dx1, dx2, dx3, mom,
+                               maxwelltime=1e7)
+    tau = tau0 + 0.4 * p - 0.05
+    gd = 2e-3 * 2 * np.sinh(tau / 1.5)
+    expect = float(np.float32(tau / MU / gd * min(L, W) / np.sqrt(dx1 * dx3)))
+    if mut == 0:
+        assert np.abs(mom).max() > 0
+        assert tm == pytest.approx(expect, rel=1e-6)
+    else:
+        assert abs(tm / expect - 1) > 0.1
+
+
+def _monitor_setup():
+    h3, sx = 32, 48
+    dx1, dx2, dx3 = 1.0, 0.75, 0.5
+    st = np.zeros(h3, O.LAYER)
+    st["gammadot0"] = 1e-3 * (1.0 + np.arange(h3))        # v0 by depth
+    st["stressexponent"] = 1.0 + 0.1 * np.arange(h3)       # (a-b) sigma by depth
+    st["friction"] = 0.3
+    st["cohesion"] = 0.05
+    P = dict(x=0.3, y=-0.4, z=4.0, L=6.0, W=4.0, strike=0.4, dip=0.5)
+    return h3, sx, (dx1, dx2, dx3), st, P
+
+
+@pytest.mark.parametrize("mut", [0, 28, 30])
+def test_monitorfriction_patch_geometry(mut, mutation):
+    """The patch centres (friction3d.f90:318-340) lie on the fault plane,
+    px2 x px3 of them spaced dx2 along the (horizontal) strike and dx3 down
+    dip, patch (px2/2+1, px3/2+1) at the plane's centre; lx and lz are the
+    in-plane coordinates of each centre."""
+    h3, sx, (dx1, dx2, dx3), st, P = _monitor_setup()
+    s, n, d = _plane_frame(P["strike"], P["dip"])
+    px2, px3 = 6, 4
+    patch = np.zeros((px3, px2), O.PATCH)
+    sig = np.zeros((h3, sx, sx, 6))
+    mutation(mut)
+    O.monitorfriction(P["x"], P["y"], P["z"], P["L"], P["W"], P["strike"], P["dip"], 0.0, 0.2, 2 * h3,
+                      dx1, dx2, dx3, sig, st, patch)
+    C = np.array([P["x"], P["y"], P["z"]])
+    X = np.stack([patch["x1"], patch["x2"], patch["x3"]], -1)
+    j2 = np.arange(1, px2 + 1) - px2 / 2 - 1
+    j3 = np.arange(1, px3 + 1) - px3 / 2 - 1
+    expect = C + j2[None, :, None] * dx2 * s + j3[:, None, None] * dx3 * d
+    err = np.abs(X - expect).max()
+    if mut == 0:
+        assert err < 1e-5  # the in-plane offsets are rounded to single precision
+        assert np.abs((X - C) @ n).max() < 1e-5
+        np.testing.assert_allclose(patch["lx"] - patch["lx"][:, :1],
+                                   np.broadcast_to((j2 - j2[0]) * dx2, (px3, px2)), atol=1e-5)
+        np.testing.assert_allclose(patch["lz"] - patch["lz"][:1, :],
+                                   np.broadcast_to(((j3 - j3[0]) * dx3)[:, None], (px3, px2)), atol=1e-5)
+    else:
+        assert err > 1e-2
+
+
+@pytest.mark.parametrize("mut", [0, 29])
+def test_monitorfriction_velocity_law(mut, mutation):
+    """v = 2 v0 sinh(max(0, tau_s + mu_f tau_n - c) / (a-b)sigma) with the
+    friction parameters of the depth index of each patch centre (nearest
+    grid point, ties down), split into the strike and (up-)dip directions;
+    the rake test skips patches whose shear traction opposes the rake unless
+    |rake| >= 3 pi (friction3d.f90:341-407)."""
+    h3, sx, (dx1, dx2, dx3), st, P = _monitor_setup()
+    s, n, d = _plane_frame(P["strike"], P["dip"])
+    px2, px3 = 6, 4
+    tau0, p = 0.8, -0.3
+    # rake 0 points along the strike, rake +90 degrees down dip (the vector r
+    # of friction3d.f90:303-305); the dip-direction rate (dv, :291) is positive up dip
+    cases = [("strike", s, 0.0, 1.0, 0.0), ("updip", -d, -np.pi / 2, 0.0, 1.0), ("downdip", d, np.pi / 2, 0.0, -1.0)]
+    for name, e, rake, fss, fds in cases:
+        sig = np.zeros((h3, sx, sx, 6))
+        sig[...] = _uniform_traction_stress(n, e, tau0, p)
+        patch = np.zeros((px3, px2), O.PATCH)
+        mutation(mut)
+        O.monitorfriction(P["x"], P["y"], P["z"], P["L"], P["W"], P["strike"], P["dip"], rake, 0.2, 2 * h3,
+                          dx1, dx2, dx3, sig, st, patch)
+        mutation(0)
+        i3 = np.ceil(patch["x3"] / dx3 - 0.5).astype(int)  # nearest, ties to the lower index
+        v0, tc = st["gammadot0"][i3], st["stressexponent"][i3]
+        tau = max(0.0, tau0 + 0.3 * p - 0.05)
+        v = 2 * v0 * np.sinh(tau / tc)
+        if mut:
+            assert np.abs(patch["v"] / v - 1).max() > 1e-2
+            return
+        np.testing.assert_allclose(patch["v"], v, rtol=1e-6, err_msg=name)
+        np.testing.assert_allclose(patch["vss"], fss * v, rtol=1e-6, atol=1e-12, err_msg=name)
+        np.testing.assert_allclose(patch["vds"], fds * v, rtol=1e-6, atol=1e-12, err_msg=name)
+        np.testing.assert_allclose(patch["taus"], tau0, rtol=1e-6)
+    # shear traction against the rake: the patch keeps its previous values ...
+    sig = np.zeros((h3, sx, sx, 6))
+    sig[...] = _uniform_traction_stress(n, -s, tau0, p)
+    patch = np.zeros((px3, px2), O.PATCH)
+    patch["v"] = 12345.0
+    O.monitorfriction(P["x"], P["y"], P["z"], P["L"], P["W"], P["strike"], P["dip"], 0.0, 0.2, 2 * h3,
+                      dx1, dx2, dx3, sig, st, patch)
+    assert np.all(patch["v"] == 12345.0)
+    # ... unless |rake| >= 3 pi, where the test is off
+    O.monitorfriction(P["x"], P["y"], P["z"], P["L"], P["W"], P["strike"], P["dip"], 3 * np.pi + 0.1, 0.2,
+                      2 * h3, dx1, dx2, dx3, sig, st, patch)
+    assert np.all(patch["v"] > 0) and np.all(patch["v"] < 1.0)
