@@ -296,6 +296,18 @@ int relax_model_synchronize(relax_model_t* m);
  * in both stages (relax.f90:684-685, :805-806) until the next event.  Not
  * on a member of a batch. */
 int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist);
+/* Inter-seismic loading (in%inter: relax.f90:669-674, :795-799, with
+ * dislocations' eigenstress form :1116-1289): ev->s are shear segments whose
+ * slip is a rate (each with its own taper beta), ev->ts tensile segments
+ * whose opening is a rate (the model's taper); ev->m and ev->l must be
+ * empty, as the deck has no loading of those kinds (input.f90:1114-1250).
+ * Every later corrector adds their moment density m to the moment of the
+ * rheology and to tau, unscaled and times Dt (tau += m + Dt (moment + m)),
+ * and a step where nothing relaxes takes tm = interval/20.  The model keeps
+ * m (24 bytes per point, beyond relax_model_footprint) and steps with the
+ * extended stencil kernels.  NULL or no segments removes the loading.  Not
+ * on a member of a batch. */
+int relax_model_set_interseismic(relax_model_t* m, const relax_event_t* ev);
 /* Utilities of the same set-up: add_shear adds momentdensityshear
  * (elastic3d.f90:2661-2750) of n rectangular patches (centre x,y,z, width,
  * length, strike, dip, rake in the internal convention) with slip[k] to tau

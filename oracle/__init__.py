@@ -50,7 +50,9 @@ class OrcModel(ctypes.Structure):
                 ("ev_src", ctypes.c_void_p), ("ev_off", ctypes.c_void_p), ("ev_loads", ctypes.c_void_p),
                 ("ev_loff", ctypes.c_void_p), ("nyquist", ctypes.c_double),
                 ("ev_ts", ctypes.c_void_p), ("ev_tsoff", ctypes.c_void_p), ("ev_mg", ctypes.c_void_p),
-                ("ev_mgoff", ctypes.c_void_p)]
+                ("ev_mgoff", ctypes.c_void_p),
+                ("inter_s", ctypes.c_void_p), ("inter_ns", ctypes.c_int),
+                ("inter_ts", ctypes.c_void_p), ("inter_nt", ctypes.c_int)]
 
 
 def lib():
@@ -304,8 +306,10 @@ def rk2_run(model, state, nsteps, dt_replay=None):
     entry per event after the first) of SOURCE arrays, event_loads,
     event_tensile and event_mogi likewise (surface loads, tensile cracks,
     point dilatation sources; optional), loads0 (the first event's surface
-    loads, whose rate the steps add until the next event), and nyquist
-    (dislocations' size gate).
+    loads, whose rate the steps add until the next event), nyquist
+    (dislocations' size gate), and the inter-seismic loading: inter and
+    inter_tensile, SOURCE arrays of shear patches and tensile cracks whose
+    slip or opening is a rate (relax.f90:669-674, :795-799).
     state["e"] is the current event (1-based, default 1).  Returns an
     (nsteps, 10) array of
     Dt, tm, maxwell(1:3), power, C:E^i, t, maxwell(4:5).
@@ -340,6 +344,11 @@ def rk2_run(model, state, nsteps, dt_replay=None):
             setattr(m, fs, ptr(np.concatenate(lists).astype(SOURCE)))
             setattr(m, fo, ptr(off))
     m.nyquist = model.get("nyquist", 1.0)
+    for key, fp, fn in (("inter", "inter_s", "inter_ns"), ("inter_tensile", "inter_ts", "inter_nt")):
+        a = model.get(key)
+        if a is not None and len(a):
+            setattr(m, fp, ptr(np.asarray(a, SOURCE)))
+            setattr(m, fn, len(a))
     for k, key in enumerate(("dg_linear", "dg_nonlinear", "dg_ltransient", "dg_nltransient")):
         a = model.get(key)
         if a is not None:

@@ -1340,10 +1340,40 @@ struct orc_model {
   const int* ev_tsoff;
   const orc_source* ev_mg;
   const int* ev_mgoff;
+  // inter-seismic loading (in%inter): shear patches and tensile cracks whose
+  // slip or opening is a rate (input.f90:1114-1250); NULL/0: none
+  const orc_source* inter_s;
+  int inter_ns;
+  const orc_source* inter_ts;
+  int inter_nt;
 };
 }
 
 static const double STEP_MAX = 1e7;
+
+// dislocations(in%inter, ..., tau, eigenstress=moment), relax.f90:795-799
+// with :1116-1289 in its eigenstress form: every loading source's moment
+// density into the corrector's moment, then (the loops that run in either
+// form, :1199-1206, :1247-1254) into tau as well, unscaled by the time step.
+template <class R>
+static void interseismic(const orc_model* m, int sx1, int sx2, int h3, R* moment, R* tau) {
+  const orc_source* s = m->inter_s;
+  const orc_source* ts = m->inter_ts;
+  for (int k = 0; k < m->inter_ns; ++k)
+    momentdensityshear(m->mu, s[k].slip, s[k].x, s[k].y, s[k].z, s[k].width, s[k].length, s[k].strike, s[k].dip,
+                       s[k].rake, s[k].beta, sx1, sx2, h3, m->dx1, m->dx2, m->dx3, moment);
+  if (!MUT(31))
+    for (int k = 0; k < m->inter_ns; ++k)
+      momentdensityshear(m->mu, s[k].slip, s[k].x, s[k].y, s[k].z, s[k].width, s[k].length, s[k].strike,
+                         s[k].dip, s[k].rake, s[k].beta, sx1, sx2, h3, m->dx1, m->dx2, m->dx3, tau);
+  for (int k = 0; k < m->inter_nt; ++k)
+    momentdensitytensile(m->lambda, m->mu, ts[k].opening, ts[k].x, ts[k].y, ts[k].z, ts[k].width, ts[k].length,
+                         ts[k].strike, ts[k].dip, m->beta, sx1, sx2, h3, m->dx1, m->dx2, m->dx3, moment);
+  if (!MUT(31))
+    for (int k = 0; k < m->inter_nt; ++k)
+      momentdensitytensile(m->lambda, m->mu, ts[k].opening, ts[k].x, ts[k].y, ts[k].z, ts[k].width, ts[k].length,
+                           ts[k].strike, ts[k].dip, m->beta, sx1, sx2, h3, m->dx1, m->dx2, m->dx3, tau);
+}
 
 template <class R>
 static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* gamma, R* epsilonik, double* t_io,
@@ -1399,6 +1429,10 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
     double tm = 1.0 / (mech[0] / maxwell[0] + mech[1] / maxwell[1] + mech[2] / maxwell[2] + mech[3] / maxwell[3] +
                        mech[4] / maxwell[4]);
     tm = std::min(tm, STEP_MAX);
+    // with inter-seismic loading and nothing relaxing, a small step
+    // (relax.f90:669-674)
+    const bool inter = m->inter_ns > 0 || m->inter_nt > 0;
+    if (inter && tm == STEP_MAX) tm = m->interval / 20.0;
     // integrationstep, relax.f90:1420-1452
     double Dt = tm / 10.0;
     if (0 == m->skip) {
@@ -1475,6 +1509,8 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
       for (long o = 0; o < 6 * npts; ++o) epsilonik[o] = epsilonik[o] + cdt * epsilonikdot[o];
       std::fill(epsilonikdot.begin(), epsilonikdot.end(), (R)0);
     }
+    // inter-seismic loading (relax.f90:795-799)
+    if (inter) interseismic(m, sx1, sx2, h3, moment.data(), tau);
     std::fill(v1.begin(), v1.end(), (R)0); std::fill(v2.begin(), v2.end(), (R)0); std::fill(v3.begin(), v3.end(), (R)0);
     std::fill(t1.begin(), t1.end(), (R)0); std::fill(t2.begin(), t2.end(), (R)0); std::fill(t3.begin(), t3.end(), (R)0);
     rc = equivalentbodyforce(moment.data(), m->dx1, m->dx2, m->dx3, sx1, sx2, h3, v1.data(), v2.data(), v3.data(),

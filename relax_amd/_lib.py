@@ -110,6 +110,7 @@ SIGNATURES = {
     "relax_model_add_shear": (c_int, [P, P, P, c_int]),
     "relax_model_coseismic": (c_int, [P]),
     "relax_model_event": (c_int, [P, P, c_double]),
+    "relax_model_set_interseismic": (c_int, [P, P]),
     "relax_tensilesource": (c_int, [c_double] * 11 + [c_int, c_int, c_int, c_double, c_double, c_double] + [P] * 3
                             + [c_int]),
     "relax_mogisource": (c_int, [c_double] * 6 + [c_int, c_int, c_int, c_double, c_double, c_double] + [P] * 3

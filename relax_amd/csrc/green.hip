@@ -308,11 +308,23 @@ __global__ __launch_bounds__(NT) void k_green_columns(ColumnConsts C, double mu,
 //     w_H3^(j k1), through LDS ([c][j][k1][column]);
 //   step B (thread half, k1, column): DFT_Q over j for the three components
 //     in registers -> X[k1 + 16 k2].
-// The even (E) and odd (O: x w_n3^z) halves go through the one 48 KB LDS
+// The even (E) and odd (O: x w_n3^z) halves go through the one 52 KB LDS
 // buffer in turn; E ends in threads 0..255 and O in 256..511, which then
 // apply the Green's function and the traction sums to their own 3 Q
 // frequencies, reduce the sums per column, and run the inverse the same way
 // back (B^-1 then A^-1, E then O) to y = E' + w^-z O'.
+//
+// The kernel is bound by its vector instruction issue, so:
+//  - the tile is read and written with buffer instructions: 32-bit offsets
+//    from the tile's base, with the depth stride in the scalar offset, so a
+//    load or store costs no vector address arithmetic (the work spectrum of
+//    a grid of this kernel, H3 <= 128 and n1, n2 <= 1024, is under 4 GiB);
+//  - the twiddles of the split, w_H3^(a b) for a, b < 16, come from an LDS
+//    table whose row is the thread's index, read at constant offsets;
+//  - the traction sum of surfacetractioncowling (green.f90:357-359) is
+//    linear in u, so each thread sums k3 u1, k3 u2, k3 u3, u3 and
+//    k1 u1 + k2 u2 over its frequencies (12 operations per frequency) and
+//    the column's traction is formed from the five sums once.
 template <int H3>
 __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu, double alpha, double gamma, int n1,
                                                    int n2, int n3, MPtr<float2> W_m, MPtr<float2> NQ_m,
@@ -324,18 +336,20 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
   constexpr int NB = 16 * TX;     // step-B threads per half
   constexpr int RS = 16 * TX + 16;  // float2 per (c, j) group (padded: j groups on alternate banks)
   constexpr int NW = NT / 64;
+  constexpr unsigned RSRC3 = 0x00020000u;  // raw buffer, 32-bit data format
   float2* W = W_m.at(blockIdx.y);
   float2* NQ = NQ_m.at(blockIdx.y);
   const float2* __restrict__ TM = TM_m.at(blockIdx.y);
   const float2* __restrict__ TN = TN_m.at(blockIdx.y);
   __shared__ float2 buf[3 * Q * RS];
   __shared__ float2 stw[2 * H3];
+  __shared__ float2 tws[16 * 16];  // w_H3^(a b), row a
   // the per-column reductions reuse the transpose buffer
   float2(*red)[NW][TX] = reinterpret_cast<float2(*)[NW][TX]>(buf);
-  float2(*pv)[TX] = reinterpret_cast<float2(*)[TX]>(buf + 3 * NW * TX);
+  float2(*pv)[TX] = reinterpret_cast<float2(*)[TX]>(buf + 5 * NW * TX);
   // the Cerruti constants of the tile's columns follow pv
-  CerrutiCol* ccs = reinterpret_cast<CerrutiCol*>(buf + 3 * NW * TX + 3 * TX);
-  static_assert((3 * NW * TX + 3 * TX) * sizeof(float2) + TX * sizeof(CerrutiCol) <= 3 * Q * RS * sizeof(float2),
+  CerrutiCol* ccs = reinterpret_cast<CerrutiCol*>(buf + 5 * NW * TX + 3 * TX);
+  static_assert((5 * NW * TX + 3 * TX) * sizeof(float2) + TX * sizeof(CerrutiCol) <= 3 * Q * RS * sizeof(float2),
                 "reductions and column constants fit the buffer");
 
   const int tid = threadIdx.x;
@@ -343,16 +357,21 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
   const int bcol = tid % TX, bk1 = (tid / TX) % 16, half = tid / NB;    // step B
   const bool isA = tid < NA;
   const GreenTileAddr A = green_tile<TX>(xcd_tile(blockIdx.x, gridDim.x), bcol, n1, n2, H3, l2, W, NQ, TM, TN);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(A.base, (short)0, 0x7fffffff, RSRC3);
+  const unsigned voff = (unsigned)((ac * A.cs + aj * A.zs + acol) * (long)sizeof(float2));
+  const unsigned zstep = (unsigned)(Q * A.zs * (long)sizeof(float2));  // uniform: the scalar offset
 
   // step A of one half: DFT16, twiddle, into the buffer.  The odd half
   // loads the column again (from L2) rather than keeping it in registers.
   float2 x[16];
   auto load = [&]() {
 #pragma unroll
-    for (int m = 0; m < 16; ++m) x[m] = gload(A.base + ac * A.cs + (Q * m + aj) * A.zs + acol);
+    for (int m = 0; m < 16; ++m)
+      x[m] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, m * zstep, 0));
   };
   if (isA) load();
   for (int e = tid; e < 2 * H3; e += NT) stw[e] = tw3[e];
+  if (tid < 256) tws[tid] = tw3[2 * (((tid >> 4) * (tid & 15)) % H3)];
   __syncthreads();
 
   auto stepA = [&](bool odd) {
@@ -365,11 +384,14 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
     }
     dft16<-1>(y);
     float2* row = buf + (ac * Q + aj) * RS + acol;
+    const float2* tw = tws + 16 * aj;
+    row[0] = y[0];
 #pragma unroll
-    for (int k1 = 0; k1 < 16; ++k1) row[k1 * TX] = k1 && aj ? cmul(y[k1], stw[(2 * aj * k1) % (2 * H3)]) : y[k1];
+    for (int k1 = 1; k1 < 16; ++k1) row[k1 * TX] = cmul(y[k1], tw[k1]);
   };
   // step B: this thread's 3 Q frequencies of its half
   float2 X[3][Q];
+  const float2* twb = tws + 16 * bk1;
   auto stepB = [&]() {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -386,42 +408,75 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
   __syncthreads();
   if (half == 1) stepB();
 
-  // Green's function and the traction partial sums
+  // Green's function and the sums of k3 u and u over this thread's
+  // frequencies
   const float k1w = (float)A.k1i * C.ik1;
   const float k2w = wavenumber23f(A.k2i, n2, C.ik2);
-  float2 ps1 = make_float2(0.f, 0.f), ps2 = ps1, ps3 = ps1;
+  const float kh2 = k1w * k1w + k2w * k2w;
+  // the traction sums: k3 u1, k3 u2, k3 u3, u3 and k1 u1 + k2 u2
+  float2 S5[5];
+#pragma unroll
+  for (int a = 0; a < 5; ++a) S5[a] = make_float2(0.f, 0.f);
+  const bool origin = A.k1i == 0 && A.k2i == 0 && bk1 == 0 && half == 0;
 #pragma unroll
   for (int k2 = 0; k2 < Q; ++k2) {
-    const int k3i = 2 * (bk1 + 16 * k2) + half;
-    const float k3 = wavenumber23f(k3i, n3, C.ik3);
+    const float k3 = wavenumber23f(2 * (bk1 + 16 * k2) + half, n3, C.ik3);
     float2 a1 = X[0][k2], a2 = X[1][k2], a3 = X[2][k2];
-    if (A.k1i == 0 && A.k2i == 0 && k3i == 0) {
-      a1 = a2 = a3 = make_float2(0.f, 0.f);
-    } else {
-      elastic_point(C, k1w, k2w, k3, a1, a2, a3);
-    }
+    // elasticresponse (green.f90:63-86) as s c - k (k.c); the k = 0
+    // frequency is zeroed (:96-99)
+    const float r2 = fmaf(k3, k3, kh2);
+    const float denom = C.ratio1 * __builtin_amdgcn_rcpf(r2 * r2);
+    const float s = r2 * C.ratio2p1;
+    const float2 kc = make_float2(k1w * a1.x + k2w * a2.x + k3 * a3.x, k1w * a1.y + k2w * a2.y + k3 * a3.y);
+    a1 = make_float2((s * a1.x - k1w * kc.x) * denom, (s * a1.y - k1w * kc.y) * denom);
+    a2 = make_float2((s * a2.x - k2w * kc.x) * denom, (s * a2.y - k2w * kc.y) * denom);
+    a3 = make_float2((s * a3.x - k3 * kc.x) * denom, (s * a3.y - k3 * kc.y) * denom);
+    if (origin && k2 == 0) a1 = a2 = a3 = make_float2(0.f, 0.f);
     X[0][k2] = a1; X[1][k2] = a2; X[2][k2] = a3;
-    float2 s1, s2, s3;
-    traction_point(C, k1w, k2w, k3, a1, a2, a3, s1, s2, s3);
-    ps1 = cadd(ps1, s1); ps2 = cadd(ps2, s2); ps3 = cadd(ps3, s3);
+    S5[0] = make_float2(fmaf(k3, a1.x, S5[0].x), fmaf(k3, a1.y, S5[0].y));
+    S5[1] = make_float2(fmaf(k3, a2.x, S5[1].x), fmaf(k3, a2.y, S5[1].y));
+    S5[2] = make_float2(fmaf(k3, a3.x, S5[2].x), fmaf(k3, a3.y, S5[2].y));
+    S5[3] = cadd(S5[3], a3);
+    S5[4] = make_float2(fmaf(k1w, a1.x, fmaf(k2w, a2.x, S5[4].x)), fmaf(k1w, a1.y, fmaf(k2w, a2.y, S5[4].y)));
   }
   // reduce over the threads of a column (fixed order): lanes 16 apart in a
   // wave, then the waves
 #pragma unroll
   for (int off = 32; off >= TX; off >>= 1) {
-    ps1.x += __shfl_xor(ps1.x, off); ps1.y += __shfl_xor(ps1.y, off);
-    ps2.x += __shfl_xor(ps2.x, off); ps2.y += __shfl_xor(ps2.y, off);
-    ps3.x += __shfl_xor(ps3.x, off); ps3.y += __shfl_xor(ps3.y, off);
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+      S5[a].x += __shfl_xor(S5[a].x, off);
+      S5[a].y += __shfl_xor(S5[a].y, off);
+    }
   }
   __syncthreads();  // the last step B has read the buffer
   if ((tid & 63) < TX) {
-    red[0][tid >> 6][bcol] = ps1; red[1][tid >> 6][bcol] = ps2; red[2][tid >> 6][bcol] = ps3;
+#pragma unroll
+    for (int a = 0; a < 5; ++a) red[a][tid >> 6][bcol] = S5[a];
   }
   __syncthreads();
   if (tid < 3 * TX) {
     const int c = tid / TX, t = tid % TX;
-    float2 acc = red[c][0][t];
-    for (int w = 1; w < NW; ++w) acc = cadd(acc, red[c][w][t]);
+    // this column's wavenumbers (thread t of the first wave holds column t)
+    const float q1 = __shfl(k1w, t), q2 = __shfl(k2w, t);
+    float2 S[5];
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+      S[a] = red[a][0][t];
+      for (int w = 1; w < NW; ++w) S[a] = cadd(S[a], red[a][w][t]);
+    }
+    // sum_k3 of the traction summands (green.f90:357-359):
+    //   i 2 pi mu (k3 u1 + k1 u3), i 2 pi mu (k3 u2 + k2 u3),
+    //   i 2 pi ((l+2m) k3 u3 + l (k1 u1 + k2 u2)) - 2 mu alpha gamma u3
+    float2 acc;
+    if (c < 2) {
+      const float qc = c == 0 ? q1 : q2;
+      const float2 a = make_float2(S[c].x + qc * S[3].x, S[c].y + qc * S[3].y);
+      acc = make_float2(-C.mu2pi * a.y, C.mu2pi * a.x);
+    } else {
+      const float2 d = make_float2(C.mod2pi * S[2].x + C.lam2pi * S[4].x, C.mod2pi * S[2].y + C.lam2pi * S[4].y);
+      acc = make_float2(-d.y - C.grav * S[3].x, d.x - C.grav * S[3].y);
+    }
     const float2 tt = gload(A.tbase + c * A.tcs + t);
     // p = t - p / (sx3 dx3)   (green.f90:373-375, :1082-1084)
     pv[c][t] = make_float2(tt.x - acc.x / C.inv_n3dx3, tt.y - acc.y / C.inv_n3dx3);
@@ -435,10 +490,9 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
     const CerrutiCol cc = ccs[bcol];
     const float2 p1 = pv[0][bcol], p2 = pv[1][bcol];
 #pragma unroll
-    for (int k2 = 0; k2 < Q; ++k2) {
-      const float k3 = wavenumber23f(2 * (bk1 + 16 * k2) + half, n3, C.ik3);
-      cerruti_point(C, cc, p1, p2, k3, X[0][k2], X[1][k2], X[2][k2]);
-    }
+    for (int k2 = 0; k2 < Q; ++k2)
+      cerruti_point(C, cc, p1, p2, wavenumber23f(2 * (bk1 + 16 * k2) + half, n3, C.ik3), X[0][k2], X[1][k2],
+                    X[2][k2]);
   }
   __syncthreads();  // pv and the constants are read before the buffer is refilled
 
@@ -447,10 +501,9 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       dft<Q, +1>(X[c]);
+      buf[(c * Q) * RS + bk1 * TX + bcol] = X[c][0];
 #pragma unroll
-      for (int j = 0; j < Q; ++j)
-        buf[(c * Q + j) * RS + bk1 * TX + bcol] =
-            j && bk1 ? cmul(X[c][j], cconj(stw[(2 * j * bk1) % (2 * H3)])) : X[c][j];
+      for (int j = 1; j < Q; ++j) buf[(c * Q + j) * RS + bk1 * TX + bcol] = cmul(X[c][j], cconj(twb[j]));
     }
   };
   auto stepAinv = [&](float2* y) {
@@ -468,10 +521,11 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
   if (isA) {
     float2 o[16];
     stepAinv(o);  // O'
+    typedef unsigned u2v __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
-      const int z = Q * m + aj;
-      gstore(A.base + ac * A.cs + z * A.zs + acol, cadd(x[m], cmul(o[m], cconj(stw[z]))));
+      const float2 v = cadd(x[m], cmul(o[m], cconj(stw[Q * m + aj])));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, v), rs, voff, m * zstep, 0);
     }
   }
 }

@@ -34,6 +34,8 @@ struct relax_model {
   float* E = nullptr;       // 6 planes of H: transient strain (with a transient structure)
   float* Em = nullptr;      // 6 planes of H: its mid-step value
   float* dg[4] = {nullptr, nullptr, nullptr, nullptr};  // weak-zone fields (H each)
+  float* mi = nullptr;      // 6 planes of H: moment density of the inter-seismic loading
+  int inter = 0;            // inter-seismic loading set (RheoParams::inter = mi)
   WeakDev* zones_dev = nullptr;  // staging of set_weakzones' zones
   void* pts_dev = nullptr;        // staging of relax_model_points (grow-only)
   double* ev_dev = nullptr;       // event times on the device

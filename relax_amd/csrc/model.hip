@@ -57,6 +57,9 @@ __global__ void k_finalize_dt(const ModelDev* __restrict__ md, int nm, int use_o
   for (int k = 0; k < NMECH; ++k) rate = rate + (double)tl.mech[k] / maxwell[k];
   double tm = 1.0 / rate;
   tm = fmin(tm, STEP_MAX);
+  // inter-seismic loading with nothing relaxing: a small step
+  // (relax.f90:669-674)
+  if (tl.inter && tm == STEP_MAX) tm = tl.interval / 20.0;
   const double t = sc->t;
   double Dt = tm / 10.0;
   if (tl.skip == 0) {
@@ -264,7 +267,7 @@ static void free_model(relax_model_t* m) {
   if (!m) return;
   void* ptrs[] = {m->u, m->sig, m->tau, m->gam, m->W, m->NQ, m->T, m->TN, m->sc, m->partials, m->tables, m->ftab,
                   m->planes_dev, m->patches_dev, m->md_dev, m->shear_dev, m->shear_slip, m->loads_dev, m->E, m->Em,
-                  m->dg[0], m->dg[1], m->dg[2], m->dg[3], m->zones_dev, m->pts_dev, m->ev_dev};
+                  m->dg[0], m->dg[1], m->dg[2], m->dg[3], m->zones_dev, m->pts_dev, m->ev_dev, m->mi};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->host_sc) (void)hipHostFree(m->host_sc);
@@ -303,6 +306,7 @@ static TimeLine timeline(const relax_model_t* m) {
   tl.ev = m->ev_dev;
   tl.ne = (int)m->events.size();
   for (int k = 0; k < NMECH; ++k) tl.mech[k] = m->mech[k];
+  tl.inter = m->inter;
   return tl;
 }
 
@@ -310,7 +314,7 @@ static TimeLine timeline(const relax_model_t* m) {
 // transient creep, the fault-creep kernels with afterslip planes.
 static int rheo_class(const relax_model_t* m) {
   const RheoParams& R = m->R;
-  if (R.tr[0].active || R.tr[1].active || R.mech[0].dg || R.mech[1].dg) return RHEO_EXT;
+  if (R.tr[0].active || R.tr[1].active || R.mech[0].dg || R.mech[1].dg || R.inter) return RHEO_EXT;
   return (R.fr.active && R.fr.np > 0) ? RHEO_CREEP : RHEO_VISCOUS;
 }
 
@@ -480,6 +484,8 @@ int relax_model_reset(relax_model_t* m) {
   m->patch_off.assign(1, 0);
   m->R.fr.np = 0;
   m->R.fr.active = 0;
+  m->R.inter = nullptr;
+  m->inter = 0;
   m->md_dirty = true;
   return sync_md(m);
 }
@@ -1659,6 +1665,59 @@ int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist)
   if ((rc = launch_stress(STRESS_SET, g, m->S, c.md, c.one, 1, RHEO_VISCOUS, m->stream))) return rc;
   HIP_TRY(hipStreamSynchronize(m->stream));
   return RELAX_OK;
+}
+
+// Inter-seismic loading (in%inter, input.f90:1114-1250; relax.f90:669-674,
+// :795-799): the moment density of the loading segments -- shear segments
+// with their own taper, tensile segments with the model's -- is formed once
+// on the device; every corrector adds it to the moment of the rheology
+// before the divergence, and to tau unscaled as well as times Dt.  The
+// model then runs the extended stencil class.
+int relax_model_set_interseismic(relax_model_t* m, const relax_event_t* ev) {
+  if (!m || m->batch) return RELAX_ERR_ARG;
+  if (ev && (ev->ns < 0 || ev->nt < 0 || ev->nm != 0 || ev->nl != 0 || (ev->ns > 0 && !ev->s) ||
+             (ev->nt > 0 && !ev->ts)))
+    return RELAX_ERR_ARG;
+  const int ns = ev ? ev->ns : 0, nt = ev ? ev->nt : 0;
+  int rc;
+  if (ns + nt == 0) {
+    m->R.inter = nullptr;
+    m->inter = 0;
+    m->md_dirty = true;
+    return sync_md(m);
+  }
+  const Grid& g = m->g;
+  const long H = g.H;
+  if (!m->mi) HIP_TRY(hipMalloc(&m->mi, 6 * H * sizeof(float)));
+  HIP_TRY(hipMemsetAsync(m->mi, 0, 6 * H * sizeof(float), m->stream));
+  std::vector<PlaneDev>& hp = m->shear_host;
+  std::vector<double> amp;
+  hp.clear();
+  for (int k = 0; k < ns; ++k) {
+    hp.push_back(source_plane(ev->s[k], m->desc.dx1, m->desc.dx3));
+    amp.push_back(ev->s[k].slip);
+  }
+  for (int k = 0; k < nt; ++k) {
+    relax_source_t y = ev->ts[k];
+    y.beta = m->desc.beta;
+    hp.push_back(source_plane(y, m->desc.dx1, m->desc.dx3));
+    amp.push_back(ev->ts[k].opening);
+  }
+  if ((rc = stage_planes(m, amp.data()))) return rc;
+  const double dx1 = m->desc.dx1, dx2 = m->desc.dx2, dx3 = m->desc.dx3;
+  if (ns) {
+    k_add_shear<<<grid_1d(H), 256, 0, m->stream>>>(m->shear_dev, m->shear_slip, ns, 2.0f * (float)m->desc.mu, g,
+                                                   dx1, dx2, dx3, m->mi);
+    if (hipGetLastError() != hipSuccess) return RELAX_ERR_HIP;
+  }
+  if (nt && (rc = launch_add_tensile(m->shear_dev + ns, m->shear_slip + ns, nt, m->desc.lambda, m->desc.mu, g.n1,
+                                     g.n2, g.h3, dx1, dx2, dx3, m->mi, H, m->stream)))
+    return rc;
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  m->R.inter = m->mi;
+  m->inter = 1;
+  m->md_dirty = true;
+  return sync_md(m);
 }
 
 int relax_model_coseismic(relax_model_t* m) {

@@ -19,6 +19,7 @@ struct TimeLine {
   int ne;
   int skip;
   int mech[NMECH];
+  int inter;  // inter-seismic loading: tm = interval/20 when nothing relaxes
 };
 
 struct ModelDev {

@@ -17,6 +17,10 @@ struct RheoParams {
   FrictionTables fr;
   int n1, n2;
   double dx1, dx2, dx3;
+  // inter-seismic loading: the moment density of the loading segments,
+  // six planes of H (SoA), or NULL.  The corrector adds it to the moment of
+  // the rheology and, unscaled, to tau (relax.f90:795-799, :1199-1206).
+  const float* inter;
 };
 
 // tensordeviatoric, elastic3d.f90:275-285

@@ -124,6 +124,11 @@ __device__ __forceinline__ void bodyforce_tile(const Grid& g, const RheoParams& 
           const long o = pj1[k] + (long)n1 * (pj2[k] + (long)n2 * z);
           m = rheology_ext<TM, !TM>(R, cur[k], pj1[k], pj2[k], z, o, cur_e[RC == RHEO_EXT ? k : 0], &ed, tmin,
                                     nullptr, nullptr, &status);
+          // the corrector's moment includes the inter-seismic loading
+          if (!TM && R.inter) {
+            const T6 a = load_t6(R.inter, H, o);
+            m = T6{m.s11 + a.s11, m.s12 + a.s12, m.s13 + a.s13, m.s22 + a.s22, m.s23 + a.s23, m.s33 + a.s33};
+          }
         } else {
           m = rheology<TM, RC == RHEO_CREEP>(R, cur[k], pj1[k], pj2[k], z, tmin, nullptr, nullptr, &status);
         }
@@ -239,8 +244,17 @@ __device__ __forceinline__ void stress_point(const RheoParams& R, const StencilC
     } else {
       m = rheology<false, RC == RHEO_CREEP>(R, s, i1, i2, i3, nullptr, &gl, &gn, &status);
     }
-    const T6 tn{t.s11 + dt * m.s11, t.s12 + dt * m.s12, t.s13 + dt * m.s13,
-                t.s22 + dt * m.s22, t.s23 + dt * m.s23, t.s33 + dt * m.s33};
+    T6 t0 = t;
+    if (RC == RHEO_EXT && R.inter) {
+      // inter-seismic loading: its moment density joins the corrector's
+      // moment and, unscaled, tau (relax.f90:795-799, :1199-1206), before
+      // tau += Dt moment (:833)
+      const T6 a = load_t6_nt(R.inter, H, o);
+      t0 = T6{t.s11 + a.s11, t.s12 + a.s12, t.s13 + a.s13, t.s22 + a.s22, t.s23 + a.s23, t.s33 + a.s33};
+      m = T6{m.s11 + a.s11, m.s12 + a.s12, m.s13 + a.s13, m.s22 + a.s22, m.s23 + a.s23, m.s33 + a.s33};
+    }
+    const T6 tn{t0.s11 + dt * m.s11, t0.s12 + dt * m.s12, t0.s13 + dt * m.s13,
+                t0.s22 + dt * m.s22, t0.s23 + dt * m.s23, t0.s33 + dt * m.s33};
     store_t6_nt(tau, H, o, tn);
     if (R.mech[0].active || R.mech[1].active) {
       float gv = gld_nt(gam + o);

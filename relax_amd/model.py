@@ -137,6 +137,14 @@ class Model:
         check(self._lib.relax_model_run(self._h, nsteps, ptr(dts), recs, ctypes.byref(done)), "run")
         return [_info(recs[k]) for k in range(done.value)]
 
+    def set_interseismic(self, sources=None, tensile=None):
+        """Inter-seismic loading (relax.f90:669-674, :795-799): SOURCE arrays
+        of shear segments (slip rate) and tensile segments (opening rate);
+        none removes it."""
+        arr = [np.ascontiguousarray(np.zeros(0, SOURCE) if a is None else a, SOURCE) for a in (sources, tensile)]
+        ev = EventDesc(ptr(arr[0]), len(arr[0]), ptr(arr[1]), len(arr[1]), None, 0, None, 0)
+        check(self._lib.relax_model_set_interseismic(self._h, ctypes.byref(ev)), "set_interseismic")
+
     def add_shear(self, patches, slip):
         """momentdensityshear of each patch (PLANE array, internal convention) into tau."""
         patches = np.ascontiguousarray(patches, PLANE)

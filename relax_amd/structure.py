@@ -100,6 +100,45 @@ def sources(deck, beta):
     return out
 
 
+def antialiasingfilter(slip, length, width, dx1, dx2, dx3, nyquist):
+    """antialiasingfilter, input.f90:1836-1856: a segment shorter or narrower
+    than nyquist*min(dx) is widened to that size with its slip scaled down in
+    proportion.  Returns (slip, length, width)."""
+    dx = min(dx1, dx2, dx3) * nyquist
+    if length < dx:
+        slip = slip * length / dx
+        length = dx
+    if width < dx:
+        slip = slip * width / dx
+        width = dx
+    return slip, length, width
+
+
+def interseismic(deck, tensile_deck, beta, dx, nyquist):
+    """The inter-seismic loading of a deck (input.f90:1114-1250): shear
+    segment lines (slip rate, x, y, z, length, width, strike, dip, rake) and
+    tensile segment lines (opening rate, x, y, z, length, width, strike, dip)
+    as SOURCE arrays.  Each segment goes through antialiasingfilter, then
+    Wang's convention.  For a tensile segment the filter scales its slip
+    field, which is zero, so only its size changes and the opening is kept
+    (input.f90:1205, :1233-1236).  The reference never sets the taper of
+    these segments (types.f90 SOURCE_STRUCT%beta is left as ALLOCATE leaves
+    it); they take the deck's taper beta here, as the coseismic segments do
+    (input.f90:1316).  dx is (dx1, dx2, dx3)."""
+    dx1, dx2, dx3 = dx
+    shear = np.zeros(len(deck or ()), SOURCE)
+    for k, (s, x, y, z, L, W, st, dp, rk) in enumerate(deck or ()):
+        s, L, W = antialiasingfilter(s, L, W, dx1, dx2, dx3, nyquist)
+        s2, x2, y2, z2, st2, dp2, rk2 = wangconvention(s, x, y, z, L, W, st, dp, rk)
+        shear[k] = (s2, x2, y2, z2, W, L, st2, dp2, rk2, 0.0, 0.0, beta, 0.0)
+    tens = np.zeros(len(tensile_deck or ()), SOURCE)
+    for k, (op, x, y, z, L, W, st, dp) in enumerate(tensile_deck or ()):
+        _, L, W = antialiasingfilter(0.0, L, W, dx1, dx2, dx3, nyquist)
+        _, x2, y2, z2, st2, dp2, _ = wangconvention(0.0, x, y, z, L, W, st, dp, 0.0)
+        tens[k] = (0.0, x2, y2, z2, W, L, st2, dp2, 0.0, 0.0, 0.0, beta, op)
+    return shear, tens
+
+
 def loads(deck, beta):
     """Convert deck surface-load lines (x, y, length, width, t3, period,
     phase) to the SOURCE array of an event (input.f90:1686-1716, no frame

@@ -175,6 +175,17 @@ def run(m, s, nsteps, dt_replay=None):
     return recs
 
 
+def interseismic(s):
+    """The spec's inter-seismic loading as (shear, tensile) SOURCE arrays:
+    s["inter"] are deck lines (slip rate, x, y, z, length, width, strike, dip,
+    rake), s["inter_tensile"] (opening rate, x, y, z, length, width, strike,
+    dip); None without any."""
+    if not (s.get("inter") or s.get("inter_tensile")):
+        return None
+    return structure.interseismic(s.get("inter"), s.get("inter_tensile"), BETA, spacing(s),
+                                  s.get("nyquist", NYQUIST))
+
+
 def event_sources(s, e):
     """The source lists of event e (1 = the set-up at t = 0) as SOURCE arrays:
     {sources, loads, tensile, mogi} (shear patches, surface loads, tensile
@@ -205,6 +216,9 @@ def configure(m, s):
     if pl is not None:
         m.set_planes(pl)
     m.set_structure(FAULTCREEP, t["friction"])
+    inter = interseismic(s)
+    if inter is not None:
+        m.set_interseismic(*inter)
     # the first event at t = 0 (relax.f90:316-371)
     m.event(nyquist=s.get("nyquist", NYQUIST), **event_sources(s, 1))
     return m

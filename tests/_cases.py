@@ -65,6 +65,9 @@ def oracle_model(s):
              tscale=s["tscale"], skip=s["skip"], event_times=list(workloads.event_times(s)),
              ltransient=t["ltransient"], nltransient=t["nltransient"])
     m["nyquist"] = s.get("nyquist", workloads.NYQUIST)
+    inter = workloads.interseismic(s)
+    if inter is not None:
+        m["inter"], m["inter_tensile"] = inter
     m["loads0"] = workloads.event_sources(s, 1)["loads"].astype(O.SOURCE)
     if s.get("events"):
         srcs = [workloads.event_sources(s, e) for e in range(2, len(s["events"]) + 2)]

@@ -473,6 +473,35 @@ def test_rk2_trajectory_with_unequal_spacing(kind, dx):
         assert rel_l2(patches["v"], ref["v"]) < 1e-4
 
 
+@pytest.mark.parametrize("kind", ["maxwell", "elastic"])
+def test_interseismic_loading_trajectory_matches_oracle(kind):
+    """Inter-seismic loading (relax.f90:669-674, :795-799): a shear and a
+    tensile loading segment under a Maxwell layer, and with nothing relaxing
+    (tm = interval/20); the adaptive trajectory, the power (which includes the
+    loading moment) and C:E^i follow the oracle's."""
+    s = small_spec("maxwell")
+    s["inter"] = [(1e-3, -8.0, 2.0, 0.0, 12.0, 5.0, 10.0, 70.0, 30.0), (5e-4, 3.0, -6.0, 1.0, 0.5, 6.0, 90.0, 50.0, 0.0)]
+    s["inter_tensile"] = [(4e-4, 4.0, 5.0, 2.0, 5.0, 3.0, 0.0, 90.0)]
+    if kind == "elastic":
+        s["linear"] = None
+        s["interval"] = 40.0
+    st, ost, info, out, recs, om, patches = run_pair(s, 4, replay=False)
+    assert len(recs) == 4
+    if kind == "elastic":
+        np.testing.assert_allclose([r["tm"] for r in recs], s["interval"] / 20.0)
+    for k, r in enumerate(recs):
+        assert abs(r["Dt"] / info[k, 0] - 1) < 1e-5, k
+        assert abs(r["power"] / info[k, 5] - 1) < 1e-4
+        assert abs(r["cei"] / info[k, 6] - 1) < 1e-4
+    for k in ("u1", "u2", "u3"):
+        assert rel_l2(out[k][..., :s["sx1"]], ost[k][..., :s["sx1"]]) < 1e-4, k
+        d, do = (out[k] - st[k])[..., :s["sx1"]], (ost[k] - st[k])[..., :s["sx1"]]
+        assert rel_l2(d, do) < 1e-4, k
+    for k in ("sig", "tau"):
+        assert rel_l2(out[k], ost[k]) < 1e-4, k
+        assert rel_l2(out[k] - st[k], ost[k] - st[k]) < 1e-4, k
+
+
 @pytest.mark.parametrize("kind", ["maxwell", "powerlaw", "afterslip", "transient"])
 def test_rk2_adaptive_step_matches_oracle(kind):
     """Without replay the device-side Dt (min 1/gammadot over the five

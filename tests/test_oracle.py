@@ -1580,3 +1580,64 @@ def test_monitorfriction_velocity_law(mut, mutation):
     O.monitorfriction(P["x"], P["y"], P["z"], P["L"], P["W"], P["strike"], P["dip"], 3 * np.pi + 0.1, 0.2,
                       2 * h3, dx1, dx2, dx3, sig, st, patch)
     assert np.all(patch["v"] > 0) and np.all(patch["v"] < 1.0)
+
+
+# ------------------------------------------------ inter-seismic loading --
+@pytest.mark.parametrize("mut", [0, 31])
+def test_rk2_interseismic_loading_closed_form(mut, mutation):
+    """Inter-seismic loading with nothing relaxing (relax.f90:669-674,
+    :795-799): tm = interval/20, so every step is Dt = interval/200; each
+    corrector's moment is the loading moment density m, so u grows linearly,
+    u(t) - u(0) = t G[-div m, m.e3] (the elastic response to m's body force
+    and surface traction), and tau gains m unscaled plus Dt m per step (the
+    moment-density loops of dislocations run in its eigenstress form too,
+    relax.f90:1199-1206): tau_n - tau_0 = (n + t_n) m; the power of each step
+    is |m|."""
+    from relax_amd import structure
+
+    sx, sx3, dx = 32, 32, 1.0
+    h3 = sx3 // 2
+    shear, tens = structure.interseismic([(1e-3, -6.0, 2.0, 0.0, 10.0, 4.0, 10.0, 70.0, 30.0)],
+                                         [(5e-4, 4.0, -5.0, 2.0, 5.0, 3.0, 0.0, 90.0)], 0.2, (dx, dx, dx), 2.0)
+    pre = np.zeros(h3, O.TENSOR_LAYER)
+    interval = 40.0
+    model = dict(sx1=sx, sx2=sx, sx3=sx3, dx1=dx, dx2=dx, dx3=dx, lam=LAM, mu=MU, gam=8.33e-4, beta=0.2,
+                 linear=None, nonlinear=None, faultcreep=None, prestress=pre, interval=interval, odt=1.0,
+                 tscale=1.0, skip=1, event_times=[0.0], inter=shear, inter_tensile=tens)
+    rng = np.random.default_rng(3)
+    st = dict(u1=np.zeros((h3, sx, sx + 2)), u2=np.zeros((h3, sx, sx + 2)), u3=np.zeros((h3, sx, sx + 2)),
+              tau=rng.standard_normal((h3, sx, sx, 6)) * 0.1, sig=np.zeros((h3, sx, sx, 6)),
+              gamma=np.zeros((h3, sx, sx + 2)), t=0.0, oi=1, i=0)
+    st0 = {k: v.copy() for k, v in st.items() if isinstance(v, np.ndarray)}
+    # the loading moment density m and its elastic response per unit time
+    m = np.zeros((h3, sx, sx, 6))
+    # (dislocations passes a segment's width as L, the down-dip size, and
+    # its length as W, relax.f90:1183-1187)
+    for s in shear:
+        O.momentdensityshear(MU, s["slip"], s["x"], s["y"], s["z"], s["width"], s["length"], s["strike"], s["dip"],
+                             s["rake"], s["beta"], dx, dx, dx, m)
+    for s in tens:
+        O.momentdensitytensile(LAM, MU, s["opening"], s["x"], s["y"], s["z"], s["width"], s["length"], s["strike"],
+                               s["dip"], 0.2, dx, dx, dx, m)
+    assert np.abs(m).max() > 0
+    c = [np.zeros((sx3, sx, sx + 2)) for _ in range(3)]
+    tr = [np.zeros((sx, sx + 2)) for _ in range(3)]
+    O.equivalentbodyforce(m, dx, dx, dx, c[0][:h3], c[1][:h3], c[2][:h3], tr[0], tr[1], tr[2])
+    O.greenfunctioncowling(c[0], c[1], c[2], tr[0], tr[1], tr[2], dx, dx, dx, LAM, MU, 8.33e-4)
+    U = [a[:h3] for a in c]
+    mutation(mut)
+    info = O.rk2_run(model, st, 4)
+    mutation(0)
+    np.testing.assert_allclose(info[:, 1], interval / 20.0)
+    np.testing.assert_allclose(info[:, 0], interval / 200.0)
+    tn = info[-1, 7]
+    assert tn == pytest.approx(4 * interval / 200.0)
+    for k in range(3):
+        du = st["u%d" % (k + 1)] - st0["u%d" % (k + 1)]
+        np.testing.assert_allclose(du[..., :sx], tn * U[k][..., :sx], rtol=1e-9, atol=1e-12 * np.abs(U[k]).max())
+    np.testing.assert_allclose(info[:, 5], O.tensoramplitude(m, dx, dx, dx), rtol=1e-12)
+    err = np.abs(st["tau"] - st0["tau"] - (4 + tn) * m).max() / np.abs(m).max()
+    if mut == 0:
+        assert err < 1e-12
+    else:
+        assert err > 0.5
