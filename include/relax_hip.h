@@ -63,12 +63,25 @@ typedef struct {                                                                
    * (angles in radians, the internal convention of wangconvention) */
   double slip, x, y, z, width, length, strike, dip, rake, period, phase, beta, opening;
 } relax_source_t;
+typedef struct {                                                                 /* types.f90:70-74 */
+  /* a tetrahedron of uniform eigenstrain e between four vertices (1-based
+   * indices into the event's vertex list) */
+  relax_tensor_t e;
+  int i1, i2, i3, i4;
+} relax_tetrahedron_t;
 typedef struct {                                                                 /* types.f90:99-106 */
   /* the sources of one event (EVENT_STRUC without its time and copies) */
   const relax_source_t* s;  int ns;   /* shear patches (slip) */
   const relax_source_t* ts; int nt;   /* tensile cracks (opening) */
   const relax_source_t* m;  int nm;   /* point dilatation sources (slip: volume change) */
   const relax_source_t* l;  int nl;   /* surface loads */
+  /* distributed eigenstrain (--with-eigenstrain, input.f90:1496-1671):
+   * cuboids as WEAK_STRUCT (the eigenstrain in e; centre, width, length,
+   * thickness, strike and dip after Wang's convention), tetrahedra and their
+   * vertices (nv rows of x1, x2, x3; not moved by the frame) */
+  const relax_weak_t* c; int nc;
+  const relax_tetrahedron_t* th; int nth;
+  const double* v; int nv;
 } relax_event_t;
 
 /* ---- library ------------------------------------------------------------ */
@@ -156,6 +169,15 @@ int relax_momentdensitytensile(double lambda, double mu, double s, double x, dou
                                double dx2, double dx3, relax_tensor_t* sig);
 int relax_momentdensitymogi(double lambda, double mu, double o, double xs, double ys, double zs, int sx1, int sx2,
                             int sx3, double dx1, double dx2, double dx3, relax_tensor_t* sig);
+
+/* grnfct_okada, okada/green_space.f90:119-146: Okada's displacement of one
+ * rectangular dislocation (slip, opening; upper corner x, y, z at the start
+ * of the strike, north-east-down; width, length; strike, dip, rake in
+ * degrees) added to u1..u3 ((sx1+2*align) x sx2 x sx3, depth i3 at
+ * (i3-1) dx3). */
+int relax_grnfct_okada(double lambda, double mu, double slip, double opening, double x, double y, double z,
+                       double width, double length, double strike, double dip, double rake, int sx1, int sx2, int sx3,
+                       double dx1, double dx2, double dx3, float* u1, float* u2, float* u3, int align_data);
 
 /* ---- viscoelastic3d.f90 / friction3d.f90 --------------------------------- */
 /* viscouseigenstress, viscoelastic3d.f90:217-314.  maxwelltime, dgammadot0
@@ -252,6 +274,11 @@ int relax_model_set_weakzones(relax_model_t* m, int which, const relax_weak_t* z
  * read.  RELAX_ERR_ARG without a transient structure. */
 int relax_model_set_transient_strain(relax_model_t* m, const relax_tensor_t* epsilonik);
 int relax_model_get_transient_strain(relax_model_t* m, relax_tensor_t* epsilonik);
+/* The background stress (TENSOR_LAYER_STRUCT, sx3/2 depths, as
+ * tensorstructure leaves in%stressstruc, elastic3d.f90:630-702): the
+ * deviatoric prestress the viscous rheologies read, and the initial
+ * tau(:,:,k) = -prestress(k) of relax.f90:286-297, which overwrites tau --
+ * call it before the first event, as the reference's set-up does. */
 int relax_model_set_prestress(relax_model_t* m, const relax_tensor_layer_t* prestress, int n);
 /* friction planes (in%n), each with its px2*px3 patches (may be NULL) */
 int relax_model_set_planes(relax_model_t* m, const relax_plane_t* planes, int np,
@@ -284,9 +311,13 @@ int relax_model_synchronize(relax_model_t* m);
  * solve: f, t = the source() body force and traction of the shear patches
  * and the tensilesource() body force of the tensile cracks longer or wider
  * than nyquist*min(dx1,dx2,dx3), the mogisource() body force of every point
- * dilatation source, minus the surface loads at the model's time;
- * u += greenfunctioncowling(f, t); tau += the moment density of every
- * source; sig = -tau + C:eps(u).  Tensile cracks and point sources reach
+ * dilatation source, the cuboidSource() and tetrahedronSource() body
+ * forces of the eigenstrain volumes (elastic3d.f90:1680-1884, :1916-2039),
+ * minus the surface loads at the model's time; u +=
+ * greenfunctioncowling(f, t); tau += the moment density of every source
+ * (momentDensityCuboid, momentDensityTetrahedron, elastic3d.f90:2927-3029,
+ * :2788-2873, with the model's taper); sig = -tau + C:eps(u).  A
+ * tetrahedron's vertex index outside 1..nv is RELAX_ERR_ARG.  Tensile cracks and point sources reach
  * above the surface (their images), so such an event takes the full-depth
  * Green solve on temporary buffers; shear patches and loads alone take the
  * step's pruned solve.  Sources below the Nyquist size get no displacement
@@ -308,6 +339,18 @@ int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist)
  * extended stencil kernels.  NULL or no segments removes the loading.  Not
  * on a member of a batch. */
 int relax_model_set_interseismic(relax_model_t* m, const relax_event_t* ev);
+/* dislocations_disp (relax.f90:357-360, :1300-1349) after the first event:
+ * Okada's (1992) closed-form displacement (okada/dc3d.f through
+ * okada/green_space.f90) of the shear patches sc[0..ns) and tensile cracks
+ * tsc[0..nt) at or below nyquist*min(dx1,dx2,dx3) in length or width,
+ * added to u, then sig = -tau + C:eps(u).  The sources are the deck's
+ * copies (in%events(1)%sc and %tsc, input.f90:1319): the upper corner at the
+ * start of the strike, north-east-down, strike, dip and rake in degrees and
+ * the slip as the deck gives them; the reference evaluates them in the grid's
+ * frame without the deck's origin shift or rotation.  A grid point on a
+ * source's edge gets no displacement from it (dc3d.f's singular case). */
+int relax_model_okada(relax_model_t* m, const relax_source_t* sc, int ns, const relax_source_t* tsc, int nt,
+                      double nyquist);
 /* Utilities of the same set-up: add_shear adds momentdensityshear
  * (elastic3d.f90:2661-2750) of n rectangular patches (centre x,y,z, width,
  * length, strike, dip, rake in the internal convention) with slip[k] to tau

@@ -33,6 +33,7 @@ WEAK = np.dtype([("dgammadot0", "f8"), ("x", "f8"), ("y", "f8"), ("z", "f8"), ("
 # SOURCE_STRUCT, types.f90:26-29 (shear patches and surface loads of an event)
 SOURCE = np.dtype([(k, "f8") for k in ("slip", "x", "y", "z", "width", "length", "strike", "dip", "rake", "period",
                                        "phase", "beta", "opening")])
+TETRA = np.dtype([("e", "f4", (6,)), ("i1", "i4"), ("i2", "i4"), ("i3", "i4"), ("i4", "i4")])
 
 
 class OrcModel(ctypes.Structure):
@@ -52,7 +53,9 @@ class OrcModel(ctypes.Structure):
                 ("ev_ts", ctypes.c_void_p), ("ev_tsoff", ctypes.c_void_p), ("ev_mg", ctypes.c_void_p),
                 ("ev_mgoff", ctypes.c_void_p),
                 ("inter_s", ctypes.c_void_p), ("inter_ns", ctypes.c_int),
-                ("inter_ts", ctypes.c_void_p), ("inter_nt", ctypes.c_int)]
+                ("inter_ts", ctypes.c_void_p), ("inter_nt", ctypes.c_int),
+                ("ev_cb", ctypes.c_void_p), ("ev_cboff", ctypes.c_void_p), ("ev_th", ctypes.c_void_p),
+                ("ev_thoff", ctypes.c_void_p), ("ev_vx", ctypes.c_void_p), ("ev_vxoff", ctypes.c_void_p)]
 
 
 def lib():
@@ -68,6 +71,7 @@ def lib():
         assert _LIB.orc_sizeof_model() == ctypes.sizeof(OrcModel)
         assert _LIB.orc_sizeof_source() == SOURCE.itemsize
         assert _LIB.orc_sizeof_weak() == WEAK.itemsize
+        assert _LIB.orc_sizeof_tetra() == TETRA.itemsize
     return _LIB
 
 
@@ -221,21 +225,83 @@ def traction(loads, dx1, dx2, t, t3, Dt=0.0, rate=False):
 
 
 def dislocations(sources, loads, mu, nyquist, t, dx1, dx2, dx3, v, tr, tau, lam=None, beta=0.2, tensile=None,
-                 mogi=None):
+                 mogi=None, cuboids=None, tetrahedra=None, vertices=None):
     """relax.f90:1116-1289 then traction: the source() body force of the
     shear patches (SOURCE array) longer or wider than nyquist*min(dx) into
     the padded fields v (3 arrays (sx3, sx2, sx1+2)) and planes tr, every
-    patch's moment density into tau; the same for the tensile cracks
+    patch's moment density into tau; the same for the eigenstrain cuboids
+    (WEAK) and tetrahedra (TETRA with vertices (n, 3)), the tensile cracks
     (SOURCE, opening) and the point dilatation sources (SOURCE, slip = volume
     change), whose body forces fill the full depth of v; the surface loads
     into tr[2].  lam defaults to mu."""
     sx3, sx2, p1 = v[0].shape
     arr = [np.ascontiguousarray(np.zeros(0, SOURCE) if a is None else a, SOURCE)
            for a in (sources, tensile, mogi, loads)]
+    cb = np.ascontiguousarray(np.zeros(0, WEAK) if cuboids is None else cuboids, WEAK)
+    th = np.ascontiguousarray(np.zeros(0, TETRA) if tetrahedra is None else tetrahedra, TETRA)
+    vx = np.ascontiguousarray(np.zeros((0, 3)) if vertices is None else vertices, np.float64).reshape(-1, 3)
     getattr(lib(), "orc_dislocations_" + _sfx(tau))(
         _p(arr[0]), len(arr[0]), _p(arr[1]), len(arr[1]), _p(arr[2]), len(arr[2]), _p(arr[3]), len(arr[3]),
         _d(mu if lam is None else lam), _d(mu), _d(beta), _d(nyquist), _d(t), p1 - 2, sx2, sx3, _d(dx1), _d(dx2),
-        _d(dx3), _p(v[0]), _p(v[1]), _p(v[2]), _p(tr[0]), _p(tr[1]), _p(tr[2]), _p(tau))
+        _d(dx3), _p(v[0]), _p(v[1]), _p(v[2]), _p(tr[0]), _p(tr[1]), _p(tr[2]), _p(tau), _p(cb), len(cb), _p(th),
+        len(th), _p(vx), len(vx))
+
+
+def eigenstrain(cuboids, tetrahedra, vertices, lam, mu, beta, dx1, dx2, dx3, v, tau):
+    """relax.f90:1151-1178, :1192-1223: the body forces of the cuboids (WEAK
+    arrays, the eigenstrain in e) and tetrahedra (TETRA, 1-based indices into
+    vertices, (n, 3) float64) into the top half of the padded fields v (3
+    arrays (sx3, sx2, sx1+2)), then their moment densities into tau."""
+    sx3, sx2, p1 = v[0].shape
+    cb = np.ascontiguousarray(np.zeros(0, WEAK) if cuboids is None else cuboids, WEAK)
+    th = np.ascontiguousarray(np.zeros(0, TETRA) if tetrahedra is None else tetrahedra, TETRA)
+    vx = np.ascontiguousarray(np.zeros((0, 3)) if vertices is None else vertices, np.float64).reshape(-1, 3)
+    getattr(lib(), "orc_eigenstrain_" + _sfx(tau))(
+        _p(cb), len(cb), _p(th), len(th), _p(vx), len(vx), _d(lam), _d(mu), _d(beta), p1 - 2, sx2, sx3, _d(dx1),
+        _d(dx2), _d(dx3), _p(v[0]), _p(v[1]), _p(v[2]), _p(tau))
+
+
+def _e6(e):
+    return np.ascontiguousarray(e, np.float32).reshape(6)
+
+
+def cuboidsource(lam, mu, e, x, y, z, L, W, T, strike, dip, beta, dx1, dx2, dx3, f1, f2, f3):
+    """elastic3d.f90:1680-1884 on padded fields (sx3, sx2, sx1+2), top half."""
+    sx3, sx2, p1 = f1.shape
+    e = _e6(e)
+    getattr(lib(), "orc_cuboidsource_" + _sfx(f1))(
+        _d(lam), _d(mu), _p(e), _d(x), _d(y), _d(z), _d(L), _d(W), _d(T), _d(strike), _d(dip), _d(beta), p1 - 2,
+        sx2, sx3, _d(dx1), _d(dx2), _d(dx3), _p(f1), _p(f2), _p(f3))
+
+
+def momentdensitycuboid(mu, lam, e, x, y, z, L, W, T, strike, dip, beta, dx1, dx2, dx3, sig):
+    """elastic3d.f90:2927-3029, added to the tensor field sig (sx3, sx2, sx1, 6)."""
+    sx3, sx2, sx1, _ = sig.shape
+    e = _e6(e)
+    getattr(lib(), "orc_momentdensitycuboid_" + _sfx(sig))(
+        _d(mu), _d(lam), _p(e), _d(x), _d(y), _d(z), _d(L), _d(W), _d(T), _d(strike), _d(dip), _d(beta), sx1, sx2,
+        sx3, _d(dx1), _d(dx2), _d(dx3), _p(sig))
+
+
+def _v3(a):
+    return np.ascontiguousarray(a, np.float64).reshape(3)
+
+
+def tetrahedronsource(lam, mu, e, A, B, C, D, dx1, dx2, dx3, f1, f2, f3):
+    """elastic3d.f90:1916-2039 on padded fields (sx3, sx2, sx1+2), top half."""
+    sx3, sx2, p1 = f1.shape
+    e, A, B, C, D = _e6(e), _v3(A), _v3(B), _v3(C), _v3(D)
+    getattr(lib(), "orc_tetrahedronsource_" + _sfx(f1))(
+        _d(lam), _d(mu), _p(e), _p(A), _p(B), _p(C), _p(D), p1 - 2, sx2, sx3, _d(dx1), _d(dx2), _d(dx3), _p(f1),
+        _p(f2), _p(f3))
+
+
+def momentdensitytetrahedron(mu, lam, e, A, B, C, D, dx1, dx2, dx3, sig):
+    """elastic3d.f90:2788-2873, added to the tensor field sig (sx3, sx2, sx1, 6)."""
+    sx3, sx2, sx1, _ = sig.shape
+    e, A, B, C, D = _e6(e), _v3(A), _v3(B), _v3(C), _v3(D)
+    getattr(lib(), "orc_momentdensitytetrahedron_" + _sfx(sig))(
+        _d(mu), _d(lam), _p(e), _p(A), _p(B), _p(C), _p(D), sx1, sx2, sx3, _d(dx1), _d(dx2), _d(dx3), _p(sig))
 
 
 def tensilesource(lam, mu, s, x, y, z, L, W, strike, dip, beta, dx1, dx2, dx3, f1, f2, f3):
@@ -288,6 +354,63 @@ def exportpoints(u1, u2, u3, sig, dx1, dx2, dx3, pts, rot=0.0):
     return out
 
 
+# --- okada/dc3d.f, okada/green_space.f90, relax.f90:1300-1349 -------------
+def dc3d_core(alpha, x, y, z, depth, dip, al1, al2, aw1, aw2, U):
+    """DC3D's displacement in double precision (dip in degrees, z <= 0 up);
+    returns (u (3,), singular flag)."""
+    Ua = np.ascontiguousarray(U, np.float64)
+    u = np.zeros(3)
+    rc = lib().orc_dc3d_core(*[_d(v) for v in (alpha, x, y, z, depth, dip, al1, al2, aw1, aw2)], _p(Ua), _p(u))
+    return u, rc
+
+
+def dc3d(alpha, x, y, z, depth, dip, al1, al2, aw1, aw2, d1, d2, d3):
+    """DC3D with single-precision arguments and results, as dc3d.f."""
+    u = np.zeros(3, np.float32)
+    f = ctypes.c_float
+    rc = lib().orc_dc3d(*[f(v) for v in (alpha, x, y, z, depth, dip, al1, al2, aw1, aw2, d1, d2, d3)], _p(u))
+    return u, rc
+
+
+def okada_point(lam, mu, slip, opening, x, y, z, width, length, strike, dip, rake, xrec, yrec, zrec):
+    """green_space.f90:29-105: displacement (north, east, down) at one point."""
+    d = np.zeros(3)
+    lib().orc_okada_point(*[_d(v) for v in (lam, mu, slip, opening, x, y, z, width, length, strike, dip, rake, xrec,
+                                            yrec, zrec)], _p(d))
+    return d
+
+
+def grnfct_okada(lam, mu, slip, opening, x, y, z, width, length, strike, dip, rake, dx1, dx2, dx3, u1, u2, u3):
+    """green_space.f90:119-146 on padded fields (sx3, sx2, sx1+2) (added to)."""
+    sx3, sx2, p1 = u1.shape
+    getattr(lib(), "orc_grnfct_okada_" + _sfx(u1))(
+        *[_d(v) for v in (lam, mu, slip, opening, x, y, z, width, length, strike, dip, rake)], p1 - 2, sx2, sx3,
+        _d(dx1), _d(dx2), _d(dx3), _p(u1), _p(u2), _p(u3))
+
+
+def dislocations_disp(sc, tsc, lam, mu, nyquist, dx1, dx2, dx3, u1, u2, u3):
+    """relax.f90:1300-1349: Okada's displacement of the shear patches sc and
+    tensile cracks tsc (SOURCE arrays in the deck's convention) at or below
+    the nyquist size, added to the half-grid fields u (h3, sx2, sx1+2)."""
+    h3, sx2, p1 = u1.shape
+    arr = [np.ascontiguousarray(np.zeros(0, SOURCE) if a is None else a, SOURCE) for a in (sc, tsc)]
+    getattr(lib(), "orc_dislocations_disp_" + _sfx(u1))(
+        _p(arr[0]), len(arr[0]), _p(arr[1]), len(arr[1]), _d(lam), _d(mu), _d(nyquist), p1 - 2, sx2, h3, _d(dx1),
+        _d(dx2), _d(dx3), _p(u1), _p(u2), _p(u3))
+
+
+def dislocations_disp_points(sc, tsc, lam, mu, nyquist, sx1, sx2, dx1, dx2, dx3, idx):
+    """The displacement dislocations_disp adds at the grid points idx ((n, 3)
+    0-based i1, i2, i3), summed from zero in single precision: (n, 3)."""
+    idx = np.ascontiguousarray(idx, np.int32).reshape(-1, 3)
+    out = np.zeros((len(idx), 3), np.float32)
+    arr = [np.ascontiguousarray(np.zeros(0, SOURCE) if a is None else a, SOURCE) for a in (sc, tsc)]
+    lib().orc_dislocations_disp_points(_p(arr[0]), len(arr[0]), _p(arr[1]), len(arr[1]), _d(lam), _d(mu),
+                                       _d(nyquist), sx1, sx2, _d(dx1), _d(dx2), _d(dx3), _p(idx),
+                                       ctypes.c_long(len(idx)), _p(out))
+    return out
+
+
 def frictionadd(patch, dt):
     lib().orc_frictionadd(_p(patch), ctypes.c_long(patch.size), _d(dt))
 
@@ -305,7 +428,8 @@ def rk2_run(model, state, nsteps, dt_replay=None):
     the state's dtype), and the later events: event_sources, a list (one
     entry per event after the first) of SOURCE arrays, event_loads,
     event_tensile and event_mogi likewise (surface loads, tensile cracks,
-    point dilatation sources; optional), loads0 (the first event's surface
+    point dilatation sources; optional), event_cuboids (WEAK arrays),
+    event_tetrahedra (TETRA arrays) and event_vertices ((n, 3) arrays), loads0 (the first event's surface
     loads, whose rate the steps add until the next event), nyquist
     (dislocations' size gate), and the inter-seismic loading: inter and
     inter_tensile, SOURCE arrays of shear patches and tensile cracks whose
@@ -343,6 +467,22 @@ def rk2_run(model, state, nsteps, dt_replay=None):
             off[1:] = np.cumsum([len(a) for a in lists])
             setattr(m, fs, ptr(np.concatenate(lists).astype(SOURCE)))
             setattr(m, fo, ptr(off))
+    for key, fs, fo, dt in (("event_cuboids", "ev_cb", "ev_cboff", WEAK),
+                            ("event_tetrahedra", "ev_th", "ev_thoff", TETRA)):
+        evs = model.get(key) or []
+        if evs:
+            lists = [np.zeros(0, dt)] + [np.asarray(a, dt) for a in evs]
+            off = np.zeros(len(lists) + 1, np.int32)
+            off[1:] = np.cumsum([len(a) for a in lists])
+            setattr(m, fs, ptr(np.concatenate(lists).astype(dt)))
+            setattr(m, fo, ptr(off))
+    evs = model.get("event_vertices") or []
+    if evs:
+        lists = [np.zeros((0, 3))] + [np.asarray(a, np.float64).reshape(-1, 3) for a in evs]
+        off = np.zeros(len(lists) + 1, np.int32)
+        off[1:] = np.cumsum([len(a) for a in lists])
+        m.ev_vx = ptr(np.ascontiguousarray(np.concatenate(lists)))
+        m.ev_vxoff = ptr(off)
     m.nyquist = model.get("nyquist", 1.0)
     for key, fp, fn in (("inter", "inter_s", "inter_ns"), ("inter_tensile", "inter_ts", "inter_nt")):
         a = model.get(key)

@@ -935,6 +935,236 @@ static void frictionadd(orc_patch* patch, long npatch, double dt) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Okada (1992) displacement of a rectangular dislocation in a half-space:
+// DC3D of okada/dc3d.f (displacement only), restated from the paper
+// (Y. Okada, BSSA 82:1018-1040, Table 6: the A, B and C parts of the
+// strike-slip, dip-slip and tensile sources, Chinnery's sum over the four
+// corners) with dc3d.f's conventions:
+//  - arguments in single precision, arithmetic in double (dc3d.f:1-5);
+//  - dip constants: |cos(dip)| < 1e-6 is taken as 0, sin as +-1 (DCCON0,
+//    dc3d.f:509-545);
+//  - |xi|, |eta|, |q| < 1e-6 are taken as 0 (:52-67, :578-580);
+//  - a point on a fault edge (q = 0 and on an edge segment) is singular:
+//    the whole result is zero (:70-74, :160-176);
+//  - on the negative extension of an edge, where R + xi (R + eta) vanishes,
+//    ln(R + xi) is taken as -ln(R - xi) and X11 = X32 = 0 (KXI, KET,
+//    :75-86, :597-617); atan(xi eta / (q R)) is 0 for q = 0 (:591-595);
+//  - the real source contributes -A, its image A + B + z C with the vertical
+//    part of C reversed (:89-107, :133-158).
+// Checked by tests/test_oracle.py against the Navier equation, the
+// traction-free surface, the displacement jump across the fault and
+// Okada (1985)'s surface formulas.
+// ---------------------------------------------------------------------------
+namespace okada {
+static const double EPS = 1.0e-6;
+
+struct Dip { double sd, cd; };
+
+static Dip dip_constants(double dipdeg) {
+  const double p18 = 6.283185307179586 / 360.0;
+  Dip D{std::sin(dipdeg * p18), std::cos(dipdeg * p18)};
+  if (std::fabs(D.cd) < EPS) {
+    D.cd = 0.0;
+    if (D.sd > 0.0) D.sd = 1.0;
+    if (D.sd < 0.0) D.sd = -1.0;
+  }
+  return D;
+}
+
+// f[part][component] of one corner (xi, eta, q) for the dislocation u =
+// (U1, U2, U3), each part (A, B, C) already combined over the three source
+// kinds and divided by 2 pi.
+static void corner(double xi, double et, double q, double z, const Dip& D, double alpha, const double U[3],
+                   bool kxi, bool ket, bool want_bc, double fA[3], double fB[3], double fC[3]) {
+  const double sd = D.sd, cd = D.cd;
+  const double R = std::sqrt(xi * xi + et * et + q * q);
+  const double yt = et * cd + q * sd, dt = et * sd - q * cd;
+  const double th = (q == 0.0) ? 0.0 : std::atan(xi * et / (q * R));
+  double lx, X11, X32, le, Y11, Y32;
+  if (kxi) { lx = -std::log(R - xi); X11 = 0.0; X32 = 0.0; }
+  else { lx = std::log(R + xi); X11 = 1.0 / (R * (R + xi)); X32 = (2.0 * R + xi) / (R * R * R * (R + xi) * (R + xi)); }
+  if (ket) { le = -std::log(R - et); Y11 = 0.0; Y32 = 0.0; }
+  else { le = std::log(R + et); Y11 = 1.0 / (R * (R + et)); Y32 = (2.0 * R + et) / (R * R * R * (R + et) * (R + et)); }
+  const double a = alpha, b = 1.0 - alpha, k = (1.0 - alpha) / alpha;
+  const double pi2 = 6.283185307179586;
+  const double q2 = q * q;
+  // A: the infinite-medium part
+  const double A[3][3] = {
+      {th / 2 + a / 2 * xi * q * Y11, a / 2 * q / R, b / 2 * le - a / 2 * q2 * Y11},
+      {a / 2 * q / R, th / 2 + a / 2 * et * q * X11, b / 2 * lx - a / 2 * q2 * X11},
+      {-b / 2 * le - a / 2 * q2 * Y11, -b / 2 * lx - a / 2 * q2 * X11, th / 2 - a / 2 * q * (et * X11 + xi * Y11)}};
+  for (int c = 0; c < 3; ++c) fA[c] = (U[0] * A[0][c] + U[1] * A[1][c] + U[2] * A[2][c]) / pi2;
+  if (!want_bc) return;
+  const double Rd = R + dt;
+  double I3, I4;
+  if (cd != 0.0) {
+    if (xi == 0.0) {
+      I4 = 0.0;
+    } else {
+      const double X = std::sqrt(xi * xi + q2);
+      I4 = (xi / Rd * sd * cd + 2.0 * std::atan((et * (X + q * cd) + X * (R + X) * sd) / (xi * (R + X) * cd))) /
+           (cd * cd);
+    }
+    I3 = (yt * cd / Rd - le + sd * std::log(Rd)) / (cd * cd);
+  } else {
+    I3 = (et / Rd + yt * q / (Rd * Rd) - le) / 2.0;
+    I4 = xi * yt / (Rd * Rd) / 2.0;
+  }
+  const double I1 = -xi / Rd * cd - I4 * sd;
+  const double I2 = std::log(Rd) + I3 * sd;
+  const double B[3][3] = {
+      {-xi * q * Y11 - th - k * I1 * sd, -q / R + k * yt / Rd * sd, q2 * Y11 - k * I2 * sd},
+      {-q / R + k * I3 * sd * cd, -et * q * X11 - th - k * xi / Rd * sd * cd, q2 * X11 + k * I4 * sd * cd},
+      {q2 * Y11 - k * I3 * sd * sd, q2 * X11 + k * xi / Rd * sd * sd, q * (et * X11 + xi * Y11) - th - k * I4 * sd * sd}};
+  const double R3 = R * R * R;
+  const double ct = dt + z, h = q * cd - z;
+  const double Z32 = sd / R3 - h * Y32;
+  const double C[3][3] = {
+      {b * xi * Y11 * cd - a * xi * q * Z32, b * (cd / R + 2.0 * q * Y11 * sd) - a * ct * q / R3,
+       b * q * Y11 * cd - a * (ct * et / R3 - z * Y11 + xi * xi * Z32)},
+      {b * cd / R - q * Y11 * sd - a * ct * q / R3, b * yt * X11 - a * ct * et * q * X32,
+       -dt * X11 - xi * Y11 * sd - a * ct * (X11 - q2 * X32)},
+      {-b * (sd / R + q * Y11 * cd) - a * (z * Y11 - q2 * Z32),
+       b * 2.0 * xi * Y11 * sd + dt * X11 - a * ct * (X11 - q2 * X32),
+       b * (yt * X11 + xi * Y11 * cd) + a * q * (ct * et * X32 + xi * Z32)}};
+  for (int c = 0; c < 3; ++c) {
+    fB[c] = (U[0] * B[0][c] + U[1] * B[1][c] + U[2] * B[2][c]) / pi2;
+    fC[c] = (U[0] * C[0][c] + U[1] * C[1][c] + U[2] * C[2][c]) / pi2;
+  }
+}
+
+// DC3D's displacement in double precision; returns 1 (and zero
+// displacement) at a singular point, 0 otherwise.
+static int dc3d_core(double alpha, double x, double y, double z, double c, double dipdeg, double al1, double al2,
+                     double aw1, double aw2, const double U[3], double u[3]) {
+  const Dip D = dip_constants(dipdeg);
+  double xi[2] = {x - al1, x - al2};
+  for (int j = 0; j < 2; ++j)
+    if (std::fabs(xi[j]) < EPS) xi[j] = 0.0;
+  u[0] = u[1] = u[2] = 0.0;
+  for (int section = 0; section < 2; ++section) {  // 0: real source, 1: image
+    const double d = section == 0 ? c + z : c - z;
+    const double p = y * D.cd + d * D.sd;
+    double q = y * D.sd - d * D.cd;
+    double et[2] = {p - aw1, p - aw2};
+    if (std::fabs(q) < EPS) q = 0.0;
+    for (int k = 0; k < 2; ++k)
+      if (std::fabs(et[k]) < EPS) et[k] = 0.0;
+    // on a fault edge
+    if (q == 0.0 && ((xi[0] * xi[1] <= 0.0 && et[0] * et[1] == 0.0) ||
+                     (et[0] * et[1] <= 0.0 && xi[0] * xi[1] == 0.0))) {
+      u[0] = u[1] = u[2] = 0.0;
+      return 1;
+    }
+    // on the negative extension of an edge: kxi per eta corner, ket per xi
+    // corner (the tests dc3d.f makes, :78-86)
+    const double r12 = std::sqrt(xi[0] * xi[0] + et[1] * et[1] + q * q);
+    const double r21 = std::sqrt(xi[1] * xi[1] + et[0] * et[0] + q * q);
+    const double r22 = std::sqrt(xi[1] * xi[1] + et[1] * et[1] + q * q);
+    bool kxi[2] = {xi[0] < 0.0 && r21 + xi[1] < EPS, xi[0] < 0.0 && r22 + xi[1] < EPS};
+    bool ket[2] = {et[0] < 0.0 && r12 + et[1] < EPS, et[0] < 0.0 && r22 + et[1] < EPS};
+    if (MUT(33)) kxi[0] = kxi[1] = ket[0] = ket[1] = false;
+    for (int kk = 0; kk < 2; ++kk)
+      for (int j = 0; j < 2; ++j) {
+        double fA[3], fB[3], fC[3];
+        corner(xi[j], et[kk], q, z, D, alpha, U, kxi[kk], ket[j], section == 1, fA, fB, fC);
+        double du[3];
+        if (section == 0) {
+          du[0] = -fA[0];
+          du[1] = -fA[1] * D.cd + fA[2] * D.sd;
+          du[2] = -fA[1] * D.sd - fA[2] * D.cd;
+        } else {
+          const double zc = MUT(32) ? -z : z;
+          du[0] = fA[0] + fB[0] + z * fC[0];
+          du[1] = (fA[1] + fB[1] + z * fC[1]) * D.cd - (fA[2] + fB[2] + z * fC[2]) * D.sd;
+          du[2] = (fA[1] + fB[1] - zc * fC[1]) * D.sd + (fA[2] + fB[2] - zc * fC[2]) * D.cd;
+        }
+        const double sgn = (j + kk == 1) ? -1.0 : 1.0;  // Chinnery's sign
+        for (int m = 0; m < 3; ++m) u[m] += sgn * du[m];
+      }
+  }
+  return 0;
+}
+
+// DC3D's interface: single-precision arguments and results.
+static int dc3d(float ALPHA, float X, float Y, float Z, float DEPTH, float DIP, float AL1, float AL2, float AW1,
+                float AW2, float DISL1, float DISL2, float DISL3, float* UX, float* UY, float* UZ) {
+  const double U[3] = {DISL1, DISL2, DISL3};
+  double u[3];
+  const int iret = dc3d_core(ALPHA, X, Y, Z, DEPTH, DIP, AL1, AL2, AW1, AW2, U, u);
+  *UX = (float)u[0]; *UY = (float)u[1]; *UZ = (float)u[2];
+  return iret;
+}
+
+// okada, okada/green_space.f90:29-105: RELAX's fault (upper corner x, y, z
+// at the start of the strike, north-east-down, angles in degrees, deck slip
+// and rake) to DC3D's arguments and back.
+static void point(double lambda, double mu, double slip, double opening, double x1, double x2, double x3,
+                  double width, double length, double strike, double dipd, double rake, double xrec, double yrec,
+                  double zrec, double disp[3]) {
+  const double degtorad = (double)1.745329252e-2f;  // a single-precision literal (green_space.f90:47)
+  const float ALPHA = (float)((lambda + mu) / (lambda + 2.0 * mu));
+  const float Z = (float)(-zrec);
+  const double st = strike * degtorad, csst = std::cos(st), ssst = std::sin(st);
+  const double ra = rake * degtorad, csra = std::cos(ra), ssra = std::sin(ra);
+  const float X = (float)((xrec - x1) * csst + (yrec - x2) * ssst);
+  const float Yv = (float)((xrec - x1) * ssst - (yrec - x2) * csst);
+  float UX, UY, UZ;
+  const int iret = dc3d(ALPHA, X, Yv, Z, (float)x3, (float)dipd, 0.0f, (float)length, -(float)width, 0.0f,
+                        (float)(slip * csra), (float)(slip * ssra), (float)opening, &UX, &UY, &UZ);
+  if (iret == 0) {
+    disp[0] = (double)UX * csst + (double)UY * ssst;
+    disp[1] = (double)UX * ssst - (double)UY * csst;
+    disp[2] = -(double)UZ;
+  } else {
+    disp[0] = disp[1] = disp[2] = 0.0;
+  }
+}
+}  // namespace okada
+
+// grnfct_okada, okada/green_space.f90:119-146: the displacement of one
+// fault added to u1..u3 (sx3 planes from the surface, padded rows).
+template <class R>
+static void grnfct_okada(double lambda, double mu, double slip, double opening, double x, double y, double z,
+                         double width, double length, double strike, double dip, double rake, int sx1, int sx2,
+                         int sx3, double dx1, double dx2, double dx3, R* u1, R* u2, R* u3) {
+  const long pitch = sx1 + 2;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    const double x3 = (double)i3 * dx3;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        double d[3];
+        okada::point(lambda, mu, slip, opening, x, y, z, width, length, strike, dip, rake, shifted1(i1, sx1, dx1),
+                     shifted1(i2, sx2, dx2), x3, d);
+        const long o = i1 + pitch * (i2 + (long)sx2 * i3);
+        u1[o] = u1[o] + (R)d[0];
+        u2[o] = u2[o] + (R)d[1];
+        u3[o] = u3[o] + (R)d[2];
+      }
+  }
+}
+
+// dislocations_disp, relax.f90:1300-1349: the Okada displacement of the
+// shear patches and tensile cracks at or below the nyquist size, from the
+// deck's copies of the sources (sc, tsc: upper corner, degrees, deck slip);
+// the size gate reads the sources' length and width.  u is the half grid.
+template <class R>
+static void dislocations_disp(const orc_source* sc, int ns, const orc_source* tsc, int nt, double lambda, double mu,
+                              double nyquist, int sx1, int sx2, int h3, double dx1, double dx2, double dx3, R* u1,
+                              R* u2, R* u3) {
+  const double dmin = nyquist * std::min(dx1, std::min(dx2, dx3));
+  for (int k = 0; k < ns; ++k)
+    if (dmin >= sc[k].length || dmin >= sc[k].width)
+      grnfct_okada(lambda, mu, sc[k].slip, 0.0, sc[k].x, sc[k].y, sc[k].z, sc[k].width, sc[k].length, sc[k].strike,
+                   sc[k].dip, sc[k].rake, sx1, sx2, h3, dx1, dx2, dx3, u1, u2, u3);
+  for (int k = 0; k < nt; ++k)
+    if (dmin >= tsc[k].length || dmin >= tsc[k].width)
+      grnfct_okada(lambda, mu, 0.0, tsc[k].opening, tsc[k].x, tsc[k].y, tsc[k].z, tsc[k].width, tsc[k].length,
+                   tsc[k].strike, tsc[k].dip, 0.0, sx1, sx2, h3, dx1, dx2, dx3, u1, u2, u3);
+}
+
 // tensoramplitude, elastic3d.f90:446-466
 template <class R>
 static double tensoramplitude(const R* t, long npts, double dx1, double dx2, double dx3) {
@@ -1255,6 +1485,289 @@ static void momentdensitymogi(double lambda, double mu, double o, double xs, dou
 }
 
 // ---------------------------------------------------------------------------
+// Distributed eigenstrain (--with-eigenstrain): cuboids, elastic3d.f90
+// cuboidSource :1680-1884 and momentDensityCuboid :2927-3029, and
+// tetrahedra (--curvilinear), tetrahedronSource :1916-2039 and
+// momentDensityTetrahedron :2788-2873.  A cuboid is a WEAK_STRUCT (its
+// eigenstrain in e); the driver passes (width, length) as the routines'
+// (L, W) (relax.f90:1153-1162, :1192-1198).  TENSOR arithmetic (the
+// operators .times., .plus., .sdyad., elastic3d.f90:186-255) rounds to the
+// storage type R after each operation, as REAL*4 does in the reference.
+// ---------------------------------------------------------------------------
+extern "C" {
+struct orc_tetra {
+  float e[6];
+  int i1, i2, i3, i4;  // 1-based vertex indices
+};
+}
+
+template <class R>
+static T6<R> sdyad(const double a[3], const double b[3]) {
+  return T6<R>{(R)(a[0] * b[0]), (R)((a[0] * b[1] + a[1] * b[0]) / 2.0), (R)((a[0] * b[2] + a[2] * b[0]) / 2.0),
+               (R)(a[1] * b[1]), (R)((a[1] * b[2] + a[2] * b[1]) / 2.0), (R)(a[2] * b[2])};
+}
+template <class R>
+static T6<R> tscale(R c, const T6<R>& t) {
+  return T6<R>{(R)(c * t.s11), (R)(c * t.s12), (R)(c * t.s13), (R)(c * t.s22), (R)(c * t.s23), (R)(c * t.s33)};
+}
+template <class R>
+static T6<R> tplus(const T6<R>& a, const T6<R>& b) {
+  return T6<R>{(R)(a.s11 + b.s11), (R)(a.s12 + b.s12), (R)(a.s13 + b.s13),
+               (R)(a.s22 + b.s22), (R)(a.s23 + b.s23), (R)(a.s33 + b.s33)};
+}
+// the cull of both cuboid routines: SQRT(2.) is the single-precision root
+static const double SQRT2_SINGLE = (double)std::sqrt(2.0f);
+
+template <class R>
+static void cuboidsource(double lambda, double mu, const float* e, double x, double y, double z, double L, double W,
+                         double T, double strike, double dip, double beta, int sx1, int sx2, int sx3, double dx1,
+                         double dx2, double dx3, R* f1, R* f2, R* f3) {
+  const long pitch = sx1 + 2;
+  const double e11 = e[0], e12 = e[1], e13 = e[2], e22 = e[3], e23 = e[4], e33 = e[5];
+  const double ekk = e11 + e22 + e33;
+  const double cs = std::cos(strike), ss = std::sin(strike), cd = std::cos(dip), sd = std::sin(dip);
+  const double Wp = W * (1.0 + 2.0 * beta) / 2.0, Lp = L * (1.0 + 2.0 * beta) / 2.0, Tp = T * (1.0 + 2.0 * beta) / 2.0;
+  const double x2r0 = cs * x - ss * y;
+  const double xr = cd * x2r0 - sd * z, yr = ss * x + cs * y, zr = sd * x2r0 + cd * z;
+  const double lim = SQRT2_SINGLE * std::max(Wp, std::max(Lp, Tp));
+  // the surface-traction loop of cuboidSource (:1743-1775) writes nothing
+  const double le11 = lambda * ekk + 2.0 * mu * e11, le22 = lambda * ekk + 2.0 * mu * e22,
+               le33 = lambda * ekk + 2.0 * mu * e33;
+  const double m12 = 2.0 * mu * e12, m13 = 2.0 * mu * e13, m23 = 2.0 * mu * e23;
+  for (int i3 = 0; i3 < sx3 / 2; ++i3) {
+    const double x3 = (double)i3 * dx3;
+    if (std::fabs(x3 - z) > lim) continue;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        const double x1 = shifted1(i1, sx1, dx1), x2 = shifted1(i2, sx2, dx2);
+        if ((std::fabs(x1 - x) > lim) || (std::fabs(x2 - y) > lim) || (std::fabs(x3 - z) > lim)) continue;
+        const double x2r = cs * x1 - ss * x2;
+        const double x1s = cd * x2r - sd * x3, x1i = cd * x2r + sd * x3;
+        const double x2s = ss * x1 + cs * x2, x3s = sd * x2r + cd * x3, x3i = -sd * x2r + cd * x3;
+        double t1 = omega((x1s - xr) / T, beta);
+        const double t2 = omega((x2s - yr) / W, beta);
+        double t3 = omega((x3s - zr) / L, beta);
+        const double sourc = 1.0 / T * omegap((x1s - xr) / T, beta) * t2 * t3;
+        const double dblcp = 1.0 / W * t1 * omegap((x2s - yr) / W, beta) * t3;
+        const double dipcs = 1.0 / L * t1 * t2 * omegap((x3s - zr) / L, beta);
+        t1 = omega((x1i - xr) / T, beta);
+        t3 = omega((x3i + zr) / L, beta);
+        const double image = MUT(35) ? 0.0 : 1.0 / T * omegap((x1i - xr) / T, beta) * t2 * t3;
+        const double cplei = 1.0 / W * t1 * omegap((x2s - yr) / W, beta) * t3;
+        const long o = i1 + pitch * (i2 + (long)sx2 * i3);
+        if (2.01 * DEG2RAD > dip) {
+          // subvertical: method of images
+          f1[o] = (R)((double)f1[o] + m12 * ss * (sourc + image) + m12 * cd * cs * (dblcp + cplei) -
+                      le11 * ss * (dblcp + cplei) - le22 * cd * cs * (sourc + image));
+          f2[o] = (R)((double)f2[o] + m12 * cs * (sourc + image) - m12 * cd * ss * (dblcp + cplei) -
+                      le11 * cs * (dblcp + cplei) + le22 * cd * ss * (sourc + image));
+          f3[o] = (R)((double)f3[o] - m12 * sd * (dblcp - cplei) + le22 * sd * (sourc - image));
+        } else {
+          f1[o] = (R)((double)f1[o] + m12 * ss * sourc + m12 * cd * cs * dblcp - le11 * ss * dblcp -
+                      le22 * cd * cs * sourc);
+          f2[o] = (R)((double)f2[o] + m12 * cs * sourc - m12 * cd * ss * dblcp - le11 * cs * dblcp +
+                      le22 * cd * ss * sourc);
+          f3[o] = (R)((double)f3[o] - m12 * sd * dblcp + le22 * sd * sourc);
+        }
+        // vertical shear components
+        f1[o] = (R)((double)f1[o] + m13 * cd * ss * dipcs + m13 * sd * cs * dblcp - m23 * cd * cs * dipcs -
+                    m23 * sd * cs * sourc - le33 * sd * dipcs * cs);
+        f2[o] = (R)((double)f2[o] + m13 * cd * cs * dipcs - m13 * sd * ss * dblcp + m23 * cd * ss * dipcs +
+                    m23 * sd * ss * sourc + le33 * sd * dipcs * ss);
+        f3[o] = (R)((double)f3[o] + m13 * cd * dblcp - m23 * cd * sourc - le33 * cd * dipcs);
+      }
+  }
+}
+
+template <class R>
+static void momentdensitycuboid(double mu, double lambda, const float* e, double x, double y, double z, double L,
+                                double W, double T, double strike, double dip, double beta, int sx1, int sx2,
+                                int sx3, double dx1, double dx2, double dx3, R* sig) {
+  T6<R>* S = reinterpret_cast<T6<R>*>(sig);
+  const T6<R> E{(R)e[0], (R)e[1], (R)e[2], (R)e[3], (R)e[4], (R)e[5]};
+  const R lam = (R)lambda, rmu = (R)2 * (R)mu;
+  const double cs = std::cos(strike), ss = std::sin(strike), cd = std::cos(dip), sd = std::sin(dip);
+  const double Wp = W * (1.0 + 2.0 * beta) / 2.0, Lp = L * (1.0 + 2.0 * beta) / 2.0, Tp = T * (1.0 + 2.0 * beta) / 2.0;
+  const double x2r0 = cs * x - ss * y;
+  const double xr = cd * x2r0 - sd * z, yr = ss * x + cs * y, zr = sd * x2r0 + cd * z;
+  const double lim = SQRT2_SINGLE * std::max(Wp, std::max(Lp, Tp));
+  // normal, strike and dip directions
+  const double n[3] = {cd * cs, -cd * ss, -sd}, sv[3] = {-ss, MUT(36) ? cs : -cs, 0.0}, d[3] = {cs * sd, -ss * sd, cd};
+  const T6<R> SS = sdyad<R>(sv, sv), NS = sdyad<R>(n, sv), SD = sdyad<R>(sv, d), NN = sdyad<R>(n, n),
+              ND = sdyad<R>(n, d), DD = sdyad<R>(d, d);
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    const double x3 = (double)i3 * dx3;
+    if (std::fabs(x3 - z) > lim) continue;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        const double x1 = shifted1(i1, sx1, dx1), x2 = shifted1(i2, sx2, dx2);
+        if ((std::fabs(x1 - x) > lim) || (std::fabs(x2 - y) > lim) || (std::fabs(x3 - z) > lim)) continue;
+        const double x2r = cs * x1 - ss * x2;
+        const double x1s = cd * x2r - sd * x3, x2s = ss * x1 + cs * x2, x3s = sd * x2r + cd * x3;
+        // the image aperture is formed but not added (:2993-2995)
+        const R ap = (R)(omega((x1s - xr) / T, beta) * omega((x2s - yr) / W, beta) * omega((x3s - zr) / L, beta));
+        const T6<R> ea = tscale(ap, E);
+        T6<R> Ei = tscale(ea.s11, SS);
+        Ei = tplus(Ei, tscale((R)((R)2 * ea.s12), NS));
+        Ei = tplus(Ei, tscale((R)((R)2 * ea.s13), SD));
+        Ei = tplus(Ei, tscale(ea.s22, NN));
+        Ei = tplus(Ei, tscale((R)((R)2 * ea.s23), ND));
+        Ei = tplus(Ei, tscale(ea.s33, DD));
+        const R ekk = (R)((double)Ei.s11 + (double)Ei.s22 + (double)Ei.s33);
+        T6<R>& q = S[i1 + (long)sx1 * (i2 + (long)sx2 * i3)];
+        const T6<R> lk{(R)(ekk * lam), (R)0, (R)0, (R)(ekk * lam), (R)0, (R)(ekk * lam)};
+        q = tplus(tplus(q, lk), tscale(rmu, Ei));
+      }
+  }
+}
+
+// the tetrahedron's outward unit normals and face centroids; false for a
+// tetrahedron without volume (:1952-1983)
+static bool tetra_faces(const double* A, const double* B, const double* C, const double* D, double n[4][3],
+                        double c[4][3]) {
+  const double* P[4] = {A, B, C, D};
+  // face opposite vertex k: (B,C,D), (C,D,A), (D,A,B), (A,B,C)
+  for (int k = 0; k < 4; ++k) {
+    const double* p = P[(k + 1) % 4];
+    const double* q = P[(k + 2) % 4];
+    const double* r = P[(k + 3) % 4];
+    // nA=(C-B)x(D-B), nB=(D-C)x(A-C), nC=(A-D)x(B-D), nD=(B-A)x(C-A)
+    const double u[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]}, v[3] = {r[0] - p[0], r[1] - p[1], r[2] - p[2]};
+    n[k][0] = u[1] * v[2] - u[2] * v[1];
+    n[k][1] = u[2] * v[0] - u[0] * v[2];
+    n[k][2] = u[0] * v[1] - u[1] * v[0];
+    for (int j = 0; j < 3; ++j) c[k][j] = (p[j] + q[j] + r[j]) / 3.0;
+  }
+  for (int k = 0; k < 4; ++k)
+    if (0.0 == std::sqrt(n[k][0] * n[k][0] + n[k][1] * n[k][1] + n[k][2] * n[k][2])) return false;
+  for (int k = 0; k < 4; ++k) {
+    const double a = std::sqrt(n[k][0] * n[k][0] + n[k][1] * n[k][1] + n[k][2] * n[k][2]);
+    for (int j = 0; j < 3; ++j) n[k][j] = n[k][j] / a;
+  }
+  for (int k = 0; k < 4; ++k) {
+    // outward: (vertex - centroid of the opposite face) . n <= 0
+    const double* w = P[k];
+    const double dot = n[k][0] * (w[0] - c[k][0]) + n[k][1] * (w[1] - c[k][1]) + n[k][2] * (w[2] - c[k][2]);
+    if (MUT(34) ? dot < 0 : dot > 0)
+      for (int j = 0; j < 3; ++j) n[k][j] = -n[k][j];
+  }
+  return true;
+}
+static inline double gaussi(double x, double sigma) { return (1.0 + std::erf(x / std::sqrt(2.0) / sigma)) / 2.0; }
+
+template <class R>
+static void tetrahedronsource(double lambda, double mu, const float* e, const double* A, const double* B,
+                              const double* C, const double* D, int sx1, int sx2, int sx3, double dx1, double dx2,
+                              double dx3, R* f1, R* f2, R* f3) {
+  const long pitch = sx1 + 2;
+  // m = C:e (isotropicstressstrain, :129-142)
+  const double epskk = (double)e[0] + (double)e[3] + (double)e[5];
+  const R twomu = (R)(2.0 * mu);
+  T6<R> m{(R)(twomu * e[0]), (R)(twomu * e[1]), (R)(twomu * e[2]), (R)(twomu * e[3]), (R)(twomu * e[4]),
+          (R)(twomu * e[5])};
+  m.s11 = (R)(m.s11 + lambda * epskk);
+  m.s22 = (R)(m.s22 + lambda * epskk);
+  m.s33 = (R)(m.s33 + lambda * epskk);
+  double n[4][3], c[4][3];
+  if (!tetra_faces(A, B, C, D, n, c)) return;
+  double tr[4][3];
+  for (int k = 0; k < 4; ++k) {
+    tr[k][0] = m.s11 * n[k][0] + m.s12 * n[k][1] + m.s13 * n[k][2];
+    tr[k][1] = m.s12 * n[k][0] + m.s22 * n[k][1] + m.s23 * n[k][2];
+    tr[k][2] = m.s13 * n[k][0] + m.s23 * n[k][1] + m.s33 * n[k][2];
+  }
+  for (int i3 = 0; i3 < sx3 / 2; ++i3) {
+    const double x3 = (double)i3 * dx3;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        const double x[3] = {shifted1(i1, sx1, dx1), shifted1(i2, sx2, dx2), x3};
+        double dist[4];
+        bool far = true;
+        for (int k = 0; k < 4; ++k) {
+          dist[k] = (c[k][0] - x[0]) * n[k][0] + (c[k][1] - x[1]) * n[k][1] + (c[k][2] - x[2]) * n[k][2];
+          far = far && std::fabs(dist[k]) > 7.0 * dx1;
+        }
+        if (far) continue;
+        double H[4], dl[4];
+        for (int k = 0; k < 4; ++k) {
+          H[k] = gaussi(dist[k], dx1);
+          dl[k] = gauss(dist[k], dx1);
+        }
+        const double w[4] = {dl[0] * H[1] * H[2] * H[3], H[0] * dl[1] * H[2] * H[3], H[0] * H[1] * dl[2] * H[3],
+                             H[0] * H[1] * H[2] * dl[3]};
+        const long o = i1 + pitch * (i2 + (long)sx2 * i3);
+        R* f[3] = {f1, f2, f3};
+        for (int j = 0; j < 3; ++j)
+          f[j][o] = (R)((double)f[j][o] + tr[0][j] * w[0] + tr[1][j] * w[1] + tr[2][j] * w[2] + tr[3][j] * w[3]);
+      }
+  }
+}
+
+template <class R>
+static void momentdensitytetrahedron(double mu, double lambda, const float* e, const double* A, const double* B,
+                                     const double* C, const double* D, int sx1, int sx2, int sx3, double dx1,
+                                     double dx2, double dx3, R* sig) {
+  T6<R>* S = reinterpret_cast<T6<R>*>(sig);
+  const T6<R> E{(R)e[0], (R)e[1], (R)e[2], (R)e[3], (R)e[4], (R)e[5]};
+  const R lam = (R)lambda, rmu = (R)2 * (R)mu;
+  double n[4][3], c[4][3];
+  if (!tetra_faces(A, B, C, D, n, c)) return;
+  for (int i3 = 0; i3 < sx3; ++i3) {
+    const double x3 = (double)i3 * dx3;
+    for (int i2 = 0; i2 < sx2; ++i2)
+      for (int i1 = 0; i1 < sx1; ++i1) {
+        const double x[3] = {shifted1(i1, sx1, dx1), shifted1(i2, sx2, dx2), x3};
+        double dist[4];
+        bool near = true;
+        for (int k = 0; k < 4; ++k) {
+          dist[k] = (c[k][0] - x[0]) * n[k][0] + (c[k][1] - x[1]) * n[k][1] + (c[k][2] - x[2]) * n[k][2];
+          near = near && dist[k] < 7.0 * dx1;
+        }
+        // the reference skips the points within 7 dx1 of the inner side of
+        // every face (:2838-2841), as written
+        if (near) continue;
+        const R h = (R)(gaussi(dist[0], dx1) * gaussi(dist[1], dx1) * gaussi(dist[2], dx1) * gaussi(dist[3], dx1));
+        const T6<R> ea = tscale(h, E);
+        const R ekk = (R)((double)ea.s11 + (double)ea.s22 + (double)ea.s33);
+        T6<R>& q = S[i1 + (long)sx1 * (i2 + (long)sx2 * i3)];
+        const T6<R> lk{(R)(ekk * lam), (R)0, (R)0, (R)(ekk * lam), (R)0, (R)(ekk * lam)};
+        q = tplus(tplus(q, lk), tscale(rmu, ea));
+      }
+  }
+}
+
+// the eigenstrain sources of an event: body forces then moment densities
+// (relax.f90:1151-1178, :1192-1223)
+// (body forces into v1..v3 when given, moment densities into tau when given)
+template <class R>
+static void eigenstrain(const orc_weak* cb, int nc, const orc_tetra* th, int nth, const double* vtx, int nv,
+                        double lambda, double mu, double beta, int sx1, int sx2, int sx3, double dx1, double dx2,
+                        double dx3, R* v1, R* v2, R* v3, R* tau) {
+  auto V = [&](int i) { return vtx + 3 * (long)(i - 1); };
+  auto ok = [&](const orc_tetra& t) {
+    return t.i1 >= 1 && t.i1 <= nv && t.i2 >= 1 && t.i2 <= nv && t.i3 >= 1 && t.i3 <= nv && t.i4 >= 1 && t.i4 <= nv;
+  };
+  if (v1) {
+    for (int k = 0; k < nc; ++k)
+      cuboidsource(lambda, mu, cb[k].e, cb[k].x, cb[k].y, cb[k].z, cb[k].width, cb[k].length, cb[k].thickness,
+                   cb[k].strike, cb[k].dip, beta, sx1, sx2, sx3, dx1, dx2, dx3, v1, v2, v3);
+    for (int k = 0; k < nth; ++k)
+      if (ok(th[k]))
+        tetrahedronsource(lambda, mu, th[k].e, V(th[k].i1), V(th[k].i2), V(th[k].i3), V(th[k].i4), sx1, sx2, sx3,
+                          dx1, dx2, dx3, v1, v2, v3);
+  }
+  if (tau) {
+    for (int k = 0; k < nc; ++k)
+      momentdensitycuboid(mu, lambda, cb[k].e, cb[k].x, cb[k].y, cb[k].z, cb[k].width, cb[k].length,
+                          cb[k].thickness, cb[k].strike, cb[k].dip, beta, sx1, sx2, sx3 / 2, dx1, dx2, dx3, tau);
+    for (int k = 0; k < nth; ++k)
+      if (ok(th[k]))
+        momentdensitytetrahedron(mu, lambda, th[k].e, V(th[k].i1), V(th[k].i2), V(th[k].i3), V(th[k].i4), sx1, sx2,
+                                 sx3 / 2, dx1, dx2, dx3, tau);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // dislocations, relax.f90:1116-1289 (shear patches), then traction
 // (relax.f90:319, :874): the source() body force of the patches longer or
 // wider than nyquist*min(dx) into v (padded, sx3 deep) and t, the moment
@@ -1264,7 +1777,8 @@ template <class R>
 static void dislocations(const orc_source* s, int ns, const orc_source* ts, int nt, const orc_source* mg, int nm,
                          const orc_source* l, int nl, double lambda, double mu, double beta, double nyquist,
                          double t, int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, R* v1, R* v2,
-                         R* v3, R* t1, R* t2, R* t3, R* tau) {
+                         R* v3, R* t1, R* t2, R* t3, R* tau, const orc_weak* cb = nullptr, int nc = 0,
+                         const orc_tetra* th = nullptr, int nth = 0, const double* vtx = nullptr, int nv = 0) {
   const double dmin = nyquist * std::min(dx1, std::min(dx2, dx3));
   for (int k = 0; k < ns; ++k) {
     const bool big = dmin < s[k].length || dmin < s[k].width;
@@ -1272,9 +1786,14 @@ static void dislocations(const orc_source* s, int ns, const orc_source* ts, int 
       source(mu, s[k].slip, s[k].x, s[k].y, s[k].z, s[k].width, s[k].length, s[k].strike, s[k].dip, s[k].rake,
              s[k].beta, sx1, sx2, sx3, dx1, dx2, dx3, v1, v2, v3, t1, t2, t3);
   }
+  // the eigenstrain volumes' body forces follow the shear patches'
+  // (:1151-1178); their moment densities follow the patches' (:1192-1223)
+  eigenstrain(cb, nc, th, nth, vtx, nv, lambda, mu, beta, sx1, sx2, sx3, dx1, dx2, dx3, v1, v2, v3, (R*)nullptr);
   for (int k = 0; k < ns; ++k)
     momentdensityshear(mu, s[k].slip, s[k].x, s[k].y, s[k].z, s[k].width, s[k].length, s[k].strike, s[k].dip,
                        s[k].rake, s[k].beta, sx1, sx2, sx3 / 2, dx1, dx2, dx3, tau);
+  eigenstrain(cb, nc, th, nth, vtx, nv, lambda, mu, beta, sx1, sx2, sx3, dx1, dx2, dx3, (R*)nullptr, (R*)nullptr,
+              (R*)nullptr, tau);
   // tensile cracks (:1225-1254): the body force of the large ones with the
   // model's taper, every crack's moment density
   for (int k = 0; k < nt; ++k)
@@ -1346,6 +1865,14 @@ struct orc_model {
   int inter_ns;
   const orc_source* inter_ts;
   int inter_nt;
+  // eigenstrain volumes of the later events, listed as the shear patches:
+  // cuboids, tetrahedra and their vertices (rows of x1, x2, x3)
+  const orc_weak* ev_cb;
+  const int* ev_cboff;
+  const orc_tetra* ev_th;
+  const int* ev_thoff;
+  const double* ev_vx;
+  const int* ev_vxoff;
 };
 }
 
@@ -1545,7 +2072,10 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
       const int l0 = m->ev_loff ? m->ev_loff[e - 1] : 0, l1 = m->ev_loff ? m->ev_loff[e] : 0;
       const int a0 = m->ev_tsoff ? m->ev_tsoff[e - 1] : 0, a1 = m->ev_tsoff ? m->ev_tsoff[e] : 0;
       const int b0 = m->ev_mgoff ? m->ev_mgoff[e - 1] : 0, b1 = m->ev_mgoff ? m->ev_mgoff[e] : 0;
-      if (s1 > s0 || l1 > l0 || a1 > a0 || b1 > b0) {
+      const int c0 = m->ev_cboff ? m->ev_cboff[e - 1] : 0, c1 = m->ev_cboff ? m->ev_cboff[e] : 0;
+      const int h0 = m->ev_thoff ? m->ev_thoff[e - 1] : 0, h1 = m->ev_thoff ? m->ev_thoff[e] : 0;
+      const int w0 = m->ev_vxoff ? m->ev_vxoff[e - 1] : 0, w1 = m->ev_vxoff ? m->ev_vxoff[e] : 0;
+      if (s1 > s0 || l1 > l0 || a1 > a0 || b1 > b0 || c1 > c0 || h1 > h0) {
         std::fill(v1.begin(), v1.end(), (R)0); std::fill(v2.begin(), v2.end(), (R)0);
         std::fill(v3.begin(), v3.end(), (R)0); std::fill(t1.begin(), t1.end(), (R)0);
         std::fill(t2.begin(), t2.end(), (R)0); std::fill(t3.begin(), t3.end(), (R)0);
@@ -1553,7 +2083,9 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
         dislocations(m->ev_src ? m->ev_src + s0 : nullptr, s1 - s0, m->ev_ts ? m->ev_ts + a0 : nullptr, a1 - a0,
                      m->ev_mg ? m->ev_mg + b0 : nullptr, b1 - b0, m->ev_loads ? m->ev_loads + l0 : nullptr, l1 - l0,
                      m->lambda, m->mu, m->beta, m->nyquist, t, sx1, sx2, sx3, m->dx1, m->dx2, m->dx3, v1.data(),
-                     v2.data(), v3.data(), t1.data(), t2.data(), t3.data(), MUT(18) ? lost.data() : tau);
+                     v2.data(), v3.data(), t1.data(), t2.data(), t3.data(), MUT(18) ? lost.data() : tau,
+                     m->ev_cb ? m->ev_cb + c0 : nullptr, c1 - c0, m->ev_th ? m->ev_th + h0 : nullptr, h1 - h0,
+                     m->ev_vx ? m->ev_vx + 3L * w0 : nullptr, w1 - w0);
         rc = greenfunctioncowling(v1.data(), v2.data(), v3.data(), t1.data(), t2.data(), t3.data(), sx1, sx2, sx3,
                                   m->dx1, m->dx2, m->dx3, m->lambda, m->mu, m->gam);
         if (rc) return rc;
@@ -1664,9 +2196,10 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
                                             const orc_source* mg, int nm, const orc_source* l, int nl,              \
                                             double lambda, double mu, double beta, double nyquist, double t,        \
                                             int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, R* v1,   \
-                                            R* v2, R* v3, R* t1, R* t2, R* t3, R* tau) {                            \
+                                            R* v2, R* v3, R* t1, R* t2, R* t3, R* tau, const orc_weak* cb, int nc,  \
+                                            const orc_tetra* th, int nth, const double* vtx, int nv) {              \
     dislocations(s, ns, ts, nt, mg, nm, l, nl, lambda, mu, beta, nyquist, t, sx1, sx2, sx3, dx1, dx2, dx3, v1, v2,  \
-                 v3, t1, t2, t3, tau);                                                                               \
+                 v3, t1, t2, t3, tau, cb, nc, th, nth, vtx, nv);                                                     \
   }                                                                                                                  \
   ORC_EXPORT void orc_tensilesource_##SUFFIX(double lambda, double mu, double s, double x, double y, double z,       \
                                              double L, double W, double strike, double dip, double beta, int sx1,  \
@@ -1713,11 +2246,107 @@ static int rk2_run(const orc_model* m, R* u1, R* u2, R* u3, R* tau, R* sig, R* g
 ORC_BOTH(f, float)
 ORC_BOTH(d, double)
 
+// Okada (1992): DC3D's displacement (single-precision arguments, as dc3d.f)
+// and the green_space.f90 wrapper at one point
+ORC_EXPORT int orc_dc3d(float alpha, float x, float y, float z, float depth, float dip, float al1, float al2,
+                        float aw1, float aw2, float d1, float d2, float d3, float* u) {
+  return okada::dc3d(alpha, x, y, z, depth, dip, al1, al2, aw1, aw2, d1, d2, d3, u, u + 1, u + 2);
+}
+ORC_EXPORT int orc_dc3d_core(double alpha, double x, double y, double z, double depth, double dip, double al1,
+                             double al2, double aw1, double aw2, const double* U, double* u) {
+  return okada::dc3d_core(alpha, x, y, z, depth, dip, al1, al2, aw1, aw2, U, u);
+}
+ORC_EXPORT void orc_okada_point(double lambda, double mu, double slip, double opening, double x1, double x2,
+                                double x3, double width, double length, double strike, double dip, double rake,
+                                double xrec, double yrec, double zrec, double* disp) {
+  okada::point(lambda, mu, slip, opening, x1, x2, x3, width, length, strike, dip, rake, xrec, yrec, zrec, disp);
+}
+// dislocations_disp at a list of grid points only (0-based i1, i2, i3
+// triples of the half grid): out (n x 3) receives the displacement each
+// point would gain, accumulated in single precision as the grid's REAL*4
+// field does, from zero.
+ORC_EXPORT void orc_dislocations_disp_points(const orc_source* sc, int ns, const orc_source* tsc, int nt,
+                                             double lambda, double mu, double nyquist, int sx1, int sx2, double dx1,
+                                             double dx2, double dx3, const int* idx, long n, float* out) {
+  const double dmin = nyquist * std::min(dx1, std::min(dx2, dx3));
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long p = 0; p < n; ++p) {
+    const double xr = shifted1(idx[3 * p], sx1, dx1), yr = shifted1(idx[3 * p + 1], sx2, dx2);
+    const double zr = (double)idx[3 * p + 2] * dx3;
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    double d[3];
+    for (int k = 0; k < ns; ++k)
+      if (dmin >= sc[k].length || dmin >= sc[k].width) {
+        okada::point(lambda, mu, sc[k].slip, 0.0, sc[k].x, sc[k].y, sc[k].z, sc[k].width, sc[k].length,
+                     sc[k].strike, sc[k].dip, sc[k].rake, xr, yr, zr, d);
+        for (int c = 0; c < 3; ++c) acc[c] = acc[c] + (float)d[c];
+      }
+    for (int k = 0; k < nt; ++k)
+      if (dmin >= tsc[k].length || dmin >= tsc[k].width) {
+        okada::point(lambda, mu, 0.0, tsc[k].opening, tsc[k].x, tsc[k].y, tsc[k].z, tsc[k].width, tsc[k].length,
+                     tsc[k].strike, tsc[k].dip, 0.0, xr, yr, zr, d);
+        for (int c = 0; c < 3; ++c) acc[c] = acc[c] + (float)d[c];
+      }
+    for (int c = 0; c < 3; ++c) out[3 * p + c] = acc[c];
+  }
+}
+#define ORC_OKADA(SUFFIX, R)                                                                                      \
+  ORC_EXPORT void orc_grnfct_okada_##SUFFIX(double lambda, double mu, double slip, double opening, double x,      \
+                                            double y, double z, double width, double length, double strike,       \
+                                            double dip, double rake, int sx1, int sx2, int sx3, double dx1,       \
+                                            double dx2, double dx3, R* u1, R* u2, R* u3) {                        \
+    grnfct_okada(lambda, mu, slip, opening, x, y, z, width, length, strike, dip, rake, sx1, sx2, sx3, dx1, dx2,  \
+                 dx3, u1, u2, u3);                                                                               \
+  }                                                                                                              \
+  ORC_EXPORT void orc_dislocations_disp_##SUFFIX(const orc_source* sc, int ns, const orc_source* tsc, int nt,    \
+                                                 double lambda, double mu, double nyquist, int sx1, int sx2,     \
+                                                 int h3, double dx1, double dx2, double dx3, R* u1, R* u2,       \
+                                                 R* u3) {                                                        \
+    dislocations_disp(sc, ns, tsc, nt, lambda, mu, nyquist, sx1, sx2, h3, dx1, dx2, dx3, u1, u2, u3);            \
+  }
+ORC_OKADA(f, float)
+ORC_OKADA(d, double)
+
+#define ORC_EIGEN(SUFFIX, R)                                                                                      \
+  ORC_EXPORT void orc_eigenstrain_##SUFFIX(const orc_weak* cb, int nc, const orc_tetra* th, int nth,             \
+                                           const double* vtx, int nv, double lambda, double mu, double beta,     \
+                                           int sx1, int sx2, int sx3, double dx1, double dx2, double dx3, R* v1, \
+                                           R* v2, R* v3, R* tau) {                                              \
+    eigenstrain(cb, nc, th, nth, vtx, nv, lambda, mu, beta, sx1, sx2, sx3, dx1, dx2, dx3, v1, v2, v3, tau);     \
+  }                                                                                                              \
+  ORC_EXPORT void orc_cuboidsource_##SUFFIX(double lambda, double mu, const float* e, double x, double y,        \
+                                            double z, double L, double W, double T, double strike, double dip,  \
+                                            double beta, int sx1, int sx2, int sx3, double dx1, double dx2,     \
+                                            double dx3, R* f1, R* f2, R* f3) {                                  \
+    cuboidsource(lambda, mu, e, x, y, z, L, W, T, strike, dip, beta, sx1, sx2, sx3, dx1, dx2, dx3, f1, f2, f3);  \
+  }                                                                                                              \
+  ORC_EXPORT void orc_momentdensitycuboid_##SUFFIX(double mu, double lambda, const float* e, double x, double y, \
+                                                   double z, double L, double W, double T, double strike,       \
+                                                   double dip, double beta, int sx1, int sx2, int sx3,          \
+                                                   double dx1, double dx2, double dx3, R* sig) {                \
+    momentdensitycuboid(mu, lambda, e, x, y, z, L, W, T, strike, dip, beta, sx1, sx2, sx3, dx1, dx2, dx3, sig);  \
+  }                                                                                                              \
+  ORC_EXPORT void orc_tetrahedronsource_##SUFFIX(double lambda, double mu, const float* e, const double* A,      \
+                                                 const double* B, const double* C, const double* D, int sx1,    \
+                                                 int sx2, int sx3, double dx1, double dx2, double dx3, R* f1,   \
+                                                 R* f2, R* f3) {                                                \
+    tetrahedronsource(lambda, mu, e, A, B, C, D, sx1, sx2, sx3, dx1, dx2, dx3, f1, f2, f3);                      \
+  }                                                                                                              \
+  ORC_EXPORT void orc_momentdensitytetrahedron_##SUFFIX(double mu, double lambda, const float* e,                \
+                                                        const double* A, const double* B, const double* C,      \
+                                                        const double* D, int sx1, int sx2, int sx3, double dx1, \
+                                                        double dx2, double dx3, R* sig) {                       \
+    momentdensitytetrahedron(mu, lambda, e, A, B, C, D, sx1, sx2, sx3, dx1, dx2, dx3, sig);                      \
+  }
+ORC_EIGEN(f, float)
+ORC_EIGEN(d, double)
+
 ORC_EXPORT void orc_frictionadd(orc_patch* patch, long npatch, double dt) { frictionadd(patch, npatch, dt); }
 ORC_EXPORT void orc_set_mutation(int k) { g_mut = k; }
 ORC_EXPORT int orc_sizeof_patch(void) { return (int)sizeof(orc_patch); }
 ORC_EXPORT int orc_sizeof_model(void) { return (int)sizeof(orc_model); }
 ORC_EXPORT int orc_sizeof_weak(void) { return (int)sizeof(orc_weak); }
+ORC_EXPORT int orc_sizeof_tetra(void) { return (int)sizeof(orc_tetra); }
 ORC_EXPORT int orc_sizeof_source(void) { return (int)sizeof(orc_source); }
 ORC_EXPORT int orc_shiftedindex(double x1, double x2, double x3, int sx1, int sx2, int sx3, double dx1, double dx2,
                                 double dx3, int* idx) {

@@ -31,10 +31,15 @@ SOURCE = np.dtype([(k, "f8") for k in ("slip", "x", "y", "z", "width", "length",
                                        "phase", "beta", "opening")])
 
 
+TETRA = np.dtype([("e", "f4", (6,)), ("i1", "i4"), ("i2", "i4"), ("i3", "i4"), ("i4", "i4")])
+
+
 class EventDesc(ctypes.Structure):
-    """relax_event_t: the source lists of one event (SOURCE arrays)."""
+    """relax_event_t: the source lists of one event (SOURCE arrays), its
+    cuboids (WEAK), tetrahedra (TETRA) and their vertices ((n, 3) float64)."""
     _fields_ = [("s", c_void_p), ("ns", c_int), ("ts", c_void_p), ("nt", c_int), ("m", c_void_p), ("nm", c_int),
-                ("l", c_void_p), ("nl", c_int)]
+                ("l", c_void_p), ("nl", c_int), ("c", c_void_p), ("nc", c_int), ("th", c_void_p), ("nth", c_int),
+                ("v", c_void_p), ("nv", c_int)]
 
 
 class ModelDesc(ctypes.Structure):
@@ -111,6 +116,9 @@ SIGNATURES = {
     "relax_model_coseismic": (c_int, [P]),
     "relax_model_event": (c_int, [P, P, c_double]),
     "relax_model_set_interseismic": (c_int, [P, P]),
+    "relax_model_okada": (c_int, [P, P, c_int, P, c_int, c_double]),
+    "relax_grnfct_okada": (c_int, [c_double] * 12 + [c_int, c_int, c_int, c_double, c_double, c_double, P, P, P,
+                                                      c_int]),
     "relax_tensilesource": (c_int, [c_double] * 11 + [c_int, c_int, c_int, c_double, c_double, c_double] + [P] * 3
                             + [c_int]),
     "relax_mogisource": (c_int, [c_double] * 6 + [c_int, c_int, c_int, c_double, c_double, c_double] + [P] * 3

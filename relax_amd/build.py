@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librelax_hip.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["fft.hip", "green.hip", "stencil.hip", "model.hip", "compat.hip"]
+SOURCES = ["fft.hip", "green.hip", "stencil.hip", "model.hip", "compat.hip", "okada.hip", "eigen.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -412,22 +412,22 @@ int sync_status(int rc) {
 }
 
 // 3-D (nz > 1) or 2-D (nz == 1) transform of a padded device buffer
-int fftn_padded(float* d, int n1, int n2, int nz, double dx1, double dx2, double dx3, int dir) {
+int fftn_padded(float* d, int n1, int n2, int nz, double dx1, double dx2, double dx3, int dir, hipStream_t st) {
   const int M = n1 / 2;
   const long P = M + 1;  // complex row pitch
   float2* c = reinterpret_cast<float2*>(d);
   int rc;
   if (dir == -1) {
     const double sc = nz > 1 ? dx1 * dx2 * dx3 : dx1 * dx2;
-    if ((rc = launch_fft_x_r2c(n1, d, n1 + 2, c, P, c + M, P, (long)n2 * nz, (float)sc, 0))) return rc;
-    if ((rc = launch_fft_cols(n2, -1, c, P, (long)n2 * P, M + 1, nz, 0))) return rc;
-    if (nz > 1 && (rc = launch_fft_cols(nz, -1, c, (long)n2 * P, P, M + 1, n2, 0))) return rc;
+    if ((rc = launch_fft_x_r2c(n1, d, n1 + 2, c, P, c + M, P, (long)n2 * nz, (float)sc, st))) return rc;
+    if ((rc = launch_fft_cols(n2, -1, c, P, (long)n2 * P, M + 1, nz, st))) return rc;
+    if (nz > 1 && (rc = launch_fft_cols(nz, -1, c, (long)n2 * P, P, M + 1, n2, st))) return rc;
   } else {
     const double sc = nz > 1 ? 1.0 / (n1 * dx1 * n2 * dx2 * nz * dx3) : 1.0 / (n1 * dx1 * n2 * dx2);
-    if (nz > 1 && (rc = launch_fft_cols(nz, +1, c, (long)n2 * P, P, M + 1, n2, 0))) return rc;
-    if ((rc = launch_fft_cols(n2, +1, c, P, (long)n2 * P, M + 1, nz, 0))) return rc;
+    if (nz > 1 && (rc = launch_fft_cols(nz, +1, c, (long)n2 * P, P, M + 1, n2, st))) return rc;
+    if ((rc = launch_fft_cols(n2, +1, c, P, (long)n2 * P, M + 1, nz, st))) return rc;
     if ((rc = launch_fft_x_c2r(n1, EPI_PLAIN, c, P, c + M, P, d, n1 + 2, (long)n2 * nz, (float)sc, nullptr, 0,
-                               nullptr, 0)))
+                               nullptr, st)))
       return rc;
   }
   return RELAX_OK;
@@ -469,7 +469,7 @@ int relax_fft3(float* data, int sx1, int sx2, int sx3, double dx1, double dx2, d
   const bool real_in = direction == -1;
   if (real_in) { if ((rc = put_real(d.f(), data, sx1, rows, align_data))) return rc; }
   else HIP_TRY(hipMemcpy(d.p, data, (size_t)(sx1 + 2) * rows * 4, hipMemcpyHostToDevice));
-  if ((rc = sync_status(fftn_padded(d.f(), sx1, sx2, sx3, dx1, dx2, dx3, direction)))) return rc;
+  if ((rc = sync_status(fftn_padded(d.f(), sx1, sx2, sx3, dx1, dx2, dx3, direction, 0)))) return rc;
   if (real_in) HIP_TRY(hipMemcpy(data, d.p, (size_t)(sx1 + 2) * rows * 4, hipMemcpyDeviceToHost));
   else if ((rc = get_real(data, d.f(), sx1, rows, align_data))) return rc;
   return RELAX_OK;
@@ -485,7 +485,7 @@ int relax_fft2(float* data, int sx1, int sx2, double dx1, double dx2, int direct
   const bool real_in = direction == -1;
   if (real_in) { if ((rc = put_real(d.f(), data, sx1, sx2, align_data))) return rc; }
   else HIP_TRY(hipMemcpy(d.p, data, (size_t)(sx1 + 2) * sx2 * 4, hipMemcpyHostToDevice));
-  if ((rc = sync_status(fftn_padded(d.f(), sx1, sx2, 1, dx1, dx2, 1.0, direction)))) return rc;
+  if ((rc = sync_status(fftn_padded(d.f(), sx1, sx2, 1, dx1, dx2, 1.0, direction, 0)))) return rc;
   if (real_in) HIP_TRY(hipMemcpy(data, d.p, (size_t)(sx1 + 2) * sx2 * 4, hipMemcpyDeviceToHost));
   else if ((rc = get_real(data, d.f(), sx1, sx2, align_data))) return rc;
   return RELAX_OK;
@@ -565,25 +565,29 @@ int relax_cerrutimodified(const float* p1, const float* p2, const float* p3, dou
 
 // greenfunctioncowling on device buffers in the padded layout: v, three
 // fields of (sx1+2) sx2 sx3 floats over the full depth; t, three planes of
-// (sx1+2) sx2; p, three planes of scratch.  Synchronous, on the null stream.
+// (sx1+2) sx2; p, three planes of scratch.  Enqueued on stream st; with sync
+// the call waits for it and reports the kernels' status.
 int green_full_device(float* const v[3], float* const t[3], float* const p[3], int sx1, int sx2, int sx3,
-                      double dx1, double dx2, double dx3, double lambda, double mu, double gamma) {
+                      double dx1, double dx2, double dx3, double lambda, double mu, double gamma, hipStream_t st,
+                      bool sync) {
   int rc;
   const GreenConst gc = green_const(sx1, sx2, sx3, dx1, dx2, dx3, lambda, mu, gamma);
   float2* vc[3] = {(float2*)v[0], (float2*)v[1], (float2*)v[2]};
   float2* tc[3] = {(float2*)t[0], (float2*)t[1], (float2*)t[2]};
   float2* pc[3] = {(float2*)p[0], (float2*)p[1], (float2*)p[2]};
   for (int k = 0; k < 3; ++k) {
-    if ((rc = fftn_padded(v[k], sx1, sx2, sx3, dx1, dx2, dx3, -1))) return rc;
-    if ((rc = fftn_padded(t[k], sx1, sx2, 1, dx1, dx2, dx3, -1))) return rc;
+    if ((rc = fftn_padded(v[k], sx1, sx2, sx3, dx1, dx2, dx3, -1, st))) return rc;
+    if ((rc = fftn_padded(t[k], sx1, sx2, 1, dx1, dx2, dx3, -1, st))) return rc;
   }
-  if ((rc = launch_elastic_padded(gc, vc[0], vc[1], vc[2], 0))) return rc;
-  if ((rc = launch_surface_padded(gc, vc[0], vc[1], vc[2], tc[0], tc[1], tc[2], pc[0], pc[1], pc[2], 0))) return rc;
-  if ((rc = launch_cerruti_padded(gc, pc[0], pc[1], pc[2], vc[0], vc[1], vc[2], 0))) return rc;
+  if ((rc = launch_elastic_padded(gc, vc[0], vc[1], vc[2], st))) return rc;
+  if ((rc = launch_surface_padded(gc, vc[0], vc[1], vc[2], tc[0], tc[1], tc[2], pc[0], pc[1], pc[2], st))) return rc;
+  if ((rc = launch_cerruti_padded(gc, pc[0], pc[1], pc[2], vc[0], vc[1], vc[2], st))) return rc;
   for (int k = 0; k < 3; ++k) {
-    if ((rc = fftn_padded(v[k], sx1, sx2, sx3, dx1, dx2, dx3, +1))) return rc;
-    if ((rc = fftn_padded(t[k], sx1, sx2, 1, dx1, dx2, dx3, +1))) return rc;
+    if ((rc = fftn_padded(v[k], sx1, sx2, sx3, dx1, dx2, dx3, +1, st))) return rc;
+    if ((rc = fftn_padded(t[k], sx1, sx2, 1, dx1, dx2, dx3, +1, st))) return rc;
   }
+  if (!sync) return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+  HIP_TRY(hipStreamSynchronize(st));
   return sync_status(RELAX_OK);
 }
 
@@ -608,7 +612,7 @@ int relax_greenfunctioncowling(float* c1, float* c2, float* c3, float* t1, float
   float* vd[3] = {v[0].f(), v[1].f(), v[2].f()};
   float* td[3] = {t[0].f(), t[1].f(), t[2].f()};
   float* pd[3] = {p[0].f(), p[1].f(), p[2].f()};
-  if ((rc = green_full_device(vd, td, pd, sx1, sx2, sx3, dx1, dx2, dx3, lambda, mu, gamma))) return rc;
+  if ((rc = green_full_device(vd, td, pd, sx1, sx2, sx3, dx1, dx2, dx3, lambda, mu, gamma, 0, true))) return rc;
   for (int k = 0; k < 3; ++k) {
     if ((rc = get_real(hc[k], v[k].f(), sx1, rows, align_data))) return rc;
     if ((rc = get_real(ht[k], t[k].f(), sx1, sx2, align_data))) return rc;

@@ -18,28 +18,35 @@
 #include "kernels.h"
 
 // ------------------------------------------------------------- twiddles ----
+// The tables exp(-2 pi i j / L), j < L, of every power of two L = 2 ..
+// 4096, back to back (table L at offset L - 2), in one static device array.
+// They are filled once per device, at the first transform on it, so no
+// transform ever allocates device memory.  The values are evaluated in long
+// double and rounded once, so each table is correctly rounded.
+static const int TW_MAX = 4096;
+static __device__ __align__(256) float2 twiddles[2 * TW_MAX - 2];
 static std::mutex g_tw_mu;
-static std::map<std::pair<int, int>, float2*> g_tw;  // (device, L) -> table
+static std::map<int, float2*> g_tw;  // device -> address of its twiddle array
 
 const float2* twiddle_table(int L) {
+  if (L < 2 || L > TW_MAX || (L & (L - 1))) return nullptr;
   int dev = 0;
-  (void)hipGetDevice(&dev);
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   std::lock_guard<std::mutex> lk(g_tw_mu);
-  auto key = std::make_pair(dev, L);
-  auto it = g_tw.find(key);
-  if (it != g_tw.end()) return it->second;
-  std::vector<float2> h(L);
-  for (int j = 0; j < L; ++j) {
-    // exp(-2 pi i j / L), evaluated in double through the exact symmetric
-    // octant reduction of sin/cos so that the table is correctly rounded
-    long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)j / (long double)L;
-    h[j] = make_float2((float)cosl(a), (float)sinl(a));
+  auto it = g_tw.find(dev);
+  if (it == g_tw.end()) {
+    float2* base = nullptr;
+    if (hipGetSymbolAddress((void**)&base, HIP_SYMBOL(twiddles)) != hipSuccess || !base) return nullptr;
+    std::vector<float2> h(2 * TW_MAX - 2);
+    for (int n = 2; n <= TW_MAX; n *= 2)
+      for (int j = 0; j < n; ++j) {
+        const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)j / (long double)n;
+        h[n - 2 + j] = make_float2((float)cosl(a), (float)sinl(a));
+      }
+    if (hipMemcpy(base, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    it = g_tw.emplace(dev, base).first;
   }
-  float2* d = nullptr;
-  if (hipMalloc(&d, sizeof(float2) * L) != hipSuccess) return nullptr;
-  if (hipMemcpy(d, h.data(), sizeof(float2) * L, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
-  g_tw[key] = d;
-  return d;
+  return it->second + (L - 2);
 }
 
 // ---------------------------------------------------------------- x R2C ----

@@ -80,9 +80,10 @@ int launch_traction(const relax_source_t* l, int nl, int n1, int n2, double dx1,
 
 // compat.hip: greenfunctioncowling over the full depth on device buffers in
 // the padded layout (v: fields of (sx1+2) sx2 sx3, t and p: planes of
-// (sx1+2) sx2), synchronous on the null stream
+// (sx1+2) sx2), enqueued on stream st (and waited for with sync)
 int green_full_device(float* const v[3], float* const t[3], float* const p[3], int sx1, int sx2, int sx3,
-                      double dx1, double dx2, double dx3, double lambda, double mu, double gamma);
+                      double dx1, double dx2, double dx3, double lambda, double mu, double gamma, hipStream_t st,
+                      bool sync);
 
 // model.hip: tensilesource (elastic3d.f90:2315-2411) and mogisource
 // (:2458-2509) of np cracks / point sources into f (3 fields of n1 n2 n3

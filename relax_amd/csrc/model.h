@@ -54,6 +54,9 @@ struct relax_model {
   relax_source_t* loads_dev = nullptr;  // an event's surface loads, then its point sources
   long loads_cap = 0;
   int cur_nloads = 0;  // the current event's loads (loads_dev[0..)), whose rate each step adds
+  void* eig_dev = nullptr;  // an event's eigenstrain volumes (cuboid then tetrahedron records, grow-only)
+  size_t eig_cap = 0;
+  int cur_ncub = 0, cur_ntet = 0;
   std::vector<relax_plane_t> host_planes;
   std::vector<PlaneDev> plane_dev_host;
   std::vector<long> patch_off;
@@ -89,3 +92,11 @@ int launch_monitorfriction(const PlaneDev& P, double rake, int px2, int px3, con
                            const double* fric, const double* coh, relax_slippatch_t* patch, StepScalars* sc,
                            hipStream_t st);
 int launch_aos_soa(const float* aos, float* soa, long H, bool to_soa, hipStream_t st);
+// sig = -tau + C:eps(u) on the model's stream
+int model_stress_set(relax_model_t* m);
+// eigen.hip: the eigenstrain volumes of an event (cuboids, tetrahedra):
+// staged on the device, their body forces added to the top half of f (row
+// stride ld, component stride fs), their moment densities added to tau
+int eigen_stage(relax_model_t* m, const relax_event_t* ev);
+int eigen_body_force(relax_model_t* m, float* f, long ld, long fs);
+int eigen_moment(relax_model_t* m);

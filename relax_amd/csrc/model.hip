@@ -267,7 +267,8 @@ static void free_model(relax_model_t* m) {
   if (!m) return;
   void* ptrs[] = {m->u, m->sig, m->tau, m->gam, m->W, m->NQ, m->T, m->TN, m->sc, m->partials, m->tables, m->ftab,
                   m->planes_dev, m->patches_dev, m->md_dev, m->shear_dev, m->shear_slip, m->loads_dev, m->E, m->Em,
-                  m->dg[0], m->dg[1], m->dg[2], m->dg[3], m->zones_dev, m->pts_dev, m->ev_dev, m->mi};
+                  m->dg[0], m->dg[1], m->dg[2], m->dg[3], m->zones_dev, m->pts_dev, m->ev_dev, m->mi,
+                  m->eig_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->host_sc) (void)hipHostFree(m->host_sc);
@@ -663,10 +664,31 @@ int relax_model_set_structure(relax_model_t* m, int which, const relax_layer_t* 
   return RELAX_OK;
 }
 
+// tau(:,:,k) = -prestress(k) over the half grid (relax.f90:295-297); p6 holds
+// the six components of each depth
+__global__ void k_background_stress(float* __restrict__ tau, const float* __restrict__ p6, long plane, long H) {
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < H; o += (long)gridDim.x * blockDim.x) {
+    const long k = o / plane;
+    for (int c = 0; c < 6; ++c) tau[c * H + o] = -p6[6 * k + c];
+  }
+}
+
 int relax_model_set_prestress(relax_model_t* m, const relax_tensor_layer_t* p, int n) {
   if (!m || !p || n != m->g.h3) return RELAX_ERR_ARG;
   if (int rc = wait_batch(m)) return rc;
   const int h3 = m->g.h3;
+  {
+    // the initial tau: staged through the work field W (3H floats >= 6 h3)
+    std::vector<float> p6(6 * h3);
+    for (int i = 0; i < h3; ++i) {
+      const relax_tensor_t& t = p[i].t;
+      const float v[6] = {t.s11, t.s12, t.s13, t.s22, t.s23, t.s33};
+      for (int c = 0; c < 6; ++c) p6[6 * i + c] = v[c];
+    }
+    HIP_TRY(hipMemcpyAsync(m->W, p6.data(), 6L * h3 * 4, hipMemcpyHostToDevice, m->stream));
+    k_background_stress<<<grid_1d(m->g.H), 256, 0, m->stream>>>(m->tau, m->W, (long)m->g.n1 * m->g.n2, m->g.H);
+    HIP_TRY(hipGetLastError());
+  }
   std::vector<float> sp(6 * h3), taup(h3);
   for (int i = 0; i < h3; ++i) {
     const relax_tensor_t& t = p[i].t;
@@ -1550,10 +1572,13 @@ __global__ void k_add_padded_top(float* __restrict__ u, const float* __restrict_
 
 int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist) {
   if (!m || m->batch || !ev || ev->ns < 0 || ev->nt < 0 || ev->nm < 0 || ev->nl < 0 || (ev->ns > 0 && !ev->s) ||
-      (ev->nt > 0 && !ev->ts) || (ev->nm > 0 && !ev->m) || (ev->nl > 0 && !ev->l))
+      (ev->nt > 0 && !ev->ts) || (ev->nm > 0 && !ev->m) || (ev->nl > 0 && !ev->l) || ev->nc < 0 || ev->nth < 0 ||
+      ev->nv < 0 || (ev->nc > 0 && !ev->c) || (ev->nth > 0 && (!ev->th || !ev->v)))
     return RELAX_ERR_ARG;
   int rc;
   if ((rc = sync_md(m))) return rc;
+  // the eigenstrain volumes (relax.f90:1151-1178, :1192-1223)
+  if ((rc = eigen_stage(m, ev))) return rc;
   const Grid& g = m->g;
   const long H = g.H;
   const int ns = ev->ns, nt = ev->nt, nm = ev->nm, nl = ev->nl;
@@ -1612,6 +1637,7 @@ int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist)
     if (nbig && (rc = launch_source(m->shear_dev, m->shear_slip, nbig, mu, g.n1, g.n2, g.h3, dx1, dx2, dx3, m->W,
                                     g.n1, H, m->T, (long)g.n1 * g.n2, m->stream)))
       return rc;
+    if ((rc = eigen_body_force(m, m->W, g.n1, H))) return rc;
     if (nl && (rc = launch_traction(m->loads_dev, nl, g.n1, g.n2, dx1, dx2, t, 0.0, 0, m->T + 2L * g.n1 * g.n2, g.n1,
                                     m->stream)))
       return rc;
@@ -1633,6 +1659,7 @@ int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist)
     if (!rc && nbig)
       rc = launch_source(m->shear_dev, m->shear_slip, nbig, mu, g.n1, g.n2, g.h3, dx1, dx2, dx3, v[0], ld, vs, tr[0],
                          ts, m->stream);
+    if (!rc) rc = eigen_body_force(m, v[0], ld, vs);
     if (!rc && ntbig)
       rc = launch_tensile_source(m->shear_dev + t0, m->shear_slip + t0, ntbig, lam, mu, g.n1, g.n2, g.n3, dx1, dx2,
                                  dx3, v[0], ld, vs, m->stream);
@@ -1642,7 +1669,7 @@ int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist)
     if (!rc && nl)
       rc = launch_traction(m->loads_dev, nl, g.n1, g.n2, dx1, dx2, t, 0.0, 0, tr[2], ld, m->stream);
     if (!rc && hipStreamSynchronize(m->stream) != hipSuccess) rc = RELAX_ERR_HIP;
-    if (!rc) rc = green_full_device(v, tr, pp, g.n1, g.n2, g.n3, dx1, dx2, dx3, lam, mu, m->desc.gamma);
+    if (!rc) rc = green_full_device(v, tr, pp, g.n1, g.n2, g.n3, dx1, dx2, dx3, lam, mu, m->desc.gamma, m->stream, true);
     if (!rc) {
       k_add_padded_top<<<blocks_of(H), 256, 0, m->stream>>>(m->u, scratch, g.n1, g.n2, g.h3, H, vs);
       if (hipGetLastError() != hipSuccess || hipStreamSynchronize(m->stream) != hipSuccess) rc = RELAX_ERR_HIP;
@@ -1654,6 +1681,7 @@ int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist)
   // the moment densities of every source into tau, then
   // sig = -tau + C:eps(u) (relax.f90:887-889)
   if ((rc = add_shear(m, nbig, ns))) return rc;
+  if ((rc = eigen_moment(m))) return rc;
   if (nt && (rc = launch_add_tensile(m->shear_dev + t0 + ntbig, m->shear_slip + t0 + ntbig, nt, lam, mu, g.n1, g.n2,
                                      g.h3, dx1, dx2, dx3, m->tau, H, m->stream)))
     return rc;
@@ -1675,8 +1703,8 @@ int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist)
 // model then runs the extended stencil class.
 int relax_model_set_interseismic(relax_model_t* m, const relax_event_t* ev) {
   if (!m || m->batch) return RELAX_ERR_ARG;
-  if (ev && (ev->ns < 0 || ev->nt < 0 || ev->nm != 0 || ev->nl != 0 || (ev->ns > 0 && !ev->s) ||
-             (ev->nt > 0 && !ev->ts)))
+  if (ev && (ev->ns < 0 || ev->nt < 0 || ev->nm != 0 || ev->nl != 0 || ev->nc != 0 || ev->nth != 0 ||
+             (ev->ns > 0 && !ev->s) || (ev->nt > 0 && !ev->ts)))
     return RELAX_ERR_ARG;
   const int ns = ev ? ev->ns : 0, nt = ev ? ev->nt : 0;
   int rc;
@@ -1718,6 +1746,14 @@ int relax_model_set_interseismic(relax_model_t* m, const relax_event_t* ev) {
   m->inter = 1;
   m->md_dirty = true;
   return sync_md(m);
+}
+
+// sig = -tau + C:eps(u) on the model's stream (relax.f90:369-371)
+int model_stress_set(relax_model_t* m) {
+  int rc;
+  if ((rc = sync_md(m))) return rc;
+  Ctx c = single(m);
+  return launch_stress(STRESS_SET, m->g, m->S, c.md, c.one, 1, RHEO_VISCOUS, m->stream);
 }
 
 int relax_model_coseismic(relax_model_t* m) {

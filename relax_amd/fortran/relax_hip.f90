@@ -67,6 +67,11 @@ MODULE relax_hip_types
      REAL*8 :: dgammadot0,x,y,z,width,length,thickness,strike,dip
      TYPE(TENSOR) :: e
   END TYPE WEAK_STRUCT
+  TYPE TETRAHEDRON_STRUCT
+     SEQUENCE
+     TYPE(TENSOR) :: e
+     INTEGER :: i1,i2,i3,i4
+  END TYPE TETRAHEDRON_STRUCT
 END MODULE relax_hip_types
 
 MODULE relax_hip
@@ -118,6 +123,12 @@ MODULE relax_hip
      INTEGER(c_int) :: nm=0
      TYPE(c_ptr) :: l=C_NULL_PTR
      INTEGER(c_int) :: nl=0
+     TYPE(c_ptr) :: c=C_NULL_PTR
+     INTEGER(c_int) :: nc=0
+     TYPE(c_ptr) :: th=C_NULL_PTR
+     INTEGER(c_int) :: nth=0
+     TYPE(c_ptr) :: v=C_NULL_PTR
+     INTEGER(c_int) :: nv=0
   END TYPE relax_event_c
   INTEGER, PARAMETER, PUBLIC :: RELAX_LINEAR=0, RELAX_POWERLAW=1, RELAX_FAULTCREEP=2, &
        RELAX_LTRANSIENT=3, RELAX_NLTRANSIENT=4
@@ -748,13 +759,17 @@ CONTAINS
   !> a coseismic event: the first at t = 0 (relax.f90:316-371) or a later
   !! one once a step has reached its time (info%e advanced, the event switch
   !! of relax.f90:858-889): dislocations, traction, Green, stress
-  !! (shear patches s, surface loads l; optional tensile cracks ts and point
-  !! dilatation sources m)
-  SUBROUTINE relax_hip_model_event(model,s,l,nyquist,ts,m)
+  !! (shear patches s, surface loads l; optional tensile cracks ts, point
+  !! dilatation sources m, eigenstrain cuboids and tetrahedra with their
+  !! vertices, as EVENT_STRUC holds them, types.f90:99-106)
+  SUBROUTINE relax_hip_model_event(model,s,l,nyquist,ts,m,cuboid,tetrahedron,vertex)
     TYPE(relax_model), INTENT(IN) :: model
     TYPE(SOURCE_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: s,l
     REAL*8, INTENT(IN) :: nyquist
     TYPE(SOURCE_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS, OPTIONAL :: ts,m
+    TYPE(WEAK_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS, OPTIONAL :: cuboid
+    TYPE(TETRAHEDRON_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS, OPTIONAL :: tetrahedron
+    REAL*8, DIMENSION(:,:), INTENT(IN), TARGET, CONTIGUOUS, OPTIONAL :: vertex
     TYPE(relax_event_c), TARGET :: ev
     ev%ns=INT(SIZE(s),c_int)
     IF (ev%ns .GT. 0) ev%s=C_LOC(s)
@@ -767,6 +782,16 @@ CONTAINS
     IF (PRESENT(m)) THEN
        ev%nm=INT(SIZE(m),c_int)
        IF (ev%nm .GT. 0) ev%m=C_LOC(m)
+    END IF
+    IF (PRESENT(cuboid)) THEN
+       ev%nc=INT(SIZE(cuboid),c_int)
+       IF (ev%nc .GT. 0) ev%c=C_LOC(cuboid)
+    END IF
+    IF (PRESENT(tetrahedron) .AND. PRESENT(vertex)) THEN
+       ev%nth=INT(SIZE(tetrahedron),c_int)
+       IF (ev%nth .GT. 0) ev%th=C_LOC(tetrahedron)
+       ev%nv=INT(SIZE(vertex,2),c_int)
+       IF (ev%nv .GT. 0) ev%v=C_LOC(vertex)
     END IF
     CALL check(c_model_event(model%h,C_LOC(ev),nyquist),"relax_model_event")
   END SUBROUTINE relax_hip_model_event

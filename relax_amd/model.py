@@ -3,7 +3,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import LAYER, PATCH, PLANE, SOURCE, TENSOR_LAYER, WEAK, EventDesc, ModelDesc, StepInfo, check, load, ptr
+from ._lib import LAYER, PATCH, PLANE, SOURCE, TENSOR_LAYER, TETRA, WEAK, EventDesc, ModelDesc, StepInfo, check, load, ptr
 
 # mechanisms of set_structure (relax.f90:565-650): linear and power-law
 # viscosity, fault creep, linear and power-law transient creep
@@ -145,6 +145,15 @@ class Model:
         ev = EventDesc(ptr(arr[0]), len(arr[0]), ptr(arr[1]), len(arr[1]), None, 0, None, 0)
         check(self._lib.relax_model_set_interseismic(self._h, ctypes.byref(ev)), "set_interseismic")
 
+    def okada(self, sources=None, tensile=None, nyquist=1.0):
+        """dislocations_disp (relax.f90:357-360, :1300-1349): Okada's
+        displacement of the shear patches and tensile cracks at or below the
+        nyquist size added to u, then sig = -tau + C:eps(u).  SOURCE arrays
+        in the deck's convention (structure.deck_copies)."""
+        arr = [np.ascontiguousarray(np.zeros(0, SOURCE) if a is None else a, SOURCE) for a in (sources, tensile)]
+        check(self._lib.relax_model_okada(self._h, ptr(arr[0]), len(arr[0]), ptr(arr[1]), len(arr[1]),
+                                          float(nyquist)), "okada")
+
     def add_shear(self, patches, slip):
         """momentdensityshear of each patch (PLANE array, internal convention) into tau."""
         patches = np.ascontiguousarray(patches, PLANE)
@@ -155,18 +164,25 @@ class Model:
         """u = Green(body force of tau), sig = -tau + C:eps(u)."""
         check(self._lib.relax_model_coseismic(self._h), "coseismic")
 
-    def event(self, sources=None, loads=None, nyquist=1.0, tensile=None, mogi=None):
+    def event(self, sources=None, loads=None, nyquist=1.0, tensile=None, mogi=None, cuboids=None, tetrahedra=None,
+              vertices=None):
         """A coseismic event (dislocations, relax.f90:1116-1289, and traction):
         the first at t = 0 on a new model, a later one once a step has reached
         its time (the step's "e" advanced).  SOURCE arrays (or None) of shear
         patches, surface loads, tensile cracks (opening) and point dilatation
-        sources (slip = volume change).  u += Green(the body forces of the
-        sources larger than nyquist*min(dx), of every point source, and the
-        loads), tau += their moment densities, sig = -tau + C:eps(u)."""
+        sources (slip = volume change); eigenstrain cuboids (WEAK array, the
+        eigenstrain in e) and tetrahedra (TETRA array, 1-based indices into
+        vertices, (n, 3) float64).  u += Green(the body forces of the sources
+        larger than nyquist*min(dx), of every point source, of the eigenstrain
+        volumes, and the loads), tau += their moment densities,
+        sig = -tau + C:eps(u)."""
         arr = [np.ascontiguousarray(np.zeros(0, SOURCE) if a is None else a, SOURCE)
                for a in (sources, tensile, mogi, loads)]
+        cb = np.ascontiguousarray(np.zeros(0, WEAK) if cuboids is None else cuboids, WEAK)
+        th = np.ascontiguousarray(np.zeros(0, TETRA) if tetrahedra is None else tetrahedra, TETRA)
+        vx = np.ascontiguousarray(np.zeros((0, 3)) if vertices is None else vertices, np.float64).reshape(-1, 3)
         ev = EventDesc(ptr(arr[0]), len(arr[0]), ptr(arr[1]), len(arr[1]), ptr(arr[2]), len(arr[2]), ptr(arr[3]),
-                       len(arr[3]))
+                       len(arr[3]), ptr(cb), len(cb), ptr(th), len(th), ptr(vx), len(vx))
         check(self._lib.relax_model_event(self._h, ctypes.byref(ev), float(nyquist)), "event")
 
     def profile(self, nsteps):

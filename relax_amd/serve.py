@@ -1,7 +1,8 @@
 """Serving forward-model requests with continuous batching.
 
 A request is one RELAX forward model: a grid, a rheology, coseismic slip
-patches, optionally an afterslip plane, and a number of RK2 steps.  The
+patches, optionally an afterslip plane, and a number of RK2 steps -- a
+workload description, or a RELAX input deck (deck_request).  The
 server keeps every admitted model's state (u, sigma, tau, gamma, fault
 patches) on the device from its coseismic set-up to its last step, and steps
 the models of one grid class together as a batch (relax_batch_*: shared
@@ -41,7 +42,25 @@ class Request:
     arrival: float       # seconds after the start of the stream
     spec: dict           # a workload description (relax_amd.workloads format)
     nsteps: int
-    grid: int            # index into GRIDS
+    grid: int            # index into GRIDS (-1: a grid of its own, e.g. a deck's)
+
+    @property
+    def cls(self):
+        return grid_class(self.spec)
+
+
+def grid_class(s):
+    """The batch class of a spec: models step together only on the same
+    grid, spacing and elastic constants (relax_batch_add)."""
+    return (s["sx1"], s["sx2"], s["sx3"]) + workloads.spacing(s) + workloads.constants(s)[:3]
+
+
+def deck_request(rid, source, nsteps, arrival=0.0, **options):
+    """A request from a RELAX input deck (its text or a file name; options
+    are relax_amd.deck.read's section switches)."""
+    from . import deck
+    s = deck.read(source, **options).spec(name="deck %d" % rid, nsteps=nsteps)
+    return Request(rid, arrival, s, nsteps, -1)
 
 
 @dataclasses.dataclass
@@ -122,6 +141,11 @@ def _footprint(s):
     return footprint(s["sx1"], s["sx2"], s["sx3"])
 
 
+def batch_cap(cls):
+    """Members of one batch class (grid_class) at most."""
+    return max(1, MAX_BATCH_POINTS // (cls[0] * cls[1] * cls[2]))
+
+
 def run_back_to_back(requests, realtime=True):
     """Each request alone through the single-model path, in arrival order."""
     t0 = time.perf_counter()
@@ -148,11 +172,6 @@ def run_back_to_back(requests, realtime=True):
     return results
 
 
-def batch_cap(gi):
-    sx1, sx2, sx3, _ = GRIDS[gi]
-    return max(1, MAX_BATCH_POINTS // (sx1 * sx2 * sx3))
-
-
 class Server:
     """Continuous batching over grid classes (see the module docstring).
 
@@ -170,26 +189,27 @@ class Server:
             budget_bytes = int(free * (1.0 - reserve))
         self.budget = budget_bytes
         self.used = 0
-        self.batches = {}   # grid class -> Batch
-        self.members = {}   # grid class -> list of [request, model, steps taken, start]
-        self.pool = {}      # grid class -> idle models
+        self.batches = {}   # batch class (grid_class) -> Batch
+        self.members = {}   # batch class -> list of [request, model, steps taken, start]
+        self.pool = {}      # batch class -> idle models
 
     def _fits(self, r):
-        if self.pool.get(r.grid):
+        if self.pool.get(r.cls):
             return True
         need = self.used + _footprint(r.spec) - self.budget
         if need > 0:  # release idle models of other classes
-            for gi, idle in self.pool.items():
+            for cls, idle in self.pool.items():
                 while idle and need > 0:
                     m = idle.pop()
                     m.close()
-                    fp = footprint(*GRIDS[gi][:3])
+                    fp = footprint(*cls[:3])
                     self.used -= fp
                     need -= fp
         return need <= 0
 
     def _admit(self, r, now):
-        idle = self.pool.get(r.grid)
+        cls = r.cls
+        idle = self.pool.get(cls)
         if idle:
             m = idle.pop()
             m.reset()
@@ -197,12 +217,12 @@ class Server:
         else:
             m = workloads.build(r.spec)
             self.used += _footprint(r.spec)
-        b = self.batches.get(r.grid)
+        b = self.batches.get(cls)
         if b is None:
-            b = self.batches[r.grid] = Batch()
-            self.members[r.grid] = []
+            b = self.batches[cls] = Batch()
+            self.members[cls] = []
         b.add(m)
-        self.members[r.grid].append([r, m, 0, now])
+        self.members[cls].append([r, m, 0, now])
 
     def run(self, requests, realtime=True, poll=2e-4):
         pending = sorted(requests, key=lambda r: r.arrival)
@@ -230,7 +250,7 @@ class Server:
                     if k >= r.nsteps or rec["status"]:
                         b.remove(m)
                         surf = _surface(m, r.spec)
-                        self.pool.setdefault(r.grid, []).append(m)
+                        self.pool.setdefault(gi, []).append(m)
                         results.append(Result(r.rid, r.arrival if realtime else 0.0, start,
                                               time.perf_counter() - t0, k, rec["t"], rec["status"], surf))
                         worked = True
@@ -244,20 +264,21 @@ class Server:
             i = 0
             while i < len(queue) and admitted < self.admit_per_pass:
                 r = queue[i]
-                if (r.grid in self.batches and r.grid not in ready) or \
-                        len(self.members.get(r.grid, ())) >= batch_cap(r.grid):
+                cls = r.cls
+                if (cls in self.batches and cls not in ready) or \
+                        len(self.members.get(cls, ())) >= batch_cap(cls):
                     i += 1
                     continue
                 if not self._fits(r):
                     break
                 queue.pop(i)
                 self._admit(r, time.perf_counter() - t0)
-                ready.add(r.grid)
+                ready.add(cls)
                 admitted += 1
                 worked = True
             # next steps of every idle class: up to `chunk` steps at once (no
             # member past its last step), one when requests wait for the class
-            waiting = {r.grid for r in queue}
+            waiting = {r.cls for r in queue}
             for gi in ready:
                 mem = self.members[gi]
                 if mem:

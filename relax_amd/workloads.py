@@ -8,10 +8,11 @@
        strain rate growing exponentially with depth (e-fold 10 km); 20 steps
   c2   256x256x128, dx 1: rate-strengthening afterslip plane (64 x 32 km,
        64 x 32 patches, dip 20, rake 90) loaded by a 32 x 16 km coseismic
-       thrust up dip of it (slip 0-1 m), power law n = 3.5 below 30 km
-       (eta0 1e18); 30 steps.  With 3 m of slip the stress on the adjacent
-       friction plane exceeds 20 (a-b)sigma, where the reference stops
-       (friction3d.f90:200-215).
+       thrust up dip of it (slip 0-1 m; its 1 km patches are below the
+       Nyquist size, so their displacement is Okada's), power law n = 3.5
+       below 30 km (eta0 1e18); 30 steps.  With 3 m of slip the stress on
+       the adjacent friction plane exceeds 20 (a-b)sigma, where the
+       reference stops (friction3d.f90:200-215).
   c3   one of 64 batched models on 128^3, dx 1: 1-10 patches, log-uniform
        eta in [1e17, 1e20], n in {1, 2, 3, 3.5}, 15-40 km lid; 10 steps
 
@@ -77,11 +78,12 @@ def spec(name):
         ytop = -16 * math.cos(math.radians(dip))
         plane = dict(x=-32.0, y=ytop, z=ztop, length=64.0, width=32.0, strike=0.0, dip=dip, rake=90.0)
         nonlin = [(30.0, gdot0(1e18, 3.5), 3.5, 0.0, 0.6, 0.0)]
-        # nyquist 0.5: the 1 km patches on the 1 km grid are given the
-        # source() body force
+        # nyquist 2 (the default): the 1 km patches on the 1 km grid are
+        # below the Nyquist size, so they take Okada's displacement and add
+        # their moment density to tau, as the reference's set-up does
         return dict(name=name, sx1=256, sx2=256, sx3=128, dx=1.0, linear=None, nonlinear=nonlin,
                     friction=[(0.0, 1e-3, 1.0, 0.0, 0.6, 0.0)], planes=[plane], deck=deck,
-                    interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=30, nyquist=0.5)
+                    interval=1e9, odt=1.0, skip=1, tscale=1.0, nsteps=30)
     if name.startswith("c3"):
         idx = int(name[3:]) if len(name) > 3 else 0
         rng = np.random.default_rng(SEED0 + 3 + 1000 * idx)
@@ -101,6 +103,17 @@ def spec(name):
     raise KeyError(name)
 
 
+def constants(s):
+    """(lambda, mu, gamma, beta) of a spec: its own, or the module's."""
+    return (float(s.get("lam", LAM)), float(s.get("mu", MU)), float(s.get("gam", GAM)),
+            float(s.get("beta", BETA)))
+
+
+def frame(s):
+    """(x0, y0, rot) of a spec: the deck's origin and rotation (input.f90:264)."""
+    return float(s.get("x0", 0.0)), float(s.get("y0", 0.0)), float(s.get("rot", 0.0))
+
+
 def spacing(s):
     """(dx1, dx2, dx3) of a spec, whose "dx" is one spacing or three."""
     dx = s["dx"]
@@ -118,7 +131,8 @@ def tables(s):
     for key in ("linear", "nonlinear", "friction", "ltransient", "nltransient"):
         lay = s.get(key)
         out[key] = None if lay is None else structure.viscoelasticstructure(lay, h3, dx3)
-    out["prestress"] = np.zeros(h3, TENSOR_LAYER)
+    pre = s.get("prestress_layers")
+    out["prestress"] = np.zeros(h3, TENSOR_LAYER) if not pre else structure.tensorstructure(pre, h3, dx3)
     return out
 
 
@@ -129,7 +143,7 @@ WEAK_KEYS = (("weak_linear", LINEAR), ("weak_nonlinear", POWERLAW), ("weak_ltran
 
 def weak_zones(s):
     """{mechanism: WEAK array} of the spec's weak zones."""
-    return {mech: structure.weakzones(s[key]) for key, mech in WEAK_KEYS if s.get(key)}
+    return {mech: structure.weakzones(s[key], frame(s)) for key, mech in WEAK_KEYS if s.get(key)}
 
 
 def planes(s):
@@ -138,7 +152,7 @@ def planes(s):
         return None
     _, dx2, dx3 = spacing(s)
     ps = [structure.friction_plane(p["x"], p["y"], p["z"], p["length"], p["width"], p["strike"], p["dip"],
-                                   p["rake"], dx2, dx3) for p in s["planes"]]
+                                   p["rake"], dx2, dx3, frame(s)) for p in s["planes"]]
     return np.concatenate(ps)
 
 
@@ -149,12 +163,14 @@ def event_times(s):
     return (0.0,) + tuple(float(ev[0]) for ev in s.get("events", ()))
 
 
-def build(s):
-    """Create the model of a spec on the GPU with its coseismic state."""
+def build(s, okada=True):
+    """Create the model of a spec on the GPU with its coseismic state
+    (okada=False leaves out Okada's displacement of the sources below the
+    Nyquist size)."""
     dx1, dx2, dx3 = spacing(s)
-    m = Model(s["sx1"], s["sx2"], s["sx3"], dx1, dx2, dx3, LAM, MU, GAM, BETA, interval=s["interval"], odt=s["odt"],
+    m = Model(s["sx1"], s["sx2"], s["sx3"], dx1, dx2, dx3, *constants(s), interval=s["interval"], odt=s["odt"],
               tscale=s["tscale"], skip=s["skip"], event_times=event_times(s))
-    return configure(m, s)
+    return configure(m, s, okada)
 
 
 def run(m, s, nsteps, dt_replay=None):
@@ -182,26 +198,35 @@ def interseismic(s):
     dip); None without any."""
     if not (s.get("inter") or s.get("inter_tensile")):
         return None
-    return structure.interseismic(s.get("inter"), s.get("inter_tensile"), BETA, spacing(s),
-                                  s.get("nyquist", NYQUIST))
+    return structure.interseismic(s.get("inter"), s.get("inter_tensile"), constants(s)[3], spacing(s),
+                                  s.get("nyquist", NYQUIST), frame(s))
 
 
 def event_sources(s, e):
-    """The source lists of event e (1 = the set-up at t = 0) as SOURCE arrays:
-    {sources, loads, tensile, mogi} (shear patches, surface loads, tensile
-    cracks, point dilatation sources)."""
+    """The source lists of event e (1 = the set-up at t = 0): {sources,
+    loads, tensile, mogi} as SOURCE arrays (shear patches, surface loads,
+    tensile cracks, point dilatation sources) and the eigenstrain volumes
+    {cuboids (WEAK), tetrahedra (TETRA), vertices ((n, 3) float64)}.  A later
+    event of s["events"] is (time, deck[, loads[, tensile[, mogi[, cuboids[,
+    tetrahedra[, vertices]]]]]])."""
     if e == 1:
-        decks = (s["deck"], s.get("loads"), s.get("tensile"), s.get("mogi"))
+        decks = (s["deck"], s.get("loads"), s.get("tensile"), s.get("mogi"), s.get("cuboids"), s.get("tetrahedra"),
+                 s.get("vertices"))
     else:
         ev = tuple(s["events"][e - 2][1:])
-        decks = ev + (None,) * (4 - len(ev))
-    return dict(sources=structure.sources(decks[0], BETA), loads=structure.loads(decks[1], BETA),
-                tensile=structure.tensile(decks[2], BETA), mogi=structure.mogi(decks[3]))
+        decks = ev + (None,) * (7 - len(ev))
+    beta, fr = constants(s)[3], frame(s)
+    vx = np.zeros((0, 3)) if not decks[6] else np.asarray(decks[6], np.float64).reshape(-1, 3)
+    return dict(sources=structure.sources(decks[0], beta, fr), loads=structure.loads(decks[1], beta, fr[2]),
+                tensile=structure.tensile(decks[2], beta, fr), mogi=structure.mogi(decks[3], fr[2]),
+                cuboids=structure.cuboids(decks[4], fr), tetrahedra=structure.tetrahedra(decks[5]), vertices=vx)
 
 
-def configure(m, s):
-    """Set up a new (or reset) model of the spec's grid: rheology, planes,
-    coseismic state.  The time line (interval, odt, skip) is the model's."""
+def configure(m, s, okada=True):
+    """Set up a new (or reset) model of the spec's grid: rheology, the
+    prestress (its tables, and tau = -prestress before the first event,
+    relax.f90:286-297), planes, coseismic state.  The time line (interval,
+    odt, skip) is the model's."""
     t = tables(s)
     m.set_prestress(t["prestress"])
     m.set_structure(LINEAR, t["linear"])
@@ -219,6 +244,20 @@ def configure(m, s):
     inter = interseismic(s)
     if inter is not None:
         m.set_interseismic(*inter)
-    # the first event at t = 0 (relax.f90:316-371)
-    m.event(nyquist=s.get("nyquist", NYQUIST), **event_sources(s, 1))
+    # the first event at t = 0 (relax.f90:316-371), with Okada's displacement
+    # of its sources below the Nyquist size (:357-360)
+    nyq = s.get("nyquist", NYQUIST)
+    m.event(nyquist=nyq, **event_sources(s, 1))
+    if okada:
+        sc, tsc = structure.deck_copies(s["deck"], s.get("tensile"))
+        m.okada(sc, tsc, nyquist=nyq)
     return m
+
+
+def okada_sources(s):
+    """The deck copies (shear, tensile) of the spec's first event and the
+    number of them at or below the Nyquist size (Okada's solution)."""
+    sc, tsc = structure.deck_copies(s["deck"], s.get("tensile"))
+    dmin = s.get("nyquist", NYQUIST) * min(spacing(s))
+    n = sum(1 for a in (sc, tsc) for r in a if dmin >= r["length"] or dmin >= r["width"])
+    return sc, tsc, n

@@ -2,7 +2,7 @@
 import numpy as np
 
 import oracle as O
-from relax_amd import workloads
+from relax_amd import structure, workloads
 from relax_amd.model import LINEAR, LTRANSIENT, NLTRANSIENT, POWERLAW
 
 
@@ -59,8 +59,9 @@ def oracle_model(s):
     t = workloads.tables(s)
     pl = workloads.planes(s)
     dx1, dx2, dx3 = workloads.spacing(s)
-    m = dict(sx1=s["sx1"], sx2=s["sx2"], sx3=s["sx3"], dx1=dx1, dx2=dx2, dx3=dx3, lam=workloads.LAM,
-             mu=workloads.MU, gam=workloads.GAM, beta=workloads.BETA, linear=t["linear"], nonlinear=t["nonlinear"],
+    lam, mu, gam, beta = workloads.constants(s)
+    m = dict(sx1=s["sx1"], sx2=s["sx2"], sx3=s["sx3"], dx1=dx1, dx2=dx2, dx3=dx3, lam=lam,
+             mu=mu, gam=gam, beta=beta, linear=t["linear"], nonlinear=t["nonlinear"],
              faultcreep=t["friction"], prestress=t["prestress"], interval=s["interval"], odt=s["odt"],
              tscale=s["tscale"], skip=s["skip"], event_times=list(workloads.event_times(s)),
              ltransient=t["ltransient"], nltransient=t["nltransient"])
@@ -74,6 +75,9 @@ def oracle_model(s):
         for key, name in (("event_sources", "sources"), ("event_loads", "loads"), ("event_tensile", "tensile"),
                           ("event_mogi", "mogi")):
             m[key] = [d[name].astype(O.SOURCE) for d in srcs]
+        m["event_cuboids"] = [d["cuboids"].astype(O.WEAK) for d in srcs]
+        m["event_tetrahedra"] = [d["tetrahedra"].astype(O.TETRA) for d in srcs]
+        m["event_vertices"] = [d["vertices"] for d in srcs]
     h3 = s["sx3"] // 2
     keys = {LINEAR: "dg_linear", POWERLAW: "dg_nonlinear", LTRANSIENT: "dg_ltransient",
             NLTRANSIENT: "dg_nltransient"}
@@ -87,26 +91,36 @@ def oracle_model(s):
     return m
 
 
-def oracle_initial_state(s, dtype=np.float32):
-    """The first event at t = 0 (relax.f90:316-371): dislocations and traction
-    of the deck's patches and loads, u from the Green's function of their
-    body force, sig = -tau + C:eps(u) -- the set-up the model's event performs
-    on the device."""
+def oracle_initial_state(s, dtype=np.float32, okada=True):
+    """The first event at t = 0 (relax.f90:316-371) on tau = -prestress
+    (:286-297): dislocations and traction of the deck's patches and loads, u
+    from the Green's function of their body force, sig = -tau + C:eps(u) --
+    the set-up the model's event performs on the device."""
     sx1, sx2, sx3 = s["sx1"], s["sx2"], s["sx3"]
     dx1, dx2, dx3 = workloads.spacing(s)
+    lam, mu, gam, beta = workloads.constants(s)
     h3 = sx3 // 2
     tau = np.zeros((h3, sx2, sx1, 6), dtype)
+    pre = workloads.tables(s)["prestress"]["t"]
+    tau[...] = -pre[:, None, None, :]
     c = [np.zeros((sx3, sx2, sx1 + 2), dtype) for _ in range(3)]
     t = [np.zeros((sx2, sx1 + 2), dtype) for _ in range(3)]
-    ev = {k: v.astype(O.SOURCE) for k, v in workloads.event_sources(s, 1).items()}
-    O.dislocations(ev["sources"], ev["loads"], workloads.MU, s.get("nyquist", workloads.NYQUIST), 0.0, dx1, dx2, dx3, c,
-                   t, tau, lam=workloads.LAM, beta=workloads.BETA, tensile=ev["tensile"], mogi=ev["mogi"])
-    O.greenfunctioncowling(c[0], c[1], c[2], t[0], t[1], t[2], dx1, dx2, dx3, workloads.LAM, workloads.MU,
-                           workloads.GAM)
+    ev = workloads.event_sources(s, 1)
+    src = {k: ev[k].astype(O.SOURCE) for k in ("sources", "loads", "tensile", "mogi")}
+    O.dislocations(src["sources"], src["loads"], mu, s.get("nyquist", workloads.NYQUIST), 0.0, dx1, dx2, dx3, c,
+                   t, tau, lam=lam, beta=beta, tensile=src["tensile"], mogi=src["mogi"],
+                   cuboids=ev["cuboids"].astype(O.WEAK), tetrahedra=ev["tetrahedra"].astype(O.TETRA),
+                   vertices=ev["vertices"])
+    O.greenfunctioncowling(c[0], c[1], c[2], t[0], t[1], t[2], dx1, dx2, dx3, lam, mu, gam)
     u = [np.ascontiguousarray(a[:h3]) for a in c]
+    # Okada's displacement of the sources below the Nyquist size (relax.f90:357-360)
+    if okada:
+        sc, tsc = structure.deck_copies(s["deck"], s.get("tensile"))
+        O.dislocations_disp(sc.astype(O.SOURCE), tsc.astype(O.SOURCE), lam, mu,
+                            s.get("nyquist", workloads.NYQUIST), dx1, dx2, dx3, *u)
     sig = -tau
     sig = np.ascontiguousarray(sig)
-    O.stressupdate(u[0], u[1], u[2], workloads.LAM, workloads.MU, dx1, dx2, dx3, sig)
+    O.stressupdate(u[0], u[1], u[2], lam, mu, dx1, dx2, dx3, sig)
     st = dict(u1=u[0], u2=u[1], u3=u[2], tau=tau, sig=sig, gamma=np.zeros_like(u[0]), t=0.0, oi=1, i=0)
     if s.get("ltransient") or s.get("nltransient"):
         st["epsilonik"] = np.zeros((h3, sx2, sx1, 6), dtype)

@@ -515,7 +515,39 @@ def test_rk2_adaptive_step_matches_oracle(kind):
     assert rel_l2(out["sig"], ost["sig"]) < 1e-4
 
 
-@pytest.mark.parametrize("kind", ["powerlaw", "loads", "tensile"])
+OKADA_SOURCES = [
+    # slip, opening, x, y, z, width, length, strike, dip, rake (deck convention, degrees)
+    (1e-3, 0.0, -3.0, 2.0, 0.0, 4.0, 6.0, 0.0, 90.0, 0.0),        # vertical, corners on grid points
+    (2e-3, 0.0, 1.5, -2.5, 1.0, 3.0, 5.0, 37.0, 25.0, 80.0),      # shallow thrust
+    (0.0, 5e-4, -4.0, -4.0, 2.0, 2.0, 2.0, 120.0, 60.0, 0.0),     # tensile crack
+    (1e-3, 0.0, 2.0, 3.0, 1.0, 2.0, 3.0, 90.0, 89.99999, 30.0),   # cos(dip) below 1e-6
+]
+
+
+@pytest.mark.parametrize("align", [1, 0])
+def test_grnfct_okada_matches_oracle(align):
+    """grnfct_okada (green_space.f90:119-146) through the per-routine entry
+    against the oracle's restatement of dc3d.f, on a grid whose points fall on
+    fault edges and edge extensions (dc3d.f's singular cases)."""
+    from relax_amd import okada
+
+    sx1 = sx2 = 32
+    sx3 = 16
+    dx = (1.0, 1.0, 0.5)
+    for src in OKADA_SOURCES:
+        ref = [np.zeros((sx3, sx2, sx1 + 2), np.float32) for _ in range(3)]
+        O.grnfct_okada(LAM, MU, *src, *dx, *ref)
+        u = [np.zeros((sx3, sx2, sx1 + 2 if align else sx1), np.float32) for _ in range(3)]
+        okada.grnfct_okada(LAM, MU, *src, *dx, *u, align_data=align)
+        for k in range(3):
+            a, b = u[k][..., :sx1], ref[k][..., :sx1]
+            assert np.abs(b).max() > 0
+            assert rel_l2(a, b) < 1e-6, (src, k, rel_l2(a, b))
+            # the same points are singular (no displacement) on both sides
+            np.testing.assert_array_equal(a == 0, b == 0)
+
+
+@pytest.mark.parametrize("kind", ["powerlaw", "loads", "tensile", "okada"])
 def test_coseismic_setup_matches_oracle(kind):
     """The first event on the device (dislocations, traction, Green, stress)
     against the oracle's; "loads" adds a patch below the nyquist size (moment
@@ -529,6 +561,13 @@ def test_coseismic_setup_matches_oracle(kind):
     if kind == "tensile":
         s["tensile"] = [(1.5e-3, -6.0, 4.0, 1.0, 6.0, 4.0, 30.0, 70.0), (1e-3, 5.0, 5.0, 3.0, 0.9, 0.8, 0.0, 90.0)]
         s["mogi"] = [(4e-3, 8.0, -6.0, 4.0), (-2e-3, -9.0, -9.0, 1.5)]
+    if kind == "okada":
+        # a 6 x 4 km thrust of 1 km patches on the 1 km grid, below the
+        # Nyquist size: their displacement is Okada's (relax.f90:357-360)
+        rng = np.random.default_rng(5)
+        dip = 25.0
+        s["deck"] = [(rng.uniform(0, 2e-3), -3.0 + j2, -j3 * np.cos(np.radians(dip)), 1.0 + j3 * np.sin(np.radians(dip)),
+                      1.0, 1.0, 10.0, dip, 90.0) for j3 in range(4) for j2 in range(6)]
     st = oracle_initial_state(s)
     g = workloads.build(s)
     out = g.download()

@@ -1641,3 +1641,424 @@ def test_rk2_interseismic_loading_closed_form(mut, mutation):
         assert err < 1e-12
     else:
         assert err > 0.5
+
+
+# ------------------------------------------- Okada (1992), okada/dc3d.f --
+ALPHA = (LAM + MU) / (LAM + 2 * MU)
+
+
+def _okada85_surface(x, y, d, dipdeg, L, W, U, lam=LAM, mu=MU):
+    """Okada (1985, BSSA 75:1135-1154) surface displacement of a rectangular
+    fault (x along strike, z up; the fault spans 0..L along strike and 0..W
+    up dip from its reference point at depth d), a formulation independent
+    of the 1992 one the oracle restates: at z = 0 the two must agree."""
+    dl = np.radians(dipdeg)
+    sd, cd = np.sin(dl), np.cos(dl)
+    p, q = y * cd + d * sd, y * sd - d * cd
+    k = mu / (lam + mu)
+
+    def f(xi, et):
+        R = np.sqrt(xi * xi + et * et + q * q)
+        yt, dt = et * cd + q * sd, et * sd - q * cd
+        X = np.sqrt(xi * xi + q * q)
+        th = np.arctan(xi * et / (q * R))
+        I5 = k * 2 / cd * np.arctan((et * (X + q * cd) + X * (R + X) * sd) / (xi * (R + X) * cd))
+        I4 = k / cd * (np.log(R + dt) - sd * np.log(R + et))
+        I3 = k * (yt / (cd * (R + dt)) - np.log(R + et)) + sd / cd * I4
+        I2 = k * (-np.log(R + et)) - I3
+        I1 = k * (-xi / (cd * (R + dt))) - sd / cd * I5
+        ss = -np.array([xi * q / (R * (R + et)) + th + I1 * sd, yt * q / (R * (R + et)) + q * cd / (R + et) + I2 * sd,
+                        dt * q / (R * (R + et)) + q * sd / (R + et) + I4 * sd])
+        ds = -np.array([q / R - I3 * sd * cd, yt * q / (R * (R + xi)) + cd * th - I1 * sd * cd,
+                        dt * q / (R * (R + xi)) + sd * th - I5 * sd * cd])
+        tf = np.array([q * q / (R * (R + et)) - I3 * sd * sd,
+                       -dt * q / (R * (R + xi)) - sd * (xi * q / (R * (R + et)) - th) - I1 * sd * sd,
+                       yt * q / (R * (R + xi)) + cd * (xi * q / (R * (R + et)) - th) - I5 * sd * sd])
+        return (U[0] * ss + U[1] * ds + U[2] * tf) / (2 * np.pi)
+
+    return f(x, p) - f(x, p - W) - f(x - L, p) + f(x - L, p - W)
+
+
+@pytest.mark.parametrize("dip,U", [(40.0, (0.7, -0.4, 0.3)), (70.0, (0.0, 1.0, 0.0)), (25.0, (0.0, 0.0, 1.0)),
+                                   (60.0, (1.0, 0.0, 0.0)), (89.0, (0.3, 0.5, 0.0))])
+def test_dc3d_surface_is_okada_1985(dip, U):
+    for x, y in ((1.3, 2.1), (-3.0, -1.2), (6.0, 0.5), (2.0, -7.0)):
+        u, rc = O.dc3d_core(ALPHA, x, y, 0.0, 3.0, dip, 0.0, 4.0, 0.0, 2.5, U)
+        assert rc == 0
+        np.testing.assert_allclose(u, _okada85_surface(x, y, 3.0, dip, 4.0, 2.5, U), rtol=1e-10, atol=1e-14)
+
+
+def _fd_grad(fn, P, h):
+    G = np.zeros((3, 3))
+    for j in range(3):
+        e = np.zeros(3)
+        e[j] = h
+        G[:, j] = (fn(P + e) - fn(P - e)) / (2 * h)
+    return G
+
+
+@pytest.mark.parametrize("mut", [0, 32])
+@pytest.mark.parametrize("dip,U", [(40.0, (0.7, -0.4, 0.3)), (90.0, (1.0, 0.0, 0.0)), (15.0, (0.0, 0.0, 1.0)),
+                                   (70.0, (0.0, 1.0, 0.0))])
+def test_dc3d_solves_the_half_space_problem(dip, U, mut, mutation):
+    """Away from the fault the displacement satisfies Navier's equation
+    mu lap u + (lambda + mu) grad div u = 0, and the surface z = 0 is free of
+    traction (second and first finite differences of the FP64 core)."""
+    geo = (3.0, dip, -2.0, 3.0, -2.5, 0.5)
+    mutation(mut)
+
+    def fn(P):
+        return O.dc3d_core(ALPHA, P[0], P[1], P[2], *geo, U)[0]
+
+    worst = 0.0
+    for P in ((1.3, 2.1, -1.7), (-0.7, -1.9, -4.2), (4.5, 0.3, -2.6)):
+        P = np.array(P)
+        h = 2e-3
+        lap = sum((fn(P + h * e) - 2 * fn(P) + fn(P - h * e)) / h ** 2 for e in np.eye(3))
+        gd = np.array([(np.trace(_fd_grad(fn, P + h * e, h)) - np.trace(_fd_grad(fn, P - h * e, h))) / (2 * h)
+                       for e in np.eye(3)])
+        res = MU * lap + (LAM + MU) * gd
+        worst = max(worst, np.abs(res).max() / (MU * np.abs(lap).max()))
+    for x, y in ((0.4, 1.1), (-2.4, -0.6), (5.0, 3.0)):
+        G = _fd_grad(fn, np.array([x, y, 0.0]), 1e-4)
+        eps = 0.5 * (G + G.T)
+        S = LAM * np.trace(eps) * np.eye(3) + 2 * MU * eps
+        worst = max(worst, np.abs(S[:, 2]).max() / np.abs(S).max())
+    mutation(0)
+    if mut == 0:
+        assert worst < 1e-3
+    else:
+        assert worst > 1e-2
+
+
+@pytest.mark.parametrize("dip", [40.0, 90.0, 20.0])
+def test_dc3d_jump_is_the_dislocation(dip):
+    """Across the fault the displacement jumps by the dislocation: the hanging
+    wall moves U1 along the strike, U2 up the dip and U3 along the normal
+    into the hanging wall, relative to the foot wall."""
+    c = 3.0
+    sd, cd = np.sin(np.radians(dip)), np.cos(np.radians(dip))
+    P = np.array([1.0, -1.0 * cd, -c - 1.0 * sd])  # strike 1, 1 down dip from the reference point
+    to_footwall = np.array([0.0, sd, -cd])
+    e = 1e-5
+    basis = (np.array([1.0, 0.0, 0.0]), np.array([0.0, cd, sd]), np.array([0.0, -sd, cd]))
+    for k in range(3):
+        U = np.zeros(3)
+        U[k] = 1.0
+        uh, _ = O.dc3d_core(ALPHA, *(P - e * to_footwall), c, dip, -2.0, 3.0, -2.5, 0.5, U)
+        uf, _ = O.dc3d_core(ALPHA, *(P + e * to_footwall), c, dip, -2.0, 3.0, -2.5, 0.5, U)
+        np.testing.assert_allclose(uh - uf, basis[k], atol=1e-4)
+
+
+@pytest.mark.parametrize("mut", [0, 33])
+def test_dc3d_singular_points(mut, mutation):
+    """dc3d.f's special points: on a fault edge the result is flagged and
+    zero; on the negative extension of an edge (R + xi = 0) it is the limit
+    of the neighbouring values; |q| below 1e-6 counts as on the plane."""
+    geo = (3.0, 90.0, 0.0, 4.0, -2.5, 0.0)  # vertical, top edge at depth 3
+    U = (1.0, 0.0, 0.0)
+    u, rc = O.dc3d_core(ALPHA, 2.0, 0.0, -3.0, *geo, U)  # on the top edge
+    assert rc == 1 and np.all(u == 0)
+    u, rc = O.dc3d_core(ALPHA, 0.0, 0.0, -4.0, *geo, U)  # on the start edge
+    assert rc == 1 and np.all(u == 0)
+    mutation(mut)
+    on, rc = O.dc3d_core(ALPHA, -2.0, 0.0, -3.0, *geo, U)  # beyond the start, on the top edge's line
+    near = [O.dc3d_core(ALPHA, -2.0, dy, -3.0 + dz, *geo, U)[0] for dy, dz in ((1e-3, 0), (-1e-3, 0), (0, 1e-3),
+                                                                                (0, -1e-3))]
+    mutation(0)
+    avg = np.mean(near, axis=0)
+    if mut == 0:
+        assert rc == 0 and np.all(np.isfinite(on))
+        np.testing.assert_allclose(on, avg, atol=2e-3 * np.abs(avg).max())
+    else:
+        assert not (np.all(np.isfinite(on)) and np.allclose(on, avg, atol=2e-3 * np.abs(avg).max()))
+    # |q| < 1e-6 is on the plane: the same as q = 0 (inside the fault, theta = 0)
+    a, _ = O.dc3d_core(ALPHA, 2.0, 5e-7, -4.0, *geo, U)
+    b, _ = O.dc3d_core(ALPHA, 2.0, 0.0, -4.0, *geo, U)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_okada_point_frame():
+    """green_space.f90's frame change: a fault given in RELAX's frame
+    (north, east, down; upper corner at the start of the strike) at strike 0
+    is DC3D's fault with x north and y west; turning the fault and the
+    receiver by 90 degrees about the vertical turns the displacement."""
+    args = dict(lam=LAM, mu=MU, slip=1.0, opening=0.2, x=-5.0, y=1.0, z=2.0, width=4.0, length=10.0, dip=50.0,
+                rake=30.0)
+
+    def pt(strike, xr, yr, zr, **kw):
+        a = dict(args, **kw)
+        return O.okada_point(a["lam"], a["mu"], a["slip"], a["opening"], a["x"], a["y"], a["z"], a["width"],
+                             a["length"], strike, a["dip"], a["rake"], xr, yr, zr)
+
+    d = pt(0.0, 1.5, -2.0, 1.0)
+    r = np.radians(30.0)
+    U = (np.float32(np.cos(r)), np.float32(np.sin(r)), np.float32(0.2))
+    u, rc = O.dc3d(ALPHA, 1.5 + 5.0, -(-2.0 - 1.0), -1.0, 2.0, 50.0, 0.0, 10.0, -4.0, 0.0, *U)
+    np.testing.assert_allclose(d, [u[0], -u[1], -u[2]], rtol=2e-6, atol=1e-9)
+    # strike 90: the fault runs east from (x, y); rotate the receiver with it
+    d90 = pt(90.0, -5.0 + 2.0, 1.0 + 6.5, 1.0)
+    d0 = pt(0.0, -5.0 + 6.5, 1.0 - 2.0, 1.0)
+    np.testing.assert_allclose([d90[1], -d90[0], d90[2]], d0, rtol=1e-5, atol=1e-7)
+
+
+def test_dislocations_disp_takes_the_small_sources_only():
+    """relax.f90:1300-1349: only the sources at or below nyquist*min(dx) add
+    Okada's displacement, each once, in order."""
+    sx, h3, dx = 16, 8, 1.0
+    from relax_amd import structure  # noqa: F401  (deck conventions are the test's input format)
+    small = np.zeros(1, O.SOURCE)
+    small[0] = (1e-3, -1.0, 0.5, 1.0, 1.0, 1.5, 30.0, 60.0, 20.0, 0, 0, 0.2, 0)
+    big = np.zeros(1, O.SOURCE)
+    big[0] = (1e-3, -3.0, 0.5, 1.0, 3.0, 6.0, 30.0, 60.0, 20.0, 0, 0, 0.2, 0)
+    tens = np.zeros(1, O.SOURCE)
+    tens[0] = (0, 2.0, -2.0, 2.0, 1.0, 1.0, 0.0, 90.0, 0.0, 0, 0, 0.2, 5e-4)
+    u = [np.zeros((h3, sx, sx + 2)) for _ in range(3)]
+    O.dislocations_disp(np.concatenate([small, big]), tens, LAM, MU, 2.0, dx, dx, dx, *u)
+    ref = [np.zeros((h3, sx, sx + 2)) for _ in range(3)]
+    s = small[0]
+    O.grnfct_okada(LAM, MU, s["slip"], 0.0, s["x"], s["y"], s["z"], s["width"], s["length"], s["strike"], s["dip"],
+                   s["rake"], dx, dx, dx, *ref)
+    t = tens[0]
+    O.grnfct_okada(LAM, MU, 0.0, t["opening"], t["x"], t["y"], t["z"], t["width"], t["length"], t["strike"],
+                   t["dip"], 0.0, dx, dx, dx, *ref)
+    for a, b in zip(u, ref):
+        assert np.abs(b).max() > 0
+        np.testing.assert_array_equal(a, b)
+
+
+# ------------------------------------------------ eigenstrain volumes -----
+# cuboidSource / momentDensityCuboid (elastic3d.f90:1680-1884, :2927-3029)
+# and tetrahedronSource / momentDensityTetrahedron (:1916-2039, :2788-2873).
+EIG_SX, EIG_H3, EIG_DX = 128, 64, 0.25
+
+
+def _spectral_grad(a, dx=EIG_DX):
+    """Spectral gradient (d/dx1, d/dx2, d/dx3) of a (h3, sx2, sx1) field that
+    vanishes towards every edge."""
+    h3, s2, s1 = a.shape
+    F = np.fft.fftn(a)
+    ks = (2j * np.pi * np.fft.fftfreq(s1, dx)[None, None, :], 2j * np.pi * np.fft.fftfreq(s2, dx)[None, :, None],
+          2j * np.pi * np.fft.fftfreq(h3, dx)[:, None, None])
+    return [np.real(np.fft.ifftn(k * F)) for k in ks]
+
+
+_T6 = {(0, 0): 0, (0, 1): 1, (0, 2): 2, (1, 1): 3, (1, 2): 4, (2, 2): 5}
+
+
+def _minus_div(m):
+    """-div of a symmetric tensor field (h3, sx2, sx1, 6), spectrally."""
+    out = np.zeros(m.shape[:3] + (3,))
+    for i in range(3):
+        for j in range(3):
+            out[..., i] -= _spectral_grad(m[..., _T6[(min(i, j), max(i, j))]])[j]
+    return out
+
+
+def _hooke(E):
+    """lambda tr(E) I + 2 mu E of a 3x3 strain."""
+    return LAM * np.trace(E) * np.eye(3) + 2 * MU * E
+
+
+def _t6(M):
+    return np.array([M[0, 0], M[0, 1], M[0, 2], M[1, 1], M[1, 2], M[2, 2]])
+
+
+def _cuboid(e, strike, dip, z=5.0, beta=0.3, want=("f", "m")):
+    """A 4 x 3 x 2 km cuboid (deck length 4, width 3, thickness 2) with its
+    upper corner at (-2, -1, z): its body force (top half) and moment
+    density, on a 128 x 128 x 64 grid of 0.25 km."""
+    from relax_amd import structure
+    _, x, y, zc, st, dp, _ = structure.wangconvention(0.0, -2.0, -1.0, z, 4.0, 3.0, strike, dip, 0.0)
+    out = {"geom": (x, y, zc, st, dp)}
+    if "f" in want:
+        v = [np.zeros((2 * EIG_H3, EIG_SX, EIG_SX + 2)) for _ in range(3)]
+        # the driver passes (width, length) = (3, 4) as (L, W)
+        O.cuboidsource(LAM, MU, e, x, y, zc, 3.0, 4.0, 2.0, st, dp, beta, EIG_DX, EIG_DX, EIG_DX, *v)
+        out["f"] = np.stack([a[:EIG_H3, :, :EIG_SX] for a in v], -1)
+    if "m" in want:
+        m = np.zeros((EIG_H3, EIG_SX, EIG_SX, 6))
+        O.momentdensitycuboid(MU, LAM, e, x, y, zc, 3.0, 4.0, 2.0, st, dp, beta, EIG_DX, EIG_DX, EIG_DX, m)
+        out["m"] = m
+    return out
+
+
+@pytest.mark.parametrize("mut", [0, 36])
+def test_cuboid_moment_density_is_hooke_of_the_rotated_eigenstrain(mut, mutation):
+    """Inside the cuboid the moment density is C:E with E the eigenstrain in
+    the (strike, normal, dip) frame, and its integral is the volume times
+    that.  A vertical cuboid of strike 0 has the frame (x1, x2, x3), so
+    E = e; with strike 90 the strike runs along x2, the normal along -x1:
+    E = [[e22, -e12, -e23], [-e12, e11, e13], [-e23, e13, e33]]."""
+    mutation(mut)
+    rng = np.random.default_rng(5)
+    e = rng.uniform(-1e-3, 1e-3, 6)
+    e11, e12, e13, e22, e23, e33 = np.float32(e).astype(np.float64)
+    ok = True
+    for strike, E in ((0.0, np.array([[e11, e12, e13], [e12, e22, e23], [e13, e23, e33]])),
+                      (90.0, np.array([[e22, -e12, -e23], [-e12, e11, e13], [-e23, e13, e33]]))):
+        c = _cuboid(e, strike, 90.0, want=("m",))
+        x, y, zc = c["geom"][:3]
+        i1, i2, i3 = int(round(x / EIG_DX)) % EIG_SX, int(round(y / EIG_DX)) % EIG_SX, int(round(zc / EIG_DX))
+        want = _t6(_hooke(E))
+        ok &= np.allclose(c["m"][i3, i2, i1], want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
+        ok &= np.allclose(c["m"].sum((0, 1, 2)) * EIG_DX ** 3, 24.0 * want, rtol=1e-3, atol=1e-3 * np.abs(want).max())
+    assert ok == (mut == 0)
+
+
+def _deviation(e, geom, m, k):
+    """The body force's departure from -div(m) in the reference's dipping
+    terms of e13 and e23 (elastic3d.f90:1851-1872): the e13 dip derivatives
+    carry an extra cos(dip), and f3 lacks 2 mu e23 sin(dip) d.grad(A), where
+    A is the cuboid's aperture (read off the moment density)."""
+    x, y, zc, st, dp = geom
+    cs, ss, cd, sd = np.cos(st), np.sin(st), np.cos(dp), np.sin(dp)
+    s = np.array([-ss, -cs, 0.0])
+    n = np.array([cd * cs, -cd * ss, -sd])
+    d = np.array([cs * sd, -ss * sd, cd])
+    a, b = (s, d) if k == 2 else (n, d)
+    A = m[..., 2] / (2 * MU * e[k] * (a[0] * b[2] + a[2] * b[0]))
+    g = _spectral_grad(A)
+    dipcs = d[0] * g[0] + d[1] * g[1] + d[2] * g[2]
+    dev = np.zeros(m.shape[:3] + (3,))
+    if k == 2:
+        dev[..., 0] = 2 * MU * e[k] * (cd - 1) * ss * dipcs
+        dev[..., 1] = 2 * MU * e[k] * (cd - 1) * cs * dipcs
+    else:
+        dev[..., 2] = -2 * MU * e[k] * sd * dipcs
+    return dev
+
+
+@pytest.mark.parametrize("strike,dip", [(0.0, 90.0), (30.0, 90.0), (30.0, 60.0)])
+def test_cuboid_body_force_is_minus_the_divergence_of_its_moment(strike, dip):
+    """A buried cuboid's body force is -div of its moment density (spectral
+    derivative, relative L2 within 12 % per component on this grid, the
+    discretisation of the tapers), for each eigenstrain component -- except
+    that for a dipping cuboid the e13 and e23 parts follow the reference's
+    formula, which departs from -div(m) by the terms of _deviation (with
+    them the agreement is again within 12 %, without them it is not)."""
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = 1e-3
+        c = _cuboid(e, strike, dip)
+        f, want = c["f"], _minus_div(c["m"])
+        dev = _deviation(e, c["geom"], c["m"], k) if (dip != 90.0 and k in (2, 4)) else 0.0
+        for i in range(3):
+            nf = np.linalg.norm(f[..., i])
+            if nf < 1e-9 * np.abs(f).max() * f[..., i].size:
+                continue
+            err = np.linalg.norm(f[..., i] - want[..., i] - (dev[..., i] if np.ndim(dev) else 0.0)) / nf
+            assert err < 0.12, (k, i, err)
+            if np.ndim(dev) and np.abs(dev[..., i]).max() > 0:
+                assert np.linalg.norm(f[..., i] - want[..., i]) / nf > 0.12
+
+
+@pytest.mark.parametrize("mut", [0, 35])
+def test_cuboid_image_mirrors_a_subvertical_source(mut, mutation):
+    """A subvertical cuboid's body force carries the image above the surface
+    of its horizontal-shear part (e11, e12, e22; elastic3d.f90:1814-1831;
+    the vertical-shear part, which includes lambda tr(e), has none): with
+    e13 = e23 = e33 = tr(e) = 0, f1 and f2 of a cuboid centred at depth z
+    equal those of the same cuboid centred at -z."""
+    mutation(mut)
+    e = np.array([0.5, 0.7, 0.0, -0.5, 0.0, 0.0]) * 1e-3
+    v = {}
+    for z in (1.0, -1.0):
+        w = [np.zeros((2 * EIG_H3, EIG_SX, EIG_SX + 2)) for _ in range(3)]
+        O.cuboidsource(LAM, MU, e, 0.5, -0.25, z, 3.0, 4.0, 2.0, np.radians(-120.0), 0.0, 0.3, EIG_DX, EIG_DX,
+                       EIG_DX, *w)
+        v[z] = [a[:EIG_H3, :, :EIG_SX] for a in w]
+    scale = max(np.abs(a).max() for a in v[1.0])
+    ok = (np.allclose(v[1.0][0], v[-1.0][0], atol=1e-9 * scale) and
+          np.allclose(v[1.0][1], v[-1.0][1], atol=1e-9 * scale))
+    assert ok == (mut == 0)
+
+
+TET = np.array([[-4.5, -3.0, 3.0], [6.0, -1.5, 4.5], [0.0, 7.5, 6.0], [0.75, 0.0, 14.0]])
+
+
+def _tetra(e, P=TET, want=("f", "m")):
+    out = {}
+    if "f" in want:
+        v = [np.zeros((2 * EIG_H3, EIG_SX, EIG_SX + 2)) for _ in range(3)]
+        O.tetrahedronsource(LAM, MU, e, *P, EIG_DX, EIG_DX, EIG_DX, *v)
+        out["f"] = np.stack([a[:EIG_H3, :, :EIG_SX] for a in v], -1)
+    if "m" in want:
+        m = np.zeros((EIG_H3, EIG_SX, EIG_SX, 6))
+        O.momentdensitytetrahedron(MU, LAM, e, *P, EIG_DX, EIG_DX, EIG_DX, m)
+        out["m"] = m
+    return out
+
+
+@pytest.mark.parametrize("mut", [0, 34])
+def test_tetrahedron_moment_density_and_body_force(mut, mutation):
+    """Inside a tetrahedron the moment density is C:e, its integral is the
+    volume (|det|/6) times C:e (within 2 %: the Gaussian smoothing of the
+    faces and edges), and the body force is -div of the moment density
+    (spectral derivative, within 2 %): the faces' tractions C:e.n times a
+    smoothed delta.  Its total vanishes."""
+    mutation(mut)
+    e = np.array([0.4, -0.3, 0.2, 0.7, -0.5, -0.1]) * 1e-3
+    E = np.float32(e).astype(np.float64)
+    want = _t6(_hooke(np.array([[E[0], E[1], E[2]], [E[1], E[3], E[4]], [E[2], E[4], E[5]]])))
+    t = _tetra(e)
+    m, f = t["m"], t["f"]
+    c = TET.mean(0)
+    inside = np.allclose(m[int(round(c[2] / EIG_DX)), int(round(c[1] / EIG_DX)), int(round(c[0] / EIG_DX))],
+                         want, rtol=1e-5)
+    vol = abs(np.linalg.det(TET[1:] - TET[0])) / 6.0
+    total = np.allclose(m.sum((0, 1, 2)) * EIG_DX ** 3, vol * want, rtol=0.02, atol=0.02 * vol * np.abs(want).max())
+    div = np.linalg.norm(f - _minus_div(m)) / np.linalg.norm(f) < 0.02
+    zero = np.abs(f.sum((0, 1, 2))).max() < 1e-6 * np.abs(f).max() * f.shape[0] * f.shape[1] * f.shape[2]
+    assert (inside and total and div and zero) == (mut == 0)
+
+
+def test_tetrahedron_moment_density_skips_its_deep_inside_of_small_ones():
+    """momentDensityTetrahedron skips the points that lie within 7 dx1 of
+    the inner side of all four faces (elastic3d.f90:2838-2841, as written),
+    so a tetrahedron smaller than that gets no moment density inside, while
+    its body force is there; a degenerate (flat) one gets neither."""
+    small = np.array([[0.0, 0.0, 3.0], [2.0, 0.0, 3.0], [0.0, 2.0, 3.0], [0.0, 0.0, 5.0]])
+    e = np.array([1.0, 0, 0, 1.0, 0, 1.0]) * 1e-3
+    t = _tetra(e, small)
+    assert np.abs(t["f"]).max() > 0
+    i3 = int(round(3.5 / EIG_DX))
+    assert np.all(t["m"][i3, 1:3, 1:3] == 0)
+    assert np.abs(t["m"]).max() > 0  # outside, where some face is farther than 7 dx1
+    flat = small.copy()
+    flat[3, 2] = 3.0
+    t = _tetra(e, flat)
+    assert np.abs(t["f"]).max() == 0 and np.abs(t["m"]).max() == 0
+
+
+def test_eigenstrain_volumes_in_dislocations():
+    """dislocations adds the volumes' body forces to v and their moment
+    densities to tau, as the routines do one by one; vertex indices are
+    1-based."""
+    e = np.array([0.2, 0.1, -0.3, 0.4, 0.0, -0.2]) * 1e-3
+    cb = np.zeros(1, O.WEAK)
+    cb[0]["x"], cb[0]["y"], cb[0]["z"], cb[0]["width"], cb[0]["length"], cb[0]["thickness"] = -3.0, 2.0, 4.0, 3.0, 4.0, 2.0
+    cb[0]["strike"], cb[0]["dip"] = np.radians(-110.0), np.radians(20.0)
+    cb[0]["e"] = e
+    th = np.zeros(1, O.TETRA)
+    th[0]["e"] = e[::-1]
+    th[0]["i1"], th[0]["i2"], th[0]["i3"], th[0]["i4"] = 4, 2, 3, 1
+    vx = TET[::-1].copy()  # vertex k+1 of vx is TET[3-k]
+    v = [np.zeros((2 * EIG_H3, EIG_SX, EIG_SX + 2)) for _ in range(3)]
+    t = [np.zeros((EIG_SX, EIG_SX + 2)) for _ in range(3)]
+    tau = np.zeros((EIG_H3, EIG_SX, EIG_SX, 6))
+    O.dislocations(None, None, MU, 2.0, 0.0, EIG_DX, EIG_DX, EIG_DX, v, t, tau, lam=LAM, beta=0.3, cuboids=cb,
+                   tetrahedra=th, vertices=vx)
+    w = [np.zeros_like(a) for a in v]
+    m = np.zeros_like(tau)
+    c = cb[0]
+    O.cuboidsource(LAM, MU, e, c["x"], c["y"], c["z"], 3.0, 4.0, 2.0, c["strike"], c["dip"], 0.3, EIG_DX, EIG_DX,
+                   EIG_DX, *w)
+    O.tetrahedronsource(LAM, MU, e[::-1], TET[0], TET[2], TET[1], TET[3], EIG_DX, EIG_DX, EIG_DX, *w)
+    O.momentdensitycuboid(MU, LAM, e, c["x"], c["y"], c["z"], 3.0, 4.0, 2.0, c["strike"], c["dip"], 0.3, EIG_DX,
+                          EIG_DX, EIG_DX, m)
+    O.momentdensitytetrahedron(MU, LAM, e[::-1], TET[0], TET[2], TET[1], TET[3], EIG_DX, EIG_DX, EIG_DX, m)
+    for a, b in zip(v, w):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(tau, m)
+    assert np.abs(tau).max() > 0 and all(np.abs(a).max() == 0 for a in t)
