@@ -119,6 +119,22 @@ int relax_greenfunctioncowling(float* c1, float* c2, float* c3, float* t1, float
                                int sx2, int sx3, double dx1, double dx2, double dx3, double lambda, double mu,
                                double gamma, int align_data);
 
+/* greenfunctioncowling on device buffers, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  c1..c3 and t1..t3 are device pointers
+ * in the padded layout ((sx1+2)*sx2*sx3 and (sx1+2)*sx2 floats), transformed
+ * in place as relax_greenfunctioncowling does; the work is enqueued on
+ * `stream` (a hipStream_t, or NULL for the null stream) and the call returns
+ * without waiting.  The three traction planes it needs live in a static
+ * device array of 12.0 MiB (sx1, sx2 up to 1024; larger grids return
+ * RELAX_ERR_SIZE), one per device, shared by every call on that device:
+ * calls that overlap on one device race.  relax_hip_init sets it up for the
+ * device it selects; otherwise the first call on a device does. */
+int relax_greenfunctioncowling_noalloc(float* c1, float* c2, float* c3, float* t1, float* t2, float* t3, int sx1,
+                                       int sx2, int sx3, double dx1, double dx2, double dx3, double lambda,
+                                       double mu, double gamma, void* stream);
+
 /* ---- elastic3d.f90 ------------------------------------------------------- */
 /* stressupdate, elastic3d.f90:956-1180: sig += C:eps(v); sx3 is the depth
  * extent of the tensor field (half the FFT grid in the RK2 loop). */

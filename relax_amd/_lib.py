@@ -71,6 +71,8 @@ SIGNATURES = {
                                       c_double, c_double, c_double]),
     "relax_greenfunctioncowling": (c_int, [P, P, P, P, P, P, c_int, c_int, c_int, c_double, c_double, c_double,
                                            c_double, c_double, c_double, c_int]),
+    "relax_greenfunctioncowling_noalloc": (c_int, [P, P, P, P, P, P, c_int, c_int, c_int, c_double, c_double,
+                                                   c_double, c_double, c_double, c_double, P]),
     "relax_stressupdate": (c_int, [P, P, P, c_double, c_double, c_double, c_double, c_double, c_int, c_int, c_int,
                                    P, c_int]),
     "relax_equivalentbodyforce": (c_int, [P, c_double, c_double, c_double, c_int, c_int, c_int, P, P, P, P, P, P,

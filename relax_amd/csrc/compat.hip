@@ -7,6 +7,8 @@
 // (elastic3d.f90:846-907): fir1 for 15+ samples, fir7 for 5..14.
 #include <algorithm>
 #include <functional>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -50,12 +52,14 @@ int relax_hip_meminfo(size_t* free_bytes, size_t* total_bytes) {
   return RELAX_OK;
 }
 
+int noalloc_setup();
+
 int relax_hip_init(int device) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return RELAX_ERR_NODEVICE;
   if (device < 0 || device >= n) return RELAX_ERR_ARG;
   HIP_TRY(hipSetDevice(device));
-  return RELAX_OK;
+  return noalloc_setup();
 }
 
 namespace {
@@ -618,6 +622,60 @@ int relax_greenfunctioncowling(float* c1, float* c2, float* c3, float* t1, float
     if ((rc = get_real(ht[k], t[k].f(), sx1, sx2, align_data))) return rc;
   }
   return RELAX_OK;
+}
+
+// ------------------------------------------ greenfunctioncowling, no allocation --
+// The three traction planes of the surface correction, for grids of up to
+// 1024 x 1024 horizontally, in a static device array: every device the
+// library runs on holds one (12.0 MiB) until the process ends, and every
+// call on a device shares it.  Host code uses the address that
+// hipGetSymbolAddress returns for the current device, never the array's
+// name.
+static const long NOALLOC_MAX_N = 1024;
+static const long NOALLOC_PLANE = (NOALLOC_MAX_N + 2) * NOALLOC_MAX_N;  // floats per plane
+static __device__ __align__(256) float scratch[3 * NOALLOC_PLANE];
+static std::mutex g_scratch_mu;
+static std::map<int, float*> g_scratch;  // device -> address of its scratch
+
+// The current device's scratch address, looked up once per device.
+static int scratch_address(float** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return RELAX_ERR_HIP;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(dev);
+  if (it == g_scratch.end()) {
+    float* p = nullptr;
+    if (hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return RELAX_ERR_HIP;
+    it = g_scratch.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return RELAX_OK;
+}
+
+// Per-device set-up of the allocation-free entry: the scratch address and
+// the twiddle tables of every transform length (relax_hip_init calls it).
+int noalloc_setup() {
+  float* p = nullptr;
+  if (int rc = scratch_address(&p)) return rc;
+  return twiddle_table(2) ? RELAX_OK : RELAX_ERR_HIP;
+}
+
+int relax_greenfunctioncowling_noalloc(float* c1, float* c2, float* c3, float* t1, float* t2, float* t3, int sx1,
+                                       int sx2, int sx3, double dx1, double dx2, double dx3, double lambda,
+                                       double mu, double gamma, void* stream) {
+  if (!c1 || !c2 || !c3 || !t1 || !t2 || !t3) return RELAX_ERR_ARG;
+  if (!pow2_in(sx1, 16, NOALLOC_MAX_N) || !pow2_in(sx2, 16, NOALLOC_MAX_N) || !pow2_in(sx3, 16, 1024))
+    return RELAX_ERR_SIZE;
+  if ((long)(sx1 + 2) * sx2 > NOALLOC_PLANE) return RELAX_ERR_SIZE;
+  int rc;
+  if ((rc = has_device())) return rc;
+  float* base = nullptr;
+  if ((rc = noalloc_setup()) || (rc = scratch_address(&base))) return rc;
+  const long plane = (long)(sx1 + 2) * sx2;
+  float* v[3] = {c1, c2, c3};
+  float* t[3] = {t1, t2, t3};
+  float* pp[3] = {base, base + plane, base + 2 * plane};
+  return green_full_device(v, t, pp, sx1, sx2, sx3, dx1, dx2, dx3, lambda, mu, gamma, (hipStream_t)stream, false);
 }
 
 // ========================================================= elastic3d.f90 ==
