@@ -6,14 +6,16 @@ over the 512x512x256 workload (40 slip patches, power law n = 3 below a 25 km
 lid; relax_amd.workloads "c1"), with the whole state resident in HBM.
 
   python bench.py [--steps K] [--warmup W] [--workload c1] [--no-cpu-baseline]
-                  [--no-batched] [--no-serving]
+                  [--no-batched] [--no-serving] [--only batched|serving]
 
 Prints one JSON line.  The timed region is K back-to-back steps on the
 model's stream with HIP-event phase timers (one synchronisation at the end),
 bracketed by device synchronisations; `roofline` reports the dominant
 kernel's algorithmic bytes per launch over its event-measured average
 duration.  `cpu_baseline` times the oracle (the C++ restatement of the
-reference, OpenMP on the host cores) on one step of the same state.
+reference, OpenMP on the host cores) on two steps of the same state.
+`batched` (64 models of 128^3) and `serving` (the request stream) ride on
+the same line; --only runs one of them alone.
 """
 import argparse
 import json
@@ -192,7 +194,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 64 x 128^3 measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream measurement")
+    ap.add_argument("--only", choices=("batched", "serving"),
+                    help="run only this measurement and print its JSON (for profiling it alone)")
     args = ap.parse_args()
+    if args.only:
+        out = batched_c3() if args.only == "batched" else serving_c4()
+        print(json.dumps(out), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the RK2 step of one model per GPU "
               "(--gpus %d is not supported)" % args.gpus, file=sys.stderr)

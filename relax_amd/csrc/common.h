@@ -20,6 +20,50 @@
 
 int relax_hip_error(hipError_t e, const char* what);
 
+// Memory management and other device-wide calls, serialised with the graph
+// capture of batches (relax_batch_step).  Freeing memory or destroying a
+// stream synchronises the device, and a synchronisation that reaches a stream
+// another thread is capturing breaks that capture; every such call of the
+// library takes this process-wide (recursive) lock, as does the capture.
+void capture_lock();
+void capture_unlock();
+struct CaptureGuard {
+  CaptureGuard() { capture_lock(); }
+  ~CaptureGuard() { capture_unlock(); }
+  CaptureGuard(const CaptureGuard&) = delete;
+  CaptureGuard& operator=(const CaptureGuard&) = delete;
+};
+template <class T>
+inline hipError_t rx_malloc(T** p, size_t bytes) {
+  CaptureGuard g;
+  return hipMalloc(reinterpret_cast<void**>(p), bytes);
+}
+template <class T>
+inline hipError_t rx_host_malloc(T** p, size_t bytes) {
+  CaptureGuard g;
+  return hipHostMalloc(reinterpret_cast<void**>(p), bytes, hipHostMallocDefault);
+}
+inline hipError_t rx_free(void* p) {
+  CaptureGuard g;
+  return hipFree(p);
+}
+inline hipError_t rx_host_free(void* p) {
+  CaptureGuard g;
+  return hipHostFree(p);
+}
+inline hipError_t rx_device_sync() {
+  CaptureGuard g;
+  return hipDeviceSynchronize();
+}
+inline hipError_t rx_stream_create(hipStream_t* s) {
+  CaptureGuard g;
+  return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+inline hipError_t rx_stream_destroy(hipStream_t s) {
+  CaptureGuard g;
+  return hipStreamDestroy(s);
+}
+
 // ----------------------------------------------------------------- complex --
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }

@@ -11,6 +11,7 @@
 #include <mutex>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -19,9 +20,12 @@
 #include "rheology.h"
 
 // ------------------------------------------------------------ library ----
+// A failed HIP call becomes a status; with RELAX_HIP_VERBOSE set in the
+// environment the call and the runtime's error name go to stderr as well.
 int relax_hip_error(hipError_t e, const char* what) {
+  static const bool verbose = std::getenv("RELAX_HIP_VERBOSE") != nullptr;
+  if (verbose) std::fprintf(stderr, "relax-hip: %s failed: %s\n", what, hipGetErrorName(e));
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return RELAX_ERR_NOMEM;
-  (void)what;
   return RELAX_ERR_HIP;
 }
 
@@ -67,8 +71,8 @@ namespace {
 // device buffer owned for the duration of one call
 struct Buf {
   void* p = nullptr;
-  ~Buf() { if (p) (void)hipFree(p); }
-  int alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 16) == hipSuccess ? RELAX_OK : RELAX_ERR_NOMEM; }
+  ~Buf() { if (p) (void)rx_free(p); }
+  int alloc(size_t bytes) { return rx_malloc(&p, bytes ? bytes : 16) == hipSuccess ? RELAX_OK : RELAX_ERR_NOMEM; }
   float* f() const { return (float*)p; }
   float2* c() const { return (float2*)p; }
 };
@@ -411,7 +415,7 @@ __global__ void k_momentdensityshear(PlaneDev P, double slip, float rmu, int n1,
 
 int sync_status(int rc) {
   if (rc) return rc;
-  if (hipDeviceSynchronize() != hipSuccess) return RELAX_ERR_HIP;
+  if (rx_device_sync() != hipSuccess) return RELAX_ERR_HIP;
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }
 

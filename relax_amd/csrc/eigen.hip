@@ -286,10 +286,10 @@ int eigen_stage(relax_model_t* m, const relax_event_t* ev) {
   }
   const size_t bytes = cub.size() * sizeof(CuboidDev) + tet.size() * sizeof(TetraDev);
   if (bytes > m->eig_cap) {
-    if (m->eig_dev) (void)hipFree(m->eig_dev);
+    if (m->eig_dev) (void)rx_free(m->eig_dev);
     m->eig_dev = nullptr;
     m->eig_cap = 0;
-    HIP_TRY(hipMalloc(&m->eig_dev, bytes));
+    HIP_TRY(rx_malloc(&m->eig_dev, bytes));
     m->eig_cap = bytes;
   }
   char* p = static_cast<char*>(m->eig_dev);

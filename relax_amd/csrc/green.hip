@@ -362,7 +362,9 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
   const unsigned zstep = (unsigned)(Q * A.zs * (long)sizeof(float2));  // uniform: the scalar offset
 
   // step A of one half: DFT16, twiddle, into the buffer.  The odd half
-  // loads the column again (from L2) rather than keeping it in registers.
+  // loads the column again (from L2) rather than keeping it in registers;
+  // the loads are issued right after the even half's step A, so that they
+  // complete under the even half's step B.
   float2 x[16];
   auto load = [&]() {
 #pragma unroll
@@ -378,7 +380,6 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
     if (!isA) return;
     float2* y = x;
     if (odd) {
-      load();
 #pragma unroll
       for (int m = 0; m < 16; ++m) y[m] = cmul(x[m], stw[Q * m + aj]);
     }
@@ -401,6 +402,7 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
     }
   };
   stepA(false);
+  if (isA) load();  // the odd half's input, in flight during the even half's step B
   __syncthreads();
   if (half == 0) stepB();
   __syncthreads();
@@ -470,8 +472,10 @@ __global__ __launch_bounds__(512, 4) void k_green_reg(ColumnConsts C, double mu,
     //   i 2 pi ((l+2m) k3 u3 + l (k1 u1 + k2 u2)) - 2 mu alpha gamma u3
     float2 acc;
     if (c < 2) {
+      // (S[c] with a constant index: a run-time index would move S to LDS)
       const float qc = c == 0 ? q1 : q2;
-      const float2 a = make_float2(S[c].x + qc * S[3].x, S[c].y + qc * S[3].y);
+      const float2 Sc = c == 0 ? S[0] : S[1];
+      const float2 a = make_float2(Sc.x + qc * S[3].x, Sc.y + qc * S[3].y);
       acc = make_float2(-C.mu2pi * a.y, C.mu2pi * a.x);
     } else {
       const float2 d = make_float2(C.mod2pi * S[2].x + C.lam2pi * S[4].x, C.mod2pi * S[2].y + C.lam2pi * S[4].y);

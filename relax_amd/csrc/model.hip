@@ -270,10 +270,10 @@ static void free_model(relax_model_t* m) {
                   m->dg[0], m->dg[1], m->dg[2], m->dg[3], m->zones_dev, m->pts_dev, m->ev_dev, m->mi,
                   m->eig_dev};
   for (void* p : ptrs)
-    if (p) (void)hipFree(p);
-  if (m->host_sc) (void)hipHostFree(m->host_sc);
+    if (p) (void)rx_free(p);
+  if (m->host_sc) (void)rx_host_free(m->host_sc);
   for (hipEvent_t e : m->prof_ev) (void)hipEventDestroy(e);
-  if (m->stream) (void)hipStreamDestroy(m->stream);
+  if (m->stream) (void)rx_stream_destroy(m->stream);
   delete m;
 }
 
@@ -355,7 +355,7 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
   m->desc = *d;
   m->events.assign(d->event_times, d->event_times + (d->event_times ? d->ne : 0));
   if (!m->events.empty()) {
-    if (hipMalloc((void**)&m->ev_dev, m->events.size() * sizeof(double)) != hipSuccess ||
+    if (rx_malloc(&m->ev_dev, m->events.size() * sizeof(double)) != hipSuccess ||
         hipMemcpy(m->ev_dev, m->events.data(), m->events.size() * sizeof(double), hipMemcpyHostToDevice) !=
             hipSuccess) {
       free_model(m);
@@ -368,8 +368,8 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
   g.H = (long)g.n1 * g.n2 * g.h3;
   g.dx1 = (float)d->dx1; g.dx2 = (float)d->dx2; g.dx3 = (float)d->dx3;
   const long H = g.H;
-  bool ok = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) == hipSuccess;
-  auto alloc = [&](void** p, size_t bytes) { if (ok && hipMalloc(p, bytes) != hipSuccess) ok = false; };
+  bool ok = rx_stream_create(&m->stream) == hipSuccess;
+  auto alloc = [&](void** p, size_t bytes) { if (ok && rx_malloc(p, bytes) != hipSuccess) ok = false; };
   alloc((void**)&m->u, 3 * H * 4);
   alloc((void**)&m->sig, 6 * H * 4);
   alloc((void**)&m->tau, 6 * H * 4);
@@ -386,7 +386,7 @@ int relax_model_create(const relax_model_desc_t* d, relax_model_t** out) {
   alloc((void**)&m->ftab, 4L * g.h3 * 8);
   alloc((void**)&m->md_dev, sizeof(ModelDev));
   m->self = m;
-  if (ok && hipHostMalloc((void**)&m->host_sc, sizeof(StepScalars) * 64) != hipSuccess) ok = false;
+  if (ok && rx_host_malloc(&m->host_sc, sizeof(StepScalars) * 64) != hipSuccess) ok = false;
   m->host_cap = 64;
   if (!ok) { free_model(m); return RELAX_ERR_NOMEM; }
   // zero state; no mechanisms until set_structure
@@ -543,8 +543,8 @@ static int set_transient(relax_model_t* m, int k, const relax_layer_t* layers, i
     t[2 * h3 + i] = (float)layers[i].Gk;
   }
   if (!m->E) {
-    if (hipMalloc((void**)&m->E, 6 * H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
-    if (hipMalloc((void**)&m->Em, 6 * H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
+    if (rx_malloc(&m->E, 6 * H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
+    if (rx_malloc(&m->Em, 6 * H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
     HIP_TRY(hipMemsetAsync(m->E, 0, 6 * H * 4, m->stream));
     HIP_TRY(hipMemsetAsync(m->Em, 0, 6 * H * 4, m->stream));
   }
@@ -568,12 +568,12 @@ int relax_model_set_weakzones(relax_model_t* m, int which, const relax_weak_t* z
     return RELAX_OK;
   }
   const Grid& g = m->g;
-  if (!m->dg[slot] && hipMalloc((void**)&m->dg[slot], g.H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
+  if (!m->dg[slot] && rx_malloc(&m->dg[slot], g.H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
   if (nz > m->zones_cap) {
-    if (m->zones_dev) HIP_TRY(hipFree(m->zones_dev));
+    if (m->zones_dev) HIP_TRY(rx_free(m->zones_dev));
     m->zones_dev = nullptr;
     m->zones_cap = 0;
-    if (hipMalloc((void**)&m->zones_dev, (size_t)nz * sizeof(WeakDev)) != hipSuccess) return RELAX_ERR_NOMEM;
+    if (rx_malloc(&m->zones_dev, (size_t)nz * sizeof(WeakDev)) != hipSuccess) return RELAX_ERR_NOMEM;
     m->zones_cap = nz;
   }
   // the taper of the reference's calls is 0 (relax.f90:487, :502, :525, :540)
@@ -593,12 +593,12 @@ int relax_model_set_transient_strain(relax_model_t* m, const relax_tensor_t* eps
   if (int rc = wait_batch(m)) return rc;
   const long H = m->g.H;
   float* tmp = nullptr;
-  if (hipMalloc((void**)&tmp, 6 * H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
+  if (rx_malloc(&tmp, 6 * H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
   int rc = hipMemcpyAsync(tmp, epsilonik, 6 * H * 4, hipMemcpyHostToDevice, m->stream) == hipSuccess
                ? launch_aos_soa(tmp, m->E, H, true, m->stream)
                : RELAX_ERR_HIP;
   if (hipStreamSynchronize(m->stream) != hipSuccess && !rc) rc = RELAX_ERR_HIP;
-  (void)hipFree(tmp);
+  (void)rx_free(tmp);
   return rc;
 }
 
@@ -607,12 +607,12 @@ int relax_model_get_transient_strain(relax_model_t* m, relax_tensor_t* epsilonik
   if (int rc = wait_batch(m)) return rc;
   const long H = m->g.H;
   float* tmp = nullptr;
-  if (hipMalloc((void**)&tmp, 6 * H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
+  if (rx_malloc(&tmp, 6 * H * 4) != hipSuccess) return RELAX_ERR_NOMEM;
   int rc = launch_aos_soa(tmp, m->E, H, false, m->stream);
   if (!rc && hipMemcpyAsync(epsilonik, tmp, 6 * H * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess)
     rc = RELAX_ERR_HIP;
   if (hipStreamSynchronize(m->stream) != hipSuccess && !rc) rc = RELAX_ERR_HIP;
-  (void)hipFree(tmp);
+  (void)rx_free(tmp);
   return rc;
 }
 
@@ -748,17 +748,17 @@ int relax_model_set_planes(relax_model_t* m, const relax_plane_t* planes, int np
   // (and frees, which would wait for the whole device) nothing
   const long npatch = m->patch_off[np];
   if (np > m->planes_cap) {
-    if (m->planes_dev) (void)hipFree(m->planes_dev);
+    if (m->planes_dev) (void)rx_free(m->planes_dev);
     m->planes_dev = nullptr;
     m->planes_cap = 0;
-    HIP_TRY(hipMalloc(&m->planes_dev, sizeof(PlaneDev) * np));
+    HIP_TRY(rx_malloc(&m->planes_dev, sizeof(PlaneDev) * np));
     m->planes_cap = np;
   }
   if (npatch > m->patches_cap) {
-    if (m->patches_dev) (void)hipFree(m->patches_dev);
+    if (m->patches_dev) (void)rx_free(m->patches_dev);
     m->patches_dev = nullptr;
     m->patches_cap = 0;
-    HIP_TRY(hipMalloc(&m->patches_dev, sizeof(relax_slippatch_t) * npatch));
+    HIP_TRY(rx_malloc(&m->patches_dev, sizeof(relax_slippatch_t) * npatch));
     m->patches_cap = npatch;
   }
   if (np > 0)
@@ -811,7 +811,7 @@ int relax_model_upload(relax_model_t* m, const float* u1, const float* u2, const
   else HIP_TRY(hipMemsetAsync(m->gam, 0, H * 4, m->stream));
   // tensors: stage the AoS copy in the sig buffer's space via a temporary
   float* tmp = nullptr;
-  HIP_TRY(hipMalloc(&tmp, 6 * H * 4));
+  HIP_TRY(rx_malloc(&tmp, 6 * H * 4));
   rc = RELAX_OK;
   if (hipMemcpyAsync(tmp, tau, 6 * H * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess) rc = RELAX_ERR_HIP;
   if (!rc) rc = launch_aos_soa(tmp, m->tau, H, true, m->stream);
@@ -820,7 +820,7 @@ int relax_model_upload(relax_model_t* m, const float* u1, const float* u2, const
     if (!rc) rc = launch_aos_soa(tmp, m->sig, H, true, m->stream);
   }
   if (hipStreamSynchronize(m->stream) != hipSuccess && !rc) rc = RELAX_ERR_HIP;
-  (void)hipFree(tmp);
+  (void)rx_free(tmp);
   if (rc) return rc;
   k_reset_scalars<<<1, 1, 0, m->stream>>>(m->sc, t, oi, i);
   if (!sig) {  // sig = -tau + C:eps(u), relax.f90:369-371
@@ -845,7 +845,7 @@ int relax_model_download(relax_model_t* m, float* u1, float* u2, float* u3, rela
   if (gamma && (rc = copy_field(gamma, m->gam, g, align_data, false, m->stream))) return rc;
   if (tau || sig) {
     float* tmp = nullptr;
-    HIP_TRY(hipMalloc(&tmp, 6 * H * 4));
+    HIP_TRY(rx_malloc(&tmp, 6 * H * 4));
     rc = RELAX_OK;
     if (tau) {
       rc = launch_aos_soa(tmp, m->tau, H, false, m->stream);
@@ -857,7 +857,7 @@ int relax_model_download(relax_model_t* m, float* u1, float* u2, float* u3, rela
       if (!rc && hipMemcpyAsync(sig, tmp, 6 * H * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess) rc = RELAX_ERR_HIP;
     }
     if (hipStreamSynchronize(m->stream) != hipSuccess && !rc) rc = RELAX_ERR_HIP;
-    (void)hipFree(tmp);
+    (void)rx_free(tmp);
     if (rc) return rc;
   }
   if (t) {
@@ -917,10 +917,10 @@ int relax_model_points(relax_model_t* m, const double* x, int n, double rot, dou
   }
   const size_t need = (size_t)n * (sizeof(long) + 9 * sizeof(float));
   if (need > m->pts_cap) {
-    if (m->pts_dev) HIP_TRY(hipFree(m->pts_dev));
+    if (m->pts_dev) HIP_TRY(rx_free(m->pts_dev));
     m->pts_dev = nullptr;
     m->pts_cap = 0;
-    if (hipMalloc(&m->pts_dev, need) != hipSuccess) return RELAX_ERR_NOMEM;
+    if (rx_malloc(&m->pts_dev, need) != hipSuccess) return RELAX_ERR_NOMEM;
     m->pts_cap = need;
   }
   long* doff = static_cast<long*>(m->pts_dev);
@@ -1219,13 +1219,13 @@ __global__ void k_add_shear(const PlaneDev* __restrict__ P, const double* __rest
 static int stage_planes(relax_model_t* m, const double* slip) {
   const long n = (long)m->shear_host.size();
   if (n > m->shear_cap) {
-    if (m->shear_dev) (void)hipFree(m->shear_dev);
-    if (m->shear_slip) (void)hipFree(m->shear_slip);
+    if (m->shear_dev) (void)rx_free(m->shear_dev);
+    if (m->shear_slip) (void)rx_free(m->shear_slip);
     m->shear_dev = nullptr;
     m->shear_slip = nullptr;
     m->shear_cap = 0;
-    HIP_TRY(hipMalloc(&m->shear_dev, sizeof(PlaneDev) * n));
-    HIP_TRY(hipMalloc(&m->shear_slip, sizeof(double) * n));
+    HIP_TRY(rx_malloc(&m->shear_dev, sizeof(PlaneDev) * n));
+    HIP_TRY(rx_malloc(&m->shear_slip, sizeof(double) * n));
     m->shear_cap = n;
   }
   HIP_TRY(hipMemcpyAsync(m->shear_dev, m->shear_host.data(), sizeof(PlaneDev) * n, hipMemcpyHostToDevice,
@@ -1613,10 +1613,10 @@ int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist)
   m->cur_nloads = 0;
   if (nl + nm > 0) {
     if (nl + nm > m->loads_cap) {
-      if (m->loads_dev) (void)hipFree(m->loads_dev);
+      if (m->loads_dev) (void)rx_free(m->loads_dev);
       m->loads_dev = nullptr;
       m->loads_cap = 0;
-      HIP_TRY(hipMalloc(&m->loads_dev, sizeof(relax_source_t) * (nl + nm)));
+      HIP_TRY(rx_malloc(&m->loads_dev, sizeof(relax_source_t) * (nl + nm)));
       m->loads_cap = nl + nm;
     }
     if (nl) HIP_TRY(hipMemcpyAsync(m->loads_dev, ev->l, sizeof(relax_source_t) * nl, hipMemcpyHostToDevice, m->stream));
@@ -1651,7 +1651,7 @@ int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist)
     const long ld = g.n1 + 2, vs = ld * g.n2 * g.n3, ts = ld * g.n2;
     float* scratch = nullptr;
     const size_t bytes = (size_t)(3 * vs + 6 * ts) * 4;
-    HIP_TRY(hipMalloc(&scratch, bytes));
+    HIP_TRY(rx_malloc(&scratch, bytes));
     float* v[3] = {scratch, scratch + vs, scratch + 2 * vs};
     float* tr[3] = {scratch + 3 * vs, scratch + 3 * vs + ts, scratch + 3 * vs + 2 * ts};
     float* pp[3] = {scratch + 3 * vs + 3 * ts, scratch + 3 * vs + 4 * ts, scratch + 3 * vs + 5 * ts};
@@ -1675,7 +1675,7 @@ int relax_model_event(relax_model_t* m, const relax_event_t* ev, double nyquist)
       if (hipGetLastError() != hipSuccess || hipStreamSynchronize(m->stream) != hipSuccess) rc = RELAX_ERR_HIP;
     }
     (void)hipStreamSynchronize(m->stream);
-    (void)hipFree(scratch);
+    (void)rx_free(scratch);
     if (rc) return rc;
   }
   // the moment densities of every source into tau, then
@@ -1716,7 +1716,7 @@ int relax_model_set_interseismic(relax_model_t* m, const relax_event_t* ev) {
   }
   const Grid& g = m->g;
   const long H = g.H;
-  if (!m->mi) HIP_TRY(hipMalloc(&m->mi, 6 * H * sizeof(float)));
+  if (!m->mi) HIP_TRY(rx_malloc(&m->mi, 6 * H * sizeof(float)));
   HIP_TRY(hipMemsetAsync(m->mi, 0, 6 * H * sizeof(float), m->stream));
   std::vector<PlaneDev>& hp = m->shear_host;
   std::vector<double> amp;
@@ -1794,12 +1794,17 @@ struct relax_batch {
 // graph once the membership has been stable for this many steps.
 static const int GRAPH_AFTER = 2;
 
+static std::recursive_mutex g_capture_mu;
+void capture_lock() { g_capture_mu.lock(); }
+void capture_unlock() { g_capture_mu.unlock(); }
+
 __global__ void k_gather_scalars(const ModelDev* __restrict__ md, int nm, StepScalars* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nm) out[i] = *md[i].sc;
 }
 
 static void drop_graph(relax_batch_t* b) {
+  CaptureGuard guard;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
@@ -1820,7 +1825,7 @@ int relax_batch_create(relax_batch_t** out) {
   relax_batch_t* b = new (std::nothrow) relax_batch_t();
   if (!b) return RELAX_ERR_NOMEM;
   if (hipGetDevice(&b->device) != hipSuccess ||
-      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+      rx_stream_create(&b->stream) != hipSuccess) {
     delete b;
     return RELAX_ERR_HIP;
   }
@@ -1833,10 +1838,10 @@ int relax_batch_destroy(relax_batch_t* b) {
   (void)hipStreamSynchronize(b->stream);
   for (relax_model_t* m : b->models) m->batch = nullptr;
   drop_graph(b);
-  if (b->md_dev) (void)hipFree(b->md_dev);
-  if (b->gather) (void)hipFree(b->gather);
-  if (b->host_sc) (void)hipHostFree(b->host_sc);
-  (void)hipStreamDestroy(b->stream);
+  if (b->md_dev) (void)rx_free(b->md_dev);
+  if (b->gather) (void)rx_free(b->gather);
+  if (b->host_sc) (void)rx_host_free(b->host_sc);
+  (void)rx_stream_destroy(b->stream);
   delete b;
   return RELAX_OK;
 }
@@ -1889,14 +1894,14 @@ static int batch_refresh(relax_batch_t* b) {
   HIP_TRY(hipStreamSynchronize(b->stream));
   drop_graph(b);
   if (nm > b->cap) {
-    if (b->md_dev) { (void)hipFree(b->md_dev); b->md_dev = nullptr; }
-    if (b->gather) { (void)hipFree(b->gather); b->gather = nullptr; }
-    if (b->host_sc) { (void)hipHostFree(b->host_sc); b->host_sc = nullptr; }
+    if (b->md_dev) { (void)rx_free(b->md_dev); b->md_dev = nullptr; }
+    if (b->gather) { (void)rx_free(b->gather); b->gather = nullptr; }
+    if (b->host_sc) { (void)rx_host_free(b->host_sc); b->host_sc = nullptr; }
     b->cap = 0;
     const int cap = nm < 8 ? 8 : 2 * nm;  // room to grow without reallocating
-    HIP_TRY(hipMalloc(&b->md_dev, sizeof(ModelDev) * cap));
-    HIP_TRY(hipMalloc(&b->gather, sizeof(StepScalars) * cap));
-    HIP_TRY(hipHostMalloc((void**)&b->host_sc, sizeof(StepScalars) * cap));
+    HIP_TRY(rx_malloc(&b->md_dev, sizeof(ModelDev) * cap));
+    HIP_TRY(rx_malloc(&b->gather, sizeof(StepScalars) * cap));
+    HIP_TRY(rx_host_malloc(&b->host_sc, sizeof(StepScalars) * cap));
     b->cap = cap;
   }
   b->md_host.resize(nm);
@@ -1929,18 +1934,21 @@ int relax_batch_step(relax_batch_t* b, relax_step_info_t* infos) {
   } else if (!b->exec) {
     // capture one step of all members; replayed until the membership or a
     // member's record changes
+    // (no other thread frees memory or synchronises the device meanwhile)
+    CaptureGuard guard;
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     rc = enqueue_steps(c, nullptr);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc || e != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
-      return rc ? rc : RELAX_ERR_HIP;
+      return rc ? rc : relax_hip_error(e, "hipStreamEndCapture");
     }
     b->graph = g;
-    if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+    const hipError_t ei = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
       drop_graph(b);
-      return RELAX_ERR_HIP;
+      return relax_hip_error(ei, "hipGraphInstantiate");
     }
   }
   if (b->exec) HIP_TRY(hipGraphLaunch(b->exec, b->stream));

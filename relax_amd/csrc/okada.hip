@@ -264,13 +264,13 @@ int relax_model_okada(relax_model_t* m, const relax_source_t* sc, int ns, const 
     }
   if (hs.empty()) return RELAX_OK;
   void* src = nullptr;
-  HIP_TRY(hipMalloc(&src, sizeof(OkadaSource) * hs.size()));
+  HIP_TRY(rx_malloc(&src, sizeof(OkadaSource) * hs.size()));
   const Grid& g = m->g;
   int rc = launch_okada(hs, src, m->desc.lambda, m->desc.mu, g.n1, g.n2, g.h3, m->desc.dx1, m->desc.dx2,
                         m->desc.dx3, m->u, g.n1, g.H, m->stream);
   if (!rc) rc = model_stress_set(m);
   if (hipStreamSynchronize(m->stream) != hipSuccess && !rc) rc = RELAX_ERR_HIP;
-  (void)hipFree(src);
+  (void)rx_free(src);
   return rc;
 }
 
@@ -287,16 +287,16 @@ int relax_grnfct_okada(double lambda, double mu, double slip, double opening, do
   std::vector<OkadaSource> hs{okada_source(slip, opening, s)};
   float* d = nullptr;
   void* src = nullptr;
-  HIP_TRY(hipMalloc(&d, sizeof(float) * 3 * fs));
-  int rc = hipMalloc(&src, sizeof(OkadaSource)) == hipSuccess ? RELAX_OK : RELAX_ERR_NOMEM;
+  HIP_TRY(rx_malloc(&d, sizeof(float) * 3 * fs));
+  int rc = rx_malloc(&src, sizeof(OkadaSource)) == hipSuccess ? RELAX_OK : RELAX_ERR_NOMEM;
   float* h[3] = {u1, u2, u3};
   for (int k = 0; k < 3 && !rc; ++k)
     if (hipMemcpy(d + k * fs, h[k], sizeof(float) * fs, hipMemcpyHostToDevice) != hipSuccess) rc = RELAX_ERR_HIP;
   if (!rc) rc = launch_okada(hs, src, lambda, mu, sx1, sx2, sx3, dx1, dx2, dx3, d, ld, fs, 0);
-  if (!rc && hipDeviceSynchronize() != hipSuccess) rc = RELAX_ERR_HIP;
+  if (!rc && rx_device_sync() != hipSuccess) rc = RELAX_ERR_HIP;
   for (int k = 0; k < 3 && !rc; ++k)
     if (hipMemcpy(h[k], d + k * fs, sizeof(float) * fs, hipMemcpyDeviceToHost) != hipSuccess) rc = RELAX_ERR_HIP;
-  (void)hipFree(d);
-  if (src) (void)hipFree(src);
+  (void)rx_free(d);
+  if (src) (void)rx_free(src);
   return rc;
 }

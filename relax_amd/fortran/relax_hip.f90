@@ -354,6 +354,16 @@ MODULE relax_hip
        IMPORT :: c_int, c_ptr
        TYPE(c_ptr), VALUE :: h
      END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_okada(h,sc,ns,tsc,nt,nyquist) BIND(C,name="relax_model_okada")
+       IMPORT :: c_int, c_double, c_ptr
+       TYPE(c_ptr), VALUE :: h,sc,tsc
+       INTEGER(c_int), VALUE :: ns,nt
+       REAL(c_double), VALUE :: nyquist
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_model_set_interseismic(h,ev) BIND(C,name="relax_model_set_interseismic")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h,ev
+     END FUNCTION
   END INTERFACE
 
 CONTAINS
@@ -795,5 +805,33 @@ CONTAINS
     END IF
     CALL check(c_model_event(model%h,C_LOC(ev),nyquist),"relax_model_event")
   END SUBROUTINE relax_hip_model_event
+
+  !> Okada's displacement of the shear segments sc and tensile segments tsc
+  !! at or below the Nyquist size (dislocations_disp, relax.f90:357-360,
+  !! :1300-1349), from the deck's own copies (in%events(1)%sc, %tsc)
+  SUBROUTINE relax_hip_model_okada(model,sc,tsc,nyquist)
+    TYPE(relax_model), INTENT(IN) :: model
+    TYPE(SOURCE_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: sc,tsc
+    REAL*8, INTENT(IN) :: nyquist
+    TYPE(c_ptr) :: ps,pt
+    ps=C_NULL_PTR
+    pt=C_NULL_PTR
+    IF (SIZE(sc) .GT. 0) ps=C_LOC(sc)
+    IF (SIZE(tsc) .GT. 0) pt=C_LOC(tsc)
+    CALL check(c_model_okada(model%h,ps,INT(SIZE(sc),c_int),pt,INT(SIZE(tsc),c_int),nyquist),"relax_model_okada")
+  END SUBROUTINE relax_hip_model_okada
+
+  !> inter-seismic loading (in%inter: shear segments s and tensile segments
+  !! ts whose slip or opening is a rate; relax.f90:669-674, :795-799)
+  SUBROUTINE relax_hip_model_set_interseismic(model,s,ts)
+    TYPE(relax_model), INTENT(IN) :: model
+    TYPE(SOURCE_STRUCT), DIMENSION(:), INTENT(IN), TARGET, CONTIGUOUS :: s,ts
+    TYPE(relax_event_c), TARGET :: ev
+    ev%ns=INT(SIZE(s),c_int)
+    IF (ev%ns .GT. 0) ev%s=C_LOC(s)
+    ev%nt=INT(SIZE(ts),c_int)
+    IF (ev%nt .GT. 0) ev%ts=C_LOC(ts)
+    CALL check(c_model_set_interseismic(model%h,C_LOC(ev)),"relax_model_set_interseismic")
+  END SUBROUTINE relax_hip_model_set_interseismic
 
 END MODULE relax_hip

@@ -19,6 +19,7 @@ path: set-up, steps, surface displacement, release -- one at a time.
 import ctypes
 import dataclasses
 import math
+import threading
 import time
 
 import numpy as np
@@ -173,13 +174,21 @@ def run_back_to_back(requests, realtime=True):
 
 
 class Server:
-    """Continuous batching over grid classes (see the module docstring).
+    """Continuous batching over batch classes (see the module docstring).
+
+    Every class has a worker thread of its own that admits the class's
+    requests into its batch, steps the batch and retires finished members;
+    the library releases the interpreter lock inside its calls, so a class's
+    next steps are enqueued while another class's thread sets up a model,
+    and no class waits for the host work of another.  The calling thread
+    hands each request to its class at its arrival time.
 
     Finished models go back to a pool of their class and are reset for the
     next request: releasing device memory waits for the whole device, which
     would stall every class's queued steps.  `used` counts the footprint of
     every model held, pooled ones included; pooled models of other classes
-    are released when an admission needs their memory."""
+    are released when an admission needs their memory, and a request that
+    does not fit waits until memory is returned."""
 
     def __init__(self, budget_bytes=None, reserve=0.1, admit_per_pass=2, chunk=4):
         self.admit_per_pass = admit_per_pass
@@ -189,119 +198,145 @@ class Server:
             budget_bytes = int(free * (1.0 - reserve))
         self.budget = budget_bytes
         self.used = 0
-        self.batches = {}   # batch class (grid_class) -> Batch
-        self.members = {}   # batch class -> list of [request, model, steps taken, start]
         self.pool = {}      # batch class -> idle models
+        self._mem = threading.Condition()  # guards used and pool
 
-    def _fits(self, r):
-        if self.pool.get(r.cls):
+    def _reserve(self, r):
+        """Claim the memory of request r (an idle model of its class, or new
+        footprint, releasing idle models of other classes if needed)."""
+        with self._mem:
+            idle = self.pool.get(r.cls)
+            if idle:
+                r._model = idle.pop()  # taken here: other classes release only idle models
+                return True
+            fp = _footprint(r.spec)
+            need = self.used + fp - self.budget
+            if need > 0:
+                for cls, idle in self.pool.items():
+                    while idle and need > 0 and cls != r.cls:
+                        m = idle.pop()
+                        m.close()
+                        f = footprint(*cls[:3])
+                        self.used -= f
+                        need -= f
+            if need > 0:
+                return False
+            self.used += fp
+            r._model = None
             return True
-        need = self.used + _footprint(r.spec) - self.budget
-        if need > 0:  # release idle models of other classes
-            for cls, idle in self.pool.items():
-                while idle and need > 0:
-                    m = idle.pop()
-                    m.close()
-                    fp = footprint(*cls[:3])
-                    self.used -= fp
-                    need -= fp
-        return need <= 0
 
     def _admit(self, r, now):
-        cls = r.cls
-        idle = self.pool.get(cls)
-        if idle:
-            m = idle.pop()
+        """Set up request r's model (the pooled one _reserve took, or a new
+        one) and return it."""
+        m = getattr(r, "_model", None)
+        r._model = None
+        if m is not None:
             m.reset()
             workloads.configure(m, r.spec)
         else:
             m = workloads.build(r.spec)
-            self.used += _footprint(r.spec)
-        b = self.batches.get(cls)
-        if b is None:
-            b = self.batches[cls] = Batch()
-            self.members[cls] = []
-        b.add(m)
-        self.members[cls].append([r, m, 0, now])
+        return m
+
+    def _release(self, cls, m):
+        with self._mem:
+            self.pool.setdefault(cls, []).append(m)
+            self._mem.notify_all()
+
+    def _worker(self, cls, inbox, results, t0, realtime):
+        batch = Batch()
+        members = []  # [request, model, steps taken, start]
+        waiting = inbox["queue"]
+        try:
+            while True:
+                # admission in arrival order, a few per pass, while the
+                # class has room and the memory fits
+                admitted = 0
+                while admitted < self.admit_per_pass and len(members) < batch_cap(cls):
+                    with inbox["cv"]:
+                        r = waiting[0] if waiting else None
+                    if r is None or not self._reserve(r):
+                        break
+                    with inbox["cv"]:
+                        waiting.pop(0)
+                    start = time.perf_counter() - t0
+                    m = self._admit(r, start)
+                    batch.add(m)
+                    members.append([r, m, 0, start])
+                    admitted += 1
+                if not members:
+                    with inbox["cv"]:
+                        if not waiting and inbox["closed"]:
+                            return
+                        if not waiting:
+                            inbox["cv"].wait(0.05)
+                            continue
+                    # a request waits for memory held by other classes
+                    with self._mem:
+                        self._mem.wait(0.005)
+                    continue
+                # up to `chunk` steps at once (no member past its last
+                # step), one while requests wait for the class
+                with inbox["cv"]:
+                    more = bool(waiting)
+                nst = 1 if more else min(self.chunk, min(r.nsteps - k for r, _, k, _ in members))
+                for _ in range(nst):
+                    batch.step(info=False)
+                for e in members:
+                    e[2] += nst
+                recs = batch.scalars()  # waits for the steps
+                keep = []
+                for (r, m, k, start), rec in zip(members, recs):
+                    if k >= r.nsteps or rec["status"]:
+                        batch.remove(m)
+                        surf = _surface(m, r.spec)
+                        self._release(cls, m)
+                        res = Result(r.rid, r.arrival if realtime else 0.0, start, time.perf_counter() - t0, k,
+                                     rec["t"], rec["status"], surf)
+                        with inbox["cv"]:
+                            results.append(res)
+                    else:
+                        keep.append([r, m, k, start])
+                members = keep
+        except BaseException as e:  # surfaces in run()
+            inbox["error"] = e
+        finally:
+            batch.close()
 
     def run(self, requests, realtime=True, poll=2e-4):
         pending = sorted(requests, key=lambda r: r.arrival)
-        queue = []
+        inboxes = {}
+        threads = []
         results = []
         t0 = time.perf_counter()
-        nreq = len(pending)
-        while len(results) < nreq:
+        for r in pending:
             now = time.perf_counter() - t0
-            while pending and (not realtime or pending[0].arrival <= now):
-                queue.append(pending.pop(0))
-            worked = False
-            # classes whose previous step is done: retire finished members
-            ready = set()
-            for gi, b in self.batches.items():
-                if b.busy():
-                    continue
-                ready.add(gi)
-                mem = self.members[gi]
-                if not mem:
-                    continue
-                recs = b.scalars()
-                keep = []
-                for (r, m, k, start), rec in zip(mem, recs):
-                    if k >= r.nsteps or rec["status"]:
-                        b.remove(m)
-                        surf = _surface(m, r.spec)
-                        self.pool.setdefault(gi, []).append(m)
-                        results.append(Result(r.rid, r.arrival if realtime else 0.0, start,
-                                              time.perf_counter() - t0, k, rec["t"], rec["status"], surf))
-                        worked = True
-                    else:
-                        keep.append([r, m, k, start])
-                self.members[gi] = keep
-            # admission in arrival order while the footprint fits, into idle
-            # classes only (a busy class takes its arrivals after its step);
-            # a few per pass, so that set-up on the host overlaps the steps
-            admitted = 0
-            i = 0
-            while i < len(queue) and admitted < self.admit_per_pass:
-                r = queue[i]
-                cls = r.cls
-                if (cls in self.batches and cls not in ready) or \
-                        len(self.members.get(cls, ())) >= batch_cap(cls):
-                    i += 1
-                    continue
-                if not self._fits(r):
-                    break
-                queue.pop(i)
-                self._admit(r, time.perf_counter() - t0)
-                ready.add(cls)
-                admitted += 1
-                worked = True
-            # next steps of every idle class: up to `chunk` steps at once (no
-            # member past its last step), one when requests wait for the class
-            waiting = {r.cls for r in queue}
-            for gi in ready:
-                mem = self.members[gi]
-                if mem:
-                    nst = 1 if gi in waiting else min(self.chunk, min(r.nsteps - k for r, _, k, _ in mem))
-                    for _ in range(nst):
-                        self.batches[gi].step(info=False)
-                    for e in mem:
-                        e[2] += nst
-                    worked = True
-            if not worked:
-                if not any(self.members.values()) and pending and realtime:
-                    time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.05)))
-                else:
-                    time.sleep(poll)
-        for b in self.batches.values():
-            b.close()
-        self.batches.clear()
-        for idle in self.pool.values():
-            for m in idle:
-                m.close()
-        self.pool.clear()
-        self.used = 0
-        return results
+            if realtime and r.arrival > now:
+                time.sleep(r.arrival - now)
+            box = inboxes.get(r.cls)
+            if box is None:
+                box = inboxes[r.cls] = {"queue": [], "cv": threading.Condition(), "closed": False, "error": None}
+                th = threading.Thread(target=self._worker, args=(r.cls, box, results, t0, realtime), daemon=True)
+                threads.append(th)
+                th.start()
+            with box["cv"]:
+                box["queue"].append(r)
+                box["cv"].notify()
+        for box in inboxes.values():
+            with box["cv"]:
+                box["closed"] = True
+                box["cv"].notify()
+        for th in threads:
+            th.join()
+        with self._mem:
+            for idle in self.pool.values():
+                for m in idle:
+                    m.close()
+            self.pool.clear()
+            self.used = 0
+        for box in inboxes.values():
+            if box["error"] is not None:
+                raise box["error"]
+        return sorted(results, key=lambda x: x.finish)
 
 
 def summarize(results, label):

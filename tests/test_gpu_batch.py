@@ -153,3 +153,40 @@ def test_batch_with_transient_member():
             nr = np.linalg.norm(r)
             assert np.linalg.norm(a - r) <= 1e-6 * (nr if nr > 0 else 1.0), (i, key)
         m.close()
+
+
+def test_c3_batch_of_64_models_of_128_cubed():
+    """The batched workload at its size: 64 models of 128^3 (c3: 1-10
+    patches, log-uniform viscosity, n in {1, 2, 3, 3.5}, 15-40 km lid), 10
+    steps as one batch.  Every member equals its single run bitwise, and one
+    member's 10-step trajectory follows the oracle (relative L2 1e-4)."""
+    import oracle as O
+    from _cases import oracle_initial_state, oracle_model, rel_l2
+
+    nsteps = 10
+    specs = [workloads.spec("c3%d" % i) for i in range(64)]
+    models = [workloads.build(s) for s in specs]
+    with Batch(models) as b:
+        for _ in range(nsteps):
+            b.step(info=False)
+        recs = b.scalars()
+    outs = [m.download() for m in models]
+    for m in models:
+        m.close()
+    for i in range(0, 64, 9):  # a spread of members against their single runs
+        r, out, _ = _alone(specs[i], nsteps)
+        assert recs[i] == r[-1], i
+        _same(outs[i], out)
+    # one member against the oracle over the 10 steps
+    i = 7
+    s = specs[i]
+    print("c3 member", i, "n", (s["linear"] or s["nonlinear"])[0][2], flush=True)
+    st = oracle_initial_state(s)
+    info = O.rk2_run(oracle_model(s), st, nsteps)
+    # the adaptive step sequence (the relaxation time is an FP32 minimum)
+    assert abs(recs[i]["t"] - info[-1, 7]) <= 1e-5 * abs(info[-1, 7])
+    n1 = s["sx1"]
+    for k in ("u1", "u2", "u3"):
+        assert rel_l2(outs[i][k][..., :n1], st[k][..., :n1]) < 1e-4, k
+    for k in ("sig", "tau"):
+        assert rel_l2(outs[i][k], st[k]) < 1e-4, k

@@ -44,8 +44,9 @@ def test_server_respects_the_memory_budget():
     admit = srv._admit
 
     def tracked(r, now):
-        admit(r, now)
+        m = admit(r, now)
         peak.append(srv.used)
+        return m
 
     srv._admit = tracked
     res = srv.run(reqs, realtime=False)
@@ -77,3 +78,30 @@ def test_reset_model_matches_a_new_one():
     m.close()
     for k in ("u1", "u2", "u3", "sig", "tau", "gamma"):
         np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+
+
+def test_c4_stream_with_every_grid_class():
+    """The serving stream at its sizes: Poisson arrivals over the three grid
+    classes (128x128x64, 256x256x128 and 512x512x256) with 10-30 steps and
+    afterslip requests; every served result equals the one it gets alone."""
+    reqs = serve.request_stream(16, seed=3, steps=(10, 30))
+    classes = {r.grid for r in reqs}
+    if 2 not in classes:  # make sure the 512x512x256 class is present
+        extra = serve.request_stream(40, seed=4, steps=(10, 30))
+        big = next(r for r in extra if r.grid == 2)
+        big.rid = len(reqs)
+        reqs.append(big)
+    assert {r.grid for r in reqs} >= {0, 1, 2}
+    alone = {r.rid: r for r in serve.run_back_to_back(reqs, realtime=False)}
+    mixed = {r.rid: r for r in serve.Server().run(reqs, realtime=False)}
+    assert sorted(alone) == sorted(mixed) == sorted(r.rid for r in reqs)
+    for q in reqs:
+        a, b = alone[q.rid], mixed[q.rid]
+        assert a.status == b.status, q.rid
+        if a.status:
+            continue
+        assert b.nsteps == q.nsteps
+        assert abs(a.t - b.t) <= 1e-9 * abs(a.t), q.rid
+        for ua, ub in zip(a.surface, b.surface):
+            na = np.linalg.norm(ua.astype(np.float64))
+            assert np.linalg.norm(ua.astype(np.float64) - ub) <= 1e-5 * max(na, 1e-30), q.rid
