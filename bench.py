@@ -103,7 +103,8 @@ def batched_c3(nmodels=64, nsteps=10):
     viscosity and stress exponent), 10 RK2 steps each: one batch of shared
     launches replayed from a hipGraph, against the same models stepped back to
     back through the single-model path (no host synchronisation between
-    steps).  A solve is the 10 steps of one model from its coseismic state."""
+    steps), each path with the launch shapes its own autotuner picks.  A solve
+    is the 10 steps of one model from its coseismic state."""
     import numpy as np
 
     from relax_amd import workloads
@@ -114,8 +115,11 @@ def batched_c3(nmodels=64, nsteps=10):
     single = [workloads.build(s) for s in specs]
     setup = (time.perf_counter() - t0) / nmodels
     batched = [workloads.build(s) for s in specs]
-    # warm-up of both paths on throw-away models of the same grid
+    # warm-up of both paths on throw-away models of the same grid; each path
+    # gets the launch shapes its autotuner picks (one model, or one launch
+    # over all 64 members)
     w = workloads.build(specs[0])
+    tune_single = w.autotune()
     w.step(info=False)
     w.synchronize()
     wb = Batch([workloads.build(specs[1]), workloads.build(specs[2])])
@@ -124,6 +128,8 @@ def batched_c3(nmodels=64, nsteps=10):
     for m in wb.models + [w]:
         m.close()
     wb.close()
+    for m in single:
+        m.set_tuning(**tune_single)
     t0 = time.perf_counter()
     for m in single:
         for _ in range(nsteps):
@@ -132,6 +138,7 @@ def batched_c3(nmodels=64, nsteps=10):
         m.synchronize()
     t_single = time.perf_counter() - t0
     b = Batch(batched)
+    tune_batch = b.autotune()
     b.step(info=False)  # capture (not timed); one step more than the single path
     b.synchronize()
     for m in single:  # bring the single path level
@@ -143,10 +150,14 @@ def batched_c3(nmodels=64, nsteps=10):
         b.step(info=False)
     b.synchronize()
     t_batch = time.perf_counter() - t0
-    same = True
+    same, diff = True, 0.0
     for k in (0, nmodels // 3, nmodels - 1):
         a, c = single[k].download(), batched[k].download()
-        same = same and all(np.array_equal(a[f], c[f]) for f in ("u1", "u2", "u3", "sig", "tau", "gamma"))
+        for f in ("u1", "u2", "u3", "sig", "tau", "gamma"):
+            same = same and np.array_equal(a[f], c[f])
+            ref = np.linalg.norm(a[f].astype(np.float64))
+            if ref > 0:
+                diff = max(diff, float(np.linalg.norm(c[f].astype(np.float64) - a[f]) / ref))
     b.close()
     for m in single + batched:
         m.close()
@@ -155,7 +166,11 @@ def batched_c3(nmodels=64, nsteps=10):
             "single_path_solves_per_s": round(nmodels / t_single, 2),
             "speedup": round(t_single / t_batch, 2),
             "setup_s_per_model": round(setup, 4),
-            "identical_to_single_path": bool(same)}
+            "launch_shapes": {"single_path": tune_single, "batch": tune_batch},
+            # bitwise when both paths use the same shapes; the Green variants
+            # differ from each other in FP32 rounding only
+            "identical_to_single_path": bool(same),
+            "max_rel_l2_to_single_path": diff}
 
 
 def serving_c4(nreq=60, rate=20.0):
@@ -170,6 +185,7 @@ def serving_c4(nreq=60, rate=20.0):
     warm = serve.request_stream(6, seed=1, steps=(2, 3))
     serve.run_back_to_back(warm, realtime=False)
     serve.Server().run(warm, realtime=False)
+    tuned = serve.tune()  # launch shapes per grid class: single models and full batches
     reqs = serve.request_stream(nreq, rate=rate)
     cap_single = serve.summarize(serve.run_back_to_back(reqs, realtime=False), "single-model path, back to back")
     cap_served = serve.summarize(serve.Server().run(reqs, realtime=False), "continuous batching")
@@ -182,6 +198,8 @@ def serving_c4(nreq=60, rate=20.0):
             "speedup": round(cap_served["solves_per_s"] / cap_single["solves_per_s"], 2),
             "p50_s": lat_served["p50_s"], "p99_s": lat_served["p99_s"],
             "single_path_p50_s": lat_single["p50_s"], "single_path_p99_s": lat_single["p99_s"],
+            "launch_shapes": {"%s %s" % ("x".join(map(str, serve.GRIDS[gi][:3])), kind): [
+                c["fft_cols"], c["green"], c["bodyforce"]] for (gi, kind), c in tuned.items()},
             "capacity": [cap_single, cap_served], "at_arrival_rate": [lat_single, lat_served]}
 
 

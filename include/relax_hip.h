@@ -411,7 +411,7 @@ int relax_model_points(relax_model_t* m, const double* x, int n, double rot, dou
  * milliseconds, 3 rows of RELAX_TUNE_VARIANTS (columns, Green, divergence),
  * with -1 for a variant that does not apply to the grid.  The model's state
  * is not changed.  Not on a member of a batch. */
-#define RELAX_TUNE_VARIANTS 4
+#define RELAX_TUNE_VARIANTS 6
 int relax_model_autotune(relax_model_t* m, int* choice, float* ms);
 /* Sets the three variants (as relax_model_autotune reports them) without
  * timing, e.g. a choice kept from an earlier run; RELAX_ERR_ARG when out of
@@ -425,7 +425,8 @@ void* relax_model_stream(relax_model_t* m);
 /* A batch steps all of its models together: every kernel of the RK2 step is
  * one launch over all members (the model number is part of the launch grid),
  * and the step is replayed from a captured hipGraph while the membership is
- * unchanged.  Members must share the device, the grid (sx1, sx2, sx3), the
+ * unchanged (batches of 2^25 grid points or more run their launches eagerly:
+ * there a graph saves nothing).  Members must share the device, the grid (sx1, sx2, sx3), the
  * spacing (dx1, dx2, dx3) and lambda, mu, gamma; rheology, prestress, fault
  * planes, time line and state are per model.  There is no counterpart in the
  * reference, whose driver runs one model per process (relax.f90:185-1454);
@@ -451,6 +452,14 @@ int relax_batch_synchronize(relax_batch_t* b);
 int relax_batch_busy(relax_batch_t* b);
 /* Each member's current scalars (waits for the queued work). */
 int relax_batch_scalars(relax_batch_t* b, relax_step_info_t* infos);
+/* Autotuner of a batch: as relax_model_autotune, but each timing is one
+ * launch over all current members (their scratch buffers), so the choice
+ * fits the batch's size; the batch keeps it, and later batches of the same
+ * device, grid and rheology class in the process start from it.  A batch
+ * without a choice launches with its first member's shapes.  Members equal
+ * their single runs bitwise when those use the same shapes.  No counterpart
+ * in the reference (it has no batches). */
+int relax_batch_autotune(relax_batch_t* b, int* choice, float* ms);
 
 #ifdef __cplusplus
 }

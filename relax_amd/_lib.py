@@ -140,6 +140,7 @@ SIGNATURES = {
     "relax_batch_synchronize": (c_int, [P]),
     "relax_batch_busy": (c_int, [P]),
     "relax_batch_scalars": (c_int, [P, P]),
+    "relax_batch_autotune": (c_int, [P, P, P]),
     "relax_model_surface": (c_int, [P, P, P, P, c_int]),
     "relax_model_points": (c_int, [P, P, c_int, c_double, P, P]),
 }

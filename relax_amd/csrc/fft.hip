@@ -62,10 +62,24 @@ const float2* twiddle_table(int L) {
 // to row j1 + L2*k2.  The L2-point transforms of the blocks of L2 rows then
 // finish the y transform (k_fft_cols), leaving y frequency 8*k1 + k2 at row
 // L2*k2 + k1 of the plane.  The Nyquist column keeps natural row order.
+// complex samples of the rows a workgroup transforms together
+#ifndef RELAX_X_ELEMS
+#define RELAX_X_ELEMS 2048
+#endif
+// threads of the x transforms
+#ifndef RELAX_X_NT
+#define RELAX_X_NT 256
+#endif
+// LDS padding of the x transforms: one spare element after every
+// 2^RELAX_X_PAD (0: none; lpad of fft_lds.h)
+#ifndef RELAX_X_PAD
+#define RELAX_X_PAD 3
+#endif
+
 template <int N1, bool YS>
 struct XRows {
   static constexpr int M = N1 / 2;
-  static constexpr int RPB0 = (2048 / M) > 0 ? 2048 / M : 1;
+  static constexpr int RPB0 = (RELAX_X_ELEMS / M) > 0 ? RELAX_X_ELEMS / M : 1;
   static constexpr int RPB = YS ? (RPB0 < 8 ? 8 : RPB0) : RPB0;  // rows per workgroup
   static constexpr int G = YS ? RPB / 8 : 1;                       // groups of 8 rows
 };
@@ -83,7 +97,8 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
   constexpr int RPB = XRows<N1, YS>::RPB, G = XRows<N1, YS>::G;
   constexpr int PM = M + 1;  // row pitch in LDS (breaks bank aliasing between rows)
   constexpr int EPT = (RPB * M + NT - 1) / NT;
-  __shared__ float2 s[RPB * PM];
+  constexpr int XP = RELAX_X_PAD;
+  __shared__ float2 s[lpad<XP>(RPB * PM) + 1];
   __shared__ float2 stw[N1];  // twiddles in LDS: off the global-latency path of every stage
   __shared__ float2 ytw[G][8];
   const int tid = threadIdx.x;
@@ -104,18 +119,18 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
     const long row = rowof(b);
     float2 v = make_float2(0.f, 0.f);
     if (row < nrows) v = gload(reinterpret_cast<const float2*>(in + row * in_pitch + 2 * j));
-    s[b * PM + j] = v;
+    s[lpad<XP>(b * PM + j)] = v;
   }
   __syncthreads();
-  lds_fft<M, RPB, NT, RPB, PM, 0, 1, -1, 2, false, RELAX_X_RMAX>(s, stw, tid);
+  lds_fft<M, RPB, NT, RPB, PM, 0, 1, -1, 2, false, RELAX_X_RMAX, XP>(s, stw, tid);
   float2 X[EPT];
 #pragma unroll
   for (int i = 0; i < EPT; ++i) {
     const int e = tid + i * NT;
     if (e >= RPB * M) break;
     const int b = e / M, k = e % M;
-    const float2 zk = s[b * PM + k];
-    const float2 zr = cconj(s[b * PM + ((M - k) & (M - 1))]);
+    const float2 zk = s[lpad<XP>(b * PM + k)];
+    const float2 zr = cconj(s[lpad<XP>(b * PM + ((M - k) & (M - 1)))]);
     const float2 w = stw[k];
     const float2 sum = cadd(zk, zr), dif = csub(zk, zr);
     // X[k] = (Z[k] + conj Z[M-k])/2 - i w^k (Z[k] - conj Z[M-k])/2
@@ -131,7 +146,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
   for (int i = 0; i < EPT; ++i) {
     const int e = tid + i * NT;
     if (e >= RPB * M) break;
-    s[(e / M) * PM + (e % M)] = X[i];
+    s[lpad<XP>((e / M) * PM + (e % M))] = X[i];
   }
   __syncthreads();
   // first stage of the y transform, one column of one group per item
@@ -141,7 +156,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
     if (q * 8 >= nrows) continue;
     float2 a[8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) a[t] = s[(8 * gi + t) * PM + k];
+    for (int t = 0; t < 8; ++t) a[t] = s[lpad<XP>((8 * gi + t) * PM + k)];
     dft8<-1>(a);
     const long base_row = (q / L2) * n2 + (q % L2);
 #pragma unroll
@@ -168,7 +183,8 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long 
   constexpr int RPB = XRows<N1, YS>::RPB, G = XRows<N1, YS>::G;
   constexpr int PM = M + 1;
   constexpr int EPT = (RPB * M + NT - 1) / NT;
-  __shared__ float2 s[RPB * PM];
+  constexpr int XP = RELAX_X_PAD;
+  __shared__ float2 s[lpad<XP>(RPB * PM) + 1];
   __shared__ float2 xm[RPB];
   __shared__ float2 stw[N1];
   __shared__ float2 ytw[G][8];
@@ -190,11 +206,25 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long 
     float2 v = make_float2(0.f, 0.f);
     if (row < nrows) v = gload(in + row * in_pitch + j);
     if (!YS && j == 0) v.y = 0.f;  // a real output takes the real part of the k1=0 bin
-    s[b * PM + j] = v;
+    s[lpad<XP>(b * PM + j)] = v;
   }
   for (int b = tid; b < RPB; b += NT) {
     const long row = rowof(b);
     xm[b] = row < nrows ? make_float2(gload(nyq + row * nyq_stride).x, 0.f) : make_float2(0.f, 0.f);
+  }
+  // the epilogue's base values (the same elements this thread writes), loaded
+  // now so that their latency overlaps the transform
+  float2 ub[EPI != EPI_PLAIN ? EPT : 1];
+  if (EPI != EPI_PLAIN) {
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+      const int e = tid + i * NT;
+      ub[i] = make_float2(0.f, 0.f);
+      if (e < RPB * M) {
+        const long row = rowof(e / M);
+        if (row < nrows) ub[i] = gload(reinterpret_cast<const float2*>(base + row * base_pitch + 2 * (e % M)));
+      }
+    }
   }
   __syncthreads();
   if (YS) {
@@ -205,7 +235,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long 
       float2 a[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
-        const float2 v = s[(8 * gi + t) * PM + j];
+        const float2 v = s[lpad<XP>((8 * gi + t) * PM + j)];
         a[t] = t ? cmul(v, cconj(ytw[gi][t])) : v;
       }
       dft8<+1>(a);
@@ -213,7 +243,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long 
       for (int t = 0; t < 8; ++t) {
         float2 v = a[t];
         if (j == 0) v.y = 0.f;  // a real output takes the real part of the k1=0 bin
-        s[(8 * gi + t) * PM + j] = v;
+        s[lpad<XP>((8 * gi + t) * PM + j)] = v;
       }
     }
     __syncthreads();
@@ -224,8 +254,8 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long 
     const int e = tid + i * NT;
     if (e >= RPB * M) break;
     const int b = e / M, k = e % M;
-    const float2 xk = s[b * PM + k];
-    const float2 xr = cconj(k == 0 ? xm[b] : s[b * PM + (M - k)]);
+    const float2 xk = s[lpad<XP>(b * PM + k)];
+    const float2 xr = cconj(k == 0 ? xm[b] : s[lpad<XP>(b * PM + (M - k))]);
     const float2 w = cconj(stw[k]);  // exp(+2 pi i k / N1)
     // Z'[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k])
     z[i] = cadd(cadd(xk, xr), mul_i(cmul(w, csub(xk, xr))));
@@ -235,23 +265,25 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long 
   for (int i = 0; i < EPT; ++i) {
     const int e = tid + i * NT;
     if (e >= RPB * M) break;
-    s[(e / M) * PM + (e % M)] = z[i];
+    s[lpad<XP>((e / M) * PM + (e % M))] = z[i];
   }
   __syncthreads();
-  lds_fft<M, RPB, NT, RPB, PM, 0, 1, +1, 2, false, RELAX_X_RMAX>(s, stw, tid);
+  lds_fft<M, RPB, NT, RPB, PM, 0, 1, +1, 2, false, RELAX_X_RMAX, XP>(s, stw, tid);
   float coef = 1.f;
   if (EPI == EPI_PRED) coef = (float)(sc->Dt / 2);
   if (EPI == EPI_CORR) coef = (float)sc->Dt;
-  for (int e = tid; e < RPB * M; e += NT) {
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) {
+    const int e = tid + i * NT;
+    if (e >= RPB * M) break;
     const int b = e / M, j = e % M;
     const long row = rowof(b);
     if (row >= nrows) continue;
-    const float2 v = s[b * PM + j];
+    const float2 v = s[lpad<XP>(b * PM + j)];
     float a0 = v.x * scale, a1 = v.y * scale;
     if (EPI != EPI_PLAIN) {
-      const float2 u = gload(reinterpret_cast<const float2*>(base + row * base_pitch + 2 * j));
-      a0 = coef * a0 + u.x;
-      a1 = coef * a1 + u.y;
+      a0 = coef * a0 + ub[i].x;
+      a1 = coef * a1 + ub[i].y;
     }
     gstore(reinterpret_cast<float2*>(out + row * out_pitch + 2 * j), make_float2(a0, a1));
   }
@@ -342,12 +374,12 @@ int launch_fft_x_r2c(int n1, MPtr<const float> in, long in_pitch, MPtr<float2> o
     if (ys_n2) {                                                                                            \
       constexpr int G = XRows<N, true>::G;                                                                  \
       nb = (nrows / 8 + G - 1) / G;                                                                         \
-      k_fft_x_r2c<N, NT_FFT, true><<<dim3((unsigned)nb, (unsigned)nm), NT_FFT, 0, st>>>(                    \
+      k_fft_x_r2c<N, RELAX_X_NT, true><<<dim3((unsigned)nb, (unsigned)nm), RELAX_X_NT, 0, st>>>(                    \
           in, in_pitch, out, out_pitch, nyq, nyq_stride, nrows, scale, tw, ys_n2, twy);                     \
     } else {                                                                                                \
       constexpr int RPB = XRows<N, false>::RPB;                                                             \
       nb = (nrows + RPB - 1) / RPB;                                                                         \
-      k_fft_x_r2c<N, NT_FFT, false><<<dim3((unsigned)nb, (unsigned)nm), NT_FFT, 0, st>>>(                   \
+      k_fft_x_r2c<N, RELAX_X_NT, false><<<dim3((unsigned)nb, (unsigned)nm), RELAX_X_NT, 0, st>>>(                   \
           in, in_pitch, out, out_pitch, nyq, nyq_stride, nrows, scale, tw, 0, nullptr);                     \
     }                                                                                                       \
     break;
@@ -373,7 +405,7 @@ int launch_fft_x_c2r(int n1, int epi, MPtr<const float2> in, long in_pitch, MPtr
   {                                                                                                         \
     constexpr int RPB = XRows<N, YS>::RPB, G = XRows<N, YS>::G;                                             \
     const long nb = YS ? (nrows / 8 + G - 1) / G : (nrows + RPB - 1) / RPB;                                 \
-    k_fft_x_c2r<N, NT_FFT, E, YS><<<dim3((unsigned)nb, (unsigned)nm), NT_FFT, 0, st>>>(                     \
+    k_fft_x_c2r<N, RELAX_X_NT, E, YS><<<dim3((unsigned)nb, (unsigned)nm), RELAX_X_NT, 0, st>>>(                     \
         in, in_pitch, nyq, nyq_stride, out, out_pitch, nrows, scale, tw, base, base_pitch, sc, ys_n2, twy); \
   }
 #define X(N, E) case N: if (ys_n2) LAUNCH(N, E, true) else LAUNCH(N, E, false) break;

@@ -7,13 +7,19 @@
 // butterflies into registers, the workgroup synchronises, then every thread
 // writes its outputs.
 //
-// Element j of batch b lives at  buf[(b % B0) * S0 + (b / B0) * S1 + j * JS].
+// Element j of batch b lives at  buf[(b % B0) * S0 + (b / B0) * S1 + j * JS],
+// or, with PAD > 0, at buf[lpad<PAD>(...)] of that index: one spare element
+// after every 2^PAD, which spreads the strided writes of a Stockham stage
+// (element stride R within a batch) over the LDS banks.
 // Twiddles come from a table tw[x] = exp(-2 pi i x / (L * TWS)) in global
 // memory; SIGN = -1 is the forward transform, +1 the inverse (unnormalised).
 #pragma once
 #include "common.h"
 
 __host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
+
+template <int PAD>
+__host__ __device__ __forceinline__ constexpr int lpad(int i) { return PAD ? i + (i >> PAD) : i; }
 
 template <int SIGN>
 __device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int idx) {
@@ -99,7 +105,8 @@ __device__ __forceinline__ void dft(float2* a) {
 }
 
 // One Stockham stage of radix R on sub-transforms of length N (stride L/N).
-template <int L, int N, int R, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB>
+template <int L, int N, int R, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB,
+          int PAD = 0>
 __device__ __forceinline__ void lds_stage(float2* buf, const float2* __restrict__ tw, int tid) {
   constexpr int S = L / N;   // interleaved sub-transforms
   constexpr int M = N / R;   // length of the outputs' sub-transforms
@@ -119,7 +126,7 @@ __device__ __forceinline__ void lds_stage(float2* buf, const float2* __restrict_
       const int o = (b % B0) * S0 + (b / B0) * S1;
       ob[k] = o; pp[k] = p; qq[k] = q;
 #pragma unroll
-      for (int t = 0; t < R; ++t) a[k][t] = buf[o + (q + S * (p + t * M)) * JS];
+      for (int t = 0; t < R; ++t) a[k][t] = buf[lpad<PAD>(o + (q + S * (p + t * M)) * JS)];
     }
   }
   __syncthreads();
@@ -134,28 +141,30 @@ __device__ __forceinline__ void lds_stage(float2* buf, const float2* __restrict_
       for (int t = 1; t < R; ++t) a[k][t] = cmul(a[k][t], twiddle<SIGN>(tw, p * t * step));
     }
 #pragma unroll
-    for (int t = 0; t < R; ++t) buf[o + (q + S * (R * p + t)) * JS] = a[k][t];
+    for (int t = 0; t < R; ++t) buf[lpad<PAD>(o + (q + S * (R * p + t)) * JS)] = a[k][t];
   }
   __syncthreads();
 }
 
-template <int L, int N, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB, int RMAX>
+template <int L, int N, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB, int RMAX,
+          int PAD>
 struct LdsStages {
   static __device__ __forceinline__ void run(float2* buf, const float2* __restrict__ tw, int tid) {
     constexpr int R = N >= RMAX ? RMAX : N;
-    lds_stage<L, N, R, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB>(buf, tw, tid);
-    LdsStages<L, N / R, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB, RMAX>::run(buf, tw, tid);
+    lds_stage<L, N, R, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB, PAD>(buf, tw, tid);
+    LdsStages<L, N / R, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB, RMAX, PAD>::run(buf, tw, tid);
   }
 };
-template <int L, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB, int RMAX>
-struct LdsStages<L, 1, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB, RMAX> {
+template <int L, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB, int RMAX, int PAD>
+struct LdsStages<L, 1, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB, RMAX, PAD> {
   static __device__ __forceinline__ void run(float2*, const float2* __restrict__, int) {}
 };
 
 // MAPB: consecutive threads take consecutive batches (use when S0 == 1);
 // otherwise consecutive threads take consecutive butterflies of one batch.
 // RMAX: the largest radix (8, or 16 for fewer stages and LDS passes).
-template <int L, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB, int RMAX = 8>
+template <int L, int NB, int NT, int B0, int S0, int S1, int JS, int SIGN, int TWS, bool MAPB, int RMAX = 8,
+          int PAD = 0>
 __device__ __forceinline__ void lds_fft(float2* buf, const float2* __restrict__ tw, int tid) {
-  LdsStages<L, L, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB, RMAX>::run(buf, tw, tid);
+  LdsStages<L, L, NB, NT, B0, S0, S1, JS, SIGN, TWS, MAPB, RMAX, PAD>::run(buf, tw, tid);
 }

@@ -21,7 +21,7 @@ int launch_fft_x_c2r(int n1, int epi, MPtr<const float2> in, long in_pitch, MPtr
                      MPtr<const float> base, long base_pitch, MPtr<const StepScalars> sc, int nm, hipStream_t st,
                      int ys_n2 = 0);
 // launch-shape variants of the tunable kernels (0 = default)
-constexpr int COLS_VARIANTS = 3, GREEN_VARIANTS = 4, BODY_VARIANTS = 4;
+constexpr int COLS_VARIANTS = 3, GREEN_VARIANTS = 4, BODY_VARIANTS = 6;
 
 int launch_fft_cols(int L, int sign, MPtr<float2> data, long jstride, long pstride, int ncols, int nplanes, int nm,
                     hipStream_t st, int variant = 0);

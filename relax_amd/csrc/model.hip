@@ -977,12 +977,14 @@ struct Ctx {
   hipStream_t st;
   relax_model_t* const* models;   // host handles, for the per-plane kernels
   relax_model_t* prof;            // phase timers (single model), or nullptr
+  int tune_cols, tune_green, tune_body;  // launch-shape variants
 };
 
 static Ctx single(relax_model_t* m) {
   Ctx c;
   c.lead = m; c.md = m->md_dev; c.one = &m->md_host; c.nm = 1; c.rclass = rheo_class(m);
   c.st = m->stream; c.models = &m->self; c.prof = m;
+  c.tune_cols = m->tune_cols; c.tune_green = m->tune_green; c.tune_body = m->tune_body;
   return c;
 }
 
@@ -1014,7 +1016,7 @@ static int model_green(const Ctx& c, int epi) {
                              3L * n2, (float)(m->desc.dx1 * m->desc.dx2), nm, st)))
     return rc;
   cmark(c, RELAX_PROF_SMALL);
-  if ((rc = launch_fft_cols(ly, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st, m->tune_cols)))
+  if ((rc = launch_fft_cols(ly, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st, c.tune_cols)))
     return rc;
   cmark(c, RELAX_PROF_FFT_Y);
   if ((rc = launch_fft_cols(n2, -1, MFIELD(float2, md, T), m1, (long)n2 * m1, m1, 3, nm, st))) return rc;
@@ -1024,10 +1026,10 @@ static int model_green(const Ctx& c, int epi) {
   if ((rc = launch_fft_rows(n2, -1, MFIELD(float2, md, TN), n2, 3, nm, st))) return rc;
   cmark(c, RELAX_PROF_SMALL);
   if ((rc = launch_green_columns(m->gc, h3, MFIELD(float2, md, W), MFIELD(float2, md, NQ), MFIELD(const float2, md, T),
-                                 MFIELD(const float2, md, TN), nm, st, l2, m->tune_green)))
+                                 MFIELD(const float2, md, TN), nm, st, l2, c.tune_green)))
     return rc;
   cmark(c, RELAX_PROF_GREEN_Z);
-  if ((rc = launch_fft_cols(ly, +1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st, m->tune_cols)))
+  if ((rc = launch_fft_cols(ly, +1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st, c.tune_cols)))
     return rc;
   cmark(c, RELAX_PROF_FFT_Y);
   if ((rc = launch_fft_rows(n2, +1, MFIELD(float2, md, NQ), n2, 3L * h3, nm, st))) return rc;
@@ -1068,7 +1070,7 @@ static int enqueue_steps(const Ctx& c, const double* dt_override) {
   int rc;
   cmark(c, -1);
   // predictor
-  if ((rc = launch_bodyforce(g, S, c.md, c.one, nm, c.rclass, BODY_PRED, st, c.lead->tune_body))) return rc;
+  if ((rc = launch_bodyforce(g, S, c.md, c.one, nm, c.rclass, BODY_PRED, st, c.tune_body))) return rc;
   cmark(c, RELAX_PROF_BODYFORCE);
   k_finalize_dt<<<(nm + 63) / 64, 64, 0, st>>>(c.md, nm, dt_override ? 1 : 0, dt_override ? *dt_override : 0.0);
   cmark(c, RELAX_PROF_SMALL);
@@ -1077,7 +1079,7 @@ static int enqueue_steps(const Ctx& c, const double* dt_override) {
   if ((rc = launch_stress(STRESS_PRED, g, S, c.md, c.one, nm, c.rclass, st))) return rc;
   cmark(c, RELAX_PROF_STRESS_PRED);
   // corrector
-  if ((rc = launch_bodyforce(g, S, c.md, c.one, nm, c.rclass, BODY_CORR, st, c.lead->tune_body))) return rc;
+  if ((rc = launch_bodyforce(g, S, c.md, c.one, nm, c.rclass, BODY_CORR, st, c.tune_body))) return rc;
   cmark(c, RELAX_PROF_BODYFORCE);
   for (int i = 0; i < nm; ++i) {
     relax_model_t* m = c.models[i];
@@ -1787,12 +1789,18 @@ struct relax_batch {
   int stable = 0;                   // steps since the last change
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  bool tuned = false;               // tune holds relax_batch_autotune's choice
+  int tune[3] = {0, 0, 0};
 };
 
 // A membership that changes every step or two (a server admitting and
 // retiring requests) runs its first steps eagerly; the step is captured as a
 // graph once the membership has been stable for this many steps.
 static const int GRAPH_AFTER = 2;
+// Batches of this many grid points or more run eagerly: their launches are
+// long enough that replaying a graph saves nothing, while a capture (after
+// every membership change) costs the host a few milliseconds.
+static const long GRAPH_MAX_POINTS = 1L << 25;
 
 static std::recursive_mutex g_capture_mu;
 void capture_lock() { g_capture_mu.lock(); }
@@ -1850,6 +1858,7 @@ int relax_batch_add(relax_batch_t* b, relax_model_t* m) {
   if (!b || !m || m->batch) return RELAX_ERR_ARG;
   if (m->device != b->device) return RELAX_ERR_ARG;
   if (!b->models.empty() && !batch_compatible(b->models[0], m)) return RELAX_ERR_ARG;
+  if (b->models.empty()) b->tuned = false;  // the grid may differ from the last members'
   int rc;
   if ((rc = sync_md(m))) return rc;
   HIP_TRY(hipStreamSynchronize(m->stream));  // the model's own work is done before the batch's
@@ -1916,19 +1925,51 @@ static int batch_refresh(relax_batch_t* b) {
   return RELAX_OK;
 }
 
+// Launch shapes of batches, per device, grid and rheology class, found by
+// relax_batch_autotune and used by every later batch of that grid and class in
+// the process.  A batch without an entry launches with its first member's
+// shapes (the single-model choice).
+struct BatchTuneKey {
+  int dev, n1, n2, n3, rclass;
+  bool operator<(const BatchTuneKey& o) const {
+    return dev != o.dev ? dev < o.dev
+           : n1 != o.n1 ? n1 < o.n1
+           : n2 != o.n2 ? n2 < o.n2
+           : n3 != o.n3 ? n3 < o.n3
+                        : rclass < o.rclass;
+  }
+};
+static std::map<BatchTuneKey, TuneChoice> g_btune;  // guarded by g_tune_mu
+
+static Ctx batch_ctx(relax_batch_t* b) {
+  Ctx c;
+  c.lead = b->models[0]; c.md = b->md_dev; c.one = nullptr; c.nm = (int)b->models.size(); c.st = b->stream;
+  c.models = b->models.data();
+  c.prof = nullptr;
+  c.rclass = RHEO_VISCOUS;
+  for (relax_model_t* m : b->models) c.rclass = std::max(c.rclass, rheo_class(m));
+  c.tune_cols = c.lead->tune_cols; c.tune_green = c.lead->tune_green; c.tune_body = c.lead->tune_body;
+  if (b->tuned) {
+    c.tune_cols = b->tune[0]; c.tune_green = b->tune[1]; c.tune_body = b->tune[2];
+  } else {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const Grid& g = c.lead->g;
+    auto it = g_btune.find(BatchTuneKey{b->device, g.n1, g.n2, g.n3, c.rclass});
+    if (it != g_btune.end()) {
+      c.tune_cols = it->second.cols; c.tune_green = it->second.green; c.tune_body = it->second.body;
+    }
+  }
+  return c;
+}
+
 int relax_batch_step(relax_batch_t* b, relax_step_info_t* infos) {
   if (!b) return RELAX_ERR_ARG;
   const int nm = (int)b->models.size();
   if (nm == 0) return RELAX_OK;
   int rc;
   if ((rc = batch_refresh(b))) return rc;
-  Ctx c;
-  c.lead = b->models[0]; c.md = b->md_dev; c.one = nullptr; c.nm = nm; c.st = b->stream;
-  c.models = b->models.data();
-  c.prof = nullptr;
-  c.rclass = RHEO_VISCOUS;
-  for (relax_model_t* m : b->models) c.rclass = std::max(c.rclass, rheo_class(m));
-  if (!b->exec && b->stable < GRAPH_AFTER) {
+  const Ctx c = batch_ctx(b);
+  if (!b->exec && (b->stable < GRAPH_AFTER || (long)nm * b->models[0]->g.H >= GRAPH_MAX_POINTS)) {
     ++b->stable;
     if ((rc = enqueue_steps(c, nullptr))) return rc;
   } else if (!b->exec) {
@@ -2004,37 +2045,38 @@ int relax_model_set_tuning(relax_model_t* m, const int* choice) {
   return RELAX_OK;
 }
 
-int relax_model_autotune(relax_model_t* m, int* choice, float* ms) {
-  if (!m || m->batch) return RELAX_ERR_ARG;
+// Times every variant of the three tunable launches of context c (one launch
+// over all of its models each) and returns the fastest in best; times (3 rows
+// of RELAX_TUNE_VARIANTS, ms per launch) holds -1 for a variant that does not
+// fit the grid.
+static int tune_time(const Ctx& c, int* best, float (*times)[RELAX_TUNE_VARIANTS]) {
   static_assert(COLS_VARIANTS <= RELAX_TUNE_VARIANTS && GREEN_VARIANTS <= RELAX_TUNE_VARIANTS &&
                     BODY_VARIANTS <= RELAX_TUNE_VARIANTS,
                 "the timing table holds every variant");
-  int rc;
-  if ((rc = sync_md(m))) return rc;
+  const relax_model_t* m = c.lead;
   const Grid& g = m->g;
-  hipStream_t st = m->stream;
-  const int n2 = g.n2, h3 = g.h3, m1 = g.m1;
+  hipStream_t st = c.st;
+  const int n2 = g.n2, h3 = g.h3, m1 = g.m1, nm = c.nm;
   const int ys = n2 >= 128 ? n2 : 0, l2 = ys ? n2 / 8 : 0;
   const int ly = ys ? l2 : n2, nyp = ys ? 3 * h3 * 8 : 3 * h3;
-  const ModelDev* md = m->md_dev;
-  const int rclass = rheo_class(m);
+  const ModelDev* md = c.md;
+  int rc;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return RELAX_ERR_HIP; }
   auto run = [&](int kind, int v) -> int {
     if (kind == 0)
-      return launch_fft_cols(ly, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, 1, st, v);
+      return launch_fft_cols(ly, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st, v);
     if (kind == 1)
       return launch_green_columns(m->gc, h3, MFIELD(float2, md, W), MFIELD(float2, md, NQ),
-                                  MFIELD(const float2, md, T), MFIELD(const float2, md, TN), 1, st, l2, v);
-    return launch_bodyforce(g, m->S, md, &m->md_host, 1, rclass, BODY_CORR, st, v);
+                                  MFIELD(const float2, md, T), MFIELD(const float2, md, TN), nm, st, l2, v);
+    return launch_bodyforce(g, m->S, md, c.one, nm, c.rclass, BODY_CORR, st, v);
   };
   const int nvar[3] = {COLS_VARIANTS, GREEN_VARIANTS, BODY_VARIANTS};
   constexpr int NREP = 3;
-  int best[3] = {0, 0, 0};
-  float times[3][RELAX_TUNE_VARIANTS];
-  for (auto& row : times)
-    for (float& t : row) t = -1.f;
+  best[0] = best[1] = best[2] = 0;
+  for (int k = 0; k < 3; ++k)
+    for (int v = 0; v < RELAX_TUNE_VARIANTS; ++v) times[k][v] = -1.f;
   rc = RELAX_OK;
   for (int kind = 0; kind < 3 && !rc; ++kind) {
     float tbest = -1.f;
@@ -2056,12 +2098,48 @@ int relax_model_autotune(relax_model_t* m, int* choice, float* ms) {
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(st));
+  return RELAX_OK;
+}
+
+int relax_model_autotune(relax_model_t* m, int* choice, float* ms) {
+  if (!m || m->batch) return RELAX_ERR_ARG;
+  int rc;
+  if ((rc = sync_md(m))) return rc;
+  const Grid& g = m->g;
+  int best[3];
+  float times[3][RELAX_TUNE_VARIANTS];
+  if ((rc = tune_time(single(m), best, times))) return rc;
   m->tune_cols = best[0];
   m->tune_green = best[1];
   m->tune_body = best[2];
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[TuneKey{m->device, g.n1, g.n2, g.n3}] = TuneChoice{best[0], best[1], best[2]};
+  }
+  if (ms)
+    for (int k = 0; k < 3; ++k)
+      for (int v = 0; v < RELAX_TUNE_VARIANTS; ++v) ms[k * RELAX_TUNE_VARIANTS + v] = times[k][v];
+  if (choice) { choice[0] = best[0]; choice[1] = best[1]; choice[2] = best[2]; }
+  return RELAX_OK;
+}
+
+int relax_batch_autotune(relax_batch_t* b, int* choice, float* ms) {
+  if (!b || b->models.empty()) return RELAX_ERR_ARG;
+  int rc;
+  if ((rc = batch_refresh(b))) return rc;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->tuned = false;
+  const Ctx c = batch_ctx(b);
+  int best[3];
+  float times[3][RELAX_TUNE_VARIANTS];
+  if ((rc = tune_time(c, best, times))) return rc;
+  drop_graph(b);  // a captured step keeps the shapes it was captured with
+  b->tuned = true;
+  for (int k = 0; k < 3; ++k) b->tune[k] = best[k];
+  {
+    const Grid& g = c.lead->g;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_btune[BatchTuneKey{b->device, g.n1, g.n2, g.n3, c.rclass}] = TuneChoice{best[0], best[1], best[2]};
   }
   if (ms)
     for (int k = 0; k < 3; ++k)

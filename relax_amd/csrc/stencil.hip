@@ -162,6 +162,159 @@ __device__ __forceinline__ void bodyforce_tile(const Grid& g, const RheoParams& 
   }
 }
 
+// The same divergence with the x3 differences taken in registers: every
+// thread owns one point of the tile and keeps the s13, s23, s33 moments of
+// its column at the two planes above the current one.  Only the current
+// plane's s11, s12, s13, s22, s23 go through LDS, with a one-point halo along
+// x1 and x2 and no corners (the divergence uses none), double-buffered so
+// that one barrier per plane suffices.  The first 2*(TXB+TYB) threads also
+// evaluate one halo point each, on the tile's output planes only; the planes
+// just above and below the depth chunk need only the tile's own points.
+// Every output is the same expression as in bodyforce_tile.
+template <int TXB, int TYB, bool TM, bool RAW, int RC>
+__device__ __forceinline__ void bodyforce_tile_zr(const Grid& g, const RheoParams& R, const StencilConst& S,
+                                                  const float* __restrict__ sig, const float* __restrict__ E,
+                                                  float* __restrict__ W, float* __restrict__ T, StepScalars* sc,
+                                                  int bx, int by, int bz, int zc, int gz) {
+  constexpr int NT = TXB * TYB;
+  constexpr int HX = TXB + 2, HY = TYB + 2, HP = HX * HY;
+  constexpr int NH = 2 * (TXB + TYB);  // halo points of one plane
+  static_assert(NH <= NT, "one halo point per thread at most");
+  constexpr int NTM = RC == RHEO_EXT ? NMECH : 3;  // mechanisms whose minima are reduced
+  __shared__ float pl[2][5][HP];
+  __shared__ float rmin[NTM][NT / 64];
+  const int tid = threadIdx.x;
+  const int tx = tid % TXB, ty = tid / TXB;
+  const int n1 = g.n1, n2 = g.n2, h3 = g.h3;
+  const long H = g.H, P = (long)n1 * n2;
+  const int x0 = bx * TXB, y0 = by * TYB;
+  const int z0 = (bz % gz) * zc, z1 = min(z0 + zc, h3);
+  float tmin[NTM];
+#pragma unroll
+  for (int k = 0; k < NTM; ++k) tmin[k] = __builtin_inff();
+  int status = 0;
+  const bool trans = RC == RHEO_EXT && (R.tr[0].active || R.tr[1].active);
+  // own point
+  const int c1 = x0 + tx, c2 = y0 + ty;
+  const int ec = (ty + 1) * HX + (tx + 1);
+  const long oc0 = c1 + (long)n1 * c2;
+  // halo point: rows above and below the tile, then the columns left and right
+  const bool hal = tid < NH;
+  int hx = 0, hy = 0;
+  if (tid < TXB) { hx = tid; hy = -1; }
+  else if (tid < 2 * TXB) { hx = tid - TXB; hy = TYB; }
+  else if (tid < 2 * TXB + TYB) { hx = -1; hy = tid - 2 * TXB; }
+  else if (hal) { hx = TXB; hy = tid - 2 * TXB - TYB; }
+  const int h1 = (x0 + hx + n1) % n1, h2 = (y0 + hy + n2) % n2;
+  const int eh = (hy + 1) * HX + (hx + 1);
+  const long oh0 = h1 + (long)n1 * h2;
+
+  const T6 zero{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  T6 pc = zero, ph = zero, pce = zero, phe = zero;  // inputs of the next plane
+  auto load_plane = [&](int z, bool halo) {
+    pc = load_t6(sig, H, oc0 + P * z);
+    if (RC == RHEO_EXT && trans) pce = load_t6(E, H, oc0 + P * z);
+    if (halo && hal) {
+      ph = load_t6(sig, H, oh0 + P * z);
+      if (RC == RHEO_EXT && trans) phe = load_t6(E, H, oh0 + P * z);
+    }
+  };
+  auto moment = [&](const T6& s, const T6& e, int p1, int p2, int z, long o) -> T6 {
+    if (RAW) return s;
+    if (RC == RHEO_EXT) {
+      T6 ed;
+      T6 m = rheology_ext<TM, !TM>(R, s, p1, p2, z, o, e, &ed, tmin, nullptr, nullptr, &status);
+      // the corrector's moment includes the inter-seismic loading
+      if (!TM && R.inter) {
+        const T6 a = load_t6(R.inter, H, o);
+        m = T6{m.s11 + a.s11, m.s12 + a.s12, m.s13 + a.s13, m.s22 + a.s22, m.s23 + a.s23, m.s33 + a.s33};
+      }
+      return m;
+    }
+    return rheology<TM, RC == RHEO_CREEP>(R, s, p1, p2, z, tmin, nullptr, nullptr, &status);
+  };
+
+  const int zs = max(z0 - 1, 0), ze = min(z1, h3 - 1);
+  // moments 13, 23, 33 of the own column at planes z-2 (a) and z-1 (b), and
+  // the x1/x2 part of the divergence at plane z-1
+  float a13 = 0.f, a23 = 0.f, a33 = 0.f, b13 = 0.f, b23 = 0.f, b33 = 0.f;
+  float f1 = 0.f, f2 = 0.f, f3 = 0.f;
+  load_plane(zs, zs >= z0);
+  for (int z = zs; z <= ze; ++z) {
+    const bool outp = z >= z0 && z < z1;
+    const T6 sc0 = pc, sce = pce, sh0 = ph, she = phe;
+    if (z < ze) load_plane(z + 1, z + 1 >= z0 && z + 1 < z1);
+    const T6 mc = moment(sc0, sce, c1, c2, z, oc0 + P * z);
+    float* buf = &pl[z & 1][0][0];
+    if (outp) {
+      buf[ec] = mc.s11; buf[HP + ec] = mc.s12; buf[2 * HP + ec] = mc.s13;
+      buf[3 * HP + ec] = mc.s22; buf[4 * HP + ec] = mc.s23;
+      if (hal) {
+        const T6 mh = moment(sh0, she, h1, h2, z, oh0 + P * z);
+        buf[eh] = mh.s11; buf[HP + eh] = mh.s12; buf[2 * HP + eh] = mh.s13;
+        buf[3 * HP + eh] = mh.s22; buf[4 * HP + eh] = mh.s23;
+      }
+    }
+    // plane z-1 is complete: its x3 differences take plane z below it
+    const int zo = z - 1;
+    if (zo >= z0 && zo < z1) {
+      float v1, v2, v3;
+      if (zo == 0) {
+        v1 = (mc.s13 - b13) * S.idx3; v2 = (mc.s23 - b23) * S.idx3; v3 = (mc.s33 - b33) * S.idx3;
+      } else {
+        v1 = (mc.s13 - a13) * S.k3h; v2 = (mc.s23 - a23) * S.k3h; v3 = (mc.s33 - a33) * S.k3h;
+      }
+      const long o = oc0 + P * zo;
+      gst(W + o, -(f1 + v1));
+      gst(W + H + o, -(f2 + v2));
+      gst(W + 2 * H + o, -(f3 + v3));
+      if (zo == 0) {
+        gst(T + oc0, b13);
+        gst(T + P + oc0, b23);
+        gst(T + 2 * P + oc0, b33);
+      }
+    }
+    __syncthreads();
+    if (outp) {
+      const float* s11 = buf, * s12 = buf + HP, * s13 = buf + 2 * HP, * s22 = buf + 3 * HP, * s23 = buf + 4 * HP;
+      f1 = (s11[ec + 1] - s11[ec - 1]) * S.k1h + (s12[ec + HX] - s12[ec - HX]) * S.k2h;
+      f2 = (s12[ec + 1] - s12[ec - 1]) * S.k1h + (s22[ec + HX] - s22[ec - HX]) * S.k2h;
+      f3 = (s13[ec + 1] - s13[ec - 1]) * S.k1h + (s23[ec + HX] - s23[ec - HX]) * S.k2h;
+    }
+    a13 = b13; a23 = b23; a33 = b33;
+    b13 = mc.s13; b23 = mc.s23; b33 = mc.s33;
+  }
+  // the deepest plane of the grid: one-sided difference with the plane above
+  if (z1 == h3 && h3 - 1 >= z0) {
+    const int zo = h3 - 1;
+    const long o = oc0 + P * zo;
+    gst(W + o, -(f1 + (b13 - a13) * S.idx3));
+    gst(W + H + o, -(f2 + (b23 - a23) * S.idx3));
+    gst(W + 2 * H + o, -(f3 + (b33 - a33) * S.idx3));
+    if (zo == 0) {
+      gst(T + oc0, b13);
+      gst(T + P + oc0, b23);
+      gst(T + 2 * P + oc0, b33);
+    }
+  }
+
+  if (status) atomicMax(&sc->status, status);
+  if (TM) {
+#pragma unroll
+    for (int k = 0; k < NTM; ++k) {
+      float v = tmin[k];
+      for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+      if ((tid & 63) == 0) rmin[k][tid >> 6] = v;
+    }
+    __syncthreads();
+    if (tid < NTM) {
+      float v = rmin[tid][0];
+      for (int w = 1; w < NT / 64; ++w) v = fminf(v, rmin[tid][w]);
+      if (v < __builtin_inff()) atomicMin(&sc->tmin_bits[tid], __float_as_uint(v));
+    }
+  }
+}
+
 // BATCH: the models' records are in the device array md (one model per
 // nx*ny*gz tiles); otherwise the one model's record is the argument Dv.
 template <int TXB, int TYB, bool TM, bool RAW, int RC, bool BATCH>
@@ -176,6 +329,17 @@ __global__ __launch_bounds__(TXB* TYB) void k_bodyforce(Grid g, ModelDev Dv, con
   const ModelDev& D = BATCH ? md[bz / gz] : Dv;
   bodyforce_tile<TXB, TYB, TM, RAW, RC>(g, D.R, S, RAW ? D.tau : D.sig, TM ? D.E : D.Em, D.W, D.T, D.sc, bx, by,
                                          bz, zc, gz);
+}
+
+template <int TXB, int TYB, bool TM, bool RAW, int RC, bool BATCH>
+__global__ __launch_bounds__(TXB* TYB) void k_bodyforce_zr(Grid g, ModelDev Dv, const ModelDev* __restrict__ md,
+                                                           StencilConst S, int zc, int gz) {
+  const int nx = g.n1 / TXB, ny = g.n2 / TYB;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % nx, by = (tile / nx) % ny, bz = tile / (nx * ny);
+  const ModelDev& D = BATCH ? md[bz / gz] : Dv;
+  bodyforce_tile_zr<TXB, TYB, TM, RAW, RC>(g, D.R, S, RAW ? D.tau : D.sig, TM ? D.E : D.Em, D.W, D.T, D.sc, bx,
+                                            by, bz, zc, gz);
 }
 
 // Hooke's law and the stage update of one point from its strain e (the
@@ -352,31 +516,35 @@ __global__ __launch_bounds__(256) void k_stress(Grid g, ModelDev Dv, const Model
 int launch_bodyforce(const Grid& g, const StencilConst& S, const ModelDev* md, const ModelDev* one, int nm,
                      int rclass, int mode, hipStream_t st, int variant) {
   // depth chunk of one tile: each chunk re-reads the plane above and below it
-  const int zc = variant == 3 ? 32 : 16;
+  const int zc = variant >= 3 ? 32 : 16;
   const int gz = (g.h3 + zc - 1) / zc;
   const bool batch = !(one && nm == 1);
   const ModelDev Dv = batch ? ModelDev{} : *one;
   if (rclass < RHEO_VISCOUS || rclass > RHEO_EXT) return RELAX_ERR_ARG;
-#define LB(TX, TY, F, B)                                                                                 \
+#define LB(K, TX, TY, F, B)                                                                                 \
   {                                                                                                      \
     const dim3 grid((unsigned)((g.n1 / TX) * (g.n2 / TY) * gz * nm));                                    \
-    if (mode == BODY_PRED) k_bodyforce<TX, TY, true, false, F, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz); \
-    else if (mode == BODY_CORR) k_bodyforce<TX, TY, false, false, F, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz); \
-    else k_bodyforce<TX, TY, false, true, RHEO_VISCOUS, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz); \
+    if (mode == BODY_PRED) K<TX, TY, true, false, F, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz); \
+    else if (mode == BODY_CORR) K<TX, TY, false, false, F, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz); \
+    else K<TX, TY, false, true, RHEO_VISCOUS, B><<<grid, TX * TY, 0, st>>>(g, Dv, md, S, zc, gz); \
   }
-#define LF(TX, TY, F) { if (batch) LB(TX, TY, F, true) else LB(TX, TY, F, false) }
-#define L(TX, TY)                                   \
+#define LF(K, TX, TY, F) { if (batch) LB(K, TX, TY, F, true) else LB(K, TX, TY, F, false) }
+#define L(K, TX, TY)                                   \
   {                                                 \
-    if (rclass == RHEO_EXT) LF(TX, TY, RHEO_EXT)    \
-    else if (rclass == RHEO_CREEP) LF(TX, TY, RHEO_CREEP) \
-    else LF(TX, TY, RHEO_VISCOUS)                   \
+    if (rclass == RHEO_EXT) LF(K, TX, TY, RHEO_EXT)    \
+    else if (rclass == RHEO_CREEP) LF(K, TX, TY, RHEO_CREEP) \
+    else LF(K, TX, TY, RHEO_VISCOUS)                   \
   }
   // variant 0: 32x16 tiles, 1: 32x8 tiles, 2: 64x8 tiles, 3: 64x8 tiles
-  // 32 planes deep (16 for the others)
+  // 32 planes deep (16 for the others); 4 and 5: 64x8 and 64x16 tiles 32
+  // planes deep with the x3 differences in registers (k_bodyforce_zr)
   if (variant < 0 || variant >= BODY_VARIANTS) return RELAX_ERR_ARG;
-  if (variant == 0 && g.n1 % 32 == 0 && g.n2 % 16 == 0) L(32, 16)
-  else if (variant >= 2 && g.n1 % 64 == 0 && g.n2 % 8 == 0) L(64, 8)
-  else if (variant != 2 && g.n1 % 32 == 0 && g.n2 % 8 == 0) L(32, 8)
+  if (variant == 4 && g.n1 % 64 == 0 && g.n2 % 8 == 0) L(k_bodyforce_zr, 64, 8)
+  else if (variant == 5 && g.n1 % 64 == 0 && g.n2 % 16 == 0) L(k_bodyforce_zr, 64, 16)
+  else if (variant >= 4) return RELAX_ERR_SIZE;
+  else if (variant == 0 && g.n1 % 32 == 0 && g.n2 % 16 == 0) L(k_bodyforce, 32, 16)
+  else if (variant >= 2 && g.n1 % 64 == 0 && g.n2 % 8 == 0) L(k_bodyforce, 64, 8)
+  else if (variant != 2 && g.n1 % 32 == 0 && g.n2 % 8 == 0) L(k_bodyforce, 32, 8)
   else return RELAX_ERR_SIZE;
 #undef LB
 #undef LF

@@ -196,7 +196,7 @@ class Model:
     def synchronize(self):
         check(self._lib.relax_model_synchronize(self._h), "synchronize")
 
-    TUNE_VARIANTS = 4
+    TUNE_VARIANTS = 6
 
     def autotune(self, times=False):
         """Choose the launch shapes of the tunable kernels for this grid by
@@ -278,6 +278,20 @@ class Batch:
     def busy(self):
         """True while the batch has queued work (does not wait)."""
         return bool(self._lib.relax_batch_busy(self._h))
+
+    def autotune(self, times=False):
+        """Choose the launch shapes of the tunable kernels for this batch by
+        timing one launch over all current members per variant
+        (relax_batch_autotune); returns the choice as Model.autotune does.
+        Later batches of the grid and rheology class start from it."""
+        c = np.zeros(3, np.int32)
+        ms = np.zeros((3, Model.TUNE_VARIANTS), np.float32)
+        check(self._lib.relax_batch_autotune(self._h, ptr(c), ptr(ms)), "batch autotune")
+        out = dict(fft_cols=int(c[0]), green=int(c[1]), bodyforce=int(c[2]))
+        if times:
+            out["ms"] = {k: [None if v < 0 else round(float(v), 4) for v in ms[i]]
+                         for i, k in enumerate(("fft_cols", "green", "bodyforce"))}
+        return out
 
     def scalars(self):
         """The members' current step records (waits for queued work)."""

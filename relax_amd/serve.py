@@ -147,6 +147,37 @@ def batch_cap(cls):
     return max(1, MAX_BATCH_POINTS // (cls[0] * cls[1] * cls[2]))
 
 
+def tune(grids=None, seed=7):
+    """Choose launch shapes for the request stream's grid classes before
+    serving: per grid, the single-model autotuner on one model
+    (relax_model_autotune, used by the single-model path), and the batch
+    autotuner on a full batch of throw-away models, once without and once
+    with an afterslip member (the two rheology classes a batch of the stream
+    runs, relax_batch_autotune).  Later models and batches of these classes
+    in the process start from the choices.  Returns {(grid index,
+    "single" | "batch" | "batch afterslip"): choice}."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    for gi in range(len(GRIDS)) if grids is None else grids:
+        m = workloads.build(request_spec(rng, gi, False))
+        try:
+            out[(gi, "single")] = m.autotune()
+        finally:
+            m.close()
+        for afterslip in (False, True):
+            first = request_spec(rng, gi, afterslip)
+            specs = [first] + [request_spec(rng, gi, False) for _ in range(batch_cap(grid_class(first)) - 1)]
+            models = [workloads.build(s) for s in specs]
+            b = Batch(models)
+            try:
+                out[(gi, "batch afterslip" if afterslip else "batch")] = b.autotune()
+            finally:
+                b.close()
+                for m in models:
+                    m.close()
+    return out
+
+
 def run_back_to_back(requests, realtime=True):
     """Each request alone through the single-model path, in arrival order."""
     t0 = time.perf_counter()

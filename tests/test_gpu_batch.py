@@ -56,6 +56,34 @@ def test_batch_members_match_their_single_runs_bitwise():
         m.close()
 
 
+@pytest.mark.parametrize("tuning", [dict(fft_cols=1, green=3, bodyforce=3), dict(fft_cols=2, green=1, bodyforce=2)])
+def test_batch_with_tuned_launch_shapes(tuning):
+    """A batch launches every kernel with its first member's launch shapes
+    (the autotuner keeps one choice per grid): with the register Green pass,
+    other y-transform tiles and the deeper divergence tiles the members still
+    equal their single runs with the same shapes, bitwise."""
+    specs = [_variant("powerlaw", 3, sx3=128), _variant("maxwell", 4, sx3=128)]
+    nsteps = 3
+    ref = []
+    for s in specs:
+        m = workloads.build(s)
+        m.set_tuning(**tuning)
+        recs = [m.step() for _ in range(nsteps)]
+        ref.append((recs, m.download()))
+        m.close()
+    models = [workloads.build(s) for s in specs]
+    for m in models:
+        m.set_tuning(**tuning)
+    with Batch(models) as b:
+        for k in range(nsteps):
+            recs = b.step()
+            for i in range(2):
+                assert recs[i] == ref[i][0][k], (i, k)
+    for i, m in enumerate(models):
+        _same(m.download(), ref[i][1])
+        m.close()
+
+
 def test_batch_with_fault_creep_member():
     """An afterslip member switches the batch to the fault-creep kernels; the
     viscous members then run those kernels too, so they agree to rounding."""
@@ -190,3 +218,41 @@ def test_c3_batch_of_64_models_of_128_cubed():
         assert rel_l2(outs[i][k][..., :n1], st[k][..., :n1]) < 1e-4, k
     for k in ("sig", "tau"):
         assert rel_l2(outs[i][k], st[k]) < 1e-4, k
+
+
+def test_batch_autotune_keeps_state_and_members_match_their_single_runs():
+    """relax_batch_autotune times each variant as one launch over all members:
+    the members' state does not change; stepped with the chosen shapes each
+    member equals its single run with the same shapes, bitwise; and a later
+    batch of the same grid and rheology class starts from the choice."""
+    specs = [_variant("powerlaw", 5, sx3=256), _variant("powerlaw", 6, sx3=256), _variant("powerlaw", 7, sx3=256)]
+    nsteps = 3
+    models = [workloads.build(s) for s in specs]
+    before = [m.download() for m in models]
+    with Batch(models) as b:
+        choice = b.autotune(times=True)
+        print("batch autotune", choice, flush=True)
+        assert 0 <= choice["fft_cols"] < 3 and 0 <= choice["green"] < 4 and 0 <= choice["bodyforce"] < 6
+        for m, ref in zip(models, before):
+            _same(m.download(), ref)
+        tuned = [b.step() for _ in range(nsteps)]
+    out = [m.download() for m in models]
+    for m in models:
+        m.close()
+    shapes = {k: choice[k] for k in ("fft_cols", "green", "bodyforce")}
+    for i, s in enumerate(specs):
+        m = workloads.build(s)
+        m.set_tuning(**shapes)
+        recs = [m.step() for _ in range(nsteps)]
+        for k in range(nsteps):
+            assert recs[k] == tuned[k][i], (i, k)
+        _same(m.download(), out[i])
+        m.close()
+    # a new batch of fresh models (default single-model shapes) uses the choice
+    models = [workloads.build(s) for s in specs]
+    with Batch(models) as b:
+        for _ in range(nsteps):
+            b.step(info=False)
+    for i, m in enumerate(models):
+        _same(m.download(), out[i])
+        m.close()

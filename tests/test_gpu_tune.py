@@ -22,7 +22,7 @@ def test_autotune_keeps_state_and_results():
 
     m = workloads.build(s)
     choice = m.autotune()
-    assert 0 <= choice["fft_cols"] < 3 and 0 <= choice["green"] < 4 and 0 <= choice["bodyforce"] < 4
+    assert 0 <= choice["fft_cols"] < 3 and 0 <= choice["green"] < 4 and 0 <= choice["bodyforce"] < 6
     after = m.download()
     for k in ("u1", "u2", "u3", "sig", "tau", "gamma"):
         np.testing.assert_array_equal(after[k], before[k], err_msg=k)
@@ -58,12 +58,16 @@ def test_every_green_variant_matches_the_default(green, sx3):
         assert rel_l2(out[green][k], out[0][k]) < 1e-5, k
 
 
-@pytest.mark.parametrize("body,sx3", [(1, 64), (2, 64), (3, 32), (3, 64), (3, 128)])
-def test_every_bodyforce_tile_matches_the_default(body, sx3):
-    """The divergence tiles (32x16, 32x8, 64x8 16 or 32 planes deep) give the
-    same trajectory, with the 32-plane chunk cut short (16 planes), whole, and
-    repeated."""
-    s = small_spec("afterslip", sx=128, sx3=sx3)
+@pytest.mark.parametrize("body,sx3,kind", [(1, 64, "afterslip"), (2, 64, "afterslip"), (3, 32, "afterslip"),
+                                           (3, 64, "afterslip"), (3, 128, "afterslip"), (4, 32, "afterslip"),
+                                           (4, 64, "afterslip"), (4, 128, "afterslip"), (5, 64, "afterslip"),
+                                           (5, 128, "afterslip"), (4, 128, "powerlaw"), (5, 64, "transient")])
+def test_every_bodyforce_tile_matches_the_default(body, sx3, kind):
+    """The divergence tiles (32x16, 32x8, 64x8 16 or 32 planes deep, and 64x8
+    or 64x16 with the x3 differences in registers) give the same trajectory,
+    with the 32-plane chunk cut short (16 planes), whole, and repeated, for
+    the fault-creep, viscous and extended rheology kernels."""
+    s = small_spec(kind, sx=128, sx3=sx3)
     out = {}
     for v in (0, body):
         m = workloads.build(s)
