@@ -47,12 +47,16 @@ TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "prof
                                "rk2_step_c1_512x512x256_hbm_traffic.md")
 
 
+# the tunable kernel kinds and their autotuner field
+TUNED_KIND = {"fft_y": "fft_cols", "green_z": "green", "bodyforce": "bodyforce"}
+
+
 def measured_traffic(tuning):
     """HBM bytes per launch of each kernel kind at 512x512x256, read from the
     committed rocprofv3 summary (FETCH_SIZE and WRITE_SIZE, each scaled by its
-    reading for a device copy of known size).  The summary names the launch
-    shapes it was measured with; when the autotuner picks others now, the
-    figures do not describe the timed kernels and None is returned."""
+    reading for a device copy of known size).  The summary gives the tunable
+    kinds per launch-shape variant; a kind whose variant this run uses was
+    not measured is left out."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
@@ -63,10 +67,21 @@ def measured_traffic(tuning):
                 return None, "no traffic line in the profile"
     except OSError as e:
         return None, "profile not readable: %s" % e
-    want = {k: tuning[k] for k in ("fft_cols", "green", "bodyforce")}
-    if rec.get("tuning") != want:
-        return None, "profile measured with tuning %s, this run uses %s" % (rec.get("tuning"), want)
-    return rec["bytes"], None
+    out, missing = {}, []
+    for kind, b in rec["bytes"].items():
+        if kind in TUNED_KIND:
+            field = TUNED_KIND[kind]
+            v = str(tuning[field])
+            # {variant: bytes}, or the bytes of the variant the profile's
+            # "tuning" names
+            if not isinstance(b, dict):
+                b = {str(rec.get("tuning", {}).get(field)): b}
+            if v not in b:
+                missing.append("%s variant %s" % (kind, v))
+                continue
+            b = b[v]
+        out[kind] = b
+    return out, ("not profiled: " + ", ".join(missing)) if missing else None
 
 
 def step_bytes_model(sx1, sx2, sx3):
@@ -260,6 +275,9 @@ def main():
     achieved = kernels[dom]["achieved_GBs"]
     B_step = step_bytes_model(sx1, sx2, sx3)
     traffic, why = measured_traffic(tuned) if args.workload == "c1" else (None, "profiled at c1 only")
+    for k, e in kernels.items():
+        if traffic and k in traffic:
+            e["hbm_bytes_profiled"] = traffic[k]
     fused_bytes = sum(kb[k] * kernels[k]["launches_per_step"] for k in kb if k in kernels)
     out = {
         "metric": METRIC,

@@ -102,17 +102,18 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
   __shared__ float2 stw[N1];  // twiddles in LDS: off the global-latency path of every stage
   __shared__ float2 ytw[G][8];
   const int tid = threadIdx.x;
-  const int L2 = n2 / 8;
+  // n2 is a power of two (64 or more with YS): L2 = n2/8 as shift and mask
+  const int L2 = n2 / 8, ls2 = YS ? __builtin_ctz((unsigned)n2) - 3 : 0;
   // global row of LDS row b
   auto rowof = [&](int b) -> long {
     if (!YS) return (long)blockIdx.x * RPB + b;
     const long q = (long)blockIdx.x * G + b / 8;
-    return (q / L2) * n2 + (q % L2) + (long)L2 * (b % 8);
+    return ((q >> ls2) << (ls2 + 3)) + (q & (L2 - 1)) + (long)L2 * (b % 8);
   };
   for (int e = tid; e < N1; e += NT) stw[e] = tw[e];
   if (YS && tid < G * 8) {
-    const int j1 = (int)(((long)blockIdx.x * G + tid / 8) % L2);
-    ytw[tid / 8][tid % 8] = twy[(j1 * (tid % 8)) % n2];
+    const int j1 = (int)(((long)blockIdx.x * G + tid / 8) & (L2 - 1));
+    ytw[tid / 8][tid % 8] = twy[(j1 * (tid % 8)) & (n2 - 1)];
   }
   for (int e = tid; e < RPB * M; e += NT) {
     const int b = e / M, j = e % M;
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
 #pragma unroll
     for (int t = 0; t < 8; ++t) a[t] = s[lpad<XP>((8 * gi + t) * PM + k)];
     dft8<-1>(a);
-    const long base_row = (q / L2) * n2 + (q % L2);
+    const long base_row = ((q >> ls2) << (ls2 + 3)) + (q & (L2 - 1));
 #pragma unroll
     for (int t = 0; t < 8; ++t) gstore(out + (base_row + (long)L2 * t) * out_pitch + k, t ? cmul(a[t], ytw[gi][t]) : a[t]);
   }
@@ -189,16 +190,17 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long 
   __shared__ float2 stw[N1];
   __shared__ float2 ytw[G][8];
   const int tid = threadIdx.x;
-  const int L2 = n2 / 8;
+  // n2 is a power of two (64 or more with YS): L2 = n2/8 as shift and mask
+  const int L2 = n2 / 8, ls2 = YS ? __builtin_ctz((unsigned)n2) - 3 : 0;
   auto rowof = [&](int b) -> long {
     if (!YS) return (long)blockIdx.x * RPB + b;
     const long q = (long)blockIdx.x * G + b / 8;
-    return (q / L2) * n2 + (q % L2) + (long)L2 * (b % 8);
+    return ((q >> ls2) << (ls2 + 3)) + (q & (L2 - 1)) + (long)L2 * (b % 8);
   };
   for (int e = tid; e < N1; e += NT) stw[e] = tw[e];
   if (YS && tid < G * 8) {
-    const int j1 = (int)(((long)blockIdx.x * G + tid / 8) % L2);
-    ytw[tid / 8][tid % 8] = twy[(j1 * (tid % 8)) % n2];
+    const int j1 = (int)(((long)blockIdx.x * G + tid / 8) & (L2 - 1));
+    ytw[tid / 8][tid % 8] = twy[(j1 * (tid % 8)) & (n2 - 1)];
   }
   for (int e = tid; e < RPB * M; e += NT) {
     const int b = e / M, j = e % M;
@@ -364,7 +366,7 @@ int launch_fft_x_r2c(int n1, MPtr<const float> in, long in_pitch, MPtr<float2> o
   if (!tw) return RELAX_ERR_NOMEM;
   const float2* twy = nullptr;
   if (ys_n2) {
-    if (ys_n2 < 64 || nrows % ys_n2) return RELAX_ERR_SIZE;
+    if (ys_n2 < 64 || (ys_n2 & (ys_n2 - 1)) || nrows % ys_n2) return RELAX_ERR_SIZE;
     if (!(twy = twiddle_table(ys_n2))) return RELAX_ERR_NOMEM;
   }
   long nb = 0;
@@ -398,7 +400,7 @@ int launch_fft_x_c2r(int n1, int epi, MPtr<const float2> in, long in_pitch, MPtr
   if (!tw) return RELAX_ERR_NOMEM;
   const float2* twy = nullptr;
   if (ys_n2) {
-    if (ys_n2 < 64 || nrows % ys_n2) return RELAX_ERR_SIZE;
+    if (ys_n2 < 64 || (ys_n2 & (ys_n2 - 1)) || nrows % ys_n2) return RELAX_ERR_SIZE;
     if (!(twy = twiddle_table(ys_n2))) return RELAX_ERR_NOMEM;
   }
 #define LAUNCH(N, E, YS)                                                                                    \

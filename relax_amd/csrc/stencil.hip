@@ -474,7 +474,9 @@ __device__ __forceinline__ void stress_block(const Grid& g, const RheoParams& R,
   double pw = 0.0, ce = 0.0;
   int status = 0;
   if (o < H) {
-    const int i1 = (int)(o % n1), i2 = (int)((o / n1) % n2), i3 = (int)(o / ((long)n1 * n2));
+    // n1 and n2 are powers of two (relax_model_create): shifts and masks
+    const int l1 = __builtin_ctz((unsigned)n1), l2 = __builtin_ctz((unsigned)n2);
+    const int i1 = (int)(o & (n1 - 1)), i2 = (int)((o >> l1) & (n2 - 1)), i3 = (int)(o >> (l1 + l2));
     const long row = o - i1;
     const long xp = row + (i1 + 1 == n1 ? 0 : i1 + 1), xm = row + (i1 == 0 ? n1 - 1 : i1 - 1);
     const long plane = (long)n1 * n2;

@@ -306,9 +306,9 @@ class Server:
                         self._mem.wait(0.005)
                     continue
                 # up to `chunk` steps at once (no member past its last
-                # step), one while requests wait for the class
+                # step), one while requests wait that the class could admit
                 with inbox["cv"]:
-                    more = bool(waiting)
+                    more = bool(waiting) and len(members) < batch_cap(cls)
                 nst = 1 if more else min(self.chunk, min(r.nsteps - k for r, _, k, _ in members))
                 for _ in range(nst):
                     batch.step(info=False)
