@@ -178,6 +178,23 @@ def tune(grids=None, seed=7):
     return out
 
 
+def kernel_time(requests):
+    """Device time of the stream's RK2 steps on the single-model path: each
+    request's model, set up as for serving, runs its steps under the
+    HIP-event phase timers (relax_model_profile).  Returns the summed kernel
+    time of the steps in seconds, by grid index; the set-up (the coseismic
+    event) is not counted."""
+    out = {}
+    for r in requests:
+        m = workloads.build(r.spec)
+        try:
+            prof = m.profile(r.nsteps)
+        finally:
+            m.close()
+        out[r.grid] = out.get(r.grid, 0.0) + sum(ms for ms, _ in prof.values()) / 1e3
+    return out
+
+
 def run_back_to_back(requests, realtime=True):
     """Each request alone through the single-model path, in arrival order."""
     t0 = time.perf_counter()

@@ -105,3 +105,27 @@ def test_c4_stream_with_every_grid_class():
         for ua, ub in zip(a.surface, b.surface):
             na = np.linalg.norm(ua.astype(np.float64))
             assert np.linalg.norm(ua.astype(np.float64) - ub) <= 1e-5 * max(na, 1e-30), q.rid
+
+
+def test_served_with_tuned_launch_shapes():
+    """After serve.tune() (both autotuners for every grid class) the single
+    path and the batches may launch different shapes (the Green variants
+    differ in FP32 rounding), so each served result agrees with its result
+    alone to FP32 rounding: the adaptive time to 1e-5, the surface
+    displacement to relative L2 1e-5."""
+    tuned = serve.tune()
+    assert {g for g, _ in tuned} == {0, 1, 2}
+    reqs = serve.request_stream(10, seed=5, steps=(10, 20))
+    alone = {r.rid: r for r in serve.run_back_to_back(reqs, realtime=False)}
+    mixed = {r.rid: r for r in serve.Server().run(reqs, realtime=False)}
+    assert sorted(alone) == sorted(mixed) == sorted(r.rid for r in reqs)
+    for q in reqs:
+        a, b = alone[q.rid], mixed[q.rid]
+        assert a.status == b.status, q.rid
+        if a.status:
+            continue
+        assert b.nsteps == q.nsteps
+        assert abs(a.t - b.t) <= 1e-5 * abs(a.t), q.rid
+        for ua, ub in zip(a.surface, b.surface):
+            na = np.linalg.norm(ua.astype(np.float64))
+            assert np.linalg.norm(ua.astype(np.float64) - ub) <= 1e-5 * max(na, 1e-30), q.rid
