@@ -1019,7 +1019,10 @@ static int model_green(const Ctx& c, int epi) {
   if ((rc = launch_fft_cols(ly, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st, c.tune_cols)))
     return rc;
   cmark(c, RELAX_PROF_FFT_Y);
-  if ((rc = launch_fft_cols(n2, -1, MFIELD(float2, md, T), m1, (long)n2 * m1, m1, 3, nm, st))) return rc;
+  // the three traction planes of one model are few columns for the device:
+  // half the default columns per workgroup gives twice the workgroups
+  if ((rc = launch_fft_cols(n2, -1, MFIELD(float2, md, T), m1, (long)n2 * m1, m1, 3, nm, st, nm == 1 ? 1 : 0)))
+    return rc;
   cmark(c, RELAX_PROF_SMALL);
   if ((rc = launch_fft_rows(n2, -1, MFIELD(float2, md, NQ), n2, 3L * h3, nm, st))) return rc;
   cmark(c, RELAX_PROF_SMALL);

@@ -215,7 +215,7 @@ def test_trajectory_at_full_size(name, k):
     T["chunk"] = k + 1
 
 
-VARIANTS = [("green", v) for v in range(4)] + [("bodyforce", v) for v in range(4)] + \
+VARIANTS = [("green", v) for v in range(4)] + [("bodyforce", v) for v in range(6)] + \
            [("fft_cols", v) for v in range(3)]
 
 
