@@ -194,7 +194,8 @@ def serving_c4(nreq=60, rate=20.0):
     steps, 30 % with an afterslip plane).  Capacity: the whole stream queued
     at once, continuous batching against the single-model path back to back.
     Latency: arrivals replayed at their times, p50/p99 time-to-solution
-    (set-up, steps and the surface displacement of each request).  Also the
+    (set-up, steps and the surface displacement of each request).  Both
+    also with every request in one batch of uneven grids (mixed=True).  Also the
     device time of the stream's step kernels on the single-model path
     (relax_model_profile): the single path's makespan over it bounds the
     speedup of any scheme that only removes host work and launch gaps."""
@@ -209,6 +210,9 @@ def serving_c4(nreq=60, rate=20.0):
     cap_served = serve.summarize(serve.Server().run(reqs, realtime=False), "continuous batching")
     lat_single = serve.summarize(serve.run_back_to_back(reqs, realtime=True), "single-model path, back to back")
     lat_served = serve.summarize(serve.Server().run(reqs, realtime=True), "continuous batching")
+    # every request in one batch of uneven grids (work-list launches)
+    cap_mixed = serve.summarize(serve.Server(mixed=True).run(reqs, realtime=False), "one mixed batch")
+    lat_mixed = serve.summarize(serve.Server(mixed=True).run(reqs, realtime=True), "one mixed batch")
     # the stream's step kernels alone (single-model launches, no gaps): a
     # scheme that only removed host work and launch gaps could not finish
     # the queued stream faster than this
@@ -221,13 +225,15 @@ def serving_c4(nreq=60, rate=20.0):
             "speedup": round(cap_served["solves_per_s"] / cap_single["solves_per_s"], 2),
             "p50_s": lat_served["p50_s"], "p99_s": lat_served["p99_s"],
             "single_path_p50_s": lat_single["p50_s"], "single_path_p99_s": lat_single["p99_s"],
+            "mixed_batch_solves_per_s": cap_mixed["solves_per_s"],
+            "mixed_batch_p50_s": lat_mixed["p50_s"], "mixed_batch_p99_s": lat_mixed["p99_s"],
             "step_kernel_s": round(kernel_s, 4),
             "step_kernel_s_by_grid": {"x".join(map(str, serve.GRIDS[g][:3])): round(v, 4) for g, v in work.items()},
             "step_kernel_s_over_served_makespan": round(kernel_s / cap_served["makespan_s"], 3),
             "speedup_bound_from_step_kernels": round(cap_single["makespan_s"] / kernel_s, 2),
             "launch_shapes": {"%s %s" % ("x".join(map(str, serve.GRIDS[gi][:3])), kind): [
                 c["fft_cols"], c["green"], c["bodyforce"]] for (gi, kind), c in tuned.items()},
-            "capacity": [cap_single, cap_served], "at_arrival_rate": [lat_single, lat_served]}
+            "capacity": [cap_single, cap_served, cap_mixed], "at_arrival_rate": [lat_single, lat_served, lat_mixed]}
 
 
 def main():

@@ -1063,6 +1063,42 @@ static int traction_rates(const Ctx& c, int pred) {
   return RELAX_OK;
 }
 
+// The patch velocities of the context's afterslip planes (monitorfriction),
+// after the corrector's divergence pass.
+static int monitor_friction(const Ctx& c) {
+  int rc;
+  for (int i = 0; i < c.nm; ++i) {
+    relax_model_t* m = c.models[i];
+    if (!(m->R.fr.active && m->patches_dev)) continue;
+    for (int k = 0; k < m->R.fr.np; ++k) {
+      const relax_plane_t& p = m->host_planes[k];
+      if ((rc = launch_monitorfriction(m->plane_dev_host[k], p.rake, p.px2, p.px3, m->g, m->desc.dx1, m->desc.dx2,
+                                       m->desc.dx3, m->g.n3, m->sig, m->R.fr.v0, m->R.fr.tauc, m->R.fr.friction,
+                                       m->R.fr.cohesion, m->patches_dev + m->patch_off[k], m->sc, c.st)))
+        return rc;
+      cmark(c, RELAX_PROF_SMALL);
+    }
+  }
+  return RELAX_OK;
+}
+
+// The end of the step of every model of the context (all of one grid): the
+// patch slip (frictionadd), the amplitude reductions, t += Dt.
+static int end_steps(const Ctx& c) {
+  const int nm = c.nm;
+  for (int i = 0; i < nm; ++i) {
+    relax_model_t* m = c.models[i];
+    if (!(m->patches_dev && m->patch_off.back() > 0)) continue;
+    const long n = m->patch_off.back();
+    k_frictionadd<<<(unsigned)((n + 255) / 256), 256, 0, c.st>>>(m->patches_dev, n, m->sc, 0, 0.0);
+    cmark(c, RELAX_PROF_SMALL);
+  }
+  k_reduce_partials<<<dim3((unsigned)c.lead->nred, (unsigned)nm), 256, 0, c.st>>>(c.md, c.lead->nblk);
+  k_end_step<<<nm, 256, 0, c.st>>>(c.md, c.lead->nblk, c.lead->nred);
+  cmark(c, RELAX_PROF_SMALL);
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
 // One RK2 step of every model of the context, enqueued without host
 // synchronisation.
 static int enqueue_steps(const Ctx& c, const double* dt_override) {
@@ -1084,33 +1120,12 @@ static int enqueue_steps(const Ctx& c, const double* dt_override) {
   // corrector
   if ((rc = launch_bodyforce(g, S, c.md, c.one, nm, c.rclass, BODY_CORR, st, c.tune_body))) return rc;
   cmark(c, RELAX_PROF_BODYFORCE);
-  for (int i = 0; i < nm; ++i) {
-    relax_model_t* m = c.models[i];
-    if (!(m->R.fr.active && m->patches_dev)) continue;
-    for (int k = 0; k < m->R.fr.np; ++k) {
-      const relax_plane_t& p = m->host_planes[k];
-      if ((rc = launch_monitorfriction(m->plane_dev_host[k], p.rake, p.px2, p.px3, m->g, m->desc.dx1, m->desc.dx2,
-                                       m->desc.dx3, m->g.n3, m->sig, m->R.fr.v0, m->R.fr.tauc, m->R.fr.friction,
-                                       m->R.fr.cohesion, m->patches_dev + m->patch_off[k], m->sc, st)))
-        return rc;
-      cmark(c, RELAX_PROF_SMALL);
-    }
-  }
+  if ((rc = monitor_friction(c))) return rc;
   if ((rc = traction_rates(c, 0))) return rc;
   if ((rc = model_green(c, EPI_CORR))) return rc;
   if ((rc = launch_stress(STRESS_CORR, g, S, c.md, c.one, nm, c.rclass, st))) return rc;
   cmark(c, RELAX_PROF_STRESS_CORR);
-  for (int i = 0; i < nm; ++i) {
-    relax_model_t* m = c.models[i];
-    if (!(m->patches_dev && m->patch_off.back() > 0)) continue;
-    const long n = m->patch_off.back();
-    k_frictionadd<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(m->patches_dev, n, m->sc, 0, 0.0);
-    cmark(c, RELAX_PROF_SMALL);
-  }
-  k_reduce_partials<<<dim3((unsigned)c.lead->nred, (unsigned)nm), 256, 0, st>>>(c.md, c.lead->nblk);
-  k_end_step<<<nm, 256, 0, st>>>(c.md, c.lead->nblk, c.lead->nred);
-  cmark(c, RELAX_PROF_SMALL);
-  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+  return end_steps(c);
 }
 
 static int enqueue_step(relax_model_t* m, const double* dt_override) {
@@ -1794,6 +1809,22 @@ struct relax_batch {
   hipGraphExec_t exec = nullptr;
   bool tuned = false;               // tune holds relax_batch_autotune's choice
   int tune[3] = {0, 0, 0};
+  int tune_grid[3] = {0, 0, 0};     // the grid it was made for
+  // Members of several grid classes (grid, spacing, elastic constants): the
+  // records in md_dev are grouped by class, member i's at rec[i]; `ordered`
+  // lists the members in record order.  One class: rec is the identity.
+  struct ClassRange {
+    int off, n;
+  };
+  std::vector<int> rec;
+  std::vector<relax_model_t*> ordered;
+  std::vector<ClassRange> classes;
+  // mixed batches: the work lists of the strain pass (entries [0, nm]) and
+  // of the divergence pass ([nm + 1, 2 nm + 1]) over all members
+  WorkItem* wl = nullptr;
+  int wl_cap = 0;
+  long wl_stress = 0, wl_body = 0;
+  int body_variant = -1;
 };
 
 // A membership that changes every step or two (a server admitting and
@@ -1849,6 +1880,7 @@ int relax_batch_destroy(relax_batch_t* b) {
   (void)hipStreamSynchronize(b->stream);
   for (relax_model_t* m : b->models) m->batch = nullptr;
   drop_graph(b);
+  if (b->wl) (void)rx_free(b->wl);
   if (b->md_dev) (void)rx_free(b->md_dev);
   if (b->gather) (void)rx_free(b->gather);
   if (b->host_sc) (void)rx_host_free(b->host_sc);
@@ -1860,8 +1892,6 @@ int relax_batch_destroy(relax_batch_t* b) {
 int relax_batch_add(relax_batch_t* b, relax_model_t* m) {
   if (!b || !m || m->batch) return RELAX_ERR_ARG;
   if (m->device != b->device) return RELAX_ERR_ARG;
-  if (!b->models.empty() && !batch_compatible(b->models[0], m)) return RELAX_ERR_ARG;
-  if (b->models.empty()) b->tuned = false;  // the grid may differ from the last members'
   int rc;
   if ((rc = sync_md(m))) return rc;
   HIP_TRY(hipStreamSynchronize(m->stream));  // the model's own work is done before the batch's
@@ -1890,6 +1920,54 @@ int relax_batch_synchronize(relax_batch_t* b) {
   return RELAX_OK;
 }
 
+// Group the members by grid class (first-seen order of the classes, the
+// members' order within each): rec, ordered and classes.
+static void batch_classes(relax_batch_t* b) {
+  const int nm = (int)b->models.size();
+  std::vector<int> cls(nm, -1);
+  std::vector<int> leads;  // a member of each class
+  for (int i = 0; i < nm; ++i) {
+    for (size_t k = 0; k < leads.size(); ++k)
+      if (batch_compatible(b->models[leads[k]], b->models[i])) { cls[i] = (int)k; break; }
+    if (cls[i] < 0) { cls[i] = (int)leads.size(); leads.push_back(i); }
+  }
+  b->rec.assign(nm, 0);
+  b->ordered.clear();
+  b->classes.clear();
+  for (int k = 0; k < (int)leads.size(); ++k) {
+    relax_batch_t::ClassRange r{(int)b->ordered.size(), 0};
+    for (int i = 0; i < nm; ++i)
+      if (cls[i] == k) { b->rec[i] = (int)b->ordered.size(); b->ordered.push_back(b->models[i]); ++r.n; }
+    b->classes.push_back(r);
+  }
+}
+
+static int batch_rclass(const relax_batch_t* b) {
+  int rc = RHEO_VISCOUS;
+  for (relax_model_t* m : b->models) rc = std::max(rc, rheo_class(m));
+  return rc;
+}
+
+static bool tune_entry(int dev, const Grid& g, int rclass, TuneChoice* out);
+
+// The divergence variant of a mixed batch: the choice for its largest class
+// (relax_batch_autotune's table, else that class's first member's), if it
+// tiles every member's grid, else the 32x8 tiles (variant 1), which tile
+// every grid a model can have.
+static int mixed_body_variant(const relax_batch_t* b, int rclass) {
+  long best = -1;
+  const relax_model_t* lead = nullptr;
+  for (const auto& k : b->classes) {
+    const long pts = (long)k.n * b->ordered[k.off]->g.H;
+    if (pts > best) { best = pts; lead = b->ordered[k.off]; }
+  }
+  TuneChoice t;
+  int v = tune_entry(b->device, lead->g, rclass, &t) ? t.body : lead->tune_body;
+  for (const relax_model_t* m : b->models)
+    if (!body_variant_fits(m->g, v)) return 1;
+  return v;
+}
+
 // Bring the device records up to date; a change drops the captured step.
 static int batch_refresh(relax_batch_t* b) {
   const int nm = (int)b->models.size();
@@ -1902,6 +1980,9 @@ static int batch_refresh(relax_batch_t* b) {
     }
     if (!changed && b->versions[i] != m->md_version) changed = true;
   }
+  // a mixed batch's divergence variant follows the tuning table
+  if (!changed && b->classes.size() > 1 && mixed_body_variant(b, batch_rclass(b)) != b->body_variant)
+    changed = true;
   if (!changed) return RELAX_OK;
   HIP_TRY(hipStreamSynchronize(b->stream));
   drop_graph(b);
@@ -1916,13 +1997,43 @@ static int batch_refresh(relax_batch_t* b) {
     HIP_TRY(rx_host_malloc(&b->host_sc, sizeof(StepScalars) * cap));
     b->cap = cap;
   }
+  batch_classes(b);
   b->md_host.resize(nm);
   b->versions.resize(nm);
   for (int i = 0; i < nm; ++i) {
-    b->md_host[i] = b->models[i]->md_host;
+    b->md_host[b->rec[i]] = b->models[i]->md_host;
     b->versions[i] = b->models[i]->md_version;
   }
   HIP_TRY(hipMemcpy(b->md_dev, b->md_host.data(), sizeof(ModelDev) * nm, hipMemcpyHostToDevice));
+  b->body_variant = -1;
+  if (b->classes.size() > 1) {
+    // the work lists, in record order
+    const int v = mixed_body_variant(b, batch_rclass(b));
+    std::vector<WorkItem> wl(2 * (nm + 1));
+    long ns = 0, nb = 0;
+    for (int r = 0; r <= nm; ++r) {
+      WorkItem& a = wl[r];
+      WorkItem& c = wl[nm + 1 + r];
+      a.first = ns;
+      c.first = nb;
+      if (r == nm) break;
+      const relax_model_t* m = b->ordered[r];
+      a.g = c.g = m->g;
+      a.S = c.S = m->S;
+      ns += stress_blocks(m->g);
+      nb += body_items(m->g, v);
+    }
+    if ((int)wl.size() > b->wl_cap) {
+      if (b->wl) { (void)rx_free(b->wl); b->wl = nullptr; }
+      b->wl_cap = 0;
+      HIP_TRY(rx_malloc(&b->wl, sizeof(WorkItem) * wl.size() * 2));
+      b->wl_cap = (int)wl.size() * 2;
+    }
+    HIP_TRY(hipMemcpy(b->wl, wl.data(), sizeof(WorkItem) * wl.size(), hipMemcpyHostToDevice));
+    b->wl_stress = ns;
+    b->wl_body = nb;
+    b->body_variant = v;
+  }
   b->dirty = false;
   b->stable = 0;
   return RELAX_OK;
@@ -1944,25 +2055,70 @@ struct BatchTuneKey {
 };
 static std::map<BatchTuneKey, TuneChoice> g_btune;  // guarded by g_tune_mu
 
-static Ctx batch_ctx(relax_batch_t* b) {
+static bool tune_entry(int dev, const Grid& g, int rclass, TuneChoice* out) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_btune.find(BatchTuneKey{dev, g.n1, g.n2, g.n3, rclass});
+  if (it == g_btune.end()) return false;
+  *out = it->second;
+  return true;
+}
+
+// The launch context of grid class k of a batch (records off .. off+n-1):
+// the batch's own choice when it was made for this grid, else the table's
+// for the grid and the batch's rheology class, else the class's first
+// member's shapes.
+static Ctx class_ctx(relax_batch_t* b, const relax_batch_t::ClassRange& k, int rclass) {
   Ctx c;
-  c.lead = b->models[0]; c.md = b->md_dev; c.one = nullptr; c.nm = (int)b->models.size(); c.st = b->stream;
-  c.models = b->models.data();
+  c.lead = b->ordered[k.off]; c.md = b->md_dev + k.off; c.one = nullptr; c.nm = k.n; c.st = b->stream;
+  c.models = b->ordered.data() + k.off;
   c.prof = nullptr;
-  c.rclass = RHEO_VISCOUS;
-  for (relax_model_t* m : b->models) c.rclass = std::max(c.rclass, rheo_class(m));
+  c.rclass = rclass;
+  const Grid& g = c.lead->g;
   c.tune_cols = c.lead->tune_cols; c.tune_green = c.lead->tune_green; c.tune_body = c.lead->tune_body;
-  if (b->tuned) {
+  TuneChoice t;
+  if (b->tuned && b->tune_grid[0] == g.n1 && b->tune_grid[1] == g.n2 && b->tune_grid[2] == g.n3) {
     c.tune_cols = b->tune[0]; c.tune_green = b->tune[1]; c.tune_body = b->tune[2];
-  } else {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    const Grid& g = c.lead->g;
-    auto it = g_btune.find(BatchTuneKey{b->device, g.n1, g.n2, g.n3, c.rclass});
-    if (it != g_btune.end()) {
-      c.tune_cols = it->second.cols; c.tune_green = it->second.green; c.tune_body = it->second.body;
-    }
+  } else if (tune_entry(b->device, g, rclass, &t)) {
+    c.tune_cols = t.cols; c.tune_green = t.green; c.tune_body = t.body;
   }
   return c;
+}
+
+// The launch context of a batch of one grid class.
+static Ctx batch_ctx(relax_batch_t* b) { return class_ctx(b, b->classes[0], batch_rclass(b)); }
+
+// One RK2 step of every member of a mixed batch: the divergence and strain
+// passes as one work-list launch each over all members, the Green passes
+// (transforms of their grid's lengths) and the end of the step per class.
+static int enqueue_steps_mixed(relax_batch_t* b, int rclass) {
+  hipStream_t st = b->stream;
+  const int nm = (int)b->models.size();
+  const ModelDev* md = b->md_dev;
+  const WorkItem* wls = b->wl;
+  const WorkItem* wlb = b->wl + (nm + 1);
+  std::vector<Ctx> cs;
+  for (const auto& k : b->classes) cs.push_back(class_ctx(b, k, rclass));
+  int rc;
+  // predictor
+  if ((rc = launch_bodyforce_list(wlb, md, nm, b->wl_body, 0, rclass, BODY_PRED, st, b->body_variant))) return rc;
+  k_finalize_dt<<<(nm + 63) / 64, 64, 0, st>>>(md, nm, 0, 0.0);
+  for (const Ctx& c : cs)
+    if ((rc = traction_rates(c, 1))) return rc;
+  for (const Ctx& c : cs)
+    if ((rc = model_green(c, EPI_PRED))) return rc;
+  if ((rc = launch_stress_list(STRESS_PRED, wls, md, nm, b->wl_stress, 0, rclass, st))) return rc;
+  // corrector
+  if ((rc = launch_bodyforce_list(wlb, md, nm, b->wl_body, 0, rclass, BODY_CORR, st, b->body_variant))) return rc;
+  for (const Ctx& c : cs)
+    if ((rc = monitor_friction(c))) return rc;
+  for (const Ctx& c : cs)
+    if ((rc = traction_rates(c, 0))) return rc;
+  for (const Ctx& c : cs)
+    if ((rc = model_green(c, EPI_CORR))) return rc;
+  if ((rc = launch_stress_list(STRESS_CORR, wls, md, nm, b->wl_stress, 0, rclass, st))) return rc;
+  for (const Ctx& c : cs)
+    if ((rc = end_steps(c))) return rc;
+  return RELAX_OK;
 }
 
 int relax_batch_step(relax_batch_t* b, relax_step_info_t* infos) {
@@ -1971,17 +2127,22 @@ int relax_batch_step(relax_batch_t* b, relax_step_info_t* infos) {
   if (nm == 0) return RELAX_OK;
   int rc;
   if ((rc = batch_refresh(b))) return rc;
+  const bool mixed = b->classes.size() > 1;
+  const int rclass = batch_rclass(b);
   const Ctx c = batch_ctx(b);
-  if (!b->exec && (b->stable < GRAPH_AFTER || (long)nm * b->models[0]->g.H >= GRAPH_MAX_POINTS)) {
+  auto enqueue = [&]() { return mixed ? enqueue_steps_mixed(b, rclass) : enqueue_steps(c, nullptr); };
+  long points = 0;
+  for (relax_model_t* m : b->models) points += m->g.H;
+  if (!b->exec && (b->stable < GRAPH_AFTER || points >= GRAPH_MAX_POINTS)) {
     ++b->stable;
-    if ((rc = enqueue_steps(c, nullptr))) return rc;
+    if ((rc = enqueue())) return rc;
   } else if (!b->exec) {
     // capture one step of all members; replayed until the membership or a
     // member's record changes
     // (no other thread frees memory or synchronises the device meanwhile)
     CaptureGuard guard;
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_steps(c, nullptr);
+    rc = enqueue();
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc || e != hipSuccess) {
@@ -2002,8 +2163,8 @@ int relax_batch_step(relax_batch_t* b, relax_step_info_t* infos) {
   HIP_TRY(hipStreamSynchronize(b->stream));
   int first = RELAX_OK;
   for (int i = 0; i < nm; ++i) {
-    fill_info(b->host_sc[i], infos + i);
-    if (!first && b->host_sc[i].status) first = b->host_sc[i].status;
+    fill_info(b->host_sc[b->rec[i]], infos + i);
+    if (!first && b->host_sc[b->rec[i]].status) first = b->host_sc[b->rec[i]].status;
   }
   return first;
 }
@@ -2022,7 +2183,7 @@ int relax_batch_scalars(relax_batch_t* b, relax_step_info_t* infos) {
   k_gather_scalars<<<(nm + 63) / 64, 64, 0, b->stream>>>(b->md_dev, nm, b->gather);
   HIP_TRY(hipMemcpyAsync(b->host_sc, b->gather, sizeof(StepScalars) * nm, hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
-  for (int i = 0; i < nm; ++i) fill_info(b->host_sc[i], infos + i);
+  for (int i = 0; i < nm; ++i) fill_info(b->host_sc[b->rec[i]], infos + i);
   return RELAX_OK;
 }
 
@@ -2130,6 +2291,7 @@ int relax_batch_autotune(relax_batch_t* b, int* choice, float* ms) {
   if (!b || b->models.empty()) return RELAX_ERR_ARG;
   int rc;
   if ((rc = batch_refresh(b))) return rc;
+  if (b->classes.size() > 1) return RELAX_ERR_ARG;  // tune each grid class as a batch of its own
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->tuned = false;
   const Ctx c = batch_ctx(b);
@@ -2139,6 +2301,7 @@ int relax_batch_autotune(relax_batch_t* b, int* choice, float* ms) {
   drop_graph(b);  // a captured step keeps the shapes it was captured with
   b->tuned = true;
   for (int k = 0; k < 3; ++k) b->tune[k] = best[k];
+  b->tune_grid[0] = c.lead->g.n1; b->tune_grid[1] = c.lead->g.n2; b->tune_grid[2] = c.lead->g.n3;
   {
     const Grid& g = c.lead->g;
     std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -30,5 +30,23 @@ struct StencilConst {
 int launch_bodyforce(const Grid& g, const StencilConst& S, const ModelDev* md, const ModelDev* one, int nm,
                      int rclass, int mode, hipStream_t st, int variant = 0);
 long stress_blocks(const Grid& g);
+
+// Launches over models of different grids (a mixed batch): wl is a device
+// array of nm + 1 entries, entry m model m's grid, stencil constants and
+// first work item, entry nm's first the total; `grid` workgroups walk the
+// items (0, or the total: one item per workgroup).
+struct WorkItem {
+  long first;
+  Grid g;
+  StencilConst S;
+};
+// whether divergence variant `variant` tiles grid g, and its work items
+// (tiles) on g; the strain pass has stress_blocks(g) items
+int body_variant_fits(const Grid& g, int variant);
+long body_items(const Grid& g, int variant);
+int launch_bodyforce_list(const WorkItem* wl, const ModelDev* md, int nm, long total, long grid, int rclass,
+                          int mode, hipStream_t st, int variant);
+int launch_stress_list(int mode, const WorkItem* wl, const ModelDev* md, int nm, long total, long grid, int rclass,
+                       hipStream_t st);
 int launch_stress(int mode, const Grid& g, const StencilConst& S, const ModelDev* md, const ModelDev* one, int nm,
                   int rclass, hipStream_t st);

@@ -463,11 +463,10 @@ __device__ __forceinline__ void stress_block(const Grid& g, const RheoParams& R,
                                              const float* __restrict__ X, float* __restrict__ sig,
                                              float* __restrict__ tau, float* __restrict__ gam,
                                              float* __restrict__ E, float* __restrict__ Em, StepScalars* sc,
-                                             double2* __restrict__ partials) {
+                                             double2* __restrict__ partials, int blk) {
   const int tid = threadIdx.x;
   // blocks in address order: an XCD-aware order (neighbouring rows in one
   // L2) measured 9-15 % slower at 512x512x256 than this streaming order
-  const int blk = blockIdx.x;
   const long o = (long)blk * 256 + tid;
   const int n1 = g.n1, n2 = g.n2, h3 = g.h3;
   const long H = g.H;
@@ -512,7 +511,61 @@ __global__ __launch_bounds__(256) void k_stress(Grid g, ModelDev Dv, const Model
   // model blockIdx.y of the launch (BATCH) or the argument Dv
   const ModelDev& D = BATCH ? md[blockIdx.y] : Dv;
   stress_block<MODE, RC>(g, D.R, S, MODE == STRESS_PRED ? D.W : D.u, D.sig, D.tau, D.gam, D.E, D.Em, D.sc,
-                         D.partials);
+                         D.partials, blockIdx.x);
+}
+
+// ---------------------------------------------- models of uneven grids ----
+// A launch over the nm models of md whose grids differ walks a work list:
+// entry m gives model m's grid, stencil constants and its first work item
+// (a workgroup's share: 256 points of the strain pass, one tile of the
+// divergence pass); entry nm's `first` is the total.  Workgroup b takes
+// items b, b + gridDim.x, ... (with one item per workgroup the XCD-aware
+// order of the divergence tiles is kept).  The item's model is found by
+// binary search over the entries (uniform across the workgroup: scalar
+// loads).
+__device__ __forceinline__ int work_model(const WorkItem* __restrict__ wl, int nm, long item) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (wl[mid].first <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int MODE, int RC>
+__global__ __launch_bounds__(256) void k_stress_list(const ModelDev* __restrict__ md,
+                                                     const WorkItem* __restrict__ wl, int nm) {
+  const long total = wl[nm].first;
+  for (long item = blockIdx.x; item < total; item += gridDim.x) {
+    const int m = work_model(wl, nm, item);
+    const ModelDev& D = md[m];
+    stress_block<MODE, RC>(wl[m].g, D.R, wl[m].S, MODE == STRESS_PRED ? D.W : D.u, D.sig, D.tau, D.gam, D.E, D.Em,
+                           D.sc, D.partials, (int)(item - wl[m].first));
+    if (MODE == STRESS_CORR) __syncthreads();  // the partials' LDS is reused by the next item
+  }
+}
+
+template <int TXB, int TYB, bool TM, bool RAW, int RC, bool ZR>
+__global__ __launch_bounds__(TXB* TYB) void k_bodyforce_list(const ModelDev* __restrict__ md,
+                                                             const WorkItem* __restrict__ wl, int nm, int zc) {
+  const long total = wl[nm].first;
+  for (long b = blockIdx.x; b < total; b += gridDim.x) {
+    const long item = gridDim.x == total ? (long)xcd_tile((int)b, (int)total) : b;
+    const int m = work_model(wl, nm, item);
+    const ModelDev& D = md[m];
+    const Grid& g = wl[m].g;
+    const int nx = g.n1 / TXB, ny = g.n2 / TYB, gz = (g.h3 + zc - 1) / zc;
+    const int lt = (int)(item - wl[m].first);
+    const int bx = lt % nx, by = (lt / nx) % ny, bz = lt / (nx * ny);
+    if constexpr (ZR)
+      bodyforce_tile_zr<TXB, TYB, TM, RAW, RC>(g, D.R, wl[m].S, RAW ? D.tau : D.sig, TM ? D.E : D.Em, D.W, D.T,
+                                               D.sc, bx, by, bz, zc, gz);
+    else
+      bodyforce_tile<TXB, TYB, TM, RAW, RC>(g, D.R, wl[m].S, RAW ? D.tau : D.sig, TM ? D.E : D.Em, D.W, D.T, D.sc,
+                                            bx, by, bz, zc, gz);
+    __syncthreads();  // the tile's LDS is reused by the next item
+  }
 }
 
 int launch_bodyforce(const Grid& g, const StencilConst& S, const ModelDev* md, const ModelDev* one, int nm,
@@ -567,6 +620,74 @@ int launch_stress(int mode, const Grid& g, const StencilConst& S, const ModelDev
     if (batch) k_stress<MODE, F, true><<<grid, 256, 0, st>>>(g, Dv, md, S);  \
     else k_stress<MODE, F, false><<<grid, 256, 0, st>>>(g, Dv, md, S);       \
   }
+#define KM(MODE)                                          \
+  {                                                       \
+    if (rclass == RHEO_EXT) K(MODE, RHEO_EXT)             \
+    else if (rclass == RHEO_CREEP) K(MODE, RHEO_CREEP)    \
+    else K(MODE, RHEO_VISCOUS)                            \
+  }
+  if (mode == STRESS_SET) K(STRESS_SET, RHEO_VISCOUS)
+  else if (mode == STRESS_PRED) KM(STRESS_PRED)
+  else KM(STRESS_CORR)
+#undef K
+#undef KM
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
+// --------------------------------------------------- work-list launches ----
+int body_variant_fits(const Grid& g, int variant) {
+  switch (variant) {
+    case 0: return g.n1 % 32 == 0 && g.n2 % 16 == 0;
+    case 1: return g.n1 % 32 == 0 && g.n2 % 8 == 0;
+    case 2: case 3: case 4: return g.n1 % 64 == 0 && g.n2 % 8 == 0;
+    case 5: return g.n1 % 64 == 0 && g.n2 % 16 == 0;
+    default: return 0;
+  }
+}
+
+long body_items(const Grid& g, int variant) {
+  static const int TX[BODY_VARIANTS] = {32, 32, 64, 64, 64, 64}, TY[BODY_VARIANTS] = {16, 8, 8, 8, 8, 16};
+  const int zc = variant >= 3 ? 32 : 16;
+  return (long)(g.n1 / TX[variant]) * (g.n2 / TY[variant]) * ((g.h3 + zc - 1) / zc);
+}
+
+int launch_bodyforce_list(const WorkItem* wl, const ModelDev* md, int nm, long total, long grid, int rclass,
+                          int mode, hipStream_t st, int variant) {
+  if (rclass < RHEO_VISCOUS || rclass > RHEO_EXT || variant < 0 || variant >= BODY_VARIANTS || nm < 1)
+    return RELAX_ERR_ARG;
+  if (total < 1) return RELAX_OK;
+  const int zc = variant >= 3 ? 32 : 16;
+  const unsigned gx = (unsigned)(grid > 0 && grid < total ? grid : total);
+#define LB(TX, TY, ZR, F)                                                                                     \
+  {                                                                                                           \
+    if (mode == BODY_PRED) k_bodyforce_list<TX, TY, true, false, F, ZR><<<gx, TX * TY, 0, st>>>(md, wl, nm, zc); \
+    else if (mode == BODY_CORR) k_bodyforce_list<TX, TY, false, false, F, ZR><<<gx, TX * TY, 0, st>>>(md, wl, nm, zc); \
+    else k_bodyforce_list<TX, TY, false, true, RHEO_VISCOUS, ZR><<<gx, TX * TY, 0, st>>>(md, wl, nm, zc); \
+  }
+#define L(TX, TY, ZR)                                         \
+  {                                                           \
+    if (rclass == RHEO_EXT) LB(TX, TY, ZR, RHEO_EXT)          \
+    else if (rclass == RHEO_CREEP) LB(TX, TY, ZR, RHEO_CREEP) \
+    else LB(TX, TY, ZR, RHEO_VISCOUS)                         \
+  }
+  switch (variant) {
+    case 0: L(32, 16, false) break;
+    case 1: L(32, 8, false) break;
+    case 2: case 3: L(64, 8, false) break;
+    case 4: L(64, 8, true) break;
+    default: L(64, 16, true) break;
+  }
+#undef LB
+#undef L
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
+int launch_stress_list(int mode, const WorkItem* wl, const ModelDev* md, int nm, long total, long grid, int rclass,
+                       hipStream_t st) {
+  if (rclass < RHEO_VISCOUS || rclass > RHEO_EXT || nm < 1) return RELAX_ERR_ARG;
+  if (total < 1) return RELAX_OK;
+  const unsigned gx = (unsigned)(grid > 0 && grid < total ? grid : total);
+#define K(MODE, F) k_stress_list<MODE, F><<<gx, 256, 0, st>>>(md, wl, nm);
 #define KM(MODE)                                          \
   {                                                       \
     if (rclass == RHEO_EXT) K(MODE, RHEO_EXT)             \

@@ -13,6 +13,10 @@ while their HBM footprint (relax_model_footprint) fits the budget, joining
 their class's batch between two of its steps; finished requests leave, return
 their surface displacement, and free their memory for the queue.
 
+With mixed=True the server steps every admitted request, whatever its grid,
+in one batch: the divergence and strain passes are one work-list launch over
+all members, the transforms and Green passes one launch per grid class.
+
 The baseline runs the same requests back to back through the single-model
 path: set-up, steps, surface displacement, release -- one at a time.
 """
@@ -35,6 +39,11 @@ WEIGHTS = (6, 3, 1)
 # beyond it one batched step already keeps HBM busy, and more members only
 # lengthen every member's step.
 MAX_BATCH_POINTS = 1 << 25
+# A mixed batch (Server(mixed=True): every class in one batch) holds at most
+# this many grid points over its members: one 512x512x256 request and the
+# smaller ones beside it.
+MIXED_BATCH_POINTS = 1 << 27
+MIXED = "mixed"  # the worker key of the mixed batch
 
 
 @dataclasses.dataclass
@@ -51,8 +60,10 @@ class Request:
 
 
 def grid_class(s):
-    """The batch class of a spec: models step together only on the same
-    grid, spacing and elastic constants (relax_batch_add)."""
+    """The batch class of a spec: its grid, spacing and elastic constants.
+    The per-class server steps the models of one class as one batch (its
+    transforms and Green passes share launches); a mixed batch holds
+    several classes (relax_batch_add)."""
     return (s["sx1"], s["sx2"], s["sx3"]) + workloads.spacing(s) + workloads.constants(s)[:3]
 
 
@@ -140,6 +151,10 @@ def _surface(m, s):
 
 def _footprint(s):
     return footprint(s["sx1"], s["sx2"], s["sx3"])
+
+
+def _points(s):
+    return s["sx1"] * s["sx2"] * s["sx3"]
 
 
 def batch_cap(cls):
@@ -238,9 +253,13 @@ class Server:
     are released when an admission needs their memory, and a request that
     does not fit waits until memory is returned."""
 
-    def __init__(self, budget_bytes=None, reserve=0.1, admit_per_pass=2, chunk=4):
+    def __init__(self, budget_bytes=None, reserve=0.1, admit_per_pass=2, chunk=4, mixed=False):
         self.admit_per_pass = admit_per_pass
         self.chunk = chunk
+        # mixed: one worker steps every admitted request in one batch of
+        # uneven grids (relax_batch_* work-list launches), up to
+        # MIXED_BATCH_POINTS grid points over its members
+        self.mixed = mixed
         if budget_bytes is None:
             free, _ = meminfo()
             budget_bytes = int(free * (1.0 - reserve))
@@ -290,6 +309,13 @@ class Server:
             self.pool.setdefault(cls, []).append(m)
             self._mem.notify_all()
 
+    def _room(self, cls, members, r):
+        """Whether request r may join the batch of `members`."""
+        if cls != MIXED:
+            return len(members) < batch_cap(cls)
+        pts = sum(_points(e[0].spec) for e in members)
+        return not members or pts + _points(r.spec) <= MIXED_BATCH_POINTS
+
     def _worker(self, cls, inbox, results, t0, realtime):
         batch = Batch()
         members = []  # [request, model, steps taken, start]
@@ -299,10 +325,10 @@ class Server:
                 # admission in arrival order, a few per pass, while the
                 # class has room and the memory fits
                 admitted = 0
-                while admitted < self.admit_per_pass and len(members) < batch_cap(cls):
+                while admitted < self.admit_per_pass:
                     with inbox["cv"]:
                         r = waiting[0] if waiting else None
-                    if r is None or not self._reserve(r):
+                    if r is None or not self._room(cls, members, r) or not self._reserve(r):
                         break
                     with inbox["cv"]:
                         waiting.pop(0)
@@ -325,7 +351,7 @@ class Server:
                 # up to `chunk` steps at once (no member past its last
                 # step), one while requests wait that the class could admit
                 with inbox["cv"]:
-                    more = bool(waiting) and len(members) < batch_cap(cls)
+                    more = bool(waiting) and self._room(cls, members, waiting[0])
                 nst = 1 if more else min(self.chunk, min(r.nsteps - k for r, _, k, _ in members))
                 for _ in range(nst):
                     batch.step(info=False)
@@ -337,7 +363,7 @@ class Server:
                     if k >= r.nsteps or rec["status"]:
                         batch.remove(m)
                         surf = _surface(m, r.spec)
-                        self._release(cls, m)
+                        self._release(r.cls, m)
                         res = Result(r.rid, r.arrival if realtime else 0.0, start, time.perf_counter() - t0, k,
                                      rec["t"], rec["status"], surf)
                         with inbox["cv"]:
@@ -360,10 +386,11 @@ class Server:
             now = time.perf_counter() - t0
             if realtime and r.arrival > now:
                 time.sleep(r.arrival - now)
-            box = inboxes.get(r.cls)
+            key = MIXED if self.mixed else r.cls
+            box = inboxes.get(key)
             if box is None:
-                box = inboxes[r.cls] = {"queue": [], "cv": threading.Condition(), "closed": False, "error": None}
-                th = threading.Thread(target=self._worker, args=(r.cls, box, results, t0, realtime), daemon=True)
+                box = inboxes[key] = {"queue": [], "cv": threading.Condition(), "closed": False, "error": None}
+                th = threading.Thread(target=self._worker, args=(key, box, results, t0, realtime), daemon=True)
                 threads.append(th)
                 th.start()
             with box["cv"]:

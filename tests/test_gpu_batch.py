@@ -134,12 +134,13 @@ def test_batch_membership_changes():
         m.close()
 
 
-def test_batch_refuses_mismatched_models_and_member_steps():
+def test_batch_refuses_member_steps_and_second_batches():
     a = workloads.build(_variant("maxwell", 0))
     other = workloads.build(_variant("maxwell", 0, sx3=32))
     with Batch([a]) as b:
+        b.add(other)  # another grid: the batch becomes a mixed one
         with pytest.raises(RelaxError):
-            b.add(other)
+            Batch([other])  # a model is in one batch at a time
         with pytest.raises(RelaxError):
             a.step()
         with pytest.raises(RelaxError):
@@ -255,4 +256,84 @@ def test_batch_autotune_keeps_state_and_members_match_their_single_runs():
             b.step(info=False)
     for i, m in enumerate(models):
         _same(m.download(), out[i])
+        m.close()
+
+
+def _mixed_specs():
+    """Four grid classes: 64^3 (two members), 128x128x64, 64x64x128 and
+    64x64x32."""
+    return [_variant("maxwell", 0), small_spec("powerlaw", sx=128, sx3=64, seed=1), _variant("powerlaw", 2, sx3=128),
+            _variant("maxwell", 3), _variant("powerlaw", 4, sx3=32)]
+
+
+def test_mixed_grid_batch_matches_single_runs_bitwise():
+    """Models of different grids in one batch: the divergence and strain
+    passes are one work-list launch over all members, the transforms and
+    the Green pass one launch per grid class.  Every member equals its
+    single run bitwise, over graph replays with eager work in between."""
+    specs = _mixed_specs()
+    nsteps = 4
+    ref = [_alone(s, nsteps) for s in specs]
+    models = [workloads.build(s) for s in specs]
+    with Batch(models) as b:
+        for k in range(nsteps):
+            recs = b.step()
+            for i in range(len(specs)):
+                assert recs[i] == ref[i][0][k], (i, k)
+            models[2].download()
+    for i, m in enumerate(models):
+        _same(m.download(), ref[i][1])
+        m.close()
+
+
+def test_mixed_grid_batch_membership_changes():
+    """A mixed batch loses the members of all but one class (a uniform batch
+    again), then gains another grid's model; each trajectory equals its
+    single run bitwise."""
+    sa, sb, sc = _variant("maxwell", 0), small_spec("powerlaw", sx=128, sx3=64, seed=1), _variant("powerlaw", 2,
+                                                                                                    sx3=128)
+    ref_a, ref_b, ref_c = _alone(sa, 6), _alone(sb, 2), _alone(sc, 2)
+    a, bm, c = workloads.build(sa), workloads.build(sb), workloads.build(sc)
+    with Batch([a, bm]) as b:
+        for _ in range(2):
+            b.step(info=False)
+        b.remove(bm)
+        for _ in range(2):
+            b.step(info=False)
+        b.add(c)
+        for _ in range(2):
+            b.step(info=False)
+        b.synchronize()
+        _same(a.download(), ref_a[1])
+        _same(bm.download(), ref_b[1])
+        _same(c.download(), ref_c[1])
+    for m in (a, bm, c):
+        m.close()
+
+
+def test_mixed_grid_batch_with_fault_creep_member():
+    """An afterslip member of one grid switches every member of the mixed
+    batch to the fault-creep kernels; the others then agree with their
+    single runs to rounding."""
+    specs = [_variant("afterslip", 0), _variant("maxwell", 1, sx3=128), small_spec("powerlaw", sx=128, sx3=64, seed=2)]
+    nsteps = 3
+    ref = [_alone(s, nsteps) for s in specs]
+    models = [workloads.build(s) for s in specs]
+    with Batch(models) as b:
+        for k in range(nsteps):
+            recs = b.step()
+            for i in range(len(specs)):
+                assert abs(recs[i]["t"] - ref[i][0][k]["t"]) <= 1e-12 * abs(ref[i][0][k]["t"])
+    for i, m in enumerate(models):
+        out = m.download()
+        for key in FIELDS:
+            a, r = out[key].astype(np.float64), ref[i][1][key].astype(np.float64)
+            nr = np.linalg.norm(r)
+            assert np.linalg.norm(a - r) <= 1e-6 * (nr if nr > 0 else 1.0), (i, key)
+        m.close()
+    models = [workloads.build(s) for s in specs[1:]]
+    with Batch(models) as b:
+        with pytest.raises(RelaxError):
+            b.autotune()  # tune each grid class as a batch of its own
+    for m in models:
         m.close()

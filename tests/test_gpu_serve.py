@@ -129,3 +129,27 @@ def test_served_with_tuned_launch_shapes():
         for ua, ub in zip(a.surface, b.surface):
             na = np.linalg.norm(ua.astype(np.float64))
             assert np.linalg.norm(ua.astype(np.float64) - ub) <= 1e-5 * max(na, 1e-30), q.rid
+
+
+def test_mixed_server_results_match_single_path():
+    """Server(mixed=True): every request in one batch of uneven grids
+    (work-list divergence and strain launches over all members); each
+    served result agrees with its result alone.  Launch shapes tuned earlier
+    in the process may differ between the paths (the Green variants differ
+    in FP32 rounding), so the adaptive time is held to 1e-5 and the surface
+    displacement to relative L2 1e-5."""
+    reqs = serve.request_stream(12, seed=6, steps=(10, 20))
+    assert len({r.grid for r in reqs}) >= 2
+    alone = {r.rid: r for r in serve.run_back_to_back(reqs, realtime=False)}
+    mixed = {r.rid: r for r in serve.Server(mixed=True).run(reqs, realtime=False)}
+    assert sorted(alone) == sorted(mixed) == sorted(r.rid for r in reqs)
+    for q in reqs:
+        a, b = alone[q.rid], mixed[q.rid]
+        assert a.status == b.status, q.rid
+        if a.status:
+            continue
+        assert b.nsteps == q.nsteps
+        assert abs(a.t - b.t) <= 1e-5 * abs(a.t), q.rid
+        for ua, ub in zip(a.surface, b.surface):
+            na = np.linalg.norm(ua.astype(np.float64))
+            assert np.linalg.norm(ua.astype(np.float64) - ub) <= 1e-5 * max(na, 1e-30), q.rid
