@@ -421,22 +421,25 @@ int relax_model_set_tuning(relax_model_t* m, const int* choice);
 /* The model's HIP stream (hipStream_t) for callers that order their own work. */
 void* relax_model_stream(relax_model_t* m);
 
-/* ---- batches: many models of one grid stepped by shared launches -------- */
-/* A batch steps all of its models together: every kernel of the RK2 step is
- * one launch over all members (the model number is part of the launch grid),
- * and the step is replayed from a captured hipGraph while the membership is
- * unchanged (batches of 2^25 grid points or more run their launches eagerly:
- * there a graph saves nothing).  Members must share the device, the grid (sx1, sx2, sx3), the
- * spacing (dx1, dx2, dx3) and lambda, mu, gamma; rheology, prestress, fault
- * planes, time line and state are per model.  There is no counterpart in the
- * reference, whose driver runs one model per process (relax.f90:185-1454);
- * each member's step is the reference's RK2 step of that model. */
+/* ---- batches: many models stepped by shared launches --------------------- */
+/* A batch steps all of its models together, and the step is replayed from a
+ * captured hipGraph while the membership is unchanged (batches of 2^25 grid
+ * points or more run their launches eagerly: there a graph saves nothing).
+ * Members of one grid class (grid sx1, sx2, sx3, spacing dx1, dx2, dx3 and
+ * lambda, mu, gamma) share every launch of the step (the model number is
+ * part of the launch grid).  A batch may hold several classes (uneven
+ * grids): the divergence and strain passes are then one launch over all
+ * members that walks a work list of (model, tile) items, the transforms and
+ * Green passes one launch per class.  Members share the device; rheology,
+ * prestress, fault planes, time line and state are per model.  There is no
+ * counterpart in the reference, whose driver runs one model per process
+ * (relax.f90:185-1454); each member's step is the reference's RK2 step of
+ * that model. */
 typedef struct relax_batch relax_batch_t;
 int relax_batch_create(relax_batch_t** out);
 /* Members leave the batch when it is destroyed; the models themselves stay. */
 int relax_batch_destroy(relax_batch_t* b);
-/* RELAX_ERR_ARG if the model is already in a batch or does not match the
- * members' grid, spacing and elastic constants. */
+/* RELAX_ERR_ARG if the model is already in a batch or on another device. */
 int relax_batch_add(relax_batch_t* b, relax_model_t* m);
 int relax_batch_remove(relax_batch_t* b, relax_model_t* m);
 int relax_batch_size(const relax_batch_t* b);
@@ -457,8 +460,10 @@ int relax_batch_scalars(relax_batch_t* b, relax_step_info_t* infos);
  * fits the batch's size; the batch keeps it, and later batches of the same
  * device, grid and rheology class in the process start from it.  A batch
  * without a choice launches with its first member's shapes.  Members equal
- * their single runs bitwise when those use the same shapes.  No counterpart
- * in the reference (it has no batches). */
+ * their single runs bitwise when those use the same shapes.  RELAX_ERR_ARG
+ * for a batch of several grid classes: tune each class as a batch of its
+ * own, and a mixed batch takes those choices per class.  No counterpart in
+ * the reference (it has no batches). */
 int relax_batch_autotune(relax_batch_t* b, int* choice, float* ms);
 
 #ifdef __cplusplus
