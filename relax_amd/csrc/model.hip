@@ -2087,6 +2087,12 @@ static Ctx class_ctx(relax_batch_t* b, const relax_batch_t::ClassRange& k, int r
 // The launch context of a batch of one grid class.
 static Ctx batch_ctx(relax_batch_t* b) { return class_ctx(b, b->classes[0], batch_rclass(b)); }
 
+// Workgroups of a mixed batch's work-list launches: 0 gives every work item
+// its own workgroup; a positive count makes them walk the list.
+#ifndef RELAX_WL_GRID
+#define RELAX_WL_GRID 0
+#endif
+
 // One RK2 step of every member of a mixed batch: the divergence and strain
 // passes as one work-list launch each over all members, the Green passes
 // (transforms of their grid's lengths) and the end of the step per class.
@@ -2100,22 +2106,22 @@ static int enqueue_steps_mixed(relax_batch_t* b, int rclass) {
   for (const auto& k : b->classes) cs.push_back(class_ctx(b, k, rclass));
   int rc;
   // predictor
-  if ((rc = launch_bodyforce_list(wlb, md, nm, b->wl_body, 0, rclass, BODY_PRED, st, b->body_variant))) return rc;
+  if ((rc = launch_bodyforce_list(wlb, md, nm, b->wl_body, RELAX_WL_GRID, rclass, BODY_PRED, st, b->body_variant))) return rc;
   k_finalize_dt<<<(nm + 63) / 64, 64, 0, st>>>(md, nm, 0, 0.0);
   for (const Ctx& c : cs)
     if ((rc = traction_rates(c, 1))) return rc;
   for (const Ctx& c : cs)
     if ((rc = model_green(c, EPI_PRED))) return rc;
-  if ((rc = launch_stress_list(STRESS_PRED, wls, md, nm, b->wl_stress, 0, rclass, st))) return rc;
+  if ((rc = launch_stress_list(STRESS_PRED, wls, md, nm, b->wl_stress, RELAX_WL_GRID, rclass, st))) return rc;
   // corrector
-  if ((rc = launch_bodyforce_list(wlb, md, nm, b->wl_body, 0, rclass, BODY_CORR, st, b->body_variant))) return rc;
+  if ((rc = launch_bodyforce_list(wlb, md, nm, b->wl_body, RELAX_WL_GRID, rclass, BODY_CORR, st, b->body_variant))) return rc;
   for (const Ctx& c : cs)
     if ((rc = monitor_friction(c))) return rc;
   for (const Ctx& c : cs)
     if ((rc = traction_rates(c, 0))) return rc;
   for (const Ctx& c : cs)
     if ((rc = model_green(c, EPI_CORR))) return rc;
-  if ((rc = launch_stress_list(STRESS_CORR, wls, md, nm, b->wl_stress, 0, rclass, st))) return rc;
+  if ((rc = launch_stress_list(STRESS_CORR, wls, md, nm, b->wl_stress, RELAX_WL_GRID, rclass, st))) return rc;
   for (const Ctx& c : cs)
     if ((rc = end_steps(c))) return rc;
   return RELAX_OK;
