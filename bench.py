@@ -236,6 +236,44 @@ def serving_c4(nreq=60, rate=20.0):
             "capacity": [cap_single, cap_served, cap_mixed], "at_arrival_rate": [lat_single, lat_served, lat_mixed]}
 
 
+def dropin_green(calls=3):
+    """The literal drop-in's cost: relax_greenfunctioncowling, the
+    per-routine entry a relinked relax.f90 calls with host arrays, on the
+    512x512x256 grid (copies of c1..c3 and t1..t3 to the device and back on
+    every call, as the reference's CUDA port does), against the same Green
+    solve on the device-resident model (its transforms and Green pass, from
+    the HIP-event phase timers of one step, divided by the step's two
+    solves)."""
+    import numpy as np
+
+    from relax_amd import green, workloads
+
+    sx1, sx2, sx3 = 512, 512, 256
+    rng = np.random.default_rng(5)
+    c = [np.zeros((sx3, sx2, sx1 + 2), np.float32) for _ in range(3)]
+    t = [np.zeros((sx2, sx1 + 2), np.float32) for _ in range(3)]
+    for a in c:
+        a[: sx3 // 2, :, :sx1] = rng.standard_normal((sx3 // 2, sx2, sx1))
+    for a in t:
+        a[:, :sx1] = rng.standard_normal((sx2, sx1))
+    green.greenfunctioncowling(*c, *t, 0.5, 0.5, 0.5, workloads.LAM, workloads.MU, 0.0)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        green.greenfunctioncowling(*c, *t, 0.5, 0.5, 0.5, workloads.LAM, workloads.MU, 0.0)
+    per_call = (time.perf_counter() - t0) / calls
+    m = workloads.build(workloads.spec("c1"))
+    m.autotune()
+    m.step(info=False)
+    prof = m.profile(4)
+    m.close()
+    solve_ms = sum(prof[k][0] for k in ("fft_x_fwd", "fft_y", "green_z", "fft_x_inv") if k in prof) / (2 * 4)
+    return {"workload": "greenfunctioncowling at 512x512x256, host arrays in and out",
+            "per_routine_ms_per_call": round(per_call * 1e3, 2),
+            "device_resident_solve_ms": round(solve_ms, 3),
+            "ratio": round(per_call * 1e3 / solve_ms, 1),
+            "bytes_copied_per_call": 2 * 4 * (3 * sx3 * sx2 * (sx1 + 2) + 3 * sx2 * (sx1 + 2))}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,11 +283,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 64 x 128^3 measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream measurement")
-    ap.add_argument("--only", choices=("batched", "serving"),
-                    help="run only this measurement and print its JSON (for profiling it alone)")
+    ap.add_argument("--only", choices=("batched", "serving", "dropin"),
+                    help="run only this measurement and print its JSON (for profiling it alone); dropin: the "
+                         "per-routine Green entry with host arrays against the device-resident solve")
     args = ap.parse_args()
     if args.only:
-        out = batched_c3() if args.only == "batched" else serving_c4()
+        out = {"batched": batched_c3, "serving": serving_c4, "dropin": dropin_green}[args.only]()
         print(json.dumps(out), flush=True)
         return
     if args.gpus != 1:
