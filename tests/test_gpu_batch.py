@@ -337,3 +337,37 @@ def test_mixed_grid_batch_with_fault_creep_member():
             b.autotune()  # tune each grid class as a batch of its own
     for m in models:
         m.close()
+
+
+def test_mixed_grid_batch_with_transient_member():
+    """A transient-creep member with weak zones (extended rheology kernels)
+    in a batch with models of two other grids: every member runs the
+    extended kernels through the work lists and agrees with its single run
+    to rounding; the transient strain follows."""
+    specs = [small_spec("transient", seed=1), _variant("maxwell", 2, sx3=128),
+             small_spec("powerlaw", sx=128, sx3=64, seed=3)]
+    nsteps = 3
+    ref = []
+    for s in specs:
+        m = workloads.build(s)
+        recs = [m.step() for _ in range(nsteps)]
+        out = m.download()
+        if s.get("ltransient"):
+            out["epsilonik"] = m.transient_strain()
+        m.close()
+        ref.append((recs, out))
+    models = [workloads.build(s) for s in specs]
+    with Batch(models) as b:
+        for k in range(nsteps):
+            recs = b.step()
+            for i in range(len(specs)):
+                assert abs(recs[i]["Dt"] / ref[i][0][k]["Dt"] - 1) < 1e-6, (i, k)
+    for i, m in enumerate(models):
+        out = m.download()
+        if i == 0:
+            out["epsilonik"] = m.transient_strain()
+        for key in FIELDS + (("epsilonik",) if i == 0 else ()):
+            a, r = out[key].astype(np.float64), ref[i][1][key].astype(np.float64)
+            nr = np.linalg.norm(r)
+            assert np.linalg.norm(a - r) <= 1e-6 * (nr if nr > 0 else 1.0), (i, key)
+        m.close()

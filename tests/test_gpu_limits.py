@@ -4,7 +4,7 @@ transform runs 1024-point rows and whose split y transform runs 128-point
 columns, and 32x32x512, whose 256-deep z columns take the LDS form of the
 Green pass.  Three RK2 steps of the power-law workload, with the default
 launch shapes and with the autotuner's, at the path's parity bound
-(relative L2 1e-4)."""
+(relative L2 1e-4); and fault creep at 1024x1024x64."""
 import numpy as np
 import pytest
 
@@ -37,3 +37,28 @@ def test_largest_grids_follow_the_oracle(sx, sx3):
         assert rel_l2(out["gamma"][..., :sx], st["gamma"][..., :sx]) < 1e-4, tuned
         for k, r in enumerate(recs):
             assert abs(r["tm"] / info[k, 1] - 1) < 1e-4, (k, tuned)
+
+
+def test_afterslip_at_the_largest_horizontal_grid():
+    """Fault creep on a friction plane at 1024x1024x64 (the afterslip kernels,
+    the patch velocities of monitorfriction and frictionadd) follows the
+    oracle over 3 steps."""
+    s = small_spec("afterslip", sx=1024, sx3=64)
+    nsteps = 3
+    st = oracle_initial_state(s)
+    om = oracle_model(s)
+    info = O.rk2_run(om, st, nsteps)
+    g = workloads.build(s)
+    recs = g.run(nsteps, dt_replay=info[:, 0])
+    out = g.download()
+    pat = g.get_patches()
+    g.close()
+    assert all(r["status"] == 0 for r in recs)
+    for k in ("u1", "u2", "u3"):
+        assert rel_l2(out[k][..., :1024], st[k][..., :1024]) < 1e-4, k
+    for k in ("sig", "tau"):
+        assert rel_l2(out[k], st[k]) < 1e-4, k
+    ref = om["patches"]
+    assert np.abs(ref["v"]).max() > 0
+    assert rel_l2(pat["slip"], ref["slip"]) < 1e-4
+    assert rel_l2(pat["v"], ref["v"]) < 1e-4
