@@ -48,7 +48,7 @@ def build(verbose=False):
 
 def build_fortran():
     """Compile MODULE relax_hip and the Fortran test drivers (per-routine
-    drop-in, RK2 model loop) with flang (skipped when the image has no
+    drop-in, RK2 model loop, batch) with flang (skipped when the image has no
     flang)."""
     import shutil
     flang = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
@@ -63,7 +63,7 @@ def build_fortran():
     if r.returncode != 0:
         raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
     exes = []
-    for name in ("drop_in", "model_loop"):
+    for name in ("drop_in", "model_loop", "batch_loop"):
         exe = os.path.join(bdir, name)
         cmd = [flang, "-cpp", "-O2", "-J", bdir, mod_obj, os.path.join(root, "tests", "fortran", name + ".f90"),
                "-L" + HERE, "-lrelax_hip", "-Wl,-rpath,$ORIGIN/../../../relax_amd", "-o", exe]

@@ -113,6 +113,11 @@ MODULE relax_hip
   TYPE, PUBLIC :: relax_model
      TYPE(c_ptr) :: h=C_NULL_PTR
   END TYPE relax_model
+  !> models stepped together (relax_batch_t): a parameter search's models,
+  !! of one grid or of several
+  TYPE, PUBLIC :: relax_batch
+     TYPE(c_ptr) :: h=C_NULL_PTR
+  END TYPE relax_batch
   !> the source lists of one event (relax_event_t)
   TYPE, BIND(C), PUBLIC :: relax_event_c
      TYPE(c_ptr) :: s=C_NULL_PTR
@@ -277,6 +282,30 @@ MODULE relax_hip
        IMPORT :: c_int, c_ptr, relax_model_desc
        TYPE(relax_model_desc), INTENT(IN) :: d
        TYPE(c_ptr), INTENT(OUT) :: h
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_batch_create(h) BIND(C,name="relax_batch_create")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), INTENT(OUT) :: h
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_batch_destroy(h) BIND(C,name="relax_batch_destroy")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_batch_add(h,m) BIND(C,name="relax_batch_add")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h, m
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_batch_remove(h,m) BIND(C,name="relax_batch_remove")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h, m
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_batch_step(h,infos) BIND(C,name="relax_batch_step")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h, infos
+     END FUNCTION
+     INTEGER(c_int) FUNCTION c_batch_autotune(h,choice,ms) BIND(C,name="relax_batch_autotune")
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: h, choice, ms
      END FUNCTION
      INTEGER(c_int) FUNCTION c_model_destroy(h) BIND(C,name="relax_model_destroy")
        IMPORT :: c_int, c_ptr
@@ -656,6 +685,50 @@ CONTAINS
     CALL check(c_model_destroy(model%h),"relax_model_destroy")
     model%h=C_NULL_PTR
   END SUBROUTINE relax_hip_model_destroy
+
+  !> batches (no counterpart in the reference, whose driver runs one model
+  !! per process): models added to a batch step together, one RK2 step of
+  !! every member per relax_hip_batch_step; a member's own step calls are
+  !! refused while it is in the batch
+  SUBROUTINE relax_hip_batch_create(batch)
+    TYPE(relax_batch), INTENT(OUT) :: batch
+    CALL check(c_batch_create(batch%h),"relax_batch_create")
+  END SUBROUTINE relax_hip_batch_create
+
+  SUBROUTINE relax_hip_batch_destroy(batch)
+    TYPE(relax_batch), INTENT(INOUT) :: batch
+    CALL check(c_batch_destroy(batch%h),"relax_batch_destroy")
+    batch%h=C_NULL_PTR
+  END SUBROUTINE relax_hip_batch_destroy
+
+  SUBROUTINE relax_hip_batch_add(batch,model)
+    TYPE(relax_batch), INTENT(IN) :: batch
+    TYPE(relax_model), INTENT(IN) :: model
+    CALL check(c_batch_add(batch%h,model%h),"relax_batch_add")
+  END SUBROUTINE relax_hip_batch_add
+
+  SUBROUTINE relax_hip_batch_remove(batch,model)
+    TYPE(relax_batch), INTENT(IN) :: batch
+    TYPE(relax_model), INTENT(IN) :: model
+    CALL check(c_batch_remove(batch%h,model%h),"relax_batch_remove")
+  END SUBROUTINE relax_hip_batch_remove
+
+  !> one RK2 step of every member; infos (one entry per member, in the order
+  !! of addition) receives their scalars when given
+  SUBROUTINE relax_hip_batch_step(batch,infos)
+    TYPE(relax_batch), INTENT(IN) :: batch
+    TYPE(relax_step_info), DIMENSION(:), INTENT(OUT), OPTIONAL, TARGET, CONTIGUOUS :: infos
+    TYPE(c_ptr) :: pi
+    pi=C_NULL_PTR
+    IF (PRESENT(infos)) pi=C_LOC(infos)
+    CALL check(c_batch_step(batch%h,pi),"relax_batch_step")
+  END SUBROUTINE relax_hip_batch_step
+
+  !> launch shapes timed on the batch (a batch of one grid class)
+  SUBROUTINE relax_hip_batch_autotune(batch)
+    TYPE(relax_batch), INTENT(IN) :: batch
+    CALL check(c_batch_autotune(batch%h,C_NULL_PTR,C_NULL_PTR),"relax_batch_autotune")
+  END SUBROUTINE relax_hip_batch_autotune
 
   !> which: RELAX_LINEAR (in%linearstruc), RELAX_POWERLAW (in%nonlinearstruc)
   !! or RELAX_FAULTCREEP (in%faultcreepstruc), sampled by viscoelasticstructure

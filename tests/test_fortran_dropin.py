@@ -88,3 +88,57 @@ def test_fortran_model_loop_matches_python_driver(tmp_path):
         np.testing.assert_array_equal(u[k][..., :sx1], ref["u%d" % (k + 1)][..., :sx1])
     np.testing.assert_array_equal(tau, ref["tau"])
     assert np.abs(u[0][..., :sx1]).max() > 0  # the padding columns are not written
+
+
+def _python_model(sx3, g0, nsteps):
+    """The model batch_loop.f90 sets up, driven alone through Python."""
+    from relax_amd import LAYER, SOURCE, TENSOR_LAYER
+    from relax_amd.model import LINEAR, Model
+
+    h3 = sx3 // 2
+    with Model(32, 32, sx3, 1.0, 1.0, 1.0, 3e4, 3e4, 8.33e-4, 0.2, interval=1e9, odt=1.0, tscale=1.0, skip=1,
+               event_times=()) as m:
+        lin = np.zeros(h3, LAYER)
+        lin["z"] = np.arange(h3) * 1.0
+        lin["gammadot0"] = np.where(np.arange(1, h3 + 1) > 6, g0, 0.0)
+        lin["stressexponent"] = 1.0
+        lin["friction"] = 0.6
+        pre = np.zeros(h3, TENSOR_LAYER)
+        pre["z"] = np.arange(h3)
+        m.set_prestress(pre)
+        m.set_structure(LINEAR, lin)
+        pat = np.zeros(1, SOURCE)
+        pat[0] = (1e-3, 0.0, 0.0, 1.0, 6.0, 10.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.2, 0.0)
+        load = np.zeros(1, SOURCE)
+        load[0] = (0.5, 3.0, -2.0, 0.0, 3.0, 4.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.2, 0.0)
+        m.event(pat, load, nyquist=1.0)
+        recs = [m.step() for _ in range(nsteps)]
+        return recs, m.download()["u1"]
+
+
+@pytest.mark.gpu
+def test_fortran_batch_matches_single_python_runs(tmp_path):
+    """Three models (two grids) stepped as one batch from Fortran
+    (relax_hip_batch_*) give bitwise the scalars and u1 of each model run
+    alone through the Python binding."""
+    exe = os.path.join(ROOT, "tests", "fortran", "_bin", "batch_loop")
+    if not os.path.exists(exe):
+        pytest.skip("the Fortran driver is built by __graft_entry__.build() when flang is present")
+    out = tmp_path / "batch_loop.bin"
+    r = subprocess.run([exe, str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    nsteps, depth, g0 = 3, (32, 32, 64), (0.2, 0.05, 0.1)
+    raw = open(out, "rb").read()
+    nscal = nsteps * len(depth) * 4
+    scal = np.frombuffer(raw[:nscal * 8], np.float64).reshape(nsteps, len(depth), 4)
+    f = np.frombuffer(raw[nscal * 8:], np.float32)
+    o = 0
+    for j, (sx3, g) in enumerate(zip(depth, g0)):
+        n = 34 * 32 * (sx3 // 2)
+        u1 = f[o:o + n].reshape(sx3 // 2, 32, 34)
+        o += n
+        recs, ref = _python_model(sx3, g, nsteps)
+        np.testing.assert_array_equal(scal[:, j], [[q["Dt"], q["t"], q["power"], q["cei"]] for q in recs])
+        np.testing.assert_array_equal(u1[..., :32], ref[..., :32])
+        assert np.abs(u1[..., :32]).max() > 0
+    assert o == f.size
