@@ -6,7 +6,7 @@ over the 512x512x256 workload (40 slip patches, power law n = 3 below a 25 km
 lid; relax_amd.workloads "c1"), with the whole state resident in HBM.
 
   python bench.py [--steps K] [--warmup W] [--workload c1] [--no-cpu-baseline]
-                  [--no-batched] [--no-serving] [--only batched|serving]
+                  [--no-batched] [--no-serving] [--only batched|serving|dropin]
 
 Prints one JSON line.  The timed region is K back-to-back steps on the
 model's stream with HIP-event phase timers (one synchronisation at the end),
@@ -15,7 +15,8 @@ kernel's algorithmic bytes per launch over its event-measured average
 duration.  `cpu_baseline` times the oracle (the C++ restatement of the
 reference, OpenMP on the host cores) on two steps of the same state.
 `batched` (64 models of 128^3) and `serving` (the request stream) ride on
-the same line; --only runs one of them alone.
+the same line; --only runs one of them alone (dropin: the per-routine Green's
+function entry with host arrays, against the same solve on a resident model).
 """
 import argparse
 import json
