@@ -33,10 +33,16 @@ def kernel_bytes(sx1, sx2, sx3):
     once, every output written once (FP32 fields, H = sx1*sx2*sx3/2)."""
     H = sx1 * sx2 * (sx3 // 2)
     nq = 8 * 3 * sx2 * (sx3 // 2)  # Nyquist planes of the work spectrum
+    tp = 4 * 3 * sx1 * sx2         # the three traction planes (real), or their half spectra
+    tn = 8 * 3 * sx2               # the traction planes' Nyquist rows
     return {
         "bodyforce": 4 * (6 * H + 3 * H) + 12 * sx1 * sx2,   # read sig, write body force + traction
-        "fft_x_fwd": 4 * (3 * H + 3 * H) + nq,                # read real rows, write half spectrum
-        "fft_y": 4 * (3 * H + 3 * H),                         # read + write the half spectrum
+        # read real rows, write half spectrum; the traction planes ride along
+        "fft_x_fwd": 4 * (3 * H + 3 * H) + nq + 2 * tp + tn,
+        # read + write the half spectrum, with the Nyquist rows riding along
+        # (and, forward, the traction planes): the mean of the forward and
+        # inverse launches
+        "fft_y": 4 * (3 * H + 3 * H) + 2 * nq + (2 * tp + 2 * tn) // 2,
         "green_z": 4 * (3 * H + 3 * H) + 2 * nq,              # read + write spectrum and Nyquist planes
         "fft_x_inv": 4 * (3 * H + 3 * H + 3 * H) + nq,        # read spectrum + u, write v or u
         "stress_pred": 4 * (3 * H + 6 * H + 6 * H + 6 * H),   # read v, sig, tau; write sig

@@ -84,35 +84,29 @@ struct XRows {
   static constexpr int G = YS ? RPB / 8 : 1;                       // groups of 8 rows
 };
 
+// Rows of one workgroup: bid is the workgroup's index among those of its
+// array; s, stw and ytw are the kernel's LDS (sized for YS).
 template <int N1, int NT, bool YS>
-__global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long in_pitch, MPtr<float2> out_m,
-                                                  long out_pitch, MPtr<float2> nyq_m, long nyq_stride, long nrows,
-                                                  float scale, const float2* __restrict__ tw, int n2,
-                                                  const float2* __restrict__ twy) {
-  // in and out may be the same buffer (rows are staged in LDS before any write)
-  const float* in = in_m.at(blockIdx.y);
-  float2* out = out_m.at(blockIdx.y);
-  float2* __restrict__ nyq = nyq_m.at(blockIdx.y);
+__device__ __forceinline__ void x_r2c_block(float2* s, const float2* stw, float2 (*ytw)[8], long bid,
+                                            const float* in, long in_pitch, float2* out, long out_pitch,
+                                            float2* __restrict__ nyq, long nyq_stride, long nrows, float scale,
+                                            int n2, const float2* __restrict__ twy) {
   constexpr int M = N1 / 2;
   constexpr int RPB = XRows<N1, YS>::RPB, G = XRows<N1, YS>::G;
   constexpr int PM = M + 1;  // row pitch in LDS (breaks bank aliasing between rows)
   constexpr int EPT = (RPB * M + NT - 1) / NT;
   constexpr int XP = RELAX_X_PAD;
-  __shared__ float2 s[lpad<XP>(RPB * PM) + 1];
-  __shared__ float2 stw[N1];  // twiddles in LDS: off the global-latency path of every stage
-  __shared__ float2 ytw[G][8];
   const int tid = threadIdx.x;
   // n2 is a power of two (64 or more with YS): L2 = n2/8 as shift and mask
   const int L2 = n2 / 8, ls2 = YS ? __builtin_ctz((unsigned)n2) - 3 : 0;
   // global row of LDS row b
   auto rowof = [&](int b) -> long {
-    if (!YS) return (long)blockIdx.x * RPB + b;
-    const long q = (long)blockIdx.x * G + b / 8;
+    if (!YS) return bid * RPB + b;
+    const long q = bid * G + b / 8;
     return ((q >> ls2) << (ls2 + 3)) + (q & (L2 - 1)) + (long)L2 * (b % 8);
   };
-  for (int e = tid; e < N1; e += NT) stw[e] = tw[e];
   if (YS && tid < G * 8) {
-    const int j1 = (int)(((long)blockIdx.x * G + tid / 8) & (L2 - 1));
+    const int j1 = (int)((bid * G + tid / 8) & (L2 - 1));
     ytw[tid / 8][tid % 8] = twy[(j1 * (tid % 8)) & (n2 - 1)];
   }
   for (int e = tid; e < RPB * M; e += NT) {
@@ -153,7 +147,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
   // first stage of the y transform, one column of one group per item
   for (int e = tid; e < G * M; e += NT) {
     const int gi = e / M, k = e % M;
-    const long q = (long)blockIdx.x * G + gi;
+    const long q = bid * G + gi;
     if (q * 8 >= nrows) continue;
     float2 a[8];
 #pragma unroll
@@ -163,6 +157,31 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
 #pragma unroll
     for (int t = 0; t < 8; ++t) gstore(out + (base_row + (long)L2 * t) * out_pitch + k, t ? cmul(a[t], ytw[gi][t]) : a[t]);
   }
+}
+
+// The first side.nb workgroups transform the rows of the side array (plain
+// rows, no split y stage); the others the main array's.
+template <int N1, int NT, bool YS>
+__global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long in_pitch, MPtr<float2> out_m,
+                                                  long out_pitch, MPtr<float2> nyq_m, long nyq_stride, long nrows,
+                                                  float scale, const float2* __restrict__ tw, int n2,
+                                                  const float2* __restrict__ twy, XSide side) {
+  // in and out may be the same buffer (rows are staged in LDS before any write)
+  constexpr int M = N1 / 2;
+  constexpr int RPB = XRows<N1, YS>::RPB, G = XRows<N1, YS>::G;
+  static_assert(XRows<N1, false>::RPB <= RPB, "the side rows fit the buffer");
+  __shared__ float2 s[lpad<RELAX_X_PAD>(RPB * (M + 1)) + 1];
+  __shared__ float2 stw[N1];  // twiddles in LDS: off the global-latency path of every stage
+  __shared__ float2 ytw[G][8];
+  for (int e = threadIdx.x; e < N1; e += NT) stw[e] = tw[e];
+  const int y = blockIdx.y;
+  if ((int)blockIdx.x < side.nb) {
+    x_r2c_block<N1, NT, false>(s, stw, nullptr, blockIdx.x, side.in.at(y), side.in_pitch, side.out.at(y),
+                               side.out_pitch, side.nyq.at(y), side.nyq_stride, side.nrows, side.scale, 0, nullptr);
+    return;
+  }
+  x_r2c_block<N1, NT, YS>(s, stw, ytw, (long)blockIdx.x - side.nb, in_m.at(y), in_pitch, out_m.at(y), out_pitch,
+                          nyq_m.at(y), nyq_stride, nrows, scale, n2, twy);
 }
 
 // ---------------------------------------------------------------- x C2R ----
@@ -303,16 +322,13 @@ struct ColTile {
   static constexpr int TX = V == 0 ? T0 : (V == 1 ? T1 : T2);
 };
 
+// TX columns of one workgroup: tile bid (plane bid / chunks, column chunk
+// bid % chunks); s holds L * TX samples.
 template <int L, int NT, int SIGN, int TX>
-__global__ __launch_bounds__(NT) void k_fft_cols(MPtr<float2> data_m, long jstride, long pstride, int ncols,
-                                                 const float2* __restrict__ tw) {
-  float2* data = data_m.at(blockIdx.y);
-  __shared__ float2 s[L * TX];
-  __shared__ float2 stw[L];
+__device__ __forceinline__ void cols_block(float2* s, const float2* stw, float2* data, long jstride, long pstride,
+                                           int ncols, int bid) {
   const int tid = threadIdx.x;
-  for (int e = tid; e < L; e += NT) stw[e] = tw[e];
   const int nchunk = (ncols + TX - 1) / TX;
-  const int bid = xcd_tile(blockIdx.x, gridDim.x);
   const int col0 = (bid % nchunk) * TX;
   float2* base = data + (long)(bid / nchunk) * pstride;
   for (int e = tid; e < L * TX; e += NT) {
@@ -329,18 +345,23 @@ __global__ __launch_bounds__(NT) void k_fft_cols(MPtr<float2> data_m, long jstri
   }
 }
 
-// ---------------------------------------------------------- contiguous ----
-template <int L, int NT, int SIGN>
-__global__ __launch_bounds__(NT) void k_fft_rows(MPtr<float2> data_m, long pitch, long nrows,
+template <int L, int NT, int SIGN, int TX>
+__global__ __launch_bounds__(NT) void k_fft_cols(MPtr<float2> data_m, long jstride, long pstride, int ncols,
                                                  const float2* __restrict__ tw) {
-  float2* data = data_m.at(blockIdx.y);
-  constexpr int RPB = (2048 / L) > 0 ? 2048 / L : 1;
-  constexpr int PL = L + 1;
-  __shared__ float2 s[RPB * PL];
+  __shared__ float2 s[L * TX];
   __shared__ float2 stw[L];
+  for (int e = threadIdx.x; e < L; e += NT) stw[e] = tw[e];
+  cols_block<L, NT, SIGN, TX>(s, stw, data_m.at(blockIdx.y), jstride, pstride, ncols,
+                              xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// ---------------------------------------------------------- contiguous ----
+// RPB rows of one workgroup, row pitch PL in LDS.
+template <int L, int NT, int SIGN, int RPB, int PL>
+__device__ __forceinline__ void rows_block(float2* s, const float2* stw, float2* data, long pitch, long nrows,
+                                           long bid) {
   const int tid = threadIdx.x;
-  for (int e = tid; e < L; e += NT) stw[e] = tw[e];
-  const long row0 = (long)blockIdx.x * RPB;
+  const long row0 = bid * RPB;
   for (int e = tid; e < RPB * L; e += NT) {
     const int b = e / L, j = e % L;
     const long row = row0 + b;
@@ -355,13 +376,61 @@ __global__ __launch_bounds__(NT) void k_fft_rows(MPtr<float2> data_m, long pitch
   }
 }
 
+template <int L, int NT, int SIGN>
+__global__ __launch_bounds__(NT) void k_fft_rows(MPtr<float2> data_m, long pitch, long nrows,
+                                                 const float2* __restrict__ tw) {
+  constexpr int RPB = (2048 / L) > 0 ? 2048 / L : 1;
+  constexpr int PL = L + 1;
+  __shared__ float2 s[RPB * PL];
+  __shared__ float2 stw[L];
+  for (int e = threadIdx.x; e < L; e += NT) stw[e] = tw[e];
+  rows_block<L, NT, SIGN, RPB, PL>(s, stw, data_m.at(blockIdx.y), pitch, nrows, blockIdx.x);
+}
+
+// ---------------------------------------------------------------- y pass ----
+// The y transform of a Green solve's work spectrum (L-point columns, TX per
+// workgroup) with the small LS-point transforms of the same solve riding
+// along (YRide): the first nside workgroups take the traction planes'
+// columns and the Nyquist rows, L*TX/LS columns or rows each, with their
+// twiddles read from the global table.  One launch in place of four: the
+// small transforms are a few microseconds of work each, but as launches of
+// their own each costs a launch.
+template <int L, int LS, int NT, int SIGN, int TX>
+__global__ __launch_bounds__(NT) void k_fft_ypass(MPtr<float2> data_m, long jstride, long pstride, int ncols,
+                                                  const float2* __restrict__ tw, YRideDev rd) {
+  constexpr int TS = L * TX / LS;  // columns or rows of a side workgroup
+  static_assert(TS >= 1, "a side workgroup holds one LS-point transform at least");
+  __shared__ float2 s[L * TX];
+  __shared__ float2 stw[L];
+  const int y = blockIdx.y;
+  int b = blockIdx.x;
+  if (b < rd.nside) {
+    if (b < rd.nbt) {
+      cols_block<LS, NT, SIGN, TS>(s, rd.tw, rd.r.tc.at(y), rd.r.tjs, rd.r.tps, rd.r.tnc, b);
+      return;
+    }
+    b -= rd.nbt;
+    if (b < rd.nb0) {
+      rows_block<LS, NT, SIGN, TS, LS>(s, rd.tw, rd.r.r0.at(y), rd.r.rp0, rd.r.rn0, b);
+      return;
+    }
+    b -= rd.nb0;
+    if (b < rd.nb1) rows_block<LS, NT, SIGN, TS, LS>(s, rd.tw, rd.r.r1.at(y), rd.r.rp1, rd.r.rn1, b);
+    return;
+  }
+  for (int e = threadIdx.x; e < L; e += NT) stw[e] = tw[e];
+  cols_block<L, NT, SIGN, TX>(s, stw, data_m.at(y), jstride, pstride, ncols,
+                              xcd_tile(b - rd.nside, gridDim.x - rd.nside));
+}
+
 // ----------------------------------------------------------- launchers ----
 #define NT_FFT 256
 
 // ys_n2 > 0: the split y transform over planes of ys_n2 rows (see
 // k_fft_x_r2c); nrows is then a multiple of ys_n2.
 int launch_fft_x_r2c(int n1, MPtr<const float> in, long in_pitch, MPtr<float2> out, long out_pitch,
-                     MPtr<float2> nyq, long nyq_stride, long nrows, float scale, int nm, hipStream_t st, int ys_n2) {
+                     MPtr<float2> nyq, long nyq_stride, long nrows, float scale, int nm, hipStream_t st, int ys_n2,
+                     const XSide* side) {
   const float2* tw = twiddle_table(n1);
   if (!tw) return RELAX_ERR_NOMEM;
   const float2* twy = nullptr;
@@ -369,20 +438,23 @@ int launch_fft_x_r2c(int n1, MPtr<const float> in, long in_pitch, MPtr<float2> o
     if (ys_n2 < 64 || (ys_n2 & (ys_n2 - 1)) || nrows % ys_n2) return RELAX_ERR_SIZE;
     if (!(twy = twiddle_table(ys_n2))) return RELAX_ERR_NOMEM;
   }
+  XSide sd;
+  if (side && side->nrows > 0) sd = *side;
   long nb = 0;
   switch (n1) {
 #define X(N)                                                                                                \
   case N:                                                                                                   \
+    sd.nb = (int)((sd.nrows + XRows<N, false>::RPB - 1) / XRows<N, false>::RPB);                            \
     if (ys_n2) {                                                                                            \
       constexpr int G = XRows<N, true>::G;                                                                  \
       nb = (nrows / 8 + G - 1) / G;                                                                         \
-      k_fft_x_r2c<N, RELAX_X_NT, true><<<dim3((unsigned)nb, (unsigned)nm), RELAX_X_NT, 0, st>>>(                    \
-          in, in_pitch, out, out_pitch, nyq, nyq_stride, nrows, scale, tw, ys_n2, twy);                     \
+      k_fft_x_r2c<N, RELAX_X_NT, true><<<dim3((unsigned)(nb + sd.nb), (unsigned)nm), RELAX_X_NT, 0, st>>>(          \
+          in, in_pitch, out, out_pitch, nyq, nyq_stride, nrows, scale, tw, ys_n2, twy, sd);                 \
     } else {                                                                                                \
       constexpr int RPB = XRows<N, false>::RPB;                                                             \
       nb = (nrows + RPB - 1) / RPB;                                                                         \
-      k_fft_x_r2c<N, RELAX_X_NT, false><<<dim3((unsigned)nb, (unsigned)nm), RELAX_X_NT, 0, st>>>(                   \
-          in, in_pitch, out, out_pitch, nyq, nyq_stride, nrows, scale, tw, 0, nullptr);                     \
+      k_fft_x_r2c<N, RELAX_X_NT, false><<<dim3((unsigned)(nb + sd.nb), (unsigned)nm), RELAX_X_NT, 0, st>>>(         \
+          in, in_pitch, out, out_pitch, nyq, nyq_stride, nrows, scale, tw, 0, nullptr, sd);                 \
     }                                                                                                       \
     break;
     X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
@@ -462,5 +534,46 @@ int launch_fft_rows(int L, int sign, MPtr<float2> data, long pitch, long nrows, 
     switch (L) { X(16, 1) X(32, 1) X(64, 1) X(128, 1) X(256, 1) X(512, 1) X(1024, 1) default: return RELAX_ERR_SIZE; }
   }
 #undef X
+  return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
+}
+
+template <int L, int LS, int SIGN, int V>
+static void ypass_launch(MPtr<float2> data, long jstride, long pstride, int ncols, int nplanes, YRideDev rd, int nm,
+                         const float2* tw, hipStream_t st) {
+  constexpr int TX = ColTile<L, V>::TX, TS = L * TX / LS;
+  rd.nbt = rd.r.tnp > 0 ? ((rd.r.tnc + TS - 1) / TS) * rd.r.tnp : 0;
+  rd.nb0 = (int)((rd.r.rn0 + TS - 1) / TS);
+  rd.nb1 = (int)((rd.r.rn1 + TS - 1) / TS);
+  // a multiple of 8 workgroups, so that the main tiles keep their XCD order
+  rd.nside = (rd.nbt + rd.nb0 + rd.nb1 + 7) / 8 * 8;
+  const long nmain = ((ncols + TX - 1) / TX) * (long)nplanes;
+  const dim3 grid((unsigned)(rd.nside + nmain), (unsigned)nm);
+  k_fft_ypass<L, LS, NT_FFT, SIGN, TX><<<grid, NT_FFT, 0, st>>>(data, jstride, pstride, ncols, tw, rd);
+}
+
+int launch_fft_ypass(int L, int LS, int sign, MPtr<float2> data, long jstride, long pstride, int ncols, int nplanes,
+                     const YRide& ride, int nm, hipStream_t st, int variant) {
+  // the work spectrum's columns are LS/8 points from LS = 128 up (the
+  // split y transform), LS points below
+  if (L != (LS >= 128 ? LS / 8 : LS)) return RELAX_ERR_SIZE;
+  if (variant < 0 || variant >= COLS_VARIANTS) return RELAX_ERR_ARG;
+  if (ride.tnc < 0 || ride.tnp < 0 || ride.rn0 < 0 || ride.rn1 < 0) return RELAX_ERR_ARG;
+  const float2* tw = twiddle_table(L);
+  YRideDev rd{};
+  rd.r = ride;
+  rd.tw = twiddle_table(LS);
+  if (!tw || !rd.tw) return RELAX_ERR_NOMEM;
+#define V(N, S)                                                                                                     \
+  case N:                                                                                                           \
+    if (variant == 0) ypass_launch<(N >= 128 ? N / 8 : N), N, S, 0>(data, jstride, pstride, ncols, nplanes, rd, nm, tw, st);  \
+    else if (variant == 1) ypass_launch<(N >= 128 ? N / 8 : N), N, S, 1>(data, jstride, pstride, ncols, nplanes, rd, nm, tw, st); \
+    else ypass_launch<(N >= 128 ? N / 8 : N), N, S, 2>(data, jstride, pstride, ncols, nplanes, rd, nm, tw, st);  \
+    break;
+  if (sign < 0) {
+    switch (LS) { V(32, -1) V(64, -1) V(128, -1) V(256, -1) V(512, -1) V(1024, -1) default: return RELAX_ERR_SIZE; }
+  } else {
+    switch (LS) { V(32, 1) V(64, 1) V(128, 1) V(256, 1) V(512, 1) V(1024, 1) default: return RELAX_ERR_SIZE; }
+  }
+#undef V
   return hipGetLastError() == hipSuccess ? RELAX_OK : RELAX_ERR_HIP;
 }

@@ -13,9 +13,18 @@ const float2* twiddle_table(int L);
 // kernels do its first radix-8 stage across rows j1 + (ys_n2/8)*j2, and
 // k_fft_cols of length ys_n2/8 over blocks of ys_n2/8 rows completes it,
 // leaving y frequency 8*k1 + k2 at row (ys_n2/8)*k2 + k1 of a plane
+// side: a second array of rows that rides along in the same launch (plain
+// rows, no split y stage; nb is set by the launcher)
+struct XSide {
+  MPtr<const float> in;
+  MPtr<float2> out, nyq;
+  long in_pitch = 0, out_pitch = 0, nyq_stride = 0, nrows = 0;
+  float scale = 0.f;
+  int nb = 0;
+};
 int launch_fft_x_r2c(int n1, MPtr<const float> in, long in_pitch, MPtr<float2> out, long out_pitch,
                      MPtr<float2> nyq, long nyq_stride, long nrows, float scale, int nm, hipStream_t st,
-                     int ys_n2 = 0);
+                     int ys_n2 = 0, const XSide* side = nullptr);
 int launch_fft_x_c2r(int n1, int epi, MPtr<const float2> in, long in_pitch, MPtr<const float2> nyq,
                      long nyq_stride, MPtr<float> out, long out_pitch, long nrows, float scale,
                      MPtr<const float> base, long base_pitch, MPtr<const StepScalars> sc, int nm, hipStream_t st,
@@ -26,6 +35,27 @@ constexpr int COLS_VARIANTS = 3, GREEN_VARIANTS = 4, BODY_VARIANTS = 6;
 int launch_fft_cols(int L, int sign, MPtr<float2> data, long jstride, long pstride, int ncols, int nplanes, int nm,
                     hipStream_t st, int variant = 0);
 int launch_fft_rows(int L, int sign, MPtr<float2> data, long pitch, long nrows, int nm, hipStream_t st);
+// The small transforms of a Green solve that ride along with its y pass:
+// tnp planes of tnc columns of LS points (column j at j*tjs, plane at tps)
+// and two sets of LS-point rows (pitch rp, rn rows).  Zero counts leave a
+// job out.
+struct YRide {
+  MPtr<float2> tc;
+  long tjs = 0, tps = 0;
+  int tnc = 0, tnp = 0;
+  MPtr<float2> r0, r1;
+  long rp0 = 0, rn0 = 0, rp1 = 0, rn1 = 0;
+};
+// the launch's copy: workgroups of each job and the LS-point twiddles
+struct YRideDev {
+  YRide r;
+  const float2* tw;
+  int nbt, nb0, nb1, nside;
+};
+// The y pass: L-point columns of data (as launch_fft_cols) and the ride's
+// LS-point transforms in one launch.
+int launch_fft_ypass(int L, int LS, int sign, MPtr<float2> data, long jstride, long pstride, int ncols, int nplanes,
+                     const YRide& ride, int nm, hipStream_t st, int variant = 0);
 // one buffer
 inline int launch_fft_x_r2c(int n1, const float* in, long in_pitch, float2* out, long out_pitch, float2* nyq,
                             long nyq_stride, long nrows, float scale, hipStream_t st) {

@@ -992,6 +992,23 @@ static void cmark(const Ctx& c, int kind) {
   if (c.prof) mark(c.prof, kind);
 }
 
+// The small transforms of a Green solve's y pass: forward, the traction
+// planes' columns and the Nyquist rows of W and T; inverse, those of W.
+static YRide ypass_ride(const Ctx& c, int sign) {
+  const Grid& g = c.lead->g;
+  const ModelDev* md = c.md;
+  YRide r;
+  r.r0 = MFIELD(float2, md, NQ);
+  r.rp0 = g.n2; r.rn0 = 3L * g.h3;
+  if (sign < 0) {
+    r.tc = MFIELD(float2, md, T);
+    r.tjs = g.m1; r.tps = (long)g.n2 * g.m1; r.tnc = g.m1; r.tnp = 3;
+    r.r1 = MFIELD(float2, md, TN);
+    r.rp1 = g.n2; r.rn1 = 3;
+  }
+  return r;
+}
+
 // Green's function of the work field W (body force, upper half) with the
 // traction planes T of every model; the last pass writes u + c*G into W
 // (predictor) or u (corrector and EPI_PLAIN).
@@ -1008,35 +1025,31 @@ static int model_green(const Ctx& c, int epi) {
   const int l2 = ys ? n2 / 8 : 0;
   const int ly = ys ? l2 : n2, nyp = ys ? 3 * h3 * 8 : 3 * h3;
   int rc;
+  // the traction planes ride along with the work field's x transform
+  XSide tside;
+  tside.in = MFIELD(const float, md, T);
+  tside.out = MFIELD(float2, md, T);
+  tside.nyq = MFIELD(float2, md, TN);
+  tside.in_pitch = n1; tside.out_pitch = m1; tside.nyq_stride = 1; tside.nrows = 3L * n2;
+  tside.scale = (float)(m->desc.dx1 * m->desc.dx2);
   if ((rc = launch_fft_x_r2c(n1, MFIELD(const float, md, W), n1, MFIELD(float2, md, W), m1, MFIELD(float2, md, NQ), 1,
-                             3L * n2 * h3, (float)dv, nm, st, ys)))
+                             3L * n2 * h3, (float)dv, nm, st, ys, &tside)))
     return rc;
   cmark(c, RELAX_PROF_FFT_X_FWD);
-  if ((rc = launch_fft_x_r2c(n1, MFIELD(const float, md, T), n1, MFIELD(float2, md, T), m1, MFIELD(float2, md, TN), 1,
-                             3L * n2, (float)(m->desc.dx1 * m->desc.dx2), nm, st)))
-    return rc;
-  cmark(c, RELAX_PROF_SMALL);
-  if ((rc = launch_fft_cols(ly, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st, c.tune_cols)))
+  // the y pass of W, with the traction planes' y transform and those of the
+  // Nyquist planes of W and T in the same launch
+  if ((rc = launch_fft_ypass(ly, n2, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, ypass_ride(c, -1), nm, st,
+                             c.tune_cols)))
     return rc;
   cmark(c, RELAX_PROF_FFT_Y);
-  // the three traction planes of one model are few columns for the device:
-  // half the default columns per workgroup gives twice the workgroups
-  if ((rc = launch_fft_cols(n2, -1, MFIELD(float2, md, T), m1, (long)n2 * m1, m1, 3, nm, st, nm == 1 ? 1 : 0)))
-    return rc;
-  cmark(c, RELAX_PROF_SMALL);
-  if ((rc = launch_fft_rows(n2, -1, MFIELD(float2, md, NQ), n2, 3L * h3, nm, st))) return rc;
-  cmark(c, RELAX_PROF_SMALL);
-  if ((rc = launch_fft_rows(n2, -1, MFIELD(float2, md, TN), n2, 3, nm, st))) return rc;
-  cmark(c, RELAX_PROF_SMALL);
   if ((rc = launch_green_columns(m->gc, h3, MFIELD(float2, md, W), MFIELD(float2, md, NQ), MFIELD(const float2, md, T),
                                  MFIELD(const float2, md, TN), nm, st, l2, c.tune_green)))
     return rc;
   cmark(c, RELAX_PROF_GREEN_Z);
-  if ((rc = launch_fft_cols(ly, +1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st, c.tune_cols)))
+  if ((rc = launch_fft_ypass(ly, n2, +1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, ypass_ride(c, +1), nm, st,
+                             c.tune_cols)))
     return rc;
   cmark(c, RELAX_PROF_FFT_Y);
-  if ((rc = launch_fft_rows(n2, +1, MFIELD(float2, md, NQ), n2, 3L * h3, nm, st))) return rc;
-  cmark(c, RELAX_PROF_SMALL);
   const double inv = 1.0 / (n1 * m->desc.dx1 * n2 * m->desc.dx2 * g.n3 * m->desc.dx3);
   const MPtr<float> out = epi == EPI_PRED ? MFIELD(float, md, W) : MFIELD(float, md, u);
   rc = launch_fft_x_c2r(n1, epi, MFIELD(const float2, md, W), m1, MFIELD(const float2, md, NQ), 1, out, n1,
@@ -2236,7 +2249,8 @@ static int tune_time(const Ctx& c, int* best, float (*times)[RELAX_TUNE_VARIANTS
   if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return RELAX_ERR_HIP; }
   auto run = [&](int kind, int v) -> int {
     if (kind == 0)
-      return launch_fft_cols(ly, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, nm, st, v);
+      return launch_fft_ypass(ly, n2, -1, MFIELD(float2, md, W), m1, (long)ly * m1, m1, nyp, ypass_ride(c, -1), nm,
+                              st, v);
     if (kind == 1)
       return launch_green_columns(m->gc, h3, MFIELD(float2, md, W), MFIELD(float2, md, NQ),
                                   MFIELD(const float2, md, T), MFIELD(const float2, md, TN), nm, st, l2, v);
