@@ -465,8 +465,10 @@ __device__ __forceinline__ void stress_block(const Grid& g, const RheoParams& R,
                                              float* __restrict__ E, float* __restrict__ Em, StepScalars* sc,
                                              double2* __restrict__ partials, int blk) {
   const int tid = threadIdx.x;
-  // blocks in address order: an XCD-aware order (neighbouring rows in one
-  // L2) measured 9-15 % slower at 512x512x256 than this streaming order
+  // block blk: 256 points in address order (the corrector's partials go to
+  // entry blk, so their sum keeps one order whatever the launch order of
+  // the blocks; an order in which each XCD walks a contiguous eighth of the
+  // grid measured 9-15 % slower at 512x512x256, see slab_tile)
   const long o = (long)blk * 256 + tid;
   const int n1 = g.n1, n2 = g.n2, h3 = g.h3;
   const long H = g.H;
@@ -505,13 +507,30 @@ __device__ __forceinline__ void stress_block(const Grid& g, const RheoParams& R,
   if (status) atomicMax(&sc->status, status);
 }
 
+// Block order of the strain pass: workgroup b of nb, with P workgroups per
+// plane (P % 8 == 0), takes block slab_tile(b): XCD b % 8 walks the eighth
+// [x P/8, (x+1) P/8) of every plane (n2/8 whole rows), plane after plane.
+// The x2 and x3 neighbours of a point are then read by workgroups of the
+// same XCD a few hundred workgroups apart, and its L2 serves the re-reads:
+// at 512x512x256 the pass fetches 1.01x the bytes of its inputs, against
+// 1.45x (predictor) and 1.42x (corrector) in address order, where the
+// neighbouring rows of a block belong to other XCDs.  The predictor runs
+// 0.502 ms against 0.524, the corrector is unchanged (0.738 ms: its
+// 45 % share of writes holds it at the HBM rate of its mix).
+__device__ __forceinline__ int slab_tile(int b, int nb, int P) {
+  if (P % 8 || nb % P) return b;
+  const int x = b % 8, l = b / 8, q = P / 8;
+  return (l / q) * P + x * q + l % q;
+}
+
 template <int MODE, int RC, bool BATCH>
 __global__ __launch_bounds__(256) void k_stress(Grid g, ModelDev Dv, const ModelDev* __restrict__ md,
                                                StencilConst S) {
   // model blockIdx.y of the launch (BATCH) or the argument Dv
   const ModelDev& D = BATCH ? md[blockIdx.y] : Dv;
+  const int blk = slab_tile(blockIdx.x, gridDim.x, (g.n1 * g.n2) / 256);
   stress_block<MODE, RC>(g, D.R, S, MODE == STRESS_PRED ? D.W : D.u, D.sig, D.tau, D.gam, D.E, D.Em, D.sc,
-                         D.partials, blockIdx.x);
+                         D.partials, blk);
 }
 
 // ---------------------------------------------- models of uneven grids ----
