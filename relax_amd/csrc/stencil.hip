@@ -587,10 +587,42 @@ __global__ __launch_bounds__(TXB* TYB) void k_bodyforce_list(const ModelDev* __r
   }
 }
 
+// Compute units of the current device (cached per device; 256 if the query
+// fails).
+static int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] <= 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
+// Depth chunk of one divergence tile: each chunk re-reads the plane above
+// and below it.  The register-window forms (variants 4 and 5: one workgroup
+// of 64x16 tiles per CU, two of 64x8, at their register use) take the
+// deepest chunk that still gives every CU a workgroup, at least 8 planes:
+// at 512x512x256 one chunk of all 128 planes (256 workgroups) runs the
+// divergence in 0.250 ms against 0.275 ms for chunks of 32 planes (1024
+// workgroups, four rounds) and 0.268 ms for 64.
+int body_depth_chunk(const Grid& g, int variant, int nm) {
+  if (variant < 4) return variant >= 3 ? 32 : 16;
+  const long tiles = (long)(g.n1 / 64) * (g.n2 / (variant == 5 ? 16 : 8)) * (nm > 0 ? nm : 1);
+  if (tiles <= 0) return 32;  // the tiles do not fit the grid: the launcher refuses it
+  const long want = (long)device_cus() * (variant == 5 ? 1 : 2);
+  long gz = (want + tiles - 1) / tiles;
+  const long gmax = (g.h3 + 7) / 8;
+  if (gz > gmax) gz = gmax;
+  if (gz < 1) gz = 1;
+  return (int)((g.h3 + gz - 1) / gz);
+}
+
 int launch_bodyforce(const Grid& g, const StencilConst& S, const ModelDev* md, const ModelDev* one, int nm,
                      int rclass, int mode, hipStream_t st, int variant) {
-  // depth chunk of one tile: each chunk re-reads the plane above and below it
-  const int zc = variant >= 3 ? 32 : 16;
+  const int zc = body_depth_chunk(g, variant, nm);
   const int gz = (g.h3 + zc - 1) / zc;
   const bool batch = !(one && nm == 1);
   const ModelDev Dv = batch ? ModelDev{} : *one;
@@ -610,8 +642,8 @@ int launch_bodyforce(const Grid& g, const StencilConst& S, const ModelDev* md, c
     else LF(K, TX, TY, RHEO_VISCOUS)                   \
   }
   // variant 0: 32x16 tiles, 1: 32x8 tiles, 2: 64x8 tiles, 3: 64x8 tiles
-  // 32 planes deep (16 for the others); 4 and 5: 64x8 and 64x16 tiles 32
-  // planes deep with the x3 differences in registers (k_bodyforce_zr)
+  // 32 planes deep (16 for the others); 4 and 5: 64x8 and 64x16 tiles with
+  // the x3 differences in registers (k_bodyforce_zr), body_depth_chunk deep
   if (variant < 0 || variant >= BODY_VARIANTS) return RELAX_ERR_ARG;
   if (variant == 4 && g.n1 % 64 == 0 && g.n2 % 8 == 0) L(k_bodyforce_zr, 64, 8)
   else if (variant == 5 && g.n1 % 64 == 0 && g.n2 % 16 == 0) L(k_bodyforce_zr, 64, 16)
