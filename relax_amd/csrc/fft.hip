@@ -187,8 +187,16 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(MPtr<const float> in_m, long i
 // ---------------------------------------------------------------- x C2R ----
 // With YS the workgroup first undoes the 8-point y stage of its row groups
 // (the blocks' L2-point inverse transforms ran before, see k_fft_x_r2c).
+// Waves per SIMD the 512-point inverse is held to: 7 (72 VGPRs, no spills;
+// its LDS admits 7 workgroups per CU) runs it in 0.230 ms at 512x512x256
+// against 0.247 ms for the compiler's own choice of 73 VGPRs and 6 waves.
+// The shorter rows would spill under the same bound, and the 1024-point
+// rows are held to 3 workgroups per CU by their LDS.
+#ifndef RELAX_C2R_MINW
+#define RELAX_C2R_MINW 7
+#endif
 template <int N1, int NT, int EPI, bool YS>
-__global__ __launch_bounds__(NT) void k_fft_x_c2r(MPtr<const float2> in_m, long in_pitch, MPtr<const float2> nyq_m,
+__global__ __launch_bounds__(NT, N1 == 512 ? RELAX_C2R_MINW : 1) void k_fft_x_c2r(MPtr<const float2> in_m, long in_pitch, MPtr<const float2> nyq_m,
                                                   long nyq_stride, MPtr<float> out_m, long out_pitch, long nrows,
                                                   float scale, const float2* __restrict__ tw, MPtr<const float> base_m,
                                                   long base_pitch, MPtr<const StepScalars> sc_m, int n2,
