@@ -12,6 +12,8 @@
 //     differences at the top and bottom planes (stressupdate,
 //     elastic3d.f90:956-1180), Hooke's law, and the stage update of the RK2
 //     loop (relax.f90:679-701, :833-890) in the same pass.
+#include <atomic>
+
 #include "kernels.h"
 #include "rheology.h"
 #include "stencil.h"
@@ -587,18 +589,18 @@ __global__ __launch_bounds__(TXB* TYB) void k_bodyforce_list(const ModelDev* __r
   }
 }
 
-// Compute units of the current device (cached per device; 256 if the query
-// fails).
+// Compute units of the current device (cached per device, safe to call from
+// several host threads; 256 if the query fails).
 static int device_cus() {
-  static int cus[64] = {0};
+  static std::atomic<int> cus[64];  // zero-initialised (static storage)
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (cus[dev] <= 0) {
-    int n = 0;
+  int n = cus[dev].load(std::memory_order_relaxed);
+  if (n <= 0) {
     if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-    cus[dev] = n;
+    cus[dev].store(n, std::memory_order_relaxed);
   }
-  return cus[dev];
+  return n;
 }
 
 // Depth chunk of one divergence tile: each chunk re-reads the plane above
